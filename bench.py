@@ -1,0 +1,166 @@
+#!/usr/bin/env python3
+"""Benchmark of the explicit hex8 neo-Hookean path on one MI355X.
+
+Workload (default): 100^3 hex8 unit cube (1.0M elements, 1.03M nodes, 2x2x2 Gauss), interior nodes
+perturbed by U(-0.1h, 0.1h) (SplitMix64 seed 0x4D50530001), compressible neo-Hookean E=1e6 Pa,
+nu=0.3, rho=1000; bottom face clamped, top face prescribed z-velocity ramped to 0.1 m/s over the
+first 100 steps; lumped mass, central difference, dt = 0.5 x stable dt recomputed every step.
+
+A step = one explicit step of the whole mesh (element forces, CSR gather, central-difference update,
+stable-dt reduction).  `value` = element-Gauss-point updates per second of the FP64 form over the timed
+steps (inputs resident in HBM; wall clock between two device synchronisations).  The FP32-element /
+FP64-accumulate form is reported beside it with its displacement error against the FP64 form.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "v.1.1_amd"))
+sys.path.insert(0, os.path.join(ROOT, "tests"))
+
+import numpy as np  # noqa: E402
+
+import mps_gpu as G  # noqa: E402
+from mps_gpu import configs  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6
+
+
+def hex_elem_kernel_bytes(ne, nn, prec, per_elem_mat):
+    """Algorithmic HBM bytes of one element-kernel launch (hex8): conn 8x4 B + l0 8 B (+ material
+    24 B when per element) + element forces 24 x p_e, per element; X and u 3x8 B each, per node."""
+    pe = 4 if prec == G.PREC_FP32_ELEM else 8
+    return ne * (32 + 8 + (24 if per_elem_mat else 0) + 24 * pe) + nn * (24 + 24)
+
+
+def step_bytes(ne, nn, nen, prec, per_elem_mat):
+    """Algorithmic HBM bytes of one whole step (element + node kernels)."""
+    pe = 4 if prec == G.PREC_FP32_ELEM else 8
+    elem = ne * (4 * nen + 8 + (24 if per_elem_mat else 0) + 3 * nen * pe) + nn * 48
+    node = ne * nen * (4 + 3 * pe) + nn * (4 + 4 + 72 + 72 + 8)  # CSR col+fe read; rowptr, ninfo, u/v/a rw, mass
+    return elem + node
+
+
+def run_gpu(cfg, precision, warmup, steps):
+    m = cfg["mesh"]
+    s = G.Solver(m, law=cfg["law"], E=cfg["E"], nu=cfg["nu"], rho=cfg["rho"], dir_dof=cfg["dir_dof"],
+                 dir_val=cfg["dir_val"], bc_mode=G.BC_VELOCITY, vel_ramp_steps=100, scale=cfg["scale"],
+                 n_steps=warmup + steps, precision=precision)
+    rc, rep = s.run(warmup)
+    assert rc == 0, rep
+    s.profile(True)
+    t0 = time.perf_counter()
+    rc, rep = s.run(steps)
+    t1 = time.perf_counter()
+    assert rc == 0 and rep["steps"] == warmup + steps, (rc, rep)
+    kt = s.kernel_times()
+    u, v, a = s.get_state()
+    s.close()
+    return dict(wall_s=t1 - t0, device_ms=rep["device_ms"], kt=kt, u=u, rep=rep)
+
+
+def cpu_baseline(cfg, budget_s=12.0):
+    """Oracle (the CPU restatement of the reference loop) on the same mesh, all host threads (max 16)."""
+    import oracle_ctypes as O
+    m = cfg["mesh"]
+    nt = max(1, min(16, os.cpu_count() or 1))
+
+    def timed(nsteps, threads):
+        p = O.OracleProblem(m, law=cfg["law"], E=cfg["E"], nu=cfg["nu"], rho=cfg["rho"], dir_dof=cfg["dir_dof"],
+                            dir_val=cfg["dir_val"], bc_mode=O.BC_VELOCITY, vel_ramp_steps=100, scale=cfg["scale"],
+                            n_steps=nsteps, sum_order=O.SUM_ELEMENT, n_threads=threads)
+        t = time.perf_counter()
+        r = p.run()
+        assert r["rc"] == 0
+        return time.perf_counter() - t
+
+    out = {}
+    for threads in (nt, 1):
+        t_a = timed(1, threads)
+        t_b = timed(3, threads)
+        per = max((t_b - t_a) / 2, 1e-6)
+        n = int(max(2, min(400, (budget_s if threads > 1 else budget_s / 2) / per)))
+        t_c = timed(1 + n, threads)
+        per = (t_c - t_a) / n
+        gp = m["n_elem"] * (8 if m["nen"] == 8 else 1)
+        out[threads] = dict(value=gp / per, steps=n, per_step_s=per)
+    return nt, out
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-fp32", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only; --gpus {args.gpus} is not supported "
+              "(independent solves shard trivially, a single mesh would need an interface-force exchange)",
+              file=sys.stderr)
+        sys.exit(2)
+
+    cfg = configs.config(1)
+    m = cfg["mesh"]
+    ne, nn = m["n_elem"], m["n_nodes"]
+    gp_per_step = ne * 8
+
+    r64 = run_gpu(cfg, G.PREC_FP64, args.warmup, args.steps)
+    value = gp_per_step * args.steps / r64["wall_s"]
+    ems, en = r64["kt"]["element"]
+    nms, nn_l = r64["kt"]["node"]
+    elem_avg_ms = ems / max(en, 1)
+    node_avg_ms = nms / max(nn_l, 1)
+    eb = hex_elem_kernel_bytes(ne, nn, G.PREC_FP64, False)
+    achieved = eb / (elem_avg_ms * 1e-3) / 1e9
+    out = {
+        "metric": "elem-GP updates/s (1M hex8 NH explicit)",
+        "value": value,
+        "unit": "elem-GP updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": r64["wall_s"] * 1e3 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "100^3 hex8 cube, 10% seeded node perturbation, compressible neo-Hookean "
+                               "E=1e6 nu=0.3 rho=1000, bottom clamped, top z-velocity ramp 0.1 m/s, "
+                               "dt=0.5 x stable dt, FP64",
+                   "n_elem": ne, "n_nodes": nn, "gauss_points": 8, "form": "fp64"},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "k_elem<double,8,0> (hex8 element forces + stable dt)",
+                     "kernel_avg_ms": elem_avg_ms, "algorithmic_bytes_per_launch": eb},
+        "kernels": {"element_avg_ms": elem_avg_ms, "node_avg_ms": node_avg_ms,
+                    "step_algorithmic_bytes": step_bytes(ne, nn, 8, G.PREC_FP64, False),
+                    "step_hbm_frac": step_bytes(ne, nn, 8, G.PREC_FP64, False) / (r64["wall_s"] / args.steps)
+                    / 1e9 / HBM_PEAK_GBS},
+    }
+    if not args.no_fp32:
+        r32 = run_gpu(cfg, G.PREC_FP32_ELEM, args.warmup, args.steps)
+        v32 = gp_per_step * args.steps / r32["wall_s"]
+        err = float(np.abs(r32["u"] - r64["u"]).max() / np.abs(r64["u"]).max())
+        e32, n32 = r32["kt"]["element"]
+        out["fp32_elem"] = {"value": v32, "ms_per_step": r32["wall_s"] * 1e3 / args.steps,
+                            "speedup_vs_fp64": v32 / value, "max_rel_disp_err_vs_fp64": err,
+                            "element_avg_ms": e32 / max(n32, 1),
+                            "node_avg_ms": r32["kt"]["node"][0] / max(r32["kt"]["node"][1], 1)}
+    if not args.no_cpu:
+        nt, cb = cpu_baseline(cfg)
+        out["cpu_baseline"] = {"value": cb[nt]["value"], "unit": "elem-GP updates/s", "cores": nt, "kind": "port",
+                               "sample": f"same 1M-hex8 workload, {cb[nt]['steps']} steps after a 1-step set-up "
+                                         f"run (set-up excluded), oracle element-order, {nt} threads"}
+        out["cpu_baseline_seq"] = {"value": cb[1]["value"], "cores": 1, "steps": cb[1]["steps"]}
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,205 @@
+/*
+ * mps_gpu.h -- C-ABI of the MI355X explicit-dynamics engine (libmps_gpu.so).
+ *
+ * Drop-in for the explicit hex8/tet4 neo-Hookean path of muphisim v1.1.  The
+ * boundary is plain C: fixed-width integers, double pointers, SoA arrays and
+ * 0-based indices.  Host buffers are read only during the call (copy-in); the
+ * library owns all device memory.  No function calls exit(): every entry point
+ * returns an int status (MPS_OK or a negative MPS_ERR_*), and mps_last_error()
+ * returns the message of the last failure on a handle.
+ *
+ * Conventions kept from the reference:
+ *   - DOF index = node + n_nodes*dir          (classGPs.cpp:54, classNodes.cpp:62)
+ *   - nodal vectors are SoA by direction: u[dir*n_nodes + node]
+ *   - element connectivity is element-major, reference node order
+ *     (classLinearHexahedra.cpp:346-353; classLinearTetrahedra.cpp:270-292)
+ *
+ * A handle must not be used from two threads at once.  The server object
+ * (mps_server_*) is internally synchronised.
+ */
+#ifndef MPS_GPU_H
+#define MPS_GPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ---- */
+#define MPS_OK 0
+#define MPS_ERR_ARG (-1)      /* invalid argument or inconsistent input */
+#define MPS_ERR_HIP (-2)      /* HIP runtime failure (no device, OOM, launch) */
+#define MPS_ERR_DT (-3)       /* stable dt < 1e-20: "Negative jacobian" (solverExplicit.cpp:270-273) */
+#define MPS_ERR_NAN (-4)      /* non-finite u[1] (solverExplicit.cpp:306-308) */
+#define MPS_ERR_STATE (-5)    /* call not valid in the handle's current state */
+#define MPS_ERR_CAPACITY (-6) /* request exceeds a configured capacity */
+
+/* ---- run status reported in mps_report.status ---- */
+#define MPS_RUNNING 0
+#define MPS_DONE 1
+
+/* ---- constitutive laws (constitutiveList.h; inputFile.cpp:1008-1014) ---- */
+#define MPS_LAW_NH_COMPRESSIBLE 0 /* "*HyperElastic, 3D-Compressible-Neo-Hookean" */
+#define MPS_LAW_NH_BELYTSCHKO 1   /* "*HyperElastic, Neo-Hookean" */
+
+/* ---- Dirichlet modes ---- */
+#define MPS_BC_RAMP 0     /* *BOUNDARY, TYPE=DISPLACEMENT RAMP (classNodes.cpp:66-74) */
+#define MPS_BC_INST 1     /* *BOUNDARY, TYPE=DISPLACEMENT INST (classNodes.cpp:58-64) */
+#define MPS_BC_VELOCITY 2 /* extension: prescribed velocity V*min(step/ramp,1) */
+
+/* ---- precision forms ---- */
+#define MPS_PREC_FP64 0      /* element math and everything else in FP64 */
+#define MPS_PREC_FP32_ELEM 1 /* element math FP32; u/v/a, dt and energies FP64 */
+
+typedef struct mps_solver_s *mps_handle;
+
+typedef struct mps_mesh {
+    int32_t n_nodes;
+    int32_t n_elem;
+    int32_t nen; /* 8 (C3D8, 2x2x2 Gauss) or 4 (C3D4, 1 Gauss point) */
+    int32_t _pad;
+    const double *X;     /* [3*n_nodes] reference coordinates, SoA */
+    const int32_t *conn; /* [nen*n_elem] element-major, 0-based */
+} mps_mesh;
+
+typedef struct mps_material {
+    int32_t law;         /* MPS_LAW_* */
+    int32_t per_element; /* 0: E[0], nu[0], rho[0]; 1: arrays of n_elem */
+    const double *E;
+    const double *nu;
+    const double *rho;
+} mps_material;
+
+typedef struct mps_bcs {
+    int32_t n_dirichlet;
+    int32_t mode;           /* MPS_BC_* (one mode per solver, as the reference) */
+    int32_t vel_ramp_steps; /* MPS_BC_VELOCITY: steps to reach full amplitude */
+    int32_t _pad;
+    const int32_t *dof;  /* [n_dirichlet] node + n_nodes*dir, no duplicates */
+    const double *value; /* [n_dirichlet] U (ramp/inst) or V (velocity) */
+} mps_bcs;
+
+typedef struct mps_solver_cfg {
+    double t0, tf;       /* *Time t0, tf */
+    double scale;        /* *Scale Factor */
+    int64_t n_steps;     /* 0: reference time-window loop; >0: fixed step count (extension) */
+    int32_t precision;   /* MPS_PREC_* */
+    int32_t energy_every;/* 0: no energies; k: energies every k-th step */
+    int32_t history_cap; /* steps kept in the device history ring (0: 4096) */
+    int32_t device;      /* HIP device ordinal */
+} mps_solver_cfg;
+
+typedef struct mps_report {
+    int64_t steps;     /* steps completed by this solver so far */
+    double time;       /* timeRun after the last completed step */
+    double dt;         /* dt used by the last completed step */
+    int32_t status;    /* MPS_RUNNING, MPS_DONE or a negative MPS_ERR_* */
+    int32_t nonfinite; /* 1 if any non-finite value reached u[1] */
+    double device_ms;  /* device time spent in this call's steps */
+} mps_report;
+
+/* Build a solver: uploads mesh, material and BCs, computes the GP geometry,
+ * lumped mass and initial stable dt on the device.
+ * Replaces: classSolverExplicit ctor + calculate() set-up
+ * (solverExplicit.cpp:33-44, 63-200; computeLumpedMassVector :429-459;
+ *  calculateCriticalTimeStep(nod,elem) solvers.cpp:276-287). */
+int mps_create(const mps_mesh *mesh, const mps_material *mat, const mps_bcs *bcs,
+               const mps_solver_cfg *cfg, mps_handle *out);
+
+/* Hand over u, v, a from a previous solver (any pointer may be NULL = zeros).
+ * Valid only before the first step.
+ * Replaces: solvers::initValues (solvers.h:171-187) + setLastSolutionsToSystem
+ * (solverExplicit.cpp:577-596). */
+int mps_set_state(mps_handle h, const double *u, const double *v, const double *a);
+
+/* Copy the current u, v, a to host (any pointer may be NULL).
+ * Replaces: solvers::getFinalValues (solvers.h:171-187). */
+int mps_get_state(mps_handle h, double *u, double *v, double *a);
+
+/* Advance up to max_steps explicit steps (fewer if the time window or the
+ * fixed step count ends, or on error).  rep may be NULL.
+ * Replaces: the time loop of classSolverExplicit::calculate
+ * (solverExplicit.cpp:217-338). */
+int mps_run(mps_handle h, int64_t max_steps, mps_report *rep);
+
+/* Per-step history from the device ring: entries [first, first+count) with
+ * step index = entry + 1.  Energies are NaN on steps where they were not
+ * computed.  Any output pointer may be NULL.
+ * Replaces: classExtractData DEFO_ENERGY / KIN_ENERGY / VOLUME with the Sum
+ * operation (dataExtraction.cpp:569-776). */
+int mps_get_history(mps_handle h, int64_t first, int64_t count, double *time, double *dt,
+                    double *e_def, double *e_kin, double *volume);
+
+/* Lumped mass per node (identical for x, y, z).
+ * Replaces: classSolverExplicit::computeLumpedMassVector (solverExplicit.cpp:429-459). */
+int mps_get_mass(mps_handle h, double *mass);
+
+/* Internal force fint[3*n_nodes] for displacement u (device kernels; does not
+ * touch the solver state).
+ * Replaces: classSolverExplicit::computeInternalForce (solverExplicit.cpp:363-379). */
+int mps_internal_force(mps_handle h, const double *u, double *fint);
+
+/* Stable dt s*min_e min(l0,l(u))/c for displacement u (NULL: undeformed).
+ * Replaces: solvers::calculateCriticalTimeStep (solvers.cpp:276-320). */
+int mps_critical_dt(mps_handle h, const double *u, double *dt);
+
+/* Release the device memory of a solver.  Replaces: `delete nlsys` and the
+ * solver destructor (solverExplicit.cpp:360; main.cpp:183-185). */
+int mps_destroy(mps_handle h);
+
+/* Kernel timing: when enabled, mps_run records HIP events around every kernel
+ * launch on the solver's stream.  mps_kernel_times returns, per kernel kind
+ * [0 element, 1 node, 2 begin, 3 energy], the summed device milliseconds and the
+ * launch count since the last mps_profile call.
+ * Replaces: the wall/CPU timers of the reference (solvers.cpp:22-32,
+ * solverExplicit.cpp:358) with per-kernel device times. */
+int mps_profile(mps_handle h, int32_t enable);
+int mps_kernel_times(mps_handle h, double *ms_total /*[4]*/, int64_t *launches /*[4]*/);
+
+/* Message of the last failure on h (h may be NULL for mps_create failures). */
+const char *mps_last_error(mps_handle h);
+
+/* Name and properties of the HIP device used (no reference counterpart). */
+int mps_device_info(int32_t device, char *buf, int32_t len);
+
+/* ---------------------------------------------------------------------- */
+/* Batched form: many independent solves packed into one device arena and  */
+/* stepped together by the same kernels (no reference counterpart: the     */
+/* reference runs one solve per process).                                  */
+/* ---------------------------------------------------------------------- */
+
+typedef struct mps_batch_s *mps_batch;
+
+typedef struct mps_batch_cfg {
+    int64_t arena_bytes;  /* device memory reserved for instance state (0: 1 GiB) */
+    int32_t max_instances;/* instance slots (0: 1024) */
+    int32_t precision;    /* MPS_PREC_* for every instance */
+    int32_t steps_per_launch; /* steps captured per graph launch (0: 32) */
+    int32_t elem_block;   /* elements per workgroup (0: autotuned) */
+    int32_t grid_mult;    /* persistent grid = grid_mult * CUs workgroups (0: autotuned) */
+    int32_t device;
+} mps_batch_cfg;
+
+int mps_batch_create(const mps_batch_cfg *cfg, mps_batch *out);
+/* Admit one solve; returns its slot id in *slot.  MPS_ERR_CAPACITY when the
+ * arena or slot table is full.  The solve must use fixed n_steps (> 0). */
+int mps_batch_add(mps_batch b, const mps_mesh *mesh, const mps_material *mat,
+                  const mps_bcs *bcs, const mps_solver_cfg *cfg, int32_t *slot);
+/* Advance every active solve by up to max_steps (graph launches). */
+int mps_batch_step(mps_batch b, int64_t max_steps);
+/* Progress of one slot. */
+int mps_batch_query(mps_batch b, int32_t slot, mps_report *rep);
+/* Copy a slot's state out (any pointer NULL to skip). */
+int mps_batch_get_state(mps_batch b, int32_t slot, double *u, double *v, double *a);
+/* Free a slot for reuse. */
+int mps_batch_remove(mps_batch b, int32_t slot);
+/* Autotune elem_block / grid_mult on the current batch content; writes the choice. */
+int mps_batch_autotune(mps_batch b, int32_t *elem_block, int32_t *grid_mult);
+int mps_batch_destroy(mps_batch b);
+const char *mps_batch_last_error(mps_batch b);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,970 @@
+/*
+ * mps_oracle.c -- CPU restatement of the reference explicit solver.
+ *
+ * TEST INFRASTRUCTURE ONLY (the checker).  Only tests/, the smoke check and
+ * the benchmark's cpu_baseline leg may load this library; the product path
+ * (libmps_gpu.so) never links or calls it.
+ *
+ * Built with -O3 -ffp-contract=off (no FMA contraction), mirroring the
+ * reference's x86-64 build flags (Makefile:18,34 of the reference: no -march,
+ * no -ffast-math).  Each function restates one reference function and keeps
+ * its floating-point operation order; the citation names the file:line.
+ *
+ * Two summation orders for the internal force are offered:
+ *   ORC_SUM_REFERENCE  per-Gauss-point scatter into fint in GP order, exactly
+ *                      as solverExplicit.cpp:363-379 (sequential only);
+ *   ORC_SUM_ELEMENT    per-element sums over the element's GPs followed by a
+ *                      node-centric gather in ascending (element, slot) order
+ *                      (the association the GPU path uses).  For tet4 (one
+ *                      GP per element) both orders are bitwise identical.
+ */
+#define _GNU_SOURCE
+#include "mps_oracle.h"
+
+#include <math.h>
+#include <pthread.h>
+#include <stdlib.h>
+#include <string.h>
+
+/* ------------------------------------------------------------------ */
+/* 3x3 algebra                                                         */
+/* ------------------------------------------------------------------ */
+
+/* determinantTensor, 3x3 branch: maths.cpp:1564-1577 */
+double orc_det3(const double *A)
+{
+    return A[0] * (A[4] * A[8] - A[7] * A[5]) -
+           A[1] * (A[3] * A[8] - A[6] * A[5]) +
+           A[2] * (A[3] * A[7] - A[6] * A[4]);
+}
+
+/* inverse (adjugate / det), 3x3 branch: maths.cpp:2749-2767 */
+void orc_inverse3(const double *A, double *Ainv)
+{
+    double det = orc_det3(A);
+    double idet = 1.0 / det;
+    Ainv[0] = idet * (A[4] * A[8] - A[5] * A[7]);
+    Ainv[3] = -idet * (A[3] * A[8] - A[5] * A[6]);
+    Ainv[6] = idet * (A[3] * A[7] - A[4] * A[6]);
+    Ainv[1] = -idet * (A[1] * A[8] - A[2] * A[7]);
+    Ainv[4] = idet * (A[0] * A[8] - A[2] * A[6]);
+    Ainv[7] = -idet * (A[0] * A[7] - A[1] * A[6]);
+    Ainv[2] = idet * (A[1] * A[5] - A[2] * A[4]);
+    Ainv[5] = -idet * (A[0] * A[5] - A[2] * A[3]);
+    Ainv[8] = idet * (A[0] * A[4] - A[1] * A[3]);
+}
+
+/* multGeneralMatrices: C[m x ll] = A[m x kk] * B[kk x ll], k innermost,
+ * accumulating from zero: maths.cpp:1679-1693 */
+static void mat_general(const double *A, int m, int kk, const double *B, int ll, double *C)
+{
+    for (int i = 0; i < m * ll; i++) C[i] = 0.0;
+    for (int i = 0; i < m; i++)
+        for (int j = 0; j < ll; j++)
+            for (int k = 0; k < kk; k++)
+                C[i * ll + j] += A[i * kk + k] * B[k * ll + j];
+}
+
+/* ------------------------------------------------------------------ */
+/* Shape functions and quadrature                                      */
+/* ------------------------------------------------------------------ */
+
+/* classLinearHexahedra::getInterpolationParameters: classLinearHexahedra.cpp:344-381 */
+void orc_hex8_shape(const double *xw, double *phi, double *nd)
+{
+    const double x = xw[0], y = xw[1], z = xw[2];
+    phi[0] = 0.125 * (1 - x) * (1 - y) * (1 - z);
+    phi[1] = 0.125 * (1 + x) * (1 - y) * (1 - z);
+    phi[2] = 0.125 * (1 + x) * (1 + y) * (1 - z);
+    phi[3] = 0.125 * (1 - x) * (1 + y) * (1 - z);
+    phi[4] = 0.125 * (1 - x) * (1 - y) * (1 + z);
+    phi[5] = 0.125 * (1 + x) * (1 - y) * (1 + z);
+    phi[6] = 0.125 * (1 + x) * (1 + y) * (1 + z);
+    phi[7] = 0.125 * (1 - x) * (1 + y) * (1 + z);
+    nd[0] = -0.125 * (1 - y) * (1 - z);
+    nd[3] = 0.125 * (1 - y) * (1 - z);
+    nd[6] = 0.125 * (1 + y) * (1 - z);
+    nd[9] = -0.125 * (1 + y) * (1 - z);
+    nd[12] = -0.125 * (1 - y) * (1 + z);
+    nd[15] = 0.125 * (1 - y) * (1 + z);
+    nd[18] = 0.125 * (1 + y) * (1 + z);
+    nd[21] = -0.125 * (1 + y) * (1 + z);
+    nd[1] = -0.125 * (1 - x) * (1 - z);
+    nd[4] = -0.125 * (1 + x) * (1 - z);
+    nd[7] = 0.125 * (1 + x) * (1 - z);
+    nd[10] = 0.125 * (1 - x) * (1 - z);
+    nd[13] = -0.125 * (1 - x) * (1 + z);
+    nd[16] = -0.125 * (1 + x) * (1 + z);
+    nd[19] = 0.125 * (1 + x) * (1 + z);
+    nd[22] = 0.125 * (1 - x) * (1 + z);
+    nd[2] = -0.125 * (1 - x) * (1 - y);
+    nd[5] = -0.125 * (1 + x) * (1 - y);
+    nd[8] = -0.125 * (1 + x) * (1 + y);
+    nd[11] = -0.125 * (1 - x) * (1 + y);
+    nd[14] = 0.125 * (1 - x) * (1 - y);
+    nd[17] = 0.125 * (1 + x) * (1 - y);
+    nd[20] = 0.125 * (1 + x) * (1 + y);
+    nd[23] = 0.125 * (1 - x) * (1 + y);
+}
+
+/* classLinearTetrahedra::getInterpolationParameters: classLinearTetrahedra.cpp:270-292 */
+void orc_tet4_shape(const double *xw, double *phi, double *nd)
+{
+    phi[0] = -1.0 * xw[0] - 1.0 * xw[1] - 1.0 * xw[2] + 1.0;
+    phi[1] = 1.0 * xw[0];
+    phi[2] = 1.0 * xw[1];
+    phi[3] = 1.0 * xw[2];
+    nd[0] = -1.0 - 0.0 - 0.0 + 0.0;
+    nd[3] = 1.0;
+    nd[6] = 0.0;
+    nd[9] = 0.0;
+    nd[1] = -0.0 - 1.0 - 0.0 + 0.0;
+    nd[4] = 0.0;
+    nd[7] = 1.0;
+    nd[10] = 0.0;
+    nd[2] = -0.0 - 0.0 - 1.0 + 0.0;
+    nd[5] = 0.0;
+    nd[8] = 0.0;
+    nd[11] = 1.0;
+}
+
+/* GPsHexahedron3DValues(order 2) / GPsTetrahedron3DValues(order 1):
+ * GPsDistribution.cpp:392-432 and 339-343.  xw is [ngp][4] (xi,eta,zeta,w). */
+int32_t orc_quadrature(int32_t nen, double *xw)
+{
+    if (nen == 8) {
+        static const double s[8][3] = {{-1, -1, -1}, {1, -1, -1}, {1, 1, -1}, {-1, 1, -1},
+                                       {-1, -1, 1},  {1, -1, 1},  {1, 1, 1},  {-1, 1, 1}};
+        for (int g = 0; g < 8; g++) {
+            for (int d = 0; d < 3; d++) xw[g * 4 + d] = s[g][d] > 0 ? 0.577350269 : -0.577350269;
+            xw[g * 4 + 3] = 1;
+        }
+        return 8;
+    }
+    if (nen == 4) {
+        xw[0] = 0.250000000000;
+        xw[1] = 0.25;
+        xw[2] = 0.25;
+        xw[3] = 0.166666666667;
+        return 1;
+    }
+    return 0;
+}
+
+/* One Gauss point of classLinearHexahedra::defineGPs (classLinearHexahedra.cpp:242-342)
+ * or classLinearTetrahedra::defineGPs (classLinearTetrahedra.cpp:171-268), plus the
+ * classGPs constructor (classGPs.cpp:24-31): wJ = weight*|det J|.
+ * xyz is node-major [a*3+d]; weight is 1*xw[3] for hex, xw[3] for tet. */
+void orc_define_gp(int32_t nen, const double *xyz, const double *xw,
+                   double *phi, double *dphi, double *wJ, double *detJ)
+{
+    double xyzCoor[24], nd[24], Jac[9], invJ[9];
+    for (int i = 0; i < nen; i++) {
+        xyzCoor[i] = xyz[i * 3 + 0];
+        xyzCoor[nen + i] = xyz[i * 3 + 1];
+        xyzCoor[2 * nen + i] = xyz[i * 3 + 2];
+    }
+    if (nen == 8) orc_hex8_shape(xw, phi, nd);
+    else orc_tet4_shape(xw, phi, nd);
+    mat_general(xyzCoor, 3, nen, nd, 3, Jac);
+    double J = orc_det3(Jac);
+    orc_inverse3(Jac, invJ);
+    double weight = (nen == 8) ? 1 * xw[3] : xw[3];
+    mat_general(nd, nen, 3, invJ, 3, dphi);
+    *wJ = weight * fabs(J);
+    if (detJ) *detJ = fabs(J);
+}
+
+/* ------------------------------------------------------------------ */
+/* Characteristic length (stable time step)                            */
+/* ------------------------------------------------------------------ */
+
+/* areaTriangle, 3D branch: geometry.cpp:112-130.  tri = {x1,x2,x3,y1,y2,y3,z1,z2,z3} */
+double orc_area_triangle(const double *t)
+{
+    double A[3], B[3], C[3];
+    for (int i = 0; i < 3; i++) {
+        A[i] = t[i * 3] - t[i * 3 + 2];
+        B[i] = t[i * 3 + 1] - t[i * 3 + 2];
+    }
+    /* crossProduct: geometry.cpp:180-190 */
+    C[0] = A[1] * B[2] - A[2] * B[1];
+    C[1] = A[2] * B[0] - A[0] * B[2];
+    C[2] = A[0] * B[1] - A[1] * B[0];
+    return 0.5 * sqrt(C[0] * C[0] + C[1] * C[1] + C[2] * C[2]);
+}
+
+/* areaSquare: triangles (0,1,2)+(0,2,3): geometry.cpp:143-174.
+ * quad = {x0..x3, y0..y3, z0..z3} */
+static double area_square(const double *q)
+{
+    double t1[9], t2[9];
+    for (int i = 0; i < 3; i++) {
+        t1[i * 3] = q[i * 4];
+        t1[i * 3 + 1] = q[i * 4 + 1];
+        t1[i * 3 + 2] = q[i * 4 + 2];
+        t2[i * 3] = q[i * 4];
+        t2[i * 3 + 1] = q[i * 4 + 2];
+        t2[i * 3 + 2] = q[i * 4 + 3];
+    }
+    double a1 = orc_area_triangle(t1);
+    double a2 = orc_area_triangle(t2);
+    return a1 + a2;
+}
+
+/* inscribedSphereHex: geometry.cpp:308-355 (V is the corner parallelepiped) */
+static double inscribed_sphere_hex(const double *p /* node-major [8][3] */)
+{
+    static const int faces[6][4] = {{0, 1, 2, 3}, {4, 5, 6, 7}, {0, 1, 5, 4},
+                                    {1, 2, 6, 5}, {2, 3, 7, 6}, {0, 3, 7, 4}};
+    double s[6];
+    for (int f = 0; f < 6; f++) {
+        double q[12];
+        for (int d = 0; d < 3; d++)
+            for (int k = 0; k < 4; k++) q[d * 4 + k] = p[faces[f][k] * 3 + d];
+        s[f] = area_square(q);
+    }
+    double A = fabs(s[0]) + fabs(s[1]) + fabs(s[2]) + fabs(s[3]) + fabs(s[4]) + fabs(s[5]);
+    double M[9] = {p[3] - p[0], p[4] - p[1], p[5] - p[2],
+                   p[9] - p[0], p[10] - p[1], p[11] - p[2],
+                   p[12] - p[0], p[13] - p[1], p[14] - p[2]};
+    double V = fabs(orc_det3(M));
+    return 3.0 * V / A;
+}
+
+/* inscribedSphere (tetrahedron): geometry.cpp:264-301 */
+static double inscribed_sphere_tet(const double *p /* node-major [4][3] */)
+{
+    static const int tris[4][3] = {{0, 1, 2}, {1, 2, 3}, {0, 2, 3}, {0, 1, 3}};
+    double s[4];
+    for (int f = 0; f < 4; f++) {
+        double t[9];
+        for (int d = 0; d < 3; d++)
+            for (int k = 0; k < 3; k++) t[d * 3 + k] = p[tris[f][k] * 3 + d];
+        s[f] = orc_area_triangle(t);
+    }
+    double A = fabs(s[0]) + fabs(s[1]) + fabs(s[2]) + fabs(s[3]);
+    double M[9] = {p[3] - p[0], p[4] - p[1], p[5] - p[2],
+                   p[6] - p[0], p[7] - p[1], p[8] - p[2],
+                   p[9] - p[0], p[10] - p[1], p[11] - p[2]};
+    double V = 1.0 / 6.0 * fabs(orc_det3(M));
+    return 3.0 * V / A;
+}
+
+/* characteristicLength = 2*inscribed radius:
+ * classLinearHexahedra.cpp:209-234, classLinearTetrahedra.cpp:146-163 */
+double orc_char_length(int32_t nen, const double *xyz)
+{
+    double r = (nen == 8) ? inscribed_sphere_hex(xyz) : inscribed_sphere_tet(xyz);
+    return 2 * r;
+}
+
+/* ------------------------------------------------------------------ */
+/* Constitutive laws                                                   */
+/* ------------------------------------------------------------------ */
+
+/* Material constants: hyperElasticNeoHookean.cpp:28-30 and 180-181 */
+void orc_material(double E, double nu, double *lambda, double *mu, double *K1)
+{
+    *lambda = (E * nu) / ((1. + nu) * (1. - 2. * nu));
+    *mu = 0.5 * E / (1. + nu);
+    *K1 = E / (3 * (1 - 2 * nu));
+}
+
+/* soundSpeed: hyperElasticNeoHookean.cpp:51-54 (identical at 202-205) */
+double orc_sound_speed(double E, double nu, double rho)
+{
+    double lambda, mu, K1;
+    orc_material(E, nu, &lambda, &mu, &K1);
+    return sqrt((lambda + 2 * mu) / rho);
+}
+
+/* classHyperElastic3DCompressibleNeoHookean::updateConstitutive (no tangent):
+ * hyperElasticNeoHookean.cpp:221-257 with TensorOperations::determinant
+ * (TensorOperations.h:56-74), rightCauchy (:478-492), inverseTensor2 (:22-26),
+ * classTensor2::addDiagonal/axpy (maths.h:350-366) and dot (:215-229). */
+static void pk1_compressible(const double *F, double lambda, double mu, double *P)
+{
+    double J = orc_det3(F);
+    double C[9], invC[9], S[9];
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) {
+            double c = 0;
+            for (int k = 0; k < 3; k++) c += F[k * 3 + i] * F[k * 3 + j];
+            C[i * 3 + j] = c;
+        }
+    orc_inverse3(C, invC);
+    for (int i = 0; i < 9; i++) S[i] = 0;
+    for (int i = 0; i < 3; i++) S[i * 3 + i] += mu;
+    double alpha = lambda * log(J) - mu;
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) S[i * 3 + j] += alpha * invC[i * 3 + j];
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) {
+            double p = 0;
+            for (int k = 0; k < 3; k++) p += F[i * 3 + k] * S[k * 3 + j] * 1.;
+            P[i * 3 + j] = p;
+        }
+}
+
+/* classHyperElasticNeoHookean::updateConstitutive (no tangent):
+ * hyperElasticNeoHookean.cpp:70-126 with leftCauchy (TensorOperations.h:517-531),
+ * classTensor2::trace (maths.h:303-311) and dotSecondTranspose (:577-591). */
+static void pk1_belytschko(const double *F, double mu, double K1, double *P)
+{
+    double B[9], invF[9], sig[9];
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) {
+            double b = 0;
+            for (int k = 0; k < 3; k++) b += F[i * 3 + k] * F[j * 3 + k];
+            B[i * 3 + j] = b;
+        }
+    double J = orc_det3(F);
+    orc_inverse3(F, invF);
+    double trB = 0;
+    for (int i = 0; i < 3; i++) trB += B[i * 3 + i];
+    for (int i = 0; i < 9; i++) sig[i] = 0;
+    double fact1 = pow(J, 5.0 / (double)3);
+    double fact2 = mu / fact1;
+    for (int i = 0; i < 9; i++) sig[i] += fact2 * B[i];
+    double fact3 = -(mu / (double)3) * trB / fact1 + K1 * (J - 1.0);
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) sig[i * 3 + j] += fact3 * (i == j ? 1.0 : 0.0);
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) {
+            double p = 0;
+            for (int k = 0; k < 3; k++) p += sig[i * 3 + k] * invF[j * 3 + k] * J;
+            P[i * 3 + j] = p;
+        }
+}
+
+void orc_pk1(int32_t law, const double *F, double E, double nu, double *P)
+{
+    double lambda, mu, K1;
+    orc_material(E, nu, &lambda, &mu, &K1);
+    if (law == ORC_LAW_NH_BELYTSCHKO) pk1_belytschko(F, mu, K1, P);
+    else pk1_compressible(F, lambda, mu, P);
+}
+
+/* classHyperElastic3DCompressibleNeoHookean::defoEnergy: hyperElasticNeoHookean.cpp:291-304.
+ * The Belytschko law does not override defoEnergy and returns 0 (constitutiveModels.h:91-94). */
+static double defo_energy_lm(int32_t law, const double *F, double lambda, double mu)
+{
+    if (law == ORC_LAW_NH_BELYTSCHKO) return 0.0;
+    double J = orc_det3(F);
+    double trC = 0;
+    for (int i = 0; i < 3; i++) {
+        double c = 0;
+        for (int k = 0; k < 3; k++) c += F[k * 3 + i] * F[k * 3 + i];
+        trC += c;
+    }
+    return 0.5 * lambda * log(J) * log(J) - mu * log(J) + 0.5 * mu * trC;
+}
+
+double orc_defo_energy(int32_t law, const double *F, double E, double nu)
+{
+    double lambda, mu, K1;
+    orc_material(E, nu, &lambda, &mu, &K1);
+    return defo_energy_lm(law, F, lambda, mu);
+}
+
+/* ------------------------------------------------------------------ */
+/* Per Gauss point kinematics, force and mass                          */
+/* ------------------------------------------------------------------ */
+
+/* constitutiveManager::computeKinematicVariables: constitutiveManager.cpp:150-164.
+ * uloc is [a*3+i] (node a, direction i). */
+void orc_kinematics(int32_t nen, const double *dphi, const double *uloc, double *F)
+{
+    for (int i = 0; i < 9; i++) F[i] = 0.;
+    for (int i = 0; i < 3; i++) {
+        F[i * 3 + i] += 1.;
+        for (int a = 0; a < nen; a++)
+            for (int j = 0; j < 3; j++) F[i * 3 + j] += dphi[a * 3 + j] * uloc[a * 3 + i];
+    }
+}
+
+/* MechanicalTerm::getForceVector: constitutiveAlgorithms.cpp:33-57.
+ * f is indexed like vForce: f[a + i*nen]. */
+void orc_gp_force(int32_t nen, double wJ, const double *P, const double *dphi, double *f)
+{
+    for (int k = 0; k < 3 * nen; k++) f[k] = 0;
+    for (int a = 0; a < nen; a++)
+        for (int i = 0; i < 3; i++)
+            for (int J = 0; J < 3; J++) f[a + i * nen] += wJ * P[i * 3 + J] * dphi[a * 3 + J];
+}
+
+/* MechanicalTerm::getLumpedMassVector, linear-element branch:
+ * constitutiveAlgorithms.cpp:191-205.  m[a] (identical for the three directions). */
+void orc_gp_lumped_mass(int32_t nen, double wJ, double rho, const double *phi, double *m)
+{
+    for (int a = 0; a < nen; a++) {
+        double s = 0;
+        for (int b = 0; b < nen; b++) s += wJ * rho * phi[a] * phi[b];
+        m[a] = s;
+    }
+}
+
+/* ------------------------------------------------------------------ */
+/* Solver state                                                        */
+/* ------------------------------------------------------------------ */
+
+typedef struct {
+    const orc_problem *p;
+    int ngp, nen, N, Ne;
+    double *phi;       /* [Ne*ngp*nen] */
+    double *dphi;      /* [Ne*ngp*nen*3] */
+    double *wJ;        /* [Ne*ngp] */
+    double *l0;        /* [Ne] initial characteristic length */
+    double *c;         /* [Ne] sound speed */
+    double *lam, *mu, *K1, *rho;
+    double *mass;      /* [N] */
+    int8_t *active;    /* [3N] */
+    int32_t *rowptr, *col; /* CSR node -> (e*nen + a) ascending */
+    double *fe;        /* [Ne*nen*3] element forces, [(e*nen+a)*3+i] */
+    double *fint;      /* [3N] */
+    /* threading */
+    int nt;
+    double *tmin;      /* per-thread partials */
+    double *tsum;      /* per-thread partials, 3 per thread */
+} orc_state;
+
+static double mat_at(const double *v, int per_elem, int e) { return per_elem ? v[e] : v[0]; }
+
+static void gather_xyz(const orc_state *s, int e, const double *u, double *xyz)
+{
+    const int32_t *cn = s->p->conn + (size_t)e * s->nen;
+    for (int a = 0; a < s->nen; a++)
+        for (int d = 0; d < 3; d++) {
+            double x = s->p->X[(size_t)d * s->N + cn[a]];
+            if (u) x += u[(size_t)d * s->N + cn[a]];
+            xyz[a * 3 + d] = x;
+        }
+}
+
+static int state_init(orc_state *s, const orc_problem *p)
+{
+    memset(s, 0, sizeof(*s));
+    s->p = p;
+    s->nen = p->nen;
+    s->N = p->n_nodes;
+    s->Ne = p->n_elem;
+    double xw[32];
+    s->ngp = orc_quadrature(p->nen, xw);
+    if (s->ngp == 0) return ORC_ERR_ARG;
+    size_t Ne = (size_t)s->Ne, ng = (size_t)s->ngp, nen = (size_t)s->nen, N = (size_t)s->N;
+    s->phi = malloc(sizeof(double) * Ne * ng * nen);
+    s->dphi = malloc(sizeof(double) * Ne * ng * nen * 3);
+    s->wJ = malloc(sizeof(double) * Ne * ng);
+    s->l0 = malloc(sizeof(double) * Ne);
+    s->c = malloc(sizeof(double) * Ne);
+    s->lam = malloc(sizeof(double) * Ne);
+    s->mu = malloc(sizeof(double) * Ne);
+    s->K1 = malloc(sizeof(double) * Ne);
+    s->rho = malloc(sizeof(double) * Ne);
+    s->mass = calloc(N, sizeof(double));
+    s->active = malloc(3 * N);
+    s->rowptr = calloc(N + 1, sizeof(int32_t));
+    s->col = malloc(sizeof(int32_t) * Ne * nen);
+    s->fe = malloc(sizeof(double) * Ne * nen * 3);
+    s->fint = malloc(sizeof(double) * 3 * N);
+    if (!s->phi || !s->dphi || !s->wJ || !s->l0 || !s->c || !s->lam || !s->mu || !s->K1 ||
+        !s->rho || !s->mass || !s->active || !s->rowptr || !s->col || !s->fe || !s->fint)
+        return ORC_ERR_MEM;
+    /* GP geometry (classLinear*::defineGPs), initial char length (element ctor) */
+    for (size_t e = 0; e < Ne; e++) {
+        double xyz[24];
+        gather_xyz(s, (int)e, NULL, xyz);
+        for (size_t g = 0; g < ng; g++) {
+            size_t gi = e * ng + g;
+            orc_define_gp(p->nen, xyz, xw + g * 4, s->phi + gi * nen, s->dphi + gi * nen * 3,
+                          s->wJ + gi, NULL);
+        }
+        s->l0[e] = orc_char_length(p->nen, xyz);
+        double E = mat_at(p->E, p->mat_per_elem, (int)e), nu = mat_at(p->nu, p->mat_per_elem, (int)e);
+        s->rho[e] = mat_at(p->rho, p->mat_per_elem, (int)e);
+        orc_material(E, nu, s->lam + e, s->mu + e, s->K1 + e);
+        s->c[e] = sqrt((s->lam[e] + 2 * s->mu[e]) / s->rho[e]);
+    }
+    /* active DOF map (dispBCManagement, inputFile.cpp:3185-3212) */
+    memset(s->active, 1, 3 * N);
+    for (int k = 0; k < p->n_dir; k++) {
+        int32_t d = p->dir_dof[k];
+        if (d < 0 || (size_t)d >= 3 * N) return ORC_ERR_ARG;
+        s->active[d] = 0;
+    }
+    /* CSR node -> (element, slot), ascending element then slot */
+    for (size_t e = 0; e < Ne; e++)
+        for (size_t a = 0; a < nen; a++) {
+            int32_t n = p->conn[e * nen + a];
+            if (n < 0 || n >= s->N) return ORC_ERR_ARG;
+            s->rowptr[n + 1]++;
+        }
+    for (size_t n = 0; n < N; n++) s->rowptr[n + 1] += s->rowptr[n];
+    int32_t *fill = malloc(sizeof(int32_t) * N);
+    if (!fill) return ORC_ERR_MEM;
+    memcpy(fill, s->rowptr, sizeof(int32_t) * N);
+    for (size_t e = 0; e < Ne; e++)
+        for (size_t a = 0; a < nen; a++) {
+            int32_t n = p->conn[e * nen + a];
+            s->col[fill[n]++] = (int32_t)(e * nen + a);
+        }
+    free(fill);
+    return ORC_OK;
+}
+
+static void state_free(orc_state *s)
+{
+    free(s->phi); free(s->dphi); free(s->wJ); free(s->l0); free(s->c);
+    free(s->lam); free(s->mu); free(s->K1); free(s->rho); free(s->mass);
+    free(s->active); free(s->rowptr); free(s->col); free(s->fe); free(s->fint);
+    free(s->tmin); free(s->tsum);
+}
+
+/* ------------------------------------------------------------------ */
+/* Minimal thread pool (deterministic static partition)                */
+/* ------------------------------------------------------------------ */
+
+typedef void (*task_fn)(orc_state *s, int tid, int nt, void *arg);
+
+typedef struct {
+    orc_state *s;
+    int nt;
+    pthread_barrier_t start, done;
+    task_fn fn;
+    void *arg;
+    int quit;
+} pool_t;
+
+typedef struct { pool_t *pool; int tid; } worker_arg;
+
+static void *pool_worker(void *vp)
+{
+    worker_arg *w = (worker_arg *)vp;
+    pool_t *pl = w->pool;
+    for (;;) {
+        pthread_barrier_wait(&pl->start);
+        if (pl->quit) break;
+        pl->fn(pl->s, w->tid, pl->nt, pl->arg);
+        pthread_barrier_wait(&pl->done);
+    }
+    return NULL;
+}
+
+static pool_t *g_pool = NULL;
+
+static void run_par(orc_state *s, task_fn fn, void *arg)
+{
+    if (s->nt <= 1 || !g_pool) {
+        fn(s, 0, 1, arg);
+        return;
+    }
+    g_pool->fn = fn;
+    g_pool->arg = arg;
+    pthread_barrier_wait(&g_pool->start);
+    fn(s, 0, s->nt, arg);
+    pthread_barrier_wait(&g_pool->done);
+}
+
+static void range_of(size_t n, int tid, int nt, size_t *b, size_t *e)
+{
+    *b = n * (size_t)tid / (size_t)nt;
+    *e = n * (size_t)(tid + 1) / (size_t)nt;
+}
+
+/* ------------------------------------------------------------------ */
+/* Step components                                                     */
+/* ------------------------------------------------------------------ */
+
+/* Lumped mass: solverExplicit.cpp:429-459 (GP-order accumulation) or per-element
+ * sums + node gather (ORC_SUM_ELEMENT).  Stored per node. */
+static void compute_mass(orc_state *s)
+{
+    size_t N = s->N, Ne = s->Ne, ng = s->ngp, nen = s->nen;
+    for (size_t n = 0; n < N; n++) s->mass[n] = 0;
+    if (s->p->sum_order == ORC_SUM_REFERENCE) {
+        for (size_t e = 0; e < Ne; e++)
+            for (size_t g = 0; g < ng; g++) {
+                size_t gi = e * ng + g;
+                double m[8];
+                orc_gp_lumped_mass(s->nen, s->wJ[gi], s->rho[e], s->phi + gi * nen, m);
+                for (size_t a = 0; a < nen; a++) s->mass[s->p->conn[e * nen + a]] += m[a];
+            }
+    } else {
+        double *me = malloc(sizeof(double) * Ne * nen);
+        for (size_t e = 0; e < Ne; e++) {
+            for (size_t a = 0; a < nen; a++) me[e * nen + a] = 0;
+            for (size_t g = 0; g < ng; g++) {
+                size_t gi = e * ng + g;
+                double m[8];
+                orc_gp_lumped_mass(s->nen, s->wJ[gi], s->rho[e], s->phi + gi * nen, m);
+                for (size_t a = 0; a < nen; a++) me[e * nen + a] += m[a];
+            }
+        }
+        for (size_t n = 0; n < N; n++) {
+            double acc = 0;
+            for (int32_t k = s->rowptr[n]; k < s->rowptr[n + 1]; k++) acc += me[s->col[k]];
+            s->mass[n] = acc;
+        }
+        free(me);
+    }
+}
+
+/* Per-element work for ORC_SUM_ELEMENT: fe[e] = sum over GPs of the GP force. */
+static void element_force(orc_state *s, size_t e, const double *u)
+{
+    size_t ng = s->ngp, nen = s->nen, N = s->N;
+    const int32_t *cn = s->p->conn + e * nen;
+    double uloc[24], F[9], P[9], f[24];
+    for (size_t a = 0; a < nen; a++)
+        for (int i = 0; i < 3; i++) uloc[a * 3 + i] = u[cn[a] + N * i];
+    double *fe = s->fe + e * nen * 3;
+    for (size_t k = 0; k < nen * 3; k++) fe[k] = 0;
+    for (size_t g = 0; g < ng; g++) {
+        size_t gi = e * ng + g;
+        orc_kinematics(s->nen, s->dphi + gi * nen * 3, uloc, F);
+        if (s->p->law == ORC_LAW_NH_BELYTSCHKO) pk1_belytschko(F, s->mu[e], s->K1[e], P);
+        else pk1_compressible(F, s->lam[e], s->mu[e], P);
+        orc_gp_force(s->nen, s->wJ[gi], P, s->dphi + gi * nen * 3, f);
+        for (size_t a = 0; a < nen; a++)
+            for (int i = 0; i < 3; i++) fe[a * 3 + i] += f[a + i * nen];
+    }
+}
+
+static void task_elements(orc_state *s, int tid, int nt, void *arg)
+{
+    const double *u = (const double *)arg;
+    size_t b, e;
+    range_of(s->Ne, tid, nt, &b, &e);
+    for (size_t k = b; k < e; k++) element_force(s, k, u);
+}
+
+static void task_gather(orc_state *s, int tid, int nt, void *arg)
+{
+    (void)arg;
+    size_t b, e, N = s->N;
+    range_of(N, tid, nt, &b, &e);
+    for (size_t n = b; n < e; n++)
+        for (int i = 0; i < 3; i++) {
+            double acc = 0;
+            for (int32_t k = s->rowptr[n]; k < s->rowptr[n + 1]; k++) acc += s->fe[(size_t)s->col[k] * 3 + i];
+            s->fint[n + N * i] = acc;
+        }
+}
+
+/* computeInternalForce: solverExplicit.cpp:363-379 (+ computeState, solvers.cpp:615-641) */
+static void internal_force(orc_state *s, const double *u)
+{
+    size_t N = s->N, Ne = s->Ne, ng = s->ngp, nen = s->nen;
+    if (s->p->sum_order == ORC_SUM_REFERENCE) {
+        for (size_t k = 0; k < 3 * N; k++) s->fint[k] = 0;
+        double uloc[24], F[9], P[9], f[24];
+        for (size_t e = 0; e < Ne; e++) {
+            const int32_t *cn = s->p->conn + e * nen;
+            for (size_t a = 0; a < nen; a++)
+                for (int i = 0; i < 3; i++) uloc[a * 3 + i] = u[cn[a] + N * i];
+            for (size_t g = 0; g < ng; g++) {
+                size_t gi = e * ng + g;
+                orc_kinematics(s->nen, s->dphi + gi * nen * 3, uloc, F);
+                if (s->p->law == ORC_LAW_NH_BELYTSCHKO) pk1_belytschko(F, s->mu[e], s->K1[e], P);
+                else pk1_compressible(F, s->lam[e], s->mu[e], P);
+                orc_gp_force(s->nen, s->wJ[gi], P, s->dphi + gi * nen * 3, f);
+                /* classGPs::getLocalDofs (classGPs.cpp:37-67): vDofs[a + n*i] = node_a + N*i */
+                for (int i = 0; i < 3; i++)
+                    for (size_t a = 0; a < nen; a++) s->fint[cn[a] + N * i] += f[a + i * nen];
+            }
+        }
+    } else {
+        run_par(s, task_elements, (void *)u);
+        run_par(s, task_gather, NULL);
+    }
+}
+
+/* calculateCriticalTimeStep(nod, elem, uK): solvers.cpp:296-320 (u == NULL gives the
+ * undeformed variant solvers.cpp:276-287, since min(l0, l0) = l0). */
+static void task_dt(orc_state *s, int tid, int nt, void *arg)
+{
+    const double *u = (const double *)arg;
+    size_t b, e;
+    range_of(s->Ne, tid, nt, &b, &e);
+    double best = 0;
+    int have = 0;
+    for (size_t k = b; k < e; k++) {
+        double v;
+        if (u) {
+            double xyz[24];
+            gather_xyz(s, (int)k, u, xyz);
+            double dist = orc_char_length(s->nen, xyz);
+            double mn = (dist < s->l0[k]) ? dist : s->l0[k]; /* std::min(minDist, dist) */
+            v = mn / s->c[k];
+        } else {
+            v = s->l0[k] / s->c[k];
+        }
+        if (!have) { best = v; have = 1; }
+        else if (v < best) best = v; /* std::min_element */
+    }
+    s->tmin[tid] = have ? best : INFINITY;
+}
+
+static double critical_dt(orc_state *s, const double *u)
+{
+    run_par(s, task_dt, (void *)u);
+    double best = s->tmin[0];
+    int nt = (s->nt > 1 && g_pool) ? s->nt : 1;
+    for (int t = 1; t < nt; t++)
+        if (s->tmin[t] < best) best = s->tmin[t];
+    return best;
+}
+
+/* Energies: defoEnergy / kineticEnergy / volume with Sum operation,
+ * dataExtraction.cpp:569-629, 631-707, 709-776 (GP order). */
+static void task_energy(orc_state *s, int tid, int nt, void *arg)
+{
+    const double *const *uv = (const double *const *)arg;
+    const double *u = uv[0], *v = uv[1];
+    size_t b, e, N = s->N, ng = s->ngp, nen = s->nen;
+    range_of(s->Ne, tid, nt, &b, &e);
+    double ed = 0, ek = 0, vol = 0;
+    double uloc[24], F[9];
+    for (size_t k = b; k < e; k++) {
+        const int32_t *cn = s->p->conn + k * nen;
+        for (size_t a = 0; a < nen; a++)
+            for (int i = 0; i < 3; i++) uloc[a * 3 + i] = u[cn[a] + N * i];
+        for (size_t g = 0; g < ng; g++) {
+            size_t gi = k * ng + g;
+            orc_kinematics(s->nen, s->dphi + gi * nen * 3, uloc, F);
+            double wJ = s->wJ[gi];
+            double val = defo_energy_lm(s->p->law, F, s->lam[k], s->mu[k]);
+            ed += val * wJ;
+            double vel[3] = {0., 0., 0.};
+            const double *phi = s->phi + gi * nen;
+            for (size_t a = 0; a < nen; a++)
+                for (int j = 0; j < 3; j++) vel[j] += phi[a] * v[cn[a] + j * N];
+            double kv = 0;
+            for (int j = 0; j < 3; j++) kv += 0.5 * s->rho[k] * vel[j] * vel[j];
+            ek += kv * wJ;
+            vol += wJ * orc_det3(F);
+        }
+    }
+    s->tsum[tid * 3 + 0] = ed;
+    s->tsum[tid * 3 + 1] = ek;
+    s->tsum[tid * 3 + 2] = vol;
+}
+
+static void energies(orc_state *s, const double *u, const double *v, double *out)
+{
+    const double *uv[2] = {u, v};
+    run_par(s, task_energy, (void *)uv);
+    int nt = (s->nt > 1 && g_pool) ? s->nt : 1;
+    out[0] = out[1] = out[2] = 0;
+    for (int t = 0; t < nt; t++)
+        for (int j = 0; j < 3; j++) out[j] += s->tsum[t * 3 + j];
+}
+
+/* ------------------------------------------------------------------ */
+/* Public whole-solve entry points                                     */
+/* ------------------------------------------------------------------ */
+
+static int setup(orc_state *s, const orc_problem *p)
+{
+    if (!p || (p->nen != 8 && p->nen != 4) || p->n_nodes <= 0 || p->n_elem <= 0) return ORC_ERR_ARG;
+    if (p->n_threads > 1 && p->sum_order != ORC_SUM_ELEMENT) return ORC_ERR_ARG;
+    int rc = state_init(s, p);
+    if (rc) return rc;
+    s->nt = p->n_threads > 1 ? p->n_threads : 1;
+    s->tmin = malloc(sizeof(double) * s->nt);
+    s->tsum = malloc(sizeof(double) * 3 * s->nt);
+    if (!s->tmin || !s->tsum) return ORC_ERR_MEM;
+    return ORC_OK;
+}
+
+static pthread_t *pool_start(orc_state *s, worker_arg **wa)
+{
+    if (s->nt <= 1) return NULL;
+    g_pool = calloc(1, sizeof(pool_t));
+    g_pool->s = s;
+    g_pool->nt = s->nt;
+    pthread_barrier_init(&g_pool->start, NULL, s->nt);
+    pthread_barrier_init(&g_pool->done, NULL, s->nt);
+    pthread_t *th = malloc(sizeof(pthread_t) * s->nt);
+    *wa = malloc(sizeof(worker_arg) * s->nt);
+    for (int t = 1; t < s->nt; t++) {
+        (*wa)[t].pool = g_pool;
+        (*wa)[t].tid = t;
+        pthread_create(&th[t], NULL, pool_worker, &(*wa)[t]);
+    }
+    return th;
+}
+
+static void pool_stop(orc_state *s, pthread_t *th, worker_arg *wa)
+{
+    if (!th) return;
+    g_pool->quit = 1;
+    pthread_barrier_wait(&g_pool->start);
+    for (int t = 1; t < s->nt; t++) pthread_join(th[t], NULL);
+    pthread_barrier_destroy(&g_pool->start);
+    pthread_barrier_destroy(&g_pool->done);
+    free(g_pool);
+    g_pool = NULL;
+    free(th);
+    free(wa);
+}
+
+int32_t orc_lumped_mass(const orc_problem *p, double *mass)
+{
+    orc_state s;
+    int rc = setup(&s, p);
+    if (rc == ORC_OK) {
+        compute_mass(&s);
+        memcpy(mass, s.mass, sizeof(double) * (size_t)s.N);
+    }
+    state_free(&s);
+    return rc;
+}
+
+int32_t orc_internal_force(const orc_problem *p, const double *u, double *fint)
+{
+    orc_state s;
+    int rc = setup(&s, p);
+    worker_arg *wa = NULL;
+    pthread_t *th = NULL;
+    if (rc == ORC_OK) {
+        th = pool_start(&s, &wa);
+        internal_force(&s, u);
+        memcpy(fint, s.fint, sizeof(double) * 3 * (size_t)s.N);
+        pool_stop(&s, th, wa);
+    }
+    state_free(&s);
+    return rc;
+}
+
+double orc_critical_dt(const orc_problem *p, const double *u)
+{
+    orc_state s;
+    double dt = NAN;
+    if (setup(&s, p) == ORC_OK) dt = p->scale * critical_dt(&s, u);
+    state_free(&s);
+    return dt;
+}
+
+/* classSolverExplicit::calculate: solverExplicit.cpp:63-361 (sequential branch). */
+int32_t orc_run(const orc_problem *p, double *u, double *v, double *a, orc_history *hist,
+                orc_report *rep)
+{
+    orc_state s;
+    int rc = setup(&s, p);
+    orc_report r = {0, 0, 0, ORC_OK, 0};
+    worker_arg *wa = NULL;
+    pthread_t *th = NULL;
+    double *uzero = NULL;
+    if (rc != ORC_OK) goto out;
+    th = pool_start(&s, &wa);
+    size_t N = s.N;
+
+    compute_mass(&s); /* computeLumpedMassVector, :155 */
+
+    /* Time specifications, :158-169 */
+    double timeRun = p->t0;
+    double totalTime = p->tf;
+    double dt = critical_dt(&s, NULL) * p->scale;
+    /* uK is fetched from the system only if timeRun < 1e-12 (:109, :176-178) */
+    uzero = calloc(3 * N, sizeof(double));
+    const double *uK = (timeRun < 1E-12) ? u : uzero;
+    /* a_prev is a at the last nextStep (:149 -> nonlinearSystem.cpp:710-724); it is
+     * the same array as a until the solve overwrites a, so one array suffices. */
+    int64_t nStep = 0;
+    int every = p->energy_every;
+    int64_t last_e = -1;
+    for (;;) {
+        if (p->n_steps > 0) {
+            if (nStep >= p->n_steps) break;
+            timeRun += dt;
+        } else {
+            if (!(timeRun < totalTime - 1E-12)) break;
+            if (timeRun + dt > totalTime) { /* :227-232 */
+                dt = totalTime - timeRun;
+                timeRun = totalTime;
+            } else {
+                timeRun += dt;
+            }
+        }
+        dt = critical_dt(&s, uK); /* :267 */
+        dt = dt * p->scale;
+        uK = u;
+        if (dt < 1E-20) { rc = ORC_ERR_DT; break; } /* :270-273 */
+        nStep++;
+        /* dynamicPrediction, nonlinearSystem.cpp:1186-1201 (active DOFs) */
+        for (size_t i = 0; i < 3 * N; i++)
+            if (s.active[i]) u[i] += dt * v[i] + 0.5 * dt * dt * a[i];
+        /* applyDisplacementBC, solverExplicit.cpp:545-575 (input order) */
+        for (int k = 0; k < p->n_dir; k++) {
+            int32_t ind = p->dir_dof[k];
+            double U = 0;
+            if (p->bc_mode == ORC_BC_INST) {
+                U = (nStep == 1) ? p->dir_val[k] : 0.0; /* flagDiri resets after first use */
+            } else if (p->bc_mode == ORC_BC_RAMP) {
+                U = (p->dir_val[k] / (p->tf - p->t0)) * dt; /* UU = U/duration (classNodes.cpp:70) */
+            } else { /* velocity amplitude (extension): V_n = V*min(n/ramp, 1) */
+                double ramp = p->vel_ramp_steps > 0 ? (double)p->vel_ramp_steps : 1.0;
+                double f = (double)nStep / ramp;
+                if (f > 1.0) f = 1.0;
+                double V = p->dir_val[k] * f;
+                U = V * dt;
+                u[ind] += U;
+                v[ind] = V;
+                a[ind] = 0.;
+                continue;
+            }
+            u[ind] += U;
+            v[ind] = U;
+            a[ind] = 0.;
+        }
+        /* computeState + computeRightHandSide: fint(u) */
+        internal_force(&s, u);
+        /* systemSolve, nonlinearSystem.cpp:1096-1107: a = (-fint + fext)/m, v update */
+        for (size_t i = 0; i < 3 * N; i++) {
+            if (!s.active[i]) continue;
+            double rhs = -s.fint[i] + 0.0;
+            double m = s.mass[i % N];
+            double anew = (m != 0.0) ? rhs / m : 0.0;
+            v[i] += dt * ((1. - 0.5) * a[i] + 0.5 * anew);
+            a[i] = anew;
+        }
+        if (!isfinite(u[1])) { rc = ORC_ERR_NAN; break; } /* :306-308 */
+        int64_t k = nStep - 1;
+        if (hist && k < hist->cap) {
+            if (hist->time) hist->time[k] = timeRun;
+            if (hist->dt) hist->dt[k] = dt;
+        }
+        if (every > 0 && (nStep % every) == 0) {
+            double en[3];
+            energies(&s, u, v, en);
+            last_e = nStep;
+            if (hist && k < hist->cap) {
+                if (hist->e_def) hist->e_def[k] = en[0];
+                if (hist->e_kin) hist->e_kin[k] = en[1];
+                if (hist->volume) hist->volume[k] = en[2];
+            }
+        }
+    }
+    if (every > 0 && nStep > 0 && last_e != nStep && rc == ORC_OK) {
+        double en[3];
+        energies(&s, u, v, en);
+        int64_t k = nStep - 1;
+        if (hist && k < hist->cap) {
+            if (hist->e_def) hist->e_def[k] = en[0];
+            if (hist->e_kin) hist->e_kin[k] = en[1];
+            if (hist->volume) hist->volume[k] = en[2];
+        }
+    }
+    r.steps = nStep;
+    r.time = timeRun;
+    r.dt_last = dt;
+    r.status = rc;
+out:
+    pool_stop(&s, th, wa);
+    free(uzero);
+    state_free(&s);
+    if (rep) *rep = r;
+    return rc;
+}

@@ -1,0 +1,111 @@
+/*
+ * mps_oracle.h -- CPU oracle for the explicit hex8/tet4 neo-Hookean path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker: it restates the
+ * reference algorithm (muphisim v1.1, classSolverExplicit and its callees)
+ * in plain C so that tests, the smoke check and the CPU baseline leg of the
+ * benchmark can compare the HIP path against it.  The product library never
+ * links, loads or calls it.
+ *
+ * Every function cites the reference file:line whose operation order it
+ * follows.  Pins (what checks each function beyond retyping it) live in
+ * tests/test_oracle_pins.py.
+ *
+ * Layout conventions (same as the reference):
+ *   - DOF index = node + n_nodes * dir            (classGPs.cpp:54, classNodes.cpp:62)
+ *   - X, u, v, a are SoA by direction: X[dir*n_nodes + node]
+ *   - conn is element-major: conn[e*nen + a], 0-based node ids
+ *   - 3x3 tensors are row-major [i*3+j]            (maths.h:331-338)
+ */
+#ifndef MPS_ORACLE_H
+#define MPS_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum { ORC_LAW_NH_COMPRESSIBLE = 0, ORC_LAW_NH_BELYTSCHKO = 1 };
+enum { ORC_BC_RAMP = 0, ORC_BC_INST = 1, ORC_BC_VELOCITY = 2 };
+enum { ORC_SUM_REFERENCE = 0, ORC_SUM_ELEMENT = 1 };
+enum { ORC_OK = 0, ORC_ERR_DT = -1, ORC_ERR_NAN = -2, ORC_ERR_ARG = -3, ORC_ERR_MEM = -4 };
+
+typedef struct orc_problem {
+    int32_t n_nodes;
+    int32_t n_elem;
+    int32_t nen;            /* 8 (C3D8) or 4 (C3D4) */
+    int32_t law;            /* ORC_LAW_* */
+    const double *X;        /* [3*n_nodes] SoA */
+    const int32_t *conn;    /* [nen*n_elem] element-major */
+    int32_t mat_per_elem;   /* 0: E/nu/rho have length 1, 1: length n_elem */
+    const double *E;
+    const double *nu;
+    const double *rho;
+    int32_t n_dir;          /* Dirichlet records in input order */
+    const int32_t *dir_dof; /* node + n_nodes*dir */
+    const double *dir_val;  /* ramp/inst: prescribed U; velocity: amplitude V */
+    int32_t bc_mode;        /* ORC_BC_* */
+    int32_t vel_ramp_steps; /* velocity mode: V_n = V*min(n/ramp,1) */
+    double t0, tf, scale;
+    int64_t n_steps;        /* 0: reference time-window loop; >0: fixed step count */
+    int32_t sum_order;      /* ORC_SUM_* */
+    int32_t n_threads;      /* >1 requires ORC_SUM_ELEMENT */
+    int32_t energy_every;   /* 0: no energies; k: every k-th step (and the last) */
+    int32_t _pad;
+} orc_problem;
+
+typedef struct orc_report {
+    int64_t steps;          /* steps taken */
+    double time;            /* final timeRun */
+    double dt_last;         /* dt of the last step */
+    int32_t status;         /* ORC_* */
+    int32_t _pad;
+} orc_report;
+
+/* History buffers (any may be NULL).  Entry k describes step k+1. */
+typedef struct orc_history {
+    int64_t cap;
+    double *time;
+    double *dt;
+    double *e_def;          /* DEFO_ENERGY, Sum       (dataExtraction.cpp:569-629) */
+    double *e_kin;          /* KIN_ENERGY, Sum        (dataExtraction.cpp:631-707) */
+    double *volume;         /* VOLUME, Sum            (dataExtraction.cpp:709-776) */
+} orc_history;
+
+/* ---- component functions (exported for the pin tests) ---- */
+double orc_det3(const double *A);
+void orc_inverse3(const double *A, double *Ainv);
+void orc_hex8_shape(const double *xw, double *phi, double *natdev);
+void orc_tet4_shape(const double *xw, double *phi, double *natdev);
+int32_t orc_quadrature(int32_t nen, double *xw);
+void orc_define_gp(int32_t nen, const double *xyz, const double *xw,
+                   double *phi, double *dphi, double *wJ, double *detJ);
+double orc_area_triangle(const double *tri);
+double orc_char_length(int32_t nen, const double *xyz);
+void orc_material(double E, double nu, double *lambda, double *mu, double *K1);
+double orc_sound_speed(double E, double nu, double rho);
+void orc_pk1(int32_t law, const double *F, double E, double nu, double *P);
+double orc_defo_energy(int32_t law, const double *F, double E, double nu);
+void orc_kinematics(int32_t nen, const double *dphi, const double *uloc, double *F);
+void orc_gp_force(int32_t nen, double wJ, const double *P, const double *dphi, double *f);
+void orc_gp_lumped_mass(int32_t nen, double wJ, double rho, const double *phi, double *m);
+
+/* ---- whole explicit solve (solverExplicit.cpp:63-361) ----
+ * u, v, a: [3*n_nodes] initial state in, final state out. */
+int32_t orc_run(const orc_problem *p, double *u, double *v, double *a,
+                orc_history *hist, orc_report *rep);
+
+/* Lumped mass per node (same value for x, y, z); Dirichlet-ness ignored. */
+int32_t orc_lumped_mass(const orc_problem *p, double *mass);
+
+/* Internal force fint[3*n_nodes] for a given displacement. */
+int32_t orc_internal_force(const orc_problem *p, const double *u, double *fint);
+
+/* Critical time step s*min_e min(l0,l(u))/c (solvers.cpp:296-320); u may be NULL. */
+double orc_critical_dt(const orc_problem *p, const double *u);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

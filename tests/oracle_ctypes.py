@@ -1,0 +1,160 @@
+"""ctypes bindings for the CPU oracle (oracle/lib/libmps_oracle.so).
+
+Test infrastructure only: the oracle is the checker for the HIP path.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(ROOT, "oracle", "lib", "libmps_oracle.so")
+
+LAW_NH_COMPRESSIBLE = 0
+LAW_NH_BELYTSCHKO = 1
+BC_RAMP, BC_INST, BC_VELOCITY = 0, 1, 2
+SUM_REFERENCE, SUM_ELEMENT = 0, 1
+ORC_OK, ORC_ERR_DT, ORC_ERR_NAN = 0, -1, -2
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int32)
+
+
+class Problem(C.Structure):
+    _fields_ = [
+        ("n_nodes", C.c_int32), ("n_elem", C.c_int32), ("nen", C.c_int32), ("law", C.c_int32),
+        ("X", _dp), ("conn", _ip), ("mat_per_elem", C.c_int32),
+        ("E", _dp), ("nu", _dp), ("rho", _dp),
+        ("n_dir", C.c_int32), ("dir_dof", _ip), ("dir_val", _dp),
+        ("bc_mode", C.c_int32), ("vel_ramp_steps", C.c_int32),
+        ("t0", C.c_double), ("tf", C.c_double), ("scale", C.c_double),
+        ("n_steps", C.c_int64), ("sum_order", C.c_int32), ("n_threads", C.c_int32),
+        ("energy_every", C.c_int32), ("_pad", C.c_int32),
+    ]
+
+
+class Report(C.Structure):
+    _fields_ = [("steps", C.c_int64), ("time", C.c_double), ("dt_last", C.c_double),
+                ("status", C.c_int32), ("_pad", C.c_int32)]
+
+
+class History(C.Structure):
+    _fields_ = [("cap", C.c_int64), ("time", _dp), ("dt", _dp), ("e_def", _dp),
+                ("e_kin", _dp), ("volume", _dp)]
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        L = C.CDLL(LIB_PATH)
+        L.orc_det3.restype = C.c_double
+        L.orc_det3.argtypes = [_dp]
+        L.orc_inverse3.argtypes = [_dp, _dp]
+        L.orc_quadrature.restype = C.c_int32
+        L.orc_quadrature.argtypes = [C.c_int32, _dp]
+        L.orc_define_gp.argtypes = [C.c_int32, _dp, _dp, _dp, _dp, _dp, _dp]
+        L.orc_area_triangle.restype = C.c_double
+        L.orc_area_triangle.argtypes = [_dp]
+        L.orc_char_length.restype = C.c_double
+        L.orc_char_length.argtypes = [C.c_int32, _dp]
+        L.orc_material.argtypes = [C.c_double, C.c_double, _dp, _dp, _dp]
+        L.orc_sound_speed.restype = C.c_double
+        L.orc_sound_speed.argtypes = [C.c_double] * 3
+        L.orc_pk1.argtypes = [C.c_int32, _dp, C.c_double, C.c_double, _dp]
+        L.orc_defo_energy.restype = C.c_double
+        L.orc_defo_energy.argtypes = [C.c_int32, _dp, C.c_double, C.c_double]
+        L.orc_kinematics.argtypes = [C.c_int32, _dp, _dp, _dp]
+        L.orc_gp_force.argtypes = [C.c_int32, C.c_double, _dp, _dp, _dp]
+        L.orc_gp_lumped_mass.argtypes = [C.c_int32, C.c_double, C.c_double, _dp, _dp]
+        L.orc_run.restype = C.c_int32
+        L.orc_run.argtypes = [C.POINTER(Problem), _dp, _dp, _dp, C.POINTER(History), C.POINTER(Report)]
+        L.orc_lumped_mass.restype = C.c_int32
+        L.orc_lumped_mass.argtypes = [C.POINTER(Problem), _dp]
+        L.orc_internal_force.restype = C.c_int32
+        L.orc_internal_force.argtypes = [C.POINTER(Problem), _dp, _dp]
+        L.orc_critical_dt.restype = C.c_double
+        L.orc_critical_dt.argtypes = [C.POINTER(Problem), _dp]
+        _lib = L
+    return _lib
+
+
+def dptr(a):
+    assert a.dtype == np.float64 and a.flags.c_contiguous
+    return a.ctypes.data_as(_dp)
+
+
+def iptr(a):
+    assert a.dtype == np.int32 and a.flags.c_contiguous
+    return a.ctypes.data_as(_ip)
+
+
+class OracleProblem:
+    """Holds numpy arrays alive for the lifetime of the ctypes struct."""
+
+    def __init__(self, mesh, law=LAW_NH_COMPRESSIBLE, E=1e6, nu=0.3, rho=1000.0,
+                 dir_dof=None, dir_val=None, bc_mode=BC_VELOCITY, vel_ramp_steps=100,
+                 t0=0.0, tf=1.0, scale=0.5, n_steps=0, sum_order=SUM_REFERENCE,
+                 n_threads=1, energy_every=0):
+        self.X = np.ascontiguousarray(mesh["X"], dtype=np.float64)
+        self.conn = np.ascontiguousarray(mesh["conn"], dtype=np.int32)
+        self.E = np.atleast_1d(np.asarray(E, dtype=np.float64)).copy()
+        self.nu = np.atleast_1d(np.asarray(nu, dtype=np.float64)).copy()
+        self.rho = np.atleast_1d(np.asarray(rho, dtype=np.float64)).copy()
+        per = int(self.E.size > 1)
+        if per:
+            assert self.E.size == mesh["n_elem"] and self.nu.size == mesh["n_elem"]
+            if self.rho.size == 1:
+                self.rho = np.full(mesh["n_elem"], self.rho[0])
+        dir_dof = np.zeros(0, np.int32) if dir_dof is None else dir_dof
+        dir_val = np.zeros(0, np.float64) if dir_val is None else dir_val
+        self.dir_dof = np.ascontiguousarray(dir_dof, dtype=np.int32)
+        self.dir_val = np.ascontiguousarray(dir_val, dtype=np.float64)
+        self.n_nodes = mesh["n_nodes"]
+        self.p = Problem(
+            n_nodes=mesh["n_nodes"], n_elem=mesh["n_elem"], nen=mesh["nen"], law=law,
+            X=dptr(self.X), conn=iptr(self.conn), mat_per_elem=per,
+            E=dptr(self.E), nu=dptr(self.nu), rho=dptr(self.rho),
+            n_dir=self.dir_dof.size,
+            dir_dof=iptr(self.dir_dof) if self.dir_dof.size else None,
+            dir_val=dptr(self.dir_val) if self.dir_val.size else None,
+            bc_mode=bc_mode, vel_ramp_steps=vel_ramp_steps, t0=t0, tf=tf, scale=scale,
+            n_steps=n_steps, sum_order=sum_order, n_threads=n_threads,
+            energy_every=energy_every)
+
+    def run(self, u0=None, v0=None, a0=None, hist_cap=0):
+        n3 = 3 * self.n_nodes
+        u = np.zeros(n3) if u0 is None else np.array(u0, dtype=np.float64)
+        v = np.zeros(n3) if v0 is None else np.array(v0, dtype=np.float64)
+        a = np.zeros(n3) if a0 is None else np.array(a0, dtype=np.float64)
+        hist = None
+        harr = {}
+        if hist_cap:
+            for k in ("time", "dt", "e_def", "e_kin", "volume"):
+                harr[k] = np.full(hist_cap, np.nan)
+            hist = History(cap=hist_cap, **{k: dptr(v_) for k, v_ in harr.items()})
+        rep = Report()
+        rc = lib().orc_run(C.byref(self.p), dptr(u), dptr(v), dptr(a),
+                           C.byref(hist) if hist is not None else None, C.byref(rep))
+        return dict(rc=rc, u=u, v=v, a=a, steps=rep.steps, time=rep.time,
+                    dt_last=rep.dt_last, hist=harr)
+
+    def mass(self):
+        m = np.zeros(self.n_nodes)
+        rc = lib().orc_lumped_mass(C.byref(self.p), dptr(m))
+        assert rc == 0
+        return m
+
+    def internal_force(self, u):
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        f = np.zeros(3 * self.n_nodes)
+        rc = lib().orc_internal_force(C.byref(self.p), dptr(u), dptr(f))
+        assert rc == 0
+        return f
+
+    def critical_dt(self, u=None):
+        if u is not None:
+            u = np.ascontiguousarray(u, dtype=np.float64)
+        return lib().orc_critical_dt(C.byref(self.p), dptr(u) if u is not None else None)

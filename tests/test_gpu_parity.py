@@ -1,0 +1,148 @@
+"""HIP path (through the C-ABI) against the CPU oracle on the same inputs.
+
+Tolerances (relative, infinity norm over all DOFs, normalised by max |u_oracle|):
+  FP64 form:            <= 1e-12  (stated bar for the FP64 GPU path)
+  FP32-element form:    <= 1e-6   (stated bar for the precision form, against the FP64 oracle)
+"""
+import numpy as np
+import pytest
+
+import mps_gpu as G
+import oracle_ctypes as O
+from meshgen import box, clamp_and_pull, perturb
+
+pytestmark = pytest.mark.gpu
+
+TOL64 = 1e-12
+TOL32 = 1e-6
+
+
+def relerr(a, b):
+    return np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
+
+
+def run_pair(mesh, steps, precision=G.PREC_FP64, law=0, E=1e6, nu=0.3, rho=1000.0, bc_mode=O.BC_VELOCITY,
+             dirs=None, t0=0.0, tf=1.0, scale=0.5, energy_every=0, state0=None, chunks=None):
+    dd, dv = dirs if dirs is not None else clamp_and_pull(mesh)
+    op = O.OracleProblem(mesh, law=law, E=E, nu=nu, rho=rho, dir_dof=dd, dir_val=dv, bc_mode=bc_mode,
+                         t0=t0, tf=tf, scale=scale, n_steps=steps, sum_order=O.SUM_REFERENCE,
+                         energy_every=energy_every)
+    u0 = v0 = a0 = None
+    if state0 is not None:
+        u0, v0, a0 = state0
+    ro = op.run(u0, v0, a0, hist_cap=max(steps, 1) if steps else 20000)
+    with G.Solver(mesh, law=law, E=E, nu=nu, rho=rho, dir_dof=dd, dir_val=dv, bc_mode=bc_mode, t0=t0, tf=tf,
+                  scale=scale, n_steps=steps, precision=precision, energy_every=energy_every,
+                  history_cap=20000) as s:
+        if state0 is not None:
+            s.set_state(u0, v0, a0)
+        total = steps if steps else 10 ** 9
+        if chunks:
+            rep = None
+            for c in chunks:
+                rc, rep = s.run(c)
+        else:
+            rc, rep = s.run(total)
+        u, v, a = s.get_state()
+        nh = int(rep["steps"])
+        hist = s.history(0, nh) if nh else None
+    return ro, (rc, rep, u, v, a, hist)
+
+
+def test_cfg0_fp64_1000_steps_energies():
+    m = box(10, 10, 10)
+    ro, (rc, rep, u, v, a, hist) = run_pair(m, 1000, energy_every=1)
+    assert ro["rc"] == 0 and rc == 0
+    assert rep["steps"] == 1000 and ro["steps"] == 1000
+    assert rep["status"] == G.MPS_DONE
+    assert relerr(u, ro["u"]) <= TOL64
+    assert relerr(v, ro["v"]) <= 1e-10
+    np.testing.assert_allclose(hist["dt"], ro["hist"]["dt"], rtol=1e-13, atol=0)
+    np.testing.assert_allclose(hist["time"], ro["hist"]["time"], rtol=1e-13, atol=0)
+    for k in ("e_def", "e_kin", "volume"):
+        np.testing.assert_allclose(hist[k], ro["hist"][k], rtol=1e-11, atol=0, err_msg=k)
+
+
+def test_cfg0_first_dt_bitwise():
+    m = box(10, 10, 10)
+    dd, dv = clamp_and_pull(m)
+    op = O.OracleProblem(m, dir_dof=dd, dir_val=dv, n_steps=1)
+    with G.Solver(m, dir_dof=dd, dir_val=dv, n_steps=1) as s:
+        assert s.critical_dt() == op.critical_dt()
+        rng = np.random.default_rng(5)
+        u = rng.uniform(-1e-3, 1e-3, 3 * m["n_nodes"])
+        assert s.critical_dt(u) == op.critical_dt(u)
+
+
+def test_mass_and_internal_force():
+    m = perturb(box(6, 5, 4), 0.1, 3)
+    dd, dv = clamp_and_pull(m)
+    op = O.OracleProblem(m, dir_dof=dd, dir_val=dv, n_steps=1)
+    ope = O.OracleProblem(m, dir_dof=dd, dir_val=dv, n_steps=1, sum_order=O.SUM_ELEMENT)
+    with G.Solver(m, dir_dof=dd, dir_val=dv, n_steps=1) as s:
+        mg = s.mass()
+        me = ope.mass()
+        assert np.abs(mg - me).max() == 0.0, np.abs(mg - me).max()  # same association: bitwise
+        assert relerr(mg, op.mass()) <= 1e-15
+        rng = np.random.default_rng(7)
+        u = rng.uniform(-1e-3, 1e-3, 3 * m["n_nodes"])
+        fo = op.internal_force(u)
+        fg = s.internal_force(u)
+        assert relerr(fg, fo) <= 1e-12
+
+
+@pytest.mark.parametrize("law", [0, 1])
+def test_perturbed_hex_fp64_and_fp32(law):
+    m = perturb(box(12, 12, 12), 0.1, 11)
+    ro, (rc, rep, u, v, a, hist) = run_pair(m, 200, law=law)
+    assert rc == 0 and relerr(u, ro["u"]) <= TOL64, relerr(u, ro["u"])
+    ro2, (rc2, rep2, u2, *_r) = run_pair(m, 200, law=law, precision=G.PREC_FP32_ELEM)
+    err = relerr(u2, ro2["u"])
+    print(f"law {law} fp32-element rel err {err:.3e}")
+    assert rc2 == 0 and err <= TOL32
+
+
+def test_tet4_heterogeneous_fp64_fp32():
+    m = perturb(box(6, 6, 6, tet=True), 0.15, 2)
+    rng = np.random.default_rng(3)
+    E = 10 ** rng.uniform(5, 7, m["n_elem"])
+    nu = rng.uniform(0.2, 0.45, m["n_elem"])
+    ro, (rc, rep, u, *_r) = run_pair(m, 150, E=E, nu=nu)
+    assert rc == 0 and relerr(u, ro["u"]) <= TOL64, relerr(u, ro["u"])
+    ro2, (rc2, rep2, u2, *_r) = run_pair(m, 150, E=E, nu=nu, precision=G.PREC_FP32_ELEM)
+    err = relerr(u2, ro2["u"])
+    print(f"tet fp32-element rel err {err:.3e}")
+    assert rc2 == 0 and err <= TOL32
+
+
+@pytest.mark.parametrize("mode", [O.BC_RAMP, O.BC_INST])
+def test_reference_time_window_mode(mode):
+    # reference-expressible run: *Time 0, tf with DISPLACEMENT RAMP / INST on the top face
+    m = box(5, 5, 5)
+    dd, dv = clamp_and_pull(m, vel=0.002)
+    ro, (rc, rep, u, v, a, hist) = run_pair(m, 0, bc_mode=mode, dirs=(dd, dv), t0=0.0, tf=0.05)
+    assert ro["rc"] == 0 and rc == 0
+    assert rep["status"] == G.MPS_DONE
+    assert rep["steps"] == ro["steps"]
+    assert rep["time"] == ro["time"]
+    assert relerr(u, ro["u"]) <= TOL64
+
+
+def test_chained_solver_handoff_and_chunks():
+    m = box(4, 4, 6)
+    ro1, (rc1, rep1, u1, v1, a1, _h) = run_pair(m, 120, chunks=[7, 50, 63])
+    assert relerr(u1, ro1["u"]) <= TOL64
+    # second solver starting at t0 > 0 from the handed-over state (main.cpp:161-182)
+    ro2, (rc2, rep2, u2, *_r) = run_pair(m, 80, t0=0.3, state0=(ro1["u"], ro1["v"], ro1["a"]))
+    assert rc2 == 0 and relerr(u2, ro2["u"]) <= TOL64
+
+
+def test_nan_error_matches_oracle():
+    # x = max face driven into the body at -4000 m/s: elements invert, log(J<=0) gives NaN,
+    # which reaches DOF 1 (node 1 is free with x-faces constrained): solverExplicit.cpp:306-308
+    m = box(3, 3, 3)
+    dd, dv = clamp_and_pull(m, vel=-4000.0, axis=0)
+    ro, (rc, rep, *_r) = run_pair(m, 400, dirs=(dd, dv), scale=0.9)
+    assert ro["rc"] == O.ORC_ERR_NAN
+    assert rc == G.MPS_ERR_NAN
+    assert rep["steps"] == ro["steps"]

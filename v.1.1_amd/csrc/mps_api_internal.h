@@ -1,0 +1,31 @@
+// mps_api_internal.h -- host-side preparation shared by the single and batched forms.
+#pragma once
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/mps_gpu.h"
+
+namespace mpsh {
+
+struct HostPrep {
+    int32_t N = 0, Ne = 0, nen = 0, law = 0;
+    std::vector<double> X;
+    std::vector<int32_t> conn, rowptr, col;
+    bool per_elem = false;
+    std::vector<double> mat, cel, rho;  // per element
+    double pa = 0, pb = 0, c0 = 0, rho0 = 0;  // uniform
+    std::vector<uint32_t> ninfo;
+    std::vector<double> bcval;
+    int32_t bc_mode = 0, vel_ramp = 0;
+    mps_solver_cfg cfg{};
+};
+
+std::string fmt(const char* f, ...);
+void material_consts(int law, double E, double nu, double rho, double* pa, double* pb, double* c);
+// Validates the inputs and builds CSR, Dirichlet records and material constants.
+int prepare(const mps_mesh* mesh, const mps_material* mat, const mps_bcs* bcs, const mps_solver_cfg* cfg,
+            HostPrep* hp, std::string* err);
+
+}  // namespace mpsh

@@ -1,0 +1,82 @@
+// mps_common.h -- structures shared by the host side of libmps_gpu and its HIP kernels.
+//
+// Device data layout (one solve = one "instance"):
+//   X, u, v, a      double[3N] SoA by direction (DOF = node + N*dir, as the reference)
+//   mass            double[N]    lumped mass, identical for the three directions
+//   ninfo           uint32[N]    bits 0..2 Dirichlet mask (x,y,z), bits 3.. record index
+//   bcval           double[3*R]  per Dirichlet record: per-direction BC value
+//   conn            int32[nen*Ne] element-major (reference node order)
+//   l0              double[Ne]   undeformed characteristic length
+//   mat             double[2*Ne] per-element (lambda,mu) or (mu,K1); NULL = uniform
+//   cel             double[Ne]   per-element sound speed; NULL = uniform c0
+//   rowptr/col      CSR node -> (e*nen + a), ascending element then slot
+//   fe              T[Ne*nen*3]  element nodal forces, [(e*nen+a)*3 + dir]
+//   ctl             MpsCtl       step control (time, dt, go flags, dt reduction slots)
+#pragma once
+#include <stdint.h>
+
+#define MPS_MAX_NEN 8
+
+enum : int32_t { MPS_TIME_WINDOW = 0, MPS_TIME_FIXED = 1 };
+
+// Per-step control record, double-buffered by step parity.
+struct MpsStepSlot {
+    double time;  // timeRun of the step (lagged update, solverExplicit.cpp:227-232)
+    double dt;    // dt of the step (solverExplicit.cpp:267-268)
+    int32_t go;   // 1: step was begun (prediction + BC applied), 0: not run
+    int32_t _pad;
+};
+
+// Step protocol (all in device memory, no host round trip per step):
+//   A = next step to run, written only by the node kernel, read by begin/element kernels;
+//   B = step whose forces are in fe, written only by the element kernel, read by node/energy.
+// slot[k&1] holds step k; dtmin[k&1] receives the min over elements for step k
+// (computed from u_{k-1}); the node kernel of step k resets dtmin[k&1].
+struct MpsCtl {
+    MpsStepSlot slot[2];
+    unsigned long long dtmin[2];  // bit patterns of non-negative doubles (integer min)
+    long long A;
+    long long B;
+    int32_t status;     // MPS_RUNNING / MPS_DONE / MPS_ERR_DT
+    int32_t nonfinite;  // set when u[1] is non-finite
+    long long nan_step; // first step with non-finite u[1] (LLONG_MAX if none)
+    unsigned long long dt0min;  // min_e l0/c (setup)
+};
+
+struct MpsInst {
+    int32_t N, Ne, nen, law;
+    int32_t precision, bc_mode, vel_ramp, time_mode;
+    int64_t n_steps;
+    double t0, tf, scale;
+    double mat_a, mat_b, c0, rho0;  // uniform material: (lambda,mu) or (mu,K1), sound speed, density
+    const double* X;
+    const int32_t* conn;
+    const double* l0;
+    const double* mat;
+    const double* cel;
+    const double* rho;
+    double* u;
+    double* v;
+    double* a;
+    double* mass;
+    const uint32_t* ninfo;
+    const double* bcval;
+    const int32_t* rowptr;
+    const int32_t* col;
+    void* fe;
+    MpsCtl* ctl;
+    double* hist;      // [5*hist_cap]: time, dt, e_def, e_kin, volume
+    double* partials;  // energy partials [3*n_blocks]
+    int32_t hist_cap;
+    int32_t dof1_node, dof1_dir;
+    int32_t _pad;
+};
+
+// One unit of work for the persistent batched kernels: a contiguous range of
+// elements (or nodes) of one instance.
+struct MpsWork {
+    int32_t inst;
+    int32_t begin;
+    int32_t end;
+    int32_t first;  // 1 if this item holds index 0 of the instance (scalar updates)
+};

@@ -1,0 +1,676 @@
+// mps_device.h -- device functions of the explicit hex8/tet4 neo-Hookean step.
+//
+// Element work per hex8 (2x2x2 Gauss) uses the trilinear ("butterfly") structure of
+// the isoparametric map: the 8 nodal vectors are transformed once per element into
+// 7 derivative coefficient vectors (xi, eta, zeta, xi*eta, eta*zeta, xi*zeta,
+// xi*eta*zeta), from which the Jacobian and displacement gradient at each Gauss point
+// follow with adds only; nodal forces are accumulated as 7 moment vectors and
+// transformed back once.  In exact arithmetic this equals the reference sequence
+// dphi = natDev*J^-1 (classLinearHexahedra.cpp:301-332), F = I + sum_a u_a (x) dphi_a
+// (constitutiveManager.cpp:150-164) and f_a = wJ*P*dphi_a (constitutiveAlgorithms.cpp:46-56);
+// only the rounding differs.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdint.h>
+
+#include "mps_common.h"
+
+#define MPS_GP 0.577350269  // hex Gauss abscissa literal, GPsDistribution.cpp:392-432
+#define MPS_TET_W 0.166666666667  // tet weight literal, GPsDistribution.cpp:342
+
+#ifndef MPS_STATUS_CODES
+#define MPS_STATUS_CODES
+#define MPSD_RUNNING 0
+#define MPSD_DONE 1
+#define MPSD_ERR_DT (-3)
+#endif
+
+namespace mpsd {
+
+__device__ __forceinline__ double bits2d(unsigned long long b) { return __longlong_as_double((long long)b); }
+__device__ __forceinline__ unsigned long long d2bits(double d) { return (unsigned long long)__double_as_longlong(d); }
+
+// Node sign table of the reference hex (classLinearHexahedra.cpp:346-353).
+__device__ __forceinline__ constexpr int sxi(int a) { return (a == 1 || a == 2 || a == 5 || a == 6) ? 1 : -1; }
+__device__ __forceinline__ constexpr int seta(int a) { return (a == 2 || a == 3 || a == 6 || a == 7) ? 1 : -1; }
+__device__ __forceinline__ constexpr int szeta(int a) { return (a >= 4) ? 1 : -1; }
+
+// ---------------------------------------------------------------------------
+// 3x3 helpers (row-major)
+// ---------------------------------------------------------------------------
+template <typename T>
+__device__ __forceinline__ T det3(const T* A) {
+    return A[0] * (A[4] * A[8] - A[7] * A[5]) - A[1] * (A[3] * A[8] - A[6] * A[5]) +
+           A[2] * (A[3] * A[7] - A[6] * A[4]);
+}
+
+template <typename T>
+__device__ __forceinline__ void inv3(const T* A, T* R, T det) {
+    T idet = T(1) / det;
+    R[0] = idet * (A[4] * A[8] - A[5] * A[7]);
+    R[3] = -idet * (A[3] * A[8] - A[5] * A[6]);
+    R[6] = idet * (A[3] * A[7] - A[4] * A[6]);
+    R[1] = -idet * (A[1] * A[8] - A[2] * A[7]);
+    R[4] = idet * (A[0] * A[8] - A[2] * A[6]);
+    R[7] = -idet * (A[0] * A[7] - A[1] * A[6]);
+    R[2] = idet * (A[1] * A[5] - A[2] * A[4]);
+    R[5] = -idet * (A[0] * A[5] - A[2] * A[3]);
+    R[8] = idet * (A[0] * A[4] - A[1] * A[3]);
+}
+
+// determinantTensor with no FMA contraction (maths.cpp:1564-1577), for the set-up path.
+__device__ __forceinline__ double det3_strict(const double* A) {
+#pragma clang fp contract(off)
+    return A[0] * (A[4] * A[8] - A[7] * A[5]) - A[1] * (A[3] * A[8] - A[6] * A[5]) +
+           A[2] * (A[3] * A[7] - A[6] * A[4]);
+}
+
+// det(I+H) - 1 without cancellation: tr H + i2(H) + det H.
+template <typename T>
+__device__ __forceinline__ T detIpH_m1(const T* H) {
+    T tr = H[0] + H[4] + H[8];
+    T i2 = H[0] * H[4] + H[4] * H[8] + H[0] * H[8] - H[1] * H[3] - H[5] * H[7] - H[2] * H[6];
+    return tr + i2 + det3(H);
+}
+
+// ---------------------------------------------------------------------------
+// Constitutive laws
+// ---------------------------------------------------------------------------
+// FP64: the reference formulation on F = I + H.
+//  3D compressible NH: hyperElasticNeoHookean.cpp:235-257 (S = mu I + (lambda lnJ - mu) C^-1, P = F S)
+//  Belytschko NH:      hyperElasticNeoHookean.cpp:88-126  (sigma = mu/J^{5/3} B + (-(mu/3) trB/J^{5/3}
+//                      + K (J-1)) I, P = J sigma F^-T)
+__device__ __forceinline__ void pk1(int law, const double* H, double pa, double pb, double* P) {
+    double F[9];
+#pragma unroll
+    for (int i = 0; i < 9; i++) F[i] = H[i];
+    F[0] += 1.0;
+    F[4] += 1.0;
+    F[8] += 1.0;
+    if (law == 0) {  // pa = lambda, pb = mu
+        double J = det3(F);
+        double C[9];
+#pragma unroll
+        for (int i = 0; i < 3; i++)
+#pragma unroll
+            for (int j = 0; j < 3; j++) C[i * 3 + j] = F[i] * F[j] + F[3 + i] * F[3 + j] + F[6 + i] * F[6 + j];
+        double iC[9];
+        inv3(C, iC, det3(C));
+        double alpha = pa * log(J) - pb;
+        double S[9];
+#pragma unroll
+        for (int i = 0; i < 9; i++) S[i] = alpha * iC[i];
+        S[0] += pb;
+        S[4] += pb;
+        S[8] += pb;
+#pragma unroll
+        for (int i = 0; i < 3; i++)
+#pragma unroll
+            for (int j = 0; j < 3; j++)
+                P[i * 3 + j] = F[i * 3] * S[j] + F[i * 3 + 1] * S[3 + j] + F[i * 3 + 2] * S[6 + j];
+    } else {  // pa = mu, pb = K1
+        double B[9];
+#pragma unroll
+        for (int i = 0; i < 3; i++)
+#pragma unroll
+            for (int j = 0; j < 3; j++)
+                B[i * 3 + j] = F[i * 3] * F[j * 3] + F[i * 3 + 1] * F[j * 3 + 1] + F[i * 3 + 2] * F[j * 3 + 2];
+        double J = det3(F);
+        double iF[9];
+        inv3(F, iF, J);
+        double trB = B[0] + B[4] + B[8];
+        double f1 = pow(J, 5.0 / 3.0);
+        double f2 = pa / f1;
+        double f3 = -(pa / 3.0) * trB / f1 + pb * (J - 1.0);
+        double sg[9];
+#pragma unroll
+        for (int i = 0; i < 9; i++) sg[i] = f2 * B[i];
+        sg[0] += f3;
+        sg[4] += f3;
+        sg[8] += f3;
+#pragma unroll
+        for (int i = 0; i < 3; i++)
+#pragma unroll
+            for (int j = 0; j < 3; j++)
+                P[i * 3 + j] = (sg[i * 3] * iF[j * 3] + sg[i * 3 + 1] * iF[j * 3 + 1] + sg[i * 3 + 2] * iF[j * 3 + 2]) * J;
+    }
+}
+
+// FP32: the same laws written on H = F - I so that small strains carry no
+// cancellation (C - I = H + H^T + H^T H; J - 1 = tr H + i2 + det H).
+__device__ __forceinline__ void pk1(int law, const float* H, float pa, float pb, float* P) {
+    float Jm1 = detIpH_m1(H);
+    float lnJ = log1pf(Jm1);
+    if (law == 0) {  // S = C^-1 (mu (C - I) + lambda lnJ I), P = (I + H) S
+        float E2[9];
+#pragma unroll
+        for (int i = 0; i < 3; i++)
+#pragma unroll
+            for (int j = 0; j < 3; j++)
+                E2[i * 3 + j] = H[i * 3 + j] + H[j * 3 + i] + H[i] * H[j] + H[3 + i] * H[3 + j] + H[6 + i] * H[6 + j];
+        float C[9];
+#pragma unroll
+        for (int i = 0; i < 9; i++) C[i] = E2[i];
+        C[0] += 1.f;
+        C[4] += 1.f;
+        C[8] += 1.f;
+        float iC[9];
+        inv3(C, iC, det3(C));
+        float M[9];
+#pragma unroll
+        for (int i = 0; i < 9; i++) M[i] = pb * E2[i];
+        float lt = pa * lnJ;
+        M[0] += lt;
+        M[4] += lt;
+        M[8] += lt;
+        float S[9];
+#pragma unroll
+        for (int i = 0; i < 3; i++)
+#pragma unroll
+            for (int j = 0; j < 3; j++)
+                S[i * 3 + j] = iC[i * 3] * M[j] + iC[i * 3 + 1] * M[3 + j] + iC[i * 3 + 2] * M[6 + j];
+#pragma unroll
+        for (int i = 0; i < 3; i++)
+#pragma unroll
+            for (int j = 0; j < 3; j++)
+                P[i * 3 + j] = S[i * 3 + j] + H[i * 3] * S[j] + H[i * 3 + 1] * S[3 + j] + H[i * 3 + 2] * S[6 + j];
+    } else {  // sigma = mu J^{-5/3} dev(B - I) + K (J-1) I, P = J sigma F^-T
+        float Bm[9];
+#pragma unroll
+        for (int i = 0; i < 3; i++)
+#pragma unroll
+            for (int j = 0; j < 3; j++)
+                Bm[i * 3 + j] = H[i * 3 + j] + H[j * 3 + i] + H[i * 3] * H[j * 3] + H[i * 3 + 1] * H[j * 3 + 1] +
+                                H[i * 3 + 2] * H[j * 3 + 2];
+        float tr3 = (Bm[0] + Bm[4] + Bm[8]) * (1.f / 3.f);
+        float f2 = pa * expf(-(5.f / 3.f) * lnJ);
+        float vol = pb * Jm1;
+        float sg[9];
+#pragma unroll
+        for (int i = 0; i < 9; i++) sg[i] = f2 * Bm[i];
+        sg[0] += vol - f2 * tr3;
+        sg[4] += vol - f2 * tr3;
+        sg[8] += vol - f2 * tr3;
+        float F[9];
+#pragma unroll
+        for (int i = 0; i < 9; i++) F[i] = H[i];
+        F[0] += 1.f;
+        F[4] += 1.f;
+        F[8] += 1.f;
+        float J = 1.f + Jm1;
+        float iF[9];
+        inv3(F, iF, J);
+#pragma unroll
+        for (int i = 0; i < 3; i++)
+#pragma unroll
+            for (int j = 0; j < 3; j++)
+                P[i * 3 + j] = (sg[i * 3] * iF[j * 3] + sg[i * 3 + 1] * iF[j * 3 + 1] + sg[i * 3 + 2] * iF[j * 3 + 2]) * J;
+    }
+}
+
+// Deformation energy W(F): hyperElasticNeoHookean.cpp:291-304 (no -3mu/2 constant);
+// the Belytschko law has no defoEnergy override and contributes 0 (constitutiveModels.h:91-94).
+__device__ __forceinline__ double defo_energy(int law, const double* F, double lambda, double mu) {
+    if (law != 0) return 0.0;
+    double J = det3(F);
+    double trC = 0;
+#pragma unroll
+    for (int i = 0; i < 3; i++) trC += F[i] * F[i] + F[3 + i] * F[3 + i] + F[6 + i] * F[6 + i];
+    double l = log(J);
+    return 0.5 * lambda * l * l - mu * l + 0.5 * mu * trC;
+}
+
+// ---------------------------------------------------------------------------
+// Characteristic length (always FP64, no FMA contraction: same rounding as the
+// reference geometry.cpp, so dt is bitwise reproducible for identical u).
+// ---------------------------------------------------------------------------
+// areaTriangle 3D: geometry.cpp:112-130 ; p, q, r are vertices 1, 2, 3.
+__device__ __forceinline__ double area_tri(const double* p, const double* q, const double* r) {
+#pragma clang fp contract(off)
+    double A0 = p[0] - r[0], A1 = p[1] - r[1], A2 = p[2] - r[2];
+    double B0 = q[0] - r[0], B1 = q[1] - r[1], B2 = q[2] - r[2];
+    double C0 = A1 * B2 - A2 * B1;
+    double C1 = A2 * B0 - A0 * B2;
+    double C2 = A0 * B1 - A1 * B0;
+    return 0.5 * sqrt(C0 * C0 + C1 * C1 + C2 * C2);
+}
+
+// inscribedSphereHex (geometry.cpp:308-355) x2: classLinearHexahedra.cpp:209-234.
+__device__ __forceinline__ double hex_char_length(const double (*x)[3]) {
+#pragma clang fp contract(off)
+    const int fc[6][4] = {{0, 1, 2, 3}, {4, 5, 6, 7}, {0, 1, 5, 4}, {1, 2, 6, 5}, {2, 3, 7, 6}, {0, 3, 7, 4}};
+    double A = 0.0;
+#pragma unroll
+    for (int f = 0; f < 6; f++) {
+        double s = area_tri(x[fc[f][0]], x[fc[f][1]], x[fc[f][2]]) + area_tri(x[fc[f][0]], x[fc[f][2]], x[fc[f][3]]);
+        A = (f == 0) ? fabs(s) : A + fabs(s);
+    }
+    double M[9] = {x[1][0] - x[0][0], x[1][1] - x[0][1], x[1][2] - x[0][2],
+                   x[3][0] - x[0][0], x[3][1] - x[0][1], x[3][2] - x[0][2],
+                   x[4][0] - x[0][0], x[4][1] - x[0][1], x[4][2] - x[0][2]};
+    double V = fabs(M[0] * (M[4] * M[8] - M[7] * M[5]) - M[1] * (M[3] * M[8] - M[6] * M[5]) +
+                    M[2] * (M[3] * M[7] - M[6] * M[4]));
+    double r = 3.0 * V / A;
+    return 2 * r;
+}
+
+// inscribedSphere (geometry.cpp:264-301) x2: classLinearTetrahedra.cpp:146-163.
+__device__ __forceinline__ double tet_char_length(const double (*x)[3]) {
+#pragma clang fp contract(off)
+    double s1 = area_tri(x[0], x[1], x[2]);
+    double s2 = area_tri(x[1], x[2], x[3]);
+    double s3 = area_tri(x[0], x[2], x[3]);
+    double s4 = area_tri(x[0], x[1], x[3]);
+    double A = fabs(s1) + fabs(s2) + fabs(s3) + fabs(s4);
+    double M[9] = {x[1][0] - x[0][0], x[1][1] - x[0][1], x[1][2] - x[0][2],
+                   x[2][0] - x[0][0], x[2][1] - x[0][1], x[2][2] - x[0][2],
+                   x[3][0] - x[0][0], x[3][1] - x[0][1], x[3][2] - x[0][2]};
+    double V = 1.0 / 6.0 * fabs(M[0] * (M[4] * M[8] - M[7] * M[5]) - M[1] * (M[3] * M[8] - M[6] * M[5]) +
+                               M[2] * (M[3] * M[7] - M[6] * M[4]));
+    double r = 3.0 * V / A;
+    return 2 * r;
+}
+
+// Element contribution to the stable dt: std::min(l0, l)/c (solvers.cpp:317).
+__device__ __forceinline__ double elem_dt(double l, double l0, double c) {
+#pragma clang fp contract(off)
+    double m = (l < l0) ? l : l0;
+    return m / c;
+}
+
+// ---------------------------------------------------------------------------
+// Hex8 butterfly transform: 8 nodal vectors -> 7 derivative coefficient vectors
+// A[0]=xi, A[1]=eta, A[2]=zeta, A[3]=xi*eta, A[4]=eta*zeta, A[5]=xi*zeta, A[6]=xi*eta*zeta,
+// A[m] = sum_a sigma_m(a) x_a with sigma the product of the node signs.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void hex_butterfly(const double (*x)[3], double (*A)[3]) {
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        double s0 = x[0][d] + x[1][d], d0 = x[1][d] - x[0][d];
+        double s1 = x[3][d] + x[2][d], d1 = x[2][d] - x[3][d];
+        double s2 = x[4][d] + x[5][d], d2 = x[5][d] - x[4][d];
+        double s3 = x[7][d] + x[6][d], d3 = x[6][d] - x[7][d];
+        double ssm = s0 + s1, sdm = s1 - s0, dsm = d0 + d1, ddm = d1 - d0;
+        double ssp = s2 + s3, sdp = s3 - s2, dsp = d2 + d3, ddp = d3 - d2;
+        A[2][d] = ssp - ssm;
+        A[1][d] = sdm + sdp;
+        A[4][d] = sdp - sdm;
+        A[0][d] = dsm + dsp;
+        A[5][d] = dsp - dsm;
+        A[3][d] = ddm + ddp;
+        A[6][d] = ddp - ddm;
+    }
+}
+
+// Inverse transform: f_a = sum_m sigma_m(a) G[m]  (G[m] already scaled).
+template <typename T>
+__device__ __forceinline__ void hex_unbutterfly(const T (*G)[3], T (*f)[3]) {
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        T tssm = -G[2][d], tsdm = G[1][d] - G[4][d], tdsm = G[0][d] - G[5][d], tddm = G[3][d] - G[6][d];
+        T tssp = G[2][d], tsdp = G[1][d] + G[4][d], tdsp = G[0][d] + G[5][d], tddp = G[3][d] + G[6][d];
+        // zeta-, eta- (nodes 0,1)
+        T us = tssm - tsdm, ud = tdsm - tddm;
+        f[0][d] = us - ud;
+        f[1][d] = us + ud;
+        // zeta-, eta+ (nodes 3,2)
+        us = tssm + tsdm;
+        ud = tdsm + tddm;
+        f[3][d] = us - ud;
+        f[2][d] = us + ud;
+        // zeta+, eta- (nodes 4,5)
+        us = tssp - tsdp;
+        ud = tdsp - tddp;
+        f[4][d] = us - ud;
+        f[5][d] = us + ud;
+        // zeta+, eta+ (nodes 7,6)
+        us = tssp + tsdp;
+        ud = tdsp + tddp;
+        f[7][d] = us - ud;
+        f[6][d] = us + ud;
+    }
+}
+
+// Scale the coefficient vectors into Jacobian units: linear terms by 1/8, bilinear by
+// g/8, trilinear by g^2/8 (g = Gauss abscissa), converting to T.
+template <typename T>
+__device__ __forceinline__ void hex_scale(const double (*A)[3], T (*As)[3]) {
+    const double g = MPS_GP;
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        As[0][d] = (T)(0.125 * A[0][d]);
+        As[1][d] = (T)(0.125 * A[1][d]);
+        As[2][d] = (T)(0.125 * A[2][d]);
+        As[3][d] = (T)((0.125 * g) * A[3][d]);
+        As[4][d] = (T)((0.125 * g) * A[4][d]);
+        As[5][d] = (T)((0.125 * g) * A[5][d]);
+        As[6][d] = (T)((0.125 * g * g) * A[6][d]);
+    }
+}
+
+// Gradient matrix D(i,j) = d(.)_i / d xi_j at Gauss point with signs (sx,sy,sz).
+template <typename T>
+__device__ __forceinline__ void hex_grad_at(const T (*As)[3], int sx, int sy, int sz, T* D) {
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        D[i * 3 + 0] = As[0][i] + T(sy) * As[3][i] + T(sz) * As[5][i] + T(sy * sz) * As[6][i];
+        D[i * 3 + 1] = As[1][i] + T(sx) * As[3][i] + T(sz) * As[4][i] + T(sx * sz) * As[6][i];
+        D[i * 3 + 2] = As[2][i] + T(sx) * As[5][i] + T(sy) * As[4][i] + T(sx * sy) * As[6][i];
+    }
+}
+
+// Material parameters of element e: (pa, pb) = (lambda, mu) [compressible] or (mu, K1) [Belytschko].
+__device__ __forceinline__ void elem_mat(const MpsInst& I, int e, double& pa, double& pb, double& c) {
+    if (I.mat) {
+        pa = I.mat[2 * (size_t)e];
+        pb = I.mat[2 * (size_t)e + 1];
+        c = I.cel[e];
+    } else {
+        pa = I.mat_a;
+        pb = I.mat_b;
+        c = I.c0;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Hex8 element: nodal forces f[a][i] (T) and the element dt contribution (FP64).
+// ---------------------------------------------------------------------------
+template <typename T, int LAW>
+__device__ __forceinline__ void hex8_element(const MpsInst& I, int e, const double* __restrict__ u, T (*f)[3],
+                                             double* dte) {
+    const int32_t* cn = I.conn + (size_t)e * 8;
+    int4 q0 = *reinterpret_cast<const int4*>(cn);
+    int4 q1 = *reinterpret_cast<const int4*>(cn + 4);
+    const int nd[8] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, q1.z, q1.w};
+    const size_t N = (size_t)I.N;
+    double pa, pb, c;
+    elem_mat(I, e, pa, pb, c);
+    T Ax[7][3], Au[7][3], G[7][3];
+    {
+        double x[8][3], cx[7][3];
+#pragma unroll
+        for (int a = 0; a < 8; a++)
+#pragma unroll
+            for (int d = 0; d < 3; d++) x[a][d] = I.X[d * N + nd[a]];
+        hex_butterfly(x, cx);
+        hex_scale<T>(cx, Ax);
+    }
+    {
+        double x[8][3], cx[7][3];
+#pragma unroll
+        for (int a = 0; a < 8; a++)
+#pragma unroll
+            for (int d = 0; d < 3; d++) x[a][d] = u[d * N + nd[a]];
+        hex_butterfly(x, cx);
+        hex_scale<T>(cx, Au);
+    }
+#pragma unroll
+    for (int m = 0; m < 7; m++)
+#pragma unroll
+        for (int d = 0; d < 3; d++) G[m][d] = T(0);
+    const T tpa = (T)pa, tpb = (T)pb;
+#pragma unroll 1
+    for (int gp = 0; gp < 8; gp++) {
+        // Gauss point signs in the reference order (GPsDistribution.cpp:392-432)
+        const T sx = (gp == 1 || gp == 2 || gp == 5 || gp == 6) ? T(1) : T(-1);
+        const T sy = (gp == 2 || gp == 3 || gp == 6 || gp == 7) ? T(1) : T(-1);
+        const T sz = (gp >= 4) ? T(1) : T(-1);
+        T Jm[9], Du[9], iJ[9], H[9], P[9];
+#pragma unroll
+        for (int i = 0; i < 3; i++) {
+            Jm[i * 3 + 0] = Ax[0][i] + sy * Ax[3][i] + sz * Ax[5][i] + (sy * sz) * Ax[6][i];
+            Jm[i * 3 + 1] = Ax[1][i] + sx * Ax[3][i] + sz * Ax[4][i] + (sx * sz) * Ax[6][i];
+            Jm[i * 3 + 2] = Ax[2][i] + sx * Ax[5][i] + sy * Ax[4][i] + (sx * sy) * Ax[6][i];
+            Du[i * 3 + 0] = Au[0][i] + sy * Au[3][i] + sz * Au[5][i] + (sy * sz) * Au[6][i];
+            Du[i * 3 + 1] = Au[1][i] + sx * Au[3][i] + sz * Au[4][i] + (sx * sz) * Au[6][i];
+            Du[i * 3 + 2] = Au[2][i] + sx * Au[5][i] + sy * Au[4][i] + (sx * sy) * Au[6][i];
+        }
+        T dJ = det3(Jm);
+        inv3(Jm, iJ, dJ);
+        T wJ = fabs(dJ);  // weight 1 (GPsDistribution.cpp:392-432)
+#pragma unroll
+        for (int i = 0; i < 3; i++)
+#pragma unroll
+            for (int j = 0; j < 3; j++) H[i * 3 + j] = Du[i * 3] * iJ[j] + Du[i * 3 + 1] * iJ[3 + j] + Du[i * 3 + 2] * iJ[6 + j];
+        pk1(LAW, H, tpa, tpb, P);
+        // Q = wJ P J^-T ; columns Q(:,k)
+        T Q[3][3];
+#pragma unroll
+        for (int i = 0; i < 3; i++)
+#pragma unroll
+            for (int k = 0; k < 3; k++)
+                Q[k][i] = wJ * (P[i * 3] * iJ[k * 3] + P[i * 3 + 1] * iJ[k * 3 + 1] + P[i * 3 + 2] * iJ[k * 3 + 2]);
+#pragma unroll
+        for (int i = 0; i < 3; i++) {
+            G[0][i] += Q[0][i];
+            G[1][i] += Q[1][i];
+            G[2][i] += Q[2][i];
+            G[3][i] += sy * Q[0][i] + sx * Q[1][i];
+            G[4][i] += sz * Q[1][i] + sy * Q[2][i];
+            G[5][i] += sz * Q[0][i] + sx * Q[2][i];
+            G[6][i] += (sy * sz) * Q[0][i] + (sx * sz) * Q[1][i] + (sx * sy) * Q[2][i];
+        }
+    }
+    const T s1 = (T)0.125, s2 = (T)(0.125 * MPS_GP), s3 = (T)(0.125 * MPS_GP * MPS_GP);
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        G[0][i] *= s1;
+        G[1][i] *= s1;
+        G[2][i] *= s1;
+        G[3][i] *= s2;
+        G[4][i] *= s2;
+        G[5][i] *= s2;
+        G[6][i] *= s3;
+    }
+    hex_unbutterfly<T>(G, f);
+    if (dte) {
+        // deformed characteristic length (solvers.cpp:300-317); X and u are re-read (cache hits).
+        // The empty asm makes the node ids opaque so that the 48 gather addresses are not kept
+        // live across the Gauss-point loop.
+        int ndr[8];
+#pragma unroll
+        for (int a = 0; a < 8; a++) {
+            ndr[a] = nd[a];
+            asm volatile("" : "+v"(ndr[a]));
+        }
+        double x[8][3];
+#pragma unroll
+        for (int a = 0; a < 8; a++)
+#pragma unroll
+            for (int d = 0; d < 3; d++) x[a][d] = I.X[d * N + ndr[a]] + u[d * N + ndr[a]];
+        double l = hex_char_length(x);
+        *dte = elem_dt(l, I.l0[e], c);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Tet4 element (1 Gauss point): classLinearTetrahedra.cpp:171-292.
+// J(i,j) = x_{j+1,i} - x_{0,i}; dphi_0 = -(sum of J^-1 rows), dphi_a = row a-1 of J^-1.
+// ---------------------------------------------------------------------------
+template <typename T, int LAW>
+__device__ __forceinline__ void tet4_element(const MpsInst& I, int e, const double* __restrict__ u, T (*f)[3],
+                                             double* dte) {
+    const int4 q = *reinterpret_cast<const int4*>(I.conn + (size_t)e * 4);
+    const int nd[4] = {q.x, q.y, q.z, q.w};
+    const size_t N = (size_t)I.N;
+    double x[4][3], uu[4][3];
+#pragma unroll
+    for (int a = 0; a < 4; a++)
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            x[a][d] = I.X[d * N + nd[a]];
+            uu[a][d] = u[d * N + nd[a]];
+        }
+    double pa, pb, c;
+    elem_mat(I, e, pa, pb, c);
+    T Jm[9], Du[9];
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+            Jm[i * 3 + j] = (T)(x[j + 1][i] - x[0][i]);
+            Du[i * 3 + j] = (T)(uu[j + 1][i] - uu[0][i]);
+        }
+    if (dte) {
+#pragma unroll
+        for (int a = 0; a < 4; a++)
+#pragma unroll
+            for (int d = 0; d < 3; d++) x[a][d] = x[a][d] + uu[a][d];
+        double l = tet_char_length(x);
+        *dte = elem_dt(l, I.l0[e], c);
+    }
+    T iJ[9], H[9], P[9];
+    T dJ = det3(Jm);
+    inv3(Jm, iJ, dJ);
+    T wJ = (T)MPS_TET_W * fabs(dJ);
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) H[i * 3 + j] = Du[i * 3] * iJ[j] + Du[i * 3 + 1] * iJ[3 + j] + Du[i * 3 + 2] * iJ[6 + j];
+    pk1(LAW, H, (T)pa, (T)pb, P);
+    T dphi[4][3];
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+        dphi[0][j] = -iJ[j] - iJ[3 + j] - iJ[6 + j];
+        dphi[1][j] = iJ[j];
+        dphi[2][j] = iJ[3 + j];
+        dphi[3][j] = iJ[6 + j];
+    }
+#pragma unroll
+    for (int a = 0; a < 4; a++)
+#pragma unroll
+        for (int i = 0; i < 3; i++)
+            f[a][i] = wJ * P[i * 3] * dphi[a][0] + wJ * P[i * 3 + 1] * dphi[a][1] + wJ * P[i * 3 + 2] * dphi[a][2];
+}
+
+// ---------------------------------------------------------------------------
+// Step control (begin of step k): lagged time update, termination, stable dt.
+// solverExplicit.cpp:217-273.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int begin_scalars(const MpsInst& I, long long k, const MpsStepSlot& prev,
+                                             unsigned long long dtmin_bits, MpsStepSlot* out) {
+#pragma clang fp contract(off)
+    out->go = 0;
+    out->time = prev.time;
+    out->dt = prev.dt;
+    if (!prev.go) return MPSD_RUNNING;
+    double time;
+    if (I.time_mode == MPS_TIME_FIXED) {
+        if (k > I.n_steps) return MPSD_DONE;
+        time = prev.time + prev.dt;
+    } else {
+        if (!(prev.time < I.tf - 1E-12)) return MPSD_DONE;
+        if (prev.time + prev.dt > I.tf) time = I.tf;
+        else time = prev.time + prev.dt;
+    }
+    double dt = bits2d(dtmin_bits);
+    dt = dt * I.scale;
+    if (dt < 1E-20) return MPSD_ERR_DT;
+    out->time = time;
+    out->dt = dt;
+    out->go = 1;
+    return MPSD_RUNNING;
+}
+
+// Prediction + Dirichlet BC of node n for step k (nonlinearSystem.cpp:1186-1201,
+// solverExplicit.cpp:545-575).
+__device__ __forceinline__ void predict_node(const MpsInst& I, int n, long long k, double dt, double* u, double* v,
+                                             double* a) {
+#pragma clang fp contract(off)
+    uint32_t info = I.ninfo[n];
+    uint32_t mask = info & 7u;
+    uint32_t rec = info >> 3;
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        if (!((mask >> i) & 1u)) {
+            u[i] += dt * v[i] + 0.5 * dt * dt * a[i];
+        } else {
+            double val = I.bcval[3 * (size_t)rec + i];
+            double U, V;
+            if (I.bc_mode == 1) {  // instantaneous: applied once, at the first step
+                U = (k == 1) ? val : 0.0;
+                V = U;
+            } else if (I.bc_mode == 0) {  // ramp: UU*dt (val = U/duration)
+                U = val * dt;
+                V = U;
+            } else {  // velocity amplitude (extension)
+                double fr = (double)k / (double)(I.vel_ramp > 0 ? I.vel_ramp : 1);
+                if (fr > 1.0) fr = 1.0;
+                V = val * fr;
+                U = V * dt;
+            }
+            u[i] += U;
+            v[i] = V;
+            a[i] = 0.;
+        }
+    }
+}
+
+// Gather + lumped-mass solve of node n (nonlinearSystem.cpp:1096-1107; rhs = -fint + fext with fext = 0).
+template <typename TF>
+__device__ __forceinline__ void solve_node(const MpsInst& I, int n, double dt, double* v, double* a) {
+#pragma clang fp contract(off)
+    uint32_t mask = I.ninfo[n] & 7u;
+    if (mask == 7u) return;
+    const TF* __restrict__ fe = (const TF*)I.fe;
+    int q0 = I.rowptr[n], q1 = I.rowptr[n + 1];
+    double f0 = 0.0, f1 = 0.0, f2 = 0.0;
+    for (int q = q0; q < q1; q++) {
+        size_t cidx = (size_t)I.col[q] * 3;
+        f0 += (double)fe[cidx];
+        f1 += (double)fe[cidx + 1];
+        f2 += (double)fe[cidx + 2];
+    }
+    double m = I.mass[n];
+    double fi[3] = {f0, f1, f2};
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        if ((mask >> i) & 1u) continue;
+        double rhs = -fi[i] + 0.0;
+        double an = (m != 0.0) ? rhs / m : 0.0;
+        v[i] += dt * ((1. - 0.5) * a[i] + 0.5 * an);
+        a[i] = an;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Reductions
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ unsigned long long umin64(unsigned long long a, unsigned long long b) { return b < a ? b : a; }
+
+// Block-wide min of 64-bit patterns; result valid in thread 0.
+__device__ __forceinline__ unsigned long long block_min_u64(unsigned long long v, unsigned long long* sh) {
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v = umin64(v, (unsigned long long)__shfl_xor((long long)v, off, 64));
+    if (lane == 0) sh[wid] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const int nw = (blockDim.x + 63) >> 6;
+        for (int w = 1; w < nw; w++) v = umin64(v, sh[w]);
+    }
+    return v;
+}
+
+// Deterministic block sum of NV doubles (fixed tree order); result in thread 0.
+template <int NV>
+__device__ __forceinline__ void block_sum(double* vals, double* sh /*[NV*blockDim]*/) {
+    const int t = threadIdx.x;
+#pragma unroll
+    for (int j = 0; j < NV; j++) sh[j * blockDim.x + t] = vals[j];
+    __syncthreads();
+    for (int s = blockDim.x >> 1; s > 0; s >>= 1) {
+        if (t < s) {
+#pragma unroll
+            for (int j = 0; j < NV; j++) sh[j * blockDim.x + t] += sh[j * blockDim.x + t + s];
+        }
+        __syncthreads();
+    }
+    if (t == 0) {
+#pragma unroll
+        for (int j = 0; j < NV; j++) vals[j] = sh[j * blockDim.x];
+    }
+}
+
+}  // namespace mpsd

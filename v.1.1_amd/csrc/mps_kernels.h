@@ -1,0 +1,22 @@
+// mps_kernels.h -- host-side launchers of the HIP kernels (internal to libmps_gpu).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "mps_common.h"
+
+namespace mpsk {
+// mode 0: step (dt reduction, ctl protocol); mode 1: standalone force evaluation.
+hipError_t launch_elem(const MpsInst& I, int mode, hipStream_t st);
+// fuse = 1: also begin step k+1 (prediction + BC) in the same pass.
+hipError_t launch_node(const MpsInst& I, int fuse, hipStream_t st);
+hipError_t launch_begin(const MpsInst& I, hipStream_t st);
+hipError_t launch_dt(const MpsInst& I, const double* u, unsigned long long* out, hipStream_t st);
+hipError_t launch_setup(const MpsInst& I, double* mslot, double* l0out, unsigned long long* dt0, hipStream_t st);
+hipError_t launch_fint(const MpsInst& I, double* fint, hipStream_t st);
+hipError_t launch_energy(const MpsInst& I, hipStream_t st);
+int energy_blocks(const MpsInst& I);
+// Persistent batched phases: 0 element, 1 node (fuse flag), 2 begin.  bs in {64,128,256}.
+// The batch must be homogeneous in precision, element type (nen) and law.
+hipError_t launch_batch_phase(int phase, int bs, int prec, int nen, int law, const MpsInst* insts, const MpsWork* work,
+                              int nwork, int grid, int fuse, hipStream_t st);
+}  // namespace mpsk

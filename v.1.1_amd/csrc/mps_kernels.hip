@@ -1,0 +1,594 @@
+// mps_kernels.hip -- HIP kernels of the explicit step (gfx950).
+//
+// Per step (fixed pattern, no host round trip):
+//   element kernel  : u_k -> element nodal forces fe (T = double or float) and the
+//                     element-wise stable dt for step k+1 (block min, 64-bit integer atomicMin)
+//   node kernel     : CSR gather of fe (ascending element order, FP64, no float atomics),
+//                     a = -fint/m, v += dt(a_old + a_new)/2, and -- when fused -- the
+//                     begin of step k+1 (lagged time, dt, prediction, Dirichlet BC)
+// plus a standalone begin kernel (first step of a launch) and an energy pass.
+#include "mps_device.h"
+#include "mps_kernels.h"
+
+using namespace mpsd;
+
+#define INF_BITS 0x7FF0000000000000ull
+
+// ---------------------------------------------------------------------------
+// Element kernels
+// ---------------------------------------------------------------------------
+template <typename T, int NEN>
+__device__ __forceinline__ void store_fe(const MpsInst& I, int e, const T (*f)[3]) {
+    T* dst = (T*)I.fe + (size_t)e * (NEN * 3);
+#pragma unroll
+    for (int a = 0; a < NEN; a++)
+#pragma unroll
+        for (int d = 0; d < 3; d++) dst[a * 3 + d] = f[a][d];
+}
+
+template <typename T, int NEN, int LAW>
+__device__ __forceinline__ double elem_work(const MpsInst& I, int e, const double* u, bool want_dt) {
+    T f[NEN][3];
+    double dte = bits2d(INF_BITS);
+    if (NEN == 8) hex8_element<T, LAW>(I, e, u, (T(*)[3])f, want_dt ? &dte : nullptr);
+    else tet4_element<T, LAW>(I, e, u, (T(*)[3])f, want_dt ? &dte : nullptr);
+    store_fe<T, NEN>(I, e, f);
+    return dte;
+}
+
+// mode 0: step (reads ctl->A, writes ctl->B, reduces dt for step k+1)
+// mode 1: standalone force evaluation (no ctl access)
+template <typename T, int NEN, int LAW>
+__global__ __launch_bounds__(256) void k_elem(MpsInst I, int mode) {
+    __shared__ unsigned long long sh[4];
+    long long k = 0;
+    if (mode == 0) {
+        k = I.ctl->A;
+        if (blockIdx.x == 0 && threadIdx.x == 0) I.ctl->B = k;
+        if (!I.ctl->slot[k & 1].go) return;
+    }
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    double dte = bits2d(INF_BITS);
+    if (e < I.Ne) dte = elem_work<T, NEN, LAW>(I, e, I.u, mode == 0);
+    if (mode == 0) {
+        unsigned long long m = block_min_u64(d2bits(dte), sh);
+        if (threadIdx.x == 0) atomicMin(&I.ctl->dtmin[(k + 1) & 1], m);
+    }
+}
+
+// Persistent batched element kernel: work items are element ranges of one instance.
+template <typename T, int NEN, int LAW, int BS>
+__global__ __launch_bounds__(BS) void k_elem_batch(const MpsInst* __restrict__ insts,
+                                                    const MpsWork* __restrict__ work, int nwork) {
+    __shared__ unsigned long long sh[BS / 64 + 1];
+    for (int w = blockIdx.x; w < nwork; w += gridDim.x) {
+        const MpsWork W = work[w];
+        const MpsInst& I = insts[W.inst];
+        const long long k = I.ctl->A;
+        if (W.first && threadIdx.x == 0) I.ctl->B = k;
+        if (I.ctl->slot[k & 1].go) {
+            const int e = W.begin + threadIdx.x;
+            double dte = bits2d(INF_BITS);
+            if (e < W.end) dte = elem_work<T, NEN, LAW>(I, e, I.u, true);
+            unsigned long long m = block_min_u64(d2bits(dte), sh);
+            if (threadIdx.x == 0) atomicMin(&I.ctl->dtmin[(k + 1) & 1], m);
+        }
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Node kernels
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void write_hist_step(const MpsInst& I, long long k, const MpsStepSlot& s) {
+    if (I.hist_cap > 0) {
+        size_t idx = (size_t)((k - 1) % I.hist_cap);
+        I.hist[idx] = s.time;
+        I.hist[(size_t)I.hist_cap + idx] = s.dt;
+    }
+}
+
+template <typename TF>
+__device__ __forceinline__ void node_item(const MpsInst& I, int n, bool first, int fuse) {
+    MpsCtl* ctl = I.ctl;
+    const long long k = ctl->B;
+    const MpsStepSlot s = ctl->slot[k & 1];
+    if (!s.go) return;
+    MpsStepSlot nx;
+    nx.go = 0;
+    int st = MPSD_RUNNING;
+    if (fuse) st = begin_scalars(I, k + 1, s, ctl->dtmin[(k + 1) & 1], &nx);
+    if (first) {
+        ctl->A = k + 1;
+        ctl->dtmin[k & 1] = INF_BITS;
+        if (fuse) {
+            ctl->slot[(k + 1) & 1] = nx;
+            if (st != MPSD_RUNNING) ctl->status = st;
+            if (nx.go) write_hist_step(I, k + 1, nx);
+        }
+    }
+    if (n < 0) return;
+    const size_t N = (size_t)I.N;
+    double u[3], v[3], a[3];
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        u[i] = I.u[i * N + n];
+        v[i] = I.v[i * N + n];
+        a[i] = I.a[i * N + n];
+    }
+    solve_node<TF>(I, n, s.dt, v, a);
+    if (n == I.dof1_node) {
+        double x = u[I.dof1_dir];
+        if (!isfinite(x)) {
+            ctl->nonfinite = 1;
+            atomicMin((unsigned long long*)&ctl->nan_step, (unsigned long long)k);
+        }
+    }
+    if (fuse && nx.go) predict_node(I, n, k + 1, nx.dt, u, v, a);
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        I.u[i * N + n] = u[i];
+        I.v[i * N + n] = v[i];
+        I.a[i * N + n] = a[i];
+    }
+}
+
+template <typename TF>
+__global__ __launch_bounds__(256) void k_node(MpsInst I, int fuse) {
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool first = (n == 0);
+    node_item<TF>(I, n < I.N ? n : -1, first, fuse);
+}
+
+template <typename TF, int BS>
+__global__ __launch_bounds__(BS) void k_node_batch(const MpsInst* __restrict__ insts, const MpsWork* __restrict__ work,
+                                                    int nwork, int fuse) {
+    for (int w = blockIdx.x; w < nwork; w += gridDim.x) {
+        const MpsWork W = work[w];
+        const MpsInst& I = insts[W.inst];
+        const int n = W.begin + threadIdx.x;
+        const bool first = W.first && threadIdx.x == 0;
+        if (I.precision == 1) node_item<float>(I, n < W.end ? n : -1, first, fuse);
+        else node_item<double>(I, n < W.end ? n : -1, first, fuse);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Begin of step k = A (first step of a launch, or after an energy pass)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void begin_item(const MpsInst& I, int n, bool first) {
+    MpsCtl* ctl = I.ctl;
+    const long long k = ctl->A;
+    const MpsStepSlot prev = ctl->slot[(k - 1) & 1];
+    MpsStepSlot nx;
+    const int st = begin_scalars(I, k, prev, ctl->dtmin[k & 1], &nx);
+    if (first) {
+        ctl->slot[k & 1] = nx;
+        if (st != MPSD_RUNNING) ctl->status = st;
+        if (nx.go) write_hist_step(I, k, nx);
+    }
+    if (n < 0 || !nx.go) return;
+    const size_t N = (size_t)I.N;
+    double u[3], v[3], a[3];
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        u[i] = I.u[i * N + n];
+        v[i] = I.v[i * N + n];
+        a[i] = I.a[i * N + n];
+    }
+    predict_node(I, n, k, nx.dt, u, v, a);
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        I.u[i * N + n] = u[i];
+        I.v[i * N + n] = v[i];
+        I.a[i * N + n] = a[i];
+    }
+}
+
+__global__ __launch_bounds__(256) void k_begin(MpsInst I) {
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    begin_item(I, n < I.N ? n : -1, n == 0);
+}
+
+template <int BS>
+__global__ __launch_bounds__(BS) void k_begin_batch(const MpsInst* __restrict__ insts, const MpsWork* __restrict__ work,
+                                                     int nwork) {
+    for (int w = blockIdx.x; w < nwork; w += gridDim.x) {
+        const MpsWork W = work[w];
+        const int n = W.begin + threadIdx.x;
+        begin_item(insts[W.inst], n < W.end ? n : -1, W.first && threadIdx.x == 0);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Stable dt over all elements for a given u (NULL: undeformed), written as a
+// 64-bit pattern min into *out (solvers.cpp:276-320).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_dt(MpsInst I, const double* u, unsigned long long* out) {
+    __shared__ unsigned long long sh[4];
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    double dte = bits2d(INF_BITS);
+    if (e < I.Ne) {
+        double pa, pb, c;
+        elem_mat(I, e, pa, pb, c);
+        const size_t N = (size_t)I.N;
+        if (u == nullptr) {
+            dte = elem_dt(I.l0[e], I.l0[e], c);
+        } else if (I.nen == 8) {
+            double x[8][3];
+            for (int a = 0; a < 8; a++) {
+                int nd = I.conn[(size_t)e * 8 + a];
+                for (int d = 0; d < 3; d++) x[a][d] = I.X[d * N + nd] + u[d * N + nd];
+            }
+            dte = elem_dt(hex_char_length(x), I.l0[e], c);
+        } else {
+            double x[4][3];
+            for (int a = 0; a < 4; a++) {
+                int nd = I.conn[(size_t)e * 4 + a];
+                for (int d = 0; d < 3; d++) x[a][d] = I.X[d * N + nd] + u[d * N + nd];
+            }
+            dte = elem_dt(tet_char_length(x), I.l0[e], c);
+        }
+    }
+    unsigned long long m = block_min_u64(d2bits(dte), sh);
+    if (threadIdx.x == 0) atomicMin(out, m);
+}
+
+// ---------------------------------------------------------------------------
+// Set-up: reference-order Gauss point geometry (classLinear*::defineGPs), initial
+// characteristic length, lumped-mass slot values (constitutiveAlgorithms.cpp:191-205),
+// min l0/c.  No FMA contraction: the lumped mass is bitwise the reference's
+// per-element sums.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void hex_shape_ref(double xi, double eta, double zeta, double* phi, double* nd) {
+#pragma clang fp contract(off)
+    const double x = xi, y = eta, z = zeta;
+    phi[0] = 0.125 * (1 - x) * (1 - y) * (1 - z);
+    phi[1] = 0.125 * (1 + x) * (1 - y) * (1 - z);
+    phi[2] = 0.125 * (1 + x) * (1 + y) * (1 - z);
+    phi[3] = 0.125 * (1 - x) * (1 + y) * (1 - z);
+    phi[4] = 0.125 * (1 - x) * (1 - y) * (1 + z);
+    phi[5] = 0.125 * (1 + x) * (1 - y) * (1 + z);
+    phi[6] = 0.125 * (1 + x) * (1 + y) * (1 + z);
+    phi[7] = 0.125 * (1 - x) * (1 + y) * (1 + z);
+    nd[0] = -0.125 * (1 - y) * (1 - z);
+    nd[3] = 0.125 * (1 - y) * (1 - z);
+    nd[6] = 0.125 * (1 + y) * (1 - z);
+    nd[9] = -0.125 * (1 + y) * (1 - z);
+    nd[12] = -0.125 * (1 - y) * (1 + z);
+    nd[15] = 0.125 * (1 - y) * (1 + z);
+    nd[18] = 0.125 * (1 + y) * (1 + z);
+    nd[21] = -0.125 * (1 + y) * (1 + z);
+    nd[1] = -0.125 * (1 - x) * (1 - z);
+    nd[4] = -0.125 * (1 + x) * (1 - z);
+    nd[7] = 0.125 * (1 + x) * (1 - z);
+    nd[10] = 0.125 * (1 - x) * (1 - z);
+    nd[13] = -0.125 * (1 - x) * (1 + z);
+    nd[16] = -0.125 * (1 + x) * (1 + z);
+    nd[19] = 0.125 * (1 + x) * (1 + z);
+    nd[22] = 0.125 * (1 - x) * (1 + z);
+    nd[2] = -0.125 * (1 - x) * (1 - y);
+    nd[5] = -0.125 * (1 + x) * (1 - y);
+    nd[8] = -0.125 * (1 + x) * (1 + y);
+    nd[11] = -0.125 * (1 - x) * (1 + y);
+    nd[14] = 0.125 * (1 - x) * (1 - y);
+    nd[17] = 0.125 * (1 + x) * (1 - y);
+    nd[20] = 0.125 * (1 + x) * (1 + y);
+    nd[23] = 0.125 * (1 - x) * (1 + y);
+}
+
+template <int NEN>
+__device__ __forceinline__ void setup_element(const MpsInst& I, int e, double rho, double* ms, double* l0) {
+#pragma clang fp contract(off)
+    const size_t N = (size_t)I.N;
+    double x[NEN][3];
+    for (int a = 0; a < NEN; a++) {
+        int nd = I.conn[(size_t)e * NEN + a];
+        for (int d = 0; d < 3; d++) x[a][d] = I.X[d * N + nd];
+    }
+    *l0 = (NEN == 8) ? hex_char_length((const double(*)[3])x) : tet_char_length((const double(*)[3])x);
+    for (int a = 0; a < NEN; a++) ms[a] = 0.0;
+    const int ngp = (NEN == 8) ? 8 : 1;
+    for (int g = 0; g < ngp; g++) {
+        double phi[8], nd[24];
+        double weight;
+        if (NEN == 8) {
+            const double gx = sxi(g) > 0 ? MPS_GP : -MPS_GP;
+            const double gy = seta(g) > 0 ? MPS_GP : -MPS_GP;
+            const double gz = szeta(g) > 0 ? MPS_GP : -MPS_GP;
+            hex_shape_ref(gx, gy, gz, phi, nd);
+            weight = 1 * 1.0;
+        } else {
+            const double q = 0.250000000000, r = 0.25;
+            phi[0] = -1.0 * q - 1.0 * r - 1.0 * r + 1.0;
+            phi[1] = 1.0 * q;
+            phi[2] = 1.0 * r;
+            phi[3] = 1.0 * r;
+            for (int k = 0; k < 12; k++) nd[k] = 0.0;
+            nd[0] = -1.0 - 0.0 - 0.0 + 0.0;
+            nd[3] = 1.0;
+            nd[1] = -0.0 - 1.0 - 0.0 + 0.0;
+            nd[7] = 1.0;
+            nd[2] = -0.0 - 0.0 - 1.0 + 0.0;
+            nd[11] = 1.0;
+            weight = MPS_TET_W;
+        }
+        // Jacobian = xyzCoor * natDev, k innermost (maths.cpp:1679-1693)
+        double Jac[9];
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++) {
+                double s = 0.0;
+                for (int k = 0; k < NEN; k++) s += x[k][i] * nd[k * 3 + j];
+                Jac[i * 3 + j] = s;
+            }
+        double wJ = weight * fabs(det3_strict(Jac));
+        for (int a = 0; a < NEN; a++) {
+            double s = 0.0;
+            for (int b = 0; b < NEN; b++) s += wJ * rho * phi[a] * phi[b];
+            ms[a] += s;
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void k_setup_elem(MpsInst I, double* __restrict__ mslot, double* __restrict__ l0out,
+                                                    unsigned long long* dt0) {
+    __shared__ unsigned long long sh[4];
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    double dte = bits2d(INF_BITS);
+    if (e < I.Ne) {
+        double pa, pb, c;
+        elem_mat(I, e, pa, pb, c);
+        const double rho = I.rho ? I.rho[e] : I.rho0;
+        double l0;
+        if (I.nen == 8) setup_element<8>(I, e, rho, mslot + (size_t)e * 8, &l0);
+        else setup_element<4>(I, e, rho, mslot + (size_t)e * 4, &l0);
+        l0out[e] = l0;
+        dte = elem_dt(l0, l0, c);
+    }
+    unsigned long long m = block_min_u64(d2bits(dte), sh);
+    if (threadIdx.x == 0) atomicMin(dt0, m);
+}
+
+// Node-centric gather of per-(element,slot) scalars: out[n] = sum over CSR(n), ascending.
+__global__ __launch_bounds__(256) void k_gather_scalar(MpsInst I, const double* __restrict__ slotv, double* out) {
+#pragma clang fp contract(off)
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= I.N) return;
+    double acc = 0.0;
+    for (int q = I.rowptr[n]; q < I.rowptr[n + 1]; q++) acc += slotv[I.col[q]];
+    out[n] = acc;
+}
+
+// Internal force for all DOFs (solverExplicit.cpp:363-379), from fe.
+template <typename TF>
+__global__ __launch_bounds__(256) void k_gather_fint(MpsInst I, double* fint) {
+#pragma clang fp contract(off)
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= I.N) return;
+    const TF* fe = (const TF*)I.fe;
+    double f0 = 0.0, f1 = 0.0, f2 = 0.0;
+    for (int q = I.rowptr[n]; q < I.rowptr[n + 1]; q++) {
+        size_t c = (size_t)I.col[q] * 3;
+        f0 += (double)fe[c];
+        f1 += (double)fe[c + 1];
+        f2 += (double)fe[c + 2];
+    }
+    const size_t N = (size_t)I.N;
+    fint[n] = f0;
+    fint[N + n] = f1;
+    fint[2 * N + n] = f2;
+}
+
+// ---------------------------------------------------------------------------
+// Energies of step B: DEFO_ENERGY, KIN_ENERGY, VOLUME (Sum), dataExtraction.cpp:569-776.
+// FP64 in every precision form.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void elem_energy(const MpsInst& I, int e, double* out) {
+    const size_t N = (size_t)I.N;
+    double pa, pb, c;
+    elem_mat(I, e, pa, pb, c);
+    const double rho = I.rho ? I.rho[e] : I.rho0;
+    double ed = 0.0, ek = 0.0, vol = 0.0;
+    if (I.nen == 8) {
+        int nd[8];
+        double x[8][3], uu[8][3], vv[8][3];
+        for (int a = 0; a < 8; a++) {
+            nd[a] = I.conn[(size_t)e * 8 + a];
+            for (int d = 0; d < 3; d++) {
+                x[a][d] = I.X[d * N + nd[a]];
+                uu[a][d] = I.u[d * N + nd[a]];
+                vv[a][d] = I.v[d * N + nd[a]];
+            }
+        }
+        double cx[7][3], cu[7][3], Ax[7][3], Au[7][3];
+        hex_butterfly(x, cx);
+        hex_butterfly(uu, cu);
+        hex_scale<double>(cx, Ax);
+        hex_scale<double>(cu, Au);
+#pragma unroll 1
+        for (int gp = 0; gp < 8; gp++) {
+            const int sx = sxi(gp), sy = seta(gp), sz = szeta(gp);
+            double Jm[9], Du[9], iJ[9], F[9];
+            hex_grad_at<double>(Ax, sx, sy, sz, Jm);
+            hex_grad_at<double>(Au, sx, sy, sz, Du);
+            double dJ = det3(Jm);
+            inv3(Jm, iJ, dJ);
+            double wJ = fabs(dJ);
+            for (int i = 0; i < 3; i++)
+                for (int j = 0; j < 3; j++)
+                    F[i * 3 + j] = (i == j ? 1.0 : 0.0) + Du[i * 3] * iJ[j] + Du[i * 3 + 1] * iJ[3 + j] + Du[i * 3 + 2] * iJ[6 + j];
+            ed += defo_energy(I.law, F, pa, pb) * wJ;
+            double phi[8], ndv[24];
+            hex_shape_ref(sx > 0 ? MPS_GP : -MPS_GP, sy > 0 ? MPS_GP : -MPS_GP, sz > 0 ? MPS_GP : -MPS_GP, phi, ndv);
+            double vel[3] = {0.0, 0.0, 0.0};
+            for (int a = 0; a < 8; a++)
+                for (int j = 0; j < 3; j++) vel[j] += phi[a] * vv[a][j];
+            double kv = 0.0;
+            for (int j = 0; j < 3; j++) kv += 0.5 * rho * vel[j] * vel[j];
+            ek += kv * wJ;
+            vol += wJ * det3(F);
+        }
+    } else {
+        double x[4][3], uu[4][3], vv[4][3];
+        for (int a = 0; a < 4; a++) {
+            int nd = I.conn[(size_t)e * 4 + a];
+            for (int d = 0; d < 3; d++) {
+                x[a][d] = I.X[d * N + nd];
+                uu[a][d] = I.u[d * N + nd];
+                vv[a][d] = I.v[d * N + nd];
+            }
+        }
+        double Jm[9], Du[9], iJ[9], F[9];
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++) {
+                Jm[i * 3 + j] = x[j + 1][i] - x[0][i];
+                Du[i * 3 + j] = uu[j + 1][i] - uu[0][i];
+            }
+        double dJ = det3(Jm);
+        inv3(Jm, iJ, dJ);
+        double wJ = MPS_TET_W * fabs(dJ);
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++)
+                F[i * 3 + j] = (i == j ? 1.0 : 0.0) + Du[i * 3] * iJ[j] + Du[i * 3 + 1] * iJ[3 + j] + Du[i * 3 + 2] * iJ[6 + j];
+        ed = defo_energy(I.law, F, pa, pb) * wJ;
+        double vel[3] = {0.0, 0.0, 0.0};
+        for (int a = 0; a < 4; a++)
+            for (int j = 0; j < 3; j++) vel[j] += 0.25 * vv[a][j];
+        double kv = 0.0;
+        for (int j = 0; j < 3; j++) kv += 0.5 * rho * vel[j] * vel[j];
+        ek = kv * wJ;
+        vol = wJ * det3(F);
+    }
+    out[0] = ed;
+    out[1] = ek;
+    out[2] = vol;
+}
+
+__global__ __launch_bounds__(256) void k_energy(MpsInst I) {
+    __shared__ double sh[3 * 256];
+    const long long k = I.ctl->B;
+    if (!I.ctl->slot[k & 1].go) return;
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    double vals[3] = {0.0, 0.0, 0.0};
+    if (e < I.Ne) elem_energy(I, e, vals);
+    block_sum<3>(vals, sh);
+    if (threadIdx.x == 0)
+        for (int j = 0; j < 3; j++) I.partials[(size_t)blockIdx.x * 3 + j] = vals[j];
+}
+
+__global__ __launch_bounds__(256) void k_energy_reduce(MpsInst I, int nb) {
+    __shared__ double sh[3 * 256];
+    const long long k = I.ctl->B;
+    if (!I.ctl->slot[k & 1].go) return;
+    double vals[3] = {0.0, 0.0, 0.0};
+    for (int b = threadIdx.x; b < nb; b += blockDim.x)
+        for (int j = 0; j < 3; j++) vals[j] += I.partials[(size_t)b * 3 + j];
+    block_sum<3>(vals, sh);
+    if (threadIdx.x == 0 && I.hist_cap > 0) {
+        size_t idx = (size_t)((k - 1) % I.hist_cap);
+        for (int j = 0; j < 3; j++) I.hist[(size_t)(2 + j) * I.hist_cap + idx] = vals[j];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Host-side launchers
+// ---------------------------------------------------------------------------
+namespace mpsk {
+
+static inline unsigned blocks_for(long long n, int bs) { return (unsigned)((n + bs - 1) / bs); }
+
+hipError_t launch_elem(const MpsInst& I, int mode, hipStream_t st) {
+    const unsigned nb = blocks_for(I.Ne, 256);
+    if (nb == 0) return hipSuccess;
+#define MPS_ELEM_LAUNCH(T, NEN)                                                   \
+    do {                                                                          \
+        if (I.law == 0) k_elem<T, NEN, 0><<<nb, 256, 0, st>>>(I, mode);           \
+        else k_elem<T, NEN, 1><<<nb, 256, 0, st>>>(I, mode);                      \
+    } while (0)
+    if (I.nen == 8) {
+        if (I.precision == 1) MPS_ELEM_LAUNCH(float, 8);
+        else MPS_ELEM_LAUNCH(double, 8);
+    } else {
+        if (I.precision == 1) MPS_ELEM_LAUNCH(float, 4);
+        else MPS_ELEM_LAUNCH(double, 4);
+    }
+#undef MPS_ELEM_LAUNCH
+    return hipGetLastError();
+}
+
+hipError_t launch_node(const MpsInst& I, int fuse, hipStream_t st) {
+    const unsigned nb = blocks_for(I.N, 256);
+    if (I.precision == 1) k_node<float><<<nb, 256, 0, st>>>(I, fuse);
+    else k_node<double><<<nb, 256, 0, st>>>(I, fuse);
+    return hipGetLastError();
+}
+
+hipError_t launch_begin(const MpsInst& I, hipStream_t st) {
+    k_begin<<<blocks_for(I.N, 256), 256, 0, st>>>(I);
+    return hipGetLastError();
+}
+
+hipError_t launch_dt(const MpsInst& I, const double* u, unsigned long long* out, hipStream_t st) {
+    k_dt<<<blocks_for(I.Ne, 256), 256, 0, st>>>(I, u, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_setup(const MpsInst& I, double* mslot, double* l0out, unsigned long long* dt0, hipStream_t st) {
+    k_setup_elem<<<blocks_for(I.Ne, 256), 256, 0, st>>>(I, mslot, l0out, dt0);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    k_gather_scalar<<<blocks_for(I.N, 256), 256, 0, st>>>(I, mslot, I.mass);
+    return hipGetLastError();
+}
+
+hipError_t launch_fint(const MpsInst& I, double* fint, hipStream_t st) {
+    if (I.precision == 1) k_gather_fint<float><<<blocks_for(I.N, 256), 256, 0, st>>>(I, fint);
+    else k_gather_fint<double><<<blocks_for(I.N, 256), 256, 0, st>>>(I, fint);
+    return hipGetLastError();
+}
+
+int energy_blocks(const MpsInst& I) { return (int)blocks_for(I.Ne, 256); }
+
+hipError_t launch_energy(const MpsInst& I, hipStream_t st) {
+    const int nb = energy_blocks(I);
+    k_energy<<<nb, 256, 0, st>>>(I);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    k_energy_reduce<<<1, 256, 0, st>>>(I, nb);
+    return hipGetLastError();
+}
+
+template <int BS>
+static hipError_t batch_phase_bs(int phase, int prec, int nen, int law, const MpsInst* insts, const MpsWork* work,
+                                 int nwork, int grid, int fuse, hipStream_t st) {
+    if (phase == 0) {
+#define MPS_EB(T, NEN, LAW) k_elem_batch<T, NEN, LAW, BS><<<grid, BS, 0, st>>>(insts, work, nwork)
+        if (prec == 1) {
+            if (nen == 8) { if (law == 0) MPS_EB(float, 8, 0); else MPS_EB(float, 8, 1); }
+            else { if (law == 0) MPS_EB(float, 4, 0); else MPS_EB(float, 4, 1); }
+        } else {
+            if (nen == 8) { if (law == 0) MPS_EB(double, 8, 0); else MPS_EB(double, 8, 1); }
+            else { if (law == 0) MPS_EB(double, 4, 0); else MPS_EB(double, 4, 1); }
+        }
+#undef MPS_EB
+    } else if (phase == 1) {
+        if (prec == 1) k_node_batch<float, BS><<<grid, BS, 0, st>>>(insts, work, nwork, fuse);
+        else k_node_batch<double, BS><<<grid, BS, 0, st>>>(insts, work, nwork, fuse);
+    } else {
+        k_begin_batch<BS><<<grid, BS, 0, st>>>(insts, work, nwork);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_batch_phase(int phase, int bs, int prec, int nen, int law, const MpsInst* insts, const MpsWork* work,
+                              int nwork, int grid, int fuse, hipStream_t st) {
+    if (nwork <= 0 || grid <= 0) return hipSuccess;
+    switch (bs) {
+        case 64: return batch_phase_bs<64>(phase, prec, nen, law, insts, work, nwork, grid, fuse, st);
+        case 128: return batch_phase_bs<128>(phase, prec, nen, law, insts, work, nwork, grid, fuse, st);
+        case 256: return batch_phase_bs<256>(phase, prec, nen, law, insts, work, nwork, grid, fuse, st);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+}  // namespace mpsk

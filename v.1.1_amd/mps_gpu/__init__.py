@@ -1,0 +1,241 @@
+"""ctypes bindings of libmps_gpu.so (the C-ABI declared in include/mps_gpu.h).
+
+This module is plumbing for the test-suite and the benchmark: it passes numpy
+buffers through the C-ABI unchanged.  There is no fallback path: when the native
+library is missing, `load()` raises.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+PKG_DIR = os.path.dirname(os.path.abspath(__file__))
+LIB_DIR = os.path.join(os.path.dirname(PKG_DIR), "lib")
+LIB_PATH = os.path.join(LIB_DIR, "libmps_gpu.so")
+HEADER = os.path.join(os.path.dirname(os.path.dirname(PKG_DIR)), "include", "mps_gpu.h")
+
+MPS_OK, MPS_ERR_ARG, MPS_ERR_HIP, MPS_ERR_DT, MPS_ERR_NAN, MPS_ERR_STATE, MPS_ERR_CAPACITY = 0, -1, -2, -3, -4, -5, -6
+MPS_RUNNING, MPS_DONE = 0, 1
+LAW_NH_COMPRESSIBLE, LAW_NH_BELYTSCHKO = 0, 1
+BC_RAMP, BC_INST, BC_VELOCITY = 0, 1, 2
+PREC_FP64, PREC_FP32_ELEM = 0, 1
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int32)
+
+
+class Mesh(C.Structure):
+    _fields_ = [("n_nodes", C.c_int32), ("n_elem", C.c_int32), ("nen", C.c_int32), ("_pad", C.c_int32),
+                ("X", _dp), ("conn", _ip)]
+
+
+class Material(C.Structure):
+    _fields_ = [("law", C.c_int32), ("per_element", C.c_int32), ("E", _dp), ("nu", _dp), ("rho", _dp)]
+
+
+class Bcs(C.Structure):
+    _fields_ = [("n_dirichlet", C.c_int32), ("mode", C.c_int32), ("vel_ramp_steps", C.c_int32),
+                ("_pad", C.c_int32), ("dof", _ip), ("value", _dp)]
+
+
+class SolverCfg(C.Structure):
+    _fields_ = [("t0", C.c_double), ("tf", C.c_double), ("scale", C.c_double), ("n_steps", C.c_int64),
+                ("precision", C.c_int32), ("energy_every", C.c_int32), ("history_cap", C.c_int32),
+                ("device", C.c_int32)]
+
+
+class Report(C.Structure):
+    _fields_ = [("steps", C.c_int64), ("time", C.c_double), ("dt", C.c_double), ("status", C.c_int32),
+                ("nonfinite", C.c_int32), ("device_ms", C.c_double)]
+
+
+class BatchCfg(C.Structure):
+    _fields_ = [("arena_bytes", C.c_int64), ("max_instances", C.c_int32), ("precision", C.c_int32),
+                ("steps_per_launch", C.c_int32), ("elem_block", C.c_int32), ("grid_mult", C.c_int32),
+                ("device", C.c_int32)]
+
+
+class MpsError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"mps error {code}: {msg}")
+        self.code = code
+
+
+_lib = None
+
+
+def load(path=None):
+    """Load libmps_gpu.so (raises OSError if it is not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    p = path or LIB_PATH
+    if not os.path.exists(p):
+        raise OSError(f"libmps_gpu.so not built: {p} missing (run __graft_entry__.build())")
+    L = C.CDLL(p)
+    H = C.c_void_p
+    L.mps_create.restype = C.c_int
+    L.mps_create.argtypes = [C.POINTER(Mesh), C.POINTER(Material), C.POINTER(Bcs), C.POINTER(SolverCfg), C.POINTER(H)]
+    L.mps_set_state.argtypes = [H, _dp, _dp, _dp]
+    L.mps_get_state.argtypes = [H, _dp, _dp, _dp]
+    L.mps_run.argtypes = [H, C.c_int64, C.POINTER(Report)]
+    L.mps_get_history.argtypes = [H, C.c_int64, C.c_int64, _dp, _dp, _dp, _dp, _dp]
+    L.mps_get_mass.argtypes = [H, _dp]
+    L.mps_internal_force.argtypes = [H, _dp, _dp]
+    L.mps_critical_dt.argtypes = [H, _dp, _dp]
+    L.mps_destroy.argtypes = [H]
+    L.mps_last_error.restype = C.c_char_p
+    L.mps_last_error.argtypes = [H]
+    L.mps_device_info.argtypes = [C.c_int32, C.c_char_p, C.c_int32]
+    L.mps_profile.argtypes = [H, C.c_int32]
+    L.mps_kernel_times.argtypes = [H, _dp, C.POINTER(C.c_int64)]
+    if hasattr(L, "mps_batch_create"):
+        B = C.c_void_p
+        L.mps_batch_create.argtypes = [C.POINTER(BatchCfg), C.POINTER(B)]
+        L.mps_batch_add.argtypes = [B, C.POINTER(Mesh), C.POINTER(Material), C.POINTER(Bcs), C.POINTER(SolverCfg),
+                                    C.POINTER(C.c_int32)]
+        L.mps_batch_step.argtypes = [B, C.c_int64]
+        L.mps_batch_query.argtypes = [B, C.c_int32, C.POINTER(Report)]
+        L.mps_batch_get_state.argtypes = [B, C.c_int32, _dp, _dp, _dp]
+        L.mps_batch_remove.argtypes = [B, C.c_int32]
+        L.mps_batch_autotune.argtypes = [B, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]
+        L.mps_batch_destroy.argtypes = [B]
+        L.mps_batch_last_error.restype = C.c_char_p
+        L.mps_batch_last_error.argtypes = [B]
+    _lib = L
+    return L
+
+
+def declared_symbols(header=HEADER):
+    """Function names declared in include/mps_gpu.h."""
+    import re
+    txt = open(header).read()
+    return sorted(set(re.findall(r"^\s*(?:int|const char \*)\s*\*?\s*(mps_\w+)\s*\(", txt, re.M)))
+
+
+def device_info(device=0):
+    buf = C.create_string_buffer(256)
+    rc = load().mps_device_info(device, buf, 256)
+    return buf.value.decode() if rc == 0 else None
+
+
+def _d(a):
+    return a.ctypes.data_as(_dp)
+
+
+def _i(a):
+    return a.ctypes.data_as(_ip)
+
+
+class _Inputs:
+    """Keeps the numpy inputs and the ctypes structs of one solve alive."""
+
+    def __init__(self, mesh, law, E, nu, rho, dir_dof, dir_val, bc_mode, vel_ramp_steps,
+                 t0, tf, scale, n_steps, precision, energy_every, history_cap, device):
+        self.X = np.ascontiguousarray(mesh["X"], dtype=np.float64)
+        self.conn = np.ascontiguousarray(mesh["conn"], dtype=np.int32)
+        self.E = np.ascontiguousarray(np.atleast_1d(E), dtype=np.float64)
+        self.nu = np.ascontiguousarray(np.atleast_1d(nu), dtype=np.float64)
+        self.rho = np.ascontiguousarray(np.atleast_1d(rho), dtype=np.float64)
+        per = int(self.E.size > 1)
+        if per and self.rho.size == 1:
+            self.rho = np.full(self.E.size, self.rho[0])
+        self.dof = np.ascontiguousarray(np.zeros(0) if dir_dof is None else dir_dof, dtype=np.int32)
+        self.val = np.ascontiguousarray(np.zeros(0) if dir_val is None else dir_val, dtype=np.float64)
+        self.mesh = Mesh(n_nodes=mesh["n_nodes"], n_elem=mesh["n_elem"], nen=mesh["nen"], X=_d(self.X),
+                         conn=_i(self.conn))
+        self.mat = Material(law=law, per_element=per, E=_d(self.E), nu=_d(self.nu), rho=_d(self.rho))
+        self.bcs = Bcs(n_dirichlet=self.dof.size, mode=bc_mode, vel_ramp_steps=vel_ramp_steps,
+                       dof=_i(self.dof) if self.dof.size else None, value=_d(self.val) if self.val.size else None)
+        self.cfg = SolverCfg(t0=t0, tf=tf, scale=scale, n_steps=n_steps, precision=precision,
+                             energy_every=energy_every, history_cap=history_cap, device=device)
+        self.n_nodes = mesh["n_nodes"]
+
+
+class Solver:
+    """One explicit solve on the device (mps_create ... mps_destroy)."""
+
+    def __init__(self, mesh, law=LAW_NH_COMPRESSIBLE, E=1e6, nu=0.3, rho=1000.0, dir_dof=None, dir_val=None,
+                 bc_mode=BC_VELOCITY, vel_ramp_steps=100, t0=0.0, tf=1.0, scale=0.5, n_steps=0,
+                 precision=PREC_FP64, energy_every=0, history_cap=0, device=0):
+        self.L = load()
+        self.inp = _Inputs(mesh, law, E, nu, rho, dir_dof, dir_val, bc_mode, vel_ramp_steps, t0, tf, scale,
+                           n_steps, precision, energy_every, history_cap, device)
+        self.h = C.c_void_p()
+        rc = self.L.mps_create(C.byref(self.inp.mesh), C.byref(self.inp.mat), C.byref(self.inp.bcs),
+                               C.byref(self.inp.cfg), C.byref(self.h))
+        if rc != 0:
+            raise MpsError(rc, self.L.mps_last_error(None).decode())
+        self.n = self.inp.n_nodes
+
+    def _chk(self, rc):
+        if rc != 0:
+            raise MpsError(rc, self.L.mps_last_error(self.h).decode())
+
+    def set_state(self, u=None, v=None, a=None):
+        arrs = [None if x is None else np.ascontiguousarray(x, dtype=np.float64) for x in (u, v, a)]
+        self._chk(self.L.mps_set_state(self.h, *[None if x is None else _d(x) for x in arrs]))
+
+    def get_state(self):
+        u, v, a = (np.zeros(3 * self.n) for _ in range(3))
+        self._chk(self.L.mps_get_state(self.h, _d(u), _d(v), _d(a)))
+        return u, v, a
+
+    def run(self, max_steps):
+        """Returns (status_code, report dict); errors of the solve are codes, not exceptions."""
+        rep = Report()
+        rc = self.L.mps_run(self.h, max_steps, C.byref(rep))
+        if rc in (MPS_ERR_ARG, MPS_ERR_HIP, MPS_ERR_STATE):
+            self._chk(rc)
+        return rc, dict(steps=rep.steps, time=rep.time, dt=rep.dt, status=rep.status, nonfinite=rep.nonfinite,
+                        device_ms=rep.device_ms)
+
+    def history(self, first, count):
+        out = [np.zeros(count) for _ in range(5)]
+        self._chk(self.L.mps_get_history(self.h, first, count, *[_d(x) for x in out]))
+        return dict(zip(("time", "dt", "e_def", "e_kin", "volume"), out))
+
+    def mass(self):
+        m = np.zeros(self.n)
+        self._chk(self.L.mps_get_mass(self.h, _d(m)))
+        return m
+
+    def internal_force(self, u):
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        f = np.zeros(3 * self.n)
+        self._chk(self.L.mps_internal_force(self.h, _d(u), _d(f)))
+        return f
+
+    def critical_dt(self, u=None):
+        out = C.c_double()
+        if u is not None:
+            u = np.ascontiguousarray(u, dtype=np.float64)
+        self._chk(self.L.mps_critical_dt(self.h, _d(u) if u is not None else None, C.byref(out)))
+        return out.value
+
+    def profile(self, enable=True):
+        self._chk(self.L.mps_profile(self.h, int(enable)))
+
+    def kernel_times(self):
+        ms = np.zeros(4)
+        n = (C.c_int64 * 4)()
+        self._chk(self.L.mps_kernel_times(self.h, _d(ms), n))
+        kinds = ("element", "node", "begin", "energy")
+        return {k: (float(ms[i]), int(n[i])) for i, k in enumerate(kinds)}
+
+    def close(self):
+        if self.h:
+            self.L.mps_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

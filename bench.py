@@ -110,6 +110,7 @@ def main():
     ne, nn = m["n_elem"], m["n_nodes"]
     gp_per_step = ne * 8
 
+    copy_gbs = G.bandwidth_probe(1 << 30, 20)
     r64 = run_gpu(cfg, G.PREC_FP64, args.warmup, args.steps)
     value = gp_per_step * args.steps / r64["wall_s"]
     ems, en = r64["kt"]["element"]
@@ -138,7 +139,8 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                      "kernel": "k_elem<double,8,0> (hex8 element forces + stable dt)",
-                     "kernel_avg_ms": elem_avg_ms, "algorithmic_bytes_per_launch": eb},
+                     "kernel_avg_ms": elem_avg_ms, "algorithmic_bytes_per_launch": eb,
+                     "measured_copy_ceiling_gbs": copy_gbs},
         "kernels": {"element_avg_ms": elem_avg_ms, "node_avg_ms": node_avg_ms,
                     "step_algorithmic_bytes": step_bytes(ne, nn, 8, G.PREC_FP64, False),
                     "step_hbm_frac": step_bytes(ne, nn, 8, G.PREC_FP64, False) / (r64["wall_s"] / args.steps)

@@ -163,6 +163,12 @@ const char *mps_last_error(mps_handle h);
 /* Name and properties of the HIP device used (no reference counterpart). */
 int mps_device_info(int32_t device, char *buf, int32_t len);
 
+/* Measured device-memory copy bandwidth (read + write bytes per second, GB/s) of a
+ * grid-stride copy of `bytes` bytes, averaged over `reps` launches.  No reference
+ * counterpart: it gives the attainable HBM ceiling for the roofline and is the
+ * calibration kernel for hardware traffic counters. */
+int mps_bandwidth_probe(int32_t device, int64_t bytes, int32_t reps, double *copy_gbs);
+
 /* ---------------------------------------------------------------------- */
 /* Batched form: many independent solves packed into one device arena and  */
 /* stepped together by the same kernels (no reference counterpart: the     */

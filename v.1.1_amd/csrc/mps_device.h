@@ -74,55 +74,104 @@ __device__ __forceinline__ T detIpH_m1(const T* H) {
     return tr + i2 + det3(H);
 }
 
+// Cofactor matrix cof(A)(i,j) = (-1)^(i+j) minor_ij; A^-1 = cof(A)^T / det A.
+template <typename T>
+__device__ __forceinline__ void cof3(const T* A, T* C) {
+    C[0] = A[4] * A[8] - A[5] * A[7];
+    C[1] = A[5] * A[6] - A[3] * A[8];
+    C[2] = A[3] * A[7] - A[4] * A[6];
+    C[3] = A[2] * A[7] - A[1] * A[8];
+    C[4] = A[0] * A[8] - A[2] * A[6];
+    C[5] = A[1] * A[6] - A[0] * A[7];
+    C[6] = A[1] * A[5] - A[2] * A[4];
+    C[7] = A[2] * A[3] - A[0] * A[5];
+    C[8] = A[0] * A[4] - A[1] * A[3];
+}
+
+// Reciprocal: hardware estimate refined by Newton steps (<= 1 ulp for FP64).
+__device__ __forceinline__ double rcp(double x) {
+    double r = __builtin_amdgcn_rcp(x);
+    double e = fma(-x, r, 1.0);
+    r = fma(e, r, r);
+    e = fma(-x, r, 1.0);
+    return fma(e, r, r);
+}
+__device__ __forceinline__ float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+
+// log(J) near J = 1 (FP64).  With x = J - 1 (exact for J in [0.5, 2]) and t = x/(2+x),
+// log(1+x) = 2 atanh(t) = 2 (t + t^3/3 + ... + t^15/15); for |x| < 0.1, |t| < 0.0527 and the
+// truncation error is below 2^-60 relative.  Outside that range the library log is used.
+__device__ __forceinline__ double log_near1(double J) {
+    const double x = J - 1.0;
+    if (fabs(x) >= 0.1) return log(J);
+    const double t = x * rcp(2.0 + x);
+    const double t2 = t * t;
+    double p = 1.0 / 15.0;
+    p = fma(p, t2, 1.0 / 13.0);
+    p = fma(p, t2, 1.0 / 11.0);
+    p = fma(p, t2, 1.0 / 9.0);
+    p = fma(p, t2, 1.0 / 7.0);
+    p = fma(p, t2, 1.0 / 5.0);
+    p = fma(p, t2, 1.0 / 3.0);
+    return 2.0 * fma(p * t2, t, t);
+}
+
+// log(1+x) (FP32): Taylor series to x^9 for |x| < 0.05 (truncation < 2e-13 relative);
+// otherwise Goldberg's form log(u) * x / (u - 1), u = 1 + x.
+__device__ __forceinline__ float log1p_fast(float x) {
+    if (fabsf(x) < 0.05f) {
+        float p = 1.f / 9.f;
+        p = fmaf(p, x, -1.f / 8.f);
+        p = fmaf(p, x, 1.f / 7.f);
+        p = fmaf(p, x, -1.f / 6.f);
+        p = fmaf(p, x, 1.f / 5.f);
+        p = fmaf(p, x, -1.f / 4.f);
+        p = fmaf(p, x, 1.f / 3.f);
+        p = fmaf(p, x, -1.f / 2.f);
+        return fmaf(p * x, x, x);
+    }
+    const float u = 1.f + x;
+    return __logf(u) * (x / (u - 1.f));
+}
+
 // ---------------------------------------------------------------------------
-// Constitutive laws
+// Constitutive laws: first Piola-Kirchhoff stress P(H), H = F - I.
+//  3D compressible NH (hyperElasticNeoHookean.cpp:235-257): S = mu I + (lambda lnJ - mu) C^-1,
+//     P = F S.  Since F C^-1 = F^-T this is P = mu F + (lambda lnJ - mu) F^-T, F^-T = cof(F)/J.
+//  Belytschko NH (hyperElasticNeoHookean.cpp:88-126): sigma = mu/J^{5/3} B + (-(mu/3) trB/J^{5/3}
+//     + K (J-1)) I, P = J sigma F^-T = sigma cof(F).
 // ---------------------------------------------------------------------------
-// FP64: the reference formulation on F = I + H.
-//  3D compressible NH: hyperElasticNeoHookean.cpp:235-257 (S = mu I + (lambda lnJ - mu) C^-1, P = F S)
-//  Belytschko NH:      hyperElasticNeoHookean.cpp:88-126  (sigma = mu/J^{5/3} B + (-(mu/3) trB/J^{5/3}
-//                      + K (J-1)) I, P = J sigma F^-T)
-__device__ __forceinline__ void pk1(int law, const double* H, double pa, double pb, double* P) {
+// FP64: F-based, as the reference.
+template <int LAW>
+__device__ __forceinline__ void pk1(const double* H, double pa, double pb, double* P) {
     double F[9];
 #pragma unroll
     for (int i = 0; i < 9; i++) F[i] = H[i];
     F[0] += 1.0;
     F[4] += 1.0;
     F[8] += 1.0;
-    if (law == 0) {  // pa = lambda, pb = mu
-        double J = det3(F);
-        double C[9];
+    double cF[9];
+    cof3(F, cF);
+    const double J = F[0] * cF[0] + F[1] * cF[1] + F[2] * cF[2];
+    if (LAW == 0) {  // pa = lambda, pb = mu
+        const double alpha = pa * log_near1(J) - pb;
+        const double s = alpha * rcp(J);
 #pragma unroll
-        for (int i = 0; i < 3; i++)
-#pragma unroll
-            for (int j = 0; j < 3; j++) C[i * 3 + j] = F[i] * F[j] + F[3 + i] * F[3 + j] + F[6 + i] * F[6 + j];
-        double iC[9];
-        inv3(C, iC, det3(C));
-        double alpha = pa * log(J) - pb;
-        double S[9];
-#pragma unroll
-        for (int i = 0; i < 9; i++) S[i] = alpha * iC[i];
-        S[0] += pb;
-        S[4] += pb;
-        S[8] += pb;
-#pragma unroll
-        for (int i = 0; i < 3; i++)
-#pragma unroll
-            for (int j = 0; j < 3; j++)
-                P[i * 3 + j] = F[i * 3] * S[j] + F[i * 3 + 1] * S[3 + j] + F[i * 3 + 2] * S[6 + j];
+        for (int i = 0; i < 9; i++) P[i] = pb * F[i] + s * cF[i];
     } else {  // pa = mu, pb = K1
         double B[9];
 #pragma unroll
         for (int i = 0; i < 3; i++)
 #pragma unroll
-            for (int j = 0; j < 3; j++)
+            for (int j = i; j < 3; j++) {
                 B[i * 3 + j] = F[i * 3] * F[j * 3] + F[i * 3 + 1] * F[j * 3 + 1] + F[i * 3 + 2] * F[j * 3 + 2];
-        double J = det3(F);
-        double iF[9];
-        inv3(F, iF, J);
-        double trB = B[0] + B[4] + B[8];
-        double f1 = pow(J, 5.0 / 3.0);
-        double f2 = pa / f1;
-        double f3 = -(pa / 3.0) * trB / f1 + pb * (J - 1.0);
+                B[j * 3 + i] = B[i * 3 + j];
+            }
+        const double trB = B[0] + B[4] + B[8];
+        const double f1 = pow(J, 5.0 / 3.0);
+        const double rf1 = rcp(f1);
+        const double f2 = pa * rf1;
+        const double f3 = -(pa / 3.0) * trB * rf1 + pb * (J - 1.0);
         double sg[9];
 #pragma unroll
         for (int i = 0; i < 9; i++) sg[i] = f2 * B[i];
@@ -133,79 +182,61 @@ __device__ __forceinline__ void pk1(int law, const double* H, double pa, double 
         for (int i = 0; i < 3; i++)
 #pragma unroll
             for (int j = 0; j < 3; j++)
-                P[i * 3 + j] = (sg[i * 3] * iF[j * 3] + sg[i * 3 + 1] * iF[j * 3 + 1] + sg[i * 3 + 2] * iF[j * 3 + 2]) * J;
+                P[i * 3 + j] = sg[i * 3] * cF[j] + sg[i * 3 + 1] * cF[3 + j] + sg[i * 3 + 2] * cF[6 + j];
     }
 }
 
-// FP32: the same laws written on H = F - I so that small strains carry no
-// cancellation (C - I = H + H^T + H^T H; J - 1 = tr H + i2 + det H).
-__device__ __forceinline__ void pk1(int law, const float* H, float pa, float pb, float* P) {
-    float Jm1 = detIpH_m1(H);
-    float lnJ = log1pf(Jm1);
-    if (law == 0) {  // S = C^-1 (mu (C - I) + lambda lnJ I), P = (I + H) S
-        float E2[9];
+// FP32: H-based so that small strains carry no cancellation:
+//   J - 1 = tr H + i2(H) + det H,  F - F^-T = H + H^T F^-T,  B - I = H + H^T + H H^T.
+template <int LAW>
+__device__ __forceinline__ void pk1(const float* H, float pa, float pb, float* P) {
+    const float Jm1 = detIpH_m1(H);
+    const float lnJ = log1p_fast(Jm1);
+    float F[9];
+#pragma unroll
+    for (int i = 0; i < 9; i++) F[i] = H[i];
+    F[0] += 1.f;
+    F[4] += 1.f;
+    F[8] += 1.f;
+    float cF[9];
+    cof3(F, cF);
+    if (LAW == 0) {  // P = mu (H + H^T F^-T) + lambda lnJ F^-T
+        const float rJ = rcp(1.f + Jm1);
+        float FiT[9];
+#pragma unroll
+        for (int i = 0; i < 9; i++) FiT[i] = cF[i] * rJ;
+        const float lt = pa * lnJ;
 #pragma unroll
         for (int i = 0; i < 3; i++)
 #pragma unroll
-            for (int j = 0; j < 3; j++)
-                E2[i * 3 + j] = H[i * 3 + j] + H[j * 3 + i] + H[i] * H[j] + H[3 + i] * H[3 + j] + H[6 + i] * H[6 + j];
-        float C[9];
-#pragma unroll
-        for (int i = 0; i < 9; i++) C[i] = E2[i];
-        C[0] += 1.f;
-        C[4] += 1.f;
-        C[8] += 1.f;
-        float iC[9];
-        inv3(C, iC, det3(C));
-        float M[9];
-#pragma unroll
-        for (int i = 0; i < 9; i++) M[i] = pb * E2[i];
-        float lt = pa * lnJ;
-        M[0] += lt;
-        M[4] += lt;
-        M[8] += lt;
-        float S[9];
-#pragma unroll
-        for (int i = 0; i < 3; i++)
-#pragma unroll
-            for (int j = 0; j < 3; j++)
-                S[i * 3 + j] = iC[i * 3] * M[j] + iC[i * 3 + 1] * M[3 + j] + iC[i * 3 + 2] * M[6 + j];
-#pragma unroll
-        for (int i = 0; i < 3; i++)
-#pragma unroll
-            for (int j = 0; j < 3; j++)
-                P[i * 3 + j] = S[i * 3 + j] + H[i * 3] * S[j] + H[i * 3 + 1] * S[3 + j] + H[i * 3 + 2] * S[6 + j];
-    } else {  // sigma = mu J^{-5/3} dev(B - I) + K (J-1) I, P = J sigma F^-T
+            for (int j = 0; j < 3; j++) {
+                const float t = H[i] * FiT[j] + H[3 + i] * FiT[3 + j] + H[6 + i] * FiT[6 + j];
+                P[i * 3 + j] = pb * (H[i * 3 + j] + t) + lt * FiT[i * 3 + j];
+            }
+    } else {  // sigma = mu J^{-5/3} dev(B - I) + K (J-1) I ; P = sigma cof(F)
         float Bm[9];
 #pragma unroll
         for (int i = 0; i < 3; i++)
 #pragma unroll
-            for (int j = 0; j < 3; j++)
+            for (int j = i; j < 3; j++) {
                 Bm[i * 3 + j] = H[i * 3 + j] + H[j * 3 + i] + H[i * 3] * H[j * 3] + H[i * 3 + 1] * H[j * 3 + 1] +
                                 H[i * 3 + 2] * H[j * 3 + 2];
-        float tr3 = (Bm[0] + Bm[4] + Bm[8]) * (1.f / 3.f);
-        float f2 = pa * expf(-(5.f / 3.f) * lnJ);
-        float vol = pb * Jm1;
+                Bm[j * 3 + i] = Bm[i * 3 + j];
+            }
+        const float tr3 = (Bm[0] + Bm[4] + Bm[8]) * (1.f / 3.f);
+        const float f2 = pa * __expf(-(5.f / 3.f) * lnJ);
+        const float vol = pb * Jm1 - f2 * tr3;
         float sg[9];
 #pragma unroll
         for (int i = 0; i < 9; i++) sg[i] = f2 * Bm[i];
-        sg[0] += vol - f2 * tr3;
-        sg[4] += vol - f2 * tr3;
-        sg[8] += vol - f2 * tr3;
-        float F[9];
-#pragma unroll
-        for (int i = 0; i < 9; i++) F[i] = H[i];
-        F[0] += 1.f;
-        F[4] += 1.f;
-        F[8] += 1.f;
-        float J = 1.f + Jm1;
-        float iF[9];
-        inv3(F, iF, J);
+        sg[0] += vol;
+        sg[4] += vol;
+        sg[8] += vol;
 #pragma unroll
         for (int i = 0; i < 3; i++)
 #pragma unroll
             for (int j = 0; j < 3; j++)
-                P[i * 3 + j] = (sg[i * 3] * iF[j * 3] + sg[i * 3 + 1] * iF[j * 3 + 1] + sg[i * 3 + 2] * iF[j * 3 + 2]) * J;
+                P[i * 3 + j] = sg[i * 3] * cF[j] + sg[i * 3 + 1] * cF[3 + j] + sg[i * 3 + 2] * cF[6 + j];
     }
 }
 
@@ -416,7 +447,7 @@ __device__ __forceinline__ void hex8_element(const MpsInst& I, int e, const doub
         const T sx = (gp == 1 || gp == 2 || gp == 5 || gp == 6) ? T(1) : T(-1);
         const T sy = (gp == 2 || gp == 3 || gp == 6 || gp == 7) ? T(1) : T(-1);
         const T sz = (gp >= 4) ? T(1) : T(-1);
-        T Jm[9], Du[9], iJ[9], H[9], P[9];
+        T Jm[9], Du[9], H[9], P[9];
 #pragma unroll
         for (int i = 0; i < 3; i++) {
             Jm[i * 3 + 0] = Ax[0][i] + sy * Ax[3][i] + sz * Ax[5][i] + (sy * sz) * Ax[6][i];
@@ -426,21 +457,25 @@ __device__ __forceinline__ void hex8_element(const MpsInst& I, int e, const doub
             Du[i * 3 + 1] = Au[1][i] + sx * Au[3][i] + sz * Au[4][i] + (sx * sz) * Au[6][i];
             Du[i * 3 + 2] = Au[2][i] + sx * Au[5][i] + sy * Au[4][i] + (sx * sy) * Au[6][i];
         }
-        T dJ = det3(Jm);
-        inv3(Jm, iJ, dJ);
-        T wJ = fabs(dJ);  // weight 1 (GPsDistribution.cpp:392-432)
+        // J^-1 = cof(J)^T / det J ; weight 1 (GPsDistribution.cpp:392-432) so wJ J^-T = sign(det J) cof(J)
+        T cJ[9];
+        cof3(Jm, cJ);
+        const T dJ = Jm[0] * cJ[0] + Jm[1] * cJ[1] + Jm[2] * cJ[2];
+        const T rJ = rcp(dJ);
+        // H = Du J^-1: H(i,j) = sum_k Du(i,k) cof(J)(j,k) / det J
 #pragma unroll
         for (int i = 0; i < 3; i++)
 #pragma unroll
-            for (int j = 0; j < 3; j++) H[i * 3 + j] = Du[i * 3] * iJ[j] + Du[i * 3 + 1] * iJ[3 + j] + Du[i * 3 + 2] * iJ[6 + j];
-        pk1(LAW, H, tpa, tpb, P);
-        // Q = wJ P J^-T ; columns Q(:,k)
+            for (int j = 0; j < 3; j++)
+                H[i * 3 + j] = (Du[i * 3] * cJ[j * 3] + Du[i * 3 + 1] * cJ[j * 3 + 1] + Du[i * 3 + 2] * cJ[j * 3 + 2]) * rJ;
+        pk1<LAW>(H, tpa, tpb, P);
+        const T sg = dJ < T(0) ? T(-1) : T(1);
+        // Q = wJ P J^-T: Q(i,k) = sign(det J) sum_J P(i,J) cof(J)(J,k); columns Q(:,k)
         T Q[3][3];
 #pragma unroll
         for (int i = 0; i < 3; i++)
 #pragma unroll
-            for (int k = 0; k < 3; k++)
-                Q[k][i] = wJ * (P[i * 3] * iJ[k * 3] + P[i * 3 + 1] * iJ[k * 3 + 1] + P[i * 3 + 2] * iJ[k * 3 + 2]);
+            for (int k = 0; k < 3; k++) Q[k][i] = sg * (P[i * 3] * cJ[k] + P[i * 3 + 1] * cJ[3 + k] + P[i * 3 + 2] * cJ[6 + k]);
 #pragma unroll
         for (int i = 0; i < 3; i++) {
             G[0][i] += Q[0][i];
@@ -528,7 +563,7 @@ __device__ __forceinline__ void tet4_element(const MpsInst& I, int e, const doub
     for (int i = 0; i < 3; i++)
 #pragma unroll
         for (int j = 0; j < 3; j++) H[i * 3 + j] = Du[i * 3] * iJ[j] + Du[i * 3 + 1] * iJ[3 + j] + Du[i * 3 + 2] * iJ[6 + j];
-    pk1(LAW, H, (T)pa, (T)pb, P);
+    pk1<LAW>(H, (T)pa, (T)pb, P);
     T dphi[4][3];
 #pragma unroll
     for (int j = 0; j < 3; j++) {
@@ -614,10 +649,24 @@ __device__ __forceinline__ void solve_node(const MpsInst& I, int n, double dt, d
     uint32_t mask = I.ninfo[n] & 7u;
     if (mask == 7u) return;
     const TF* __restrict__ fe = (const TF*)I.fe;
-    int q0 = I.rowptr[n], q1 = I.rowptr[n + 1];
+    int q = I.rowptr[n];
+    const int q1 = I.rowptr[n + 1];
     double f0 = 0.0, f1 = 0.0, f2 = 0.0;
-    for (int q = q0; q < q1; q++) {
-        size_t cidx = (size_t)I.col[q] * 3;
+    // 4 entries per iteration: all loads issue before the (ordered) additions
+    for (; q + 4 <= q1; q += 4) {
+        const size_t c0 = (size_t)I.col[q] * 3, c1 = (size_t)I.col[q + 1] * 3;
+        const size_t c2 = (size_t)I.col[q + 2] * 3, c3 = (size_t)I.col[q + 3] * 3;
+        const double a0 = fe[c0], b0 = fe[c0 + 1], d0 = fe[c0 + 2];
+        const double a1 = fe[c1], b1 = fe[c1 + 1], d1 = fe[c1 + 2];
+        const double a2 = fe[c2], b2 = fe[c2 + 1], d2 = fe[c2 + 2];
+        const double a3 = fe[c3], b3 = fe[c3 + 1], d3 = fe[c3 + 2];
+        f0 += a0; f1 += b0; f2 += d0;
+        f0 += a1; f1 += b1; f2 += d1;
+        f0 += a2; f1 += b2; f2 += d2;
+        f0 += a3; f1 += b3; f2 += d3;
+    }
+    for (; q < q1; q++) {
+        const size_t cidx = (size_t)I.col[q] * 3;
         f0 += (double)fe[cidx];
         f1 += (double)fe[cidx + 1];
         f2 += (double)fe[cidx + 2];
@@ -632,6 +681,18 @@ __device__ __forceinline__ void solve_node(const MpsInst& I, int n, double dt, d
         v[i] += dt * ((1. - 0.5) * a[i] + 0.5 * an);
         a[i] = an;
     }
+}
+
+// ---------------------------------------------------------------------------
+// XCD-aware tile mapping.  Workgroups are dispatched round-robin over the 8 XCDs
+// (each with its own L2).  Remapping physical block b to logical tile
+// L = x*q + min(x,r) + b/8 (x = b%8, q = nb/8, r = nb%8) gives every XCD one contiguous
+// range of tiles, so mesh rows and layers that share nodes (element kernel) or element
+// forces (node kernel) are processed by the same L2.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int xcd_tile(int b, int nb) {
+    const int x = b & 7, i = b >> 3, q = nb >> 3, r = nb & 7;
+    return x * q + (x < r ? x : r) + i;
 }
 
 // ---------------------------------------------------------------------------

@@ -47,7 +47,7 @@ __global__ __launch_bounds__(256) void k_elem(MpsInst I, int mode) {
         if (blockIdx.x == 0 && threadIdx.x == 0) I.ctl->B = k;
         if (!I.ctl->slot[k & 1].go) return;
     }
-    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    const int e = xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     double dte = bits2d(INF_BITS);
     if (e < I.Ne) dte = elem_work<T, NEN, LAW>(I, e, I.u, mode == 0);
     if (mode == 0) {
@@ -135,7 +135,7 @@ __device__ __forceinline__ void node_item(const MpsInst& I, int n, bool first, i
 
 template <typename TF>
 __global__ __launch_bounds__(256) void k_node(MpsInst I, int fuse) {
-    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    const int n = xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     const bool first = (n == 0);
     node_item<TF>(I, n < I.N ? n : -1, first, fuse);
 }
@@ -186,7 +186,7 @@ __device__ __forceinline__ void begin_item(const MpsInst& I, int n, bool first) 
 }
 
 __global__ __launch_bounds__(256) void k_begin(MpsInst I) {
-    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    const int n = xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     begin_item(I, n < I.N ? n : -1, n == 0);
 }
 

@@ -87,6 +87,7 @@ def load(path=None):
     L.mps_last_error.restype = C.c_char_p
     L.mps_last_error.argtypes = [H]
     L.mps_device_info.argtypes = [C.c_int32, C.c_char_p, C.c_int32]
+    L.mps_bandwidth_probe.argtypes = [C.c_int32, C.c_int64, C.c_int32, _dp]
     L.mps_profile.argtypes = [H, C.c_int32]
     L.mps_kernel_times.argtypes = [H, _dp, C.POINTER(C.c_int64)]
     if hasattr(L, "mps_batch_create"):
@@ -117,6 +118,15 @@ def device_info(device=0):
     buf = C.create_string_buffer(256)
     rc = load().mps_device_info(device, buf, 256)
     return buf.value.decode() if rc == 0 else None
+
+
+def bandwidth_probe(nbytes=1 << 30, reps=20, device=0):
+    """Measured copy bandwidth (read+write GB/s) of the device."""
+    out = C.c_double()
+    rc = load().mps_bandwidth_probe(device, nbytes, reps, C.byref(out))
+    if rc != 0:
+        raise MpsError(rc, "bandwidth probe failed")
+    return out.value
 
 
 def _d(a):

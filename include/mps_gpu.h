@@ -192,8 +192,15 @@ int mps_batch_create(const mps_batch_cfg *cfg, mps_batch *out);
  * arena or slot table is full.  The solve must use fixed n_steps (> 0). */
 int mps_batch_add(mps_batch b, const mps_mesh *mesh, const mps_material *mat,
                   const mps_bcs *bcs, const mps_solver_cfg *cfg, int32_t *slot);
-/* Advance every active solve by up to max_steps (graph launches). */
+/* Advance every active solve by max_steps (rounded up to whole graph launches of
+ * steps_per_launch steps; solves stop by themselves at their own n_steps) and wait. */
 int mps_batch_step(mps_batch b, int64_t max_steps);
+/* Enqueue `launches` graph launches without waiting (continuous batching). */
+int mps_batch_launch(mps_batch b, int64_t launches);
+/* Wait for enqueued launches and refresh the per-slot progress read by mps_batch_query. */
+int mps_batch_sync(mps_batch b);
+/* Arena occupancy: free bytes, largest free block, active solves (any pointer may be NULL). */
+int mps_batch_info(mps_batch b, int64_t *free_bytes, int64_t *largest_block, int32_t *active);
 /* Progress of one slot. */
 int mps_batch_query(mps_batch b, int32_t slot, mps_report *rep);
 /* Copy a slot's state out (any pointer NULL to skip). */

@@ -8,7 +8,7 @@ import os
 import numpy as np
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-LIB_PATH = os.path.join(ROOT, "oracle", "lib", "libmps_oracle.so")
+LIB_PATH = os.environ.get("MPS_ORACLE_LIB", os.path.join(ROOT, "oracle", "lib", "libmps_oracle.so"))
 
 LAW_NH_COMPRESSIBLE = 0
 LAW_NH_BELYTSCHKO = 1

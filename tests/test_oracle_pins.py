@@ -1,0 +1,468 @@
+"""Pins of the CPU oracle (oracle/mps_oracle.c) against closed forms, invariants and an
+independent textbook computation, function by function.
+
+The reference ships no golden vectors for this path (no fixtures, no unit tests), so each
+oracle function is checked against what the mathematics fixes, chosen so that a dropped term,
+a wrong sign or a transposed operand fails at least one check.
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import oracle_ctypes as O
+from meshgen import box, clamp_and_pull, perturb
+
+L = O.lib()
+G = 0.577350269  # Gauss abscissa literal of the reference (GPsDistribution.cpp:392-432)
+HEX_S = np.array([[-1, -1, -1], [1, -1, -1], [1, 1, -1], [-1, 1, -1],
+                  [-1, -1, 1], [1, -1, 1], [1, 1, 1], [-1, 1, 1]], dtype=float)
+
+
+def arr(x):
+    return np.ascontiguousarray(x, dtype=np.float64)
+
+
+def pk1(law, F, E=1e6, nu=0.3):
+    P = np.zeros(9)
+    L.orc_pk1(law, O.dptr(arr(F).ravel()), E, nu, O.dptr(P))
+    return P.reshape(3, 3)
+
+
+def energy(law, F, E=1e6, nu=0.3):
+    return L.orc_defo_energy(law, O.dptr(arr(F).ravel()), E, nu)
+
+
+def lame(E, nu):
+    return E * nu / ((1 + nu) * (1 - 2 * nu)), E / (2 * (1 + nu)), E / (3 * (1 - 2 * nu))
+
+
+def gp_data(nen, xyz):
+    xw = np.zeros(32)
+    ngp = L.orc_quadrature(nen, O.dptr(xw))
+    out = []
+    for g in range(ngp):
+        phi, dphi, wJ, dJ = np.zeros(8), np.zeros(24), C.c_double(), C.c_double()
+        L.orc_define_gp(nen, O.dptr(arr(xyz).ravel()), O.dptr(xw[4 * g:4 * g + 4].copy()), O.dptr(phi), O.dptr(dphi),
+                        C.byref(wJ), C.byref(dJ))
+        out.append((phi[:nen].copy(), dphi[:3 * nen].reshape(nen, 3).copy(), wJ.value))
+    return out
+
+
+def rand_F(rng, eps=0.05):
+    return np.eye(3) + rng.uniform(-eps, eps, (3, 3))
+
+
+# ------------------------------------------------------------------ 3x3 algebra
+
+def test_det_inverse_against_numpy():
+    rng = np.random.default_rng(0)
+    for _ in range(50):
+        A = rng.normal(size=(3, 3))
+        inv = np.zeros(9)
+        L.orc_inverse3(O.dptr(arr(A).ravel()), O.dptr(inv))
+        assert L.orc_det3(O.dptr(arr(A).ravel())) == pytest.approx(np.linalg.det(A), rel=1e-12)
+        np.testing.assert_allclose(inv.reshape(3, 3), np.linalg.inv(A), rtol=1e-10, atol=1e-12)
+
+
+# ------------------------------------------------------------------ quadrature, shape functions
+
+def test_quadrature_literals():
+    xw = np.zeros(32)
+    assert L.orc_quadrature(8, O.dptr(xw)) == 8
+    q = xw.reshape(8, 4)
+    np.testing.assert_array_equal(q[:, :3], HEX_S * G)  # same order as the hex nodes
+    np.testing.assert_array_equal(q[:, 3], 1.0)
+    assert L.orc_quadrature(4, O.dptr(xw)) == 1
+    np.testing.assert_array_equal(xw[:4], [0.25, 0.25, 0.25, 0.166666666667])
+
+
+def test_hex_shape_functions_partition_and_kronecker():
+    phi, nd = np.zeros(8), np.zeros(24)
+    for a in range(8):  # phi_b(node a) = delta_ab
+        L.orc_hex8_shape(O.dptr(arr([*HEX_S[a], 0.0])), O.dptr(phi), O.dptr(nd))
+        np.testing.assert_array_equal(phi, np.eye(8)[a])
+    rng = np.random.default_rng(1)
+    for _ in range(10):
+        xi = rng.uniform(-1, 1, 3)
+        L.orc_hex8_shape(O.dptr(arr([*xi, 0.0])), O.dptr(phi), O.dptr(nd))
+        assert phi.sum() == pytest.approx(1.0, abs=1e-15)
+        np.testing.assert_allclose(nd.reshape(8, 3).sum(axis=0), 0.0, atol=1e-15)
+        # natural derivative = finite difference of phi
+        h = 1e-6
+        for d in range(3):
+            p1, p2, tmp = np.zeros(8), np.zeros(8), np.zeros(24)
+            xp, xm = xi.copy(), xi.copy()
+            xp[d] += h
+            xm[d] -= h
+            L.orc_hex8_shape(O.dptr(arr([*xp, 0.0])), O.dptr(p1), O.dptr(tmp))
+            L.orc_hex8_shape(O.dptr(arr([*xm, 0.0])), O.dptr(p2), O.dptr(tmp))
+            np.testing.assert_allclose((p1 - p2) / (2 * h), nd.reshape(8, 3)[:, d], atol=1e-9)
+
+
+@pytest.mark.parametrize("nen", [8, 4])
+def test_define_gp_isoparametric_reproduction(nen):
+    rng = np.random.default_rng(2)
+    A = np.eye(3) + rng.uniform(-0.3, 0.3, (3, 3))
+    b = rng.uniform(-1, 1, 3)
+    ref = (HEX_S + 1) / 2 if nen == 8 else np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0], [0, 0, 1.0]])
+    xyz = ref @ A.T + b
+    gps = gp_data(nen, xyz)
+    wsum = 0.0
+    for phi, dphi, wJ in gps:
+        assert phi.sum() == pytest.approx(1.0, abs=1e-14)
+        np.testing.assert_allclose(dphi.sum(axis=0), 0.0, atol=1e-12)
+        np.testing.assert_allclose(xyz.T @ dphi, np.eye(3), atol=1e-12)  # sum_a X_a (x) dphi_a = I
+        wsum += wJ
+    vol = abs(np.linalg.det(A)) * (1.0 if nen == 8 else 1.0 / 6.0)
+    if nen == 8:
+        assert wsum == pytest.approx(vol, rel=1e-13)  # affine hex, weight 1: exact
+    else:
+        assert wsum == pytest.approx(vol * 6 * 0.166666666667, rel=1e-13)  # tet weight literal (1+2e-12)
+
+
+# ------------------------------------------------------------------ characteristic length
+
+def char_len(nen, xyz):
+    return L.orc_char_length(nen, O.dptr(arr(xyz).ravel()))
+
+
+def test_char_length_cube_and_regular_tet():
+    for h in (0.1, 1.0, 3.7):
+        assert char_len(8, (HEX_S + 1) / 2 * h) == pytest.approx(h, rel=1e-15)  # r = 3h^3/(6h^2)
+    a = 0.8  # regular tetrahedron of edge a: inradius a/(2 sqrt 6)
+    tet = np.array([[1, 1, 1], [1, -1, -1], [-1, 1, -1], [-1, -1, 1.0]]) * a / (2 * np.sqrt(2))
+    assert char_len(4, tet) == pytest.approx(a / np.sqrt(6), rel=1e-14)
+
+
+def test_char_length_rigid_invariance_and_parallelepiped_volume():
+    rng = np.random.default_rng(3)
+    x = (HEX_S + 1) / 2 + rng.uniform(-0.1, 0.1, (8, 3))
+    Q, _ = np.linalg.qr(rng.normal(size=(3, 3)))
+    Q *= np.sign(np.linalg.det(Q))
+    assert char_len(8, x @ Q.T + 5.0) == pytest.approx(char_len(8, x), rel=1e-13)
+    # the hex "volume" is |det(p1-p0, p3-p0, p4-p0)| (geometry.cpp:348-351): moving node 6 along the
+    # normal of none of the faces used by V changes l only through the face areas
+    y = x.copy()
+    y[6] += np.array([0.05, 0.02, 0.03])
+    faces = [[0, 1, 2, 3], [4, 5, 6, 7], [0, 1, 5, 4], [1, 2, 6, 5], [2, 3, 7, 6], [0, 3, 7, 4]]
+
+    def area(p):
+        return sum(0.5 * np.linalg.norm(np.cross(p[f[i]] - p[f[j]], p[f[k]] - p[f[j]]))
+                   for f in faces for (i, k, j) in ((0, 1, 2), (0, 2, 3)))
+    V = abs(np.linalg.det(np.array([x[1] - x[0], x[3] - x[0], x[4] - x[0]])))
+    assert char_len(8, y) == pytest.approx(2 * 3 * V / area(y), rel=1e-13)
+
+
+# ------------------------------------------------------------------ constitutive laws
+
+def test_material_constants_and_sound_speed():
+    for E, nu, rho in ((1e6, 0.3, 1000.0), (3e9, 0.45, 7800.0)):
+        lam, mu, K = np.zeros(1), np.zeros(1), np.zeros(1)
+        L.orc_material(E, nu, O.dptr(lam), O.dptr(mu), O.dptr(K))
+        assert K[0] == pytest.approx(lam[0] + 2 * mu[0] / 3, rel=1e-14)
+        c_p = np.sqrt(E * (1 - nu) / ((1 + nu) * (1 - 2 * nu) * rho))  # P-wave speed
+        assert L.orc_sound_speed(E, nu, rho) == pytest.approx(c_p, rel=1e-14)
+
+
+@pytest.mark.parametrize("law", [0, 1])
+def test_pk1_reference_state_and_objectivity(law):
+    np.testing.assert_allclose(pk1(law, np.eye(3)), 0.0, atol=1e-9)
+    rng = np.random.default_rng(4 + law)
+    for _ in range(10):
+        F = rand_F(rng, 0.1)
+        Q, _ = np.linalg.qr(rng.normal(size=(3, 3)))
+        Q *= np.sign(np.linalg.det(Q))
+        np.testing.assert_allclose(pk1(law, Q @ F), Q @ pk1(law, F), rtol=1e-9, atol=1e-6)
+
+
+def test_pk1_compressible_closed_form_and_energy_gradient():
+    lam, mu, _ = lame(1e6, 0.3)
+    rng = np.random.default_rng(6)
+    for _ in range(10):
+        F = rand_F(rng, 0.2)
+        Fit = np.linalg.inv(F).T
+        J = np.linalg.det(F)
+        np.testing.assert_allclose(pk1(0, F), mu * (F - Fit) + lam * np.log(J) * Fit, rtol=1e-9, atol=1e-6)
+        h = 1e-6  # hyperelasticity: P = dW/dF
+        dW = np.zeros((3, 3))
+        for i in range(3):
+            for j in range(3):
+                Fp, Fm = F.copy(), F.copy()
+                Fp[i, j] += h
+                Fm[i, j] -= h
+                dW[i, j] = (energy(0, Fp) - energy(0, Fm)) / (2 * h)
+        np.testing.assert_allclose(dW, pk1(0, F), rtol=1e-5, atol=1e-2)
+    assert energy(0, np.eye(3)) == pytest.approx(1.5 * mu, rel=1e-15)  # no -3mu/2 constant
+    assert energy(1, rand_F(rng)) == 0.0  # Belytschko law has no defoEnergy
+
+
+def test_pk1_belytschko_closed_form_and_pure_dilation():
+    _, mu, K = lame(1e6, 0.3)
+    rng = np.random.default_rng(7)
+    for _ in range(10):
+        F = rand_F(rng, 0.2)
+        J = np.linalg.det(F)
+        B = F @ F.T
+        sig = mu * J ** (-5 / 3) * (B - np.trace(B) / 3 * np.eye(3)) + K * (J - 1) * np.eye(3)
+        np.testing.assert_allclose(pk1(1, F), J * sig @ np.linalg.inv(F).T, rtol=1e-9, atol=1e-6)
+    s = 1.02
+    F = s * np.eye(3)
+    J = s ** 3
+    np.testing.assert_allclose(pk1(1, F), K * (J - 1) * J * np.linalg.inv(F).T, rtol=1e-12)
+
+
+# ------------------------------------------------------------------ kinematics, force, mass
+
+def element_force(nen, xyz, uloc, law=0, E=1e6, nu=0.3):
+    """Element nodal forces [nen,3] through orc_kinematics/orc_pk1/orc_gp_force."""
+    f = np.zeros((nen, 3))
+    for phi, dphi, wJ in gp_data(nen, xyz):
+        F = np.zeros(9)
+        L.orc_kinematics(nen, O.dptr(arr(dphi).ravel()), O.dptr(arr(uloc).ravel()), O.dptr(F))
+        P = pk1(law, F.reshape(3, 3), E, nu)
+        fg = np.zeros(3 * nen)
+        L.orc_gp_force(nen, wJ, O.dptr(arr(P).ravel()), O.dptr(arr(dphi).ravel()), O.dptr(fg))
+        f += fg.reshape(3, nen).T  # vForce(a + i*nen)
+    return f
+
+
+@pytest.mark.parametrize("nen", [8, 4])
+def test_kinematics_patch_test(nen):
+    rng = np.random.default_rng(8)
+    ref = (HEX_S + 1) / 2 if nen == 8 else np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0], [0, 0, 1.0]])
+    xyz = ref + rng.uniform(-0.1, 0.1, ref.shape)
+    Gm = rand_F(rng, 0.1)
+    u = xyz @ (Gm - np.eye(3)).T  # u = (G - I) X  =>  F = G at every Gauss point
+    for phi, dphi, wJ in gp_data(nen, xyz):
+        F = np.zeros(9)
+        L.orc_kinematics(nen, O.dptr(arr(dphi).ravel()), O.dptr(arr(u).ravel()), O.dptr(F))
+        np.testing.assert_allclose(F.reshape(3, 3), Gm, atol=1e-13)
+        L.orc_kinematics(nen, O.dptr(arr(dphi).ravel()), O.dptr(arr(np.ones((nen, 3)) * 0.3).ravel()), O.dptr(F))
+        np.testing.assert_allclose(F.reshape(3, 3), np.eye(3), atol=1e-14)
+
+
+@pytest.mark.parametrize("law", [0, 1])
+def test_element_force_equilibrium_and_rigid_rotation(law):
+    rng = np.random.default_rng(9)
+    xyz = (HEX_S + 1) / 2 + rng.uniform(-0.1, 0.1, (8, 3))
+    u = rng.uniform(-0.02, 0.02, (8, 3))
+    f = element_force(8, xyz, u, law)
+    assert np.abs(f.sum(axis=0)).max() <= 1e-9 * np.abs(f).max()  # sum_a f_a = 0
+    Q, _ = np.linalg.qr(rng.normal(size=(3, 3)))
+    Q *= np.sign(np.linalg.det(Q))
+    c = xyz.mean(axis=0)
+    urot = (xyz - c) @ Q.T + c - xyz  # rigid rotation: stress free
+    assert np.abs(element_force(8, xyz, urot, law)).max() <= 1e-7 * np.abs(f).max()
+
+
+def element_energy(nen, xyz, uloc, E=1e6, nu=0.3):
+    e = 0.0
+    for phi, dphi, wJ in gp_data(nen, xyz):
+        F = np.zeros(9)
+        L.orc_kinematics(nen, O.dptr(arr(dphi).ravel()), O.dptr(arr(uloc).ravel()), O.dptr(F))
+        e += wJ * energy(0, F.reshape(3, 3), E, nu)
+    return e
+
+
+@pytest.mark.parametrize("nen", [8, 4])
+def test_element_force_is_energy_gradient_at_finite_strain(nen):
+    # f_a = d/du_a sum_gp wJ W(F): pins the P-dphi contraction order (P is not symmetric here)
+    rng = np.random.default_rng(11)
+    ref = (HEX_S + 1) / 2 if nen == 8 else np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0], [0, 0, 1.0]])
+    xyz = ref + rng.uniform(-0.1, 0.1, ref.shape)
+    ang = 0.4
+    R = np.array([[np.cos(ang), -np.sin(ang), 0], [np.sin(ang), np.cos(ang), 0], [0, 0, 1]])
+    u = xyz @ (R @ np.diag([1.15, 0.9, 1.05]) - np.eye(3)).T + rng.uniform(-0.02, 0.02, ref.shape)
+    f = element_force(nen, xyz, u)
+    h = 1e-7
+    g = np.zeros_like(u)
+    for a in range(nen):
+        for i in range(3):
+            up, um = u.copy(), u.copy()
+            up[a, i] += h
+            um[a, i] -= h
+            g[a, i] = (element_energy(nen, xyz, up) - element_energy(nen, xyz, um)) / (2 * h)
+    np.testing.assert_allclose(f, g, rtol=1e-5, atol=1e-5 * np.abs(f).max())
+
+
+def textbook_hex_stiffness(xyz, E, nu):
+    """Small-strain isotropic stiffness of a hex8 (Voigt B-matrix, 2x2x2 Gauss at +-0.577350269)."""
+    lam, mu, _ = lame(E, nu)
+    D = np.zeros((6, 6))
+    D[:3, :3] = lam
+    D[np.arange(3), np.arange(3)] += 2 * mu
+    D[3:, 3:] = np.eye(3) * mu
+    K = np.zeros((24, 24))
+    for s in HEX_S * G:
+        dN = np.array([0.125 * HEX_S[a] * np.array([(1 + HEX_S[a][1] * s[1]) * (1 + HEX_S[a][2] * s[2]),
+                                                    (1 + HEX_S[a][0] * s[0]) * (1 + HEX_S[a][2] * s[2]),
+                                                    (1 + HEX_S[a][0] * s[0]) * (1 + HEX_S[a][1] * s[1])])
+                       for a in range(8)])
+        Jm = xyz.T @ dN
+        dX = dN @ np.linalg.inv(Jm)
+        B = np.zeros((6, 24))
+        for a in range(8):
+            bx, by, bz = dX[a]
+            B[:, 3 * a:3 * a + 3] = [[bx, 0, 0], [0, by, 0], [0, 0, bz], [by, bx, 0], [0, bz, by], [bz, 0, bx]]
+        K += B.T @ D @ B * abs(np.linalg.det(Jm))
+    return K
+
+
+@pytest.mark.parametrize("law", [0, 1])
+def test_element_force_linearises_to_textbook_stiffness(law):
+    rng = np.random.default_rng(10)
+    xyz = (HEX_S + 1) / 2 * 0.1 + rng.uniform(-0.005, 0.005, (8, 3))
+    K = textbook_hex_stiffness(xyz, 1e6, 0.3)
+    r = rng.uniform(-1, 1, (8, 3))
+    eps = 1e-8
+    f = element_force(8, xyz, eps * r, law)
+    np.testing.assert_allclose(f.ravel() / eps, K @ r.ravel(), rtol=1e-5, atol=1e-5 * np.abs(K @ r.ravel()).max())
+
+
+def test_lumped_mass_totals_and_corner_mass():
+    m = box(3, 2, 4)
+    p = O.OracleProblem(m, rho=1000.0, n_steps=1)
+    mass = p.mass()
+    assert mass.sum() == pytest.approx(1000.0 * 1.0, rel=1e-13)  # rho * volume of the unit box
+    h3 = (1 / 3) * (1 / 2) * (1 / 4)
+    assert mass[0] == pytest.approx(1000.0 * h3 / 8, rel=1e-13)  # corner node of one element
+    mt = box(2, 2, 2, tet=True)
+    assert O.OracleProblem(mt, rho=1000.0, n_steps=1).mass().sum() == pytest.approx(1000.0 * 6 * 0.166666666667,
+                                                                                    rel=1e-13)
+
+
+def test_internal_force_homogeneous_patch_interior_nodes():
+    m = perturb(box(3, 3, 3), 0.15, 5)
+    Gm = np.eye(3) + np.array([[0.01, 0.003, 0], [0.002, -0.004, 0.001], [0, 0.002, 0.006]])
+    X = m["X"].reshape(3, -1)
+    u = ((Gm - np.eye(3)) @ X).ravel()
+    f = O.OracleProblem(m, n_steps=1).internal_force(u).reshape(3, -1)
+    interior = [n for n in range(m["n_nodes"]) if np.all((X[:, n] > 1e-9) & (X[:, n] < 1 - 1e-9))]
+    assert len(interior) == 8
+    assert np.abs(f[:, interior]).max() <= 1e-10 * np.abs(f).max()  # homogeneous stress: no interior force
+
+
+# ------------------------------------------------------------------ stable dt
+
+def test_critical_dt_cube_and_compression():
+    m = box(4, 4, 4)
+    c = L.orc_sound_speed(1e6, 0.3, 1000.0)
+    p = O.OracleProblem(m, n_steps=1, scale=0.5)
+    assert p.critical_dt() == pytest.approx(0.5 * 0.25 / c, rel=1e-15)
+    X = m["X"].reshape(3, -1)
+    u = np.zeros((3, m["n_nodes"]))
+    u[2] = -0.1 * X[2]  # uniform 10% compression in z
+    hx, hz = 0.25, 0.225
+    l_def = 2 * 3 * (hx * hx * hz) / (2 * hx * hx + 4 * hx * hz)
+    assert p.critical_dt(u.ravel()) == pytest.approx(0.5 * l_def / c, rel=1e-14)
+    u[2] = +0.1 * X[2]  # stretching cannot raise dt above the undeformed value: min(l0, l)
+    assert p.critical_dt(u.ravel()) == pytest.approx(0.5 * 0.25 / c, rel=1e-15)
+
+
+# ------------------------------------------------------------------ integrator and quirks
+
+def test_rigid_translation_integrates_prescribed_velocity_exactly():
+    m = box(2, 2, 2)
+    N = m["n_nodes"]
+    dof = np.arange(N, dtype=np.int32)  # every node prescribed in x, velocity ramp over 10 steps
+    val = np.full(N, 0.3)
+    p = O.OracleProblem(m, dir_dof=dof, dir_val=val, bc_mode=O.BC_VELOCITY, vel_ramp_steps=10, n_steps=25,
+                        energy_every=1)
+    r = p.run(hist_cap=25)
+    dts = r["hist"]["dt"]
+    V = 0.3 * np.minimum(np.arange(1, 26) / 10, 1.0)
+    ux = np.cumsum(V * dts)[-1]
+    np.testing.assert_allclose(r["u"][:N], ux, rtol=1e-14)
+    assert np.abs(r["u"][N:]).max() <= 1e-14 * ux  # stress-free: only round-off moves y, z
+    np.testing.assert_allclose(r["v"][:N], 0.3)
+    # uniform velocity: E_kin = rho V v^2 / 2 ; undeformed E_def = 1.5 mu V ; VOLUME = V
+    _, mu, _ = lame(1e6, 0.3)
+    assert r["hist"]["e_kin"][-1] == pytest.approx(0.5 * 1000.0 * 0.3 ** 2, rel=1e-13)
+    assert r["hist"]["e_def"][-1] == pytest.approx(1.5 * mu, rel=1e-13)
+    assert r["hist"]["volume"][-1] == pytest.approx(1.0, rel=1e-13)
+
+
+def test_single_dof_central_difference_recurrence():
+    m = box(1, 1, 1)
+    N = m["n_nodes"]
+    free = 6 + 2 * N  # node 6 (the +,+,+ corner), z
+    dofs = np.array([d for d in range(3 * N) if d != free], np.int32)
+    p = O.OracleProblem(m, dir_dof=dofs, dir_val=np.zeros(dofs.size), bc_mode=O.BC_VELOCITY, n_steps=60,
+                        scale=0.5)
+    v0 = np.zeros(3 * N)
+    v0[free] = 1e-9
+    r = p.run(v0=v0, hist_cap=60)
+    mass = p.mass()[6]
+    du = 1e-10
+    e = np.zeros(3 * N)
+    e[free] = du
+    k = (p.internal_force(e)[free] - p.internal_force(-e)[free]) / (2 * du)
+    u, v, a = 0.0, 1e-9, 0.0  # a0 = 0 (solverExplicit.cpp:143)
+    for dt in r["hist"]["dt"]:
+        u = u + dt * v + 0.5 * dt * dt * a
+        an = -k * u / mass
+        v = v + dt * (0.5 * a + 0.5 * an)
+        a = an
+    assert r["u"][free] == pytest.approx(u, rel=1e-6)
+    assert r["v"][free] == pytest.approx(v, rel=1e-6)
+
+
+def test_ramp_quirks_lagged_time_and_window():
+    m = box(2, 2, 2)
+    dd, dv = clamp_and_pull(m, vel=0.01)  # ramp: total U = 0.01 over [t0, tf]
+    t0, tf = 0.0, 0.02
+    p = O.OracleProblem(m, dir_dof=dd, dir_val=dv, bc_mode=O.BC_RAMP, t0=t0, tf=tf, scale=0.5, n_steps=0)
+    r = p.run(hist_cap=10000)
+    n = r["steps"]
+    t, dt = r["hist"]["time"][:n], r["hist"]["dt"][:n]
+    dt0 = p.critical_dt()
+    lag = np.concatenate([[dt0], dt[:-1]])  # time advances by the previous dt (solverExplicit.cpp:227-232)
+    np.testing.assert_allclose(t[:-1], t0 + np.cumsum(lag)[:-1], rtol=1e-13)
+    assert t[-1] == tf and t[-2] < tf - 1e-12  # last time clamped to tf, window terminates
+    top = dd[-1]
+    UU = 0.01 / (tf - t0)
+    assert r["v"][top] == pytest.approx(UU * dt[-1], rel=1e-14)  # v := UU*dt on ramp DOFs
+    assert r["u"][top] == pytest.approx(UU * dt.sum(), rel=1e-12)
+
+
+def test_instantaneous_bc_applied_once():
+    m = box(2, 2, 2)
+    dd, dv = clamp_and_pull(m, vel=0.001)
+    p = O.OracleProblem(m, dir_dof=dd, dir_val=dv, bc_mode=O.BC_INST, n_steps=1)
+    r1 = p.run()
+    top = dd[-1]
+    assert r1["u"][top] == 0.001 and r1["v"][top] == 0.001
+    p5 = O.OracleProblem(m, dir_dof=dd, dir_val=dv, bc_mode=O.BC_INST, n_steps=5)
+    r5 = p5.run()
+    assert r5["u"][top] == 0.001 and r5["v"][top] == 0.0
+
+
+def test_free_vibration_conserves_energy():
+    # No BCs, smooth initial velocity: the central-difference scheme with lumped mass M keeps
+    # 1/2 v^T M v + W(u) bounded to O((omega dt)^2); W(u) - W(0) from DEFO_ENERGY (W(0) = 1.5 mu V).
+    m = box(6, 6, 6)
+    X = m["X"].reshape(3, -1)
+    v0 = np.zeros((3, m["n_nodes"]))
+    v0[0] = 0.01 * np.sin(np.pi * X[2])
+    v0[0] -= v0[0].mean()
+    p = O.OracleProblem(m, n_steps=300, scale=0.5, energy_every=1)
+    r = p.run(v0=v0.ravel(), hist_cap=300)
+    mass = np.tile(p.mass(), 3)
+    _, mu, _ = lame(1e6, 0.3)
+    t0 = 0.5 * np.sum(mass * v0.ravel() ** 2)
+    t1 = 0.5 * np.sum(mass * r["v"] ** 2)
+    w1 = r["hist"]["e_def"][-1] - 1.5 * mu * 1.0
+    assert abs(t1 + w1 - t0) <= 0.05 * t0
+    ek = r["hist"]["e_kin"]
+    assert ek.min() < 0.5 * ek.max()  # kinetic and strain energy are actually exchanged
+
+
+def test_summation_orders_and_threads_agree():
+    m = perturb(box(5, 4, 3), 0.1, 12)
+    dd, dv = clamp_and_pull(m)
+    runs = []
+    for so, nt in ((O.SUM_REFERENCE, 1), (O.SUM_ELEMENT, 1), (O.SUM_ELEMENT, 4)):
+        runs.append(O.OracleProblem(m, dir_dof=dd, dir_val=dv, n_steps=200, sum_order=so, n_threads=nt).run()["u"])
+    assert np.abs(runs[0] - runs[1]).max() <= 1e-13 * np.abs(runs[0]).max()
+    assert np.array_equal(runs[1], runs[2])

@@ -29,3 +29,6 @@ int prepare(const mps_mesh* mesh, const mps_material* mat, const mps_bcs* bcs, c
             HostPrep* hp, std::string* err);
 
 }  // namespace mpsh
+
+// Device bytes of one solve in a batch arena (mps_batch.hip).
+size_t mps_instance_bytes(int64_t N, int64_t Ne, int nen, int precision, bool per_elem, size_t nrec);

@@ -15,8 +15,11 @@ hipError_t launch_setup(const MpsInst& I, double* mslot, double* l0out, unsigned
 hipError_t launch_fint(const MpsInst& I, double* fint, hipStream_t st);
 hipError_t launch_energy(const MpsInst& I, hipStream_t st);
 int energy_blocks(const MpsInst& I);
+hipError_t launch_init_ctl(const MpsInst& I, const unsigned long long* dt0, hipStream_t st);
 // Persistent batched phases: 0 element, 1 node (fuse flag), 2 begin.  bs in {64,128,256}.
 // The batch must be homogeneous in precision, element type (nen) and law.
+// nwork points to the item count in device memory (graphs stay valid when tables change).
 hipError_t launch_batch_phase(int phase, int bs, int prec, int nen, int law, const MpsInst* insts, const MpsWork* work,
-                              int nwork, int grid, int fuse, hipStream_t st);
+                              const int* nwork, int grid, int fuse, hipStream_t st);
+hipError_t launch_fill_u64(unsigned long long* p, unsigned long long v, hipStream_t st);
 }  // namespace mpsk

@@ -56,12 +56,28 @@ __global__ __launch_bounds__(256) void k_elem(MpsInst I, int mode) {
     }
 }
 
+// Persistent batched kernels walk the work list XCD-aware: with gridDim.x a multiple of 8, the
+// blocks of XCD x (= blockIdx.x % 8) stride over the contiguous chunk [x*n/8, (x+1)*n/8) of items,
+// so neighbouring items of an instance share one L2.  Any other grid falls back to a plain stride.
+__device__ __forceinline__ int batch_first(int n) {
+    if (gridDim.x & 7) return blockIdx.x;
+    const int x = blockIdx.x & 7;
+    return (int)(((long long)n * x) >> 3) + (blockIdx.x >> 3);
+}
+__device__ __forceinline__ int batch_end(int n) {
+    if (gridDim.x & 7) return n;
+    const int x = blockIdx.x & 7;
+    return (int)(((long long)n * (x + 1)) >> 3);
+}
+__device__ __forceinline__ int batch_stride() { return (gridDim.x & 7) ? gridDim.x : (gridDim.x >> 3); }
+
 // Persistent batched element kernel: work items are element ranges of one instance.
 template <typename T, int NEN, int LAW, int BS>
 __global__ __launch_bounds__(BS) void k_elem_batch(const MpsInst* __restrict__ insts,
-                                                    const MpsWork* __restrict__ work, int nwork) {
+                                                    const MpsWork* __restrict__ work, const int* __restrict__ nwork_p) {
     __shared__ unsigned long long sh[BS / 64 + 1];
-    for (int w = blockIdx.x; w < nwork; w += gridDim.x) {
+    const int nwork = *nwork_p;
+    for (int w = batch_first(nwork); w < batch_end(nwork); w += batch_stride()) {
         const MpsWork W = work[w];
         const MpsInst& I = insts[W.inst];
         const long long k = I.ctl->A;
@@ -142,8 +158,9 @@ __global__ __launch_bounds__(256) void k_node(MpsInst I, int fuse) {
 
 template <typename TF, int BS>
 __global__ __launch_bounds__(BS) void k_node_batch(const MpsInst* __restrict__ insts, const MpsWork* __restrict__ work,
-                                                    int nwork, int fuse) {
-    for (int w = blockIdx.x; w < nwork; w += gridDim.x) {
+                                                    const int* __restrict__ nwork_p, int fuse) {
+    const int nwork = *nwork_p;
+    for (int w = batch_first(nwork); w < batch_end(nwork); w += batch_stride()) {
         const MpsWork W = work[w];
         const MpsInst& I = insts[W.inst];
         const int n = W.begin + threadIdx.x;
@@ -192,8 +209,9 @@ __global__ __launch_bounds__(256) void k_begin(MpsInst I) {
 
 template <int BS>
 __global__ __launch_bounds__(BS) void k_begin_batch(const MpsInst* __restrict__ insts, const MpsWork* __restrict__ work,
-                                                     int nwork) {
-    for (int w = blockIdx.x; w < nwork; w += gridDim.x) {
+                                                     const int* __restrict__ nwork_p) {
+    const int nwork = *nwork_p;
+    for (int w = batch_first(nwork); w < batch_end(nwork); w += batch_stride()) {
         const MpsWork W = work[w];
         const int n = W.begin + threadIdx.x;
         begin_item(insts[W.inst], n < W.end ? n : -1, W.first && threadIdx.x == 0);
@@ -347,6 +365,32 @@ __global__ __launch_bounds__(256) void k_setup_elem(MpsInst I, double* __restric
     }
     unsigned long long m = block_min_u64(d2bits(dte), sh);
     if (threadIdx.x == 0) atomicMin(dt0, m);
+}
+
+// Initial control block of a solve from the set-up reduction (device side, so that admitting
+// a solve into a running batch needs no host round trip): slot[0] = {t0, s*min(l0/c), go}.
+__global__ void k_fill_u64(unsigned long long* p, unsigned long long v) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) *p = v;
+}
+
+__global__ void k_init_ctl(MpsInst I, const unsigned long long* dt0) {
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    MpsCtl* c = I.ctl;
+    const unsigned long long inf = INF_BITS;
+    c->slot[0].time = I.t0;
+    c->slot[0].dt = bits2d(*dt0) * I.scale;  // solverExplicit.cpp:165-166
+    c->slot[0].go = 1;
+    c->slot[1].time = I.t0;
+    c->slot[1].dt = 0.0;
+    c->slot[1].go = 0;
+    c->dtmin[0] = inf;
+    c->dtmin[1] = inf;
+    c->A = 1;
+    c->B = 0;
+    c->status = MPSD_RUNNING;
+    c->nonfinite = 0;
+    c->nan_step = 0x7FFFFFFFFFFFFFFFll;
+    c->dt0min = *dt0;
 }
 
 // Node-centric gather of per-(element,slot) scalars: out[n] = sum over CSR(n), ascending.
@@ -549,6 +593,16 @@ hipError_t launch_fint(const MpsInst& I, double* fint, hipStream_t st) {
 
 int energy_blocks(const MpsInst& I) { return (int)blocks_for(I.Ne, 256); }
 
+hipError_t launch_fill_u64(unsigned long long* p, unsigned long long v, hipStream_t st) {
+    k_fill_u64<<<1, 64, 0, st>>>(p, v);
+    return hipGetLastError();
+}
+
+hipError_t launch_init_ctl(const MpsInst& I, const unsigned long long* dt0, hipStream_t st) {
+    k_init_ctl<<<1, 64, 0, st>>>(I, dt0);
+    return hipGetLastError();
+}
+
 hipError_t launch_energy(const MpsInst& I, hipStream_t st) {
     const int nb = energy_blocks(I);
     k_energy<<<nb, 256, 0, st>>>(I);
@@ -560,7 +614,7 @@ hipError_t launch_energy(const MpsInst& I, hipStream_t st) {
 
 template <int BS>
 static hipError_t batch_phase_bs(int phase, int prec, int nen, int law, const MpsInst* insts, const MpsWork* work,
-                                 int nwork, int grid, int fuse, hipStream_t st) {
+                                 const int* nwork, int grid, int fuse, hipStream_t st) {
     if (phase == 0) {
 #define MPS_EB(T, NEN, LAW) k_elem_batch<T, NEN, LAW, BS><<<grid, BS, 0, st>>>(insts, work, nwork)
         if (prec == 1) {
@@ -581,8 +635,8 @@ static hipError_t batch_phase_bs(int phase, int prec, int nen, int law, const Mp
 }
 
 hipError_t launch_batch_phase(int phase, int bs, int prec, int nen, int law, const MpsInst* insts, const MpsWork* work,
-                              int nwork, int grid, int fuse, hipStream_t st) {
-    if (nwork <= 0 || grid <= 0) return hipSuccess;
+                              const int* nwork, int grid, int fuse, hipStream_t st) {
+    if (grid <= 0) return hipSuccess;
     switch (bs) {
         case 64: return batch_phase_bs<64>(phase, prec, nen, law, insts, work, nwork, grid, fuse, st);
         case 128: return batch_phase_bs<128>(phase, prec, nen, law, insts, work, nwork, grid, fuse, st);

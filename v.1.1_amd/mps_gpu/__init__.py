@@ -101,6 +101,9 @@ def load(path=None):
         L.mps_batch_remove.argtypes = [B, C.c_int32]
         L.mps_batch_autotune.argtypes = [B, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]
         L.mps_batch_destroy.argtypes = [B]
+        L.mps_batch_launch.argtypes = [B, C.c_int64]
+        L.mps_batch_sync.argtypes = [B]
+        L.mps_batch_info.argtypes = [B, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_int32)]
         L.mps_batch_last_error.restype = C.c_char_p
         L.mps_batch_last_error.argtypes = [B]
     _lib = L
@@ -237,6 +240,89 @@ class Solver:
         if self.h:
             self.L.mps_destroy(self.h)
             self.h = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Batch:
+    """Batched form: many independent fixed-step solves stepped together (mps_batch_*)."""
+
+    def __init__(self, arena_bytes=1 << 30, max_instances=1024, precision=PREC_FP64, steps_per_launch=32,
+                 elem_block=0, grid_mult=0, device=0):
+        self.L = load()
+        self.cfg = BatchCfg(arena_bytes=arena_bytes, max_instances=max_instances, precision=precision,
+                            steps_per_launch=steps_per_launch, elem_block=elem_block, grid_mult=grid_mult,
+                            device=device)
+        self.b = C.c_void_p()
+        rc = self.L.mps_batch_create(C.byref(self.cfg), C.byref(self.b))
+        if rc != 0:
+            raise MpsError(rc, "mps_batch_create failed")
+        self.inputs = {}
+        self.precision = precision
+
+    def _chk(self, rc):
+        if rc != 0:
+            raise MpsError(rc, self.L.mps_batch_last_error(self.b).decode())
+
+    def add(self, mesh, law=LAW_NH_COMPRESSIBLE, E=1e6, nu=0.3, rho=1000.0, dir_dof=None, dir_val=None,
+            bc_mode=BC_VELOCITY, vel_ramp_steps=100, scale=0.5, n_steps=1000, t0=0.0, tf=1.0):
+        inp = _Inputs(mesh, law, E, nu, rho, dir_dof, dir_val, bc_mode, vel_ramp_steps, t0, tf, scale, n_steps,
+                      self.precision, 0, 0, self.cfg.device)
+        slot = C.c_int32()
+        rc = self.L.mps_batch_add(self.b, C.byref(inp.mesh), C.byref(inp.mat), C.byref(inp.bcs), C.byref(inp.cfg),
+                                  C.byref(slot))
+        self._chk(rc)
+        self.inputs[slot.value] = inp
+        return slot.value
+
+    def step(self, max_steps):
+        self._chk(self.L.mps_batch_step(self.b, max_steps))
+
+    def launch(self, launches):
+        self._chk(self.L.mps_batch_launch(self.b, launches))
+
+    def sync(self):
+        self._chk(self.L.mps_batch_sync(self.b))
+
+    def query(self, slot):
+        rep = Report()
+        self._chk(self.L.mps_batch_query(self.b, slot, C.byref(rep)))
+        return dict(steps=rep.steps, time=rep.time, dt=rep.dt, status=rep.status, nonfinite=rep.nonfinite)
+
+    def get_state(self, slot):
+        n = self.inputs[slot].n_nodes
+        u, v, a = (np.zeros(3 * n) for _ in range(3))
+        self._chk(self.L.mps_batch_get_state(self.b, slot, _d(u), _d(v), _d(a)))
+        return u, v, a
+
+    def remove(self, slot):
+        self._chk(self.L.mps_batch_remove(self.b, slot))
+        self.inputs.pop(slot, None)
+
+    def info(self):
+        f, l_, a = C.c_int64(), C.c_int64(), C.c_int32()
+        self._chk(self.L.mps_batch_info(self.b, C.byref(f), C.byref(l_), C.byref(a)))
+        return dict(free=f.value, largest=l_.value, active=a.value)
+
+    def autotune(self):
+        bs, gm = C.c_int32(), C.c_int32()
+        self._chk(self.L.mps_batch_autotune(self.b, C.byref(bs), C.byref(gm)))
+        return bs.value, gm.value
+
+    def close(self):
+        if self.b:
+            self.L.mps_batch_destroy(self.b)
+            self.b = C.c_void_p()
 
     def __enter__(self):
         return self

@@ -212,6 +212,51 @@ int mps_batch_autotune(mps_batch b, int32_t *elem_block, int32_t *grid_mult);
 int mps_batch_destroy(mps_batch b);
 const char *mps_batch_last_error(mps_batch b);
 
+/* ---------------------------------------------------------------------- */
+/* Serving form: a request loop with continuous batching over one batch     */
+/* (no reference counterpart: the reference runs one solve per process).   */
+/* Requests are admitted in arrival order when a slot and an arena block of */
+/* their device footprint are free; their state stays on the device until  */
+/* they finish.  The server object is internally synchronised.              */
+/* ---------------------------------------------------------------------- */
+
+#define MPS_REQ_QUEUED 0
+#define MPS_REQ_RUNNING 1
+#define MPS_REQ_DONE 2
+#define MPS_REQ_FAILED 3
+
+typedef struct mps_server_s *mps_server;
+
+typedef struct mps_server_cfg {
+    mps_batch_cfg batch; /* arena, slots, precision, steps per graph launch, tuning */
+} mps_server_cfg;
+
+typedef struct mps_server_stats {
+    int64_t submitted, completed, failed, launches;
+    double p50_ms, p99_ms, max_ms;  /* latency: submission -> result on the host */
+    double queue_p50_ms;            /* submission -> admission */
+    double span_ms;                 /* first submission -> last completion */
+    double solves_per_s;            /* completed / span */
+    double busy_ms;                 /* time with a graph launch in flight */
+} mps_server_stats;
+
+int mps_server_create(const mps_server_cfg *cfg, mps_server *out);
+/* Validate and prepare a request on the calling thread, then queue it; returns its id. */
+int mps_server_submit(mps_server s, const mps_mesh *mesh, const mps_material *mat, const mps_bcs *bcs,
+                      const mps_solver_cfg *cfg, int64_t *id);
+/* Non-blocking: *state = MPS_REQ_*.  When finished, returns the solve's status code and, if u
+ * is not NULL, copies the final displacement [3*n_nodes] and releases the request; lat_ms
+ * (may be NULL) receives {latency, queue wait} in milliseconds. */
+int mps_server_poll(mps_server s, int64_t id, int32_t *state, double *u, double *lat_ms);
+/* Block until the request finishes or timeout_s elapses. */
+int mps_server_wait(mps_server s, int64_t id, double timeout_s, int32_t *state);
+/* Block until every submitted request finished (MPS_ERR_STATE on timeout). */
+int mps_server_drain(mps_server s, double timeout_s);
+int mps_server_get_stats(mps_server s, mps_server_stats *out);
+int mps_server_reset_stats(mps_server s);
+int mps_server_destroy(mps_server s);
+const char *mps_server_last_error(mps_server s);
+
 #ifdef __cplusplus
 }
 #endif

@@ -32,3 +32,8 @@ int prepare(const mps_mesh* mesh, const mps_material* mat, const mps_bcs* bcs, c
 
 // Device bytes of one solve in a batch arena (mps_batch.hip).
 size_t mps_instance_bytes(int64_t N, int64_t Ne, int nen, int precision, bool per_elem, size_t nrec);
+// Batch internals used by the server (mps_batch.hip).
+int mps_batch_check(mps_batch b, const mpsh::HostPrep& hp, std::string* why);
+size_t mps_batch_footprint(mps_batch b, const mpsh::HostPrep& hp);
+void mps_batch_room(mps_batch b, int* free_slots, size_t* largest);
+int mps_batch_add_prepared(mps_batch b, const mpsh::HostPrep& hp, int32_t* slot_out);

@@ -229,15 +229,54 @@ static int wait_launches(mps_batch_s* b) {
     return MPS_OK;
 }
 
+// Checks that a prepared solve can join batch b (fixed steps, same precision / element / law).
+int mps_batch_check(mps_batch b, const HostPrep& hp, std::string* why) {
+    if (hp.cfg.n_steps <= 0) {
+        *why = "batched solves need a fixed step count (n_steps > 0)";
+        return MPS_ERR_ARG;
+    }
+    if (hp.cfg.precision != b->cfg.precision) {
+        *why = "precision differs from the batch";
+        return MPS_ERR_ARG;
+    }
+    if (hp.cfg.energy_every != 0) {
+        *why = "energy history is not kept in the batched form";
+        return MPS_ERR_ARG;
+    }
+    if (b->nen >= 0 && (hp.nen != b->nen || hp.law != b->law)) {
+        *why = "a batch holds one element type and one constitutive law";
+        return MPS_ERR_ARG;
+    }
+    return MPS_OK;
+}
+
+size_t mps_batch_footprint(mps_batch b, const HostPrep& hp) {
+    return mps_instance_bytes(hp.N, hp.Ne, hp.nen, b->cfg.precision, hp.per_elem, hp.bcval.size() / 3);
+}
+
+// Free slot and largest free arena block (admission by footprint).
+void mps_batch_room(mps_batch b, int* free_slots, size_t* largest) {
+    int n = 0;
+    for (auto& s : b->slots) n += !s.used;
+    *free_slots = n;
+    *largest = b->alloc.largest();
+}
+
 extern "C" int mps_batch_add(mps_batch b, const mps_mesh* mesh, const mps_material* mat, const mps_bcs* bcs,
                              const mps_solver_cfg* cfg, int32_t* slot_out) {
     if (!b || !slot_out) return MPS_ERR_ARG;
-    if (!cfg || cfg->n_steps <= 0) return b->fail(MPS_ERR_ARG, "batched solves need a fixed step count (n_steps > 0)");
-    if (cfg->precision != b->cfg.precision) return b->fail(MPS_ERR_ARG, "precision differs from the batch");
-    if (cfg->energy_every != 0) return b->fail(MPS_ERR_ARG, "energy history is not kept in the batched form");
     HostPrep hp;
     std::string e;
     int rc = prepare(mesh, mat, bcs, cfg, &hp, &e);
+    if (rc) return b->fail(rc, e);
+    return mps_batch_add_prepared(b, hp, slot_out);
+}
+
+// Admission of an already prepared solve: arena block, uploads, set-up kernels.
+int mps_batch_add_prepared(mps_batch b, const HostPrep& hp, int32_t* slot_out) {
+    const mps_solver_cfg* cfg = &hp.cfg;
+    std::string e;
+    int rc = mps_batch_check(b, hp, &e);
     if (rc) return b->fail(rc, e);
     if (b->nen >= 0 && (hp.nen != b->nen || hp.law != b->law))
         return b->fail(MPS_ERR_ARG, "a batch holds one element type and one constitutive law");

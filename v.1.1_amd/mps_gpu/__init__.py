@@ -55,6 +55,19 @@ class BatchCfg(C.Structure):
                 ("device", C.c_int32)]
 
 
+class ServerCfg(C.Structure):
+    _fields_ = [("batch", BatchCfg)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("submitted", C.c_int64), ("completed", C.c_int64), ("failed", C.c_int64), ("launches", C.c_int64),
+                ("p50_ms", C.c_double), ("p99_ms", C.c_double), ("max_ms", C.c_double), ("queue_p50_ms", C.c_double),
+                ("span_ms", C.c_double), ("solves_per_s", C.c_double), ("busy_ms", C.c_double)]
+
+
+REQ_QUEUED, REQ_RUNNING, REQ_DONE, REQ_FAILED = 0, 1, 2, 3
+
+
 class MpsError(RuntimeError):
     def __init__(self, code, msg):
         super().__init__(f"mps error {code}: {msg}")
@@ -104,6 +117,19 @@ def load(path=None):
         L.mps_batch_launch.argtypes = [B, C.c_int64]
         L.mps_batch_sync.argtypes = [B]
         L.mps_batch_info.argtypes = [B, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_int32)]
+    if hasattr(L, "mps_server_create"):
+        S = C.c_void_p
+        L.mps_server_create.argtypes = [C.POINTER(ServerCfg), C.POINTER(S)]
+        L.mps_server_submit.argtypes = [S, C.POINTER(Mesh), C.POINTER(Material), C.POINTER(Bcs),
+                                        C.POINTER(SolverCfg), C.POINTER(C.c_int64)]
+        L.mps_server_poll.argtypes = [S, C.c_int64, C.POINTER(C.c_int32), _dp, _dp]
+        L.mps_server_wait.argtypes = [S, C.c_int64, C.c_double, C.POINTER(C.c_int32)]
+        L.mps_server_drain.argtypes = [S, C.c_double]
+        L.mps_server_get_stats.argtypes = [S, C.POINTER(ServerStats)]
+        L.mps_server_reset_stats.argtypes = [S]
+        L.mps_server_destroy.argtypes = [S]
+        L.mps_server_last_error.restype = C.c_char_p
+        L.mps_server_last_error.argtypes = [S]
         L.mps_batch_last_error.restype = C.c_char_p
         L.mps_batch_last_error.argtypes = [B]
     _lib = L
@@ -323,6 +349,72 @@ class Batch:
         if self.b:
             self.L.mps_batch_destroy(self.b)
             self.b = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Server:
+    """Serving form (mps_server_*): continuous batching request loop on the device."""
+
+    def __init__(self, arena_bytes=4 << 30, max_instances=1024, precision=PREC_FP32_ELEM, steps_per_launch=32,
+                 elem_block=0, grid_mult=0, device=0):
+        self.L = load()
+        self.cfg = ServerCfg(batch=BatchCfg(arena_bytes=arena_bytes, max_instances=max_instances,
+                                            precision=precision, steps_per_launch=steps_per_launch,
+                                            elem_block=elem_block, grid_mult=grid_mult, device=device))
+        self.s = C.c_void_p()
+        rc = self.L.mps_server_create(C.byref(self.cfg), C.byref(self.s))
+        if rc != 0:
+            raise MpsError(rc, "mps_server_create failed")
+        self.precision = precision
+        self.nodes = {}
+
+    def submit(self, mesh, law=LAW_NH_COMPRESSIBLE, E=1e6, nu=0.3, rho=1000.0, dir_dof=None, dir_val=None,
+               bc_mode=BC_VELOCITY, vel_ramp_steps=100, scale=0.5, n_steps=1000):
+        inp = _Inputs(mesh, law, E, nu, rho, dir_dof, dir_val, bc_mode, vel_ramp_steps, 0.0, 1.0, scale, n_steps,
+                      self.precision, 0, 0, self.cfg.batch.device)
+        rid = C.c_int64()
+        rc = self.L.mps_server_submit(self.s, C.byref(inp.mesh), C.byref(inp.mat), C.byref(inp.bcs),
+                                      C.byref(inp.cfg), C.byref(rid))
+        if rc != 0:
+            raise MpsError(rc, self.L.mps_server_last_error(self.s).decode())
+        self.nodes[rid.value] = inp.n_nodes
+        return rid.value
+
+    def result(self, rid, timeout=600.0):
+        """Waits for request rid; returns (status, u, latency_ms, queue_ms)."""
+        st = C.c_int32()
+        self.L.mps_server_wait(self.s, rid, timeout, C.byref(st))
+        u = np.zeros(3 * self.nodes[rid])
+        lat = np.zeros(2)
+        rc = self.L.mps_server_poll(self.s, rid, C.byref(st), _d(u), _d(lat))
+        return rc, (u if st.value == REQ_DONE else None), lat[0], lat[1]
+
+    def drain(self, timeout=600.0):
+        return self.L.mps_server_drain(self.s, timeout)
+
+    def stats(self):
+        st = ServerStats()
+        self.L.mps_server_get_stats(self.s, C.byref(st))
+        return {k: getattr(st, k) for k, _ in ServerStats._fields_}
+
+    def reset_stats(self):
+        self.L.mps_server_reset_stats(self.s)
+
+    def close(self):
+        if self.s:
+            self.L.mps_server_destroy(self.s)
+            self.s = C.c_void_p()
 
     def __enter__(self):
         return self

@@ -91,6 +91,96 @@ def cpu_baseline(cfg, budget_s=12.0):
     return nt, out
 
 
+def plain_solve(c, precision):
+    """One solve alone through the single-solve path with plain kernel launches (no graph replay)."""
+    with G.Solver(c["mesh"], E=c["E"], nu=c["nu"], dir_dof=c["dir_dof"], dir_val=c["dir_val"],
+                  n_steps=c["n_steps"], precision=precision) as s:
+        s.set_graphs(False)
+        t = time.perf_counter()
+        rc, rep = s.run(c["n_steps"])
+        dt = time.perf_counter() - t
+        assert rc == 0 and rep["status"] == G.MPS_DONE, rep
+        return dt, s.get_state()[0]
+
+
+def batch_leg(precision, n_inst=512, steps=1000, plain_sample=48):
+    """512 independent cubes n^3, n ~ U{6..40}, 1000 steps each, in one arena; autotuned hipGraph replay,
+    against the same solves launched one at a time with plain launches (random sample)."""
+    sols = configs.batch_meshes(n_inst, steps=steps)
+    gp = sum(c["mesh"]["n_elem"] * 8 for c in sols) * steps
+    with G.Batch(arena_bytes=16 << 30, max_instances=n_inst, precision=precision, steps_per_launch=32) as b:
+        t = time.perf_counter()
+        slots = [b.add(c["mesh"], E=c["E"], nu=c["nu"], dir_dof=c["dir_dof"], dir_val=c["dir_val"],
+                       n_steps=steps) for c in sols]
+        b.sync()
+        t_admit = time.perf_counter() - t
+        bs, gm = b.autotune()
+        t0 = time.perf_counter()
+        b.step(steps)
+        wall = time.perf_counter() - t0
+        assert all(b.query(s_)["status"] == G.MPS_DONE for s_ in slots)
+        rng = np.random.default_rng(3)
+        pick = sorted(rng.choice(n_inst, plain_sample, replace=False))
+        plain_t, err = 0.0, 0.0
+        for i in pick:
+            dt, u = plain_solve(sols[i], precision)
+            plain_t += dt
+            ub = b.get_state(slots[i])[0]
+            assert np.array_equal(ub, u)  # same solve alone and in the batch: identical
+            if precision == G.PREC_FP32_ELEM and i == pick[0]:
+                _, u64 = plain_solve(sols[i], G.PREC_FP64)
+                err = float(np.abs(u - u64).max() / np.abs(u64).max())
+    return {"workload": f"{n_inst} clamped cubes n^3, n~U{{6..40}}, E log-U[1e5,1e7], nu U[0.2,0.45], "
+                        f"{steps} steps each, hipGraph replay of {32}-step launches",
+            "precision": "fp32-element" if precision == G.PREC_FP32_ELEM else "fp64",
+            "elements": sum(c["mesh"]["n_elem"] for c in sols), "wall_s": wall, "solves_per_s": n_inst / wall,
+            "elem_gp_updates_per_s": gp / wall, "autotuned": {"elem_block": bs, "grid_mult": gm},
+            "admission_s": t_admit,
+            "plain_one_at_a_time_solves_per_s": plain_sample / plain_t, "plain_sample": plain_sample,
+            "speedup_vs_plain": (n_inst / wall) / (plain_sample / plain_t),
+            "batch_equals_alone": True,
+            **({"max_rel_disp_err_vs_fp64": err} if precision == G.PREC_FP32_ELEM else {})}
+
+
+def serving_leg(lambdas, n_req=48, precision=G.PREC_FP32_ELEM, plain_sample=16):
+    """Poisson arrivals at rate lam of hex8 boxes (N_e log-U[1e3,2e5], aspect U[1,4], steps U{500..5000});
+    requests are admitted by device footprint into the running batch (continuous batching)."""
+    import threading
+    reqs = configs.serving_requests(n_req)
+    points = []
+    with G.Server(arena_bytes=16 << 30, max_instances=1024, precision=precision, steps_per_launch=32) as srv:
+        for lam in lambdas:
+            srv.reset_stats()
+            rng = np.random.default_rng(int(lam))
+            ids = []
+
+            def producer():
+                for c in reqs:
+                    ids.append(srv.submit(c["mesh"], E=c["E"], nu=c["nu"], dir_dof=c["dir_dof"],
+                                          dir_val=c["dir_val"], n_steps=c["n_steps"]))
+                    time.sleep(rng.exponential(1.0 / lam))
+            th = threading.Thread(target=producer)
+            th.start()
+            th.join()
+            assert srv.drain(600.0) == 0
+            st = srv.stats()
+            for rid in ids:
+                rc, u, _, _ = srv.result(rid)
+                assert rc == 0
+            points.append({"lambda_per_s": lam, "solves_per_s": st["solves_per_s"], "p50_ms": st["p50_ms"],
+                           "p99_ms": st["p99_ms"], "queue_p50_ms": st["queue_p50_ms"], "completed": st["completed"]})
+    rng = np.random.default_rng(5)
+    pick = sorted(rng.choice(n_req, plain_sample, replace=False))
+    plain_t = sum(plain_solve(reqs[i], precision)[0] for i in pick)
+    plain_rate = plain_sample / plain_t
+    best = max(p["solves_per_s"] for p in points)
+    return {"workload": f"{n_req} requests per rate: hex8 boxes N_e log-U[1e3,2e5], aspect U[1,4], steps U{{500..5000}}, "
+                        "Poisson arrivals",
+            "precision": "fp32-element" if precision == G.PREC_FP32_ELEM else "fp64",
+            "points": points, "plain_one_at_a_time_solves_per_s": plain_rate, "plain_sample": plain_sample,
+            "speedup_vs_plain": best / plain_rate}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -98,6 +188,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-fp32", action="store_true")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched (512 solves) leg")
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving (Poisson arrivals) leg")
+    ap.add_argument("--lambdas", default="50,100,200,500,1000,2000")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only; --gpus {args.gpus} is not supported "
@@ -120,7 +213,7 @@ def main():
     eb = hex_elem_kernel_bytes(ne, nn, G.PREC_FP64, False)
     achieved = eb / (elem_avg_ms * 1e-3) / 1e9
     out = {
-        "metric": "elem-GP updates/s (1M hex8 NH explicit)",
+        "metric": "elem-GP updates/s (1M hex8 NH explicit) + batched solves/s, p99 latency",
         "value": value,
         "unit": "elem-GP updates/s",
         "n_gpus": 1,
@@ -155,6 +248,10 @@ def main():
                             "speedup_vs_fp64": v32 / value, "max_rel_disp_err_vs_fp64": err,
                             "element_avg_ms": e32 / max(n32, 1),
                             "node_avg_ms": r32["kt"]["node"][0] / max(r32["kt"]["node"][1], 1)}
+    if not args.no_batch:
+        out["batched"] = batch_leg(G.PREC_FP32_ELEM)
+    if not args.no_serve:
+        out["serving"] = serving_leg([float(x) for x in args.lambdas.split(",")])
     if not args.no_cpu:
         nt, cb = cpu_baseline(cfg)
         out["cpu_baseline"] = {"value": cb[nt]["value"], "unit": "elem-GP updates/s", "cores": nt, "kind": "port",

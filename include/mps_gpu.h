@@ -148,6 +148,11 @@ int mps_critical_dt(mps_handle h, const double *u, double *dt);
  * solver destructor (solverExplicit.cpp:360; main.cpp:183-185). */
 int mps_destroy(mps_handle h);
 
+/* Graph replay of step chunks (default on): when enabled and no energies or kernel timing
+ * are requested, mps_run replays each chunk of steps from a captured hipGraph instead of
+ * launching its kernels one by one.  No reference counterpart. */
+int mps_set_graphs(mps_handle h, int32_t enable);
+
 /* Kernel timing: when enabled, mps_run records HIP events around every kernel
  * launch on the solver's stream.  mps_kernel_times returns, per kernel kind
  * [0 element, 1 node, 2 begin, 3 energy], the summed device milliseconds and the

@@ -216,8 +216,8 @@ struct mps_solver_s {
     bool primed = false;       // prologue (first dt) done
     long long next_step = 1;   // host mirror of ctl->A
     bool step_begun = false;   // step next_step already predicted on the device
-    hipGraphExec_t graph = nullptr;
-    int graph_steps = 0;
+    std::vector<std::pair<long long, hipGraphExec_t>> graphs;  // captured chunks by step count
+    bool use_graphs = true;
     // optional kernel timing (HIP events around each element / node launch)
     bool profiling = false;
     std::vector<hipEvent_t> evpool;
@@ -254,7 +254,8 @@ static void release(mps_solver_s* h) {
     for (hipEvent_t e : h->evpool) (void)hipEventDestroy(e);
     h->evpool.clear();
     if (h->device >= 0) (void)hipSetDevice(h->device);
-    if (h->graph) (void)hipGraphExecDestroy(h->graph);
+    for (auto& g : h->graphs) (void)hipGraphExecDestroy(g.second);
+    h->graphs.clear();
     if (h->stream) (void)hipStreamSynchronize(h->stream);
     for (void* p : h->allocs) (void)hipFree(p);
     h->allocs.clear();
@@ -588,7 +589,33 @@ extern "C" int mps_run(mps_handle h, int64_t max_steps, mps_report* rep) {
     while (remaining > 0) {
         long long n = std::min(remaining, chunk);
         std::vector<int> kinds;
-        hipError_t e = enqueue_steps(h, n, &kinds);
+        hipError_t e = hipSuccess;
+        if (h->use_graphs && !h->profiling && h->cfg.energy_every == 0 && !h->step_begun) {
+            // every chunk of n steps is the same launch sequence: replay it from a captured graph
+            hipGraphExec_t ge = nullptr;
+            for (auto& g : h->graphs)
+                if (g.first == n) ge = g.second;
+            if (!ge) {
+                hipGraph_t gr = nullptr;
+                const long long k0 = h->next_step;
+                e = hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal);
+                if (e == hipSuccess) e = enqueue_steps(h, n, &kinds);
+                hipError_t e2 = hipStreamEndCapture(h->stream, &gr);
+                if (e == hipSuccess) e = e2;
+                if (e == hipSuccess) e = hipGraphInstantiate(&ge, gr, nullptr, nullptr, 0);
+                if (gr) (void)hipGraphDestroy(gr);
+                h->next_step = k0;  // capture does not execute; replay below advances the host mirror
+                h->step_begun = false;
+                if (e == hipSuccess) h->graphs.emplace_back(n, ge);
+            }
+            if (e == hipSuccess) e = hipGraphLaunch(ge, h->stream);
+            if (e == hipSuccess) {
+                h->next_step += n;
+                h->step_begun = false;
+            }
+        } else {
+            e = enqueue_steps(h, n, &kinds);
+        }
         if (e != hipSuccess) return h->fail(MPS_ERR_HIP, fmt("step launch: %s", hipGetErrorString(e)));
         remaining -= n;
         HCHK(h, hipEventRecord(h->ev1, h->stream));
@@ -661,6 +688,12 @@ extern "C" int mps_critical_dt(mps_handle h, const double* u, double* dt) {
     double m;
     memcpy(&m, &bits, 8);
     *dt = m * h->I.scale;
+    return MPS_OK;
+}
+
+extern "C" int mps_set_graphs(mps_handle h, int32_t enable) {
+    if (!h) return MPS_ERR_ARG;
+    h->use_graphs = enable != 0;
     return MPS_OK;
 }
 

@@ -102,6 +102,7 @@ def load(path=None):
     L.mps_device_info.argtypes = [C.c_int32, C.c_char_p, C.c_int32]
     L.mps_bandwidth_probe.argtypes = [C.c_int32, C.c_int64, C.c_int32, _dp]
     L.mps_profile.argtypes = [H, C.c_int32]
+    L.mps_set_graphs.argtypes = [H, C.c_int32]
     L.mps_kernel_times.argtypes = [H, _dp, C.POINTER(C.c_int64)]
     if hasattr(L, "mps_batch_create"):
         B = C.c_void_p
@@ -251,6 +252,9 @@ class Solver:
             u = np.ascontiguousarray(u, dtype=np.float64)
         self._chk(self.L.mps_critical_dt(self.h, _d(u) if u is not None else None, C.byref(out)))
         return out.value
+
+    def set_graphs(self, enable=True):
+        self._chk(self.L.mps_set_graphs(self.h, int(enable)))
 
     def profile(self, enable=True):
         self._chk(self.L.mps_profile(self.h, int(enable)))

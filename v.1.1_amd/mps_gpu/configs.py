@@ -138,3 +138,36 @@ def batch_instances(count=512, seed=SEED_CFG3, n_lo=6, n_hi=40, steps=1000):
     E = 10.0 ** (5.0 + 2.0 * uniform01(seed, 3 * idx + 1))
     nu = 0.2 + 0.25 * uniform01(seed, 3 * idx + 2)
     return [dict(n=int(s), E=float(e), nu=float(v), steps=steps) for s, e, v in zip(sides, E, nu)]
+
+
+def serving_requests(count, seed=SEED_CFG4, ne_lo=1e3, ne_hi=2e5, steps_lo=500, steps_hi=5000):
+    """Requests of the serving workload: hex8 boxes with N_e log-uniform in [ne_lo, ne_hi], aspect ratio
+    (long / short side) uniform in [1, 4] along z, element size 1/n_z (longest side 1 m), material E
+    log-uniform [1e5, 1e7] Pa, nu uniform [0.2, 0.45], rho 1000, bottom clamped, top pulled along z at
+    0.1 m/s (ramp over 100 steps), step count uniform in {steps_lo..steps_hi}."""
+    idx = np.arange(count, dtype=np.int64)
+    u = [uniform01(seed, 6 * idx + j) for j in range(6)]
+    ne = 10.0 ** (np.log10(ne_lo) + (np.log10(ne_hi) - np.log10(ne_lo)) * u[0])
+    asp = 1.0 + 3.0 * u[1]
+    out = []
+    for i in range(count):
+        s = (ne[i] / asp[i]) ** (1.0 / 3.0)
+        nx = max(1, int(round(s)))
+        nz = max(1, int(round(asp[i] * s)))
+        h = 1.0 / nz
+        m = box(nx, nx, nz, lx=nx * h, ly=nx * h, lz=1.0)
+        dof, val = clamp_bottom_pull_top(m, vel=0.1)
+        out.append(dict(mesh=m, E=float(10.0 ** (5.0 + 2.0 * u[2][i])), nu=float(0.2 + 0.25 * u[3][i]),
+                        dir_dof=dof, dir_val=val,
+                        n_steps=int(steps_lo + int(u[4][i] * (steps_hi - steps_lo + 1)))))
+    return out
+
+
+def batch_meshes(count=512, seed=SEED_CFG3, steps=1000):
+    """Solves of the batched workload (unit cubes n^3, see batch_instances), ready for submission."""
+    out = []
+    for spec in batch_instances(count, seed, steps=steps):
+        m = box(spec["n"], spec["n"], spec["n"])
+        dof, val = clamp_bottom_pull_top(m, vel=0.1)
+        out.append(dict(mesh=m, E=spec["E"], nu=spec["nu"], dir_dof=dof, dir_val=val, n_steps=steps))
+    return out

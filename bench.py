@@ -28,6 +28,22 @@ from mps_gpu import configs  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
+TRAFFIC_TABLE = os.path.join(ROOT, "profiles", "hex8_cube100_hbm_traffic.md")
+
+
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary
+    (copy-kernel-calibrated, one row per kernel: | `name` | fetch KiB | write KiB | fetched MB | written MB |
+    total MB | ...).  None when the summary or the row is absent."""
+    try:
+        with open(TRAFFIC_TABLE) as f:
+            for line in f:
+                cells = [c.strip() for c in line.strip().strip("|").split("|")]
+                if len(cells) >= 6 and cells[0] == f"`{kernel}`":
+                    return float(cells[5].replace(",", "")) * 1e6
+    except OSError:
+        pass
+    return None
 
 
 def hex_elem_kernel_bytes(ne, nn, prec, per_elem_mat):
@@ -230,7 +246,8 @@ def main():
                                "dt=0.5 x stable dt, FP64",
                    "n_elem": ne, "n_nodes": nn, "gauss_points": 8, "form": "fp64"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic("k_elem<double,8,0>"),
+                     "traffic_source": os.path.relpath(TRAFFIC_TABLE, ROOT),
                      "kernel": "k_elem<double,8,0> (hex8 element forces + stable dt)",
                      "kernel_avg_ms": elem_avg_ms, "algorithmic_bytes_per_launch": eb,
                      "measured_copy_ceiling_gbs": copy_gbs},

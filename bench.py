@@ -46,6 +46,11 @@ def measured_traffic(kernel):
     return None
 
 
+def note(msg):
+    """Progress line on stderr (the JSON result is the only stdout line)."""
+    print(f"bench: {msg}", file=sys.stderr, flush=True)
+
+
 def hex_elem_kernel_bytes(ne, nn, prec, per_elem_mat):
     """Algorithmic HBM bytes of one element-kernel launch (hex8): conn 8x4 B + l0 8 B (+ material
     24 B when per element) + element forces 24 x p_e, per element; X and u 3x8 B each, per node."""
@@ -185,6 +190,8 @@ def serving_leg(lambdas, n_req=48, precision=G.PREC_FP32_ELEM, plain_sample=16):
                 assert rc == 0
             points.append({"lambda_per_s": lam, "solves_per_s": st["solves_per_s"], "p50_ms": st["p50_ms"],
                            "p99_ms": st["p99_ms"], "queue_p50_ms": st["queue_p50_ms"], "completed": st["completed"]})
+            note(f"serving lambda={lam:g}/s: {st['solves_per_s']:.1f} solves/s, p50 {st['p50_ms']:.0f} ms, "
+                 f"p99 {st['p99_ms']:.0f} ms")
     rng = np.random.default_rng(5)
     pick = sorted(rng.choice(n_req, plain_sample, replace=False))
     plain_t = sum(plain_solve(reqs[i], precision)[0] for i in pick)
@@ -219,8 +226,11 @@ def main():
     ne, nn = m["n_elem"], m["n_nodes"]
     gp_per_step = ne * 8
 
+    note(f"workload {ne} hex8 elements, {nn} nodes; copy probe")
     copy_gbs = G.bandwidth_probe(1 << 30, 20)
+    note(f"copy ceiling {copy_gbs:.0f} GB/s; FP64 leg")
     r64 = run_gpu(cfg, G.PREC_FP64, args.warmup, args.steps)
+    note(f"FP64 leg {r64['wall_s'] * 1e3 / args.steps:.4f} ms/step")
     value = gp_per_step * args.steps / r64["wall_s"]
     ems, en = r64["kt"]["element"]
     nms, nn_l = r64["kt"]["node"]
@@ -257,6 +267,7 @@ def main():
                     / 1e9 / HBM_PEAK_GBS},
     }
     if not args.no_fp32:
+        note("FP32-element leg")
         r32 = run_gpu(cfg, G.PREC_FP32_ELEM, args.warmup, args.steps)
         v32 = gp_per_step * args.steps / r32["wall_s"]
         err = float(np.abs(r32["u"] - r64["u"]).max() / np.abs(r64["u"]).max())
@@ -266,10 +277,14 @@ def main():
                             "element_avg_ms": e32 / max(n32, 1),
                             "node_avg_ms": r32["kt"]["node"][0] / max(r32["kt"]["node"][1], 1)}
     if not args.no_batch:
+        note("batched leg (512 solves)")
         out["batched"] = batch_leg(G.PREC_FP32_ELEM)
+        note(f"batched {out['batched'].get('solves_per_s', 0):.1f} solves/s")
     if not args.no_serve:
+        note("serving leg")
         out["serving"] = serving_leg([float(x) for x in args.lambdas.split(",")])
     if not args.no_cpu:
+        note("CPU baseline (oracle, bounded sample)")
         nt, cb = cpu_baseline(cfg)
         out["cpu_baseline"] = {"value": cb[nt]["value"], "unit": "elem-GP updates/s", "cores": nt, "kind": "port",
                                "sample": f"same 1M-hex8 workload, {cb[nt]['steps']} steps after a 1-step set-up "

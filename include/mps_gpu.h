@@ -117,6 +117,13 @@ int mps_set_state(mps_handle h, const double *u, const double *v, const double *
  * Replaces: solvers::getFinalValues (solvers.h:171-187). */
 int mps_get_state(mps_handle h, double *u, double *v, double *a);
 
+/* Gather n entries of u (which=0), v (1) or a (2) at DOF indices
+ * dof[i] = node + n_nodes*dir into out[i].
+ * Replaces: the nodal value look-up of classExtractData::writeDataFromSolution
+ * (dataExtraction.cpp:784-808), done on the device so that per-step nodal
+ * extraction does not copy the whole field. */
+int mps_get_dofs(mps_handle h, int32_t which, int64_t n, const int32_t *dof, double *out);
+
 /* Advance up to max_steps explicit steps (fewer if the time window or the
  * fixed step count ends, or on error).  rep may be NULL.
  * Replaces: the time loop of classSolverExplicit::calculate

@@ -1,0 +1,127 @@
+"""Host driver end to end on the device: .inp in the reference grammar -> mps_explicit -> final
+displacement and CSV extractions, against the CPU oracle on the same problem."""
+import csv
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import inpgen
+import oracle_ctypes as O
+from mps_gpu import configs
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+EXE = os.path.join(ROOT, "v.1.1_amd", "lib", "mps_explicit")
+
+
+def run(inp, outdir, *extra):
+    dump = os.path.join(outdir, "u.txt")
+    r = subprocess.run([EXE, "-input", str(inp), "-outputDir", str(outdir), "-dumpU", dump, *extra],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    return np.loadtxt(dump), r.stdout
+
+
+def read_csv(path):
+    with open(path) as f:
+        rows = list(csv.reader(f))
+    return rows[0], np.array([[float(x) for x in r] for r in rows[1:]])
+
+
+def g16(x):
+    """Values as the CSV writer prints them (%.16g, dataExtraction.cpp:289-300)."""
+    return np.array([float(f"{v:.16g}") for v in np.ravel(x)])
+
+
+def rel(a, b):
+    return float(np.abs(a - b).max() / np.abs(b).max())
+
+
+def test_reference_ramp_time_window_with_extractions(tmp_path):
+    m = configs.perturb(configs.box(4, 4, 4), 0.1, 11)
+    N = m["n_nodes"]
+    dof, val = configs.clamp_bottom_pull_top(m, vel=0.002)  # top: total ramp displacement 2 mm
+    bc = inpgen.bc_triples(dof, val, N)
+    probe = O.OracleProblem(m, dir_dof=dof, dir_val=val, bc_mode=O.BC_RAMP, t0=0.0, tf=1.0, scale=0.5)
+    tf = 60.5 * probe.critical_dt()  # about 60 steps of the time-window loop
+    top = [int(n) for n in np.arange(N)[-5:]]
+    bottom = list(range(5))
+    inp = tmp_path / "cube.inp"
+    inpgen.write_inp(inp, m, [dict(rho=1000.0, E=1e6, nu=0.3)],
+                     [dict(scale=0.5, t0=0.0, tf=tf, outputs=3, bc=bc)],
+                     node_extractions=[("Top", "Unknown", "Rough", 2, top),
+                                       ("Bot", "InternalForce", "Sum", 2, bottom)],
+                     element_extractions=[("E", "DEFO_ENERGY", "Sum"), ("K", "KIN_ENERGY", "Sum"),
+                                          ("V", "VOLUME", "Sum")])
+    u, log = run(inp, tmp_path)
+    assert "Solver used is EXPLICIT" in log and "done solving" in log
+    o = O.OracleProblem(m, dir_dof=dof, dir_val=val, bc_mode=O.BC_RAMP, t0=0.0, tf=tf, scale=0.5,
+                        energy_every=1)
+    r = o.run(hist_cap=4096)
+    steps = r["steps"]
+    assert steps > 50
+    assert rel(u, r["u"]) <= 1e-12
+    # energies and volume: one row per step, same times (bitwise: dt is computed in strict FP64)
+    h = r["hist"]
+    for name, key in (("All_DEFO_ENERGY_Sum", "e_def"), ("All_KIN_ENERGY_Sum", "e_kin"), ("All_VOLUME_Sum", "volume")):
+        head, rows = read_csv(tmp_path / f"{name}.csv")
+        assert head == ["Time", "Value"] and rows.shape == (steps, 2)
+        assert np.array_equal(rows[:, 0], g16(h["time"][:steps]))
+        assert np.allclose(rows[:, 1], h[key][:steps], rtol=1e-11, atol=1e-12 * np.abs(h[key][:steps]).max())
+    head, rows = read_csv(tmp_path / "Top_Unknown2_Rough.csv")
+    assert head == ["Time"] + [f"Node{n + 1}Comp2" for n in top] and rows.shape == (steps, 6)
+    assert np.array_equal(rows[-1, 1:], g16(u[np.array(top) + 2 * N]))
+    assert np.abs(rows[-1, 1:] - r["u"][np.array(top) + 2 * N]).max() <= 1e-12 * np.abs(r["u"]).max()
+    _, rows = read_csv(tmp_path / "Bot_InternalForce2_Sum.csv")
+    f = o.internal_force(r["u"])
+    ref = sum(f[n + 2 * N] for n in bottom)
+    assert abs(rows[-1, 1] - ref) <= 1e-10 * np.abs(f).max()
+
+
+def test_chained_solvers_hand_over_state(tmp_path):
+    """Two *SOLVER, EXPLICIT blocks: the second starts at the first one's tf from its u, v, a
+    (initValues / getFinalValues); its first dt uses u = 0 (t0 > 0), as the reference does."""
+    m = configs.box(3, 3, 4)
+    N = m["n_nodes"]
+    dof, val = configs.clamp_bottom_pull_top(m, vel=0.001)
+    bc = inpgen.bc_triples(dof, val, N)
+    dt0 = O.OracleProblem(m, dir_dof=dof, dir_val=val, bc_mode=O.BC_RAMP, tf=1.0, scale=0.5).critical_dt()
+    t1, t2 = 20.5 * dt0, 45.5 * dt0
+    inp = tmp_path / "chain.inp"
+    inpgen.write_inp(inp, m, [dict(rho=1000.0, E=1e6, nu=0.3)],
+                     [dict(scale=0.5, t0=0.0, tf=t1, outputs=1, bc=bc),
+                      dict(scale=0.4, t0=t1, tf=t2, outputs=1, bc=bc)])
+    u, _ = run(inp, tmp_path)
+    a = O.OracleProblem(m, dir_dof=dof, dir_val=val, bc_mode=O.BC_RAMP, t0=0.0, tf=t1, scale=0.5).run()
+    b = O.OracleProblem(m, dir_dof=dof, dir_val=val, bc_mode=O.BC_RAMP, t0=t1, tf=t2, scale=0.4).run(
+        u0=a["u"], v0=a["v"], a0=a["a"])
+    assert rel(u, b["u"]) <= 1e-12
+
+
+def test_fixed_steps_velocity_fp32_and_materials(tmp_path):
+    """Extensions: fixed step count, velocity BC ramp, FP32 element form; two materials by element list."""
+    m = configs.perturb(configs.box(4, 3, 3), 0.1, 5)
+    N = m["n_nodes"]
+    dof, val = configs.clamp_bottom_pull_top(m, vel=0.05)
+    bc = inpgen.bc_triples(dof, val, N)
+    hard = list(range(0, m["n_elem"], 3))
+    inp = tmp_path / "ext.inp"
+    inpgen.write_inp(inp, m, [dict(name="A", rho=1000.0, E=1e6, nu=0.3),
+                              dict(name="B", rho=1100.0, E=4e6, nu=0.35, elems=hard)],
+                     [dict(scale=0.5, outputs=1, bc=bc, bc_type="VELOCITY",
+                           extra_lines=["*MPS Fixed Steps", "80", "*MPS Velocity Ramp Steps", "20",
+                                        "*MPS Precision", "FP32_ELEM"])])
+    u32, _ = run(inp, tmp_path)
+    u64, _ = run(inp, tmp_path, "-precision", "FP64")
+    E = np.full(m["n_elem"], 1e6)
+    nu = np.full(m["n_elem"], 0.3)
+    rho = np.full(m["n_elem"], 1000.0)
+    E[hard], nu[hard], rho[hard] = 4e6, 0.35, 1100.0
+    r = O.OracleProblem(m, E=E, nu=nu, rho=rho, dir_dof=dof, dir_val=val, bc_mode=O.BC_VELOCITY,
+                        vel_ramp_steps=20, n_steps=80, scale=0.5).run()
+    assert r["steps"] == 80
+    assert rel(u64, r["u"]) <= 1e-12
+    assert rel(u32, r["u"]) <= 1e-6

@@ -1,0 +1,144 @@
+"""Host driver (mps_explicit) on the CPU: the .inp reader's view of a model (-check prints it as JSON,
+no device call) and the reference's input errors, each of which the reader turns into a message."""
+import json
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import inpgen
+from mps_gpu import configs
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+EXE = os.path.join(ROOT, "v.1.1_amd", "lib", "mps_explicit")
+
+pytestmark = pytest.mark.skipif(not os.path.exists(EXE), reason="mps_explicit not built (run __graft_entry__.build())")
+
+
+def check(path):
+    r = subprocess.run([EXE, "-check", "-input", str(path)], capture_output=True, text=True, timeout=60)
+    return r.returncode, (json.loads(r.stdout) if r.returncode == 0 else None), r.stderr
+
+
+def cube(n=3):
+    m = configs.box(n, n, n)
+    dof, val = configs.clamp_bottom_pull_top(m, vel=0.01)
+    return m, inpgen.bc_triples(dof, val, m["n_nodes"])
+
+
+def test_reads_model_solvers_materials_extractions(tmp_path):
+    m, bc = cube(3)
+    p = tmp_path / "a.inp"
+    half = list(range(0, m["n_elem"], 2))
+    inpgen.write_inp(p, m,
+                     [dict(name="Soft", rho=1000.0, E=1e6, nu=0.3),
+                      dict(name="Hard", rho=1200.0, E=5e6, nu=0.25, elems=half)],
+                     [dict(scale=0.5, t0=0.0, tf=0.01, outputs=4, bc=bc),
+                      dict(scale=0.25, t0=0.01, tf=0.02, outputs=2, bc=bc, extra_lines=["*MPS Fixed Steps", "7"])],
+                     node_extractions=[("U", "Unknown", "Rough", 2, [62, 63]), ("F", "InternalForce", "Sum", 2, [0])],
+                     element_extractions=[("E", "DEFO_ENERGY", "Sum"), ("K", "KIN_ENERGY", "Sum")])
+    rc, j, err = check(p)
+    assert rc == 0, err
+    assert j["nodes"] == 64 and j["elements"] == 27 and j["type"] == "C3D8"
+    assert [x["name"] for x in j["materials"]] == ["Soft", "Hard"]
+    assert j["materials"][1] == dict(name="Hard", law="3D-Compressible-Neo-Hookean", rho=1200.0, E=5e6, nu=0.25)
+    s0, s1 = j["solvers"]
+    assert (s0["scale"], s0["t0"], s0["tf"], s0["outputs"], s0["dirichlet"]) == (0.5, 0.0, 0.01, 4, len(bc))
+    assert s0["bc_mode"] == 0 and s0["fixed_steps"] == 0
+    assert (s1["scale"], s1["t0"], s1["fixed_steps"]) == (0.25, 0.01, 7)
+    assert j["node_extractions"] == ["F_InternalForce2_Sum", "U_Unknown2_Rough"]
+    assert j["element_extractions"] == ["All_DEFO_ENERGY_Sum", "All_KIN_ENERGY_Sum"]
+
+
+def test_carry_over_between_solver_blocks(tmp_path):
+    """Scale factor, time window, *Outputs and the instantaneous flag are function-scope variables of
+    solverManagement (inputFile.cpp:2253-2830): a later block that does not set them inherits them."""
+    m, bc = cube(2)
+    p = tmp_path / "b.inp"
+    inpgen.write_inp(p, m, [dict(rho=1000.0, E=1e6, nu=0.3)],
+                     [dict(scale=0.3, t0=0.0, tf=0.01, outputs=5, bc=bc, bc_type="DISPLACEMENT INST"),
+                      dict(scale=0.3, t0=0.01, tf=0.02, outputs=None, bc=bc)])
+    rc, j, err = check(p)
+    assert rc == 0, err
+    s0, s1 = j["solvers"]
+    assert s0["bc_mode"] == 1 and s1["bc_mode"] == 1  # INST persists into the RAMP block
+    assert s1["outputs"] == 5
+
+
+def test_velocity_extension_and_precision(tmp_path):
+    m, bc = cube(2)
+    p = tmp_path / "c.inp"
+    inpgen.write_inp(p, m, [dict(rho=1000.0, E=1e6, nu=0.3, law=1)],
+                     [dict(scale=0.5, outputs=1, bc=bc, bc_type="VELOCITY",
+                           extra_lines=["*MPS Fixed Steps", "100", "*MPS Velocity Ramp Steps", "10",
+                                        "*MPS Precision", "FP32_ELEM"])])
+    rc, j, err = check(p)
+    assert rc == 0, err
+    s = j["solvers"][0]
+    assert (s["bc_mode"], s["fixed_steps"], s["vel_ramp_steps"], s["precision"]) == (2, 100, 10, 1)
+    assert j["materials"][0]["law"] == "Neo-Hookean"
+
+
+@pytest.mark.parametrize("case,needle", [
+    ("bc_before_time", "*Time must precede *BOUNDARY"),
+    ("no_outputs", "*Outputs is mandatory"),
+    ("duplicate_dof", "duplicate Dirichlet DOF"),
+    ("no_dirichlet", "at least one mechanical Dirichlet DOF"),
+    ("implicit", "is not on this path"),
+    ("stvk", "has no device descriptor"),
+    ("neumann", "Neumann loads"),
+    ("unassigned_element", "has no material"),
+    ("bad_direction", "direction"),
+    ("cps3", "element type"),
+])
+def test_input_errors(tmp_path, case, needle):
+    m, bc = cube(2)
+    mats = [dict(rho=1000.0, E=1e6, nu=0.3)]
+    sol = dict(scale=0.5, t0=0.0, tf=0.01, outputs=1, bc=bc)
+    extra = ""
+    if case == "bc_before_time":
+        sol = dict(scale=0.5, outputs=1, bc=bc, extra_lines=[])
+        sol.pop("t0", None)
+    elif case == "no_outputs":
+        sol["outputs"] = None
+    elif case == "duplicate_dof":
+        sol["bc"] = bc + [bc[0]]
+    elif case == "no_dirichlet":
+        sol["bc"] = []
+    elif case == "unassigned_element":
+        mats = [dict(rho=1000.0, E=1e6, nu=0.3, elems=[0, 1])]
+    elif case == "bad_direction":
+        sol["bc"] = [(0, 3, 0.0)]
+    elif case == "neumann":
+        extra = "*BOUNDARY, TYPE=PRESSURE INST\n0,2,1e3,2\n4"
+    p = tmp_path / f"{case}.inp"
+    inpgen.write_inp(p, m, mats, [sol], extra=extra)
+    text = p.read_text()
+    if case == "bc_before_time":
+        text = text.replace("*END SOLVER", "*Time\n0, 0.01\n*END SOLVER")
+    if case == "implicit":
+        text = text.replace("*SOLVER, EXPLICIT", "*SOLVER, IMPLICIT STATIC")
+    if case == "stvk":
+        text = text.replace("3D-Compressible-Neo-Hookean", "St-Venant-Kirchhoff")
+    if case == "cps3":
+        text = text.replace("type=C3D8", "type=CPS3")
+    p.write_text(text)
+    rc, _, err = check(p)
+    assert rc != 0
+    assert needle in err, err
+
+
+def test_coordinates_round_trip(tmp_path):
+    """The reader gets every coordinate bit for bit (written with %.17g) and every connectivity entry."""
+    m = configs.perturb(configs.box(6, 5, 4), 0.1, 7)
+    dof, val = configs.clamp_bottom_pull_top(m, vel=0.01)
+    p = tmp_path / "d.inp"
+    inpgen.write_inp(p, m, [dict(rho=1000.0, E=1e6, nu=0.3)],
+                     [dict(scale=0.5, t0=0.0, tf=0.01, outputs=1, bc=inpgen.bc_triples(dof, val, m["n_nodes"]))])
+    rc, j, err = check(p)
+    assert rc == 0, err
+    assert j["nodes"] == m["n_nodes"] and j["elements"] == m["n_elem"]
+    xor = int(np.bitwise_xor.reduce(np.ascontiguousarray(m["X"]).view(np.uint64)))
+    assert int(j["x_xor"], 16) == xor
+    assert j["conn_sum"] == int(m["conn"].astype(np.int64).sum())

@@ -212,6 +212,9 @@ struct mps_solver_s {
     double* scratch_u = nullptr;
     double* scratch_f = nullptr;
     unsigned long long* scratch_dt = nullptr;
+    int32_t* pick_idx = nullptr;  // mps_get_dofs buffers (grow-only, freed with the handle)
+    double* pick_out = nullptr;
+    int64_t pick_cap = 0;
     int energy_nb = 0;
     bool primed = false;       // prologue (first dt) done
     long long next_step = 1;   // host mirror of ctl->A
@@ -259,6 +262,10 @@ static void release(mps_solver_s* h) {
     if (h->stream) (void)hipStreamSynchronize(h->stream);
     for (void* p : h->allocs) (void)hipFree(p);
     h->allocs.clear();
+    if (h->pick_idx) (void)hipFree(h->pick_idx);
+    if (h->pick_out) (void)hipFree(h->pick_out);
+    h->pick_idx = nullptr;
+    h->pick_out = nullptr;
     if (h->ev0) (void)hipEventDestroy(h->ev0);
     if (h->ev1) (void)hipEventDestroy(h->ev1);
     if (h->stream) (void)hipStreamDestroy(h->stream);
@@ -451,6 +458,39 @@ extern "C" int mps_get_state(mps_handle h, double* u, double* v, double* a) {
     if (u) HCHK(h, hipMemcpy(u, h->I.u, bytes, hipMemcpyDeviceToHost));
     if (v) HCHK(h, hipMemcpy(v, h->I.v, bytes, hipMemcpyDeviceToHost));
     if (a) HCHK(h, hipMemcpy(a, h->I.a, bytes, hipMemcpyDeviceToHost));
+    return MPS_OK;
+}
+
+__global__ void k_pick(const double* __restrict__ src, const int32_t* __restrict__ idx, int64_t n,
+                       double* __restrict__ out) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = src[idx[i]];
+}
+
+extern "C" int mps_get_dofs(mps_handle h, int32_t which, int64_t n, const int32_t* dof, double* out) {
+    if (!h || which < 0 || which > 2 || n < 0 || (n > 0 && (!dof || !out))) return MPS_ERR_ARG;
+    if (n == 0) return MPS_OK;
+    const int64_t ndof = 3 * (int64_t)h->I.N;
+    for (int64_t i = 0; i < n; i++)
+        if (dof[i] < 0 || dof[i] >= ndof) return h->fail(MPS_ERR_ARG, fmt("mps_get_dofs: dof %d outside [0, %lld)", dof[i], (long long)ndof));
+    (void)hipSetDevice(h->device);
+    if (n > h->pick_cap) {
+        HCHK(h, hipStreamSynchronize(h->stream));
+        if (h->pick_idx) (void)hipFree(h->pick_idx);
+        if (h->pick_out) (void)hipFree(h->pick_out);
+        h->pick_idx = nullptr;
+        h->pick_out = nullptr;
+        h->pick_cap = 0;
+        HCHK(h, hipMalloc(&h->pick_idx, (size_t)n * 4));
+        HCHK(h, hipMalloc(&h->pick_out, (size_t)n * 8));
+        h->pick_cap = n;
+    }
+    const double* src = which == 0 ? h->I.u : (which == 1 ? h->I.v : h->I.a);
+    HCHK(h, hipMemcpyAsync(h->pick_idx, dof, (size_t)n * 4, hipMemcpyHostToDevice, h->stream));
+    hipLaunchKernelGGL(k_pick, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, h->stream, src, h->pick_idx, n, h->pick_out);
+    HCHK(h, hipGetLastError());
+    HCHK(h, hipMemcpyAsync(out, h->pick_out, (size_t)n * 8, hipMemcpyDeviceToHost, h->stream));
+    HCHK(h, hipStreamSynchronize(h->stream));
     return MPS_OK;
 }
 

@@ -1,0 +1,89 @@
+// extraction.h -- CSV time series of the explicit path.
+//
+// Mirrors classExtractData / classNodeData / classElementData
+// (dataExtraction.cpp:150-505, 569-912): one CSV file per extraction in the
+// output directory, header "Time,Value" (or "Time,Node<i>Comp<c>,..." for the
+// Rough node operation), one row "%.16g,%.16g" per step whose time is larger
+// than the last written time.  File names:
+//   node:    <prefix>_<Unknown|InternalForce|ExternalForce><comp>_<operation>.csv
+//   element: All_<type>_<operation>.csv for "All", else <prefix>_<type>_<operation>.csv
+// The first extraction of a given name wins (addClassNodeDataToSave, :469-487).
+//
+// Values come from the device: nodal values through mps_get_dofs /
+// mps_internal_force, element sums (DEFO_ENERGY, KIN_ENERGY, VOLUME with the
+// Sum operation over All elements) from the per-step history ring.
+#ifndef MPS_EXTRACTION_H
+#define MPS_EXTRACTION_H
+
+#include <cstdio>
+#include <map>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "inp_reader.h"
+
+namespace mps {
+
+class classDataBase {
+  public:
+    classDataBase(std::string name, std::string op) : _name(std::move(name)), _operation(std::move(op)) {}
+    virtual ~classDataBase() { closeFile(); }
+    const std::string &getName() const { return _name; }
+    const std::string &getOperation() const { return _operation; }
+    double getLastTime() const { return _lastTime; }
+    void closeFile();
+
+  protected:
+    bool openFile(const std::string &dir, const std::string &header);
+    std::string _name, _operation;
+    FILE *_pFile = nullptr;
+    double _lastTime = 0.0;
+};
+
+class classNodeData : public classDataBase {
+  public:
+    classNodeData(const InpNodeExtraction &x);
+    const std::vector<int32_t> &getNodes() const { return _nodes; }
+    const std::string &getDataType() const { return _dataType; }
+    int getComp() const { return _comp; }
+    // Writes one row for `time` (skipped when time <= last time).
+    bool writeData(const std::string &dir, double time, const std::vector<double> &vals);
+    double getValue(const std::vector<double> &vals) const; // Mean / Max / Min / Sum (dataExtraction.cpp:247-280)
+
+  private:
+    std::vector<int32_t> _nodes;
+    std::string _dataType;
+    int _comp;
+};
+
+class classElementData : public classDataBase {
+  public:
+    classElementData(const InpElementExtraction &x);
+    const std::string &getDataType() const { return _dataType; }
+    bool writeData(const std::string &dir, double time, double val);
+
+  private:
+    std::string _dataType;
+};
+
+class classExtractData {
+  public:
+    // Validates the extractions against what the device path provides; false + err otherwise.
+    bool init(const InpModel &model, std::string &err);
+    bool empty() const { return _node.empty() && _element.empty(); }
+    bool needsNodal() const { return !_node.empty(); }
+    bool needsInternalForce() const;
+    bool needsEnergies() const { return !_element.empty(); }
+    std::map<std::string, std::unique_ptr<classNodeData>> &nodeData() { return _node; }
+    std::map<std::string, std::unique_ptr<classElementData>> &elementData() { return _element; }
+    void finalise();
+
+  private:
+    std::map<std::string, std::unique_ptr<classNodeData>> _node;
+    std::map<std::string, std::unique_ptr<classElementData>> _element;
+};
+
+} // namespace mps
+
+#endif

@@ -1,0 +1,465 @@
+// inp_reader.cpp -- .inp reader of the explicit solid path (see inp_reader.h
+// for the keyword list and the reference functions each part follows).
+#include "inp_reader.h"
+
+#include <cstdio>
+#include <cstdlib>
+#include <fstream>
+#include <set>
+#include <sstream>
+
+#include "mps_gpu.h"
+
+namespace mps {
+namespace {
+
+std::string rtrim(const std::string &s) {
+    size_t e = s.size();
+    while (e > 0 && (s[e - 1] == '\r' || s[e - 1] == ' ' || s[e - 1] == '\t')) e--;
+    return s.substr(0, e);
+}
+
+bool starts(const std::string &line, const char *kw) { return line.compare(0, std::char_traits<char>::length(kw), kw) == 0; }
+
+// Text after column `col` (the reference's line.compare(col, n, "...") tests).
+std::string tail(const std::string &line, size_t col) { return line.size() > col ? line.substr(col) : std::string(); }
+
+std::vector<std::string> fields(const std::string &line) {
+    std::vector<std::string> out;
+    std::string cur;
+    for (char c : line) {
+        if (c == ',') {
+            out.push_back(cur);
+            cur.clear();
+        } else {
+            cur.push_back(c);
+        }
+    }
+    out.push_back(cur);
+    return out;
+}
+
+bool isBlank(const std::string &s) { return s.find_first_not_of(" \t") == std::string::npos; }
+
+// atoi/atof semantics (leading blanks skipped, trailing text ignored), with a
+// check that a number was present.
+bool toInt(const std::string &s, long long &v) {
+    const char *p = s.c_str();
+    char *end = nullptr;
+    v = std::strtoll(p, &end, 10);
+    return end != p;
+}
+bool toDouble(const std::string &s, double &v) {
+    const char *p = s.c_str();
+    char *end = nullptr;
+    v = std::strtod(p, &end);
+    return end != p;
+}
+
+struct Reader {
+    std::vector<std::string> L;
+    std::string err;
+    std::string where(size_t i) const {
+        std::ostringstream o;
+        o << "line " << (i + 1) << " \"" << L[i] << "\"";
+        return o.str();
+    }
+    bool fail(size_t i, const std::string &msg) {
+        err = msg + " (" + where(i) + ")";
+        return false;
+    }
+};
+
+// nodesManagement (inputFile.cpp:20-111): first block after a line starting
+// "*Node", data until the next keyword; the comma count sets ndim.
+bool readNodes(Reader &R, InpModel &m) {
+    size_t i = 0;
+    while (i < R.L.size() && !starts(R.L[i], "*Node")) i++;
+    if (i == R.L.size()) {
+        R.err = "no *Node block";
+        return false;
+    }
+    std::vector<std::pair<long long, std::vector<double>>> rows;
+    for (i++; i < R.L.size() && !starts(R.L[i], "*"); i++) {
+        if (isBlank(R.L[i])) continue;
+        std::vector<std::string> f = fields(R.L[i]);
+        int nd = (int)f.size() - 1;
+        if (nd != 3) return R.fail(i, "this path is 3-D: a node line needs id, x, y, z");
+        if (m.ndim == 0) m.ndim = nd;
+        long long id;
+        std::vector<double> x(3);
+        if (!toInt(f[0], id) || !toDouble(f[1], x[0]) || !toDouble(f[2], x[1]) || !toDouble(f[3], x[2]))
+            return R.fail(i, "malformed node line");
+        rows.emplace_back(id, x);
+    }
+    const long long N = (long long)rows.size();
+    if (N == 0) {
+        R.err = "empty *Node block";
+        return false;
+    }
+    m.nNodes = (int32_t)N;
+    m.X.assign(3 * (size_t)N, 0.0);
+    std::vector<char> seen((size_t)N, 0);
+    for (auto &r : rows) {
+        if (r.first < 1 || r.first > N)
+            return (R.err = "node id " + std::to_string(r.first) + " outside 1.." + std::to_string(N) +
+                            " (node ids must number the nodes 1..N)",
+                    false);
+        size_t k = (size_t)(r.first - 1);
+        if (seen[k]) return (R.err = "duplicate node id " + std::to_string(r.first), false);
+        seen[k] = 1;
+        for (int d = 0; d < 3; d++) m.X[(size_t)d * N + k] = r.second[d];
+    }
+    return true;
+}
+
+// elementsManagement (inputFile.cpp:121-330): first "*Element" block; the
+// type is the text after column 15 ("*Element, type=").
+bool readElements(Reader &R, InpModel &m) {
+    size_t i = 0;
+    while (i < R.L.size() && !starts(R.L[i], "*Element")) i++;
+    if (i == R.L.size()) {
+        R.err = "There is not a list of elements in the input file";
+        return false;
+    }
+    std::string type = tail(R.L[i], 15);
+    if (type == "C3D8")
+        m.nen = 8;
+    else if (type == "C3D4")
+        m.nen = 4;
+    else
+        return R.fail(i, "element type \"" + type + "\" is not on this path (C3D8 and C3D4 are)");
+    m.elemType = type;
+    std::vector<std::pair<long long, std::vector<int32_t>>> rows;
+    for (i++; i < R.L.size() && !starts(R.L[i], "*"); i++) {
+        if (isBlank(R.L[i])) continue;
+        std::vector<std::string> f = fields(R.L[i]);
+        if ((int)f.size() != m.nen + 1) return R.fail(i, "element line needs id and " + std::to_string(m.nen) + " nodes");
+        long long id;
+        if (!toInt(f[0], id)) return R.fail(i, "malformed element id");
+        std::vector<int32_t> c(m.nen);
+        for (int a = 0; a < m.nen; a++) {
+            long long n;
+            if (!toInt(f[a + 1], n)) return R.fail(i, "malformed node id");
+            if (n < 1 || n > m.nNodes) return R.fail(i, "element node id outside 1..N");
+            c[a] = (int32_t)(n - 1);
+        }
+        rows.emplace_back(id, c);
+    }
+    for (size_t j = i; j < R.L.size(); j++)
+        if (starts(R.L[j], "*Element"))
+            m.warnings.push_back("only the first *Element block is read, as the reference does (" + R.where(j) + ")");
+    const long long E = (long long)rows.size();
+    if (E == 0) return (R.err = "empty *Element block", false);
+    m.nElem = (int32_t)E;
+    m.conn.assign((size_t)E * m.nen, 0);
+    std::vector<char> seen((size_t)E, 0);
+    for (auto &r : rows) {
+        if (r.first < 1 || r.first > E)
+            return (R.err = "element id " + std::to_string(r.first) + " outside 1.." + std::to_string(E) +
+                            " (element ids must number the elements 1..N_e)",
+                    false);
+        size_t k = (size_t)(r.first - 1);
+        if (seen[k]) return (R.err = "duplicate element id " + std::to_string(r.first), false);
+        seen[k] = 1;
+        for (int a = 0; a < m.nen; a++) m.conn[k * m.nen + a] = r.second[a];
+    }
+    return true;
+}
+
+// *FEM Nodes / *FEM Elements (inputFile.cpp:331-378): the path is pure FEM,
+// so both must be present and read "All".
+bool readFemSets(Reader &R) {
+    const char *kws[2] = {"*FEM Nodes", "*FEM Elements"};
+    for (const char *kw : kws) {
+        size_t i = 0;
+        while (i < R.L.size() && !starts(R.L[i], kw)) i++;
+        if (i == R.L.size()) return (R.err = std::string("no ") + kw + " block (meshless domains are not on this path)", false);
+        if (i + 1 >= R.L.size() || !starts(R.L[i + 1], "All"))
+            return R.fail(i, std::string(kw) + " must be \"All\" on this path");
+    }
+    return true;
+}
+
+// materialManagement (inputFile.cpp:838-1018, 2010-2054, 2118-2150).
+bool readMaterials(Reader &R, InpModel &m) {
+    bool inMat = false, listing = false, haveLaw = false;
+    double rho = 0; // function-scope in the reference: carries over between *MATERIAL blocks
+    InpMaterial cur;
+    auto flush = [&]() {
+        if (inMat && haveLaw) m.materials.push_back(cur);
+    };
+    for (size_t i = 0; i < R.L.size(); i++) {
+        const std::string &line = R.L[i];
+        if (starts(line, "*MATERIAL")) {
+            flush();
+            cur = InpMaterial();
+            cur.name = tail(line, 16);
+            inMat = true;
+            listing = false;
+            haveLaw = false;
+            continue;
+        }
+        if (!inMat) continue;
+        if (starts(line, "*Density")) {
+            if (i + 1 >= R.L.size() || !toDouble(R.L[i + 1], rho)) return R.fail(i, "*Density needs a value on the next line");
+            i++;
+        } else if (starts(line, "*HyperElastic")) {
+            std::string model = tail(line, 15);
+            if (model != "Neo-Hookean" && model != "3D-Compressible-Neo-Hookean")
+                return R.fail(i, "constitutive model \"" + model +
+                                     "\" has no device descriptor (Neo-Hookean and 3D-Compressible-Neo-Hookean do)");
+            if (i + 1 >= R.L.size()) return R.fail(i, "*HyperElastic needs \"E, nu\" on the next line");
+            std::vector<std::string> f = fields(R.L[i + 1]);
+            if (f.size() < 2 || !toDouble(f[0], cur.young) || !toDouble(f[1], cur.nu))
+                return R.fail(i + 1, "*HyperElastic needs \"E, nu\"");
+            if (rho <= 0) return R.fail(i, "density must be positive and given (*Density) before the law keyword");
+            cur.nameCons = model;
+            cur.rho = rho;
+            haveLaw = true;
+            i++;
+        } else if (starts(line, "*List of Elements")) {
+            if (!haveLaw) return R.fail(i, "The constitutive model has not been properly defined");
+            if (i + 1 >= R.L.size()) return R.fail(i, "*List of Elements needs a list");
+            const std::string &t = R.L[i + 1];
+            if (starts(t, "All")) {
+                cur.all = true;
+            } else {
+                listing = true;
+                long long id;
+                if (!toInt(t, id) || id < 1 || id > m.nElem) return R.fail(i + 1, "element id outside 1..N_e");
+                cur.elems.push_back((int32_t)(id - 1));
+            }
+            i++;
+        } else if (starts(line, "*END") || starts(line, "*End")) {
+            break;
+        } else if (starts(line, "*")) {
+            return R.fail(i, "keyword inside a *MATERIAL block that is not on this path");
+        } else if (listing) {
+            if (isBlank(line)) continue;
+            long long id;
+            if (!toInt(line, id) || id < 1 || id > m.nElem) return R.fail(i, "element id outside 1..N_e");
+            cur.elems.push_back((int32_t)(id - 1));
+        }
+    }
+    flush();
+    if (m.materials.empty()) return (R.err = "no *MATERIAL with a constitutive law", false);
+    return true;
+}
+
+// dispBCManagement (inputFile.cpp:2960-3075): node,dir,U lines until the
+// next keyword; node 1-based, dir 0-based.
+bool readDirichlet(Reader &R, size_t &i, const InpModel &m, InpSolver &s) {
+    for (i++; i < R.L.size() && !starts(R.L[i], "*"); i++) {
+        if (isBlank(R.L[i])) continue;
+        std::vector<std::string> f = fields(R.L[i]);
+        long long node, dir;
+        double U;
+        if (f.size() != 3 || !toInt(f[0], node) || !toInt(f[1], dir) || !toDouble(f[2], U))
+            return R.fail(i, "boundary line needs node, direction, value");
+        if (node < 1 || node > m.nNodes) return R.fail(i, "boundary node outside 1..N");
+        if (dir < 0 || dir >= 3) return R.fail(i, "The direction has not been recognized (0-x, 1-y, 2-z)");
+        s.dirichlet.push_back({(int32_t)(node - 1), (int32_t)dir, U});
+    }
+    i--; // the caller's loop advances to the keyword line
+    return true;
+}
+
+bool nextValue(Reader &R, size_t i, double &v) {
+    return i + 1 < R.L.size() && toDouble(R.L[i + 1], v);
+}
+
+// solverManagement (inputFile.cpp:2253-2830).  Scale factor, time window,
+// *Outputs and the instantaneous-BC flag are function-scope variables in the
+// reference and carry over from one *SOLVER block to the next.
+bool readSolvers(Reader &R, InpModel &m) {
+    double scale = 1.0, t0 = 0.0, tf = 0.0;
+    int numFiles = -1;
+    bool instantaneous = false;
+    for (size_t i = 0; i < R.L.size(); i++) {
+        if (!starts(R.L[i], "*SOLVER")) continue;
+        std::string type = tail(R.L[i], 9);
+        if (type != "EXPLICIT")
+            return R.fail(i, "solver \"" + type + "\" is not on this path (EXPLICIT is)");
+        InpSolver s;
+        bool timeSeen = false, velocity = false, displacement = false, closed = false;
+        for (i++; i < R.L.size(); i++) {
+            const std::string &l = R.L[i];
+            double v;
+            if (starts(l, "*END SOLVER")) {
+                closed = true;
+                break;
+            } else if (starts(l, "*Scale Factor")) {
+                if (!nextValue(R, i, v)) return R.fail(i, "*Scale Factor needs a value");
+                scale = v;
+                i++;
+            } else if (starts(l, "*Number Of Steps")) {
+                if (!nextValue(R, i, v)) return R.fail(i, "*Number Of Steps needs a value");
+                s.numStepsIgnored = (int)v;
+                m.warnings.push_back("*Number Of Steps is ignored by the EXPLICIT solver (" + R.where(i) + ")");
+                i++;
+            } else if (starts(l, "*Time")) {
+                if (i + 1 >= R.L.size()) return R.fail(i, "*Time needs \"t0, tf\"");
+                std::vector<std::string> f = fields(R.L[i + 1]);
+                if (f.size() < 2 || !toDouble(f[0], t0) || !toDouble(f[1], tf)) return R.fail(i + 1, "*Time needs \"t0, tf\"");
+                timeSeen = true;
+                i++;
+            } else if (starts(l, "*BOUNDARY, TYPE=DISPLACEMENT RAMP") || starts(l, "*BOUNDARY, TYPE=DISPLACEMENT INST")) {
+                if (starts(l, "*BOUNDARY, TYPE=DISPLACEMENT INST")) instantaneous = true;
+                displacement = true;
+                if (!timeSeen)
+                    return R.fail(i, "*Time must precede *BOUNDARY inside *SOLVER (the ramp duration is taken at parse time)");
+                if (!readDirichlet(R, i, m, s)) return false;
+            } else if (starts(l, "*BOUNDARY, TYPE=VELOCITY")) {
+                velocity = true;
+                if (!readDirichlet(R, i, m, s)) return false;
+            } else if (starts(l, "*Outputs")) {
+                if (!nextValue(R, i, v)) return R.fail(i, "*Outputs needs the number of output files");
+                numFiles = (int)v;
+            } else if (starts(l, "*MPS Fixed Steps")) {
+                if (!nextValue(R, i, v) || v < 1) return R.fail(i, "*MPS Fixed Steps needs a positive count");
+                s.fixedSteps = (int64_t)v;
+                i++;
+            } else if (starts(l, "*MPS Velocity Ramp Steps")) {
+                if (!nextValue(R, i, v) || v < 0) return R.fail(i, "*MPS Velocity Ramp Steps needs a count >= 0");
+                s.velRampSteps = (int)v;
+                i++;
+            } else if (starts(l, "*MPS Precision")) {
+                std::string p = i + 1 < R.L.size() ? R.L[i + 1] : "";
+                if (p == "FP64")
+                    s.precision = MPS_PREC_FP64;
+                else if (p == "FP32_ELEM")
+                    s.precision = MPS_PREC_FP32_ELEM;
+                else
+                    return R.fail(i, "*MPS Precision is FP64 or FP32_ELEM");
+                i++;
+            } else if (starts(l, "*Bulk viscosity") || starts(l, "*Dev viscosity") || starts(l, "*ACTIVATION") ||
+                       starts(l, "*FSI") || starts(l, "*ADAPTIVE TIME STEP") || starts(l, "*TIME STEPPING PLAN") ||
+                       starts(l, "*Forces")) {
+                return R.fail(i, "solver option not on this path");
+            } else if (starts(l, "*")) {
+                m.warnings.push_back("solver option ignored by the EXPLICIT solver (" + R.where(i) + ")");
+            }
+        }
+        if (!closed) return (R.err = "*SOLVER without *END SOLVER", false);
+        if (velocity && displacement)
+            return (R.err = "TYPE=VELOCITY cannot be mixed with TYPE=DISPLACEMENT in one solver", false);
+        s.scale = scale;
+        s.t0 = t0;
+        s.tf = tf;
+        s.timeSet = timeSeen;
+        s.numFiles = numFiles;
+        s.bcMode = velocity ? MPS_BC_VELOCITY : (instantaneous ? MPS_BC_INST : MPS_BC_RAMP);
+        if (s.numFiles < 0) return (R.err = "*Outputs is mandatory for the EXPLICIT solver", false);
+        if (s.dirichlet.empty())
+            return (R.err = "the EXPLICIT solver needs at least one mechanical Dirichlet DOF", false);
+        std::set<int64_t> dofs;
+        for (const InpDirichlet &d : s.dirichlet)
+            if (!dofs.insert((int64_t)d.node + (int64_t)m.nNodes * d.dir).second)
+                return (R.err = "duplicate Dirichlet DOF: node " + std::to_string(d.node + 1) + " direction " +
+                                std::to_string(d.dir),
+                        false);
+        if (s.fixedSteps == 0 && !(s.tf > s.t0))
+            return (R.err = "*Time needs tf > t0 (or *MPS Fixed Steps)", false);
+        m.solvers.push_back(s);
+    }
+    if (m.solvers.empty()) return (R.err = "no *SOLVER, EXPLICIT block", false);
+    return true;
+}
+
+// dataExtractionManagement (inputFile.cpp:2229-2243, 4950-5120).
+bool readExtractions(Reader &R, InpModel &m) {
+    for (size_t i = 0; i < R.L.size(); i++) {
+        const bool node = starts(R.L[i], "*EXTRACTION, NODE");
+        const bool elem = starts(R.L[i], "*EXTRACTION, ELEMENT");
+        if (!node && !elem) continue;
+        if (i + 1 >= R.L.size()) return R.fail(i, "extraction needs a definition line");
+        std::vector<std::string> f = fields(R.L[i + 1]);
+        if (node) {
+            InpNodeExtraction x;
+            long long comp;
+            if (f.size() != 4 || !toInt(f[3], comp)) return R.fail(i + 1, "node extraction needs name,type,operation,comp");
+            x.prefix = f[0];
+            x.dataType = f[1];
+            x.operation = f[2];
+            x.comp = (int)comp;
+            for (i += 2; i < R.L.size() && !starts(R.L[i], "*"); i++) {
+                long long id;
+                if (isBlank(R.L[i])) continue;
+                if (!toInt(R.L[i], id) || id < 1 || id > m.nNodes) return R.fail(i, "node id outside 1..N");
+                x.nodes.push_back((int32_t)(id - 1));
+            }
+            i--;
+            if (!x.nodes.empty()) m.nodeExtractions.push_back(x);
+        } else {
+            InpElementExtraction x;
+            if (f.size() != 3) return R.fail(i + 1, "element extraction needs name,type,operation");
+            x.prefix = f[0];
+            x.dataType = f[1];
+            x.operation = f[2];
+            for (i += 2; i < R.L.size() && !starts(R.L[i], "*"); i++) {
+                long long id;
+                if (isBlank(R.L[i])) continue;
+                if (starts(R.L[i], "All")) {
+                    x.all = true;
+                    continue;
+                }
+                if (!toInt(R.L[i], id) || id < 1 || id > m.nElem) return R.fail(i, "element id outside 1..N_e");
+                x.elems.push_back((int32_t)(id - 1));
+            }
+            i--;
+            if (x.all || !x.elems.empty()) m.elementExtractions.push_back(x);
+        }
+    }
+    return true;
+}
+
+// Keywords outside *SOLVER that would change the physics of the path.
+bool rejectUnsupported(Reader &R) {
+    bool inSolver = false;
+    for (size_t i = 0; i < R.L.size(); i++) {
+        const std::string &l = R.L[i];
+        if (starts(l, "*SOLVER")) inSolver = true;
+        if (starts(l, "*END SOLVER")) inSolver = false;
+        if (inSolver) continue;
+        if (starts(l, "*BOUNDARY") || starts(l, "*GRAVITY") || starts(l, "*Gravity"))
+            return R.fail(i, "Neumann loads and body forces are not on this path");
+    }
+    return true;
+}
+
+} // namespace
+
+bool readInp(const std::string &path, InpModel &m, std::string &err) {
+    std::ifstream ifs(path);
+    if (!ifs) {
+        err = "cannot open input file " + path;
+        return false;
+    }
+    Reader R;
+    std::string line;
+    while (std::getline(ifs, line)) R.L.push_back(rtrim(line));
+    m = InpModel();
+    bool ok = readNodes(R, m) && readElements(R, m) && readFemSets(R) && readMaterials(R, m) && readSolvers(R, m) &&
+              readExtractions(R, m) && rejectUnsupported(R);
+    if (!ok) err = R.err;
+    return ok;
+}
+
+bool elementMaterials(const InpModel &m, std::vector<int32_t> &matOfElem, std::string &err) {
+    matOfElem.assign((size_t)m.nElem, -1);
+    for (size_t k = 0; k < m.materials.size(); k++) {
+        const InpMaterial &mat = m.materials[k];
+        if (mat.all)
+            for (auto &x : matOfElem) x = (int32_t)k;
+        for (int32_t e : mat.elems) matOfElem[(size_t)e] = (int32_t)k;
+    }
+    for (int32_t e = 0; e < m.nElem; e++)
+        if (matOfElem[(size_t)e] < 0) {
+            err = "element " + std::to_string(e + 1) + " has no material";
+            return false;
+        }
+    return true;
+}
+
+} // namespace mps

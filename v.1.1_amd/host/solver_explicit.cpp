@@ -1,0 +1,258 @@
+// solver_explicit.cpp -- classSolverExplicit on the device engine (solver_explicit.h).
+#include "solver_explicit.h"
+
+#include <sys/stat.h>
+#include <sys/time.h>
+
+#include <algorithm>
+#include <cstdarg>
+#include <cstdio>
+#include <ctime>
+
+#include "mps_gpu.h"
+
+namespace mps {
+namespace {
+
+double wallNow() {
+    timeval tv;
+    gettimeofday(&tv, nullptr);
+    return tv.tv_sec + 1e-6 * tv.tv_usec;
+}
+double cpuNow() { return (double)std::clock() / CLOCKS_PER_SEC; }
+
+struct Log {
+    bool quiet;
+    __attribute__((format(printf, 2, 3))) void info(const char *f, ...) const {
+        if (quiet) return;
+        va_list ap;
+        va_start(ap, f);
+        std::printf("INFO ::: ");
+        std::vprintf(f, ap);
+        std::printf("\n");
+        std::fflush(stdout);
+        va_end(ap);
+    }
+};
+
+struct Handle {
+    mps_handle h = nullptr;
+    ~Handle() {
+        if (h) mps_destroy(h);
+    }
+};
+
+constexpr int kHistoryCap = 4096;
+
+} // namespace
+
+int classSolverExplicit::calculate(const InpModel &m, const std::vector<std::unique_ptr<constitutiveModels>> &laws,
+                                   const std::vector<int32_t> &lawOfElem, classExtractData &extract,
+                                   const RunOptions &opt, std::string &err) {
+    const Log log{opt.quiet};
+    const double startWall = wallNow(), startCpu = cpuNow();
+    const int32_t N = m.nNodes, Ne = m.nElem;
+
+    // Device material descriptors: one law per solve; E, nu, rho per element
+    // when the elements carry different materials.
+    const DeviceMaterial d0 = laws[(size_t)lawOfElem[0]]->deviceDescriptor();
+    bool uniform = true;
+    for (auto &l : laws) {
+        DeviceMaterial d = l->deviceDescriptor();
+        if (d.law != d0.law) {
+            err = "elements carry different constitutive laws; one solve runs one law (" + laws[0]->getNameCons() +
+                  " and " + l->getNameCons() + ")";
+            return -1;
+        }
+        uniform = uniform && d.young == d0.young && d.nu == d0.nu && d.rho == d0.rho;
+    }
+    std::vector<double> E, nu, rho;
+    if (uniform) {
+        E = {d0.young};
+        nu = {d0.nu};
+        rho = {d0.rho};
+    } else {
+        E.resize((size_t)Ne);
+        nu.resize((size_t)Ne);
+        rho.resize((size_t)Ne);
+        for (int32_t e = 0; e < Ne; e++) {
+            DeviceMaterial d = laws[(size_t)lawOfElem[(size_t)e]]->deviceDescriptor();
+            E[(size_t)e] = d.young;
+            nu[(size_t)e] = d.nu;
+            rho[(size_t)e] = d.rho;
+        }
+    }
+    std::vector<int32_t> dof;
+    std::vector<double> val;
+    for (const InpDirichlet &d : _s.dirichlet) {
+        dof.push_back(d.node + N * d.dir);
+        val.push_back(d.value);
+    }
+
+    mps_mesh mesh{N, Ne, m.nen, 0, m.X.data(), m.conn.data()};
+    mps_material mat{d0.law, uniform ? 0 : 1, E.data(), nu.data(), rho.data()};
+    mps_bcs bcs{(int32_t)dof.size(), _s.bcMode, _s.velRampSteps, 0, dof.data(), val.data()};
+    mps_solver_cfg cfg{};
+    cfg.t0 = _s.t0;
+    cfg.tf = _s.tf;
+    cfg.scale = _s.scale;
+    cfg.n_steps = _s.fixedSteps;
+    cfg.precision = opt.precision >= 0 ? opt.precision : _s.precision;
+    cfg.energy_every = extract.needsEnergies() ? 1 : 0;
+    cfg.history_cap = kHistoryCap;
+    cfg.device = opt.device;
+
+    Handle H;
+    if (mps_create(&mesh, &mat, &bcs, &cfg, &H.h) != MPS_OK) {
+        err = std::string("mps_create: ") + mps_last_error(nullptr);
+        return -1;
+    }
+    mps_handle h = H.h;
+    auto fail = [&](const char *what) {
+        err = std::string(what) + ": " + mps_last_error(h);
+        return -1;
+    };
+    if (!_u0.empty() || !_v0.empty() || !_a0.empty()) {
+        if (mps_set_state(h, _u0.empty() ? nullptr : _u0.data(), _v0.empty() ? nullptr : _v0.data(),
+                          _a0.empty() ? nullptr : _a0.data()) != MPS_OK)
+            return fail("mps_set_state");
+    }
+    double dt0 = 0;
+    if (mps_critical_dt(h, nullptr, &dt0) != MPS_OK) return fail("mps_critical_dt");
+
+    // Progress sampler (solverExplicit.cpp:168-170).
+    long long numStepsT = _s.fixedSteps > 0 ? _s.fixedSteps : (long long)((_s.tf - _s.t0) / dt0);
+    const int times = std::max(1, _s.numFiles);
+    const long long sampler = std::max(1LL, numStepsT / times);
+    log.info("Solver used is EXPLICIT");
+    log.info("The initial time step is %g", dt0);
+    log.info("done preprocessing in solver (Wall %gs, CPU %gs)", wallNow() - startWall, cpuNow() - startCpu);
+
+    // Nodal extraction DOFs, gathered on the device every step.
+    std::vector<int32_t> pickDof;
+    for (auto &kv : extract.nodeData())
+        if (kv.second->getDataType() == "Unknown")
+            for (int32_t n : kv.second->getNodes()) pickDof.push_back(n + N * kv.second->getComp());
+    const bool nodal = extract.needsNodal();
+    const bool needF = extract.needsInternalForce();
+    std::vector<double> picked(pickDof.size()), uFull, fint;
+    if (needF) {
+        uFull.resize(3 * (size_t)N);
+        fint.resize(3 * (size_t)N);
+    }
+    std::vector<double> ht(kHistoryCap), hdef(kHistoryCap), hkin(kHistoryCap), hvol(kHistoryCap);
+
+    const long long chunk = nodal ? 1 : (extract.needsEnergies() ? kHistoryCap : 1LL << 40);
+    long long nextInfo = sampler;
+    mps_report rep{};
+    for (;;) {
+        const long long before = rep.steps;
+        const int rc = mps_run(h, chunk, &rep);
+        if (rc == MPS_ERR_NAN || rep.status == MPS_ERR_NAN) {
+            err = "There are NANs in the displacements. Try a smaller scale factor";
+            return -1;
+        }
+        if (rc == MPS_ERR_DT || rep.status == MPS_ERR_DT) {
+            err = "Negative jacobian";
+            return -1;
+        }
+        if (rc != MPS_OK) return fail("mps_run");
+        const long long done = rep.steps - before;
+        if (done > 0 && extract.needsEnergies()) {
+            if (mps_get_history(h, before, done, ht.data(), nullptr, hdef.data(), hkin.data(), hvol.data()) != MPS_OK)
+                return fail("mps_get_history");
+            for (long long k = 0; k < done; k++)
+                for (auto &kv : extract.elementData()) {
+                    const std::string &t = kv.second->getDataType();
+                    const double v = t == "DEFO_ENERGY" ? hdef[k] : t == "KIN_ENERGY" ? hkin[k] : t == "VOLUME" ? hvol[k] : 0.0;
+                    if (!kv.second->writeData(opt.outputDir, ht[k], v)) {
+                        err = "cannot write " + kv.first + ".csv in " + opt.outputDir;
+                        return -1;
+                    }
+                }
+        }
+        if (done > 0 && nodal) {
+            if (!pickDof.empty() &&
+                mps_get_dofs(h, 0, (int64_t)pickDof.size(), pickDof.data(), picked.data()) != MPS_OK)
+                return fail("mps_get_dofs");
+            if (needF) {
+                if (mps_get_state(h, uFull.data(), nullptr, nullptr) != MPS_OK) return fail("mps_get_state");
+                if (mps_internal_force(h, uFull.data(), fint.data()) != MPS_OK) return fail("mps_internal_force");
+            }
+            size_t off = 0;
+            for (auto &kv : extract.nodeData()) {
+                classNodeData &d = *kv.second;
+                std::vector<double> vals(d.getNodes().size(), 0.0);
+                for (size_t j = 0; j < vals.size(); j++) {
+                    const size_t g = (size_t)d.getNodes()[j] + (size_t)N * d.getComp();
+                    if (d.getDataType() == "Unknown")
+                        vals[j] = picked[off + j];
+                    else if (d.getDataType() == "InternalForce")
+                        vals[j] = fint[g];
+                }
+                if (d.getDataType() == "Unknown") off += vals.size();
+                if (!d.writeData(opt.outputDir, rep.time, vals)) {
+                    err = "cannot write " + kv.first + ".csv in " + opt.outputDir;
+                    return -1;
+                }
+            }
+        }
+        if (rep.steps >= nextInfo || rep.status == MPS_DONE) {
+            log.info("Time %g out of %g", rep.time, _s.fixedSteps > 0 ? rep.time : _s.tf);
+            while (nextInfo <= rep.steps) nextInfo += sampler;
+        }
+        if (rep.status == MPS_DONE) break;
+        if (done == 0) {
+            err = "mps_run made no progress";
+            return -1;
+        }
+    }
+    _steps = rep.steps;
+    _time = rep.time;
+    _u.resize(3 * (size_t)N);
+    _v.resize(3 * (size_t)N);
+    _a.resize(3 * (size_t)N);
+    if (mps_get_state(h, _u.data(), _v.data(), _a.data()) != MPS_OK) return fail("mps_get_state");
+    log.info("done solving (Wall %gs, CPU %gs)", wallNow() - startWall, cpuNow() - startCpu);
+    return 0;
+}
+
+int runInputFile(const std::string &path, const RunOptions &opt, std::string &err, std::vector<double> *finalU) {
+    const Log log{opt.quiet};
+    InpModel model;
+    if (!readInp(path, model, err)) return -1;
+    for (const std::string &w : model.warnings)
+        if (!opt.quiet) std::fprintf(stderr, "WARNING ::: %s\n", w.c_str());
+    log.info("Type of elements %s", model.elemType.c_str());
+    std::vector<int32_t> matOfElem;
+    if (!elementMaterials(model, matOfElem, err)) return -1;
+    std::vector<std::unique_ptr<constitutiveModels>> laws;
+    for (const InpMaterial &mat : model.materials) {
+        laws.push_back(makeConstitutiveModel(mat.name, mat.nameCons, mat.rho, mat.young, mat.nu));
+        if (!laws.back()) {
+            err = "constitutive model " + mat.nameCons + " has no device descriptor";
+            return -1;
+        }
+    }
+    classExtractData extract;
+    if (!extract.init(model, err)) return -1;
+    if (!extract.empty()) {
+        struct stat st;
+        if (stat(opt.outputDir.c_str(), &st) != 0 && mkdir(opt.outputDir.c_str(), 0755) != 0) {
+            err = "cannot create output directory " + opt.outputDir;
+            return -1;
+        }
+    }
+    std::vector<double> u, v, a;
+    for (size_t k = 0; k < model.solvers.size(); k++) {
+        classSolverExplicit s(model.solvers[k]);
+        if (k > 0) s.initValues(u, v, a);
+        if (s.calculate(model, laws, matOfElem, extract, opt, err) != 0) return -1;
+        s.getFinalValues(u, v, a);
+    }
+    extract.finalise();
+    if (finalU) *finalU = u;
+    return 0;
+}
+
+} // namespace mps

@@ -47,6 +47,8 @@ extern "C" {
 #define MPS_BC_RAMP 0     /* *BOUNDARY, TYPE=DISPLACEMENT RAMP (classNodes.cpp:66-74) */
 #define MPS_BC_INST 1     /* *BOUNDARY, TYPE=DISPLACEMENT INST (classNodes.cpp:58-64) */
 #define MPS_BC_VELOCITY 2 /* extension: prescribed velocity V*min(step/ramp,1) */
+#define MPS_BC_RAMP_RATE 3 /* ramp whose value is already UU = U/(tf-t0), as classDirichlet
+                              stores it (classNodes.cpp:66-74, getUU classNodes.h:75-77) */
 
 /* ---- precision forms ---- */
 #define MPS_PREC_FP64 0      /* element math and everything else in FP64 */

@@ -128,6 +128,20 @@ def test_reference_time_window_mode(mode):
     assert relerr(u, ro["u"]) <= TOL64
 
 
+def test_ramp_given_as_rate_equals_ramp():
+    """MPS_BC_RAMP_RATE takes UU = U/(tf-t0) as classDirichlet stores it; the run is bitwise the RAMP run."""
+    m = box(4, 4, 4)
+    dd, dv = clamp_and_pull(m, vel=0.002)
+    tf = 0.03
+    runs = []
+    for mode, vals in ((G.BC_RAMP, dv), (G.BC_RAMP_RATE, dv / (tf - 0.0))):
+        with G.Solver(m, dir_dof=dd, dir_val=vals, bc_mode=mode, t0=0.0, tf=tf, n_steps=0) as s:
+            rc, rep = s.run(10 ** 9)
+            assert rc == 0 and rep["status"] == G.MPS_DONE
+            runs.append(s.get_state()[0])
+    assert np.array_equal(runs[0], runs[1])
+
+
 def test_chained_solver_handoff_and_chunks():
     m = box(4, 4, 6)
     ro1, (rc1, rep1, u1, v1, a1, _h) = run_pair(m, 120, chunks=[7, 50, 63])

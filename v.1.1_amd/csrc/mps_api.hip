@@ -103,7 +103,7 @@ int prepare(const mps_mesh* mesh, const mps_material* mat, const mps_bcs* bcs, c
         *err = "need n_steps > 0 or a time window tf > t0";
         return MPS_ERR_ARG;
     }
-    if (bcs->mode < MPS_BC_RAMP || bcs->mode > MPS_BC_VELOCITY) {
+    if (bcs->mode < MPS_BC_RAMP || bcs->mode > MPS_BC_RAMP_RATE) {
         *err = fmt("unknown Dirichlet mode %d", bcs->mode);
         return MPS_ERR_ARG;
     }
@@ -189,7 +189,7 @@ int prepare(const mps_mesh* mesh, const mps_material* mat, const mps_bcs* bcs, c
         if (rec_of[n] >= 0) hp->ninfo[n] |= ((uint32_t)rec_of[n] << 3);
     if (hp->bcval.empty()) hp->bcval.assign(3, 0.0);
     hp->X.assign(mesh->X, mesh->X + 3 * N);
-    hp->bc_mode = bcs->mode;
+    hp->bc_mode = bcs->mode == MPS_BC_RAMP_RATE ? MPS_BC_RAMP : bcs->mode;  // a given rate runs as a ramp
     hp->vel_ramp = bcs->vel_ramp_steps;
     hp->cfg = *cfg;
     return MPS_OK;

@@ -136,6 +136,43 @@ int prepare(const mps_mesh* mesh, const mps_material* mat, const mps_bcs* bcs, c
         std::vector<int32_t> fill(hp->rowptr.begin(), hp->rowptr.end() - 1);
         for (int64_t k = 0; k < Ne * nen; k++) hp->col[fill[hp->conn[k]]++] = (int32_t)k;
     }
+    // patches: MPS_PATCH consecutive elements each; distinct nodes ascending, their
+    // contributions (local element, slot) ascending; node -> partials ascending by patch
+    {
+        const int64_t P = MPS_PATCH, np = (Ne + P - 1) / P;
+        hp->n_patch = (int32_t)np;
+        hp->poff.assign(np + 1, 0);
+        hp->pcol.resize(Ne * nen);
+        hp->prow.clear();
+        hp->prow.reserve((size_t)(3 * Ne + np));
+        std::vector<int32_t> pnode;
+        pnode.reserve((size_t)(3 * Ne));
+        std::vector<uint64_t> key((size_t)(P * nen));
+        for (int64_t p = 0; p < np; p++) {
+            const int64_t e0 = p * P, m = (std::min(e0 + P, Ne) - e0) * nen;
+            for (int64_t q = 0; q < m; q++) key[q] = ((uint64_t)hp->conn[e0 * nen + q] << 16) | (uint64_t)q;
+            std::sort(key.begin(), key.begin() + m);
+            int64_t U = 0;
+            for (int64_t q = 0; q < m; q++) {
+                const int32_t n = (int32_t)(key[q] >> 16);
+                if (q == 0 || n != pnode.back()) {
+                    hp->prow.push_back((uint16_t)q);
+                    pnode.push_back(n);
+                    U++;
+                }
+                hp->pcol[e0 * nen + q] = (uint16_t)(key[q] & 0xffffu);
+            }
+            hp->prow.push_back((uint16_t)m);
+            hp->poff[p + 1] = hp->poff[p] + (int32_t)U;
+        }
+        hp->n_part = hp->poff[np];
+        hp->nrow.assign(N + 1, 0);
+        for (int32_t n : pnode) hp->nrow[n + 1]++;
+        for (int64_t n = 0; n < N; n++) hp->nrow[n + 1] += hp->nrow[n];
+        hp->ncol.resize(pnode.size());
+        std::vector<int32_t> fill(hp->nrow.begin(), hp->nrow.end() - 1);
+        for (size_t g = 0; g < pnode.size(); g++) hp->ncol[fill[pnode[g]]++] = (int32_t)g;
+    }
     // materials
     hp->per_elem = mat->per_element != 0;
     if (hp->per_elem) {
@@ -293,11 +330,13 @@ static int upload_instance(mps_solver_s* h, const HostPrep& hp) {
     I.dof1_node = (int32_t)(1 % N);
     I.dof1_dir = (int32_t)(1 / N);
     double *X, *u, *v, *a, *mass, *l0, *matp = nullptr, *cel = nullptr, *rho = nullptr, *bcval, *hist, *part;
-    int32_t *conn, *rowptr, *col;
+    double* pforce;
+    int32_t *conn, *rowptr, *col, *poff, *nrow, *ncol;
+    uint16_t *prow, *pcol;
     uint32_t* ninfo;
-    void* fe;
     MpsCtl* ctl;
-    const size_t fe_bytes = Ne * nen * 3 * (hp.cfg.precision == MPS_PREC_FP32_ELEM ? 4 : 8);
+    const size_t NP = (size_t)hp.n_part, NPA = (size_t)hp.n_patch;
+    const size_t pf_bytes = std::max(NP * 3 * 8, Ne * nen * 8);  // partial forces; set-up scratch first
     const int cap = hp.cfg.history_cap > 0 ? hp.cfg.history_cap : 4096;
     HCHK(h, dalloc(h, &X, 3 * N * 8));
     HCHK(h, dalloc(h, &u, 3 * N * 8));
@@ -310,7 +349,12 @@ static int upload_instance(mps_solver_s* h, const HostPrep& hp) {
     HCHK(h, dalloc(h, &col, Ne * nen * 4));
     HCHK(h, dalloc(h, &ninfo, N * 4));
     HCHK(h, dalloc(h, &bcval, hp.bcval.size() * 8));
-    HCHK(h, dalloc(h, &fe, fe_bytes));
+    HCHK(h, dalloc(h, &pforce, pf_bytes));
+    HCHK(h, dalloc(h, &poff, (NPA + 1) * 4));
+    HCHK(h, dalloc(h, &prow, (NP + NPA) * 2));
+    HCHK(h, dalloc(h, &pcol, Ne * nen * 2));
+    HCHK(h, dalloc(h, &nrow, (N + 1) * 4));
+    HCHK(h, dalloc(h, &ncol, NP * 4));
     HCHK(h, dalloc(h, &ctl, sizeof(MpsCtl)));
     HCHK(h, dalloc(h, &hist, 5 * (size_t)cap * 8));
     h->energy_nb = (int)((Ne + 255) / 256);
@@ -333,6 +377,11 @@ static int upload_instance(mps_solver_s* h, const HostPrep& hp) {
     HCHK(h, hipMemcpyAsync(conn, hp.conn.data(), Ne * nen * 4, hipMemcpyHostToDevice, h->stream));
     HCHK(h, hipMemcpyAsync(rowptr, hp.rowptr.data(), (N + 1) * 4, hipMemcpyHostToDevice, h->stream));
     HCHK(h, hipMemcpyAsync(col, hp.col.data(), Ne * nen * 4, hipMemcpyHostToDevice, h->stream));
+    HCHK(h, hipMemcpyAsync(poff, hp.poff.data(), (NPA + 1) * 4, hipMemcpyHostToDevice, h->stream));
+    HCHK(h, hipMemcpyAsync(prow, hp.prow.data(), (NP + NPA) * 2, hipMemcpyHostToDevice, h->stream));
+    HCHK(h, hipMemcpyAsync(pcol, hp.pcol.data(), Ne * nen * 2, hipMemcpyHostToDevice, h->stream));
+    HCHK(h, hipMemcpyAsync(nrow, hp.nrow.data(), (N + 1) * 4, hipMemcpyHostToDevice, h->stream));
+    HCHK(h, hipMemcpyAsync(ncol, hp.ncol.data(), NP * 4, hipMemcpyHostToDevice, h->stream));
     HCHK(h, hipMemcpyAsync(ninfo, hp.ninfo.data(), N * 4, hipMemcpyHostToDevice, h->stream));
     HCHK(h, hipMemcpyAsync(bcval, hp.bcval.data(), hp.bcval.size() * 8, hipMemcpyHostToDevice, h->stream));
     HCHK(h, hipMemsetAsync(hist, 0xFF, 5 * (size_t)cap * 8, h->stream));  // NaN
@@ -350,7 +399,14 @@ static int upload_instance(mps_solver_s* h, const HostPrep& hp) {
     I.bcval = bcval;
     I.rowptr = rowptr;
     I.col = col;
-    I.fe = fe;
+    I.poff = poff;
+    I.prow = prow;
+    I.pcol = pcol;
+    I.nrow = nrow;
+    I.ncol = ncol;
+    I.part = pforce;
+    I.n_patch = hp.n_patch;
+    I.n_part = hp.n_part;
     I.ctl = ctl;
     I.hist = hist;
     I.partials = part;
@@ -361,13 +417,11 @@ static int upload_instance(mps_solver_s* h, const HostPrep& hp) {
 // Set-up kernels: l0, lumped mass, dt0 = s*min(l0/c); initial control block.
 static int device_setup(mps_solver_s* h) {
     MpsInst& I = h->I;
-    double* mslot = nullptr;
-    HCHK(h, hipMalloc(&mslot, (size_t)I.Ne * I.nen * 8));
+    double* mslot = I.part;  // per-(element, slot) mass values; the partial-force buffer is free until the first step
     unsigned long long inf = 0x7FF0000000000000ull;
     HCHK(h, hipMemcpyAsync(h->scratch_dt, &inf, 8, hipMemcpyHostToDevice, h->stream));
     hipError_t e = mpsk::launch_setup(I, mslot, (double*)I.l0, h->scratch_dt, h->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
-    (void)hipFree(mslot);
     if (e != hipSuccess) return h->fail(MPS_ERR_HIP, fmt("set-up kernels: %s", hipGetErrorString(e)));
     unsigned long long bits = 0;
     HCHK(h, hipMemcpy(&bits, h->scratch_dt, 8, hipMemcpyDeviceToHost));

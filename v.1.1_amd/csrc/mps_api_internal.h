@@ -13,6 +13,10 @@ struct HostPrep {
     int32_t N = 0, Ne = 0, nen = 0, law = 0;
     std::vector<double> X;
     std::vector<int32_t> conn, rowptr, col;
+    // patch tables (see mps_common.h)
+    int32_t n_patch = 0, n_part = 0;
+    std::vector<int32_t> poff, nrow, ncol;
+    std::vector<uint16_t> prow, pcol;
     bool per_elem = false;
     std::vector<double> mat, cel, rho;  // per element
     double pa = 0, pb = 0, c0 = 0, rho0 = 0;  // uniform
@@ -31,7 +35,7 @@ int prepare(const mps_mesh* mesh, const mps_material* mat, const mps_bcs* bcs, c
 }  // namespace mpsh
 
 // Device bytes of one solve in a batch arena (mps_batch.hip).
-size_t mps_instance_bytes(int64_t N, int64_t Ne, int nen, int precision, bool per_elem, size_t nrec);
+size_t mps_instance_bytes(const mpsh::HostPrep& hp, int precision);
 // Batch internals used by the server (mps_batch.hip).
 int mps_batch_check(mps_batch b, const mpsh::HostPrep& hp, std::string* why);
 size_t mps_batch_footprint(mps_batch b, const mpsh::HostPrep& hp);

@@ -3,8 +3,9 @@
 //
 // Layout: every solve ("instance") owns one contiguous block of the arena holding its SoA
 // arrays (same layout as the single-solve path) and one MpsCtl in a separate control array.
-// Work is described by two device tables of MpsWork items (element ranges, node ranges), each
-// item at most one workgroup wide and never spanning two instances.  The graph replays
+// Work is described by two device tables of MpsWork items, never spanning two instances:
+// element items are patches (MPS_PATCH elements, processed by one workgroup of elem_block
+// work-items), node items are node ranges one workgroup wide.  The graph replays
 //   begin, (element, node+begin)x(S-1), element, node
 // over the tables; instances at different step counts coexist because each carries its own
 // step protocol (MpsCtl), and finished instances are skipped by the go flags.
@@ -109,18 +110,24 @@ struct Table {
 }  // namespace
 
 // Bytes of device state of one solve (arena block).
-size_t mps_instance_bytes(int64_t N, int64_t Ne, int nen, int precision, bool per_elem, size_t nrec) {
-    const size_t pe = precision == MPS_PREC_FP32_ELEM ? 4 : 8;
+size_t mps_instance_bytes(const HostPrep& hp, int precision) {
+    (void)precision;  // element forces are summed into FP64 partials in every precision form
+    const size_t N = hp.N, Ne = hp.Ne, nen = hp.nen, NP = hp.n_part, NPA = hp.n_patch;
+    const size_t nrec = hp.bcval.size() / 3;
     size_t b = 0;
     b += align_up(3 * N * 8) * 4;                 // X, u, v, a
     b += align_up(N * 8);                         // mass
     b += align_up(N * 4);                         // ninfo
-    b += align_up((N + 1) * 4);                   // rowptr
+    b += align_up((N + 1) * 4) * 2;               // rowptr, nrow
     b += align_up(Ne * nen * 4) * 2;              // conn, col
     b += align_up(Ne * 8);                        // l0
-    b += align_up(std::max<size_t>(Ne * nen * 3 * pe, Ne * nen * 8));  // fe (also set-up scratch)
+    b += align_up(std::max<size_t>(NP * 3 * 8, Ne * nen * 8));  // partial forces (also set-up scratch)
+    b += align_up((NPA + 1) * 4);                 // poff
+    b += align_up((NP + NPA) * 2);                // prow
+    b += align_up(Ne * nen * 2);                  // pcol
+    b += align_up(NP * 4);                        // ncol
     b += align_up(std::max<size_t>(nrec, 1) * 3 * 8);  // bcval
-    if (per_elem) b += align_up(2 * Ne * 8) + align_up(Ne * 8) * 2;  // mat, cel, rho
+    if (hp.per_elem) b += align_up(2 * Ne * 8) + align_up(Ne * 8) * 2;  // mat, cel, rho
     return b;
 }
 
@@ -251,7 +258,7 @@ int mps_batch_check(mps_batch b, const HostPrep& hp, std::string* why) {
 }
 
 size_t mps_batch_footprint(mps_batch b, const HostPrep& hp) {
-    return mps_instance_bytes(hp.N, hp.Ne, hp.nen, b->cfg.precision, hp.per_elem, hp.bcval.size() / 3);
+    return mps_instance_bytes(hp, b->cfg.precision);
 }
 
 // Free slot and largest free arena block (admission by footprint).
@@ -288,7 +295,8 @@ int mps_batch_add_prepared(mps_batch b, const HostPrep& hp, int32_t* slot_out) {
         }
     if (slot < 0) return b->fail(MPS_ERR_CAPACITY, "no free instance slot");
     const size_t N = hp.N, Ne = hp.Ne, nen = hp.nen;
-    const size_t bytes = mps_instance_bytes(N, Ne, (int)nen, cfg->precision, hp.per_elem, hp.bcval.size() / 3);
+    const size_t bytes = mps_instance_bytes(hp, cfg->precision);
+    const size_t NP = hp.n_part, NPA = hp.n_patch;
     size_t off = 0;
     if (!b->alloc.alloc(bytes, &off)) return b->fail(MPS_ERR_CAPACITY, fmt("arena full (%zu bytes requested)", bytes));
     (void)hipSetDevice(b->device);
@@ -328,8 +336,14 @@ int mps_batch_add_prepared(mps_batch b, const HostPrep& hp, int32_t* slot_out) {
     I.conn = (int32_t*)take(Ne * nen * 4);
     I.col = (int32_t*)take(Ne * nen * 4);
     I.l0 = (double*)take(Ne * 8);
-    const size_t pe = cfg->precision == MPS_PREC_FP32_ELEM ? 4 : 8;
-    I.fe = take(std::max<size_t>(Ne * nen * 3 * pe, Ne * nen * 8));
+    I.part = (double*)take(std::max<size_t>(NP * 3 * 8, Ne * nen * 8));
+    I.poff = (int32_t*)take((NPA + 1) * 4);
+    I.prow = (uint16_t*)take((NP + NPA) * 2);
+    I.pcol = (uint16_t*)take(Ne * nen * 2);
+    I.nrow = (int32_t*)take((N + 1) * 4);
+    I.ncol = (int32_t*)take(NP * 4);
+    I.n_patch = hp.n_patch;
+    I.n_part = hp.n_part;
     I.bcval = (double*)take(hp.bcval.size() * 8);
     if (hp.per_elem) {
         I.mat = (double*)take(2 * Ne * 8);
@@ -354,6 +368,11 @@ int mps_batch_add_prepared(mps_batch b, const HostPrep& hp, int32_t* slot_out) {
     if (he == hipSuccess) he = up((void*)I.rowptr, hp.rowptr.data(), (N + 1) * 4);
     if (he == hipSuccess) he = up((void*)I.conn, hp.conn.data(), Ne * nen * 4);
     if (he == hipSuccess) he = up((void*)I.col, hp.col.data(), Ne * nen * 4);
+    if (he == hipSuccess) he = up((void*)I.poff, hp.poff.data(), (NPA + 1) * 4);
+    if (he == hipSuccess) he = up((void*)I.prow, hp.prow.data(), (NP + NPA) * 2);
+    if (he == hipSuccess) he = up((void*)I.pcol, hp.pcol.data(), Ne * nen * 2);
+    if (he == hipSuccess) he = up((void*)I.nrow, hp.nrow.data(), (N + 1) * 4);
+    if (he == hipSuccess) he = up((void*)I.ncol, hp.ncol.data(), NP * 4);
     if (he == hipSuccess) he = up((void*)I.bcval, hp.bcval.data(), hp.bcval.size() * 8);
     if (hp.per_elem) {
         if (he == hipSuccess) he = up((void*)I.mat, hp.mat.data(), 2 * Ne * 8);
@@ -366,7 +385,7 @@ int mps_batch_add_prepared(mps_batch b, const HostPrep& hp, int32_t* slot_out) {
     // set-up kernels on the compute stream, after whatever is queued there
     unsigned long long* dt0 = &b->ctl[slot].dt0min;
     if (he == hipSuccess) he = mpsk::launch_fill_u64(dt0, 0x7FF0000000000000ull, b->stream);
-    if (he == hipSuccess) he = mpsk::launch_setup(I, (double*)I.fe, (double*)I.l0, dt0, b->stream);
+    if (he == hipSuccess) he = mpsk::launch_setup(I, I.part, (double*)I.l0, dt0, b->stream);
     if (he == hipSuccess) he = mpsk::launch_init_ctl(I, dt0, b->stream);
     // dt of step 1 from u0 = 0 (solverExplicit.cpp:176-178)
     if (he == hipSuccess) he = mpsk::launch_dt(I, I.t0 < 1E-12 ? I.u : nullptr, &I.ctl->dtmin[1], b->stream);
@@ -430,7 +449,8 @@ static int flush_tables(mps_batch_s* b) {
     for (int s = 0; s < (int)b->slots.size(); s++) {
         const Slot& S = b->slots[s];
         if (!S.used) continue;
-        for (int e0 = 0; e0 < S.inst.Ne; e0 += bs) we.push_back({s, e0, std::min(e0 + bs, S.inst.Ne), e0 == 0});
+        for (int e0 = 0; e0 < S.inst.Ne; e0 += MPS_PATCH)  // one item per patch
+            we.push_back({s, e0, std::min(e0 + MPS_PATCH, S.inst.Ne), e0 == 0});
         for (int n0 = 0; n0 < S.inst.N; n0 += bs) wn.push_back({s, n0, std::min(n0 + bs, S.inst.N), n0 == 0});
     }
     const int nt = 1 - b->cur;

@@ -9,13 +9,22 @@
 //   l0              double[Ne]   undeformed characteristic length
 //   mat             double[2*Ne] per-element (lambda,mu) or (mu,K1); NULL = uniform
 //   cel             double[Ne]   per-element sound speed; NULL = uniform c0
-//   rowptr/col      CSR node -> (e*nen + a), ascending element then slot
-//   fe              T[Ne*nen*3]  element nodal forces, [(e*nen+a)*3 + dir]
+//   rowptr/col      CSR node -> (e*nen + a), ascending element then slot (set-up gathers)
+//   patches         elements [p*MPS_PATCH, (p+1)*MPS_PATCH) form patch p; its U_p distinct
+//                   nodes (ascending) own partial forces part[poff[p] .. poff[p+1])
+//     poff          int32[n_patch+1]       first partial of each patch
+//     prow          uint16[n_part+n_patch] per patch, U_p+1 offsets into its pcol list
+//     pcol          uint16[nen*Ne]         per patch, contributions i*nen+a (local element i,
+//                                          slot a) grouped by local node, ascending
+//     nrow/ncol     CSR node -> partials, ascending patch (= ascending element)
+//   part            double[3*n_part]       partial nodal forces [(g)*3 + dir]; also the
+//                                          set-up scratch (>= nen*Ne doubles)
 //   ctl             MpsCtl       step control (time, dt, go flags, dt reduction slots)
 #pragma once
 #include <stdint.h>
 
 #define MPS_MAX_NEN 8
+#define MPS_PATCH 256  // elements per patch (element-kernel work unit and partial-sum scope)
 
 enum : int32_t { MPS_TIME_WINDOW = 0, MPS_TIME_FIXED = 1 };
 
@@ -29,7 +38,7 @@ struct MpsStepSlot {
 
 // Step protocol (all in device memory, no host round trip per step):
 //   A = next step to run, written only by the node kernel, read by begin/element kernels;
-//   B = step whose forces are in fe, written only by the element kernel, read by node/energy.
+//   B = step whose forces are in part, written only by the element kernel, read by node/energy.
 // slot[k&1] holds step k; dtmin[k&1] receives the min over elements for step k
 // (computed from u_{k-1}); the node kernel of step k resets dtmin[k&1].
 struct MpsCtl {
@@ -63,13 +72,20 @@ struct MpsInst {
     const double* bcval;
     const int32_t* rowptr;
     const int32_t* col;
-    void* fe;
+    const int32_t* poff;
+    const uint16_t* prow;
+    const uint16_t* pcol;
+    const int32_t* nrow;
+    const int32_t* ncol;
+    double* part;
     MpsCtl* ctl;
     double* hist;      // [5*hist_cap]: time, dt, e_def, e_kin, volume
     double* partials;  // energy partials [3*n_blocks]
     int32_t hist_cap;
     int32_t dof1_node, dof1_dir;
-    int32_t _pad;
+    int32_t n_patch;
+    int32_t n_part;
+    int32_t _pad2;
 };
 
 // One unit of work for the persistent batched kernels: a contiguous range of

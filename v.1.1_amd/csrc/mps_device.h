@@ -643,33 +643,29 @@ __device__ __forceinline__ void predict_node(const MpsInst& I, int n, long long 
 }
 
 // Gather + lumped-mass solve of node n (nonlinearSystem.cpp:1096-1107; rhs = -fint + fext with fext = 0).
-template <typename TF>
+// The node's internal force is the sum of its patch partials in ascending patch order.
 __device__ __forceinline__ void solve_node(const MpsInst& I, int n, double dt, double* v, double* a) {
 #pragma clang fp contract(off)
     uint32_t mask = I.ninfo[n] & 7u;
     if (mask == 7u) return;
-    const TF* __restrict__ fe = (const TF*)I.fe;
-    int q = I.rowptr[n];
-    const int q1 = I.rowptr[n + 1];
+    const double* __restrict__ part = I.part;
+    int q = I.nrow[n];
+    const int q1 = I.nrow[n + 1];
     double f0 = 0.0, f1 = 0.0, f2 = 0.0;
-    // 4 entries per iteration: all loads issue before the (ordered) additions
-    for (; q + 4 <= q1; q += 4) {
-        const size_t c0 = (size_t)I.col[q] * 3, c1 = (size_t)I.col[q + 1] * 3;
-        const size_t c2 = (size_t)I.col[q + 2] * 3, c3 = (size_t)I.col[q + 3] * 3;
-        const double a0 = fe[c0], b0 = fe[c0 + 1], d0 = fe[c0 + 2];
-        const double a1 = fe[c1], b1 = fe[c1 + 1], d1 = fe[c1 + 2];
-        const double a2 = fe[c2], b2 = fe[c2 + 1], d2 = fe[c2 + 2];
-        const double a3 = fe[c3], b3 = fe[c3 + 1], d3 = fe[c3 + 2];
+    // 2 entries per iteration: both loads issue before the (ordered) additions
+    for (; q + 2 <= q1; q += 2) {
+        const double* c0 = part + 3 * (size_t)I.ncol[q];
+        const double* c1 = part + 3 * (size_t)I.ncol[q + 1];
+        const double a0 = c0[0], b0 = c0[1], d0 = c0[2];
+        const double a1 = c1[0], b1 = c1[1], d1 = c1[2];
         f0 += a0; f1 += b0; f2 += d0;
         f0 += a1; f1 += b1; f2 += d1;
-        f0 += a2; f1 += b2; f2 += d2;
-        f0 += a3; f1 += b3; f2 += d3;
     }
-    for (; q < q1; q++) {
-        const size_t cidx = (size_t)I.col[q] * 3;
-        f0 += (double)fe[cidx];
-        f1 += (double)fe[cidx + 1];
-        f2 += (double)fe[cidx + 2];
+    if (q < q1) {
+        const double* c = part + 3 * (size_t)I.ncol[q];
+        f0 += c[0];
+        f1 += c[1];
+        f2 += c[2];
     }
     double m = I.mass[n];
     double fi[3] = {f0, f1, f2};

@@ -1,9 +1,10 @@
 // mps_kernels.hip -- HIP kernels of the explicit step (gfx950).
 //
 // Per step (fixed pattern, no host round trip):
-//   element kernel  : u_k -> element nodal forces fe (T = double or float) and the
-//                     element-wise stable dt for step k+1 (block min, 64-bit integer atomicMin)
-//   node kernel     : CSR gather of fe (ascending element order, FP64, no float atomics),
+//   element kernel  : u_k -> element nodal forces (T = double or float), summed per patch in
+//                     LDS into FP64 partial nodal forces, and the element-wise stable dt for
+//                     step k+1 (block min, 64-bit integer atomicMin)
+//   node kernel     : CSR gather of the partials (ascending patch order, FP64, no float atomics),
 //                     a = -fint/m, v += dt(a_old + a_new)/2, and -- when fused -- the
 //                     begin of step k+1 (lagged time, dt, prediction, Dirichlet BC)
 // plus a standalone begin kernel (first step of a launch) and an energy pass.
@@ -17,41 +18,68 @@ using namespace mpsd;
 // ---------------------------------------------------------------------------
 // Element kernels
 // ---------------------------------------------------------------------------
-template <typename T, int NEN>
-__device__ __forceinline__ void store_fe(const MpsInst& I, int e, const T (*f)[3]) {
-    T* dst = (T*)I.fe + (size_t)e * (NEN * 3);
+// Forces of one patch (elements [p*MPS_PATCH, ...)): each element's nodal forces go to LDS
+// (SoA sf[(a*3+d)*MPS_PATCH + i]); after a barrier the block sums, per distinct node of the
+// patch, its contributions in ascending (element, slot) order in FP64 and writes one partial
+// force per node to part.  No float atomics: the node kernel adds the partials of a node in
+// ascending patch order, so the sum order is fixed.  Returns the block's min element dt
+// candidate (64-bit pattern, lanes other than those computing elements hold +inf).
+template <typename T, int NEN, int LAW, int BS>
+__device__ __forceinline__ unsigned long long patch_forces(const MpsInst& I, int p, bool want_dt, T* __restrict__ sf) {
+    const int e0 = p * MPS_PATCH;
+    const int ne = min(MPS_PATCH, I.Ne - e0);
+    unsigned long long dmin = INF_BITS;
+    for (int i = threadIdx.x; i < ne; i += BS) {
+        T f[NEN][3];
+        double dte = bits2d(INF_BITS);
+        if (NEN == 8) hex8_element<T, LAW>(I, e0 + i, I.u, (T(*)[3])f, want_dt ? &dte : nullptr);
+        else tet4_element<T, LAW>(I, e0 + i, I.u, (T(*)[3])f, want_dt ? &dte : nullptr);
 #pragma unroll
-    for (int a = 0; a < NEN; a++)
+        for (int a = 0; a < NEN; a++)
 #pragma unroll
-        for (int d = 0; d < 3; d++) dst[a * 3 + d] = f[a][d];
-}
-
-template <typename T, int NEN, int LAW>
-__device__ __forceinline__ double elem_work(const MpsInst& I, int e, const double* u, bool want_dt) {
-    T f[NEN][3];
-    double dte = bits2d(INF_BITS);
-    if (NEN == 8) hex8_element<T, LAW>(I, e, u, (T(*)[3])f, want_dt ? &dte : nullptr);
-    else tet4_element<T, LAW>(I, e, u, (T(*)[3])f, want_dt ? &dte : nullptr);
-    store_fe<T, NEN>(I, e, f);
-    return dte;
+            for (int d = 0; d < 3; d++) sf[(a * 3 + d) * MPS_PATCH + i] = f[a][d];
+        dmin = umin64(dmin, d2bits(dte));
+    }
+    __syncthreads();
+    const int base = I.poff[p];
+    const int U = I.poff[p + 1] - base;
+    const uint16_t* __restrict__ pr = I.prow + base + p;
+    const uint16_t* __restrict__ pc = I.pcol + (size_t)e0 * NEN;
+    for (int j = threadIdx.x; j < U; j += BS) {
+        double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+        const int q1 = pr[j + 1];
+        for (int q = pr[j]; q < q1; q++) {
+            const int c = pc[q];
+            const int i = c / NEN, a = c % NEN;
+            s0 += (double)sf[(a * 3 + 0) * MPS_PATCH + i];
+            s1 += (double)sf[(a * 3 + 1) * MPS_PATCH + i];
+            s2 += (double)sf[(a * 3 + 2) * MPS_PATCH + i];
+        }
+        double* dst = I.part + 3 * (size_t)(base + j);
+        dst[0] = s0;
+        dst[1] = s1;
+        dst[2] = s2;
+    }
+    return dmin;
 }
 
 // mode 0: step (reads ctl->A, writes ctl->B, reduces dt for step k+1)
 // mode 1: standalone force evaluation (no ctl access)
+// One workgroup per patch, one element per work-item.
 template <typename T, int NEN, int LAW>
-__global__ __launch_bounds__(256) void k_elem(MpsInst I, int mode) {
-    __shared__ unsigned long long sh[4];
+__global__ __launch_bounds__(MPS_PATCH) void k_elem(MpsInst I, int mode) {
+    __shared__ T sf[NEN * 3 * MPS_PATCH];
+    __shared__ unsigned long long sh[MPS_PATCH / 64];
     long long k = 0;
     if (mode == 0) {
         k = I.ctl->A;
         if (blockIdx.x == 0 && threadIdx.x == 0) I.ctl->B = k;
         if (!I.ctl->slot[k & 1].go) return;
     }
-    const int e = xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-    double dte = bits2d(INF_BITS);
-    if (e < I.Ne) dte = elem_work<T, NEN, LAW>(I, e, I.u, mode == 0);
+    const int p = xcd_tile(blockIdx.x, gridDim.x);
+    unsigned long long m = patch_forces<T, NEN, LAW, MPS_PATCH>(I, p, mode == 0, sf);
     if (mode == 0) {
-        unsigned long long m = block_min_u64(d2bits(dte), sh);
+        m = block_min_u64(m, sh);
         if (threadIdx.x == 0) atomicMin(&I.ctl->dtmin[(k + 1) & 1], m);
     }
 }
@@ -71,10 +99,12 @@ __device__ __forceinline__ int batch_end(int n) {
 }
 __device__ __forceinline__ int batch_stride() { return (gridDim.x & 7) ? gridDim.x : (gridDim.x >> 3); }
 
-// Persistent batched element kernel: work items are element ranges of one instance.
+// Persistent batched element kernel: a work item is one patch of one instance (begin = its
+// first element); BS work-items share the patch's elements.
 template <typename T, int NEN, int LAW, int BS>
 __global__ __launch_bounds__(BS) void k_elem_batch(const MpsInst* __restrict__ insts,
                                                     const MpsWork* __restrict__ work, const int* __restrict__ nwork_p) {
+    __shared__ T sf[NEN * 3 * MPS_PATCH];
     __shared__ unsigned long long sh[BS / 64 + 1];
     const int nwork = *nwork_p;
     for (int w = batch_first(nwork); w < batch_end(nwork); w += batch_stride()) {
@@ -83,10 +113,8 @@ __global__ __launch_bounds__(BS) void k_elem_batch(const MpsInst* __restrict__ i
         const long long k = I.ctl->A;
         if (W.first && threadIdx.x == 0) I.ctl->B = k;
         if (I.ctl->slot[k & 1].go) {
-            const int e = W.begin + threadIdx.x;
-            double dte = bits2d(INF_BITS);
-            if (e < W.end) dte = elem_work<T, NEN, LAW>(I, e, I.u, true);
-            unsigned long long m = block_min_u64(d2bits(dte), sh);
+            unsigned long long m = patch_forces<T, NEN, LAW, BS>(I, W.begin / MPS_PATCH, true, sf);
+            m = block_min_u64(m, sh);
             if (threadIdx.x == 0) atomicMin(&I.ctl->dtmin[(k + 1) & 1], m);
         }
         __syncthreads();
@@ -104,7 +132,6 @@ __device__ __forceinline__ void write_hist_step(const MpsInst& I, long long k, c
     }
 }
 
-template <typename TF>
 __device__ __forceinline__ void node_item(const MpsInst& I, int n, bool first, int fuse) {
     MpsCtl* ctl = I.ctl;
     const long long k = ctl->B;
@@ -132,7 +159,7 @@ __device__ __forceinline__ void node_item(const MpsInst& I, int n, bool first, i
         v[i] = I.v[i * N + n];
         a[i] = I.a[i * N + n];
     }
-    solve_node<TF>(I, n, s.dt, v, a);
+    solve_node(I, n, s.dt, v, a);
     if (n == I.dof1_node) {
         double x = u[I.dof1_dir];
         if (!isfinite(x)) {
@@ -149,14 +176,13 @@ __device__ __forceinline__ void node_item(const MpsInst& I, int n, bool first, i
     }
 }
 
-template <typename TF>
 __global__ __launch_bounds__(256) void k_node(MpsInst I, int fuse) {
     const int n = xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     const bool first = (n == 0);
-    node_item<TF>(I, n < I.N ? n : -1, first, fuse);
+    node_item(I, n < I.N ? n : -1, first, fuse);
 }
 
-template <typename TF, int BS>
+template <int BS>
 __global__ __launch_bounds__(BS) void k_node_batch(const MpsInst* __restrict__ insts, const MpsWork* __restrict__ work,
                                                     const int* __restrict__ nwork_p, int fuse) {
     const int nwork = *nwork_p;
@@ -165,8 +191,7 @@ __global__ __launch_bounds__(BS) void k_node_batch(const MpsInst* __restrict__ i
         const MpsInst& I = insts[W.inst];
         const int n = W.begin + threadIdx.x;
         const bool first = W.first && threadIdx.x == 0;
-        if (I.precision == 1) node_item<float>(I, n < W.end ? n : -1, first, fuse);
-        else node_item<double>(I, n < W.end ? n : -1, first, fuse);
+        node_item(I, n < W.end ? n : -1, first, fuse);
     }
 }
 
@@ -403,19 +428,17 @@ __global__ __launch_bounds__(256) void k_gather_scalar(MpsInst I, const double* 
     out[n] = acc;
 }
 
-// Internal force for all DOFs (solverExplicit.cpp:363-379), from fe.
-template <typename TF>
+// Internal force for all DOFs (solverExplicit.cpp:363-379), from the patch partials.
 __global__ __launch_bounds__(256) void k_gather_fint(MpsInst I, double* fint) {
 #pragma clang fp contract(off)
     const int n = blockIdx.x * blockDim.x + threadIdx.x;
     if (n >= I.N) return;
-    const TF* fe = (const TF*)I.fe;
     double f0 = 0.0, f1 = 0.0, f2 = 0.0;
-    for (int q = I.rowptr[n]; q < I.rowptr[n + 1]; q++) {
-        size_t c = (size_t)I.col[q] * 3;
-        f0 += (double)fe[c];
-        f1 += (double)fe[c + 1];
-        f2 += (double)fe[c + 2];
+    for (int q = I.nrow[n]; q < I.nrow[n + 1]; q++) {
+        const double* c = I.part + 3 * (size_t)I.ncol[q];
+        f0 += c[0];
+        f1 += c[1];
+        f2 += c[2];
     }
     const size_t N = (size_t)I.N;
     fint[n] = f0;
@@ -542,12 +565,12 @@ namespace mpsk {
 static inline unsigned blocks_for(long long n, int bs) { return (unsigned)((n + bs - 1) / bs); }
 
 hipError_t launch_elem(const MpsInst& I, int mode, hipStream_t st) {
-    const unsigned nb = blocks_for(I.Ne, 256);
+    const unsigned nb = (unsigned)I.n_patch;
     if (nb == 0) return hipSuccess;
 #define MPS_ELEM_LAUNCH(T, NEN)                                                   \
     do {                                                                          \
-        if (I.law == 0) k_elem<T, NEN, 0><<<nb, 256, 0, st>>>(I, mode);           \
-        else k_elem<T, NEN, 1><<<nb, 256, 0, st>>>(I, mode);                      \
+        if (I.law == 0) k_elem<T, NEN, 0><<<nb, MPS_PATCH, 0, st>>>(I, mode);     \
+        else k_elem<T, NEN, 1><<<nb, MPS_PATCH, 0, st>>>(I, mode);                \
     } while (0)
     if (I.nen == 8) {
         if (I.precision == 1) MPS_ELEM_LAUNCH(float, 8);
@@ -561,9 +584,7 @@ hipError_t launch_elem(const MpsInst& I, int mode, hipStream_t st) {
 }
 
 hipError_t launch_node(const MpsInst& I, int fuse, hipStream_t st) {
-    const unsigned nb = blocks_for(I.N, 256);
-    if (I.precision == 1) k_node<float><<<nb, 256, 0, st>>>(I, fuse);
-    else k_node<double><<<nb, 256, 0, st>>>(I, fuse);
+    k_node<<<blocks_for(I.N, 256), 256, 0, st>>>(I, fuse);
     return hipGetLastError();
 }
 
@@ -586,8 +607,7 @@ hipError_t launch_setup(const MpsInst& I, double* mslot, double* l0out, unsigned
 }
 
 hipError_t launch_fint(const MpsInst& I, double* fint, hipStream_t st) {
-    if (I.precision == 1) k_gather_fint<float><<<blocks_for(I.N, 256), 256, 0, st>>>(I, fint);
-    else k_gather_fint<double><<<blocks_for(I.N, 256), 256, 0, st>>>(I, fint);
+    k_gather_fint<<<blocks_for(I.N, 256), 256, 0, st>>>(I, fint);
     return hipGetLastError();
 }
 
@@ -626,8 +646,7 @@ static hipError_t batch_phase_bs(int phase, int prec, int nen, int law, const Mp
         }
 #undef MPS_EB
     } else if (phase == 1) {
-        if (prec == 1) k_node_batch<float, BS><<<grid, BS, 0, st>>>(insts, work, nwork, fuse);
-        else k_node_batch<double, BS><<<grid, BS, 0, st>>>(insts, work, nwork, fuse);
+        k_node_batch<BS><<<grid, BS, 0, st>>>(insts, work, nwork, fuse);
     } else {
         k_begin_batch<BS><<<grid, BS, 0, st>>>(insts, work, nwork);
     }

@@ -28,6 +28,7 @@ from mps_gpu import configs  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
+VALU_ISSUE_PEAK = 256 * 2.4e9  # MI355X: 256 CUs x 2.4 GHz, one wave64 VALU instruction per CU per cycle
 TRAFFIC_TABLE = os.path.join(ROOT, "profiles", "hex8_cube100_hbm_traffic.md")
 
 
@@ -46,24 +47,52 @@ def measured_traffic(kernel):
     return None
 
 
+def measured_valu_per_wave(kernel):
+    """VALU instructions per wave of `kernel` from the committed rocprofv3 SQ_INSTS_VALU/SQ_WAVES table
+    (| `name` | VALU instructions / wave | ...), or None."""
+    try:
+        with open(TRAFFIC_TABLE) as f:
+            for line in f:
+                cells = [c.strip() for c in line.strip().strip("|").split("|")]
+                if len(cells) == 4 and cells[0] == f"`{kernel}`":
+                    return float(cells[1].replace(",", ""))
+    except OSError:
+        pass
+    return None
+
+
 def note(msg):
     """Progress line on stderr (the JSON result is the only stdout line)."""
     print(f"bench: {msg}", file=sys.stderr, flush=True)
 
 
-def hex_elem_kernel_bytes(ne, nn, prec, per_elem_mat):
-    """Algorithmic HBM bytes of one element-kernel launch (hex8): conn 8x4 B + l0 8 B (+ material
-    24 B when per element) + element forces 24 x p_e, per element; X and u 3x8 B each, per node."""
-    pe = 4 if prec == G.PREC_FP32_ELEM else 8
-    return ne * (32 + 8 + (24 if per_elem_mat else 0) + 24 * pe) + nn * (24 + 24)
+PATCH = 256  # elements per patch of the element kernel (MPS_PATCH)
 
 
-def step_bytes(ne, nn, nen, prec, per_elem_mat):
+def patch_partials(mesh):
+    """Number of partial nodal forces: distinct nodes per patch of PATCH consecutive elements, summed."""
+    conn = mesh["conn"].reshape(mesh["n_elem"], mesh["nen"])
+    return int(sum(np.unique(conn[s:s + PATCH]).size for s in range(0, mesh["n_elem"], PATCH)))
+
+
+def elem_kernel_bytes(ne, nn, nen, n_part, per_elem_mat):
+    """Algorithmic HBM bytes of one element-kernel launch: per element conn 4 nen + l0 8 (+ material
+    24 when per element) + patch contribution list 2 nen; per partial its offset 2 + FP64 force write 24;
+    per node X and u, 24 B each (read once)."""
+    n_patch = -(-ne // PATCH)
+    return ne * (4 * nen + 8 + (24 if per_elem_mat else 0) + 2 * nen) + n_part * (2 + 24) + 4 * (n_patch + 1) \
+        + nn * (24 + 24)
+
+
+def node_kernel_bytes(nn, n_part):
+    """Algorithmic HBM bytes of one node-kernel launch: per node CSR offset 4, BC info 4, u/v/a read 72 and
+    write 72, mass 8; per partial its index 4 and FP64 force read 24."""
+    return nn * (4 + 4 + 72 + 72 + 8) + n_part * (4 + 24)
+
+
+def step_bytes(ne, nn, nen, n_part, per_elem_mat):
     """Algorithmic HBM bytes of one whole step (element + node kernels)."""
-    pe = 4 if prec == G.PREC_FP32_ELEM else 8
-    elem = ne * (4 * nen + 8 + (24 if per_elem_mat else 0) + 3 * nen * pe) + nn * 48
-    node = ne * nen * (4 + 3 * pe) + nn * (4 + 4 + 72 + 72 + 8)  # CSR col+fe read; rowptr, ninfo, u/v/a rw, mass
-    return elem + node
+    return elem_kernel_bytes(ne, nn, nen, n_part, per_elem_mat) + node_kernel_bytes(nn, n_part)
 
 
 def run_gpu(cfg, precision, warmup, steps):
@@ -236,8 +265,16 @@ def main():
     nms, nn_l = r64["kt"]["node"]
     elem_avg_ms = ems / max(en, 1)
     node_avg_ms = nms / max(nn_l, 1)
-    eb = hex_elem_kernel_bytes(ne, nn, G.PREC_FP64, False)
+    n_part = patch_partials(m)
+    eb = elem_kernel_bytes(ne, nn, 8, n_part, False)
+    sb = step_bytes(ne, nn, 8, n_part, False)
     achieved = eb / (elem_avg_ms * 1e-3) / 1e9
+    valu_wave = measured_valu_per_wave("k_elem<double,8,0>")
+    n_waves = -(-ne // PATCH) * (PATCH // 64)
+    valu_peak = VALU_ISSUE_PEAK  # wave64 VALU instructions per second (one per CU per cycle)
+    valu = None if valu_wave is None else {
+        "achieved": valu_wave * n_waves / (elem_avg_ms * 1e-3), "peak": valu_peak, "unit": "wave-instr/s",
+        "frac": valu_wave * n_waves / (elem_avg_ms * 1e-3) / valu_peak, "instr_per_element": valu_wave}
     out = {
         "metric": "elem-GP updates/s (1M hex8 NH explicit) + batched solves/s, p99 latency",
         "value": value,
@@ -260,11 +297,13 @@ def main():
                      "traffic_source": os.path.relpath(TRAFFIC_TABLE, ROOT),
                      "kernel": "k_elem<double,8,0> (hex8 element forces + stable dt)",
                      "kernel_avg_ms": elem_avg_ms, "algorithmic_bytes_per_launch": eb,
-                     "measured_copy_ceiling_gbs": copy_gbs},
-        "kernels": {"element_avg_ms": elem_avg_ms, "node_avg_ms": node_avg_ms,
-                    "step_algorithmic_bytes": step_bytes(ne, nn, 8, G.PREC_FP64, False),
-                    "step_hbm_frac": step_bytes(ne, nn, 8, G.PREC_FP64, False) / (r64["wall_s"] / args.steps)
-                    / 1e9 / HBM_PEAK_GBS},
+                     "measured_copy_ceiling_gbs": copy_gbs,
+                     "valu_check": valu},
+        "kernels": {"element_avg_ms": elem_avg_ms, "node_avg_ms": node_avg_ms, "partials": n_part,
+                    "node_algorithmic_bytes": node_kernel_bytes(nn, n_part),
+                    "node_achieved_gbs": node_kernel_bytes(nn, n_part) / (node_avg_ms * 1e-3) / 1e9,
+                    "step_algorithmic_bytes": sb,
+                    "step_hbm_frac": sb / (r64["wall_s"] / args.steps) / 1e9 / HBM_PEAK_GBS},
     }
     if not args.no_fp32:
         note("FP32-element leg")

@@ -102,6 +102,20 @@ def test_perturbed_hex_fp64_and_fp32(law):
     assert rc2 == 0 and err <= TOL32
 
 
+@pytest.mark.parametrize("precision", [G.PREC_FP64, G.PREC_FP32_ELEM])
+def test_mixed_orientation_hex(precision):
+    """Half of the elements (random) use a mirrored local numbering (xi reversed), so det J < 0 there
+    and waves hold both signs; the reference integrates with w|det J| (classGPs.cpp:26-27)."""
+    m = perturb(box(9, 8, 7), 0.1, 21)
+    conn = m["conn"].reshape(-1, 8).copy()
+    flip = np.random.default_rng(3).random(conn.shape[0]) < 0.5
+    conn[flip] = conn[flip][:, [1, 0, 3, 2, 5, 4, 7, 6]]
+    m = dict(m, conn=np.ascontiguousarray(conn.ravel()))
+    ro, (rc, rep, u, *_r) = run_pair(m, 150, precision=precision)
+    tol = TOL64 if precision == G.PREC_FP64 else TOL32
+    assert rc == 0 and relerr(u, ro["u"]) <= tol, relerr(u, ro["u"])
+
+
 def test_tet4_heterogeneous_fp64_fp32():
     m = perturb(box(6, 6, 6, tet=True), 0.15, 2)
     rng = np.random.default_rng(3)

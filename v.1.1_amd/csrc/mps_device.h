@@ -469,13 +469,19 @@ __device__ __forceinline__ void hex8_element(const MpsInst& I, int e, const doub
             for (int j = 0; j < 3; j++)
                 H[i * 3 + j] = (Du[i * 3] * cJ[j * 3] + Du[i * 3 + 1] * cJ[j * 3 + 1] + Du[i * 3 + 2] * cJ[j * 3 + 2]) * rJ;
         pk1<LAW>(H, tpa, tpb, P);
-        const T sg = dJ < T(0) ? T(-1) : T(1);
-        // Q = wJ P J^-T: Q(i,k) = sign(det J) sum_J P(i,J) cof(J)(J,k); columns Q(:,k)
+        // Q = wJ P J^-T: Q(i,k) = sign(det J) sum_J P(i,J) cof(J)(J,k); columns Q(:,k).  The sign is
+        // folded into P only when some lane of the wave has det J <= 0 (a positively oriented
+        // mesh never takes that branch).
+        if (!__all(dJ > T(0))) {
+            const T sg = dJ < T(0) ? T(-1) : T(1);
+#pragma unroll
+            for (int i = 0; i < 9; i++) P[i] *= sg;
+        }
         T Q[3][3];
 #pragma unroll
         for (int i = 0; i < 3; i++)
 #pragma unroll
-            for (int k = 0; k < 3; k++) Q[k][i] = sg * (P[i * 3] * cJ[k] + P[i * 3 + 1] * cJ[3 + k] + P[i * 3 + 2] * cJ[6 + k]);
+            for (int k = 0; k < 3; k++) Q[k][i] = P[i * 3] * cJ[k] + P[i * 3 + 1] * cJ[3 + k] + P[i * 3 + 2] * cJ[6 + k];
 #pragma unroll
         for (int i = 0; i < 3; i++) {
             G[0][i] += Q[0][i];

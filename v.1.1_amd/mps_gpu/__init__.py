@@ -78,11 +78,11 @@ _lib = None
 
 
 def load(path=None):
-    """Load libmps_gpu.so (raises OSError if it is not built)."""
+    """Load libmps_gpu.so (raises OSError if it is not built).  MPS_GPU_LIB overrides the path."""
     global _lib
     if _lib is not None:
         return _lib
-    p = path or LIB_PATH
+    p = path or os.environ.get("MPS_GPU_LIB") or LIB_PATH
     if not os.path.exists(p):
         raise OSError(f"libmps_gpu.so not built: {p} missing (run __graft_entry__.build())")
     L = C.CDLL(p)

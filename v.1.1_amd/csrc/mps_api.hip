@@ -356,6 +356,8 @@ static int upload_instance(mps_solver_s* h, const HostPrep& hp) {
     HCHK(h, dalloc(h, &nrow, (N + 1) * 4));
     HCHK(h, dalloc(h, &ncol, NP * 4));
     HCHK(h, dalloc(h, &ctl, sizeof(MpsCtl)));
+    unsigned long long* dtred;
+    HCHK(h, dalloc(h, &dtred, 2 * MPS_DT_SLOTS * MPS_DT_STRIDE * 8));
     HCHK(h, dalloc(h, &hist, 5 * (size_t)cap * 8));
     h->energy_nb = (int)((Ne + 255) / 256);
     HCHK(h, dalloc(h, &part, 3 * (size_t)h->energy_nb * 8));
@@ -405,6 +407,7 @@ static int upload_instance(mps_solver_s* h, const HostPrep& hp) {
     I.nrow = nrow;
     I.ncol = ncol;
     I.part = pforce;
+    I.dtred = dtred;
     I.n_patch = hp.n_patch;
     I.n_part = hp.n_part;
     I.ctl = ctl;
@@ -432,8 +435,6 @@ static int device_setup(mps_solver_s* h) {
     c.slot[0].dt = dt0 * I.scale;  // solverExplicit.cpp:165-166
     c.slot[0].go = 1;
     c.slot[1].go = 0;
-    c.dtmin[0] = inf;
-    c.dtmin[1] = inf;
     c.A = 1;
     c.B = 0;
     c.status = MPS_RUNNING;
@@ -441,6 +442,9 @@ static int device_setup(mps_solver_s* h) {
     c.nan_step = LLONG_MAX;
     c.dt0min = bits;
     HCHK(h, hipMemcpy(I.ctl, &c, sizeof(c), hipMemcpyHostToDevice));
+    hipError_t ef = mpsk::launch_fill_u64(I.dtred, 2 * MPS_DT_SLOTS * MPS_DT_STRIDE, inf, h->stream);
+    if (ef == hipSuccess) ef = hipStreamSynchronize(h->stream);
+    if (ef != hipSuccess) return h->fail(MPS_ERR_HIP, fmt("dt slots: %s", hipGetErrorString(ef)));
     return MPS_OK;
 }
 
@@ -671,7 +675,7 @@ extern "C" int mps_run(mps_handle h, int64_t max_steps, mps_report* rep) {
     if (!h->primed) {
         // dt of step 1 uses uK = u0 only when t0 < 1e-12 (solverExplicit.cpp:109,176-178)
         const double* uK = (h->I.t0 < 1E-12) ? h->I.u : nullptr;
-        hipError_t e = mpsk::launch_dt(h->I, uK, &h->I.ctl->dtmin[1], h->stream);
+        hipError_t e = mpsk::launch_dt(h->I, uK, h->I.dtred + MPS_DT_SLOTS * MPS_DT_STRIDE, h->stream);  // parity 1, slot 0
         if (e != hipSuccess) return h->fail(MPS_ERR_HIP, fmt("first dt: %s", hipGetErrorString(e)));
         h->primed = true;
     }

@@ -127,6 +127,7 @@ size_t mps_instance_bytes(const HostPrep& hp, int precision) {
     b += align_up(Ne * nen * 2);                  // pcol
     b += align_up(NP * 4);                        // ncol
     b += align_up(std::max<size_t>(nrec, 1) * 3 * 8);  // bcval
+    b += align_up(2 * MPS_DT_SLOTS * MPS_DT_STRIDE * 8);  // dt reduction slots
     if (hp.per_elem) b += align_up(2 * Ne * 8) + align_up(Ne * 8) * 2;  // mat, cel, rho
     return b;
 }
@@ -344,6 +345,7 @@ int mps_batch_add_prepared(mps_batch b, const HostPrep& hp, int32_t* slot_out) {
     I.ncol = (int32_t*)take(NP * 4);
     I.n_patch = hp.n_patch;
     I.n_part = hp.n_part;
+    I.dtred = (unsigned long long*)take(2 * MPS_DT_SLOTS * MPS_DT_STRIDE * 8);
     I.bcval = (double*)take(hp.bcval.size() * 8);
     if (hp.per_elem) {
         I.mat = (double*)take(2 * Ne * 8);
@@ -384,11 +386,12 @@ int mps_batch_add_prepared(mps_batch b, const HostPrep& hp, int32_t* slot_out) {
     if (he == hipSuccess) he = hipStreamSynchronize(b->up);
     // set-up kernels on the compute stream, after whatever is queued there
     unsigned long long* dt0 = &b->ctl[slot].dt0min;
-    if (he == hipSuccess) he = mpsk::launch_fill_u64(dt0, 0x7FF0000000000000ull, b->stream);
+    if (he == hipSuccess) he = mpsk::launch_fill_u64(dt0, 1, 0x7FF0000000000000ull, b->stream);
+    if (he == hipSuccess) he = mpsk::launch_fill_u64(I.dtred, 2 * MPS_DT_SLOTS * MPS_DT_STRIDE, 0x7FF0000000000000ull, b->stream);
     if (he == hipSuccess) he = mpsk::launch_setup(I, I.part, (double*)I.l0, dt0, b->stream);
     if (he == hipSuccess) he = mpsk::launch_init_ctl(I, dt0, b->stream);
     // dt of step 1 from u0 = 0 (solverExplicit.cpp:176-178)
-    if (he == hipSuccess) he = mpsk::launch_dt(I, I.t0 < 1E-12 ? I.u : nullptr, &I.ctl->dtmin[1], b->stream);
+    if (he == hipSuccess) he = mpsk::launch_dt(I, I.t0 < 1E-12 ? I.u : nullptr, I.dtred + MPS_DT_SLOTS * MPS_DT_STRIDE, b->stream);
     if (he != hipSuccess) {
         b->alloc.release(off, bytes);
         return b->fail(MPS_ERR_HIP, fmt("admission: %s", hipGetErrorString(he)));

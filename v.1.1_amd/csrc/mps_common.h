@@ -19,12 +19,15 @@
 //     nrow/ncol     CSR node -> partials, ascending patch (= ascending element)
 //   part            double[3*n_part]       partial nodal forces [(g)*3 + dir]; also the
 //                                          set-up scratch (>= nen*Ne doubles)
-//   ctl             MpsCtl       step control (time, dt, go flags, dt reduction slots)
+//   ctl             MpsCtl       step control (time, dt, go flags)
+//   dtred           u64[2*MPS_DT_SLOTS*MPS_DT_STRIDE]  stable-dt reduction slots by step parity
 #pragma once
 #include <stdint.h>
 
 #define MPS_MAX_NEN 8
 #define MPS_PATCH 256  // elements per patch (element-kernel work unit and partial-sum scope)
+#define MPS_DT_SLOTS 16   // stable-dt reduction slots per step parity: workgroup b min-reduces into slot b % 16
+#define MPS_DT_STRIDE 32  // u64 between two slots (256 B), so the slots' atomics do not share a line
 
 enum : int32_t { MPS_TIME_WINDOW = 0, MPS_TIME_FIXED = 1 };
 
@@ -39,11 +42,10 @@ struct MpsStepSlot {
 // Step protocol (all in device memory, no host round trip per step):
 //   A = next step to run, written only by the node kernel, read by begin/element kernels;
 //   B = step whose forces are in part, written only by the element kernel, read by node/energy.
-// slot[k&1] holds step k; dtmin[k&1] receives the min over elements for step k
-// (computed from u_{k-1}); the node kernel of step k resets dtmin[k&1].
+// slot[k&1] holds step k; the dtred slots of parity k&1 receive the min over elements for
+// step k (computed from u_{k-1}); the node kernel of step k resets them.
 struct MpsCtl {
     MpsStepSlot slot[2];
-    unsigned long long dtmin[2];  // bit patterns of non-negative doubles (integer min)
     long long A;
     long long B;
     int32_t status;     // MPS_RUNNING / MPS_DONE / MPS_ERR_DT
@@ -78,6 +80,7 @@ struct MpsInst {
     const int32_t* nrow;
     const int32_t* ncol;
     double* part;
+    unsigned long long* dtred;  // dt reduction slots (bit patterns of non-negative doubles, integer min)
     MpsCtl* ctl;
     double* hist;      // [5*hist_cap]: time, dt, e_def, e_kin, volume
     double* partials;  // energy partials [3*n_blocks]

@@ -702,6 +702,24 @@ __device__ __forceinline__ int xcd_tile(int b, int nb) {
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ unsigned long long umin64(unsigned long long a, unsigned long long b) { return b < a ? b : a; }
 
+// Stable-dt reduction slots (MpsInst::dtred): workgroups atomicMin into one of MPS_DT_SLOTS slots of the
+// step parity, 256 B apart, so that thousands of workgroups do not queue on one address.
+__device__ __forceinline__ unsigned long long* dt_slot(const MpsInst& I, int parity, int s) {
+    return I.dtred + (size_t)(parity * MPS_DT_SLOTS + s) * MPS_DT_STRIDE;
+}
+// Min over the slots of one parity, returned in every lane; all 64 lanes of the wave must call it.
+__device__ __forceinline__ unsigned long long dt_read_min(const MpsInst& I, int parity) {
+    unsigned long long v = *dt_slot(I, parity, threadIdx.x & (MPS_DT_SLOTS - 1));
+#pragma unroll
+    for (int off = MPS_DT_SLOTS / 2; off > 0; off >>= 1)
+        v = umin64(v, (unsigned long long)__shfl_xor((long long)v, off, 64));
+    return v;
+}
+__device__ __forceinline__ void dt_reset(const MpsInst& I, int parity) {
+#pragma unroll
+    for (int s = 0; s < MPS_DT_SLOTS; s++) *dt_slot(I, parity, s) = 0x7FF0000000000000ull;
+}
+
 // Block-wide min of 64-bit patterns; result valid in thread 0.
 __device__ __forceinline__ unsigned long long block_min_u64(unsigned long long v, unsigned long long* sh) {
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;

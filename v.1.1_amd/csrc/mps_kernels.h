@@ -21,5 +21,6 @@ hipError_t launch_init_ctl(const MpsInst& I, const unsigned long long* dt0, hipS
 // nwork points to the item count in device memory (graphs stay valid when tables change).
 hipError_t launch_batch_phase(int phase, int bs, int prec, int nen, int law, const MpsInst* insts, const MpsWork* work,
                               const int* nwork, int grid, int fuse, hipStream_t st);
-hipError_t launch_fill_u64(unsigned long long* p, unsigned long long v, hipStream_t st);
+// Fill n 64-bit words with v (device kernel: no memset node in captured graphs).
+hipError_t launch_fill_u64(unsigned long long* p, long long n, unsigned long long v, hipStream_t st);
 }  // namespace mpsk

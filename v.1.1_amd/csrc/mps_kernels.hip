@@ -80,7 +80,7 @@ __global__ __launch_bounds__(MPS_PATCH) void k_elem(MpsInst I, int mode) {
     unsigned long long m = patch_forces<T, NEN, LAW, MPS_PATCH>(I, p, mode == 0, sf);
     if (mode == 0) {
         m = block_min_u64(m, sh);
-        if (threadIdx.x == 0) atomicMin(&I.ctl->dtmin[(k + 1) & 1], m);
+        if (threadIdx.x == 0) atomicMin(dt_slot(I, (k + 1) & 1, blockIdx.x & (MPS_DT_SLOTS - 1)), m);
     }
 }
 
@@ -115,7 +115,7 @@ __global__ __launch_bounds__(BS) void k_elem_batch(const MpsInst* __restrict__ i
         if (I.ctl->slot[k & 1].go) {
             unsigned long long m = patch_forces<T, NEN, LAW, BS>(I, W.begin / MPS_PATCH, true, sf);
             m = block_min_u64(m, sh);
-            if (threadIdx.x == 0) atomicMin(&I.ctl->dtmin[(k + 1) & 1], m);
+            if (threadIdx.x == 0) atomicMin(dt_slot(I, (k + 1) & 1, w & (MPS_DT_SLOTS - 1)), m);
         }
         __syncthreads();
     }
@@ -140,10 +140,10 @@ __device__ __forceinline__ void node_item(const MpsInst& I, int n, bool first, i
     MpsStepSlot nx;
     nx.go = 0;
     int st = MPSD_RUNNING;
-    if (fuse) st = begin_scalars(I, k + 1, s, ctl->dtmin[(k + 1) & 1], &nx);
+    if (fuse) st = begin_scalars(I, k + 1, s, dt_read_min(I, (k + 1) & 1), &nx);
     if (first) {
         ctl->A = k + 1;
-        ctl->dtmin[k & 1] = INF_BITS;
+        dt_reset(I, k & 1);
         if (fuse) {
             ctl->slot[(k + 1) & 1] = nx;
             if (st != MPSD_RUNNING) ctl->status = st;
@@ -203,7 +203,7 @@ __device__ __forceinline__ void begin_item(const MpsInst& I, int n, bool first) 
     const long long k = ctl->A;
     const MpsStepSlot prev = ctl->slot[(k - 1) & 1];
     MpsStepSlot nx;
-    const int st = begin_scalars(I, k, prev, ctl->dtmin[k & 1], &nx);
+    const int st = begin_scalars(I, k, prev, dt_read_min(I, k & 1), &nx);
     if (first) {
         ctl->slot[k & 1] = nx;
         if (st != MPSD_RUNNING) ctl->status = st;
@@ -394,22 +394,20 @@ __global__ __launch_bounds__(256) void k_setup_elem(MpsInst I, double* __restric
 
 // Initial control block of a solve from the set-up reduction (device side, so that admitting
 // a solve into a running batch needs no host round trip): slot[0] = {t0, s*min(l0/c), go}.
-__global__ void k_fill_u64(unsigned long long* p, unsigned long long v) {
-    if (blockIdx.x == 0 && threadIdx.x == 0) *p = v;
+__global__ void k_fill_u64(unsigned long long* p, long long n, unsigned long long v) {
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+        p[i] = v;
 }
 
 __global__ void k_init_ctl(MpsInst I, const unsigned long long* dt0) {
     if (blockIdx.x != 0 || threadIdx.x != 0) return;
     MpsCtl* c = I.ctl;
-    const unsigned long long inf = INF_BITS;
     c->slot[0].time = I.t0;
     c->slot[0].dt = bits2d(*dt0) * I.scale;  // solverExplicit.cpp:165-166
     c->slot[0].go = 1;
     c->slot[1].time = I.t0;
     c->slot[1].dt = 0.0;
     c->slot[1].go = 0;
-    c->dtmin[0] = inf;
-    c->dtmin[1] = inf;
     c->A = 1;
     c->B = 0;
     c->status = MPSD_RUNNING;
@@ -613,8 +611,8 @@ hipError_t launch_fint(const MpsInst& I, double* fint, hipStream_t st) {
 
 int energy_blocks(const MpsInst& I) { return (int)blocks_for(I.Ne, 256); }
 
-hipError_t launch_fill_u64(unsigned long long* p, unsigned long long v, hipStream_t st) {
-    k_fill_u64<<<1, 64, 0, st>>>(p, v);
+hipError_t launch_fill_u64(unsigned long long* p, long long n, unsigned long long v, hipStream_t st) {
+    k_fill_u64<<<(unsigned)std::min<long long>((n + 255) / 256, 1024), 256, 0, st>>>(p, n, v);
     return hipGetLastError();
 }
 

@@ -42,6 +42,8 @@ extern "C" {
 /* ---- constitutive laws (constitutiveList.h; inputFile.cpp:1008-1014) ---- */
 #define MPS_LAW_NH_COMPRESSIBLE 0 /* "*HyperElastic, 3D-Compressible-Neo-Hookean" */
 #define MPS_LAW_NH_BELYTSCHKO 1   /* "*HyperElastic, Neo-Hookean" */
+#define MPS_LAW_SVK 2             /* "*HyperElastic, St-Venant-Kirchhoff" (inputFile.cpp:1016-1020) */
+#define MPS_LAW_LINEAR_ELASTIC 3  /* "*Linear-Elastic" (inputFile.cpp:886-928) */
 
 /* ---- Dirichlet modes ---- */
 #define MPS_BC_RAMP 0     /* *BOUNDARY, TYPE=DISPLACEMENT RAMP (classNodes.cpp:66-74) */

@@ -338,12 +338,72 @@ static void pk1_belytschko(const double *F, double mu, double K1, double *P)
         }
 }
 
+/* PK2 = mu2 E + lam tr(E) I (hyperElasticStVenantKirchhoff.cpp / linearElastic.cpp stress(), 3-D branch):
+ * PK2 = E; multiSTensorScalar(PK2, mu2); PK2_ii += lam*traceE.  mu2 = E/(1+nu) equals 2*mu bitwise
+ * (mu = 0.5*E/(1+nu) differs by an exact power-of-two scaling); lam is the NH lambda expression. */
+static void stress_lin(const double *E, double lambda, double mu, double *S)
+{
+    double mu2 = 2.0 * mu, trE = 0;
+    for (int i = 0; i < 3; i++) trE += E[i * 3 + i];
+    for (int i = 0; i < 9; i++) S[i] = E[i] * mu2;
+    for (int i = 0; i < 3; i++) S[i * 3 + i] += lambda * trE;
+}
+
+/* Green-Lagrange strain: computeGL (maths.cpp:1227-1240) with multSTensor3FirstTranspose
+ * (maths.cpp:1760-1772, accumulating into a zeroed C). */
+static void green_lagrange(const double *F, double *E)
+{
+    double C[9] = {0};
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++)
+            for (int k = 0; k < 3; k++) C[i * 3 + j] += F[k * 3 + i] * F[k * 3 + j];
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) E[i * 3 + j] = 0.5 * (C[i * 3 + j] - (i == j ? 1.0 : 0.0));
+}
+
+/* classHyperElasticStVenantKirchhoff::updateConstitutive (no tangent): hyperElasticStVenantKirchhoff.cpp:203-226;
+ * PK1 = F PK2 by multTensorTensor3 (maths.cpp:1703-1715) into the PK1 zeroed by startFromOtherState. */
+static void pk1_svk(const double *F, double lambda, double mu, double *P)
+{
+    double E[9], S[9];
+    green_lagrange(F, E);
+    stress_lin(E, lambda, mu, S);
+    for (int i = 0; i < 9; i++) P[i] = 0;
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++)
+            for (int k = 0; k < 3; k++) P[i * 3 + j] += F[i * 3 + k] * S[k * 3 + j];
+}
+
+/* linearElastic::updateConstitutive: linearElastic.cpp:63-95, E = sym(F) - I, PK1 = stress(E). */
+static void pk1_linear(const double *F, double lambda, double mu, double *P)
+{
+    double E[9];
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) E[i * 3 + j] = 0.5 * (F[i * 3 + j] + F[j * 3 + i] - 2. * (i == j ? 1.0 : 0.0));
+    stress_lin(E, lambda, mu, P);
+}
+
+static void pk1_law(int32_t law, const double *F, double lambda, double mu, double K1, double *P)
+{
+    if (law == ORC_LAW_NH_BELYTSCHKO) pk1_belytschko(F, mu, K1, P);
+    else if (law == ORC_LAW_SVK) pk1_svk(F, lambda, mu, P);
+    else if (law == ORC_LAW_LINEAR) pk1_linear(F, lambda, mu, P);
+    else pk1_compressible(F, lambda, mu, P);
+}
+
 void orc_pk1(int32_t law, const double *F, double E, double nu, double *P)
 {
     double lambda, mu, K1;
     orc_material(E, nu, &lambda, &mu, &K1);
-    if (law == ORC_LAW_NH_BELYTSCHKO) pk1_belytschko(F, mu, K1, P);
-    else pk1_compressible(F, lambda, mu, P);
+    pk1_law(law, F, lambda, mu, K1, P);
+}
+
+/* soundSpeed: NH hyperElasticNeoHookean.cpp:51-54, 202-205; St-Venant-Kirchhoff
+ * hyperElasticStVenantKirchhoff.cpp (sqrt(E/rho)); linearElastic.cpp:49-54. */
+double orc_law_sound_speed(int32_t law, double E, double nu, double rho)
+{
+    if (law == ORC_LAW_SVK || law == ORC_LAW_LINEAR) return sqrt(E / rho);
+    return orc_sound_speed(E, nu, rho);
 }
 
 /* classHyperElastic3DCompressibleNeoHookean::defoEnergy: hyperElasticNeoHookean.cpp:291-304.
@@ -351,6 +411,20 @@ void orc_pk1(int32_t law, const double *F, double E, double nu, double *P)
 static double defo_energy_lm(int32_t law, const double *F, double lambda, double mu)
 {
     if (law == ORC_LAW_NH_BELYTSCHKO) return 0.0;
+    if (law == ORC_LAW_SVK) {  /* hyperElasticStVenantKirchhoff.cpp:251-274: 0.5 PK2:E */
+        double E[9], S[9], w = 0;
+        green_lagrange(F, E);
+        stress_lin(E, lambda, mu, S);
+        for (int i = 0; i < 9; i++) w += S[i] * E[i];
+        return 0.5 * w;
+    }
+    if (law == ORC_LAW_LINEAR) {  /* linearElastic.cpp:97-110: 0.5 (P:F - P:I) */
+        double P[9], pf = 0, pd = 0;
+        pk1_linear(F, lambda, mu, P);
+        for (int i = 0; i < 9; i++) pf += P[i] * F[i];
+        for (int i = 0; i < 3; i++) pd += P[i * 3 + i];
+        return 0.5 * (pf - pd);
+    }
     double J = orc_det3(F);
     double trC = 0;
     for (int i = 0; i < 3; i++) {
@@ -484,7 +558,8 @@ static int state_init(orc_state *s, const orc_problem *p)
         double E = mat_at(p->E, p->mat_per_elem, (int)e), nu = mat_at(p->nu, p->mat_per_elem, (int)e);
         s->rho[e] = mat_at(p->rho, p->mat_per_elem, (int)e);
         orc_material(E, nu, s->lam + e, s->mu + e, s->K1 + e);
-        s->c[e] = sqrt((s->lam[e] + 2 * s->mu[e]) / s->rho[e]);
+        s->c[e] = (p->law == ORC_LAW_SVK || p->law == ORC_LAW_LINEAR) ? sqrt(E / s->rho[e])
+                                                                      : sqrt((s->lam[e] + 2 * s->mu[e]) / s->rho[e]);
     }
     /* active DOF map (dispBCManagement, inputFile.cpp:3185-3212) */
     memset(s->active, 1, 3 * N);
@@ -623,8 +698,7 @@ static void element_force(orc_state *s, size_t e, const double *u)
     for (size_t g = 0; g < ng; g++) {
         size_t gi = e * ng + g;
         orc_kinematics(s->nen, s->dphi + gi * nen * 3, uloc, F);
-        if (s->p->law == ORC_LAW_NH_BELYTSCHKO) pk1_belytschko(F, s->mu[e], s->K1[e], P);
-        else pk1_compressible(F, s->lam[e], s->mu[e], P);
+        pk1_law(s->p->law, F, s->lam[e], s->mu[e], s->K1[e], P);
         orc_gp_force(s->nen, s->wJ[gi], P, s->dphi + gi * nen * 3, f);
         for (size_t a = 0; a < nen; a++)
             for (int i = 0; i < 3; i++) fe[a * 3 + i] += f[a + i * nen];
@@ -666,8 +740,7 @@ static void internal_force(orc_state *s, const double *u)
             for (size_t g = 0; g < ng; g++) {
                 size_t gi = e * ng + g;
                 orc_kinematics(s->nen, s->dphi + gi * nen * 3, uloc, F);
-                if (s->p->law == ORC_LAW_NH_BELYTSCHKO) pk1_belytschko(F, s->mu[e], s->K1[e], P);
-                else pk1_compressible(F, s->lam[e], s->mu[e], P);
+                pk1_law(s->p->law, F, s->lam[e], s->mu[e], s->K1[e], P);
                 orc_gp_force(s->nen, s->wJ[gi], P, s->dphi + gi * nen * 3, f);
                 /* classGPs::getLocalDofs (classGPs.cpp:37-67): vDofs[a + n*i] = node_a + N*i */
                 for (int i = 0; i < 3; i++)
@@ -769,6 +842,7 @@ static int setup(orc_state *s, const orc_problem *p)
 {
     if (!p || (p->nen != 8 && p->nen != 4) || p->n_nodes <= 0 || p->n_elem <= 0) return ORC_ERR_ARG;
     if (p->n_threads > 1 && p->sum_order != ORC_SUM_ELEMENT) return ORC_ERR_ARG;
+    if (p->law < ORC_LAW_NH_COMPRESSIBLE || p->law > ORC_LAW_LINEAR) return ORC_ERR_ARG;
     int rc = state_init(s, p);
     if (rc) return rc;
     s->nt = p->n_threads > 1 ? p->n_threads : 1;

@@ -26,7 +26,7 @@
 extern "C" {
 #endif
 
-enum { ORC_LAW_NH_COMPRESSIBLE = 0, ORC_LAW_NH_BELYTSCHKO = 1 };
+enum { ORC_LAW_NH_COMPRESSIBLE = 0, ORC_LAW_NH_BELYTSCHKO = 1, ORC_LAW_SVK = 2, ORC_LAW_LINEAR = 3 };
 enum { ORC_BC_RAMP = 0, ORC_BC_INST = 1, ORC_BC_VELOCITY = 2 };
 enum { ORC_SUM_REFERENCE = 0, ORC_SUM_ELEMENT = 1 };
 enum { ORC_OK = 0, ORC_ERR_DT = -1, ORC_ERR_NAN = -2, ORC_ERR_ARG = -3, ORC_ERR_MEM = -4 };
@@ -85,6 +85,8 @@ double orc_area_triangle(const double *tri);
 double orc_char_length(int32_t nen, const double *xyz);
 void orc_material(double E, double nu, double *lambda, double *mu, double *K1);
 double orc_sound_speed(double E, double nu, double rho);
+/* Sound speed of any law: NH sqrt((lambda+2mu)/rho), St-Venant-Kirchhoff and linear elastic sqrt(E/rho). */
+double orc_law_sound_speed(int32_t law, double E, double nu, double rho);
 void orc_pk1(int32_t law, const double *F, double E, double nu, double *P);
 double orc_defo_energy(int32_t law, const double *F, double E, double nu);
 void orc_kinematics(int32_t nen, const double *dphi, const double *uloc, double *F);

@@ -12,6 +12,8 @@ LIB_PATH = os.environ.get("MPS_ORACLE_LIB", os.path.join(ROOT, "oracle", "lib", 
 
 LAW_NH_COMPRESSIBLE = 0
 LAW_NH_BELYTSCHKO = 1
+LAW_SVK = 2
+LAW_LINEAR = 3
 BC_RAMP, BC_INST, BC_VELOCITY = 0, 1, 2
 SUM_REFERENCE, SUM_ELEMENT = 0, 1
 ORC_OK, ORC_ERR_DT, ORC_ERR_NAN = 0, -1, -2
@@ -63,6 +65,8 @@ def lib():
         L.orc_material.argtypes = [C.c_double, C.c_double, _dp, _dp, _dp]
         L.orc_sound_speed.restype = C.c_double
         L.orc_sound_speed.argtypes = [C.c_double] * 3
+        L.orc_law_sound_speed.restype = C.c_double
+        L.orc_law_sound_speed.argtypes = [C.c_int32, C.c_double, C.c_double, C.c_double]
         L.orc_pk1.argtypes = [C.c_int32, _dp, C.c_double, C.c_double, _dp]
         L.orc_defo_energy.restype = C.c_double
         L.orc_defo_energy.argtypes = [C.c_int32, _dp, C.c_double, C.c_double]

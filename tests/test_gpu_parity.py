@@ -102,6 +102,19 @@ def test_perturbed_hex_fp64_and_fp32(law):
     assert rc2 == 0 and err <= TOL32
 
 
+@pytest.mark.parametrize("law", [G.LAW_SVK, G.LAW_LINEAR_ELASTIC])
+@pytest.mark.parametrize("tet", [False, True])
+def test_svk_and_linear_elastic(law, tet):
+    """St-Venant-Kirchhoff and linear elastic laws (sound speed sqrt(E/rho)) in both forms, hex and tet."""
+    m = perturb(box(8, 7, 6, tet=tet), 0.1, 31 + law)
+    ro, (rc, rep, u, v, a, hist) = run_pair(m, 150, law=law, energy_every=1)
+    assert rc == 0 and relerr(u, ro["u"]) <= TOL64, relerr(u, ro["u"])
+    ed = ro["hist"]["e_def"][:150]
+    assert np.abs(hist["e_def"] - ed).max() <= 1e-10 * np.abs(ed).max()
+    ro2, (rc2, rep2, u2, *_r) = run_pair(m, 150, law=law, precision=G.PREC_FP32_ELEM)
+    assert rc2 == 0 and relerr(u2, ro2["u"]) <= TOL32, relerr(u2, ro2["u"])
+
+
 @pytest.mark.parametrize("precision", [G.PREC_FP64, G.PREC_FP32_ELEM])
 def test_mixed_orientation_hex(precision):
     """Half of the elements (random) use a mirrored local numbering (xi reversed), so det J < 0 there

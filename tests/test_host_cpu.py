@@ -80,6 +80,19 @@ def test_velocity_extension_and_precision(tmp_path):
     assert j["materials"][0]["law"] == "Neo-Hookean"
 
 
+def test_reads_svk_and_linear_elastic(tmp_path):
+    m, bc = cube(2)
+    p = tmp_path / "laws.inp"
+    inpgen.write_inp(p, m, [dict(rho=1000.0, E=1e6, nu=0.3)], [dict(scale=0.5, t0=0.0, tf=0.01, outputs=1, bc=bc)])
+    for law_line, name in (("*HyperElastic, St-Venant-Kirchhoff", "St-Venant-Kirchhoff"),
+                           ("*Linear-Elastic", "Linear-Elastic")):
+        p.write_text(p.read_text().replace("*HyperElastic, 3D-Compressible-Neo-Hookean", law_line)
+                     .replace("*HyperElastic, St-Venant-Kirchhoff", law_line))
+        rc, j, err = check(p)
+        assert rc == 0, err
+        assert j["materials"][0]["law"] == name
+
+
 @pytest.mark.parametrize("case,needle", [
     ("bc_before_time", "*Time must precede *BOUNDARY"),
     ("no_outputs", "*Outputs is mandatory"),
@@ -120,7 +133,7 @@ def test_input_errors(tmp_path, case, needle):
     if case == "implicit":
         text = text.replace("*SOLVER, EXPLICIT", "*SOLVER, IMPLICIT STATIC")
     if case == "stvk":
-        text = text.replace("3D-Compressible-Neo-Hookean", "St-Venant-Kirchhoff")
+        text = text.replace("3D-Compressible-Neo-Hookean", "St-Venant-Kirchhoff-ThermoMechanics")
     if case == "cps3":
         text = text.replace("type=C3D8", "type=CPS3")
     p.write_text(text)

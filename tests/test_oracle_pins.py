@@ -165,7 +165,7 @@ def test_material_constants_and_sound_speed():
         assert L.orc_sound_speed(E, nu, rho) == pytest.approx(c_p, rel=1e-14)
 
 
-@pytest.mark.parametrize("law", [0, 1])
+@pytest.mark.parametrize("law", [0, 1, 2])
 def test_pk1_reference_state_and_objectivity(law):
     np.testing.assert_allclose(pk1(law, np.eye(3)), 0.0, atol=1e-9)
     rng = np.random.default_rng(4 + law)
@@ -195,6 +195,40 @@ def test_pk1_compressible_closed_form_and_energy_gradient():
         np.testing.assert_allclose(dW, pk1(0, F), rtol=1e-5, atol=1e-2)
     assert energy(0, np.eye(3)) == pytest.approx(1.5 * mu, rel=1e-15)  # no -3mu/2 constant
     assert energy(1, rand_F(rng)) == 0.0  # Belytschko law has no defoEnergy
+
+
+def test_pk1_svk_and_linear_closed_forms_and_energy_gradients():
+    """St-Venant-Kirchhoff: P = F (lam tr(E) I + 2 mu E), E = (F^T F - I)/2; linear elastic: P = lam tr(e) I
+    + 2 mu e, e = sym(F) - I (not objective: a rotation is not stress free).  Both: P = dW/dF."""
+    lam, mu, _ = lame(1e6, 0.3)
+    rng = np.random.default_rng(12)
+    for law in (2, 3):
+        np.testing.assert_allclose(pk1(law, np.eye(3)), 0.0, atol=1e-9)
+        for _ in range(6):
+            F = rand_F(rng, 0.2)
+            if law == 2:
+                E = 0.5 * (F.T @ F - np.eye(3))
+                ref = F @ (lam * np.trace(E) * np.eye(3) + 2 * mu * E)
+            else:
+                e = 0.5 * (F + F.T) - np.eye(3)
+                ref = lam * np.trace(e) * np.eye(3) + 2 * mu * e
+            np.testing.assert_allclose(pk1(law, F), ref, rtol=1e-9, atol=1e-6)
+            h = 1e-6
+            dW = np.zeros((3, 3))
+            for i in range(3):
+                for j in range(3):
+                    Fp, Fm = F.copy(), F.copy()
+                    Fp[i, j] += h
+                    Fm[i, j] -= h
+                    dW[i, j] = (energy(law, Fp) - energy(law, Fm)) / (2 * h)
+            np.testing.assert_allclose(dW, pk1(law, F), rtol=1e-5, atol=1e-2)
+    ang = 0.3
+    R = np.array([[np.cos(ang), -np.sin(ang), 0], [np.sin(ang), np.cos(ang), 0], [0, 0, 1]])
+    assert np.abs(pk1(2, R)).max() < 1e-6 * mu       # SVK: rotation is stress free
+    assert np.abs(pk1(3, R)).max() > 1e-2 * mu       # small-strain law: it is not
+    for law in (2, 3):  # sqrt(E/rho) (hyperElasticStVenantKirchhoff.cpp, linearElastic.cpp:49-54)
+        assert L.orc_law_sound_speed(law, 2e9, 0.3, 7800.0) == np.sqrt(2e9 / 7800.0)
+    assert L.orc_law_sound_speed(0, 2e9, 0.3, 7800.0) == L.orc_sound_speed(2e9, 0.3, 7800.0)
 
 
 def test_pk1_belytschko_closed_form_and_pure_dilation():
@@ -242,7 +276,7 @@ def test_kinematics_patch_test(nen):
         np.testing.assert_allclose(F.reshape(3, 3), np.eye(3), atol=1e-14)
 
 
-@pytest.mark.parametrize("law", [0, 1])
+@pytest.mark.parametrize("law", [0, 1, 2])
 def test_element_force_equilibrium_and_rigid_rotation(law):
     rng = np.random.default_rng(9)
     xyz = (HEX_S + 1) / 2 + rng.uniform(-0.1, 0.1, (8, 3))
@@ -256,17 +290,18 @@ def test_element_force_equilibrium_and_rigid_rotation(law):
     assert np.abs(element_force(8, xyz, urot, law)).max() <= 1e-7 * np.abs(f).max()
 
 
-def element_energy(nen, xyz, uloc, E=1e6, nu=0.3):
+def element_energy(nen, xyz, uloc, E=1e6, nu=0.3, law=0):
     e = 0.0
     for phi, dphi, wJ in gp_data(nen, xyz):
         F = np.zeros(9)
         L.orc_kinematics(nen, O.dptr(arr(dphi).ravel()), O.dptr(arr(uloc).ravel()), O.dptr(F))
-        e += wJ * energy(0, F.reshape(3, 3), E, nu)
+        e += wJ * energy(law, F.reshape(3, 3), E, nu)
     return e
 
 
 @pytest.mark.parametrize("nen", [8, 4])
-def test_element_force_is_energy_gradient_at_finite_strain(nen):
+@pytest.mark.parametrize("law", [0, 2, 3])
+def test_element_force_is_energy_gradient_at_finite_strain(nen, law):
     # f_a = d/du_a sum_gp wJ W(F): pins the P-dphi contraction order (P is not symmetric here)
     rng = np.random.default_rng(11)
     ref = (HEX_S + 1) / 2 if nen == 8 else np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0], [0, 0, 1.0]])
@@ -274,7 +309,7 @@ def test_element_force_is_energy_gradient_at_finite_strain(nen):
     ang = 0.4
     R = np.array([[np.cos(ang), -np.sin(ang), 0], [np.sin(ang), np.cos(ang), 0], [0, 0, 1]])
     u = xyz @ (R @ np.diag([1.15, 0.9, 1.05]) - np.eye(3)).T + rng.uniform(-0.02, 0.02, ref.shape)
-    f = element_force(nen, xyz, u)
+    f = element_force(nen, xyz, u, law)
     h = 1e-7
     g = np.zeros_like(u)
     for a in range(nen):
@@ -282,7 +317,7 @@ def test_element_force_is_energy_gradient_at_finite_strain(nen):
             up, um = u.copy(), u.copy()
             up[a, i] += h
             um[a, i] -= h
-            g[a, i] = (element_energy(nen, xyz, up) - element_energy(nen, xyz, um)) / (2 * h)
+            g[a, i] = (element_energy(nen, xyz, up, law=law) - element_energy(nen, xyz, um, law=law)) / (2 * h)
     np.testing.assert_allclose(f, g, rtol=1e-5, atol=1e-5 * np.abs(f).max())
 
 
@@ -309,7 +344,7 @@ def textbook_hex_stiffness(xyz, E, nu):
     return K
 
 
-@pytest.mark.parametrize("law", [0, 1])
+@pytest.mark.parametrize("law", [0, 1, 2, 3])
 def test_element_force_linearises_to_textbook_stiffness(law):
     rng = np.random.default_rng(10)
     xyz = (HEX_S + 1) / 2 * 0.1 + rng.uniform(-0.005, 0.005, (8, 3))

@@ -59,7 +59,11 @@ void material_consts(int law, double E, double nu, double rho, double* pa, doubl
     double mu = 0.5 * E / (1. + nu);
     double K1 = E / (3 * (1 - 2 * nu));
     *c = std::sqrt((lambda + 2 * mu) / rho);
-    if (law == MPS_LAW_NH_COMPRESSIBLE) {
+    if (law == MPS_LAW_SVK || law == MPS_LAW_LINEAR_ELASTIC) {
+        *c = std::sqrt(E / rho);  // hyperElasticStVenantKirchhoff.cpp soundSpeed, linearElastic.cpp:49-54
+        *pa = lambda;
+        *pb = mu;
+    } else if (law == MPS_LAW_NH_COMPRESSIBLE) {
         *pa = lambda;
         *pb = mu;
     } else {
@@ -83,7 +87,7 @@ int prepare(const mps_mesh* mesh, const mps_material* mat, const mps_bcs* bcs, c
         *err = "mesh too large for 32-bit indices";
         return MPS_ERR_ARG;
     }
-    if (mat->law != MPS_LAW_NH_COMPRESSIBLE && mat->law != MPS_LAW_NH_BELYTSCHKO) {
+    if (mat->law < MPS_LAW_NH_COMPRESSIBLE || mat->law > MPS_LAW_LINEAR_ELASTIC) {
         *err = fmt("unknown constitutive law %d", mat->law);
         return MPS_ERR_ARG;
     }

@@ -141,9 +141,48 @@ __device__ __forceinline__ float log1p_fast(float x) {
 //  Belytschko NH (hyperElasticNeoHookean.cpp:88-126): sigma = mu/J^{5/3} B + (-(mu/3) trB/J^{5/3}
 //     + K (J-1)) I, P = J sigma F^-T = sigma cof(F).
 // ---------------------------------------------------------------------------
-// FP64: F-based, as the reference.
+//  St-Venant-Kirchhoff (hyperElasticStVenantKirchhoff.cpp:203-226): P = F S, S = lambda tr(E) I + 2 mu E,
+//     E = (F^T F - I)/2 = (H + H^T + H^T H)/2.
+//  Linear elastic (linearElastic.cpp:63-95): P = lambda tr(e) I + 2 mu e, e = sym(F) - I = (H + H^T)/2.
+// Both are written in H, which carries no cancellation at small strain.  (pa, pb) = (lambda, mu).
+template <typename T, int LAW>
+__device__ __forceinline__ void pk1_svk_lin(const T* H, T lam, T mu, T* P) {
+    T E[9];
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+            T e = T(0.5) * (H[i * 3 + j] + H[j * 3 + i]);
+            if (LAW == 2) e += T(0.5) * (H[i] * H[j] + H[3 + i] * H[3 + j] + H[6 + i] * H[6 + j]);
+            E[i * 3 + j] = e;
+        }
+    const T ltr = lam * (E[0] + E[4] + E[8]);
+    const T mu2 = T(2) * mu;
+    T S[9];
+#pragma unroll
+    for (int i = 0; i < 9; i++) S[i] = mu2 * E[i];
+    S[0] += ltr;
+    S[4] += ltr;
+    S[8] += ltr;
+    if (LAW == 3) {
+#pragma unroll
+        for (int i = 0; i < 9; i++) P[i] = S[i];
+        return;
+    }
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++)
+            P[i * 3 + j] = S[i * 3 + j] + H[i * 3] * S[j] + H[i * 3 + 1] * S[3 + j] + H[i * 3 + 2] * S[6 + j];
+}
+
+// FP64: F-based, as the reference (neo-Hookean laws).
 template <int LAW>
 __device__ __forceinline__ void pk1(const double* H, double pa, double pb, double* P) {
+    if (LAW >= 2) {
+        pk1_svk_lin<double, LAW>(H, pa, pb, P);
+        return;
+    }
     double F[9];
 #pragma unroll
     for (int i = 0; i < 9; i++) F[i] = H[i];
@@ -190,6 +229,10 @@ __device__ __forceinline__ void pk1(const double* H, double pa, double pb, doubl
 //   J - 1 = tr H + i2(H) + det H,  F - F^-T = H + H^T F^-T,  B - I = H + H^T + H H^T.
 template <int LAW>
 __device__ __forceinline__ void pk1(const float* H, float pa, float pb, float* P) {
+    if (LAW >= 2) {
+        pk1_svk_lin<float, LAW>(H, pa, pb, P);
+        return;
+    }
     const float Jm1 = detIpH_m1(H);
     const float lnJ = log1p_fast(Jm1);
     float F[9];
@@ -241,9 +284,30 @@ __device__ __forceinline__ void pk1(const float* H, float pa, float pb, float* P
 }
 
 // Deformation energy W(F): hyperElasticNeoHookean.cpp:291-304 (no -3mu/2 constant);
-// the Belytschko law has no defoEnergy override and contributes 0 (constitutiveModels.h:91-94).
+// the Belytschko law has no defoEnergy override and contributes 0 (constitutiveModels.h:91-94);
+// St-Venant-Kirchhoff 0.5 S:E (hyperElasticStVenantKirchhoff.cpp:251-274); linear elastic
+// 0.5 (P:F - P:I) (linearElastic.cpp:97-110).
 __device__ __forceinline__ double defo_energy(int law, const double* F, double lambda, double mu) {
-    if (law != 0) return 0.0;
+    if (law == 1) return 0.0;
+    if (law >= 2) {
+        double H[9], P[9];
+#pragma unroll
+        for (int i = 0; i < 9; i++) H[i] = F[i] - ((i % 4) == 0 ? 1.0 : 0.0);
+        double w = 0.0;
+        if (law == 2) {
+            double E[9], S[9];
+            for (int i = 0; i < 3; i++)
+                for (int j = 0; j < 3; j++)
+                    E[i * 3 + j] = 0.5 * (H[i * 3 + j] + H[j * 3 + i] + H[i] * H[j] + H[3 + i] * H[3 + j] + H[6 + i] * H[6 + j]);
+            const double ltr = lambda * (E[0] + E[4] + E[8]);
+            for (int i = 0; i < 9; i++) S[i] = 2.0 * mu * E[i] + ((i % 4) == 0 ? ltr : 0.0);
+            for (int i = 0; i < 9; i++) w += S[i] * E[i];
+            return 0.5 * w;
+        }
+        pk1_svk_lin<double, 3>(H, lambda, mu, P);
+        for (int i = 0; i < 9; i++) w += P[i] * H[i];  // P:F - P:I = P:H
+        return 0.5 * w;
+    }
     double J = det3(F);
     double trC = 0;
 #pragma unroll

@@ -567,8 +567,12 @@ hipError_t launch_elem(const MpsInst& I, int mode, hipStream_t st) {
     if (nb == 0) return hipSuccess;
 #define MPS_ELEM_LAUNCH(T, NEN)                                                   \
     do {                                                                          \
-        if (I.law == 0) k_elem<T, NEN, 0><<<nb, MPS_PATCH, 0, st>>>(I, mode);     \
-        else k_elem<T, NEN, 1><<<nb, MPS_PATCH, 0, st>>>(I, mode);                \
+        switch (I.law) {                                                          \
+            case 0: k_elem<T, NEN, 0><<<nb, MPS_PATCH, 0, st>>>(I, mode); break;  \
+            case 1: k_elem<T, NEN, 1><<<nb, MPS_PATCH, 0, st>>>(I, mode); break;  \
+            case 2: k_elem<T, NEN, 2><<<nb, MPS_PATCH, 0, st>>>(I, mode); break;  \
+            default: k_elem<T, NEN, 3><<<nb, MPS_PATCH, 0, st>>>(I, mode); break; \
+        }                                                                         \
     } while (0)
     if (I.nen == 8) {
         if (I.precision == 1) MPS_ELEM_LAUNCH(float, 8);
@@ -635,13 +639,19 @@ static hipError_t batch_phase_bs(int phase, int prec, int nen, int law, const Mp
                                  const int* nwork, int grid, int fuse, hipStream_t st) {
     if (phase == 0) {
 #define MPS_EB(T, NEN, LAW) k_elem_batch<T, NEN, LAW, BS><<<grid, BS, 0, st>>>(insts, work, nwork)
+#define MPS_EB_LAW(T, NEN)                          \
+    switch (law) {                                  \
+        case 0: MPS_EB(T, NEN, 0); break;           \
+        case 1: MPS_EB(T, NEN, 1); break;           \
+        case 2: MPS_EB(T, NEN, 2); break;           \
+        default: MPS_EB(T, NEN, 3); break;          \
+    }
         if (prec == 1) {
-            if (nen == 8) { if (law == 0) MPS_EB(float, 8, 0); else MPS_EB(float, 8, 1); }
-            else { if (law == 0) MPS_EB(float, 4, 0); else MPS_EB(float, 4, 1); }
+            if (nen == 8) { MPS_EB_LAW(float, 8) } else { MPS_EB_LAW(float, 4) }
         } else {
-            if (nen == 8) { if (law == 0) MPS_EB(double, 8, 0); else MPS_EB(double, 8, 1); }
-            else { if (law == 0) MPS_EB(double, 4, 0); else MPS_EB(double, 4, 1); }
+            if (nen == 8) { MPS_EB_LAW(double, 8) } else { MPS_EB_LAW(double, 4) }
         }
+#undef MPS_EB_LAW
 #undef MPS_EB
     } else if (phase == 1) {
         k_node_batch<BS><<<grid, BS, 0, st>>>(insts, work, nwork, fuse);

@@ -32,6 +32,16 @@ DeviceMaterial classHyperElastic3DCompressibleNeoHookean::deviceDescriptor() con
     return DeviceMaterial{MPS_LAW_NH_COMPRESSIBLE, _Young, _nu, _rho};
 }
 
+double classHyperElasticStVenantKirchhoff::soundSpeed() const { return std::sqrt(_Young / _rho); }
+
+DeviceMaterial classHyperElasticStVenantKirchhoff::deviceDescriptor() const {
+    return DeviceMaterial{MPS_LAW_SVK, _Young, _nu, _rho};
+}
+
+double linearElastic::soundSpeed() const { return std::sqrt(_Young / _rho); }  // linearElastic.cpp:49-54
+
+DeviceMaterial linearElastic::deviceDescriptor() const { return DeviceMaterial{MPS_LAW_LINEAR_ELASTIC, _Young, _nu, _rho}; }
+
 std::unique_ptr<constitutiveModels> makeConstitutiveModel(const std::string &name, const std::string &nameCons,
                                                           double rho, double young, double nu) {
     if (nameCons == "Neo-Hookean")
@@ -39,6 +49,11 @@ std::unique_ptr<constitutiveModels> makeConstitutiveModel(const std::string &nam
     if (nameCons == "3D-Compressible-Neo-Hookean")
         return std::unique_ptr<constitutiveModels>(
             new classHyperElastic3DCompressibleNeoHookean(3, name, nameCons, rho, young, nu));
+    if (nameCons == "St-Venant-Kirchhoff")
+        return std::unique_ptr<constitutiveModels>(
+            new classHyperElasticStVenantKirchhoff(3, name, nameCons, rho, young, nu));
+    if (nameCons == "Linear-Elastic")
+        return std::unique_ptr<constitutiveModels>(new linearElastic(3, name, nameCons, rho, young, nu));
     return nullptr;
 }
 

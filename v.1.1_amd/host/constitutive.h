@@ -1,7 +1,8 @@
 // constitutive.h -- host side of the constitutive plugin of the explicit path.
 //
 // Mirrors the reference's class constitutiveModels (constitutiveModels.h:16-96)
-// for the two neo-Hookean laws of the path.  The per-Gauss-point update
+// for the laws with a device descriptor: the two neo-Hookean laws, St-Venant-Kirchhoff
+// and linear elastic.  The per-Gauss-point update
 // (updateConstitutive, defoEnergy) runs in the device element kernels; the
 // host object keeps the law's name, parameters and sound speed, and hands the
 // kernels a device descriptor (law id + E, nu, rho).  A law without a
@@ -66,6 +67,32 @@ class classHyperElastic3DCompressibleNeoHookean : public constitutiveModels {
 
   private:
     double _Young, _nu, _lambda, _mu;
+};
+
+// "*HyperElastic, St-Venant-Kirchhoff" (hyperElasticStVenantKirchhoff.cpp): S = lambda tr(E) I + 2 mu E,
+// P = F S, E the Green-Lagrange strain; sound speed sqrt(E/rho).
+class classHyperElasticStVenantKirchhoff : public constitutiveModels {
+  public:
+    classHyperElasticStVenantKirchhoff(int ndim, std::string name, std::string nameCons, double rho, double young,
+                                       double nu)
+        : constitutiveModels(ndim, std::move(name), std::move(nameCons), rho), _Young(young), _nu(nu) {}
+    double soundSpeed() const override;
+    DeviceMaterial deviceDescriptor() const override;
+
+  private:
+    double _Young, _nu;
+};
+
+// "*Linear-Elastic" (linearElastic.cpp): P = lambda tr(e) I + 2 mu e, e = sym(F) - I; sound speed sqrt(E/rho).
+class linearElastic : public constitutiveModels {
+  public:
+    linearElastic(int ndim, std::string name, std::string nameCons, double rho, double young, double nu)
+        : constitutiveModels(ndim, std::move(name), std::move(nameCons), rho), _Young(young), _nu(nu) {}
+    double soundSpeed() const override;
+    DeviceMaterial deviceDescriptor() const override;
+
+  private:
+    double _Young, _nu;
 };
 
 // Factory keyed by the law keyword, as materialManagement does

@@ -204,11 +204,15 @@ bool readMaterials(Reader &R, InpModel &m) {
         if (starts(line, "*Density")) {
             if (i + 1 >= R.L.size() || !toDouble(R.L[i + 1], rho)) return R.fail(i, "*Density needs a value on the next line");
             i++;
-        } else if (starts(line, "*HyperElastic")) {
-            std::string model = tail(line, 15);
-            if (model != "Neo-Hookean" && model != "3D-Compressible-Neo-Hookean")
+        } else if (starts(line, "*HyperElastic") || starts(line, "*Linear-Elastic")) {
+            // "*HyperElastic, <model>" (model = text after column 15, inputFile.cpp:930-1024) or
+            // "*Linear-Elastic" (inputFile.cpp:886-928); next line "E, nu[, stress state]"
+            std::string model = starts(line, "*Linear-Elastic") ? std::string("Linear-Elastic") : tail(line, 15);
+            if (model != "Neo-Hookean" && model != "3D-Compressible-Neo-Hookean" && model != "St-Venant-Kirchhoff" &&
+                model != "Linear-Elastic")
                 return R.fail(i, "constitutive model \"" + model +
-                                     "\" has no device descriptor (Neo-Hookean and 3D-Compressible-Neo-Hookean do)");
+                                     "\" has no device descriptor (Neo-Hookean, 3D-Compressible-Neo-Hookean, "
+                                     "St-Venant-Kirchhoff and Linear-Elastic do)");
             if (i + 1 >= R.L.size()) return R.fail(i, "*HyperElastic needs \"E, nu\" on the next line");
             std::vector<std::string> f = fields(R.L[i + 1]);
             if (f.size() < 2 || !toDouble(f[0], cur.young) || !toDouble(f[1], cur.nu))

@@ -5,6 +5,7 @@
 //   *Element, type=C3D8|C3D4    elementsManagement     inputFile.cpp:121-330
 //   *FEM Nodes / *FEM Elements  (All)                  inputFile.cpp:331-378
 //   *MATERIAL ... *END          materialManagement     inputFile.cpp:838-1018, 2010-2054, 2130
+//     (*HyperElastic, Neo-Hookean | 3D-Compressible-Neo-Hookean | St-Venant-Kirchhoff; *Linear-Elastic)
 //   *SOLVER, EXPLICIT ... *END SOLVER                  solverManagement inputFile.cpp:2253-2762
 //   *BOUNDARY, TYPE=DISPLACEMENT RAMP|INST             dispBCManagement inputFile.cpp:2960-3217
 //   *Outputs                    outManagement          inputFile.cpp:4769-4829
@@ -31,7 +32,7 @@ namespace mps {
 
 struct InpMaterial {
     std::string name;     // text after "*MATERIAL, name=" (inputFile.cpp:876)
-    std::string nameCons; // "Neo-Hookean" | "3D-Compressible-Neo-Hookean"
+    std::string nameCons; // "Neo-Hookean" | "3D-Compressible-Neo-Hookean" | "St-Venant-Kirchhoff" | "Linear-Elastic"
     double rho = 0, young = 0, nu = 0;
     bool all = false;           // *List of Elements / All
     std::vector<int32_t> elems; // 0-based element indices otherwise

@@ -16,7 +16,7 @@ HEADER = os.path.join(os.path.dirname(os.path.dirname(PKG_DIR)), "include", "mps
 
 MPS_OK, MPS_ERR_ARG, MPS_ERR_HIP, MPS_ERR_DT, MPS_ERR_NAN, MPS_ERR_STATE, MPS_ERR_CAPACITY = 0, -1, -2, -3, -4, -5, -6
 MPS_RUNNING, MPS_DONE = 0, 1
-LAW_NH_COMPRESSIBLE, LAW_NH_BELYTSCHKO = 0, 1
+LAW_NH_COMPRESSIBLE, LAW_NH_BELYTSCHKO, LAW_SVK, LAW_LINEAR_ELASTIC = 0, 1, 2, 3
 BC_RAMP, BC_INST, BC_VELOCITY, BC_RAMP_RATE = 0, 1, 2, 3
 PREC_FP64, PREC_FP32_ELEM = 0, 1
 

@@ -142,6 +142,14 @@ int mps_run(mps_handle h, int64_t max_steps, mps_report *rep);
 int mps_get_history(mps_handle h, int64_t first, int64_t count, double *time, double *dt,
                     double *e_def, double *e_kin, double *volume);
 
+/* Element fields of the current state for the VTK output: out[21*n_elem], per element the Gauss-point
+ * averages of Cauchy stress (9, row-major), Green-Lagrange strain (9; the small strain for the linear
+ * law), von Mises stress, volumetric strain and equivalent strain, law by law as the reference's
+ * updateConstitutive sets them (0 where a law does not).
+ * Replaces: the per-GP state accumulation of vtkPlotting2D3D (output.cpp:567-625) behind the
+ * TENSORS Cauchy / E and SCALARS VMS / VolumetricStrain / EquivalentStrain sections. */
+int mps_element_outputs(mps_handle h, double *out);
+
 /* Lumped mass per node (identical for x, y, z).
  * Replaces: classSolverExplicit::computeLumpedMassVector (solverExplicit.cpp:429-459). */
 int mps_get_mass(mps_handle h, double *mass);

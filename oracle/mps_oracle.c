@@ -310,7 +310,7 @@ static void pk1_compressible(const double *F, double lambda, double mu, double *
 /* classHyperElasticNeoHookean::updateConstitutive (no tangent):
  * hyperElasticNeoHookean.cpp:70-126 with leftCauchy (TensorOperations.h:517-531),
  * classTensor2::trace (maths.h:303-311) and dotSecondTranspose (:577-591). */
-static void pk1_belytschko(const double *F, double mu, double K1, double *P)
+static void pk1_belytschko_sig(const double *F, double mu, double K1, double *P, double *sig_out)
 {
     double B[9], invF[9], sig[9];
     for (int i = 0; i < 3; i++)
@@ -336,6 +336,13 @@ static void pk1_belytschko(const double *F, double mu, double K1, double *P)
             for (int k = 0; k < 3; k++) p += sig[i * 3 + k] * invF[j * 3 + k] * J;
             P[i * 3 + j] = p;
         }
+    if (sig_out)
+        for (int i = 0; i < 9; i++) sig_out[i] = sig[i];
+}
+
+static void pk1_belytschko(const double *F, double mu, double K1, double *P)
+{
+    pk1_belytschko_sig(F, mu, K1, P, NULL);
 }
 
 /* PK2 = mu2 E + lam tr(E) I (hyperElasticStVenantKirchhoff.cpp / linearElastic.cpp stress(), 3-D branch):
@@ -907,6 +914,66 @@ int32_t orc_internal_force(const orc_problem *p, const double *u, double *fint)
         internal_force(&s, u);
         memcpy(fint, s.fint, sizeof(double) * 3 * (size_t)s.N);
         pool_stop(&s, th, wa);
+    }
+    state_free(&s);
+    return rc;
+}
+
+/* State quantities at one GP as updateConstitutive leaves them (hyperElasticNeoHookean.cpp:70-126, 221-257;
+ * hyperElasticStVenantKirchhoff.cpp:203-227; linearElastic.cpp:63-95) with computeCauchy, computeVMS,
+ * computeVolumetricStrain, computeEquivalentStrain (maths.cpp:1241-1299).  q[0..8] Cauchy, q[9..17] GL strain
+ * (small strain for the linear law), q[18] VMS, q[19] volumetric strain, q[20] equivalent strain; quantities a
+ * law does not set keep the zero of the classStates constructor (classStates.cpp:15-44). */
+static void gp_state_outputs(int32_t law, const double *F, double lam, double mu, double K1, double *q)
+{
+    double P[9], *cau = q, *E = q + 9;
+    for (int i = 0; i < 21; i++) q[i] = 0;
+    if (law == ORC_LAW_NH_BELYTSCHKO) {
+        green_lagrange(F, E);
+        pk1_belytschko_sig(F, mu, K1, P, cau);
+    } else {
+        if (law == ORC_LAW_SVK) green_lagrange(F, E);
+        if (law == ORC_LAW_LINEAR)
+            for (int i = 0; i < 3; i++)
+                for (int j = 0; j < 3; j++) E[i * 3 + j] = 0.5 * (F[i * 3 + j] + F[j * 3 + i] - 2. * (i == j ? 1.0 : 0.0));
+        pk1_law(law, F, lam, mu, K1, P);
+        double detJinv = 1. / orc_det3(F);
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++) {
+                double c = 0;
+                for (int k = 0; k < 3; k++) c += P[i * 3 + k] * F[j * 3 + k];
+                cau[i * 3 + j] = c;
+            }
+        for (int i = 0; i < 9; i++) cau[i] = cau[i] * detJinv;
+    }
+    q[18] = sqrt(0.5 * ((cau[0] - cau[4]) * (cau[0] - cau[4]) + (cau[4] - cau[8]) * (cau[4] - cau[8]) +
+                        (cau[8] - cau[0]) * (cau[8] - cau[0]) + 6 * (cau[1] * cau[1] + cau[2] * cau[2] + cau[5] * cau[5])));
+    if (law == ORC_LAW_NH_BELYTSCHKO || law == ORC_LAW_SVK) {
+        double v = 0;
+        for (int i = 0; i < 3; i++) v += E[i + i * 3];
+        q[19] = v;
+        q[20] = 0.6666 * sqrt(1.5 * (E[0] * E[0] + E[4] * E[4] + E[8] * E[8]) + 3 * (E[1] * E[1] + E[2] * E[2] + E[5] * E[5]));
+    }
+}
+
+int32_t orc_element_outputs(const orc_problem *p, const double *u, double *out)
+{
+    orc_state s;
+    int rc = setup(&s, p);
+    if (rc == ORC_OK) {
+        size_t ng = s.ngp, nen = s.nen, N = s.N;
+        for (size_t e = 0; e < (size_t)s.Ne; e++) {
+            const int32_t *cn = p->conn + e * nen;
+            double uloc[24], F[9], q[21], acc[21] = {0};
+            for (size_t a = 0; a < nen; a++)
+                for (int i = 0; i < 3; i++) uloc[a * 3 + i] = u[cn[a] + N * i];
+            for (size_t g = 0; g < ng; g++) {  /* classElements::sumCauchy & co., GP order */
+                orc_kinematics(s.nen, s.dphi + (e * ng + g) * nen * 3, uloc, F);
+                gp_state_outputs(p->law, F, s.lam[e], s.mu[e], s.K1[e], q);
+                for (int i = 0; i < 21; i++) acc[i] += q[i];
+            }
+            for (int i = 0; i < 21; i++) out[e * 21 + i] = acc[i] / (double)ng;  /* / cellCount (output.cpp:624-) */
+        }
     }
     state_free(&s);
     return rc;

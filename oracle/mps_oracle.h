@@ -106,6 +106,10 @@ int32_t orc_internal_force(const orc_problem *p, const double *u, double *fint);
 
 /* Critical time step s*min_e min(l0,l(u))/c (solvers.cpp:296-320); u may be NULL. */
 double orc_critical_dt(const orc_problem *p, const double *u);
+/* Per element (out[21*n_elem]): GP averages of Cauchy (9), Green-Lagrange strain (9; small strain for
+ * the linear law), von Mises stress, volumetric and equivalent strain, as the VTK writer prints them
+ * (output.cpp:567-820). */
+int32_t orc_element_outputs(const orc_problem *p, const double *u, double *out);
 
 #ifdef __cplusplus
 }

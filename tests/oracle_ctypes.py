@@ -78,6 +78,7 @@ def lib():
         L.orc_lumped_mass.restype = C.c_int32
         L.orc_lumped_mass.argtypes = [C.POINTER(Problem), _dp]
         L.orc_internal_force.restype = C.c_int32
+        L.orc_element_outputs.restype = C.c_int32
         L.orc_internal_force.argtypes = [C.POINTER(Problem), _dp, _dp]
         L.orc_critical_dt.restype = C.c_double
         L.orc_critical_dt.argtypes = [C.POINTER(Problem), _dp]
@@ -157,6 +158,14 @@ class OracleProblem:
         rc = lib().orc_internal_force(C.byref(self.p), dptr(u), dptr(f))
         assert rc == 0
         return f
+
+    def element_outputs(self, u):
+        """Per element [n_elem, 21]: GP-averaged Cauchy (9), strain (9), VMS, volumetric, equivalent strain."""
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        out = np.zeros(21 * self.p.n_elem)
+        rc = lib().orc_element_outputs(C.byref(self.p), dptr(u), dptr(out))
+        assert rc == 0
+        return out.reshape(-1, 21)
 
     def critical_dt(self, u=None):
         if u is not None:

@@ -125,3 +125,86 @@ def test_fixed_steps_velocity_fp32_and_materials(tmp_path):
     assert r["steps"] == 80
     assert rel(u64, r["u"]) <= 1e-12
     assert rel(u32, r["u"]) <= 1e-6
+
+
+def parse_vtk(path):
+    """Sections of a legacy ASCII VTK file written by the series writer: name -> list of value rows."""
+    lines = open(path).read().split("\n")
+    head = lines[:6]
+    sec, cur = {}, None
+    for ln in lines[5:]:
+        t = ln.strip()
+        if not t or t == "LOOKUP_TABLE default":
+            continue
+        w = t.split()
+        if w[0] in ("POINTS", "CELLS", "CELL_TYPES", "POINT_DATA", "CELL_DATA"):
+            cur = w[0] if w[0] not in ("POINT_DATA", "CELL_DATA") else None
+            if cur:
+                sec[cur] = []
+            continue
+        if w[0] in ("VECTORS", "TENSORS", "SCALARS"):
+            cur = w[1]
+            sec[cur] = []
+            continue
+        sec[cur].append(w)
+    return head, sec, lines
+
+
+def test_vtk_series_matches_state_and_oracle(tmp_path):
+    """VTK series of the reference writer (output.cpp:296-918): file numbering and .vtk.series across two
+    chained solvers, header and sections, POINTS/CELLS, nodal fields equal to the final state printed with 6
+    significant digits, element fields (GP-averaged Cauchy, Green-Lagrange strain, VMS, volumetric and
+    equivalent strain, pressure) against the oracle on the same displacement."""
+    m = configs.perturb(configs.box(3, 3, 4), 0.1, 13)
+    N, Ne = m["n_nodes"], m["n_elem"]
+    dof, val = configs.clamp_bottom_pull_top(m, vel=0.002)
+    bc = inpgen.bc_triples(dof, val, N)
+    dt0 = O.OracleProblem(m, law=1, dir_dof=dof, dir_val=val, bc_mode=O.BC_RAMP, tf=1.0, scale=0.5).critical_dt()
+    t1, t2 = 20.5 * dt0, 31.5 * dt0
+    inp = tmp_path / "vtk.inp"
+    inpgen.write_inp(inp, m, [dict(rho=1000.0, E=1e6, nu=0.3, law=1)],
+                     [dict(scale=0.5, t0=0.0, tf=t1, outputs=3, bc=bc), dict(scale=0.5, t0=t1, tf=t2, outputs=2, bc=bc)])
+    inp.write_text(inp.read_text().replace("*Outputs\n3\n", "*Outputs\n3\nAll, 2\n"))
+    u, _ = run(inp, tmp_path)
+    files = sorted(tmp_path.glob("MuPhiSimOutputs_0_.*.vtk"), key=lambda p: int(p.name.split(".")[1]))
+    assert [p.name for p in files] == [f"MuPhiSimOutputs_0_.{k}.vtk" for k in range(len(files))]
+    assert len(files) >= 5  # t0 + >= 3 for the first solver + >= 1 for the second
+    series = (tmp_path / "allOutputs_0.vtk.series").read_text().split("\n")
+    assert series[:3] == ["{", '"file-series-version" : "1.0",', '"files" : [']
+    entries = [ln for ln in series if ln.startswith("{ \"name\"")]
+    assert len(entries) == len(files)
+    times = [float(e.split(":")[-1].rstrip("},")) for e in entries]
+    assert times[0] == 0.0 and all(b > a for a, b in zip(times, times[1:]))
+    assert abs(times[-1] - t2) <= 1e-5 * t2
+    assert all(ln.endswith("},") for ln in entries[:-1]) and entries[-1].endswith("}")
+    # initial file: zero fields
+    head0, s0, _ = parse_vtk(files[0])
+    assert all(float(x) == 0.0 for r in s0["Displacement"] for x in r)
+    assert all(float(x) == 0.0 for r in s0["Cauchy"] for x in r)
+    # last file = final state of the second solver
+    head, s, lines = parse_vtk(files[-1])
+    assert head == ["# vtk DataFile Version 1.0", "3D Unstructured Grid of C3D8 elements", "ASCII", "",
+                    "DATASET UNSTRUCTURED_GRID", f"POINTS {N} float"]
+    X = m["X"].reshape(3, N)
+    assert s["POINTS"] == [[f"{X[d, n]:.6g}" for d in range(3)] for n in range(N)]
+    conn = m["conn"].reshape(Ne, 8)
+    assert s["CELLS"] == [["8"] + [str(c) for c in conn[e]] for e in range(Ne)]
+    assert s["CELL_TYPES"] == [["12"]] * Ne
+    U = u.reshape(3, N)
+    assert s["Displacement"] == [[f"{U[d, n]:.6g}" for d in range(3)] for n in range(N)]
+    assert lines[lines.index("VECTORS Displacement float ") + 1].endswith(" ")
+    q = O.OracleProblem(m, law=1, n_steps=1).element_outputs(u)
+
+    def close(rows, ref):
+        got = np.array([[float(x) for x in r] for r in rows]).reshape(ref.shape)
+        return np.abs(got - ref).max() <= 2e-6 * np.abs(ref).max() + 1e-300
+
+    assert close(s["Cauchy"], q[:, :9])
+    assert close(s["E"], q[:, 9:18])
+    assert close(s["VMS"], q[:, 18])
+    assert close(s["VolumetricStrain"], q[:, 19])
+    assert close(s["EquivalentStrain"], q[:, 20])
+    assert close(s["Pressure"], (q[:, 0] + q[:, 4] + q[:, 8]) / 3)
+    assert not any(k.startswith("IntVars") for k in s)  # the second *Outputs names no internal variables
+    _, s1, _ = parse_vtk(files[1])  # the first solver's *Outputs asked for variables 1 and 2
+    assert s1["IntVars1"] == [["0"]] * Ne and s1["IntVars2"] == [["0"]] * Ne

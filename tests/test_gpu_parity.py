@@ -115,6 +115,25 @@ def test_svk_and_linear_elastic(law, tet):
     assert rc2 == 0 and relerr(u2, ro2["u"]) <= TOL32, relerr(u2, ro2["u"])
 
 
+@pytest.mark.parametrize("law", [0, 1, 2, 3])
+@pytest.mark.parametrize("tet", [False, True])
+def test_element_outputs(law, tet):
+    """VTK element fields (GP-averaged Cauchy, strain, VMS, volumetric/equivalent strain) of the device state
+    against the oracle evaluated on the same displacement."""
+    m = perturb(box(6, 5, 4, tet=tet), 0.1, 41 + law)
+    dd, dv = clamp_and_pull(m, vel=0.5)
+    with G.Solver(m, law=law, dir_dof=dd, dir_val=dv, n_steps=80) as s:
+        rc, rep = s.run(80)
+        assert rc == 0
+        u = s.get_state()[0]
+        q = s.element_outputs()
+    qo = O.OracleProblem(m, law=law, n_steps=1).element_outputs(u)
+    assert np.abs(qo[:, :9]).max() > 0
+    for cols in (slice(0, 9), slice(9, 18), slice(18, 19), slice(19, 21)):
+        scale = max(np.abs(qo[:, cols]).max(), 1e-300)
+        assert np.abs(q[:, cols] - qo[:, cols]).max() <= 1e-10 * scale, cols
+
+
 @pytest.mark.parametrize("precision", [G.PREC_FP64, G.PREC_FP32_ELEM])
 def test_mixed_orientation_hex(precision):
     """Half of the elements (random) use a mirrored local numbering (xi reversed), so det J < 0 there

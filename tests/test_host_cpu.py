@@ -80,6 +80,20 @@ def test_velocity_extension_and_precision(tmp_path):
     assert j["materials"][0]["law"] == "Neo-Hookean"
 
 
+@pytest.mark.parametrize("line,expect", [("All, 3", [0, 1, 2]), ("5", [5]), ("4, 1, 3", [0, 2]), (None, [])])
+def test_outputs_internal_variable_line(tmp_path, line, expect):
+    """outManagement (inputFile.cpp:4769-4829): "All, n" -> 0..n-1; a single k -> k as is; "max, i, j" ->
+    i-1, j-1."""
+    m, bc = cube(2)
+    p = tmp_path / "ov.inp"
+    inpgen.write_inp(p, m, [dict(rho=1000.0, E=1e6, nu=0.3)], [dict(scale=0.5, t0=0.0, tf=0.01, outputs=2, bc=bc)])
+    if line is not None:
+        p.write_text(p.read_text().replace("*Outputs\n2\n", f"*Outputs\n2\n{line}\n"))
+    rc, j, err = check(p)
+    assert rc == 0, err
+    assert j["solvers"][0]["out_vars"] == expect
+
+
 def test_reads_svk_and_linear_elastic(tmp_path):
     m, bc = cube(2)
     p = tmp_path / "laws.inp"

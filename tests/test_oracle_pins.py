@@ -501,3 +501,34 @@ def test_summation_orders_and_threads_agree():
         runs.append(O.OracleProblem(m, dir_dof=dd, dir_val=dv, n_steps=200, sum_order=so, n_threads=nt).run()["u"])
     assert np.abs(runs[0] - runs[1]).max() <= 1e-13 * np.abs(runs[0]).max()
     assert np.array_equal(runs[1], runs[2])
+
+
+@pytest.mark.parametrize("law", [0, 1, 2, 3])
+def test_element_outputs_homogeneous_deformation(law):
+    """u = (G - I) X gives F = G at every GP, so each element's averages are the closed forms at G:
+    Cauchy J^-1 P G^T (Belytschko: sigma), Green-Lagrange E (small strain for the linear law), von Mises,
+    tr E, 0.6666 sqrt(1.5 sum E_ii^2 + 3 sum_{i<j} E_ij^2); quantities a law does not set are 0."""
+    m = perturb(box(3, 2, 2), 0.15, 5)
+    G = np.array([[1.03, 0.02, -0.01], [0.015, 0.98, 0.025], [-0.02, 0.01, 1.05]])
+    X = m["X"].reshape(3, -1)
+    u = ((G - np.eye(3)) @ X).ravel()
+    q = O.OracleProblem(m, law=law, n_steps=1).element_outputs(u)
+    J = np.linalg.det(G)
+    P = pk1(law, G)
+    cau = P @ G.T / J
+    if law == 3:
+        E = 0.5 * (G + G.T) - np.eye(3)
+    else:
+        E = 0.5 * (G.T @ G - np.eye(3))
+    vms = np.sqrt(0.5 * ((cau[0, 0] - cau[1, 1]) ** 2 + (cau[1, 1] - cau[2, 2]) ** 2 + (cau[2, 2] - cau[0, 0]) ** 2
+                         + 6 * (cau[0, 1] ** 2 + cau[0, 2] ** 2 + cau[1, 2] ** 2)))
+    exp = np.zeros(21)
+    exp[:9] = cau.ravel()
+    if law != 0:
+        exp[9:18] = E.ravel()
+    exp[18] = vms
+    if law in (1, 2):
+        exp[19] = np.trace(E)
+        exp[20] = 0.6666 * np.sqrt(1.5 * np.sum(np.diag(E) ** 2) + 3 * (E[0, 1] ** 2 + E[0, 2] ** 2 + E[1, 2] ** 2))
+    for row in q:
+        np.testing.assert_allclose(row, exp, rtol=1e-9, atol=1e-9 * np.abs(exp).max())

@@ -253,6 +253,7 @@ struct mps_solver_s {
     double* scratch_u = nullptr;
     double* scratch_f = nullptr;
     unsigned long long* scratch_dt = nullptr;
+    double* eout = nullptr;       // mps_element_outputs buffer (allocated on first use)
     int32_t* pick_idx = nullptr;  // mps_get_dofs buffers (grow-only, freed with the handle)
     double* pick_out = nullptr;
     int64_t pick_cap = 0;
@@ -305,8 +306,10 @@ static void release(mps_solver_s* h) {
     h->allocs.clear();
     if (h->pick_idx) (void)hipFree(h->pick_idx);
     if (h->pick_out) (void)hipFree(h->pick_out);
+    if (h->eout) (void)hipFree(h->eout);
     h->pick_idx = nullptr;
     h->pick_out = nullptr;
+    h->eout = nullptr;
     if (h->ev0) (void)hipEventDestroy(h->ev0);
     if (h->ev1) (void)hipEventDestroy(h->ev1);
     if (h->stream) (void)hipStreamDestroy(h->stream);
@@ -732,6 +735,18 @@ extern "C" int mps_run(mps_handle h, int64_t max_steps, mps_report* rep) {
     r.device_ms = total_ms;
     if (rep) *rep = r;
     return status_code(r.status);
+}
+
+extern "C" int mps_element_outputs(mps_handle h, double* out) {
+    if (!h || !out) return MPS_ERR_ARG;
+    (void)hipSetDevice(h->device);
+    const size_t bytes = (size_t)h->I.Ne * 21 * 8;
+    if (!h->eout) HCHK(h, hipMalloc(&h->eout, bytes));
+    hipError_t e = mpsk::launch_elem_outputs(h->I, h->eout, h->stream);
+    if (e != hipSuccess) return h->fail(MPS_ERR_HIP, fmt("element outputs: %s", hipGetErrorString(e)));
+    HCHK(h, hipMemcpyAsync(out, h->eout, bytes, hipMemcpyDeviceToHost, h->stream));
+    HCHK(h, hipStreamSynchronize(h->stream));
+    return MPS_OK;
 }
 
 extern "C" int mps_get_history(mps_handle h, int64_t first, int64_t count, double* time, double* dt, double* e_def,

@@ -15,6 +15,8 @@ hipError_t launch_setup(const MpsInst& I, double* mslot, double* l0out, unsigned
 hipError_t launch_fint(const MpsInst& I, double* fint, hipStream_t st);
 hipError_t launch_energy(const MpsInst& I, hipStream_t st);
 int energy_blocks(const MpsInst& I);
+// Element fields of the VTK output: out[21*Ne] (Cauchy 9, strain 9, VMS, volumetric, equivalent strain).
+hipError_t launch_elem_outputs(const MpsInst& I, double* out, hipStream_t st);
 hipError_t launch_init_ctl(const MpsInst& I, const unsigned long long* dt0, hipStream_t st);
 // Persistent batched phases: 0 element, 1 node (fuse flag), 2 begin.  bs in {64,128,256}.
 // The batch must be homogeneous in precision, element type (nen) and law.

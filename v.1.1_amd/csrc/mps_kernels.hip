@@ -529,6 +529,114 @@ __device__ __forceinline__ void elem_energy(const MpsInst& I, int e, double* out
     out[2] = vol;
 }
 
+// ---------------------------------------------------------------------------
+// Element fields of the VTK output (output.cpp:567-820): per element the GP averages of the
+// state quantities updateConstitutive leaves (Cauchy, Green-Lagrange strain -- the small strain for
+// the linear law --, von Mises stress, volumetric and equivalent strain; maths.cpp:1241-1299).
+// Quantities a law does not set are 0 (classStates.cpp:15-44).  FP64 in every precision form.
+// ---------------------------------------------------------------------------
+__device__ void gp_outputs(int law, const double* F, double pa, double pb, double* q) {
+    double H[9], P[9];
+    double* cau = q;
+    double* E = q + 9;
+#pragma unroll
+    for (int i = 0; i < 21; i++) q[i] = 0.0;
+#pragma unroll
+    for (int i = 0; i < 9; i++) H[i] = F[i] - ((i % 4) == 0 ? 1.0 : 0.0);
+    if (law == 1 || law == 2) {
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++)
+                E[i * 3 + j] = 0.5 * (H[i * 3 + j] + H[j * 3 + i] + H[i] * H[j] + H[3 + i] * H[3 + j] + H[6 + i] * H[6 + j]);
+    } else if (law == 3) {
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++) E[i * 3 + j] = 0.5 * (H[i * 3 + j] + H[j * 3 + i]);
+    }
+    const double J = det3(F);
+    if (law == 1) {  // sigma = mu J^{-5/3} B + (-(mu/3) tr B J^{-5/3} + K (J-1)) I
+        double B[9];
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++) B[i * 3 + j] = F[i * 3] * F[j * 3] + F[i * 3 + 1] * F[j * 3 + 1] + F[i * 3 + 2] * F[j * 3 + 2];
+        const double f1 = pow(J, 5.0 / 3.0);
+        const double f3 = -(pa / 3.0) * (B[0] + B[4] + B[8]) / f1 + pb * (J - 1.0);
+        for (int i = 0; i < 9; i++) cau[i] = (pa / f1) * B[i] + ((i % 4) == 0 ? f3 : 0.0);
+    } else {
+        if (law == 0) pk1<0>(H, pa, pb, P);
+        else if (law == 2) pk1<2>(H, pa, pb, P);
+        else pk1<3>(H, pa, pb, P);
+        const double rJ = 1.0 / J;
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++)
+                cau[i * 3 + j] = (P[i * 3] * F[j * 3] + P[i * 3 + 1] * F[j * 3 + 1] + P[i * 3 + 2] * F[j * 3 + 2]) * rJ;
+    }
+    q[18] = sqrt(0.5 * ((cau[0] - cau[4]) * (cau[0] - cau[4]) + (cau[4] - cau[8]) * (cau[4] - cau[8]) +
+                        (cau[8] - cau[0]) * (cau[8] - cau[0]) + 6.0 * (cau[1] * cau[1] + cau[2] * cau[2] + cau[5] * cau[5])));
+    if (law == 1 || law == 2) {
+        q[19] = E[0] + E[4] + E[8];
+        q[20] = 0.6666 * sqrt(1.5 * (E[0] * E[0] + E[4] * E[4] + E[8] * E[8]) + 3.0 * (E[1] * E[1] + E[2] * E[2] + E[5] * E[5]));
+    }
+}
+
+__global__ __launch_bounds__(256) void k_elem_outputs(MpsInst I, double* __restrict__ out) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= I.Ne) return;
+    const size_t N = (size_t)I.N;
+    double pa, pb, c;
+    elem_mat(I, e, pa, pb, c);
+    double acc[21], q[21];
+    for (int i = 0; i < 21; i++) acc[i] = 0.0;
+    int ngp;
+    if (I.nen == 8) {
+        double x[8][3], uu[8][3];
+        for (int a = 0; a < 8; a++) {
+            const int nd = I.conn[(size_t)e * 8 + a];
+            for (int d = 0; d < 3; d++) {
+                x[a][d] = I.X[d * N + nd];
+                uu[a][d] = I.u[d * N + nd];
+            }
+        }
+        double cx[7][3], cu[7][3], Ax[7][3], Au[7][3];
+        hex_butterfly(x, cx);
+        hex_butterfly(uu, cu);
+        hex_scale<double>(cx, Ax);
+        hex_scale<double>(cu, Au);
+        for (int gp = 0; gp < 8; gp++) {
+            const int sx = sxi(gp), sy = seta(gp), sz = szeta(gp);
+            double Jm[9], Du[9], iJ[9], F[9];
+            hex_grad_at<double>(Ax, sx, sy, sz, Jm);
+            hex_grad_at<double>(Au, sx, sy, sz, Du);
+            inv3(Jm, iJ, det3(Jm));
+            for (int i = 0; i < 3; i++)
+                for (int j = 0; j < 3; j++)
+                    F[i * 3 + j] = (i == j ? 1.0 : 0.0) + Du[i * 3] * iJ[j] + Du[i * 3 + 1] * iJ[3 + j] + Du[i * 3 + 2] * iJ[6 + j];
+            gp_outputs(I.law, F, pa, pb, q);
+            for (int i = 0; i < 21; i++) acc[i] += q[i];
+        }
+        ngp = 8;
+    } else {
+        double x[4][3], uu[4][3];
+        for (int a = 0; a < 4; a++) {
+            const int nd = I.conn[(size_t)e * 4 + a];
+            for (int d = 0; d < 3; d++) {
+                x[a][d] = I.X[d * N + nd];
+                uu[a][d] = I.u[d * N + nd];
+            }
+        }
+        double Jm[9], Du[9], iJ[9], F[9];
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++) {
+                Jm[i * 3 + j] = x[j + 1][i] - x[0][i];
+                Du[i * 3 + j] = uu[j + 1][i] - uu[0][i];
+            }
+        inv3(Jm, iJ, det3(Jm));
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++)
+                F[i * 3 + j] = (i == j ? 1.0 : 0.0) + Du[i * 3] * iJ[j] + Du[i * 3 + 1] * iJ[3 + j] + Du[i * 3 + 2] * iJ[6 + j];
+        gp_outputs(I.law, F, pa, pb, acc);
+        ngp = 1;
+    }
+    for (int i = 0; i < 21; i++) out[(size_t)e * 21 + i] = acc[i] / (double)ngp;
+}
+
 __global__ __launch_bounds__(256) void k_energy(MpsInst I) {
     __shared__ double sh[3 * 256];
     const long long k = I.ctl->B;
@@ -614,6 +722,11 @@ hipError_t launch_fint(const MpsInst& I, double* fint, hipStream_t st) {
 }
 
 int energy_blocks(const MpsInst& I) { return (int)blocks_for(I.Ne, 256); }
+
+hipError_t launch_elem_outputs(const MpsInst& I, double* out, hipStream_t st) {
+    k_elem_outputs<<<blocks_for(I.Ne, 256), 256, 0, st>>>(I, out);
+    return hipGetLastError();
+}
 
 hipError_t launch_fill_u64(unsigned long long* p, long long n, unsigned long long v, hipStream_t st) {
     k_fill_u64<<<(unsigned)std::min<long long>((n + 255) / 256, 1024), 256, 0, st>>>(p, n, v);

@@ -279,6 +279,7 @@ bool nextValue(Reader &R, size_t i, double &v) {
 bool readSolvers(Reader &R, InpModel &m) {
     double scale = 1.0, t0 = 0.0, tf = 0.0;
     int numFiles = -1;
+    std::vector<int> outVars;
     bool instantaneous = false;
     for (size_t i = 0; i < R.L.size(); i++) {
         if (!starts(R.L[i], "*SOLVER")) continue;
@@ -318,8 +319,33 @@ bool readSolvers(Reader &R, InpModel &m) {
                 velocity = true;
                 if (!readDirichlet(R, i, m, s)) return false;
             } else if (starts(l, "*Outputs")) {
+                // outManagement (inputFile.cpp:4769-4829): number of files, then optionally the
+                // internal variables to print: "All, n" (0..n-1), one index k (printed as is), or
+                // "max, i1, i2, ..." (1-based, each <= max)
                 if (!nextValue(R, i, v)) return R.fail(i, "*Outputs needs the number of output files");
                 numFiles = (int)v;
+                if (numFiles < 1) return R.fail(i, "*Outputs needs at least one output file");
+                outVars.clear();
+                if (i + 2 < R.L.size() && !starts(R.L[i + 2], "*")) {
+                    const std::string &t = R.L[i + 2];
+                    std::vector<std::string> f = fields(t);
+                    long long x;
+                    if (starts(t, "All")) {
+                        if (!toInt(tail(t, 4), x)) return R.fail(i + 2, "*Outputs: \"All, n\" needs n");
+                        for (long long k = 0; k < x; k++) outVars.push_back((int)k);
+                    } else if (f.size() == 1) {
+                        if (!toInt(t, x)) return R.fail(i + 2, "*Outputs: malformed internal variable");
+                        outVars.push_back((int)x);
+                    } else {
+                        long long mx;
+                        if (!toInt(f[0], mx)) return R.fail(i + 2, "*Outputs: malformed internal variable list");
+                        for (size_t k = 1; k < f.size(); k++) {
+                            if (!toInt(f[k], x)) return R.fail(i + 2, "*Outputs: malformed internal variable list");
+                            if (x - 1 > mx) return R.fail(i + 2, "*Outputs: internal variable beyond the maximum");
+                            outVars.push_back((int)(x - 1));
+                        }
+                    }
+                }
             } else if (starts(l, "*MPS Fixed Steps")) {
                 if (!nextValue(R, i, v) || v < 1) return R.fail(i, "*MPS Fixed Steps needs a positive count");
                 s.fixedSteps = (int64_t)v;
@@ -353,6 +379,7 @@ bool readSolvers(Reader &R, InpModel &m) {
         s.tf = tf;
         s.timeSet = timeSeen;
         s.numFiles = numFiles;
+        s.outVars = outVars;
         s.bcMode = velocity ? MPS_BC_VELOCITY : (instantaneous ? MPS_BC_INST : MPS_BC_RAMP);
         if (s.numFiles < 0) return (R.err = "*Outputs is mandatory for the EXPLICIT solver", false);
         if (s.dirichlet.empty())

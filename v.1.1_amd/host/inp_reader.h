@@ -50,6 +50,7 @@ struct InpSolver {
     double t0 = 0.0, tf = 0.0;
     bool timeSet = false;
     int numFiles = -1;       // *Outputs first line; -1 = no *Outputs block
+    std::vector<int> outVars; // *Outputs optional second line: internal variables to print (outManagement)
     int numStepsIgnored = 0; // *Number Of Steps: parsed, not used by EXPLICIT (solverExplicit.cpp:158-171)
     int bcMode = 0;          // MPS_BC_RAMP / MPS_BC_INST / MPS_BC_VELOCITY
     int velRampSteps = 0;

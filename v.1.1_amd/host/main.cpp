@@ -9,6 +9,7 @@
 //   -precision FP64|FP32_ELEM   overrides *MPS Precision
 //   -check             read and validate the input, print a summary, run nothing
 //   -dumpU <file>      write the final displacement (SoA, one "%.17g" per line)
+//   -noVtk             skip the VTK series (CSV extractions are still written)
 #include <cstdint>
 #include <cstdio>
 #include <cstdlib>
@@ -23,7 +24,7 @@
 static void usage() {
     std::printf("mps_explicit: explicit hex8/tet4 neo-Hookean dynamics on an MI355X\n");
     std::printf("Usage: mps_explicit -input file.inp [-inputDir dir] [-outputDir dir] [-device n]\n");
-    std::printf("       [-precision FP64|FP32_ELEM] [-check] [-dumpU file]\n");
+    std::printf("       [-precision FP64|FP32_ELEM] [-check] [-dumpU file] [-noVtk]\n");
 }
 
 static int check(const std::string &path) {
@@ -64,9 +65,12 @@ static int check(const std::string &path) {
     for (size_t k = 0; k < m.solvers.size(); k++) {
         const mps::InpSolver &s = m.solvers[k];
         std::printf("%s{\"scale\": %.17g, \"t0\": %.17g, \"tf\": %.17g, \"outputs\": %d, \"bc_mode\": %d, "
-                    "\"dirichlet\": %zu, \"fixed_steps\": %lld, \"vel_ramp_steps\": %d, \"precision\": %d}",
+                    "\"dirichlet\": %zu, \"fixed_steps\": %lld, \"vel_ramp_steps\": %d, \"precision\": %d, "
+                    "\"out_vars\": [",
                     k ? ", " : "", s.scale, s.t0, s.tf, s.numFiles, s.bcMode, s.dirichlet.size(),
                     (long long)s.fixedSteps, s.velRampSteps, s.precision);
+        for (size_t j = 0; j < s.outVars.size(); j++) std::printf("%s%d", j ? ", " : "", s.outVars[j]);
+        std::printf("]}");
     }
     std::printf("], \"node_extractions\": [");
     bool first = true;
@@ -101,6 +105,8 @@ int main(int argc, char **argv) {
             return 0;
         } else if (!std::strcmp(a, "-check")) {
             checkOnly = true;
+        } else if (!std::strcmp(a, "-noVtk")) {
+            opt.writeVtk = false;
         } else if (!next) {
             std::fprintf(stderr, "ERROR ::: option %s needs a value\n", a);
             return 2;

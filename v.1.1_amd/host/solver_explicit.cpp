@@ -10,6 +10,7 @@
 #include <ctime>
 
 #include "mps_gpu.h"
+#include "vtk_output.h"
 
 namespace mps {
 namespace {
@@ -143,11 +144,26 @@ int classSolverExplicit::calculate(const InpModel &m, const std::vector<std::uni
     std::vector<double> ht(kHistoryCap), hdef(kHistoryCap), hkin(kHistoryCap), hvol(kHistoryCap);
 
     const long long chunk = nodal ? 1 : (extract.needsEnergies() ? kHistoryCap : 1LL << 40);
-    long long nextInfo = sampler;
+    // VTK series (outputsManagement, solverExplicit.cpp:173-186, 310-322): at t0 when t0 < 1e-12, then
+    // whenever `sampler` steps have passed since the last output, and at the last step.
+    VtkSeriesWriter *vtk = opt.vtk;
+    std::vector<double> ou, ov, oa, eout;
+    auto output = [&](double time) -> int {
+        if (!vtk) return 0;
+        ou.resize(3 * (size_t)N);
+        ov.resize(3 * (size_t)N);
+        oa.resize(3 * (size_t)N);
+        eout.resize(21 * (size_t)Ne);
+        if (mps_get_state(h, ou.data(), ov.data(), oa.data()) != MPS_OK) return fail("mps_get_state");
+        if (mps_element_outputs(h, eout.data()) != MPS_OK) return fail("mps_element_outputs");
+        return vtk->write(m, ou, ov, oa, eout, _s.outVars, time, err) ? 0 : -1;
+    };
+    if (_s.t0 < 1E-12 && output(_s.t0) != 0) return -1;
+    long long sampler2 = 0;
     mps_report rep{};
     for (;;) {
         const long long before = rep.steps;
-        const int rc = mps_run(h, chunk, &rep);
+        const int rc = mps_run(h, std::min(chunk, std::max(1LL, sampler - sampler2)), &rep);
         if (rc == MPS_ERR_NAN || rep.status == MPS_ERR_NAN) {
             err = "There are NANs in the displacements. Try a smaller scale factor";
             return -1;
@@ -197,9 +213,11 @@ int classSolverExplicit::calculate(const InpModel &m, const std::vector<std::uni
                 }
             }
         }
-        if (rep.steps >= nextInfo || rep.status == MPS_DONE) {
+        sampler2 += done;
+        if (sampler2 >= sampler || rep.status == MPS_DONE) {
+            if (output(rep.time) != 0) return -1;
+            sampler2 = 0;
             log.info("Time %g out of %g", rep.time, _s.fixedSteps > 0 ? rep.time : _s.tf);
-            while (nextInfo <= rep.steps) nextInfo += sampler;
         }
         if (rep.status == MPS_DONE) break;
         if (done == 0) {
@@ -236,7 +254,10 @@ int runInputFile(const std::string &path, const RunOptions &opt, std::string &er
     }
     classExtractData extract;
     if (!extract.init(model, err)) return -1;
-    if (!extract.empty()) {
+    VtkSeriesWriter vtk(opt.outputDir);
+    RunOptions o2 = opt;
+    if (opt.writeVtk) o2.vtk = &vtk;
+    if (!extract.empty() || opt.writeVtk) {
         struct stat st;
         if (stat(opt.outputDir.c_str(), &st) != 0 && mkdir(opt.outputDir.c_str(), 0755) != 0) {
             err = "cannot create output directory " + opt.outputDir;
@@ -247,7 +268,7 @@ int runInputFile(const std::string &path, const RunOptions &opt, std::string &er
     for (size_t k = 0; k < model.solvers.size(); k++) {
         classSolverExplicit s(model.solvers[k]);
         if (k > 0) s.initValues(u, v, a);
-        if (s.calculate(model, laws, matOfElem, extract, opt, err) != 0) return -1;
+        if (s.calculate(model, laws, matOfElem, extract, o2, err) != 0) return -1;
         s.getFinalValues(u, v, a);
     }
     extract.finalise();

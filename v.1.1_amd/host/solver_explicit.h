@@ -20,8 +20,12 @@
 
 namespace mps {
 
+class VtkSeriesWriter;
+
 struct RunOptions {
     std::string outputDir = "output";
+    bool writeVtk = true;           // VTK series as the reference writes it (outputsManagement)
+    VtkSeriesWriter *vtk = nullptr; // set by runInputFile: one writer numbers the files of all solvers
     int device = 0;
     int precision = -1; // -1: as the input says; else MPS_PREC_*
     bool quiet = false;

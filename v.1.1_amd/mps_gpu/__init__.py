@@ -96,6 +96,8 @@ def load(path=None):
     L.mps_get_mass.argtypes = [H, _dp]
     L.mps_internal_force.argtypes = [H, _dp, _dp]
     L.mps_critical_dt.argtypes = [H, _dp, _dp]
+    L.mps_element_outputs.argtypes = [H, _dp]
+    L.mps_get_dofs.argtypes = [H, C.c_int32, C.c_int64, _ip, _dp]
     L.mps_destroy.argtypes = [H]
     L.mps_last_error.restype = C.c_char_p
     L.mps_last_error.argtypes = [H]
@@ -207,6 +209,7 @@ class Solver:
         if rc != 0:
             raise MpsError(rc, self.L.mps_last_error(None).decode())
         self.n = self.inp.n_nodes
+        self.ne = int(mesh["n_elem"])
 
     def _chk(self, rc):
         if rc != 0:
@@ -234,6 +237,12 @@ class Solver:
         out = [np.zeros(count) for _ in range(5)]
         self._chk(self.L.mps_get_history(self.h, first, count, *[_d(x) for x in out]))
         return dict(zip(("time", "dt", "e_def", "e_kin", "volume"), out))
+
+    def element_outputs(self):
+        """Per element [n_elem, 21]: GP-averaged Cauchy (9), strain (9), VMS, volumetric, equivalent strain."""
+        out = np.zeros(21 * self.ne)
+        self._chk(self.L.mps_element_outputs(self.h, _d(out)))
+        return out.reshape(-1, 21)
 
     def mass(self):
         m = np.zeros(self.n)

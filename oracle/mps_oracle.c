@@ -504,6 +504,9 @@ typedef struct {
     int32_t *rowptr, *col; /* CSR node -> (e*nen + a) ascending */
     double *fe;        /* [Ne*nen*3] element forces, [(e*nen+a)*3+i] */
     double *fint;      /* [3N] */
+    double *fgrav;     /* [N] gravity load along z (NULL: none) */
+    double *fext;      /* [3N] external force of the current step */
+    double *fm;        /* [3*n_force] F/(tf-t0) per nodal force (classNeumann ctor, classNodes.cpp:85-96) */
     /* threading */
     int nt;
     double *tmin;      /* per-thread partials */
@@ -568,6 +571,32 @@ static int state_init(orc_state *s, const orc_problem *p)
         s->c[e] = (p->law == ORC_LAW_SVK || p->law == ORC_LAW_LINEAR) ? sqrt(E / s->rho[e])
                                                                       : sqrt((s->lam[e] + 2 * s->mu[e]) / s->rho[e]);
     }
+    /* Neumann loads */
+    s->fext = calloc(3 * N, sizeof(double));
+    if (!s->fext) return ORC_ERR_MEM;
+    if (p->n_force > 0) {
+        s->fm = malloc(sizeof(double) * 3 * (size_t)p->n_force);
+        if (!s->fm) return ORC_ERR_MEM;
+        for (int k = 0; k < p->n_force; k++) {
+            if (p->force_node[k] < 0 || p->force_node[k] >= s->N) return ORC_ERR_ARG;
+            for (int d = 0; d < 3; d++) s->fm[3 * k + d] = p->force_vec[3 * k + d] / (p->force_tf[k] - p->force_t0[k]);
+        }
+    }
+    if (p->gravity != 0.0) {  /* BCsGravity: force[n_a + 2N] += -w * rho * |J| * g * phi_a, GP by GP */
+        s->fgrav = calloc(N, sizeof(double));
+        if (!s->fgrav) return ORC_ERR_MEM;
+        for (size_t e = 0; e < Ne; e++) {
+            double xyz[24];
+            gather_xyz(s, (int)e, NULL, xyz);
+            for (size_t g = 0; g < ng; g++) {
+                double phi[8], dphi[24], wJ, detJ;
+                orc_define_gp(p->nen, xyz, xw + g * 4, phi, dphi, &wJ, &detJ);
+                const double w = xw[g * 4 + 3], J = fabs(detJ);
+                for (size_t a = 0; a < nen; a++)
+                    s->fgrav[p->conn[e * nen + a]] += -w * s->rho[e] * J * p->gravity * phi[a];
+            }
+        }
+    }
     /* active DOF map (dispBCManagement, inputFile.cpp:3185-3212) */
     memset(s->active, 1, 3 * N);
     for (int k = 0; k < p->n_dir; k++) {
@@ -600,7 +629,7 @@ static void state_free(orc_state *s)
     free(s->phi); free(s->dphi); free(s->wJ); free(s->l0); free(s->c);
     free(s->lam); free(s->mu); free(s->K1); free(s->rho); free(s->mass);
     free(s->active); free(s->rowptr); free(s->col); free(s->fe); free(s->fint);
-    free(s->tmin); free(s->tsum);
+    free(s->tmin); free(s->tsum); free(s->fgrav); free(s->fext); free(s->fm);
 }
 
 /* ------------------------------------------------------------------ */
@@ -731,6 +760,20 @@ static void task_gather(orc_state *s, int tid, int nt, void *arg)
             for (int32_t k = s->rowptr[n]; k < s->rowptr[n + 1]; k++) acc += s->fe[(size_t)s->col[k] * 3 + i];
             s->fint[n + N * i] = acc;
         }
+}
+
+/* fext at time t: zero, nodal forces in input order (BCsNodalForces, boundaryConditions.cpp:1120-1134 with
+ * classNeumann::getTracAmount: F/(tf-t0) * t for t0 <= t <= tf, else 0), then gravity. */
+static void external_force(orc_state *s, double t)
+{
+    const orc_problem *p = s->p;
+    size_t N = s->N;
+    for (size_t i = 0; i < 3 * N; i++) s->fext[i] = 0;
+    for (int k = 0; k < p->n_force; k++)
+        if (t >= p->force_t0[k] && t <= p->force_tf[k])
+            for (int d = 0; d < 3; d++) s->fext[p->force_node[k] + N * d] += s->fm[3 * k + d] * t;
+    if (s->fgrav)
+        for (size_t n = 0; n < N; n++) s->fext[n + 2 * N] += s->fgrav[n];
 }
 
 /* computeInternalForce: solverExplicit.cpp:363-379 (+ computeState, solvers.cpp:615-641) */
@@ -956,6 +999,18 @@ static void gp_state_outputs(int32_t law, const double *F, double lam, double mu
     }
 }
 
+int32_t orc_external_force(const orc_problem *p, double t, double *fext)
+{
+    orc_state s;
+    int rc = setup(&s, p);
+    if (rc == ORC_OK) {
+        external_force(&s, t);
+        memcpy(fext, s.fext, sizeof(double) * 3 * (size_t)s.N);
+    }
+    state_free(&s);
+    return rc;
+}
+
 int32_t orc_element_outputs(const orc_problem *p, const double *u, double *out)
 {
     orc_state s;
@@ -1062,10 +1117,12 @@ int32_t orc_run(const orc_problem *p, double *u, double *v, double *a, orc_histo
         }
         /* computeState + computeRightHandSide: fint(u) */
         internal_force(&s, u);
+        /* NeumannBCManagement at timeRun (boundaryConditions.cpp:75-140) */
+        external_force(&s, timeRun);
         /* systemSolve, nonlinearSystem.cpp:1096-1107: a = (-fint + fext)/m, v update */
         for (size_t i = 0; i < 3 * N; i++) {
             if (!s.active[i]) continue;
-            double rhs = -s.fint[i] + 0.0;
+            double rhs = -s.fint[i] + s.fext[i];
             double m = s.mass[i % N];
             double anew = (m != 0.0) ? rhs / m : 0.0;
             v[i] += dt * ((1. - 0.5) * a[i] + 0.5 * anew);

@@ -53,6 +53,14 @@ typedef struct orc_problem {
     int32_t n_threads;      /* >1 requires ORC_SUM_ELEMENT */
     int32_t energy_every;   /* 0: no energies; k: every k-th step (and the last) */
     int32_t _pad;
+    /* Neumann loads, fext of rhs = -fint + fext (NeumannBCManagement, boundaryConditions.cpp:75-140) */
+    double gravity;            /* *GRAVITY g (0: none): f_z += -w rho |J| g phi_a per GP (:1146-1172) */
+    int32_t n_force;           /* *BOUNDARY, TYPE=FORCE nodes */
+    int32_t _pad2;
+    const int32_t *force_node; /* [n_force] 0-based node */
+    const double *force_vec;   /* [3*n_force] F * Ff (inputFile.cpp:2905-2914) */
+    const double *force_t0;    /* [n_force] load window; F(t) = F/(tf-t0) * t inside it (classNodes.h:101-108) */
+    const double *force_tf;
 } orc_problem;
 
 typedef struct orc_report {
@@ -110,6 +118,8 @@ double orc_critical_dt(const orc_problem *p, const double *u);
  * the linear law), von Mises stress, volumetric and equivalent strain, as the VTK writer prints them
  * (output.cpp:567-820). */
 int32_t orc_element_outputs(const orc_problem *p, const double *u, double *out);
+/* External force fext[3*n_nodes] at time t (gravity + nodal forces). */
+int32_t orc_external_force(const orc_problem *p, double t, double *fext);
 
 #ifdef __cplusplus
 }

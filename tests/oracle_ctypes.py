@@ -32,6 +32,8 @@ class Problem(C.Structure):
         ("t0", C.c_double), ("tf", C.c_double), ("scale", C.c_double),
         ("n_steps", C.c_int64), ("sum_order", C.c_int32), ("n_threads", C.c_int32),
         ("energy_every", C.c_int32), ("_pad", C.c_int32),
+        ("gravity", C.c_double), ("n_force", C.c_int32), ("_pad2", C.c_int32),
+        ("force_node", _ip), ("force_vec", _dp), ("force_t0", _dp), ("force_tf", _dp),
     ]
 
 
@@ -79,6 +81,8 @@ def lib():
         L.orc_lumped_mass.argtypes = [C.POINTER(Problem), _dp]
         L.orc_internal_force.restype = C.c_int32
         L.orc_element_outputs.restype = C.c_int32
+        L.orc_external_force.restype = C.c_int32
+        L.orc_external_force.argtypes = [C.c_void_p, C.c_double, _dp]
         L.orc_internal_force.argtypes = [C.POINTER(Problem), _dp, _dp]
         L.orc_critical_dt.restype = C.c_double
         L.orc_critical_dt.argtypes = [C.POINTER(Problem), _dp]
@@ -102,7 +106,8 @@ class OracleProblem:
     def __init__(self, mesh, law=LAW_NH_COMPRESSIBLE, E=1e6, nu=0.3, rho=1000.0,
                  dir_dof=None, dir_val=None, bc_mode=BC_VELOCITY, vel_ramp_steps=100,
                  t0=0.0, tf=1.0, scale=0.5, n_steps=0, sum_order=SUM_REFERENCE,
-                 n_threads=1, energy_every=0):
+                 n_threads=1, energy_every=0, gravity=0.0, forces=None):
+        """forces: None or (node[k], vec[k,3], t0[k], tf[k]) nodal loads (*BOUNDARY, TYPE=FORCE)."""
         self.X = np.ascontiguousarray(mesh["X"], dtype=np.float64)
         self.conn = np.ascontiguousarray(mesh["conn"], dtype=np.int32)
         self.E = np.atleast_1d(np.asarray(E, dtype=np.float64)).copy()
@@ -118,6 +123,12 @@ class OracleProblem:
         self.dir_dof = np.ascontiguousarray(dir_dof, dtype=np.int32)
         self.dir_val = np.ascontiguousarray(dir_val, dtype=np.float64)
         self.n_nodes = mesh["n_nodes"]
+        fn, fv, f0, f1 = forces if forces is not None else ([], np.zeros((0, 3)), [], [])
+        self.f_node = np.ascontiguousarray(fn, dtype=np.int32)
+        self.f_vec = np.ascontiguousarray(np.asarray(fv, dtype=np.float64).ravel())
+        self.f_t0 = np.ascontiguousarray(f0, dtype=np.float64)
+        self.f_tf = np.ascontiguousarray(f1, dtype=np.float64)
+        nf = int(self.f_node.size)
         self.p = Problem(
             n_nodes=mesh["n_nodes"], n_elem=mesh["n_elem"], nen=mesh["nen"], law=law,
             X=dptr(self.X), conn=iptr(self.conn), mat_per_elem=per,
@@ -127,7 +138,9 @@ class OracleProblem:
             dir_val=dptr(self.dir_val) if self.dir_val.size else None,
             bc_mode=bc_mode, vel_ramp_steps=vel_ramp_steps, t0=t0, tf=tf, scale=scale,
             n_steps=n_steps, sum_order=sum_order, n_threads=n_threads,
-            energy_every=energy_every)
+            energy_every=energy_every, gravity=gravity, n_force=nf,
+            force_node=iptr(self.f_node) if nf else None, force_vec=dptr(self.f_vec) if nf else None,
+            force_t0=dptr(self.f_t0) if nf else None, force_tf=dptr(self.f_tf) if nf else None)
 
     def run(self, u0=None, v0=None, a0=None, hist_cap=0):
         n3 = 3 * self.n_nodes
@@ -166,6 +179,12 @@ class OracleProblem:
         rc = lib().orc_element_outputs(C.byref(self.p), dptr(u), dptr(out))
         assert rc == 0
         return out.reshape(-1, 21)
+
+    def external_force(self, t):
+        f = np.zeros(3 * self.n_nodes)
+        rc = lib().orc_external_force(C.byref(self.p), t, dptr(f))
+        assert rc == 0
+        return f
 
     def critical_dt(self, u=None):
         if u is not None:

@@ -532,3 +532,72 @@ def test_element_outputs_homogeneous_deformation(law):
         exp[20] = 0.6666 * np.sqrt(1.5 * np.sum(np.diag(E) ** 2) + 3 * (E[0, 1] ** 2 + E[0, 2] ** 2 + E[1, 2] ** 2))
     for row in q:
         np.testing.assert_allclose(row, exp, rtol=1e-9, atol=1e-9 * np.abs(exp).max())
+
+
+# ------------------------------------------------------------------ Neumann loads
+
+def test_gravity_load_totals_and_lumped_mass():
+    """BCsGravity (boundaryConditions.cpp:1146-1172): f_z = -g sum_gp w rho |J| phi_a, i.e. -g times the row-sum
+    lumped mass node by node, and -g rho V in total."""
+    m = perturb(box(4, 3, 3), 0.1, 3)
+    for nen_tet in (False, True):
+        mm = perturb(box(3, 3, 2, tet=True), 0.1, 4) if nen_tet else m
+        p = O.OracleProblem(mm, rho=1200.0, n_steps=1, gravity=9.81)
+        f = p.external_force(0.37)
+        N = mm["n_nodes"]
+        assert np.all(f[:2 * N] == 0.0)
+        mass = p.mass()
+        np.testing.assert_allclose(f[2 * N:], -9.81 * mass, rtol=1e-12)
+        vol = 1.0 if not nen_tet else 1.0  # unit box, Sum wJ = V for the affine split as well
+        assert f[2 * N:].sum() == pytest.approx(-9.81 * 1200.0 * vol * (6 * 0.166666666667 if nen_tet else 1.0),
+                                                rel=1e-11)
+
+
+def test_nodal_force_window_and_ramp():
+    """classNeumann::getTracAmount (classNodes.h:101-108): F/(tf-t0) * t inside [t0, tf] -- t, not t - t0 --
+    and 0 outside, also after tf; entries on one node add up."""
+    m = box(2, 2, 2)
+    N = m["n_nodes"]
+    forces = ([5, 5, 9], np.array([[1.0, -2.0, 3.0], [0.5, 0.0, 0.0], [0.0, 0.0, 7.0]]), [0.2, 0.0, 0.1], [0.6, 1.0, 0.3])
+    p = O.OracleProblem(m, n_steps=1, forces=forces)
+    for t in (0.0, 0.1, 0.2, 0.25, 0.3, 0.6, 0.61, 1.0, 1.2):
+        f = p.external_force(t)
+        exp = np.zeros(3 * N)
+        for node, vec, t0, tf in zip(*forces):
+            if t0 <= t <= tf:
+                for d in range(3):
+                    exp[node + N * d] += vec[d] / (tf - t0) * t
+        np.testing.assert_allclose(f, exp, rtol=1e-15, atol=0)
+
+
+def test_free_fall_and_momentum_under_nodal_forces():
+    """A free body under gravity: every node accelerates at exactly -g, so u_z follows the scalar
+    central-difference recurrence with a_0 = 0 (solverExplicit.cpp:143) -- -g dt^2 n(n-1)/2 at constant dt,
+    not -g T^2/2 -- and x, y stay at rest; under nodal forces the momentum follows the trapezoid sum
+    sum_k dt_k (F(t_{k-1}) + F(t_k)) / 2 with F(t_0) = 0."""
+    m = perturb(box(3, 3, 3), 0.1, 8)
+    N = m["n_nodes"]
+    p = O.OracleProblem(m, n_steps=40, bc_mode=O.BC_VELOCITY, gravity=9.81)
+    r = p.run(hist_cap=40)
+    uz, vz, az = 0.0, 0.0, 0.0
+    for dt in r["hist"]["dt"][:40]:
+        uz += dt * vz + 0.5 * dt * dt * az
+        vz += dt * 0.5 * (az + (-9.81))
+        az = -9.81
+    np.testing.assert_allclose(r["u"][2 * N:], uz, rtol=1e-10)
+    np.testing.assert_allclose(r["v"][2 * N:], vz, rtol=1e-10)
+    assert np.abs(r["u"][:2 * N]).max() < 1e-12 * abs(uz)
+    forces = ([0, 13, 40], np.array([[1e3, 0.0, 0.0], [0.0, -2e3, 5e2], [3e2, 3e2, 3e2]]), [0.0, 0.0, 0.0],
+              [1.0, 1.0, 1.0])
+    p = O.OracleProblem(m, n_steps=30, bc_mode=O.BC_VELOCITY, forces=forces)
+    r = p.run(hist_cap=30)
+    mass = p.mass()
+    mom = np.array([(mass * r["v"][d * N:(d + 1) * N]).sum() for d in range(3)])
+    Ftot = lambda t: sum(vec * t for vec in forces[1])  # all windows [0, 1] contain the 30 steps
+    imp = np.zeros(3)
+    tprev = 0.0
+    for k in range(30):
+        t, dt = r["hist"]["time"][k], r["hist"]["dt"][k]
+        imp += dt * 0.5 * ((Ftot(tprev) if k > 0 else 0.0) + Ftot(t))
+        tprev = t
+    np.testing.assert_allclose(mom, imp, rtol=1e-9)

@@ -84,6 +84,18 @@ typedef struct mps_bcs {
     const double *value; /* [n_dirichlet] U (ramp/inst) or V (velocity) */
 } mps_bcs;
 
+/* Neumann loads of the path: nodal forces and gravity, added into one external-force vector in input
+ * order each step (NeumannBCManagement, boundaryConditions.cpp:75-140). */
+typedef struct mps_loads {
+    int32_t n_force;     /* *BOUNDARY, TYPE=FORCE entries (one per node line), in input order */
+    int32_t gravity_pos; /* gravity is added before force entry gravity_pos (outside 0..n_force: after all) */
+    double gravity;      /* *GRAVITY g (0: none): f_z += -w rho |J| g phi_a per Gauss point (:1146-1172) */
+    const int32_t *node; /* [n_force] 0-based node */
+    const double *force; /* [3*n_force] F * Ff, the force reached at tf (inputFile.cpp:2905-2914) */
+    const double *t0;    /* [n_force] window: F/(tf-t0) * t for t0 <= t <= tf, else 0 (classNodes.h:101-108); */
+    const double *tf;    /* [n_force]         tf > t0 required */
+} mps_loads;
+
 typedef struct mps_solver_cfg {
     double t0, tf;       /* *Time t0, tf */
     double scale;        /* *Scale Factor */
@@ -116,6 +128,18 @@ int mps_create(const mps_mesh *mesh, const mps_material *mat, const mps_bcs *bcs
  * Replaces: solvers::initValues (solvers.h:171-187) + setLastSolutionsToSystem
  * (solverExplicit.cpp:577-596). */
 int mps_set_state(mps_handle h, const double *u, const double *v, const double *a);
+
+/* Set the Neumann loads (replacing any earlier call; NULL or no entries and g = 0: none).  Valid only
+ * before the first step.  The loads are evaluated in the node kernel at each step's time.
+ * Replaces: the FORCE and GRAVITY entries of NeumannCommonManagement (forceBCManagement,
+ * inputFile.cpp:2841-2941; *GRAVITY :4723-4735) and their per-step evaluation in NeumannBCManagement
+ * (boundaryConditions.cpp:75-140). */
+int mps_set_loads(mps_handle h, const mps_loads *loads);
+
+/* External force fext[3*n_nodes] at time t from the loads of mps_set_loads (device kernel; does not
+ * touch the solver state).
+ * Replaces: NeumannBCManagement (boundaryConditions.cpp:75-140) for FORCE and GRAVITY. */
+int mps_external_force(mps_handle h, double t, double *fext);
 
 /* Copy the current u, v, a to host (any pointer may be NULL).
  * Replaces: solvers::getFinalValues (solvers.h:171-187). */

@@ -504,7 +504,7 @@ typedef struct {
     int32_t *rowptr, *col; /* CSR node -> (e*nen + a) ascending */
     double *fe;        /* [Ne*nen*3] element forces, [(e*nen+a)*3+i] */
     double *fint;      /* [3N] */
-    double *fgrav;     /* [N] gravity load along z (NULL: none) */
+    double *tgrav;     /* [Ne*ngp] gravity factor -w rho |J| g per GP (NULL: no gravity) */
     double *fext;      /* [3N] external force of the current step */
     double *fm;        /* [3*n_force] F/(tf-t0) per nodal force (classNeumann ctor, classNodes.cpp:85-96) */
     /* threading */
@@ -582,9 +582,9 @@ static int state_init(orc_state *s, const orc_problem *p)
             for (int d = 0; d < 3; d++) s->fm[3 * k + d] = p->force_vec[3 * k + d] / (p->force_tf[k] - p->force_t0[k]);
         }
     }
-    if (p->gravity != 0.0) {  /* BCsGravity: force[n_a + 2N] += -w * rho * |J| * g * phi_a, GP by GP */
-        s->fgrav = calloc(N, sizeof(double));
-        if (!s->fgrav) return ORC_ERR_MEM;
+    if (p->gravity != 0.0) {  /* BCsGravity (:1146-1172): per GP the factor -w * rho * |J| * g, times phi_a */
+        s->tgrav = malloc(sizeof(double) * Ne * ng);
+        if (!s->tgrav) return ORC_ERR_MEM;
         for (size_t e = 0; e < Ne; e++) {
             double xyz[24];
             gather_xyz(s, (int)e, NULL, xyz);
@@ -592,8 +592,7 @@ static int state_init(orc_state *s, const orc_problem *p)
                 double phi[8], dphi[24], wJ, detJ;
                 orc_define_gp(p->nen, xyz, xw + g * 4, phi, dphi, &wJ, &detJ);
                 const double w = xw[g * 4 + 3], J = fabs(detJ);
-                for (size_t a = 0; a < nen; a++)
-                    s->fgrav[p->conn[e * nen + a]] += -w * s->rho[e] * J * p->gravity * phi[a];
+                s->tgrav[e * ng + g] = -w * s->rho[e] * J * p->gravity;
             }
         }
     }
@@ -629,7 +628,7 @@ static void state_free(orc_state *s)
     free(s->phi); free(s->dphi); free(s->wJ); free(s->l0); free(s->c);
     free(s->lam); free(s->mu); free(s->K1); free(s->rho); free(s->mass);
     free(s->active); free(s->rowptr); free(s->col); free(s->fe); free(s->fint);
-    free(s->tmin); free(s->tsum); free(s->fgrav); free(s->fext); free(s->fm);
+    free(s->tmin); free(s->tsum); free(s->tgrav); free(s->fext); free(s->fm);
 }
 
 /* ------------------------------------------------------------------ */
@@ -762,18 +761,34 @@ static void task_gather(orc_state *s, int tid, int nt, void *arg)
         }
 }
 
-/* fext at time t: zero, nodal forces in input order (BCsNodalForces, boundaryConditions.cpp:1120-1134 with
- * classNeumann::getTracAmount: F/(tf-t0) * t for t0 <= t <= tf, else 0), then gravity. */
+/* BCsGravity (boundaryConditions.cpp:1146-1172): force[n_a + 2N] += -w * rho * |J| * g * phi_a,
+ * GP by GP in element order, onto what the loads before it have added. */
+static void add_gravity(orc_state *s)
+{
+    size_t N = s->N, Ne = s->Ne, ng = s->ngp, nen = s->nen;
+    for (size_t e = 0; e < Ne; e++)
+        for (size_t g = 0; g < ng; g++) {
+            const double t = s->tgrav[e * ng + g];
+            const double *phi = s->phi + (e * ng + g) * nen;
+            for (size_t a = 0; a < nen; a++) s->fext[s->p->conn[e * nen + a] + 2 * N] += t * phi[a];
+        }
+}
+
+/* fext at time t (NeumannBCManagement, boundaryConditions.cpp:75-140): zero, then the loads in input
+ * order -- nodal forces (BCsNodalForces, :1120-1134, with classNeumann::getTracAmount,
+ * classNodes.h:101-108: F/(tf-t0) * t for t0 <= t <= tf, else 0) and gravity at its place. */
 static void external_force(orc_state *s, double t)
 {
     const orc_problem *p = s->p;
     size_t N = s->N;
+    const int gpos = (p->gravity_pos >= 0 && p->gravity_pos <= p->n_force) ? p->gravity_pos : p->n_force;
     for (size_t i = 0; i < 3 * N; i++) s->fext[i] = 0;
-    for (int k = 0; k < p->n_force; k++)
-        if (t >= p->force_t0[k] && t <= p->force_tf[k])
-            for (int d = 0; d < 3; d++) s->fext[p->force_node[k] + N * d] += s->fm[3 * k + d] * t;
-    if (s->fgrav)
-        for (size_t n = 0; n < N; n++) s->fext[n + 2 * N] += s->fgrav[n];
+    for (int k = 0; k < p->n_force; k++) {
+        if (k == gpos && s->tgrav) add_gravity(s);
+        const int in = (t >= p->force_t0[k] && t <= p->force_tf[k]);
+        for (int d = 0; d < 3; d++) s->fext[p->force_node[k] + N * d] += in ? s->fm[3 * k + d] * t : 0.0;
+    }
+    if (gpos == p->n_force && s->tgrav) add_gravity(s);
 }
 
 /* computeInternalForce: solverExplicit.cpp:363-379 (+ computeState, solvers.cpp:615-641) */

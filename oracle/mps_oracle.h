@@ -56,7 +56,8 @@ typedef struct orc_problem {
     /* Neumann loads, fext of rhs = -fint + fext (NeumannBCManagement, boundaryConditions.cpp:75-140) */
     double gravity;            /* *GRAVITY g (0: none): f_z += -w rho |J| g phi_a per GP (:1146-1172) */
     int32_t n_force;           /* *BOUNDARY, TYPE=FORCE nodes */
-    int32_t _pad2;
+    int32_t gravity_pos;       /* loads are added in input order: gravity before force entry gravity_pos
+                                  (outside 0..n_force: after all of them) */
     const int32_t *force_node; /* [n_force] 0-based node */
     const double *force_vec;   /* [3*n_force] F * Ff (inputFile.cpp:2905-2914) */
     const double *force_t0;    /* [n_force] load window; F(t) = F/(tf-t0) * t inside it (classNodes.h:101-108) */

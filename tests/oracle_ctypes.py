@@ -32,7 +32,7 @@ class Problem(C.Structure):
         ("t0", C.c_double), ("tf", C.c_double), ("scale", C.c_double),
         ("n_steps", C.c_int64), ("sum_order", C.c_int32), ("n_threads", C.c_int32),
         ("energy_every", C.c_int32), ("_pad", C.c_int32),
-        ("gravity", C.c_double), ("n_force", C.c_int32), ("_pad2", C.c_int32),
+        ("gravity", C.c_double), ("n_force", C.c_int32), ("gravity_pos", C.c_int32),
         ("force_node", _ip), ("force_vec", _dp), ("force_t0", _dp), ("force_tf", _dp),
     ]
 
@@ -106,8 +106,9 @@ class OracleProblem:
     def __init__(self, mesh, law=LAW_NH_COMPRESSIBLE, E=1e6, nu=0.3, rho=1000.0,
                  dir_dof=None, dir_val=None, bc_mode=BC_VELOCITY, vel_ramp_steps=100,
                  t0=0.0, tf=1.0, scale=0.5, n_steps=0, sum_order=SUM_REFERENCE,
-                 n_threads=1, energy_every=0, gravity=0.0, forces=None):
-        """forces: None or (node[k], vec[k,3], t0[k], tf[k]) nodal loads (*BOUNDARY, TYPE=FORCE)."""
+                 n_threads=1, energy_every=0, gravity=0.0, forces=None, gravity_pos=-1):
+        """forces: None or (node[k], vec[k,3], t0[k], tf[k]) nodal loads (*BOUNDARY, TYPE=FORCE);
+        gravity_pos: index of the force entry that gravity precedes in input order (-1: after all)."""
         self.X = np.ascontiguousarray(mesh["X"], dtype=np.float64)
         self.conn = np.ascontiguousarray(mesh["conn"], dtype=np.int32)
         self.E = np.atleast_1d(np.asarray(E, dtype=np.float64)).copy()
@@ -138,7 +139,7 @@ class OracleProblem:
             dir_val=dptr(self.dir_val) if self.dir_val.size else None,
             bc_mode=bc_mode, vel_ramp_steps=vel_ramp_steps, t0=t0, tf=tf, scale=scale,
             n_steps=n_steps, sum_order=sum_order, n_threads=n_threads,
-            energy_every=energy_every, gravity=gravity, n_force=nf,
+            energy_every=energy_every, gravity=gravity, n_force=nf, gravity_pos=gravity_pos,
             force_node=iptr(self.f_node) if nf else None, force_vec=dptr(self.f_vec) if nf else None,
             force_t0=dptr(self.f_t0) if nf else None, force_tf=dptr(self.f_tf) if nf else None)
 

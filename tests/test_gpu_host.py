@@ -127,6 +127,55 @@ def test_fixed_steps_velocity_fp32_and_materials(tmp_path):
     assert rel(u32, r["u"]) <= 1e-6
 
 
+def test_force_and_gravity_loads_from_input(tmp_path):
+    """*BOUNDARY, TYPE=FORCE blocks and *GRAVITY read from the file drive the device loads in input order
+    (two entries on one node with gravity between them); the ExternalForce and EXTERNAL_ENERGY extractions
+    follow them (dataExtraction.cpp:812-823, 866-874)."""
+    m = configs.perturb(configs.box(3, 3, 4), 0.1, 7)
+    N = m["n_nodes"]
+    X = m["X"].reshape(3, N)
+    bot = [n for n in range(N) if X[2, n] < 1e-12]
+    top = [n for n in range(N) if abs(X[2, n] - X[2].max()) < 1e-12]
+    bc = [(n, d, 0.0) for n in bot for d in range(3)]
+    dof = np.array([n + N * d for n, d, _ in bc], np.int32)
+    val = np.zeros(dof.size)
+    dt0 = O.OracleProblem(m, dir_dof=dof, dir_val=val, bc_mode=O.BC_RAMP, tf=1.0, scale=0.5).critical_dt()
+    tf = 80.5 * dt0
+    rng = np.random.default_rng(4)
+    F1, F2 = rng.uniform(-1e3, 1e3, (4, 3)), rng.uniform(-1e3, 1e3, (4, 3))
+    n1, n2 = top[:4], top[2:6]
+    w1, w2 = (0.0, 0.6 * tf, 2.0), (0.2 * tf, tf, 1.0)
+    lines = ["*BOUNDARY, TYPE=FORCE", f"{w1[0]:.17g}, {w1[1]:.17g}, {w1[2]:.17g}"]
+    lines += [f"{n + 1}, {f[0]:.17g}, {f[1]:.17g}, {f[2]:.17g}" for n, f in zip(n1, F1)]
+    lines += ["*GRAVITY", "9.81", "*BOUNDARY, TYPE=FORCE", f"{w2[0]:.17g}, {w2[1]:.17g}, {w2[2]:.17g}"]
+    lines += [f"{n + 1}, {f[0]:.17g}, {f[1]:.17g}, {f[2]:.17g}" for n, f in zip(n2, F2)]
+    inp = tmp_path / "loads.inp"
+    inpgen.write_inp(inp, m, [dict(rho=1000.0, E=1e6, nu=0.3)], [dict(scale=0.5, t0=0.0, tf=tf, outputs=2, bc=bc)],
+                     node_extractions=[("Fz", "ExternalForce", "Sum", 2, n2), ("Fx", "ExternalForce", "Rough", 0, n1)],
+                     element_extractions=[("W", "EXTERNAL_ENERGY", "Sum")], extra="\n".join(lines))
+    u, _ = run(inp, tmp_path, "-noVtk")
+    forces = (np.array(n1 + n2, np.int32), np.vstack([F1 * w1[2], F2 * w2[2]]),
+              np.array([w1[0]] * 4 + [w2[0]] * 4), np.array([w1[1]] * 4 + [w2[1]] * 4))
+    o = O.OracleProblem(m, dir_dof=dof, dir_val=val, bc_mode=O.BC_RAMP, t0=0.0, tf=tf, scale=0.5, gravity=9.81,
+                        forces=forces, gravity_pos=4)
+    r = o.run(hist_cap=4096)
+    steps = r["steps"]
+    assert steps > 70 and rel(u, r["u"]) <= 1e-12 and np.abs(u).max() > 0
+    times = r["hist"]["time"][:steps]
+    fext = [o.external_force(t) for t in times]
+    _, rows = read_csv(tmp_path / "Fz_ExternalForce2_Sum.csv")
+    assert rows.shape == (steps, 2) and np.array_equal(rows[:, 0], g16(times))
+    ref = np.array([sum(f[n + 2 * N] for n in n2) for f in fext])
+    np.testing.assert_allclose(rows[:, 1], ref, rtol=1e-13, atol=1e-13 * np.abs(ref).max())
+    head, rows = read_csv(tmp_path / "Fx_ExternalForce0_Rough.csv")
+    assert head == ["Time"] + [f"Node{n + 1}Comp0" for n in n1]
+    np.testing.assert_allclose(rows[:, 1:], g16(np.array([[f[n] for n in n1] for f in fext])).reshape(steps, 4),
+                               rtol=1e-15, atol=0)
+    _, rows = read_csv(tmp_path / "All_EXTERNAL_ENERGY_Sum.csv")
+    assert rows.shape == (steps, 2)
+    assert rows[-1, 1] == pytest.approx(float(np.dot(r["u"], fext[-1])), rel=1e-10)
+
+
 def parse_vtk(path):
     """Sections of a legacy ASCII VTK file written by the series writer: name -> list of value rows."""
     lines = open(path).read().split("\n")

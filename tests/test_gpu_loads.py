@@ -1,0 +1,161 @@
+"""Neumann loads on the HIP path (mps_set_loads / mps_external_force) against the oracle.
+
+The node kernel adds the external force of each step -- nodal force entries (*BOUNDARY, TYPE=FORCE)
+and gravity (*GRAVITY) in input order (NeumannBCManagement, boundaryConditions.cpp:75-140) -- before
+the lumped-mass solve.  Gravity is summed per node once at set-up, in the reference's element and
+Gauss-point order, so it is bitwise the oracle's wherever no force entry precedes it on the same
+node; where one does, the two differ by the rounding of the order only (1e-14 of the largest load).
+
+Tolerances as in test_gpu_parity: FP64 <= 1e-12, FP32-element <= 1e-6 (relative displacement).
+"""
+import numpy as np
+import pytest
+
+import mps_gpu as G
+import oracle_ctypes as O
+from meshgen import box, face_nodes, perturb
+
+pytestmark = pytest.mark.gpu
+
+TOL64 = 1e-12
+TOL32 = 1e-6
+
+
+def relerr(a, b):
+    return np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
+
+
+def clamp_bottom(mesh):
+    N = mesh["n_nodes"]
+    bot = face_nodes(mesh, 2, mesh["X"].reshape(3, -1)[2].min())
+    dd = np.array([n + N * d for n in bot for d in range(3)], np.int32)
+    return dd, np.zeros(dd.size)
+
+
+def top_forces(mesh, t0, tf, scale=2e3, seed=1):
+    """Force entries on the top face in input order, one node listed twice, a window per entry."""
+    rng = np.random.default_rng(seed)
+    top = face_nodes(mesh, 2, mesh["X"].reshape(3, -1)[2].max())
+    nodes = np.concatenate([top, top[:1]])
+    vec = rng.uniform(-1, 1, size=(nodes.size, 3)) * scale
+    w0 = np.full(nodes.size, t0)
+    w1 = np.full(nodes.size, tf)
+    w1[::3] = 0.5 * (t0 + tf)  # some entries stop half-way: 0 after tf, as getTracAmount gives
+    return nodes.astype(np.int32), vec, w0, w1
+
+
+def loaded_pair(mesh, steps, forces, gravity, gravity_pos, precision=G.PREC_FP64, dirs=None, bc_mode=O.BC_VELOCITY,
+                t0=0.0, tf=1.0, law=0, rho=1000.0):
+    dd, dv = dirs if dirs is not None else clamp_bottom(mesh)
+    op = O.OracleProblem(mesh, law=law, rho=rho, dir_dof=dd, dir_val=dv, bc_mode=bc_mode, t0=t0, tf=tf, scale=0.5,
+                         n_steps=steps, sum_order=O.SUM_REFERENCE, gravity=gravity, forces=forces,
+                         gravity_pos=gravity_pos)
+    ro = op.run(hist_cap=max(steps, 1) if steps else 20000)
+    with G.Solver(mesh, law=law, rho=rho, dir_dof=dd, dir_val=dv, bc_mode=bc_mode, t0=t0, tf=tf, scale=0.5,
+                  n_steps=steps, precision=precision) as s:
+        s.set_loads(forces=forces, gravity=gravity, gravity_pos=gravity_pos)
+        rc, rep = s.run(steps if steps else 10 ** 9)
+        u, v, a = s.get_state()
+    return ro, rc, rep, u, v
+
+
+@pytest.mark.parametrize("tet", [False, True])
+def test_external_force_matches_oracle(tet):
+    m = perturb(box(4, 3, 5, tet=tet), 0.12, 21)
+    N = m["n_nodes"]
+    forces = top_forces(m, 0.1, 0.9)
+    for gpos in (0, 2, -1):
+        op = O.OracleProblem(m, rho=1100.0, n_steps=1, gravity=9.81, forces=forces, gravity_pos=gpos)
+        with G.Solver(m, rho=1100.0, n_steps=1) as s:
+            assert np.all(s.external_force(0.3) == 0.0)
+            s.set_loads(forces=forces, gravity=9.81, gravity_pos=gpos)
+            for t in (0.0, 0.1, 0.3, 0.5, 0.9, 0.95):
+                fd, fo = s.external_force(t), op.external_force(t)
+                if gpos == 0:
+                    assert np.array_equal(fd, fo), (gpos, t)
+                else:
+                    np.testing.assert_allclose(fd, fo, rtol=0, atol=1e-14 * np.abs(fo).max())
+                    # nodes without force entries: gravity alone, bitwise
+                    free = np.setdiff1d(np.arange(N), forces[0])
+                    assert np.array_equal(fd[2 * N + free], fo[2 * N + free])
+    # gravity only: -g times the lumped mass, bitwise the oracle's GP-order sum
+    op = O.OracleProblem(m, rho=1100.0, n_steps=1, gravity=-3.5)
+    with G.Solver(m, rho=1100.0, n_steps=1) as s:
+        s.set_loads(gravity=-3.5)
+        fd = s.external_force(0.0)
+        assert np.array_equal(fd, op.external_force(0.0))
+        np.testing.assert_allclose(fd[2 * N:], 3.5 * s.mass(), rtol=1e-12)
+
+
+@pytest.mark.parametrize("gpos", [0, -1])
+@pytest.mark.parametrize("tet", [False, True])
+def test_loaded_cantilever_fp64_fp32(tet, gpos):
+    m = perturb(box(3, 3, 6, lz=2.0, tet=tet), 0.1, 5)
+    forces = top_forces(m, 0.0, 0.4)
+    steps = 300
+    ro, rc, rep, u, _v = loaded_pair(m, steps, forces, 9.81, gpos)
+    assert ro["rc"] == 0 and rc == 0 and rep["steps"] == steps
+    err = relerr(u, ro["u"])
+    print(f"tet={tet} gpos={gpos} fp64 rel err {err:.3e}, time {rep['time']:.4f}")
+    assert err <= TOL64
+    _ro, rc32, _rep, u32, _v32 = loaded_pair(m, steps, forces, 9.81, gpos, precision=G.PREC_FP32_ELEM)
+    err32 = relerr(u32, ro["u"])
+    print(f"fp32-element rel err {err32:.3e}")
+    assert rc32 == 0 and err32 <= TOL32
+    # the loads matter: without them the clamped body stays at rest
+    assert np.abs(u).max() > 1e-6
+
+
+def test_time_window_run_with_ramp_and_forces():
+    """Reference time-window loop with DISPLACEMENT RAMP and force windows that end before tf."""
+    m = box(4, 4, 4)
+    N = m["n_nodes"]
+    dd, dv = clamp_bottom(m)
+    side = np.setdiff1d(face_nodes(m, 0, 1.0), face_nodes(m, 2, 0.0))
+    dd = np.concatenate([dd, (side + N * 0).astype(np.int32)])
+    dv = np.concatenate([dv, np.full(side.size, 0.001)])
+    forces = top_forces(m, 0.01, 0.03, scale=5e3, seed=3)
+    ro, rc, rep, u, _v = loaded_pair(m, 0, forces, 9.81, 1, dirs=(dd, dv), bc_mode=O.BC_RAMP, t0=0.0, tf=0.05)
+    assert ro["rc"] == 0 and rc == 0
+    assert rep["status"] == G.MPS_DONE and rep["steps"] == ro["steps"] and rep["time"] == ro["time"]
+    assert relerr(u, ro["u"]) <= TOL64
+
+
+def test_free_fall():
+    """Gravity on an unconstrained body: every node accelerates at -g, so u_z follows the scalar central-difference
+    recurrence with a_0 = 0 over the run's dt sequence, and x, y stay at rest."""
+    m = perturb(box(3, 3, 3), 0.1, 8)
+    N = m["n_nodes"]
+    steps, g = 50, 9.81
+    ro, rc, rep, u, v = loaded_pair(m, steps, None, g, -1, dirs=(np.zeros(0, np.int32), np.zeros(0)))
+    assert rc == 0 and relerr(u, ro["u"]) <= TOL64
+    uz, vz, az = 0.0, 0.0, 0.0
+    for dt in ro["hist"]["dt"][:steps]:
+        uz += dt * vz + 0.5 * dt * dt * az
+        vz += dt * 0.5 * (az - g)
+        az = -g
+    np.testing.assert_allclose(u[2 * N:], uz, rtol=1e-10)
+    np.testing.assert_allclose(v[2 * N:], vz, rtol=1e-10)
+    assert np.abs(u[:2 * N]).max() < 1e-10 * abs(uz)
+
+
+def test_set_loads_errors():
+    m = box(2, 2, 2)
+    N = m["n_nodes"]
+    with G.Solver(m, n_steps=5) as s:
+        for bad in (([N], [[1.0, 0, 0]], [0.0], [1.0]),      # node out of range
+                    ([0], [[1.0, 0, 0]], [0.5], [0.5]),      # empty window
+                    ([0], [[np.nan, 0, 0]], [0.0], [1.0])):  # non-finite force
+            with pytest.raises(G.MpsError) as ei:
+                s.set_loads(forces=bad)
+            assert ei.value.code == G.MPS_ERR_ARG
+        with pytest.raises(G.MpsError):
+            s.set_loads(gravity=np.inf)
+        s.set_loads(gravity=1.0)
+        s.set_loads()  # replaces: no loads
+        assert np.all(s.external_force(0.5) == 0.0)
+        rc, _rep = s.run(1)
+        assert rc == 0
+        with pytest.raises(G.MpsError) as ei:
+            s.set_loads(gravity=1.0)
+        assert ei.value.code == G.MPS_ERR_STATE

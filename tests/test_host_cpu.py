@@ -114,7 +114,10 @@ def test_reads_svk_and_linear_elastic(tmp_path):
     ("no_dirichlet", "at least one mechanical Dirichlet DOF"),
     ("implicit", "is not on this path"),
     ("stvk", "has no device descriptor"),
-    ("neumann", "Neumann loads"),
+    ("neumann", "TYPE=PRESSURE INST is not on this path"),
+    ("force_window", "tf > t0"),
+    ("force_entry", "node, Fx, Fy, Fz"),
+    ("two_gravity", "one gravity load"),
     ("unassigned_element", "has no material"),
     ("bad_direction", "direction"),
     ("cps3", "element type"),
@@ -139,6 +142,12 @@ def test_input_errors(tmp_path, case, needle):
         sol["bc"] = [(0, 3, 0.0)]
     elif case == "neumann":
         extra = "*BOUNDARY, TYPE=PRESSURE INST\n0,2,1e3,2\n4"
+    elif case == "force_window":
+        extra = "*BOUNDARY, TYPE=FORCE\n0.5, 0.5, 1\n1, 1, 0, 0"
+    elif case == "force_entry":
+        extra = "*BOUNDARY, TYPE=FORCE\n0, 1, 1\n1, 1, 0"
+    elif case == "two_gravity":
+        extra = "*GRAVITY\n9.81\n*GRAVITY\n1.0"
     p = tmp_path / f"{case}.inp"
     inpgen.write_inp(p, m, mats, [sol], extra=extra)
     text = p.read_text()
@@ -154,6 +163,29 @@ def test_input_errors(tmp_path, case, needle):
     rc, _, err = check(p)
     assert rc != 0
     assert needle in err, err
+
+
+def test_reads_force_and_gravity(tmp_path):
+    """NeumannCommonManagement reads the whole file in order, FORCE blocks inside *SOLVER too (the loads are
+    global); each entry is F * Ff over its block's window (forceBCManagement, inputFile.cpp:2841-2941), and
+    *GRAVITY takes its place among them."""
+    m, bc = cube(2)
+    p = tmp_path / "loads.inp"
+    inpgen.write_inp(p, m, [dict(rho=1000.0, E=1e6, nu=0.3)],
+                     [dict(scale=0.5, t0=0.0, tf=0.01, outputs=1, bc=bc,
+                           extra_lines=["*BOUNDARY, TYPE=FORCE", "0, 1, 1", "1, 0, 0, 1"])],
+                     extra="\n".join(["*BOUNDARY, TYPE=FORCE", "0.0, 0.5, 2.0", "3, 1.0, -2.0, 0.5", "5, 0, 0, 4",
+                                      "*GRAVITY", "9.81", "*BOUNDARY, TYPE=FORCE", "0.1, 0.3, 1.0", "3, 7, 0, 0"]))
+    rc, j, err = check(p)
+    assert rc == 0, err
+    ld = j["loads"]
+    assert ld["gravity"] == 9.81 and ld["gravity_pos"] == 3
+    assert ld["force"] == [[0, 0, 0, 1, 0, 1], [2, 2, -4, 1, 0, 0.5], [4, 0, 0, 8, 0, 0.5], [2, 7, 0, 0, 0.1, 0.3]]
+    assert j["solvers"][0]["dirichlet"] == len(bc)
+    # no loads: empty list and no gravity
+    inpgen.write_inp(p, m, [dict(rho=1000.0, E=1e6, nu=0.3)], [dict(scale=0.5, t0=0.0, tf=0.01, outputs=1, bc=bc)])
+    rc, j, err = check(p)
+    assert rc == 0 and j["loads"] == dict(gravity=None, gravity_pos=-1, force=[])
 
 
 def test_coordinates_round_trip(tmp_path):

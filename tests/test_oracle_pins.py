@@ -570,6 +570,30 @@ def test_nodal_force_window_and_ramp():
         np.testing.assert_allclose(f, exp, rtol=1e-15, atol=0)
 
 
+def test_load_order_gravity_position():
+    """NeumannBCManagement (boundaryConditions.cpp:75-140) adds the loads into one vector in input order,
+    gravity GP by GP onto what is there: gravity first leaves f_z = (gravity alone) + F exactly; gravity
+    after the force differs only by the rounding of the order."""
+    m = perturb(box(3, 2, 2), 0.1, 5)
+    N = m["n_nodes"]
+    forces = ([7, 11], np.array([[0.0, 1.0, 3.3], [2.0, 0.0, -1.7]]), [0.0, 0.0], [1.0, 2.0])
+    t = 0.7
+    grav = O.OracleProblem(m, rho=900.0, n_steps=1, gravity=9.81).external_force(t)
+    frc = O.OracleProblem(m, n_steps=1, forces=forces).external_force(t)
+    first = O.OracleProblem(m, rho=900.0, n_steps=1, gravity=9.81, forces=forces, gravity_pos=0).external_force(t)
+    last = O.OracleProblem(m, rho=900.0, n_steps=1, gravity=9.81, forces=forces, gravity_pos=-1).external_force(t)
+    middle = O.OracleProblem(m, rho=900.0, n_steps=1, gravity=9.81, forces=forces, gravity_pos=1).external_force(t)
+    assert np.array_equal(first, grav + frc)
+    np.testing.assert_allclose(last, grav + frc, rtol=1e-15, atol=0)
+    np.testing.assert_allclose(middle, grav + frc, rtol=1e-15, atol=0)
+    # with gravity after the force, node 7 accumulates F_z and then the GP contributions one by one
+    assert last[7 + 2 * N] == pytest.approx(grav[7 + 2 * N] + 3.3 * t, rel=1e-15)
+    loaded = {7, 11}
+    for n in range(N):
+        if n not in loaded:
+            assert last[n + 2 * N] == grav[n + 2 * N] == middle[n + 2 * N]
+
+
 def test_free_fall_and_momentum_under_nodal_forces():
     """A free body under gravity: every node accelerates at exactly -g, so u_z follows the scalar
     central-difference recurrence with a_0 = 0 (solverExplicit.cpp:143) -- -g dt^2 n(n-1)/2 at constant dt,

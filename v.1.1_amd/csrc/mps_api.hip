@@ -226,6 +226,10 @@ int prepare(const mps_mesh* mesh, const mps_material* mat, const mps_bcs* bcs, c
         hp->bcval[3 * (size_t)rec_of[n] + d] = val;
         hp->ninfo[n] |= (1u << d);
     }
+    if (nrec > (int32_t)(MPS_REC_MASK >> 3)) {
+        *err = fmt("%d Dirichlet nodes: at most %u", nrec, MPS_REC_MASK >> 3);
+        return MPS_ERR_ARG;
+    }
     for (int64_t n = 0; n < N; n++)
         if (rec_of[n] >= 0) hp->ninfo[n] |= ((uint32_t)rec_of[n] << 3);
     if (hp->bcval.empty()) hp->bcval.assign(3, 0.0);
@@ -257,6 +261,7 @@ struct mps_solver_s {
     int32_t* pick_idx = nullptr;  // mps_get_dofs buffers (grow-only, freed with the handle)
     double* pick_out = nullptr;
     int64_t pick_cap = 0;
+    std::vector<void*> load_allocs;  // mps_set_loads buffers (replaced by the next call)
     int energy_nb = 0;
     bool primed = false;       // prologue (first dt) done
     long long next_step = 1;   // host mirror of ctl->A
@@ -304,6 +309,8 @@ static void release(mps_solver_s* h) {
     if (h->stream) (void)hipStreamSynchronize(h->stream);
     for (void* p : h->allocs) (void)hipFree(p);
     h->allocs.clear();
+    for (void* p : h->load_allocs) (void)hipFree(p);
+    h->load_allocs.clear();
     if (h->pick_idx) (void)hipFree(h->pick_idx);
     if (h->pick_out) (void)hipFree(h->pick_out);
     if (h->eout) (void)hipFree(h->eout);
@@ -515,6 +522,100 @@ extern "C" int mps_set_state(mps_handle h, const double* u, const double* v, con
     return MPS_OK;
 }
 
+// Nodal force entries are sorted by node (stable, so each node keeps its entries in input order) and
+// flagged in ninfo; gravity is summed per node once, on the device.
+extern "C" int mps_set_loads(mps_handle h, const mps_loads* L) {
+    if (!h) return MPS_ERR_ARG;
+    if (h->primed || h->next_step != 1) return h->fail(MPS_ERR_STATE, "mps_set_loads: only valid before the first step");
+    const int64_t nf = L ? L->n_force : 0;
+    const double grav = L ? L->gravity : 0.0;
+    const int64_t N = h->I.N;
+    if (nf < 0) return h->fail(MPS_ERR_ARG, "mps_set_loads: n_force < 0");
+    if (!std::isfinite(grav)) return h->fail(MPS_ERR_ARG, "mps_set_loads: gravity is not finite");
+    if (nf > 0 && (!L->node || !L->force || !L->t0 || !L->tf))
+        return h->fail(MPS_ERR_ARG, "mps_set_loads: n_force > 0 needs node, force, t0 and tf");
+    for (int64_t k = 0; k < nf; k++) {
+        if (L->node[k] < 0 || L->node[k] >= N)
+            return h->fail(MPS_ERR_ARG, fmt("mps_set_loads: force entry %lld: node %d outside [0,%lld)", (long long)k,
+                                            L->node[k], (long long)N));
+        if (!std::isfinite(L->t0[k]) || !std::isfinite(L->tf[k]) || !(L->tf[k] > L->t0[k]))
+            return h->fail(MPS_ERR_ARG, fmt("mps_set_loads: force entry %lld: window needs finite tf > t0", (long long)k));
+        for (int d = 0; d < 3; d++)
+            if (!std::isfinite(L->force[3 * k + d]))
+                return h->fail(MPS_ERR_ARG, fmt("mps_set_loads: force entry %lld is not finite", (long long)k));
+    }
+    (void)hipSetDevice(h->device);
+    MpsInst& I = h->I;
+    // host copy of the node flags: clear the loaded bit, set it on the nodes with entries
+    std::vector<uint32_t> ninfo((size_t)N);
+    HCHK(h, hipMemcpy(ninfo.data(), I.ninfo, (size_t)N * 4, hipMemcpyDeviceToHost));
+    for (uint32_t& f : ninfo) f &= MPS_REC_MASK;
+    std::vector<int32_t> order((size_t)nf);
+    for (int64_t k = 0; k < nf; k++) order[(size_t)k] = (int32_t)k;
+    std::stable_sort(order.begin(), order.end(), [&](int32_t x, int32_t y) { return L->node[x] < L->node[y]; });
+    std::vector<int32_t> lnode((size_t)nf), lrec((size_t)nf);
+    std::vector<double> lm(3 * (size_t)nf), lwin(2 * (size_t)nf);
+    for (size_t j = 0; j < (size_t)nf; j++) {
+        const int32_t k = order[j];
+        lnode[j] = L->node[k];
+        lrec[j] = k;
+        for (int d = 0; d < 3; d++) lm[3 * j + d] = L->force[3 * (size_t)k + d] / (L->tf[k] - L->t0[k]);  // classNodes.cpp:93-95
+        lwin[2 * j] = L->t0[k];
+        lwin[2 * j + 1] = L->tf[k];
+        ninfo[(size_t)lnode[j]] |= MPS_NODE_LOADED;
+    }
+    HCHK(h, hipStreamSynchronize(h->stream));
+    for (void* p : h->load_allocs) (void)hipFree(p);
+    h->load_allocs.clear();
+    I.n_load = 0;
+    I.lnode = I.lrec = nullptr;
+    I.lm = I.lwin = I.fgrav = nullptr;
+    auto lalloc = [&](size_t bytes, void** p) -> hipError_t {
+        hipError_t e = hipMalloc(p, bytes < 256 ? 256 : bytes);
+        if (e == hipSuccess) h->load_allocs.push_back(*p);
+        return e;
+    };
+    if (nf > 0) {
+        void *pn, *pr, *pm, *pw;
+        HCHK(h, lalloc((size_t)nf * 4, &pn));
+        HCHK(h, lalloc((size_t)nf * 4, &pr));
+        HCHK(h, lalloc((size_t)nf * 24, &pm));
+        HCHK(h, lalloc((size_t)nf * 16, &pw));
+        HCHK(h, hipMemcpy(pn, lnode.data(), (size_t)nf * 4, hipMemcpyHostToDevice));
+        HCHK(h, hipMemcpy(pr, lrec.data(), (size_t)nf * 4, hipMemcpyHostToDevice));
+        HCHK(h, hipMemcpy(pm, lm.data(), (size_t)nf * 24, hipMemcpyHostToDevice));
+        HCHK(h, hipMemcpy(pw, lwin.data(), (size_t)nf * 16, hipMemcpyHostToDevice));
+        I.lnode = (const int32_t*)pn;
+        I.lrec = (const int32_t*)pr;
+        I.lm = (const double*)pm;
+        I.lwin = (const double*)pw;
+        I.n_load = (int32_t)nf;
+    }
+    const int32_t gp = L ? L->gravity_pos : -1;
+    I.grav_pos = (gp >= 0 && gp <= nf) ? gp : (int32_t)nf;
+    HCHK(h, hipMemcpy((void*)I.ninfo, ninfo.data(), (size_t)N * 4, hipMemcpyHostToDevice));
+    if (grav != 0.0) {
+        void* pg;
+        HCHK(h, lalloc((size_t)N * 8, &pg));
+        // the partial-force buffer (>= nen*Ne doubles) is free until the first step
+        hipError_t e = mpsk::launch_gravity(I, grav, I.part, (double*)pg, h->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+        if (e != hipSuccess) return h->fail(MPS_ERR_HIP, fmt("gravity load: %s", hipGetErrorString(e)));
+        I.fgrav = (const double*)pg;
+    }
+    return MPS_OK;
+}
+
+extern "C" int mps_external_force(mps_handle h, double t, double* fext) {
+    if (!h || !fext) return MPS_ERR_ARG;
+    (void)hipSetDevice(h->device);
+    hipError_t e = mpsk::launch_fext(h->I, t, h->scratch_f, h->stream);
+    if (e != hipSuccess) return h->fail(MPS_ERR_HIP, fmt("external force: %s", hipGetErrorString(e)));
+    HCHK(h, hipMemcpyAsync(fext, h->scratch_f, 3 * (size_t)h->I.N * 8, hipMemcpyDeviceToHost, h->stream));
+    HCHK(h, hipStreamSynchronize(h->stream));
+    return MPS_OK;
+}
+
 extern "C" int mps_get_state(mps_handle h, double* u, double* v, double* a) {
     if (!h) return MPS_ERR_ARG;
     (void)hipSetDevice(h->device);
@@ -541,7 +642,9 @@ extern "C" int mps_get_dofs(mps_handle h, int32_t which, int64_t n, const int32_
     (void)hipSetDevice(h->device);
     if (n > h->pick_cap) {
         HCHK(h, hipStreamSynchronize(h->stream));
-        if (h->pick_idx) (void)hipFree(h->pick_idx);
+        for (void* p : h->load_allocs) (void)hipFree(p);
+    h->load_allocs.clear();
+    if (h->pick_idx) (void)hipFree(h->pick_idx);
         if (h->pick_out) (void)hipFree(h->pick_out);
         h->pick_idx = nullptr;
         h->pick_out = nullptr;

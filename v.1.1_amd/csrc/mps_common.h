@@ -3,7 +3,8 @@
 // Device data layout (one solve = one "instance"):
 //   X, u, v, a      double[3N] SoA by direction (DOF = node + N*dir, as the reference)
 //   mass            double[N]    lumped mass, identical for the three directions
-//   ninfo           uint32[N]    bits 0..2 Dirichlet mask (x,y,z), bits 3.. record index
+//   ninfo           uint32[N]    bits 0..2 Dirichlet mask (x,y,z), bits 3..30 Dirichlet record index,
+//                                bit 31 node carries nodal force entries (MPS_NODE_LOADED)
 //   bcval           double[3*R]  per Dirichlet record: per-direction BC value
 //   conn            int32[nen*Ne] element-major (reference node order)
 //   l0              double[Ne]   undeformed characteristic length
@@ -19,6 +20,11 @@
 //     nrow/ncol     CSR node -> partials, ascending patch (= ascending element)
 //   part            double[3*n_part]       partial nodal forces [(g)*3 + dir]; also the
 //                                          set-up scratch (>= nen*Ne doubles)
+//   loads           n_load nodal force entries sorted by node (stable: input order per node)
+//     lnode, lrec   int32[n_load]  node, position in the input list of force entries
+//     lm            double[3*n_load] F/(tf-t0) (classNeumann, classNodes.cpp:85-96)
+//     lwin          double[2*n_load] t0, tf
+//     fgrav         double[N]      gravity load along z (NULL: no gravity)
 //   ctl             MpsCtl       step control (time, dt, go flags)
 //   dtred           u64[2*MPS_DT_SLOTS*MPS_DT_STRIDE]  stable-dt reduction slots by step parity
 #pragma once
@@ -28,6 +34,8 @@
 #define MPS_PATCH 256  // elements per patch (element-kernel work unit and partial-sum scope)
 #define MPS_DT_SLOTS 16   // stable-dt reduction slots per step parity: workgroup b min-reduces into slot b % 16
 #define MPS_DT_STRIDE 32  // u64 between two slots (256 B), so the slots' atomics do not share a line
+#define MPS_NODE_LOADED 0x80000000u  // ninfo bit: the node has nodal force entries
+#define MPS_REC_MASK 0x7FFFFFFFu     // ninfo bits below the loaded flag
 
 enum : int32_t { MPS_TIME_WINDOW = 0, MPS_TIME_FIXED = 1 };
 
@@ -88,7 +96,14 @@ struct MpsInst {
     int32_t dof1_node, dof1_dir;
     int32_t n_patch;
     int32_t n_part;
-    int32_t _pad2;
+    int32_t n_load;    // nodal force entries
+    int32_t grav_pos;  // gravity is added before the entries with lrec >= grav_pos
+    int32_t _pad3;
+    const int32_t* lnode;
+    const int32_t* lrec;
+    const double* lm;
+    const double* lwin;
+    const double* fgrav;
 };
 
 // One unit of work for the persistent batched kernels: a contiguous range of

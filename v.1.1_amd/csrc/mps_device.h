@@ -685,7 +685,7 @@ __device__ __forceinline__ void predict_node(const MpsInst& I, int n, long long 
 #pragma clang fp contract(off)
     uint32_t info = I.ninfo[n];
     uint32_t mask = info & 7u;
-    uint32_t rec = info >> 3;
+    uint32_t rec = (info & MPS_REC_MASK) >> 3;
 #pragma unroll
     for (int i = 0; i < 3; i++) {
         if (!((mask >> i) & 1u)) {
@@ -712,11 +712,46 @@ __device__ __forceinline__ void predict_node(const MpsInst& I, int n, long long 
     }
 }
 
-// Gather + lumped-mass solve of node n (nonlinearSystem.cpp:1096-1107; rhs = -fint + fext with fext = 0).
-// The node's internal force is the sum of its patch partials in ascending patch order.
-__device__ __forceinline__ void solve_node(const MpsInst& I, int n, double dt, double* v, double* a) {
+// External force of node n at time t (NeumannBCManagement, boundaryConditions.cpp:75-140): zero, then
+// the loads in input order -- the node's force entries (BCsNodalForces :1120-1134 with
+// classNeumann::getTracAmount, classNodes.h:101-108: F/(tf-t0) * t for t0 <= t <= tf, else 0) and its
+// gravity load (BCsGravity :1146-1172, constant, summed over the Gauss points at set-up) at the place
+// of *GRAVITY among them.
+__device__ __forceinline__ void node_fext(const MpsInst& I, int n, uint32_t info, double t, double* fe) {
 #pragma clang fp contract(off)
-    uint32_t mask = I.ninfo[n] & 7u;
+    fe[0] = 0.0;
+    fe[1] = 0.0;
+    fe[2] = 0.0;
+    const double g = I.fgrav ? I.fgrav[n] : 0.0;
+    if (!(info & MPS_NODE_LOADED)) {
+        fe[2] = g;
+        return;
+    }
+    int lo = 0, hi = I.n_load;  // first entry of node n (entries sorted by node)
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (I.lnode[mid] < n) lo = mid + 1;
+        else hi = mid;
+    }
+    bool gdone = (I.fgrav == nullptr);
+    for (int j = lo; j < I.n_load && I.lnode[j] == n; j++) {
+        if (!gdone && I.lrec[j] >= I.grav_pos) {
+            fe[2] += g;
+            gdone = true;
+        }
+        const bool in = t >= I.lwin[2 * (size_t)j] && t <= I.lwin[2 * (size_t)j + 1];
+#pragma unroll
+        for (int d = 0; d < 3; d++) fe[d] += in ? I.lm[3 * (size_t)j + d] * t : 0.0;
+    }
+    if (!gdone) fe[2] += g;
+}
+
+// Gather + lumped-mass solve of node n at step time t (nonlinearSystem.cpp:1096-1107; rhs = -fint + fext).
+// The node's internal force is the sum of its patch partials in ascending patch order.
+__device__ __forceinline__ void solve_node(const MpsInst& I, int n, double t, double dt, double* v, double* a) {
+#pragma clang fp contract(off)
+    const uint32_t info = I.ninfo[n];
+    const uint32_t mask = info & 7u;
     if (mask == 7u) return;
     const double* __restrict__ part = I.part;
     int q = I.nrow[n];
@@ -737,12 +772,14 @@ __device__ __forceinline__ void solve_node(const MpsInst& I, int n, double dt, d
         f1 += c[1];
         f2 += c[2];
     }
+    double fe[3];
+    node_fext(I, n, info, t, fe);
     double m = I.mass[n];
     double fi[3] = {f0, f1, f2};
 #pragma unroll
     for (int i = 0; i < 3; i++) {
         if ((mask >> i) & 1u) continue;
-        double rhs = -fi[i] + 0.0;
+        double rhs = -fi[i] + fe[i];
         double an = (m != 0.0) ? rhs / m : 0.0;
         v[i] += dt * ((1. - 0.5) * a[i] + 0.5 * an);
         a[i] = an;

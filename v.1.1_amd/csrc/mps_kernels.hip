@@ -159,7 +159,7 @@ __device__ __forceinline__ void node_item(const MpsInst& I, int n, bool first, i
         v[i] = I.v[i * N + n];
         a[i] = I.a[i * N + n];
     }
-    solve_node(I, n, s.dt, v, a);
+    solve_node(I, n, s.time, s.dt, v, a);
     if (n == I.dof1_node) {
         double x = u[I.dof1_dir];
         if (!isfinite(x)) {
@@ -320,51 +320,70 @@ __device__ __forceinline__ void hex_shape_ref(double xi, double eta, double zeta
     nd[23] = 0.125 * (1 - x) * (1 + y);
 }
 
+// Shape values, natural derivatives and weight of Gauss point g of the set-up quadrature: 2x2x2 for
+// hex (GPsDistribution.cpp:392-432), one point for tet (:339-343).
+template <int NEN>
+__device__ __forceinline__ void gp_shape(int g, double* phi, double* nd, double* weight) {
+#pragma clang fp contract(off)
+    if (NEN == 8) {
+        const double gx = sxi(g) > 0 ? MPS_GP : -MPS_GP;
+        const double gy = seta(g) > 0 ? MPS_GP : -MPS_GP;
+        const double gz = szeta(g) > 0 ? MPS_GP : -MPS_GP;
+        hex_shape_ref(gx, gy, gz, phi, nd);
+        *weight = 1 * 1.0;
+    } else {
+        const double q = 0.250000000000, r = 0.25;
+        phi[0] = -1.0 * q - 1.0 * r - 1.0 * r + 1.0;
+        phi[1] = 1.0 * q;
+        phi[2] = 1.0 * r;
+        phi[3] = 1.0 * r;
+        for (int k = 0; k < 12; k++) nd[k] = 0.0;
+        nd[0] = -1.0 - 0.0 - 0.0 + 0.0;
+        nd[3] = 1.0;
+        nd[1] = -0.0 - 1.0 - 0.0 + 0.0;
+        nd[7] = 1.0;
+        nd[2] = -0.0 - 0.0 - 1.0 + 0.0;
+        nd[11] = 1.0;
+        *weight = MPS_TET_W;
+    }
+}
+
+// det of the Jacobian xyzCoor * natDev, k innermost (maths.cpp:1679-1693).
+template <int NEN>
+__device__ __forceinline__ double gp_det(const double (*x)[3], const double* nd) {
+#pragma clang fp contract(off)
+    double Jac[9];
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) {
+            double s = 0.0;
+            for (int k = 0; k < NEN; k++) s += x[k][i] * nd[k * 3 + j];
+            Jac[i * 3 + j] = s;
+        }
+    return det3_strict(Jac);
+}
+
+template <int NEN>
+__device__ __forceinline__ void load_ref_coords(const MpsInst& I, int e, double (*x)[3]) {
+    const size_t N = (size_t)I.N;
+    for (int a = 0; a < NEN; a++) {
+        const int nd = I.conn[(size_t)e * NEN + a];
+        for (int d = 0; d < 3; d++) x[a][d] = I.X[d * N + nd];
+    }
+}
+
 template <int NEN>
 __device__ __forceinline__ void setup_element(const MpsInst& I, int e, double rho, double* ms, double* l0) {
 #pragma clang fp contract(off)
-    const size_t N = (size_t)I.N;
     double x[NEN][3];
-    for (int a = 0; a < NEN; a++) {
-        int nd = I.conn[(size_t)e * NEN + a];
-        for (int d = 0; d < 3; d++) x[a][d] = I.X[d * N + nd];
-    }
+    load_ref_coords<NEN>(I, e, x);
     *l0 = (NEN == 8) ? hex_char_length((const double(*)[3])x) : tet_char_length((const double(*)[3])x);
     for (int a = 0; a < NEN; a++) ms[a] = 0.0;
     const int ngp = (NEN == 8) ? 8 : 1;
     for (int g = 0; g < ngp; g++) {
         double phi[8], nd[24];
         double weight;
-        if (NEN == 8) {
-            const double gx = sxi(g) > 0 ? MPS_GP : -MPS_GP;
-            const double gy = seta(g) > 0 ? MPS_GP : -MPS_GP;
-            const double gz = szeta(g) > 0 ? MPS_GP : -MPS_GP;
-            hex_shape_ref(gx, gy, gz, phi, nd);
-            weight = 1 * 1.0;
-        } else {
-            const double q = 0.250000000000, r = 0.25;
-            phi[0] = -1.0 * q - 1.0 * r - 1.0 * r + 1.0;
-            phi[1] = 1.0 * q;
-            phi[2] = 1.0 * r;
-            phi[3] = 1.0 * r;
-            for (int k = 0; k < 12; k++) nd[k] = 0.0;
-            nd[0] = -1.0 - 0.0 - 0.0 + 0.0;
-            nd[3] = 1.0;
-            nd[1] = -0.0 - 1.0 - 0.0 + 0.0;
-            nd[7] = 1.0;
-            nd[2] = -0.0 - 0.0 - 1.0 + 0.0;
-            nd[11] = 1.0;
-            weight = MPS_TET_W;
-        }
-        // Jacobian = xyzCoor * natDev, k innermost (maths.cpp:1679-1693)
-        double Jac[9];
-        for (int i = 0; i < 3; i++)
-            for (int j = 0; j < 3; j++) {
-                double s = 0.0;
-                for (int k = 0; k < NEN; k++) s += x[k][i] * nd[k * 3 + j];
-                Jac[i * 3 + j] = s;
-            }
-        double wJ = weight * fabs(det3_strict(Jac));
+        gp_shape<NEN>(g, phi, nd, &weight);
+        double wJ = weight * fabs(gp_det<NEN>((const double(*)[3])x, nd));
         for (int a = 0; a < NEN; a++) {
             double s = 0.0;
             for (int b = 0; b < NEN; b++) s += wJ * rho * phi[a] * phi[b];
@@ -424,6 +443,57 @@ __global__ __launch_bounds__(256) void k_gather_scalar(MpsInst I, const double* 
     double acc = 0.0;
     for (int q = I.rowptr[n]; q < I.rowptr[n + 1]; q++) acc += slotv[I.col[q]];
     out[n] = acc;
+}
+
+// Gravity load (BCsGravity, boundaryConditions.cpp:1146-1172), constant over the run: per Gauss point the
+// factor -w * rho * |J| * g (tslot[e*ngp + g]), then per node the sum of factor * phi_a over its
+// elements in ascending order and their Gauss points in order, as the reference adds them.
+template <int NEN>
+__global__ __launch_bounds__(256) void k_grav_elem(MpsInst I, double grav, double* __restrict__ tslot) {
+#pragma clang fp contract(off)
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= I.Ne) return;
+    const double rho = I.rho ? I.rho[e] : I.rho0;
+    double x[NEN][3];
+    load_ref_coords<NEN>(I, e, x);
+    const int ngp = (NEN == 8) ? 8 : 1;
+    for (int g = 0; g < ngp; g++) {
+        double phi[8], nd[24], w;
+        gp_shape<NEN>(g, phi, nd, &w);
+        const double J = fabs(gp_det<NEN>((const double(*)[3])x, nd));
+        tslot[(size_t)e * ngp + g] = -w * rho * J * grav;
+    }
+}
+
+template <int NEN>
+__global__ __launch_bounds__(256) void k_grav_node(MpsInst I, const double* __restrict__ tslot, double* fgrav) {
+#pragma clang fp contract(off)
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= I.N) return;
+    const int ngp = (NEN == 8) ? 8 : 1;
+    double acc = 0.0;
+    for (int q = I.rowptr[n]; q < I.rowptr[n + 1]; q++) {
+        const int slot = I.col[q];
+        const int e = slot / NEN, a = slot - e * NEN;
+        for (int g = 0; g < ngp; g++) {
+            double phi[8], nd[24], w;
+            gp_shape<NEN>(g, phi, nd, &w);
+            acc += tslot[(size_t)e * ngp + g] * phi[a];
+        }
+    }
+    fgrav[n] = acc;
+}
+
+// External force of every DOF at time t (NeumannBCManagement, boundaryConditions.cpp:75-140).
+__global__ __launch_bounds__(256) void k_fext(MpsInst I, double t, double* fext) {
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= I.N) return;
+    double fe[3];
+    node_fext(I, n, I.ninfo[n], t, fe);
+    const size_t N = (size_t)I.N;
+    fext[n] = fe[0];
+    fext[N + n] = fe[1];
+    fext[2 * N + n] = fe[2];
 }
 
 // Internal force for all DOFs (solverExplicit.cpp:363-379), from the patch partials.
@@ -713,6 +783,22 @@ hipError_t launch_setup(const MpsInst& I, double* mslot, double* l0out, unsigned
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     k_gather_scalar<<<blocks_for(I.N, 256), 256, 0, st>>>(I, mslot, I.mass);
+    return hipGetLastError();
+}
+
+hipError_t launch_gravity(const MpsInst& I, double grav, double* tslot, double* fgrav, hipStream_t st) {
+    if (I.nen == 8) {
+        k_grav_elem<8><<<blocks_for(I.Ne, 256), 256, 0, st>>>(I, grav, tslot);
+        k_grav_node<8><<<blocks_for(I.N, 256), 256, 0, st>>>(I, tslot, fgrav);
+    } else {
+        k_grav_elem<4><<<blocks_for(I.Ne, 256), 256, 0, st>>>(I, grav, tslot);
+        k_grav_node<4><<<blocks_for(I.N, 256), 256, 0, st>>>(I, tslot, fgrav);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_fext(const MpsInst& I, double t, double* fext, hipStream_t st) {
+    k_fext<<<blocks_for(I.N, 256), 256, 0, st>>>(I, t, fext);
     return hipGetLastError();
 }
 

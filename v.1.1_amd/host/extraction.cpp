@@ -114,6 +114,18 @@ bool classExtractData::needsInternalForce() const {
     return false;
 }
 
+bool classExtractData::needsExternalForce() const {
+    for (auto &kv : _node)
+        if (kv.second->getDataType() == "ExternalForce") return true;
+    return false;
+}
+
+bool classExtractData::needsExternalEnergy() const {
+    for (auto &kv : _element)
+        if (kv.second->getDataType() == "EXTERNAL_ENERGY") return true;
+    return false;
+}
+
 void classExtractData::finalise() {
     for (auto &kv : _node) kv.second->closeFile();
     for (auto &kv : _element) kv.second->closeFile();

@@ -74,6 +74,8 @@ class classExtractData {
     bool empty() const { return _node.empty() && _element.empty(); }
     bool needsNodal() const { return !_node.empty(); }
     bool needsInternalForce() const;
+    bool needsExternalForce() const;   // a node extraction of ExternalForce
+    bool needsExternalEnergy() const;  // an element extraction of EXTERNAL_ENERGY
     bool needsEnergies() const { return !_element.empty(); }
     std::map<std::string, std::unique_ptr<classNodeData>> &nodeData() { return _node; }
     std::map<std::string, std::unique_ptr<classElementData>> &elementData() { return _element; }

@@ -315,6 +315,10 @@ bool readSolvers(Reader &R, InpModel &m) {
                 if (!timeSeen)
                     return R.fail(i, "*Time must precede *BOUNDARY inside *SOLVER (the ramp duration is taken at parse time)");
                 if (!readDirichlet(R, i, m, s)) return false;
+            } else if (starts(l, "*BOUNDARY") && tail(l, 16) == "FORCE") {
+                // a Neumann block: read with the whole file by readLoads, for every solver
+                for (i++; i + 1 < R.L.size() && !starts(R.L[i + 1], "*"); i++) {
+                }
             } else if (starts(l, "*BOUNDARY, TYPE=VELOCITY")) {
                 velocity = true;
                 if (!readDirichlet(R, i, m, s)) return false;
@@ -445,7 +449,58 @@ bool readExtractions(Reader &R, InpModel &m) {
     return true;
 }
 
-// Keywords outside *SOLVER that would change the physics of the path.
+// NeumannCommonManagement (inputFile.cpp:4703-4760): the whole file, in order.  FORCE blocks
+// (forceBCManagement, :2841-2941): a "t0, tf, Ff" line, then "node, Fx, Fy, Fz" lines up to the next
+// keyword, each entry F * Ff over the window [t0, tf].  *GRAVITY: the value g on the next line.
+// The other Neumann types and contact are not on this path and are rejected.
+bool readLoads(Reader &R, InpModel &m) {
+    for (size_t i = 0; i < R.L.size(); i++) {
+        const std::string &l = R.L[i];
+        if (starts(l, "*BOUNDARY")) {
+            const std::string type = tail(l, 16);
+            if (type == "FORCE") {
+                if (i + 1 >= R.L.size()) return R.fail(i, "FORCE needs a \"t0, tf, Ff\" line");
+                std::vector<std::string> f = fields(R.L[i + 1]);
+                double t0, tf, Ff;
+                if (f.size() != 3 || !toDouble(f[0], t0) || !toDouble(f[1], tf) || !toDouble(f[2], Ff))
+                    return R.fail(i + 1, "FORCE needs a \"t0, tf, Ff\" line");
+                if (!(tf > t0)) return R.fail(i + 1, "FORCE window needs tf > t0 (the load grows as F/(tf-t0) * t)");
+                for (i += 2; i < R.L.size() && !starts(R.L[i], "*"); i++) {
+                    if (isBlank(R.L[i])) continue;
+                    std::vector<std::string> g = fields(R.L[i]);
+                    long long id;
+                    double F[3];
+                    if (g.size() != 4 || !toInt(g[0], id) || !toDouble(g[1], F[0]) || !toDouble(g[2], F[1]) ||
+                        !toDouble(g[3], F[2]))
+                        return R.fail(i, "FORCE entry needs \"node, Fx, Fy, Fz\"");
+                    if (id < 1 || id > m.nNodes) return R.fail(i, "node id outside 1..N");
+                    m.loads.node.push_back((int32_t)(id - 1));
+                    for (int d = 0; d < 3; d++) m.loads.force.push_back(F[d] * Ff);
+                    m.loads.t0.push_back(t0);
+                    m.loads.tf.push_back(tf);
+                }
+                i--;
+            } else if (type == "TRACTION RAMP" || type == "TRACTION INST" || type == "PRESSURE RAMP" ||
+                       type == "PRESSURE INST") {
+                return R.fail(i, "*BOUNDARY, TYPE=" + type + " is not on this path (FORCE and *GRAVITY are)");
+            }
+        } else if (starts(l, "*GRAVITY")) {
+            double g;
+            if (i + 1 >= R.L.size() || !toDouble(R.L[i + 1], g)) return R.fail(i, "*GRAVITY needs the value g");
+            if (m.loads.hasGravity) return R.fail(i, "a second *GRAVITY: one gravity load per model");
+            m.loads.gravity = g;
+            m.loads.hasGravity = true;
+            m.loads.gravityPos = (int)m.loads.node.size();
+            i++;
+        } else if (starts(l, "*HERTZIAN CONTACT") || starts(l, "*STOCHASTIC HERTZIAN CONTACT") ||
+                   starts(l, "*FLUX EXTRADOF")) {
+            return R.fail(i, "contact and flux loads are not on this path");
+        }
+    }
+    return true;
+}
+
+// Keywords outside *SOLVER that the path does not read: Dirichlet conditions belong in a *SOLVER block.
 bool rejectUnsupported(Reader &R) {
     bool inSolver = false;
     for (size_t i = 0; i < R.L.size(); i++) {
@@ -453,8 +508,8 @@ bool rejectUnsupported(Reader &R) {
         if (starts(l, "*SOLVER")) inSolver = true;
         if (starts(l, "*END SOLVER")) inSolver = false;
         if (inSolver) continue;
-        if (starts(l, "*BOUNDARY") || starts(l, "*GRAVITY") || starts(l, "*Gravity"))
-            return R.fail(i, "Neumann loads and body forces are not on this path");
+        if (starts(l, "*BOUNDARY") && tail(l, 16) != "FORCE")
+            return R.fail(i, "Dirichlet conditions outside *SOLVER are not on this path");
     }
     return true;
 }
@@ -472,7 +527,7 @@ bool readInp(const std::string &path, InpModel &m, std::string &err) {
     while (std::getline(ifs, line)) R.L.push_back(rtrim(line));
     m = InpModel();
     bool ok = readNodes(R, m) && readElements(R, m) && readFemSets(R) && readMaterials(R, m) && readSolvers(R, m) &&
-              readExtractions(R, m) && rejectUnsupported(R);
+              readExtractions(R, m) && readLoads(R, m) && rejectUnsupported(R);
     if (!ok) err = R.err;
     return ok;
 }

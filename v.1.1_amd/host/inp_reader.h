@@ -9,6 +9,8 @@
 //   *SOLVER, EXPLICIT ... *END SOLVER                  solverManagement inputFile.cpp:2253-2762
 //   *BOUNDARY, TYPE=DISPLACEMENT RAMP|INST             dispBCManagement inputFile.cpp:2960-3217
 //   *Outputs                    outManagement          inputFile.cpp:4769-4829
+//   *BOUNDARY, TYPE=FORCE       forceBCManagement      inputFile.cpp:2841-2941  (anywhere in the file:
+//   *GRAVITY                    NeumannCommonManagement inputFile.cpp:4703-4735   global for all solvers)
 //   *EXTRACTION, NODE|ELEMENT   dataExtractionManagement inputFile.cpp:2229-2243, 4950-5120
 // Keywords are matched as prefixes, case-sensitively, as the reference does.
 // Where the reference would read garbage, dereference null or silently
@@ -71,6 +73,18 @@ struct InpElementExtraction {
     std::vector<int32_t> elems; // 0-based
 };
 
+// Neumann loads of the path in input order (NeumannCommonManagement reads the whole file; the loads
+// apply to every solver, solverExplicit.cpp:47).
+struct InpLoads {
+    std::vector<int32_t> node;   // force entries, 0-based node
+    std::vector<double> force;   // 3 per entry: F * Ff
+    std::vector<double> t0, tf;  // window of the block the entry belongs to
+    double gravity = 0;          // *GRAVITY g
+    bool hasGravity = false;
+    int gravityPos = -1;         // force entries read before *GRAVITY
+    bool empty() const { return node.empty() && !hasGravity; }
+};
+
 struct InpModel {
     int ndim = 0;
     int32_t nNodes = 0, nElem = 0, nen = 0;
@@ -81,6 +95,7 @@ struct InpModel {
     std::vector<InpSolver> solvers;
     std::vector<InpNodeExtraction> nodeExtractions;
     std::vector<InpElementExtraction> elementExtractions;
+    InpLoads loads;
     std::vector<std::string> warnings;
 };
 

@@ -84,7 +84,14 @@ static int check(const std::string &path) {
         std::printf("%s\"%s\"", first ? "" : ", ", kv.first.c_str());
         first = false;
     }
-    std::printf("], \"warnings\": %zu}\n", m.warnings.size());
+    const mps::InpLoads &ld = m.loads;
+    char gbuf[40] = "null";
+    if (ld.hasGravity) std::snprintf(gbuf, sizeof gbuf, "%.17g", ld.gravity);
+    std::printf("], \"loads\": {\"gravity\": %s, \"gravity_pos\": %d, \"force\": [", gbuf, ld.gravityPos);
+    for (size_t k = 0; k < ld.node.size(); k++)
+        std::printf("%s[%d, %.17g, %.17g, %.17g, %.17g, %.17g]", k ? ", " : "", ld.node[k], ld.force[3 * k],
+                    ld.force[3 * k + 1], ld.force[3 * k + 2], ld.t0[k], ld.tf[k]);
+    std::printf("]}, \"warnings\": %zu}\n", m.warnings.size());
     return 0;
 }
 

@@ -118,6 +118,13 @@ int classSolverExplicit::calculate(const InpModel &m, const std::vector<std::uni
                           _a0.empty() ? nullptr : _a0.data()) != MPS_OK)
             return fail("mps_set_state");
     }
+    // Neumann loads, global for all solvers (solverExplicit.cpp:47; NeumannBCManagement at every step)
+    const InpLoads &ld = m.loads;
+    if (!ld.empty()) {
+        mps_loads L{(int32_t)ld.node.size(), ld.hasGravity ? ld.gravityPos : -1, ld.hasGravity ? ld.gravity : 0.0,
+                    ld.node.data(), ld.force.data(), ld.t0.data(), ld.tf.data()};
+        if (mps_set_loads(h, &L) != MPS_OK) return fail("mps_set_loads");
+    }
     double dt0 = 0;
     if (mps_critical_dt(h, nullptr, &dt0) != MPS_OK) return fail("mps_critical_dt");
 
@@ -136,14 +143,16 @@ int classSolverExplicit::calculate(const InpModel &m, const std::vector<std::uni
             for (int32_t n : kv.second->getNodes()) pickDof.push_back(n + N * kv.second->getComp());
     const bool nodal = extract.needsNodal();
     const bool needF = extract.needsInternalForce();
-    std::vector<double> picked(pickDof.size()), uFull, fint;
-    if (needF) {
-        uFull.resize(3 * (size_t)N);
-        fint.resize(3 * (size_t)N);
-    }
+    // external force and EXTERNAL_ENERGY = sum_i u_i fext_i (dataExtraction.cpp:866-874): zero without loads
+    const bool needFext = !ld.empty() && extract.needsExternalForce();
+    const bool extEnergy = !ld.empty() && extract.needsExternalEnergy();
+    std::vector<double> picked(pickDof.size()), uFull, fint, fext;
+    if (needF || needFext || extEnergy) uFull.resize(3 * (size_t)N);
+    if (needF) fint.resize(3 * (size_t)N);
+    if (needFext || extEnergy) fext.resize(3 * (size_t)N);
     std::vector<double> ht(kHistoryCap), hdef(kHistoryCap), hkin(kHistoryCap), hvol(kHistoryCap);
 
-    const long long chunk = nodal ? 1 : (extract.needsEnergies() ? kHistoryCap : 1LL << 40);
+    const long long chunk = (nodal || extEnergy) ? 1 : (extract.needsEnergies() ? kHistoryCap : 1LL << 40);
     // VTK series (outputsManagement, solverExplicit.cpp:173-186, 310-322): at t0 when t0 < 1e-12, then
     // whenever `sampler` steps have passed since the last output, and at the last step.
     VtkSeriesWriter *vtk = opt.vtk;
@@ -174,13 +183,22 @@ int classSolverExplicit::calculate(const InpModel &m, const std::vector<std::uni
         }
         if (rc != MPS_OK) return fail("mps_run");
         const long long done = rep.steps - before;
+        double eext = 0.0;
+        if (done > 0 && (needFext || extEnergy)) {
+            if (mps_get_state(h, uFull.data(), nullptr, nullptr) != MPS_OK) return fail("mps_get_state");
+            if (mps_external_force(h, rep.time, fext.data()) != MPS_OK) return fail("mps_external_force");
+            for (size_t i = 0; i < fext.size(); i++) eext += uFull[i] * fext[i];
+        }
         if (done > 0 && extract.needsEnergies()) {
             if (mps_get_history(h, before, done, ht.data(), nullptr, hdef.data(), hkin.data(), hvol.data()) != MPS_OK)
                 return fail("mps_get_history");
             for (long long k = 0; k < done; k++)
                 for (auto &kv : extract.elementData()) {
                     const std::string &t = kv.second->getDataType();
-                    const double v = t == "DEFO_ENERGY" ? hdef[k] : t == "KIN_ENERGY" ? hkin[k] : t == "VOLUME" ? hvol[k] : 0.0;
+                    const double v = t == "DEFO_ENERGY"  ? hdef[k]
+                                     : t == "KIN_ENERGY" ? hkin[k]
+                                     : t == "VOLUME"     ? hvol[k]
+                                                         : eext;  // EXTERNAL_ENERGY (chunks of one step with loads)
                     if (!kv.second->writeData(opt.outputDir, ht[k], v)) {
                         err = "cannot write " + kv.first + ".csv in " + opt.outputDir;
                         return -1;
@@ -205,6 +223,8 @@ int classSolverExplicit::calculate(const InpModel &m, const std::vector<std::uni
                         vals[j] = picked[off + j];
                     else if (d.getDataType() == "InternalForce")
                         vals[j] = fint[g];
+                    else if (d.getDataType() == "ExternalForce" && needFext)
+                        vals[j] = fext[g];
                 }
                 if (d.getDataType() == "Unknown") off += vals.size();
                 if (!d.writeData(opt.outputDir, rep.time, vals)) {

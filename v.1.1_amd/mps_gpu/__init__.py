@@ -44,6 +44,11 @@ class SolverCfg(C.Structure):
                 ("device", C.c_int32)]
 
 
+class Loads(C.Structure):
+    _fields_ = [("n_force", C.c_int32), ("gravity_pos", C.c_int32), ("gravity", C.c_double), ("node", _ip),
+                ("force", _dp), ("t0", _dp), ("tf", _dp)]
+
+
 class Report(C.Structure):
     _fields_ = [("steps", C.c_int64), ("time", C.c_double), ("dt", C.c_double), ("status", C.c_int32),
                 ("nonfinite", C.c_int32), ("device_ms", C.c_double)]
@@ -97,6 +102,8 @@ def load(path=None):
     L.mps_internal_force.argtypes = [H, _dp, _dp]
     L.mps_critical_dt.argtypes = [H, _dp, _dp]
     L.mps_element_outputs.argtypes = [H, _dp]
+    L.mps_set_loads.argtypes = [H, C.POINTER(Loads)]
+    L.mps_external_force.argtypes = [H, C.c_double, _dp]
     L.mps_get_dofs.argtypes = [H, C.c_int32, C.c_int64, _ip, _dp]
     L.mps_destroy.argtypes = [H]
     L.mps_last_error.restype = C.c_char_p
@@ -218,6 +225,27 @@ class Solver:
     def set_state(self, u=None, v=None, a=None):
         arrs = [None if x is None else np.ascontiguousarray(x, dtype=np.float64) for x in (u, v, a)]
         self._chk(self.L.mps_set_state(self.h, *[None if x is None else _d(x) for x in arrs]))
+
+    def set_loads(self, forces=None, gravity=0.0, gravity_pos=-1):
+        """forces: None or (node[k], force[k,3], t0[k], tf[k]) in input order (*BOUNDARY, TYPE=FORCE entries,
+        force = F * Ff); gravity: g of *GRAVITY; gravity_pos: the force entry gravity precedes (-1: after all)."""
+        if forces is None:
+            node, vec, t0, tf = np.zeros(0, np.int32), np.zeros((0, 3)), np.zeros(0), np.zeros(0)
+        else:
+            node, vec, t0, tf = forces
+        self._loads = [np.ascontiguousarray(node, dtype=np.int32),
+                       np.ascontiguousarray(np.reshape(vec, (-1, 3)), dtype=np.float64),
+                       np.ascontiguousarray(t0, dtype=np.float64), np.ascontiguousarray(tf, dtype=np.float64)]
+        nf = self._loads[0].size
+        ld = Loads(n_force=nf, gravity_pos=gravity_pos, gravity=gravity,
+                   node=_i(self._loads[0]) if nf else None, force=_d(self._loads[1]) if nf else None,
+                   t0=_d(self._loads[2]) if nf else None, tf=_d(self._loads[3]) if nf else None)
+        self._chk(self.L.mps_set_loads(self.h, C.byref(ld)))
+
+    def external_force(self, t):
+        f = np.zeros(3 * self.n)
+        self._chk(self.L.mps_external_force(self.h, t, _d(f)))
+        return f
 
     def get_state(self):
         u, v, a = (np.zeros(3 * self.n) for _ in range(3))

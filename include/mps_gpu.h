@@ -84,17 +84,30 @@ typedef struct mps_bcs {
     const double *value; /* [n_dirichlet] U (ramp/inst) or V (velocity) */
 } mps_bcs;
 
-/* Neumann loads of the path: nodal forces and gravity, added into one external-force vector in input
- * order each step (NeumannBCManagement, boundaryConditions.cpp:75-140). */
-typedef struct mps_loads {
-    int32_t n_force;     /* *BOUNDARY, TYPE=FORCE entries (one per node line), in input order */
-    int32_t gravity_pos; /* gravity is added before force entry gravity_pos (outside 0..n_force: after all) */
-    double gravity;      /* *GRAVITY g (0: none): f_z += -w rho |J| g phi_a per Gauss point (:1146-1172) */
-    const int32_t *node; /* [n_force] 0-based node */
-    const double *force; /* [3*n_force] F * Ff, the force reached at tf (inputFile.cpp:2905-2914) */
-    const double *t0;    /* [n_force] window: F/(tf-t0) * t for t0 <= t <= tf, else 0 (classNodes.h:101-108); */
-    const double *tf;    /* [n_force]         tf > t0 required */
-} mps_loads;
+/* ---- Neumann loads: one block per entry of the reference's NeumannBCs list, in input order ---- */
+#define MPS_LOAD_FORCE 0         /* *BOUNDARY, TYPE=FORCE (classNodalForces; forceBCManagement, inputFile.cpp:2841-2941) */
+#define MPS_LOAD_GRAVITY 1       /* *GRAVITY (classGravity; inputFile.cpp:4723-4735) */
+#define MPS_LOAD_PRESSURE_RAMP 2 /* *BOUNDARY, TYPE=PRESSURE RAMP (classPressureRamp; inputFile.cpp:3580-3684) */
+#define MPS_LOAD_PRESSURE_INST 3 /* *BOUNDARY, TYPE=PRESSURE INST (classPressureInst; inputFile.cpp:3686-3790) */
+
+/* The external force of every step is zero plus the blocks in order (NeumannBCManagement,
+ * boundaryConditions.cpp:75-140):
+ *   FORCE:    entry k adds force[3k..]/(tf-t0) * t at node index[k] for t0 <= t <= tf, else 0
+ *             (BCsNodalForces :1120-1134; classNodes.h:101-108); force = F * Ff as read.
+ *   GRAVITY:  f_z += -w rho |J| g phi_a per bulk Gauss point, g = value (BCsGravity :1146-1172).
+ *   PRESSURE: face face[k] (getSubElement numbering: hex 1..6, tet 1..4) of element index[k] carries
+ *             P(t) w |J_s| phi_a det(F) F^-T n0 per surface Gauss point, F of the bulk element at the point
+ *             (BCsPressureRamp :370-452, BCsPressureInst :454-496); P(t) = value/(tf-t0) * (t-t0) (ramp) or
+ *             value (inst) for t0 <= t <= tf, else 0 (classNeumannBCs.h:144-176). */
+typedef struct mps_load {
+    int32_t type;         /* MPS_LOAD_* */
+    int32_t n;            /* FORCE: node entries; PRESSURE: faces; GRAVITY: 0 */
+    double t0, tf;        /* FORCE, PRESSURE: window (tf > t0; PRESSURE INST: tf >= t0) */
+    double value;         /* GRAVITY: g; PRESSURE: P */
+    const int32_t *index; /* FORCE: node [n]; PRESSURE: bulk element [n] (0-based) */
+    const int32_t *face;  /* PRESSURE: face [n] */
+    const double *force;  /* FORCE: [3n] */
+} mps_load;
 
 typedef struct mps_solver_cfg {
     double t0, tf;       /* *Time t0, tf */
@@ -129,16 +142,16 @@ int mps_create(const mps_mesh *mesh, const mps_material *mat, const mps_bcs *bcs
  * (solverExplicit.cpp:577-596). */
 int mps_set_state(mps_handle h, const double *u, const double *v, const double *a);
 
-/* Set the Neumann loads (replacing any earlier call; NULL or no entries and g = 0: none).  Valid only
- * before the first step.  The loads are evaluated in the node kernel at each step's time.
- * Replaces: the FORCE and GRAVITY entries of NeumannCommonManagement (forceBCManagement,
- * inputFile.cpp:2841-2941; *GRAVITY :4723-4735) and their per-step evaluation in NeumannBCManagement
- * (boundaryConditions.cpp:75-140). */
-int mps_set_loads(mps_handle h, const mps_loads *loads);
+/* Set the Neumann loads: n_loads blocks in input order (replacing any earlier call; 0 blocks: none).  At most
+ * one GRAVITY block.  Valid only before the first step.  Every step evaluates them at its own time and
+ * displacement: nodal forces and gravity in the node kernel, pressure faces in a surface kernel before it.
+ * Replaces: NeumannCommonManagement's FORCE, GRAVITY and PRESSURE RAMP / INST entries
+ * (inputFile.cpp:4703-4735) and their per-step evaluation in NeumannBCManagement (boundaryConditions.cpp:75-140). */
+int mps_set_loads(mps_handle h, int32_t n_loads, const mps_load *loads);
 
-/* External force fext[3*n_nodes] at time t from the loads of mps_set_loads (device kernel; does not
- * touch the solver state).
- * Replaces: NeumannBCManagement (boundaryConditions.cpp:75-140) for FORCE and GRAVITY. */
+/* External force fext[3*n_nodes] at time t for the solver's current displacement, from the loads of
+ * mps_set_loads (device kernels; does not touch the solver state).
+ * Replaces: NeumannBCManagement (boundaryConditions.cpp:75-140). */
 int mps_external_force(mps_handle h, double t, double *fext);
 
 /* Copy the current u, v, a to host (any pointer may be NULL).

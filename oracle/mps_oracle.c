@@ -490,6 +490,20 @@ void orc_gp_lumped_mass(int32_t nen, double wJ, double rho, const double *phi, d
 /* Solver state                                                        */
 /* ------------------------------------------------------------------ */
 
+/* Per Neumann block: what its constructor and the set-up fix (the rest is evaluated every step). */
+struct load_state {
+    double *fm;        /* FORCE: [3n] F/(tf-t0) (classNeumann ctor, classNodes.cpp:85-96) */
+    double *tg;        /* GRAVITY: [Ne*ngp] -w rho |J| g per bulk GP */
+    int ns;            /* PRESSURE: surface GPs (n faces x 4 or 1) */
+    int32_t *se;       /* [ns] bulk element */
+    int32_t *sn;       /* [ns*4] face nodes */
+    double *ws;        /* [ns] w |J_s| */
+    double *n0;        /* [ns*3] reference normal of the face */
+    double *phs;       /* [ns*4] face shape values */
+    double *phb;       /* [ns*8] bulk shape values at the point */
+    double *dphb;      /* [ns*24] bulk gradients at the point */
+};
+
 typedef struct {
     const orc_problem *p;
     int ngp, nen, N, Ne;
@@ -504,9 +518,8 @@ typedef struct {
     int32_t *rowptr, *col; /* CSR node -> (e*nen + a) ascending */
     double *fe;        /* [Ne*nen*3] element forces, [(e*nen+a)*3+i] */
     double *fint;      /* [3N] */
-    double *tgrav;     /* [Ne*ngp] gravity factor -w rho |J| g per GP (NULL: no gravity) */
     double *fext;      /* [3N] external force of the current step */
-    double *fm;        /* [3*n_force] F/(tf-t0) per nodal force (classNeumann ctor, classNodes.cpp:85-96) */
+    struct load_state *ld; /* [n_loads] */
     /* threading */
     int nt;
     double *tmin;      /* per-thread partials */
@@ -524,6 +537,203 @@ static void gather_xyz(const orc_state *s, int e, const double *u, double *xyz)
             if (u) x += u[(size_t)d * s->N + cn[a]];
             xyz[a * 3 + d] = x;
         }
+}
+
+/* Faces of getSubElement: hex (classLinearHexahedra.cpp:30-85), tet (classLinearTetrahedra.cpp:30-55). */
+static const int HEX_FACE[6][4] = {{0, 1, 2, 3}, {4, 7, 6, 5}, {0, 4, 5, 1}, {1, 5, 6, 2}, {2, 6, 7, 3}, {3, 7, 4, 0}};
+static const int TET_FACE[4][3] = {{0, 1, 2}, {0, 3, 1}, {1, 3, 2}, {0, 2, 3}};
+
+/* norm2: maths.cpp:1550-1557 */
+static double norm2_3(const double *a)
+{
+    double t = 0;
+    for (int i = 0; i < 3; i++) t += a[i] * a[i];
+    return sqrt(t);
+}
+
+/* crossProduct: geometry.cpp:180-190 */
+static void cross3(const double *A, const double *B, double *C)
+{
+    C[0] = A[1] * B[2] - A[2] * B[1];
+    C[1] = A[2] * B[0] - A[0] * B[2];
+    C[2] = A[0] * B[1] - A[1] * B[0];
+}
+
+/* Surface GPs of face S of bulk element e, each with the bulk shape at its position:
+ *   face nodes and getn0 (classLinearSquare.cpp:19-45, classLinearTriangles.cpp:19-45: (p2-p1)x(p3-p1)/|.|);
+ *   defineGPs of the face (classLinearSquare.cpp:196-301, 2x2 points +-0.577350269 weight 1;
+ *   classLinearTriangles.cpp:194-299, one point (0.33333333333333, 0.33333333333333) weight 0.5*1),
+ *   in the local plane basis of createOrthogonalBasisOnPlane (geometry.cpp:363-389): J = det of the 2x2
+ *   Jacobian, position = sum_k x_k phi_k;
+ *   defineShapeOnAPoint of the bulk element at that position (classLinearHexahedra.cpp:95-172: natural
+ *   coordinates 2 A^-1 (x - x_0) - 1 with A the corner edges 0-1, 0-3, 0-4; classLinearTetrahedra.cpp:66-117:
+ *   the shape at the natural origin, phi = (1, 0, 0, 0)).
+ * Returns the number of points or ORC_ERR_ARG. */
+static int surface_gps(const orc_state *s, int e, int S, int32_t *sn, double *pos, double *ws, double *n0,
+                       double *phs, double *phb, double *dphb)
+{
+    const int nen = s->nen, nf = (nen == 8) ? 4 : 3;
+    if (e < 0 || e >= s->Ne || S < 1 || S > ((nen == 8) ? 6 : 4)) return ORC_ERR_ARG;
+    const int *fl = (nen == 8) ? HEX_FACE[S - 1] : TET_FACE[S - 1];
+    double xb[24], xf[4][3];
+    gather_xyz(s, e, NULL, xb);
+    for (int k = 0; k < nf; k++) {
+        sn[k] = s->p->conn[(size_t)e * nen + fl[k]];
+        for (int d = 0; d < 3; d++) xf[k][d] = xb[fl[k] * 3 + d];
+    }
+    double nn[3];
+    for (int i = 0; i < 3; i++) {
+        const int c1 = (i == 0) ? 1 : (i == 1) ? 2 : 0, c2 = (i == 0) ? 2 : (i == 1) ? 0 : 1;
+        nn[i] = (xf[1][c1] - xf[0][c1]) * (xf[2][c2] - xf[0][c2]) - (xf[2][c1] - xf[0][c1]) * (xf[1][c2] - xf[0][c2]);
+    }
+    const double no = norm2_3(nn);
+    double a[3], b[3], normal[3], xhat[3], map[6];
+    for (int i = 0; i < 3; i++) {
+        a[i] = xf[1][i] - xf[0][i];
+        b[i] = xf[2][i] - xf[0][i];
+    }
+    cross3(b, a, normal);
+    cross3(a, normal, xhat);
+    const double magA = norm2_3(a), magX = norm2_3(xhat);
+    for (int i = 0; i < 3; i++) {
+        map[3 + i] = a[i] / magA;
+        map[i] = xhat[i] / magX;
+    }
+    double tmp[12], loc[8];
+    for (int k = 0; k < nf; k++)
+        for (int d = 0; d < 3; d++) tmp[d * nf + k] = xf[k][d] - xf[0][d];
+    mat_general(map, 2, 3, tmp, nf, loc);
+    const double G = 0.577350269;
+    const double sq[4][2] = {{G, G}, {-G, G}, {G, -G}, {-G, -G}};
+    const int ng = (nen == 8) ? 4 : 1;
+    for (int g = 0; g < ng; g++) {
+        double ph[4], nd[8], Jac[4], weight;
+        if (nen == 8) {
+            const double x = sq[g][0], y = sq[g][1];
+            ph[0] = 0.25 * (1 - x) * (1 - y);
+            ph[1] = 0.25 * (1 + x) * (1 - y);
+            ph[2] = 0.25 * (1 + x) * (1 + y);
+            ph[3] = 0.25 * (1 - x) * (1 + y);
+            nd[0] = -0.25 * (1 - y);
+            nd[2] = 0.25 * (1 - y);
+            nd[4] = 0.25 * (1 + y);
+            nd[6] = -0.25 * (1 + y);
+            nd[1] = -0.25 * (1 - x);
+            nd[3] = -0.25 * (1 + x);
+            nd[5] = 0.25 * (1 + x);
+            nd[7] = 0.25 * (1 - x);
+            weight = 1 * 1.0;
+        } else {
+            const double x = 0.33333333333333, y = 0.33333333333333;
+            ph[0] = -1.0 * x - 1.0 * y + 1.0;
+            ph[1] = 1.0 * x;
+            ph[2] = 1.0 * y;
+            ph[3] = 0.0;
+            nd[0] = -1.0 - 0.0 + 0.0;
+            nd[2] = 1.0;
+            nd[4] = 0.0;
+            nd[1] = -0.0 - 1.0 + 0.0;
+            nd[3] = 0.0;
+            nd[5] = 1.0;
+            weight = 0.5 * 1.0;
+        }
+        mat_general(loc, 2, nf, nd, 2, Jac);
+        const double J = Jac[0] * Jac[3] - Jac[2] * Jac[1];
+        double x[3];
+        for (int i = 0; i < 3; i++) {
+            x[i] = 0;
+            for (int k = 0; k < nf; k++) x[i] += xf[k][i] * ph[k];
+        }
+        double xw[4] = {0, 0, 0, 1};
+        if (nen == 8) {
+            double A[9], Ai[9], X[3];
+            const int c[3] = {1, 3, 4};
+            for (int d = 0; d < 3; d++)
+                for (int j = 0; j < 3; j++) A[d * 3 + j] = xb[c[j] * 3 + d] - xb[d];
+            for (int d = 0; d < 3; d++) X[d] = x[d] - xb[d];
+            orc_inverse3(A, Ai);
+            for (int i = 0; i < 3; i++) {
+                double r = 0;
+                for (int j = 0; j < 3; j++) r += Ai[i * 3 + j] * X[j];
+                xw[i] = 2 * r - 1;
+            }
+        }
+        double wJb, dJb;
+        orc_define_gp(nen, xb, xw, phb + g * 8, dphb + g * 24, &wJb, &dJb);
+        ws[g] = weight * fabs(J);
+        for (int i = 0; i < 3; i++) {
+            pos[g * 3 + i] = x[i];
+            n0[g * 3 + i] = nn[i] / no;
+        }
+        for (int k = 0; k < 4; k++) phs[g * 4 + k] = (k < nf) ? ph[k] : 0.0;
+        if (g > 0)
+            for (int k = 0; k < 4; k++) sn[g * 4 + k] = sn[k];
+    }
+    return ng;
+}
+
+/* Set-up of the Neumann blocks: the classNeumann constructor's F/(tf-t0), the gravity factors of every
+ * bulk GP, and the surface GPs of every pressure face (built once here; the reference rebuilds them each
+ * step from the same reference coordinates, boundaryConditions.cpp:374-376). */
+static int loads_init(orc_state *s, const double *xw)
+{
+    const orc_problem *p = s->p;
+    const size_t N = s->N, Ne = s->Ne, ng = s->ngp;
+    s->fext = calloc(3 * N, sizeof(double));
+    if (!s->fext) return ORC_ERR_MEM;
+    if (p->n_loads <= 0) return ORC_OK;
+    if (!p->loads) return ORC_ERR_ARG;
+    s->ld = calloc((size_t)p->n_loads, sizeof(struct load_state));
+    if (!s->ld) return ORC_ERR_MEM;
+    for (int b = 0; b < p->n_loads; b++) {
+        const orc_load *L = &p->loads[b];
+        struct load_state *l = &s->ld[b];
+        if (L->n < 0) return ORC_ERR_ARG;
+        if (L->type == ORC_LOAD_FORCE) {
+            l->fm = malloc(sizeof(double) * 3 * ((size_t)L->n + 1));
+            if (!l->fm) return ORC_ERR_MEM;
+            for (int k = 0; k < L->n; k++) {
+                if (L->index[k] < 0 || L->index[k] >= s->N) return ORC_ERR_ARG;
+                for (int d = 0; d < 3; d++) l->fm[3 * k + d] = L->force[3 * k + d] / (L->tf - L->t0);
+            }
+        } else if (L->type == ORC_LOAD_GRAVITY) {  /* -w * rho * |J| * g, times phi_a per node */
+            l->tg = malloc(sizeof(double) * Ne * ng);
+            if (!l->tg) return ORC_ERR_MEM;
+            for (size_t e = 0; e < Ne; e++) {
+                double xyz[24];
+                gather_xyz(s, (int)e, NULL, xyz);
+                for (size_t g = 0; g < ng; g++) {
+                    double phi[8], dphi[24], wJ, detJ;
+                    orc_define_gp(p->nen, xyz, xw + g * 4, phi, dphi, &wJ, &detJ);
+                    const double w = xw[g * 4 + 3], J = fabs(detJ);
+                    l->tg[e * ng + g] = -w * s->rho[e] * J * L->value;
+                }
+            }
+        } else if (L->type == ORC_LOAD_PRESSURE_RAMP || L->type == ORC_LOAD_PRESSURE_INST) {
+            const int per = (s->nen == 8) ? 4 : 1;
+            const size_t ns = (size_t)L->n * per + 1;
+            l->se = malloc(sizeof(int32_t) * ns);
+            l->sn = malloc(sizeof(int32_t) * ns * 4);
+            l->ws = malloc(sizeof(double) * ns);
+            l->n0 = malloc(sizeof(double) * ns * 3);
+            l->phs = malloc(sizeof(double) * ns * 4);
+            l->phb = malloc(sizeof(double) * ns * 8);
+            l->dphb = malloc(sizeof(double) * ns * 24);
+            if (!l->se || !l->sn || !l->ws || !l->n0 || !l->phs || !l->phb || !l->dphb) return ORC_ERR_MEM;
+            for (int f = 0; f < L->n; f++) {
+                double pos[12];
+                const int g0 = l->ns;
+                const int r = surface_gps(s, L->index[f], L->face[f], l->sn + 4 * g0, pos, l->ws + g0, l->n0 + 3 * g0,
+                                          l->phs + 4 * g0, l->phb + 8 * g0, l->dphb + 24 * g0);
+                if (r < 0) return r;
+                for (int g = 0; g < r; g++) l->se[g0 + g] = L->index[f];
+                l->ns += r;
+            }
+        } else {
+            return ORC_ERR_ARG;
+        }
+    }
+    return ORC_OK;
 }
 
 static int state_init(orc_state *s, const orc_problem *p)
@@ -571,31 +781,6 @@ static int state_init(orc_state *s, const orc_problem *p)
         s->c[e] = (p->law == ORC_LAW_SVK || p->law == ORC_LAW_LINEAR) ? sqrt(E / s->rho[e])
                                                                       : sqrt((s->lam[e] + 2 * s->mu[e]) / s->rho[e]);
     }
-    /* Neumann loads */
-    s->fext = calloc(3 * N, sizeof(double));
-    if (!s->fext) return ORC_ERR_MEM;
-    if (p->n_force > 0) {
-        s->fm = malloc(sizeof(double) * 3 * (size_t)p->n_force);
-        if (!s->fm) return ORC_ERR_MEM;
-        for (int k = 0; k < p->n_force; k++) {
-            if (p->force_node[k] < 0 || p->force_node[k] >= s->N) return ORC_ERR_ARG;
-            for (int d = 0; d < 3; d++) s->fm[3 * k + d] = p->force_vec[3 * k + d] / (p->force_tf[k] - p->force_t0[k]);
-        }
-    }
-    if (p->gravity != 0.0) {  /* BCsGravity (:1146-1172): per GP the factor -w * rho * |J| * g, times phi_a */
-        s->tgrav = malloc(sizeof(double) * Ne * ng);
-        if (!s->tgrav) return ORC_ERR_MEM;
-        for (size_t e = 0; e < Ne; e++) {
-            double xyz[24];
-            gather_xyz(s, (int)e, NULL, xyz);
-            for (size_t g = 0; g < ng; g++) {
-                double phi[8], dphi[24], wJ, detJ;
-                orc_define_gp(p->nen, xyz, xw + g * 4, phi, dphi, &wJ, &detJ);
-                const double w = xw[g * 4 + 3], J = fabs(detJ);
-                s->tgrav[e * ng + g] = -w * s->rho[e] * J * p->gravity;
-            }
-        }
-    }
     /* active DOF map (dispBCManagement, inputFile.cpp:3185-3212) */
     memset(s->active, 1, 3 * N);
     for (int k = 0; k < p->n_dir; k++) {
@@ -620,15 +805,22 @@ static int state_init(orc_state *s, const orc_problem *p)
             s->col[fill[n]++] = (int32_t)(e * nen + a);
         }
     free(fill);
-    return ORC_OK;
+    return loads_init(s, xw);
 }
 
 static void state_free(orc_state *s)
 {
+    if (s->ld)
+        for (int k = 0; k < s->p->n_loads; k++) {
+            struct load_state *l = &s->ld[k];
+            free(l->fm); free(l->tg); free(l->se); free(l->sn); free(l->ws); free(l->n0);
+            free(l->phs); free(l->phb); free(l->dphb);
+        }
+    free(s->ld);
     free(s->phi); free(s->dphi); free(s->wJ); free(s->l0); free(s->c);
     free(s->lam); free(s->mu); free(s->K1); free(s->rho); free(s->mass);
     free(s->active); free(s->rowptr); free(s->col); free(s->fe); free(s->fint);
-    free(s->tmin); free(s->tsum); free(s->tgrav); free(s->fext); free(s->fm);
+    free(s->tmin); free(s->tsum); free(s->fext);
 }
 
 /* ------------------------------------------------------------------ */
@@ -763,32 +955,75 @@ static void task_gather(orc_state *s, int tid, int nt, void *arg)
 
 /* BCsGravity (boundaryConditions.cpp:1146-1172): force[n_a + 2N] += -w * rho * |J| * g * phi_a,
  * GP by GP in element order, onto what the loads before it have added. */
-static void add_gravity(orc_state *s)
+static void add_gravity(orc_state *s, const double *tg)
 {
     size_t N = s->N, Ne = s->Ne, ng = s->ngp, nen = s->nen;
     for (size_t e = 0; e < Ne; e++)
         for (size_t g = 0; g < ng; g++) {
-            const double t = s->tgrav[e * ng + g];
+            const double t = tg[e * ng + g];
             const double *phi = s->phi + (e * ng + g) * nen;
             for (size_t a = 0; a < nen; a++) s->fext[s->p->conn[e * nen + a] + 2 * N] += t * phi[a];
         }
 }
 
-/* fext at time t (NeumannBCManagement, boundaryConditions.cpp:75-140): zero, then the loads in input
- * order -- nodal forces (BCsNodalForces, :1120-1134, with classNeumann::getTracAmount,
- * classNodes.h:101-108: F/(tf-t0) * t for t0 <= t <= tf, else 0) and gravity at its place. */
-static void external_force(orc_state *s, double t)
+/* BCsPressureRamp / BCsPressureInst (boundaryConditions.cpp:370-496): per surface GP, F of the bulk element at
+ * the point (classSpatialPoint::getF, classSpatialPoint.cpp:41-68: the node terms, then the identity),
+ * n = F^-T n0 (inverse, transpose, multTensorVector), and force += w |J_s| phi_a J P n_i over the bulk
+ * element's nodes and shape (ramp) or the face's (inst). */
+static void add_pressure(orc_state *s, const orc_load *L, const struct load_state *l, const double *u, double t)
+{
+    const size_t N = s->N;
+    const int nen = s->nen, ramp = (L->type == ORC_LOAD_PRESSURE_RAMP);
+    double P = 0.;
+    if (t >= L->t0 && t <= L->tf) P = ramp ? L->value / (L->tf - L->t0) * (t - L->t0) : L->value;
+    for (int g = 0; g < l->ns; g++) {
+        const int32_t *cn = s->p->conn + (size_t)l->se[g] * nen;
+        const double *dphi = l->dphb + 24 * g;
+        double F[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, Fi[9], FiT[9], n[3] = {0, 0, 0};
+        for (int a = 0; a < nen; a++) {
+            double disp[3];
+            for (int i = 0; i < 3; i++) disp[i] = u ? u[cn[a] + N * i] : 0.0;
+            for (int i = 0; i < 3; i++)
+                for (int j = 0; j < 3; j++) F[i * 3 + j] += dphi[a * 3 + j] * disp[i];
+        }
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++) F[i * 3 + j] += (i == j) ? 1 : 0;
+        const double Jac = orc_det3(F);
+        orc_inverse3(F, Fi);
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++) FiT[i * 3 + j] = Fi[j * 3 + i];
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++) n[i] += FiT[i * 3 + j] * l->n0[3 * g + j];
+        const int na = ramp ? nen : (nen == 8 ? 4 : 3);
+        for (int a = 0; a < na; a++) {
+            const int32_t node = ramp ? cn[a] : l->sn[4 * g + a];
+            const double ph = ramp ? l->phb[8 * g + a] : l->phs[4 * g + a];
+            for (int i = 0; i < 3; i++) s->fext[node + N * i] += l->ws[g] * ph * Jac * P * n[i];
+        }
+    }
+}
+
+/* fext at time t for displacement u (NeumannBCManagement, boundaryConditions.cpp:75-140): zero, then every
+ * block in input order -- nodal forces (BCsNodalForces, :1120-1134, with classNeumann::getTracAmount,
+ * classNodes.h:101-108: F/(tf-t0) * t for t0 <= t <= tf, else 0), gravity and pressure. */
+static void external_force(orc_state *s, const double *u, double t)
 {
     const orc_problem *p = s->p;
     size_t N = s->N;
-    const int gpos = (p->gravity_pos >= 0 && p->gravity_pos <= p->n_force) ? p->gravity_pos : p->n_force;
     for (size_t i = 0; i < 3 * N; i++) s->fext[i] = 0;
-    for (int k = 0; k < p->n_force; k++) {
-        if (k == gpos && s->tgrav) add_gravity(s);
-        const int in = (t >= p->force_t0[k] && t <= p->force_tf[k]);
-        for (int d = 0; d < 3; d++) s->fext[p->force_node[k] + N * d] += in ? s->fm[3 * k + d] * t : 0.0;
+    for (int b = 0; b < p->n_loads; b++) {
+        const orc_load *L = &p->loads[b];
+        const struct load_state *l = &s->ld[b];
+        if (L->type == ORC_LOAD_FORCE) {
+            const int in = (t >= L->t0 && t <= L->tf);
+            for (int k = 0; k < L->n; k++)
+                for (int d = 0; d < 3; d++) s->fext[L->index[k] + N * d] += in ? l->fm[3 * k + d] * t : 0.0;
+        } else if (L->type == ORC_LOAD_GRAVITY) {
+            add_gravity(s, l->tg);
+        } else {
+            add_pressure(s, L, l, u, t);
+        }
     }
-    if (gpos == p->n_force && s->tgrav) add_gravity(s);
 }
 
 /* computeInternalForce: solverExplicit.cpp:363-379 (+ computeState, solvers.cpp:615-641) */
@@ -1014,14 +1249,25 @@ static void gp_state_outputs(int32_t law, const double *F, double lam, double mu
     }
 }
 
-int32_t orc_external_force(const orc_problem *p, double t, double *fext)
+int32_t orc_external_force(const orc_problem *p, const double *u, double t, double *fext)
 {
     orc_state s;
     int rc = setup(&s, p);
     if (rc == ORC_OK) {
-        external_force(&s, t);
+        external_force(&s, u, t);
         memcpy(fext, s.fext, sizeof(double) * 3 * (size_t)s.N);
     }
+    state_free(&s);
+    return rc;
+}
+
+int32_t orc_pressure_face(const orc_problem *p, int32_t elem, int32_t face, double *xyz, double *ws, double *n0,
+                          double *phi_s, double *phi_b, double *dphi_b)
+{
+    orc_state s;
+    int32_t sn[16];
+    int rc = setup(&s, p);
+    if (rc == ORC_OK) rc = surface_gps(&s, elem, face, sn, xyz, ws, n0, phi_s, phi_b, dphi_b);
     state_free(&s);
     return rc;
 }
@@ -1133,7 +1379,7 @@ int32_t orc_run(const orc_problem *p, double *u, double *v, double *a, orc_histo
         /* computeState + computeRightHandSide: fint(u) */
         internal_force(&s, u);
         /* NeumannBCManagement at timeRun (boundaryConditions.cpp:75-140) */
-        external_force(&s, timeRun);
+        external_force(&s, u, timeRun);
         /* systemSolve, nonlinearSystem.cpp:1096-1107: a = (-fint + fext)/m, v update */
         for (size_t i = 0; i < 3 * N; i++) {
             if (!s.active[i]) continue;

@@ -31,6 +31,24 @@ enum { ORC_BC_RAMP = 0, ORC_BC_INST = 1, ORC_BC_VELOCITY = 2 };
 enum { ORC_SUM_REFERENCE = 0, ORC_SUM_ELEMENT = 1 };
 enum { ORC_OK = 0, ORC_ERR_DT = -1, ORC_ERR_NAN = -2, ORC_ERR_ARG = -3, ORC_ERR_MEM = -4 };
 
+enum { ORC_LOAD_FORCE = 0, ORC_LOAD_GRAVITY = 1, ORC_LOAD_PRESSURE_RAMP = 2, ORC_LOAD_PRESSURE_INST = 3 };
+
+/* One Neumann block.
+ *   FORCE (forceBCManagement, inputFile.cpp:2841-2941): n nodes index[], forces force[3n] = F * Ff over [t0, tf];
+ *     F(t) = force/(tf-t0) * t inside the window, else 0 (classNeumann, classNodes.h:101-108, classNodes.cpp:85-96).
+ *   GRAVITY (inputFile.cpp:4723-4735): value = g; f_z += -w rho |J| g phi_a per GP (boundaryConditions.cpp:1146-1172).
+ *   PRESSURE RAMP / INST (pressure*BCManagement, inputFile.cpp:3580-3790): n faces, bulk element index[] and
+ *     face[] (getSubElement numbering); value = P; P(t) = P/(tf-t0) * (t-t0) (ramp) or P (inst) inside
+ *     [t0, tf], else 0 (classNeumannBCs.h:144-176); Nanson force, boundaryConditions.cpp:370-496. */
+typedef struct orc_load {
+    int32_t type;          /* ORC_LOAD_* */
+    int32_t n;
+    double t0, tf, value;
+    const int32_t *index;  /* FORCE: node; PRESSURE: bulk element (0-based) */
+    const int32_t *face;   /* PRESSURE: face S, 1..6 (hex) or 1..4 (tet) */
+    const double *force;   /* FORCE: [3n] */
+} orc_load;
+
 typedef struct orc_problem {
     int32_t n_nodes;
     int32_t n_elem;
@@ -53,15 +71,12 @@ typedef struct orc_problem {
     int32_t n_threads;      /* >1 requires ORC_SUM_ELEMENT */
     int32_t energy_every;   /* 0: no energies; k: every k-th step (and the last) */
     int32_t _pad;
-    /* Neumann loads, fext of rhs = -fint + fext (NeumannBCManagement, boundaryConditions.cpp:75-140) */
-    double gravity;            /* *GRAVITY g (0: none): f_z += -w rho |J| g phi_a per GP (:1146-1172) */
-    int32_t n_force;           /* *BOUNDARY, TYPE=FORCE nodes */
-    int32_t gravity_pos;       /* loads are added in input order: gravity before force entry gravity_pos
-                                  (outside 0..n_force: after all of them) */
-    const int32_t *force_node; /* [n_force] 0-based node */
-    const double *force_vec;   /* [3*n_force] F * Ff (inputFile.cpp:2905-2914) */
-    const double *force_t0;    /* [n_force] load window; F(t) = F/(tf-t0) * t inside it (classNodes.h:101-108) */
-    const double *force_tf;
+    /* Neumann loads in input order, the reference's NeumannBCs list (NeumannCommonManagement,
+     * inputFile.cpp:4703-4760); fext of rhs = -fint + fext is rebuilt from them every step
+     * (NeumannBCManagement, boundaryConditions.cpp:75-140) */
+    int32_t n_loads;
+    int32_t _pad2;
+    const struct orc_load *loads;
 } orc_problem;
 
 typedef struct orc_report {
@@ -119,8 +134,15 @@ double orc_critical_dt(const orc_problem *p, const double *u);
  * the linear law), von Mises stress, volumetric and equivalent strain, as the VTK writer prints them
  * (output.cpp:567-820). */
 int32_t orc_element_outputs(const orc_problem *p, const double *u, double *out);
-/* External force fext[3*n_nodes] at time t (gravity + nodal forces). */
-int32_t orc_external_force(const orc_problem *p, double t, double *fext);
+/* External force fext[3*n_nodes] at time t for displacement u (NULL: zero), from the loads. */
+int32_t orc_external_force(const orc_problem *p, const double *u, double t, double *fext);
+
+/* Surface Gauss points of one pressure face (classLinearSquare / classLinearTriangles::defineGPs with the
+ * bulk shape at each point, defineShapeOnAPoint): returns the number of points (4 hex, 1 tet), or < 0.
+ * Per point g: xyz[3g..] position, ws[g] = w |J_s|, n0[3g..] = getn0 of the face, phi_s[g*4..] the face
+ * shape values, phi_b[g*8..] and dphi_b[g*24..] the bulk shape values and gradients at the point. */
+int32_t orc_pressure_face(const orc_problem *p, int32_t elem, int32_t face, double *xyz, double *ws, double *n0,
+                          double *phi_s, double *phi_b, double *dphi_b);
 
 #ifdef __cplusplus
 }

@@ -32,9 +32,63 @@ class Problem(C.Structure):
         ("t0", C.c_double), ("tf", C.c_double), ("scale", C.c_double),
         ("n_steps", C.c_int64), ("sum_order", C.c_int32), ("n_threads", C.c_int32),
         ("energy_every", C.c_int32), ("_pad", C.c_int32),
-        ("gravity", C.c_double), ("n_force", C.c_int32), ("gravity_pos", C.c_int32),
-        ("force_node", _ip), ("force_vec", _dp), ("force_t0", _dp), ("force_tf", _dp),
+        ("n_loads", C.c_int32), ("_pad2", C.c_int32), ("loads", C.c_void_p),
     ]
+
+
+class Load(C.Structure):
+    _fields_ = [("type", C.c_int32), ("n", C.c_int32), ("t0", C.c_double), ("tf", C.c_double),
+                ("value", C.c_double), ("index", _ip), ("face", _ip), ("force", _dp)]
+
+
+LOAD_FORCE, LOAD_GRAVITY, LOAD_PRESSURE_RAMP, LOAD_PRESSURE_INST = 0, 1, 2, 3
+
+
+def load_blocks(forces=None, gravity=0.0, gravity_pos=-1, loads=None):
+    """Neumann blocks in input order, as tuples:
+      ("force", nodes, vec[k,3], t0, tf)                 *BOUNDARY, TYPE=FORCE (entries F*Ff, one window)
+      ("gravity", g)                                     *GRAVITY
+      ("pressure_ramp" | "pressure_inst", P, t0, tf, elems, faces)
+    `loads` is such a list; otherwise forces = (node[k], vec[k,3], t0[k], tf[k]) gives one block per entry and
+    a non-zero gravity is placed before entry gravity_pos (-1: after all)."""
+    if loads is not None:
+        return list(loads)
+    out = []
+    if forces is not None:
+        fn, fv, f0, f1 = forces
+        fv = np.reshape(np.asarray(fv, dtype=np.float64), (-1, 3))
+        out = [("force", [int(fn[k])], fv[k:k + 1], float(f0[k]), float(f1[k])) for k in range(len(fn))]
+    if gravity != 0.0:
+        pos = gravity_pos if 0 <= gravity_pos <= len(out) else len(out)
+        out.insert(pos, ("gravity", float(gravity)))
+    return out
+
+
+def pack_loads(blocks):
+    """ctypes array of Load for the blocks, plus the numpy buffers it points into (keep them alive)."""
+    keep = []
+    arr = (Load * max(len(blocks), 1))()
+    for b, blk in enumerate(blocks):
+        kind = blk[0]
+        L = arr[b]
+        if kind == "force":
+            nodes = np.ascontiguousarray(blk[1], dtype=np.int32)
+            vec = np.ascontiguousarray(np.reshape(blk[2], (-1, 3)), dtype=np.float64)
+            keep += [nodes, vec]
+            L.type, L.n, L.t0, L.tf = LOAD_FORCE, nodes.size, blk[3], blk[4]
+            L.index, L.force = iptr(nodes), dptr(vec)
+        elif kind == "gravity":
+            L.type, L.n, L.value = LOAD_GRAVITY, 0, blk[1]
+        elif kind in ("pressure_ramp", "pressure_inst"):
+            el = np.ascontiguousarray(blk[4], dtype=np.int32)
+            fc = np.ascontiguousarray(blk[5], dtype=np.int32)
+            keep += [el, fc]
+            L.type = LOAD_PRESSURE_RAMP if kind == "pressure_ramp" else LOAD_PRESSURE_INST
+            L.n, L.value, L.t0, L.tf = el.size, blk[1], blk[2], blk[3]
+            L.index, L.face = iptr(el), iptr(fc)
+        else:
+            raise ValueError(kind)
+    return arr, keep
 
 
 class Report(C.Structure):
@@ -82,7 +136,9 @@ def lib():
         L.orc_internal_force.restype = C.c_int32
         L.orc_element_outputs.restype = C.c_int32
         L.orc_external_force.restype = C.c_int32
-        L.orc_external_force.argtypes = [C.c_void_p, C.c_double, _dp]
+        L.orc_external_force.argtypes = [C.c_void_p, _dp, C.c_double, _dp]
+        L.orc_pressure_face.restype = C.c_int32
+        L.orc_pressure_face.argtypes = [C.c_void_p, C.c_int32, C.c_int32, _dp, _dp, _dp, _dp, _dp, _dp]
         L.orc_internal_force.argtypes = [C.POINTER(Problem), _dp, _dp]
         L.orc_critical_dt.restype = C.c_double
         L.orc_critical_dt.argtypes = [C.POINTER(Problem), _dp]
@@ -106,9 +162,8 @@ class OracleProblem:
     def __init__(self, mesh, law=LAW_NH_COMPRESSIBLE, E=1e6, nu=0.3, rho=1000.0,
                  dir_dof=None, dir_val=None, bc_mode=BC_VELOCITY, vel_ramp_steps=100,
                  t0=0.0, tf=1.0, scale=0.5, n_steps=0, sum_order=SUM_REFERENCE,
-                 n_threads=1, energy_every=0, gravity=0.0, forces=None, gravity_pos=-1):
-        """forces: None or (node[k], vec[k,3], t0[k], tf[k]) nodal loads (*BOUNDARY, TYPE=FORCE);
-        gravity_pos: index of the force entry that gravity precedes in input order (-1: after all)."""
+                 n_threads=1, energy_every=0, gravity=0.0, forces=None, gravity_pos=-1, loads=None):
+        """Neumann loads: see load_blocks (forces / gravity / gravity_pos, or an explicit block list)."""
         self.X = np.ascontiguousarray(mesh["X"], dtype=np.float64)
         self.conn = np.ascontiguousarray(mesh["conn"], dtype=np.int32)
         self.E = np.atleast_1d(np.asarray(E, dtype=np.float64)).copy()
@@ -124,12 +179,8 @@ class OracleProblem:
         self.dir_dof = np.ascontiguousarray(dir_dof, dtype=np.int32)
         self.dir_val = np.ascontiguousarray(dir_val, dtype=np.float64)
         self.n_nodes = mesh["n_nodes"]
-        fn, fv, f0, f1 = forces if forces is not None else ([], np.zeros((0, 3)), [], [])
-        self.f_node = np.ascontiguousarray(fn, dtype=np.int32)
-        self.f_vec = np.ascontiguousarray(np.asarray(fv, dtype=np.float64).ravel())
-        self.f_t0 = np.ascontiguousarray(f0, dtype=np.float64)
-        self.f_tf = np.ascontiguousarray(f1, dtype=np.float64)
-        nf = int(self.f_node.size)
+        self.blocks = load_blocks(forces, gravity, gravity_pos, loads)
+        self.load_arr, self.load_keep = pack_loads(self.blocks)
         self.p = Problem(
             n_nodes=mesh["n_nodes"], n_elem=mesh["n_elem"], nen=mesh["nen"], law=law,
             X=dptr(self.X), conn=iptr(self.conn), mat_per_elem=per,
@@ -139,9 +190,8 @@ class OracleProblem:
             dir_val=dptr(self.dir_val) if self.dir_val.size else None,
             bc_mode=bc_mode, vel_ramp_steps=vel_ramp_steps, t0=t0, tf=tf, scale=scale,
             n_steps=n_steps, sum_order=sum_order, n_threads=n_threads,
-            energy_every=energy_every, gravity=gravity, n_force=nf, gravity_pos=gravity_pos,
-            force_node=iptr(self.f_node) if nf else None, force_vec=dptr(self.f_vec) if nf else None,
-            force_t0=dptr(self.f_t0) if nf else None, force_tf=dptr(self.f_tf) if nf else None)
+            energy_every=energy_every, n_loads=len(self.blocks),
+            loads=C.cast(self.load_arr, C.c_void_p) if self.blocks else None)
 
     def run(self, u0=None, v0=None, a0=None, hist_cap=0):
         n3 = 3 * self.n_nodes
@@ -181,11 +231,24 @@ class OracleProblem:
         assert rc == 0
         return out.reshape(-1, 21)
 
-    def external_force(self, t):
+    def external_force(self, t, u=None):
         f = np.zeros(3 * self.n_nodes)
-        rc = lib().orc_external_force(C.byref(self.p), t, dptr(f))
+        if u is not None:
+            u = np.ascontiguousarray(u, dtype=np.float64)
+        rc = lib().orc_external_force(C.byref(self.p), dptr(u) if u is not None else None, t, dptr(f))
         assert rc == 0
         return f
+
+    def pressure_face(self, elem, face):
+        """Surface GPs of one face: dict of position, ws, n0, phi_s, phi_b, dphi_b (one row per point)."""
+        out = dict(xyz=np.zeros(12), ws=np.zeros(4), n0=np.zeros(12), phi_s=np.zeros(16), phi_b=np.zeros(32),
+                   dphi_b=np.zeros(96))
+        ng = lib().orc_pressure_face(C.byref(self.p), elem, face, *[dptr(out[k]) for k in
+                                                                    ("xyz", "ws", "n0", "phi_s", "phi_b", "dphi_b")])
+        assert ng > 0, ng
+        return dict(xyz=out["xyz"][:3 * ng].reshape(ng, 3), ws=out["ws"][:ng], n0=out["n0"][:3 * ng].reshape(ng, 3),
+                    phi_s=out["phi_s"][:4 * ng].reshape(ng, 4), phi_b=out["phi_b"][:8 * ng].reshape(ng, 8),
+                    dphi_b=out["dphi_b"][:24 * ng].reshape(ng, 8, 3))
 
     def critical_dt(self, u=None):
         if u is not None:

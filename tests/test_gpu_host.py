@@ -176,6 +176,36 @@ def test_force_and_gravity_loads_from_input(tmp_path):
     assert rows[-1, 1] == pytest.approx(float(np.dot(r["u"], fext[-1])), rel=1e-10)
 
 
+def test_pressure_loads_from_input(tmp_path):
+    """*BOUNDARY, TYPE=PRESSURE RAMP / INST blocks ("t0, tf, P, S" and element ids) on the top face of a clamped
+    block, with *GRAVITY between them, against the oracle's surface-force restatement."""
+    m = configs.box(3, 3, 3)
+    N = m["n_nodes"]
+    X = m["X"].reshape(3, N)
+    conn = m["conn"].reshape(-1, 8)
+    bot = [n for n in range(N) if X[2, n] < 1e-12]
+    bc = [(n, d, 0.0) for n in bot for d in range(3)]
+    dof = np.array([n + N * d for n, d, _ in bc], np.int32)
+    val = np.zeros(dof.size)
+    top = [e for e in range(conn.shape[0]) if np.all(X[2, conn[e, 4:]] > 1 - 1e-12)]  # face 2 = nodes 4,7,6,5
+    side = [e for e in range(conn.shape[0]) if np.all(X[0, conn[e, [1, 5, 6, 2]]] > 1 - 1e-12)]  # face 4
+    dt0 = O.OracleProblem(m, dir_dof=dof, dir_val=val, bc_mode=O.BC_RAMP, tf=1.0, scale=0.5).critical_dt()
+    tf = 60.5 * dt0
+    lines = ["*BOUNDARY, TYPE=PRESSURE RAMP", f"0, {0.8 * tf:.17g}, 5e4, 2"] + [str(e + 1) for e in top]
+    lines += ["*GRAVITY", "9.81", "*BOUNDARY, TYPE=PRESSURE INST", f"{0.2 * tf:.17g}, {tf:.17g}, -2e4, 4"]
+    lines += [f"{e + 1}, ignored" for e in side]
+    inp = tmp_path / "press.inp"
+    inpgen.write_inp(inp, m, [dict(rho=1000.0, E=1e6, nu=0.3)], [dict(scale=0.5, t0=0.0, tf=tf, outputs=1, bc=bc)],
+                     extra="\n".join(lines))
+    u, _ = run(inp, tmp_path, "-noVtk")
+    loads = [("pressure_ramp", 5e4, 0.0, 0.8 * tf, top, [2] * len(top)), ("gravity", 9.81),
+             ("pressure_inst", -2e4, 0.2 * tf, tf, side, [4] * len(side))]
+    r = O.OracleProblem(m, dir_dof=dof, dir_val=val, bc_mode=O.BC_RAMP, t0=0.0, tf=tf, scale=0.5,
+                        loads=loads).run()
+    assert r["rc"] == 0 and r["steps"] > 50
+    assert rel(u, r["u"]) <= 1e-12 and np.abs(u).max() > 1e-6
+
+
 def parse_vtk(path):
     """Sections of a legacy ASCII VTK file written by the series writer: name -> list of value rows."""
     lines = open(path).read().split("\n")

@@ -114,7 +114,8 @@ def test_reads_svk_and_linear_elastic(tmp_path):
     ("no_dirichlet", "at least one mechanical Dirichlet DOF"),
     ("implicit", "is not on this path"),
     ("stvk", "has no device descriptor"),
-    ("neumann", "TYPE=PRESSURE INST is not on this path"),
+    ("neumann", "TYPE=TRACTION INST is not on this path"),
+    ("pressure_face", "face S outside 1..6"),
     ("force_window", "tf > t0"),
     ("force_entry", "node, Fx, Fy, Fz"),
     ("two_gravity", "one gravity load"),
@@ -141,7 +142,9 @@ def test_input_errors(tmp_path, case, needle):
     elif case == "bad_direction":
         sol["bc"] = [(0, 3, 0.0)]
     elif case == "neumann":
-        extra = "*BOUNDARY, TYPE=PRESSURE INST\n0,2,1e3,2\n4"
+        extra = "*BOUNDARY, TYPE=TRACTION INST\n0,2,1e3,2\n4"
+    elif case == "pressure_face":
+        extra = "*BOUNDARY, TYPE=PRESSURE INST\n0, 1, 1e3, 7\n1"
     elif case == "force_window":
         extra = "*BOUNDARY, TYPE=FORCE\n0.5, 0.5, 1\n1, 1, 0, 0"
     elif case == "force_entry":
@@ -179,13 +182,25 @@ def test_reads_force_and_gravity(tmp_path):
     rc, j, err = check(p)
     assert rc == 0, err
     ld = j["loads"]
-    assert ld["gravity"] == 9.81 and ld["gravity_pos"] == 3
-    assert ld["force"] == [[0, 0, 0, 1, 0, 1], [2, 2, -4, 1, 0, 0.5], [4, 0, 0, 8, 0, 0.5], [2, 7, 0, 0, 0.1, 0.3]]
+    assert [b["type"] for b in ld] == ["FORCE", "FORCE", "GRAVITY", "FORCE"]
+    assert ld[0]["entries"] == [[0, 0, 0, 1]] and (ld[0]["t0"], ld[0]["tf"]) == (0, 1)
+    assert ld[1]["entries"] == [[2, 2, -4, 1], [4, 0, 0, 8]] and (ld[1]["t0"], ld[1]["tf"]) == (0, 0.5)
+    assert ld[2]["value"] == 9.81
+    assert ld[3]["entries"] == [[2, 7, 0, 0]] and (ld[3]["t0"], ld[3]["tf"]) == (0.1, 0.3)
     assert j["solvers"][0]["dirichlet"] == len(bc)
-    # no loads: empty list and no gravity
+    # pressure blocks: "t0, tf, P, S" then element ids (their leading integer)
+    inpgen.write_inp(p, m, [dict(rho=1000.0, E=1e6, nu=0.3)], [dict(scale=0.5, t0=0.0, tf=0.01, outputs=1, bc=bc)],
+                     extra="\n".join(["*BOUNDARY, TYPE=PRESSURE RAMP", "0.0, 0.5, 3e3, 2", "5", "6, 1", "*GRAVITY", "2.0",
+                                      "*BOUNDARY, TYPE=PRESSURE INST", "0.25, 0.25, -1e3, 6", "8"]))
+    rc, j, err = check(p)
+    assert rc == 0, err
+    assert j["loads"] == [dict(type="PRESSURE RAMP", t0=0, tf=0.5, value=3e3, entries=[[4, 2], [5, 2]]),
+                          dict(type="GRAVITY", t0=0, tf=0, value=2.0, entries=[]),
+                          dict(type="PRESSURE INST", t0=0.25, tf=0.25, value=-1e3, entries=[[7, 6]])]
+    # no loads
     inpgen.write_inp(p, m, [dict(rho=1000.0, E=1e6, nu=0.3)], [dict(scale=0.5, t0=0.0, tf=0.01, outputs=1, bc=bc)])
     rc, j, err = check(p)
-    assert rc == 0 and j["loads"] == dict(gravity=None, gravity_pos=-1, force=[])
+    assert rc == 0 and j["loads"] == []
 
 
 def test_coordinates_round_trip(tmp_path):

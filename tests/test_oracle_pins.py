@@ -625,3 +625,165 @@ def test_free_fall_and_momentum_under_nodal_forces():
         imp += dt * 0.5 * ((Ftot(tprev) if k > 0 else 0.0) + Ftot(t))
         tprev = t
     np.testing.assert_allclose(mom, imp, rtol=1e-9)
+
+
+# ---------------------------------------------------------------------------
+# Pressure loads (BCsPressureRamp / BCsPressureInst, boundaryConditions.cpp:370-496)
+# ---------------------------------------------------------------------------
+
+def _boundary_faces(m):
+    """(element, face S) of every exterior face of a box mesh, from the getSubElement node lists."""
+    nen = m["nen"]
+    conn = m["conn"].reshape(-1, nen)
+    faces = [[0, 1, 2, 3], [4, 7, 6, 5], [0, 4, 5, 1], [1, 5, 6, 2], [2, 6, 7, 3], [3, 7, 4, 0]] if nen == 8 else \
+        [[0, 1, 2], [0, 3, 1], [1, 3, 2], [0, 2, 3]]
+    count = {}
+    for e in range(conn.shape[0]):
+        for S, f in enumerate(faces):
+            key = tuple(sorted(conn[e, f]))
+            count.setdefault(key, []).append((e, S + 1))
+    out = [v[0] for v in count.values() if len(v) == 1]
+    return np.array([e for e, _ in out], np.int32), np.array([s for _, s in out], np.int32)
+
+
+def test_pressure_face_geometry():
+    """Surface GPs: getn0 is the unit normal pointing into the element; sum w|J_s| is the face area; the face
+    shape is a partition of unity that reproduces the GP position; for a box the bulk shape at the point
+    (defineShapeOnAPoint's corner-parallelepiped inverse map) equals the face shape on the face's nodes."""
+    for tet in (False, True):
+        m = box(2, 1, 1, lx=2.0, ly=1.5, lz=0.8, tet=tet) if not tet else perturb(box(2, 2, 2, tet=True), 0.1, 3)
+        p = O.OracleProblem(m, n_steps=1)
+        nen = m["nen"]
+        X = m["X"].reshape(3, -1)
+        conn = m["conn"].reshape(-1, nen)
+        faces = [[0, 1, 2, 3], [4, 7, 6, 5], [0, 4, 5, 1], [1, 5, 6, 2], [2, 6, 7, 3], [3, 7, 4, 0]] if not tet else \
+            [[0, 1, 2], [0, 3, 1], [1, 3, 2], [0, 2, 3]]
+        for e in range(0, conn.shape[0], 3):
+            xe = X[:, conn[e]].T
+            for S in range(1, len(faces) + 1):
+                g = p.pressure_face(e, S)
+                xf = xe[faces[S - 1]]
+                nf = len(faces[S - 1])
+                if nf == 4:
+                    area = 0.5 * np.linalg.norm(np.cross(xf[2] - xf[0], xf[3] - xf[1]))
+                else:
+                    area = 0.5 * np.linalg.norm(np.cross(xf[1] - xf[0], xf[2] - xf[0]))
+                assert g["ws"].sum() == pytest.approx(area, rel=1e-8 if nf == 4 else 1e-12)
+                inward = xe.mean(axis=0) - xf.mean(axis=0)
+                for k in range(len(g["ws"])):
+                    n0 = g["n0"][k]
+                    assert np.linalg.norm(n0) == pytest.approx(1.0, abs=1e-15)
+                    assert n0 @ inward > 0
+                    assert abs(n0 @ (xf[1] - xf[0])) < 1e-12 and abs(n0 @ (xf[2] - xf[0])) < 1e-12
+                    ph = g["phi_s"][k][:nf]
+                    assert ph.sum() == pytest.approx(1.0, abs=1e-15)
+                    np.testing.assert_allclose(ph @ xf, g["xyz"][k], rtol=0, atol=1e-14)
+                    if not tet:  # box: exact inverse map
+                        pb = g["phi_b"][k]
+                        np.testing.assert_allclose(pb[faces[S - 1]], ph, rtol=0, atol=1e-8)
+                        np.testing.assert_allclose(np.delete(pb, faces[S - 1]), 0.0, atol=1e-8)
+                        np.testing.assert_allclose(pb @ xe, g["xyz"][k], rtol=0, atol=1e-8)
+                    else:  # the tet's shape at the natural origin: all weight on node 0
+                        assert np.array_equal(g["phi_b"][k][:4], [1.0, 0.0, 0.0, 0.0])
+                    # bulk gradients reproduce the element geometry: sum_a x_a (x) dphi_a = I
+                    np.testing.assert_allclose(xe.T @ g["dphi_b"][k][:nen], np.eye(3), rtol=0, atol=1e-8)
+
+
+@pytest.mark.parametrize("tet", [False, True])
+def test_pressure_undeformed_totals_and_time_function(tet):
+    """At u = 0, a pressure block on faces f gives sum_f P(t) A_f n0_f in total (inward normals: a positive P
+    pushes into the body); P(t) = P/(tf-t0) (t-t0) inside the window for RAMP, P for INST, 0 outside.  INST
+    spreads each face's force by the face shape (A/4 per node on a flat quad, A/3 on a triangle); RAMP uses
+    the bulk shape at the point -- the same on a box, but for tets all on the element's first node."""
+    m = box(2, 2, 2, tet=tet)
+    N = m["n_nodes"]
+    el, fc = _boundary_faces(m)
+    top = [i for i in range(el.size) if p_face_z(m, el[i], fc[i]) == 1.0]
+    el, fc = el[top], fc[top]
+    conn = m["conn"].reshape(-1, m["nen"])
+    P, t0, tf = 2.5e3, 0.2, 0.6
+    for kind in ("pressure_ramp", "pressure_inst"):
+        p = O.OracleProblem(m, n_steps=1, loads=[(kind, P, t0, tf, el, fc)])
+        for t in (0.1, 0.2, 0.35, 0.6, 0.7):
+            f = p.external_force(t).reshape(3, N)
+            Pt = (P / (tf - t0) * (t - t0) if kind == "pressure_ramp" else P) if t0 <= t <= tf else 0.0
+            np.testing.assert_allclose(f.sum(axis=1), [0, 0, -Pt * 1.0], rtol=1e-12, atol=1e-12 * P)
+            if Pt == 0.0:
+                assert np.all(f == 0.0)
+                continue
+            loaded = np.nonzero(np.abs(f[2]) > 1e-9 * Pt)[0]  # the inverse map leaves rounding-level weights
+            if kind == "pressure_inst" or not tet:
+                X = m["X"].reshape(3, -1)
+                assert set(loaded) == set(np.nonzero(np.abs(X[2] - 1.0) < 1e-12)[0])
+                if not tet:  # quads of area 1/4: corner nodes get Pt/16, edge nodes 2x, interior 4x
+                    counts = np.round(-f[2, loaded] / (Pt * 0.25 / 4)).astype(int)
+                    assert sorted(set(counts)) == [1, 2, 4]
+                    np.testing.assert_allclose(-f[2, loaded], counts * Pt / 16, rtol=1e-8)
+            else:
+                assert set(loaded) <= set(conn[el, 0])
+
+
+def p_face_z(m, e, S):
+    """z of the face's first node when the face is horizontal, else None."""
+    nen = m["nen"]
+    faces = [[0, 1, 2, 3], [4, 7, 6, 5], [0, 4, 5, 1], [1, 5, 6, 2], [2, 6, 7, 3], [3, 7, 4, 0]] if nen == 8 else \
+        [[0, 1, 2], [0, 3, 1], [1, 3, 2], [0, 2, 3]]
+    z = m["X"].reshape(3, -1)[2][m["conn"].reshape(-1, nen)[e, faces[S - 1]]]
+    return z[0] if np.ptp(z) < 1e-12 else None
+
+
+@pytest.mark.parametrize("tet", [False, True])
+def test_pressure_follower_load_nanson(tet):
+    """Under a homogeneous deformation u = (G - I) X the bulk F at every surface point is G, so each face carries
+    P det(G) G^-T n0 A (Nanson's formula) and a closed surface carries none; a rigid rotation turns the load
+    with the body."""
+    m = perturb(box(2, 2, 2, tet=tet), 0.08, 9)
+    X = m["X"].reshape(3, -1)
+    N = m["n_nodes"]
+    el, fc = _boundary_faces(m)
+    G = np.array([[1.1, 0.05, -0.02], [0.03, 0.92, 0.04], [-0.01, 0.06, 1.05]])
+    u = ((G - np.eye(3)) @ X).ravel()
+    P = 1.7e3
+    p = O.OracleProblem(m, n_steps=1, loads=[("pressure_inst", P, 0.0, 1.0, el, fc)])
+    f = p.external_force(0.5, u).reshape(3, N)
+    np.testing.assert_allclose(f.sum(axis=1), 0.0, atol=1e-10 * P)
+    # one face: compare with the closed form
+    e, S = int(el[0]), int(fc[0])
+    g = p.pressure_face(e, S)
+    one = O.OracleProblem(m, n_steps=1, loads=[("pressure_inst", P, 0.0, 1.0, [e], [S])])
+    f1 = one.external_force(0.5, u).reshape(3, N)
+    cof = np.linalg.det(G) * np.linalg.inv(G).T
+    expect = sum(P * w * (cof @ n0) for w, n0 in zip(g["ws"], g["n0"]))
+    np.testing.assert_allclose(f1.sum(axis=1), expect, rtol=1e-10)
+    # rigid rotation about z by 30 degrees: the face force rotates with the body
+    c, s_ = np.cos(np.pi / 6), np.sin(np.pi / 6)
+    R = np.array([[c, -s_, 0], [s_, c, 0], [0, 0, 1.0]])
+    ur = ((R - np.eye(3)) @ X).ravel()
+    fr = one.external_force(0.5, ur).reshape(3, N)
+    f0 = one.external_force(0.5).reshape(3, N)
+    np.testing.assert_allclose(fr.sum(axis=1), R @ f0.sum(axis=1), rtol=1e-10, atol=1e-10 * P)
+
+
+def test_pressure_blocks_in_input_order_and_dynamics():
+    """Blocks add in input order (a pressure block before or after a force block on the same node changes only
+    the rounding), and a cube whose faces all carry the same inward pressure contracts symmetrically: the
+    centre node stays at rest and the volume shrinks."""
+    m = box(2, 2, 2)
+    N = m["n_nodes"]
+    el, fc = _boundary_faces(m)
+    pr = ("pressure_inst", 1e4, 0.0, 1.0, el, fc)
+    fo = ("force", [26], [[0.0, 0.0, 5.0]], 0.0, 1.0)
+    a = O.OracleProblem(m, n_steps=1, loads=[pr, fo]).external_force(0.3)
+    b = O.OracleProblem(m, n_steps=1, loads=[fo, pr]).external_force(0.3)
+    np.testing.assert_allclose(a, b, rtol=1e-15, atol=1e-12)
+    r = O.OracleProblem(m, n_steps=40, bc_mode=O.BC_VELOCITY, loads=[pr], energy_every=40).run(hist_cap=40)
+    u = r["u"].reshape(3, N)
+    X = m["X"].reshape(3, -1)
+    centre = int(np.nonzero(np.all(np.abs(X - 0.5) < 1e-12, axis=0))[0][0])
+    assert np.abs(u[:, centre]).max() < 1e-12 * np.abs(u).max()
+    assert r["hist"]["volume"][39] < 1.0
+    # inward motion of every corner, symmetric in x, y, z
+    corners = np.nonzero(np.all((np.abs(X) < 1e-12) | (np.abs(X - 1) < 1e-12), axis=0))[0]
+    d = (u[:, corners] * np.sign(0.5 - X[:, corners]))
+    assert np.all(d > 0)
+    np.testing.assert_allclose(d, d[0, 0], rtol=1e-9)

@@ -19,6 +19,7 @@
 
 #include "../../include/mps_gpu.h"
 #include "mps_api_internal.h"
+#include "mps_device.h"
 #include "mps_kernels.h"
 
 namespace {
@@ -522,86 +523,365 @@ extern "C" int mps_set_state(mps_handle h, const double* u, const double* v, con
     return MPS_OK;
 }
 
-// Nodal force entries are sorted by node (stable, so each node keeps its entries in input order) and
-// flagged in ninfo; gravity is summed per node once, on the device.
-extern "C" int mps_set_loads(mps_handle h, const mps_loads* L) {
+// ---------------------------------------------------------------------------
+// Neumann loads: host set-up
+// ---------------------------------------------------------------------------
+namespace {
+
+// Faces of getSubElement: hex (classLinearHexahedra.cpp:30-85), tet (classLinearTetrahedra.cpp:30-55).
+const int kHexFace[6][4] = {{0, 1, 2, 3}, {4, 7, 6, 5}, {0, 4, 5, 1}, {1, 5, 6, 2}, {2, 6, 7, 3}, {3, 7, 4, 0}};
+const int kTetFace[4][3] = {{0, 1, 2}, {0, 3, 1}, {1, 3, 2}, {0, 2, 3}};
+
+// multGeneralMatrices: C[m x ll] = A[m x kk] B[kk x ll], k innermost from zero (maths.cpp:1679-1693).
+void mat_general(const double* A, int m, int kk, const double* B, int ll, double* C) {
+    for (int i = 0; i < m * ll; i++) C[i] = 0.0;
+    for (int i = 0; i < m; i++)
+        for (int j = 0; j < ll; j++)
+            for (int k = 0; k < kk; k++) C[i * ll + j] += A[i * kk + k] * B[k * ll + j];
+}
+
+// inverse, 3x3 branch (maths.cpp:2749-2767).
+void inverse3(const double* A, double* R) {
+    const double idet = 1.0 / mpsd::det3_strict(A);
+    R[0] = idet * (A[4] * A[8] - A[5] * A[7]);
+    R[3] = -idet * (A[3] * A[8] - A[5] * A[6]);
+    R[6] = idet * (A[3] * A[7] - A[4] * A[6]);
+    R[1] = -idet * (A[1] * A[8] - A[2] * A[7]);
+    R[4] = idet * (A[0] * A[8] - A[2] * A[6]);
+    R[7] = -idet * (A[0] * A[7] - A[1] * A[6]);
+    R[2] = idet * (A[1] * A[5] - A[2] * A[4]);
+    R[5] = -idet * (A[0] * A[5] - A[2] * A[3]);
+    R[8] = idet * (A[0] * A[4] - A[1] * A[3]);
+}
+
+double norm2_3(const double* a) {  // maths.cpp:1550-1557
+    double t = 0;
+    for (int i = 0; i < 3; i++) t += a[i] * a[i];
+    return std::sqrt(t);
+}
+
+void cross3(const double* A, const double* B, double* C) {  // geometry.cpp:180-190
+    C[0] = A[1] * B[2] - A[2] * B[1];
+    C[1] = A[2] * B[0] - A[0] * B[2];
+    C[2] = A[0] * B[1] - A[1] * B[0];
+}
+
+// Surface Gauss points of face S (1-based) of bulk element e, as the reference builds a pressure face:
+//   getn0 of the face (classLinearSquare.cpp:19-45, classLinearTriangles.cpp:19-45);
+//   the face's defineGPs in the plane basis of createOrthogonalBasisOnPlane (geometry.cpp:363-389):
+//     quad 2x2 points +-0.577350269, weight 1 (classLinearSquare.cpp:196-301, GPsDistribution.cpp:297-318);
+//     triangle one point (0.33333333333333, 0.33333333333333), weight 0.5*1 (classLinearTriangles.cpp:194-299);
+//   defineShapeOnAPoint of the bulk element at the point (classLinearHexahedra.cpp:95-172: natural coordinates
+//   2 A^-1 (x - x_0) - 1, A the corner edges 0-1, 0-3, 0-4; classLinearTetrahedra.cpp:66-117: the shape at
+//   the natural origin) for F and, with PRESSURE RAMP, the scatter.
+// Per point: out[MPS_SGP_STRIDE] = bulk dphi (8x3), n0, w|J_s|, scatter phi (8); nodes[8] scatter nodes (-1
+// where unused).  Returns the number of points, or -1 for a face outside the element.
+int surface_points(const std::vector<double>& X, int64_t N, const int32_t* cn, int nen, int S, bool ramp, double* out,
+                   int32_t* nodes) {
+    const int nf = (nen == 8) ? 4 : 3;
+    if (S < 1 || S > ((nen == 8) ? 6 : 4)) return -1;
+    const int* fl = (nen == 8) ? kHexFace[S - 1] : kTetFace[S - 1];
+    double xb[8][3], xf[4][3];
+    for (int a = 0; a < nen; a++)
+        for (int d = 0; d < 3; d++) xb[a][d] = X[(size_t)d * N + cn[a]];
+    for (int k = 0; k < nf; k++)
+        for (int d = 0; d < 3; d++) xf[k][d] = xb[fl[k]][d];
+    double nn[3];
+    for (int i = 0; i < 3; i++) {
+        const int c1 = (i == 0) ? 1 : (i == 1) ? 2 : 0, c2 = (i == 0) ? 2 : (i == 1) ? 0 : 1;
+        nn[i] = (xf[1][c1] - xf[0][c1]) * (xf[2][c2] - xf[0][c2]) - (xf[2][c1] - xf[0][c1]) * (xf[1][c2] - xf[0][c2]);
+    }
+    const double no = norm2_3(nn);
+    double a[3], b[3], normal[3], xhat[3], map[6];
+    for (int i = 0; i < 3; i++) {
+        a[i] = xf[1][i] - xf[0][i];
+        b[i] = xf[2][i] - xf[0][i];
+    }
+    cross3(b, a, normal);
+    cross3(a, normal, xhat);
+    const double magA = norm2_3(a), magX = norm2_3(xhat);
+    for (int i = 0; i < 3; i++) {
+        map[3 + i] = a[i] / magA;
+        map[i] = xhat[i] / magX;
+    }
+    double tmp[12], loc[8];
+    for (int k = 0; k < nf; k++)
+        for (int d = 0; d < 3; d++) tmp[d * nf + k] = xf[k][d] - xf[0][d];
+    mat_general(map, 2, 3, tmp, nf, loc);
+    const double G = 0.577350269;
+    const double sq[4][2] = {{G, G}, {-G, G}, {G, -G}, {-G, -G}};
+    const int ng = (nen == 8) ? 4 : 1;
+    double xyzCoor[24];
+    for (int k = 0; k < nen; k++)
+        for (int d = 0; d < 3; d++) xyzCoor[d * nen + k] = xb[k][d];
+    for (int g = 0; g < ng; g++) {
+        double ph[4] = {0, 0, 0, 0}, nd[8], Jac[4], weight;
+        if (nen == 8) {
+            const double x = sq[g][0], y = sq[g][1];
+            ph[0] = 0.25 * (1 - x) * (1 - y);
+            ph[1] = 0.25 * (1 + x) * (1 - y);
+            ph[2] = 0.25 * (1 + x) * (1 + y);
+            ph[3] = 0.25 * (1 - x) * (1 + y);
+            nd[0] = -0.25 * (1 - y);
+            nd[2] = 0.25 * (1 - y);
+            nd[4] = 0.25 * (1 + y);
+            nd[6] = -0.25 * (1 + y);
+            nd[1] = -0.25 * (1 - x);
+            nd[3] = -0.25 * (1 + x);
+            nd[5] = 0.25 * (1 + x);
+            nd[7] = 0.25 * (1 - x);
+            weight = 1 * 1.0;
+        } else {
+            const double x = 0.33333333333333, y = 0.33333333333333;
+            ph[0] = -1.0 * x - 1.0 * y + 1.0;
+            ph[1] = 1.0 * x;
+            ph[2] = 1.0 * y;
+            nd[0] = -1.0 - 0.0 + 0.0;
+            nd[2] = 1.0;
+            nd[4] = 0.0;
+            nd[1] = -0.0 - 1.0 + 0.0;
+            nd[3] = 0.0;
+            nd[5] = 1.0;
+            weight = 0.5 * 1.0;
+        }
+        mat_general(loc, 2, nf, nd, 2, Jac);
+        const double Js = Jac[0] * Jac[3] - Jac[2] * Jac[1];
+        double x[3];
+        for (int i = 0; i < 3; i++) {
+            x[i] = 0;
+            for (int k = 0; k < nf; k++) x[i] += xf[k][i] * ph[k];
+        }
+        // bulk shape at the point
+        double phb[8] = {0, 0, 0, 0, 0, 0, 0, 0}, ndb[24];
+        if (nen == 8) {
+            double A[9], Ai[9], Xr[3], xw[3];
+            const int c[3] = {1, 3, 4};
+            for (int d = 0; d < 3; d++)
+                for (int j = 0; j < 3; j++) A[d * 3 + j] = xb[c[j]][d] - xb[0][d];
+            for (int d = 0; d < 3; d++) Xr[d] = x[d] - xb[0][d];
+            inverse3(A, Ai);
+            for (int i = 0; i < 3; i++) {
+                double r = 0;
+                for (int j = 0; j < 3; j++) r += Ai[i * 3 + j] * Xr[j];
+                xw[i] = 2 * r - 1;
+            }
+            mpsd::hex_shape_ref(xw[0], xw[1], xw[2], phb, ndb);
+        } else {  // classLinearTetrahedra::getInterpolationParameters at (0, 0, 0)
+            const double q = 0, r = 0, w = 0;
+            phb[0] = -1.0 * q - 1.0 * r - 1.0 * w + 1.0;
+            phb[1] = 1.0 * q;
+            phb[2] = 1.0 * r;
+            phb[3] = 1.0 * w;
+            for (int k = 0; k < 12; k++) ndb[k] = 0.0;
+            ndb[0] = -1.0 - 0.0 - 0.0 + 0.0;
+            ndb[3] = 1.0;
+            ndb[1] = -0.0 - 1.0 - 0.0 + 0.0;
+            ndb[7] = 1.0;
+            ndb[2] = -0.0 - 0.0 - 1.0 + 0.0;
+            ndb[11] = 1.0;
+        }
+        double Jb[9], iJb[9];
+        mat_general(xyzCoor, 3, nen, ndb, 3, Jb);
+        inverse3(Jb, iJb);
+        double* o = out + (size_t)g * MPS_SGP_STRIDE;
+        mat_general(ndb, nen, 3, iJb, 3, o);  // dphi = natDev * J^-1
+        for (int k = 3 * nen; k < 24; k++) o[k] = 0.0;
+        for (int i = 0; i < 3; i++) o[24 + i] = nn[i] / no;
+        o[27] = weight * std::fabs(Js);
+        for (int k = 0; k < 8; k++) {
+            o[28 + k] = ramp ? phb[k] : (k < nf ? ph[k] : 0.0);
+            nodes[g * 8 + k] = ramp ? (k < nen ? cn[k] : -1) : (k < nf ? cn[fl[k]] : -1);
+        }
+    }
+    return ng;
+}
+
+struct LoadItem {
+    int32_t node, block;
+    int64_t seq;
+    uint32_t code;
+};
+
+}  // namespace
+
+// Blocks in input order become per-node load items: each FORCE entry, each (pressure Gauss point, scatter
+// node), and gravity for every node that has other items.  Items are sorted by node, each node's in input
+// order, and flagged in ninfo; gravity is summed per node once on the device.
+extern "C" int mps_set_loads(mps_handle h, int32_t n_loads, const mps_load* loads) {
     if (!h) return MPS_ERR_ARG;
     if (h->primed || h->next_step != 1) return h->fail(MPS_ERR_STATE, "mps_set_loads: only valid before the first step");
-    const int64_t nf = L ? L->n_force : 0;
-    const double grav = L ? L->gravity : 0.0;
-    const int64_t N = h->I.N;
-    if (nf < 0) return h->fail(MPS_ERR_ARG, "mps_set_loads: n_force < 0");
-    if (!std::isfinite(grav)) return h->fail(MPS_ERR_ARG, "mps_set_loads: gravity is not finite");
-    if (nf > 0 && (!L->node || !L->force || !L->t0 || !L->tf))
-        return h->fail(MPS_ERR_ARG, "mps_set_loads: n_force > 0 needs node, force, t0 and tf");
-    for (int64_t k = 0; k < nf; k++) {
-        if (L->node[k] < 0 || L->node[k] >= N)
-            return h->fail(MPS_ERR_ARG, fmt("mps_set_loads: force entry %lld: node %d outside [0,%lld)", (long long)k,
-                                            L->node[k], (long long)N));
-        if (!std::isfinite(L->t0[k]) || !std::isfinite(L->tf[k]) || !(L->tf[k] > L->t0[k]))
-            return h->fail(MPS_ERR_ARG, fmt("mps_set_loads: force entry %lld: window needs finite tf > t0", (long long)k));
-        for (int d = 0; d < 3; d++)
-            if (!std::isfinite(L->force[3 * k + d]))
-                return h->fail(MPS_ERR_ARG, fmt("mps_set_loads: force entry %lld is not finite", (long long)k));
-    }
-    (void)hipSetDevice(h->device);
+    if (n_loads < 0 || (n_loads > 0 && !loads)) return h->fail(MPS_ERR_ARG, "mps_set_loads: bad block list");
     MpsInst& I = h->I;
-    // host copy of the node flags: clear the loaded bit, set it on the nodes with entries
+    const int64_t N = I.N, Ne = I.Ne;
+    const int nen = I.nen;
+    int n_grav = 0, grav_block = -1;
+    double grav = 0.0;
+    int64_t n_force = 0, n_face = 0;
+    for (int b = 0; b < n_loads; b++) {
+        const mps_load& L = loads[b];
+        const std::string at = fmt("mps_set_loads: block %d: ", b);
+        if (L.n < 0) return h->fail(MPS_ERR_ARG, at + "n < 0");
+        if (L.type == MPS_LOAD_FORCE) {
+            if (L.n > 0 && (!L.index || !L.force)) return h->fail(MPS_ERR_ARG, at + "FORCE needs index and force");
+            if (!std::isfinite(L.t0) || !std::isfinite(L.tf) || !(L.tf > L.t0))
+                return h->fail(MPS_ERR_ARG, at + "FORCE window needs finite tf > t0");
+            for (int k = 0; k < L.n; k++) {
+                if (L.index[k] < 0 || L.index[k] >= N)
+                    return h->fail(MPS_ERR_ARG, at + fmt("node %d outside [0,%lld)", L.index[k], (long long)N));
+                for (int d = 0; d < 3; d++)
+                    if (!std::isfinite(L.force[3 * (size_t)k + d])) return h->fail(MPS_ERR_ARG, at + "force is not finite");
+            }
+            n_force += L.n;
+        } else if (L.type == MPS_LOAD_GRAVITY) {
+            if (!std::isfinite(L.value)) return h->fail(MPS_ERR_ARG, at + "gravity is not finite");
+            if (++n_grav > 1) return h->fail(MPS_ERR_ARG, at + "a second GRAVITY block: one gravity load per solve");
+            grav = L.value;
+            grav_block = b;
+        } else if (L.type == MPS_LOAD_PRESSURE_RAMP || L.type == MPS_LOAD_PRESSURE_INST) {
+            const bool ramp = L.type == MPS_LOAD_PRESSURE_RAMP;
+            if (L.n > 0 && (!L.index || !L.face)) return h->fail(MPS_ERR_ARG, at + "PRESSURE needs index and face");
+            if (!std::isfinite(L.value) || !std::isfinite(L.t0) || !std::isfinite(L.tf) ||
+                !(ramp ? L.tf > L.t0 : L.tf >= L.t0))
+                return h->fail(MPS_ERR_ARG, at + "PRESSURE needs a finite P and window (tf > t0 for a ramp)");
+            for (int k = 0; k < L.n; k++) {
+                if (L.index[k] < 0 || L.index[k] >= Ne)
+                    return h->fail(MPS_ERR_ARG, at + fmt("element %d outside [0,%lld)", L.index[k], (long long)Ne));
+                if (L.face[k] < 1 || L.face[k] > (nen == 8 ? 6 : 4))
+                    return h->fail(MPS_ERR_ARG, at + fmt("face %d outside 1..%d", L.face[k], nen == 8 ? 6 : 4));
+            }
+            n_face += L.n;
+        } else {
+            return h->fail(MPS_ERR_ARG, at + fmt("unknown type %d", L.type));
+        }
+    }
+    const int per_face = (nen == 8) ? 4 : 1;
+    const int64_t n_sgp = n_face * per_face;
+    if (n_force >= (int64_t)MPS_ITEM_INDEX || n_sgp * 8 >= (int64_t)MPS_ITEM_INDEX)
+        return h->fail(MPS_ERR_ARG, "mps_set_loads: too many load entries");
+    (void)hipSetDevice(h->device);
+    // reference geometry back from the device (the handle keeps no host copy)
+    std::vector<double> X(3 * (size_t)N);
+    std::vector<int32_t> conn((size_t)nen * Ne);
     std::vector<uint32_t> ninfo((size_t)N);
+    HCHK(h, hipMemcpy(X.data(), I.X, X.size() * 8, hipMemcpyDeviceToHost));
+    HCHK(h, hipMemcpy(conn.data(), I.conn, conn.size() * 4, hipMemcpyDeviceToHost));
     HCHK(h, hipMemcpy(ninfo.data(), I.ninfo, (size_t)N * 4, hipMemcpyDeviceToHost));
     for (uint32_t& f : ninfo) f &= MPS_REC_MASK;
-    std::vector<int32_t> order((size_t)nf);
-    for (int64_t k = 0; k < nf; k++) order[(size_t)k] = (int32_t)k;
-    std::stable_sort(order.begin(), order.end(), [&](int32_t x, int32_t y) { return L->node[x] < L->node[y]; });
-    std::vector<int32_t> lnode((size_t)nf), lrec((size_t)nf);
-    std::vector<double> lm(3 * (size_t)nf), lwin(2 * (size_t)nf);
-    for (size_t j = 0; j < (size_t)nf; j++) {
-        const int32_t k = order[j];
-        lnode[j] = L->node[k];
-        lrec[j] = k;
-        for (int d = 0; d < 3; d++) lm[3 * j + d] = L->force[3 * (size_t)k + d] / (L->tf[k] - L->t0[k]);  // classNodes.cpp:93-95
-        lwin[2 * j] = L->t0[k];
-        lwin[2 * j + 1] = L->tf[k];
-        ninfo[(size_t)lnode[j]] |= MPS_NODE_LOADED;
+    std::vector<LoadItem> items;
+    std::vector<double> lm, lwin, sgd, sblk(4 * (size_t)std::max(n_loads, 1), 0.0);
+    std::vector<int32_t> sge, sgb;
+    int64_t seq = 0;
+    for (int b = 0; b < n_loads; b++) {
+        const mps_load& L = loads[b];
+        sblk[4 * (size_t)b] = (double)L.type;
+        sblk[4 * (size_t)b + 1] = L.value;
+        sblk[4 * (size_t)b + 2] = L.t0;
+        sblk[4 * (size_t)b + 3] = L.tf;
+        if (L.type == MPS_LOAD_FORCE) {
+            for (int k = 0; k < L.n; k++) {
+                const size_t j = lwin.size() / 2;
+                for (int d = 0; d < 3; d++) lm.push_back(L.force[3 * (size_t)k + d] / (L.tf - L.t0));  // classNodes.cpp:93-95
+                lwin.push_back(L.t0);
+                lwin.push_back(L.tf);
+                items.push_back({L.index[k], b, seq++, (MPS_ITEM_FORCE << 30) | (uint32_t)j});
+            }
+        } else if (L.type != MPS_LOAD_GRAVITY) {
+            const bool ramp = L.type == MPS_LOAD_PRESSURE_RAMP;
+            for (int k = 0; k < L.n; k++) {
+                double pts[4 * MPS_SGP_STRIDE];
+                int32_t nodes[32];
+                const int ng = surface_points(X, N, conn.data() + (size_t)L.index[k] * nen, nen, L.face[k], ramp, pts, nodes);
+                for (int g = 0; g < ng; g++) {
+                    const size_t q = sge.size();
+                    sge.push_back(L.index[k]);
+                    sgb.push_back(b);
+                    sgd.insert(sgd.end(), pts + (size_t)g * MPS_SGP_STRIDE, pts + (size_t)(g + 1) * MPS_SGP_STRIDE);
+                    for (int a = 0; a < 8; a++)
+                        if (nodes[g * 8 + a] >= 0)
+                            items.push_back({nodes[g * 8 + a], b, seq++, (MPS_ITEM_PRESSURE << 30) | (uint32_t)(q * 8 + a)});
+                }
+            }
+        }
+    }
+    if (grav_block >= 0) {  // gravity at its place among the other items of every loaded node
+        std::vector<char> has((size_t)N, 0);
+        for (const LoadItem& it : items) has[(size_t)it.node] = 1;
+        for (int64_t n = 0; n < N; n++)
+            if (has[(size_t)n]) items.push_back({(int32_t)n, grav_block, -1, MPS_ITEM_GRAVITY << 30});
+    }
+    std::sort(items.begin(), items.end(), [](const LoadItem& x, const LoadItem& y) {
+        if (x.node != y.node) return x.node < y.node;
+        if (x.block != y.block) return x.block < y.block;
+        return x.seq < y.seq;
+    });
+    // flagged nodes in ascending order, each with its items (CSR)
+    std::vector<int32_t> lidx((size_t)N, -1), lnn, lrow(1, 0), lcode(items.size());
+    for (size_t j = 0; j < items.size(); j++) {
+        const int32_t n = items[j].node;
+        if (lnn.empty() || lnn.back() != n) {
+            lidx[(size_t)n] = (int32_t)lnn.size();
+            lnn.push_back(n);
+            lrow.push_back(lrow.back());
+            ninfo[(size_t)n] |= MPS_NODE_LOADED;
+        }
+        lcode[j] = (int32_t)items[j].code;
+        lrow.back()++;
     }
     HCHK(h, hipStreamSynchronize(h->stream));
-    for (void* p : h->load_allocs) (void)hipFree(p);
+    for (void* q : h->load_allocs) (void)hipFree(q);
     h->load_allocs.clear();
-    I.n_load = 0;
-    I.lnode = I.lrec = nullptr;
-    I.lm = I.lwin = I.fgrav = nullptr;
-    auto lalloc = [&](size_t bytes, void** p) -> hipError_t {
-        hipError_t e = hipMalloc(p, bytes < 256 ? 256 : bytes);
-        if (e == hipSuccess) h->load_allocs.push_back(*p);
-        return e;
+    I.n_lnode = 0;
+    I.n_sgp = 0;
+    I.lidx = I.lnn = I.lrow = I.lcode = I.sge = I.sgb = nullptr;
+    I.lm = I.lwin = I.fgrav = I.sgd = I.sblk = nullptr;
+    I.pc = I.lf = nullptr;
+    auto upload = [&](const void* src, size_t bytes, void** dst) -> hipError_t {
+        hipError_t e = hipMalloc(dst, bytes < 256 ? 256 : bytes);
+        if (e != hipSuccess) return e;
+        h->load_allocs.push_back(*dst);
+        return src ? hipMemcpy(*dst, src, bytes, hipMemcpyHostToDevice) : hipSuccess;
     };
-    if (nf > 0) {
-        void *pn, *pr, *pm, *pw;
-        HCHK(h, lalloc((size_t)nf * 4, &pn));
-        HCHK(h, lalloc((size_t)nf * 4, &pr));
-        HCHK(h, lalloc((size_t)nf * 24, &pm));
-        HCHK(h, lalloc((size_t)nf * 16, &pw));
-        HCHK(h, hipMemcpy(pn, lnode.data(), (size_t)nf * 4, hipMemcpyHostToDevice));
-        HCHK(h, hipMemcpy(pr, lrec.data(), (size_t)nf * 4, hipMemcpyHostToDevice));
-        HCHK(h, hipMemcpy(pm, lm.data(), (size_t)nf * 24, hipMemcpyHostToDevice));
-        HCHK(h, hipMemcpy(pw, lwin.data(), (size_t)nf * 16, hipMemcpyHostToDevice));
-        I.lnode = (const int32_t*)pn;
-        I.lrec = (const int32_t*)pr;
-        I.lm = (const double*)pm;
-        I.lwin = (const double*)pw;
-        I.n_load = (int32_t)nf;
+    void* p;
+    if (!items.empty()) {
+        HCHK(h, upload(lidx.data(), lidx.size() * 4, &p));
+        I.lidx = (const int32_t*)p;
+        HCHK(h, upload(lnn.data(), lnn.size() * 4, &p));
+        I.lnn = (const int32_t*)p;
+        HCHK(h, upload(lrow.data(), lrow.size() * 4, &p));
+        I.lrow = (const int32_t*)p;
+        HCHK(h, upload(lcode.data(), lcode.size() * 4, &p));
+        I.lcode = (const int32_t*)p;
+        HCHK(h, upload(nullptr, lnn.size() * 24, &p));
+        I.lf = (double*)p;
+        I.n_lnode = (int32_t)lnn.size();
     }
-    const int32_t gp = L ? L->gravity_pos : -1;
-    I.grav_pos = (gp >= 0 && gp <= nf) ? gp : (int32_t)nf;
+    if (!lm.empty()) {
+        HCHK(h, upload(lm.data(), lm.size() * 8, &p));
+        I.lm = (const double*)p;
+        HCHK(h, upload(lwin.data(), lwin.size() * 8, &p));
+        I.lwin = (const double*)p;
+    }
+    if (!sge.empty()) {
+        HCHK(h, upload(sge.data(), sge.size() * 4, &p));
+        I.sge = (const int32_t*)p;
+        HCHK(h, upload(sgb.data(), sgb.size() * 4, &p));
+        I.sgb = (const int32_t*)p;
+        HCHK(h, upload(sgd.data(), sgd.size() * 8, &p));
+        I.sgd = (const double*)p;
+        HCHK(h, upload(sblk.data(), sblk.size() * 8, &p));
+        I.sblk = (const double*)p;
+        HCHK(h, upload(nullptr, sge.size() * 24 * 8, &p));
+        I.pc = (double*)p;
+        I.n_sgp = (int32_t)sge.size();
+    }
     HCHK(h, hipMemcpy((void*)I.ninfo, ninfo.data(), (size_t)N * 4, hipMemcpyHostToDevice));
-    if (grav != 0.0) {
-        void* pg;
-        HCHK(h, lalloc((size_t)N * 8, &pg));
+    if (grav_block >= 0 && grav != 0.0) {
+        HCHK(h, upload(nullptr, (size_t)N * 8, &p));
         // the partial-force buffer (>= nen*Ne doubles) is free until the first step
-        hipError_t e = mpsk::launch_gravity(I, grav, I.part, (double*)pg, h->stream);
+        hipError_t e = mpsk::launch_gravity(I, grav, I.part, (double*)p, h->stream);
         if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
         if (e != hipSuccess) return h->fail(MPS_ERR_HIP, fmt("gravity load: %s", hipGetErrorString(e)));
-        I.fgrav = (const double*)pg;
+        I.fgrav = (const double*)p;
     }
     return MPS_OK;
 }
@@ -719,6 +999,7 @@ static hipError_t enqueue_steps(mps_solver_s* h, long long count, std::vector<in
             t.done();
             if (t.a) kinds->push_back(0);
         }
+        if (e == hipSuccess && (I.n_sgp > 0 || I.n_lnode > 0)) e = mpsk::launch_loads(I, h->stream);
         if (e != hipSuccess) break;
         const bool want_e = every > 0 && (k % every) == 0;
         const bool last = (i == count - 1);

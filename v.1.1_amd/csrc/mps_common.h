@@ -20,11 +20,17 @@
 //     nrow/ncol     CSR node -> partials, ascending patch (= ascending element)
 //   part            double[3*n_part]       partial nodal forces [(g)*3 + dir]; also the
 //                                          set-up scratch (>= nen*Ne doubles)
-//   loads           n_load nodal force entries sorted by node (stable: input order per node)
-//     lnode, lrec   int32[n_load]  node, position in the input list of force entries
-//     lm            double[3*n_load] F/(tf-t0) (classNeumann, classNodes.cpp:85-96)
-//     lwin          double[2*n_load] t0, tf
+//   loads           the n_lnode nodes with load items (flag bit 31 of ninfo), each with its items in input order:
+//     lidx          int32[N]         compact index of a flagged node (read only for flagged nodes)
+//     lnn, lrow     int32[n_lnode], [n_lnode+1]  node id; CSR into lcode
+//     lcode         int32[items]     kind (bits 30-31: MPS_ITEM_*) | index
+//     lf            double[3*n_lnode]  the step's external force of each flagged node
+//     lm, lwin      double[3*n_f], [2*n_f]  nodal force entry: F/(tf-t0) (classNodes.cpp:85-96); t0, tf
 //     fgrav         double[N]      gravity load along z (NULL: no gravity)
+//     sge, sgb      int32[n_sgp]   pressure surface Gauss point: bulk element, block
+//     sgd           double[MPS_SGP_STRIDE*n_sgp]  bulk dphi (24), n0 (3), w|J_s| (1), scatter phi (8)
+//     sblk          double[4*blocks]  block type, P, t0, tf
+//     pc            double[24*n_sgp]  the step's surface forces [(g*8 + a)*3 + dir]
 //   ctl             MpsCtl       step control (time, dt, go flags)
 //   dtred           u64[2*MPS_DT_SLOTS*MPS_DT_STRIDE]  stable-dt reduction slots by step parity
 #pragma once
@@ -36,6 +42,11 @@
 #define MPS_DT_STRIDE 32  // u64 between two slots (256 B), so the slots' atomics do not share a line
 #define MPS_NODE_LOADED 0x80000000u  // ninfo bit: the node has nodal force entries
 #define MPS_REC_MASK 0x7FFFFFFFu     // ninfo bits below the loaded flag
+#define MPS_ITEM_FORCE 0u            // load item kinds (lcode >> 30)
+#define MPS_ITEM_PRESSURE 1u
+#define MPS_ITEM_GRAVITY 2u
+#define MPS_ITEM_INDEX 0x3FFFFFFFu
+#define MPS_SGP_STRIDE 36            // doubles per pressure surface Gauss point
 
 enum : int32_t { MPS_TIME_WINDOW = 0, MPS_TIME_FIXED = 1 };
 
@@ -96,14 +107,21 @@ struct MpsInst {
     int32_t dof1_node, dof1_dir;
     int32_t n_patch;
     int32_t n_part;
-    int32_t n_load;    // nodal force entries
-    int32_t grav_pos;  // gravity is added before the entries with lrec >= grav_pos
-    int32_t _pad3;
-    const int32_t* lnode;
-    const int32_t* lrec;
+    int32_t n_lnode;  // nodes with load items
+    int32_t n_sgp;    // pressure surface Gauss points
+    const int32_t* lidx;
+    const int32_t* lnn;
+    const int32_t* lrow;
+    const int32_t* lcode;
+    double* lf;
     const double* lm;
     const double* lwin;
     const double* fgrav;
+    const int32_t* sge;
+    const int32_t* sgb;
+    const double* sgd;
+    const double* sblk;
+    double* pc;
 };
 
 // One unit of work for the persistent batched kernels: a contiguous range of

@@ -59,8 +59,47 @@ __device__ __forceinline__ void inv3(const T* A, T* R, T det) {
     R[8] = idet * (A[0] * A[4] - A[1] * A[3]);
 }
 
+// Trilinear shape values and natural derivatives nd[a*3 + d] at (xi, eta, zeta)
+// (classLinearHexahedra::getInterpolationParameters, classLinearHexahedra.cpp:344-381).
+__host__ __device__ inline void hex_shape_ref(double xi, double eta, double zeta, double* phi, double* nd) {
+#pragma clang fp contract(off)
+    const double x = xi, y = eta, z = zeta;
+    phi[0] = 0.125 * (1 - x) * (1 - y) * (1 - z);
+    phi[1] = 0.125 * (1 + x) * (1 - y) * (1 - z);
+    phi[2] = 0.125 * (1 + x) * (1 + y) * (1 - z);
+    phi[3] = 0.125 * (1 - x) * (1 + y) * (1 - z);
+    phi[4] = 0.125 * (1 - x) * (1 - y) * (1 + z);
+    phi[5] = 0.125 * (1 + x) * (1 - y) * (1 + z);
+    phi[6] = 0.125 * (1 + x) * (1 + y) * (1 + z);
+    phi[7] = 0.125 * (1 - x) * (1 + y) * (1 + z);
+    nd[0] = -0.125 * (1 - y) * (1 - z);
+    nd[3] = 0.125 * (1 - y) * (1 - z);
+    nd[6] = 0.125 * (1 + y) * (1 - z);
+    nd[9] = -0.125 * (1 + y) * (1 - z);
+    nd[12] = -0.125 * (1 - y) * (1 + z);
+    nd[15] = 0.125 * (1 - y) * (1 + z);
+    nd[18] = 0.125 * (1 + y) * (1 + z);
+    nd[21] = -0.125 * (1 + y) * (1 + z);
+    nd[1] = -0.125 * (1 - x) * (1 - z);
+    nd[4] = -0.125 * (1 + x) * (1 - z);
+    nd[7] = 0.125 * (1 + x) * (1 - z);
+    nd[10] = 0.125 * (1 - x) * (1 - z);
+    nd[13] = -0.125 * (1 - x) * (1 + z);
+    nd[16] = -0.125 * (1 + x) * (1 + z);
+    nd[19] = 0.125 * (1 + x) * (1 + z);
+    nd[22] = 0.125 * (1 - x) * (1 + z);
+    nd[2] = -0.125 * (1 - x) * (1 - y);
+    nd[5] = -0.125 * (1 + x) * (1 - y);
+    nd[8] = -0.125 * (1 + x) * (1 + y);
+    nd[11] = -0.125 * (1 - x) * (1 + y);
+    nd[14] = 0.125 * (1 - x) * (1 - y);
+    nd[17] = 0.125 * (1 + x) * (1 - y);
+    nd[20] = 0.125 * (1 + x) * (1 + y);
+    nd[23] = 0.125 * (1 - x) * (1 + y);
+}
+
 // determinantTensor with no FMA contraction (maths.cpp:1564-1577), for the set-up path.
-__device__ __forceinline__ double det3_strict(const double* A) {
+__host__ __device__ inline double det3_strict(const double* A) {
 #pragma clang fp contract(off)
     return A[0] * (A[4] * A[8] - A[7] * A[5]) - A[1] * (A[3] * A[8] - A[6] * A[5]) +
            A[2] * (A[3] * A[7] - A[6] * A[4]);
@@ -712,43 +751,101 @@ __device__ __forceinline__ void predict_node(const MpsInst& I, int n, long long 
     }
 }
 
-// External force of node n at time t (NeumannBCManagement, boundaryConditions.cpp:75-140): zero, then
-// the loads in input order -- the node's force entries (BCsNodalForces :1120-1134 with
-// classNeumann::getTracAmount, classNodes.h:101-108: F/(tf-t0) * t for t0 <= t <= tf, else 0) and its
-// gravity load (BCsGravity :1146-1172, constant, summed over the Gauss points at set-up) at the place
-// of *GRAVITY among them.
-__device__ __forceinline__ void node_fext(const MpsInst& I, int n, uint32_t info, double t, double* fe) {
+// External force of flagged node j (lnn[j]) at time t (NeumannBCManagement, boundaryConditions.cpp:75-140):
+// zero, then its load items in input order -- nodal force entries (BCsNodalForces :1120-1134 with
+// classNeumann::getTracAmount, classNodes.h:101-108: F/(tf-t0) * t for t0 <= t <= tf, else 0), the step's
+// surface forces of pressure faces, and its gravity load (BCsGravity :1146-1172, constant, summed over the
+// Gauss points at set-up) at the place of *GRAVITY.
+__device__ __forceinline__ void loaded_node_fext(const MpsInst& I, int j, double t, double* fe) {
 #pragma clang fp contract(off)
     fe[0] = 0.0;
     fe[1] = 0.0;
     fe[2] = 0.0;
-    const double g = I.fgrav ? I.fgrav[n] : 0.0;
-    if (!(info & MPS_NODE_LOADED)) {
-        fe[2] = g;
-        return;
-    }
-    int lo = 0, hi = I.n_load;  // first entry of node n (entries sorted by node)
-    while (lo < hi) {
-        const int mid = (lo + hi) >> 1;
-        if (I.lnode[mid] < n) lo = mid + 1;
-        else hi = mid;
-    }
-    bool gdone = (I.fgrav == nullptr);
-    for (int j = lo; j < I.n_load && I.lnode[j] == n; j++) {
-        if (!gdone && I.lrec[j] >= I.grav_pos) {
-            fe[2] += g;
-            gdone = true;
+    const int n = I.lnn[j];
+    for (int q = I.lrow[j]; q < I.lrow[j + 1]; q++) {
+        const uint32_t code = (uint32_t)I.lcode[q];
+        const uint32_t kind = code >> 30;
+        const size_t idx = code & MPS_ITEM_INDEX;
+        if (kind == MPS_ITEM_FORCE) {
+            const bool in = t >= I.lwin[2 * idx] && t <= I.lwin[2 * idx + 1];
+            for (int d = 0; d < 3; d++) fe[d] += in ? I.lm[3 * idx + d] * t : 0.0;
+        } else if (kind == MPS_ITEM_PRESSURE) {
+            for (int d = 0; d < 3; d++) fe[d] += I.pc[3 * idx + d];
+        } else {
+            fe[2] += I.fgrav ? I.fgrav[n] : 0.0;
         }
-        const bool in = t >= I.lwin[2 * (size_t)j] && t <= I.lwin[2 * (size_t)j + 1];
-#pragma unroll
-        for (int d = 0; d < 3; d++) fe[d] += in ? I.lm[3 * (size_t)j + d] * t : 0.0;
     }
-    if (!gdone) fe[2] += g;
 }
 
-// Gather + lumped-mass solve of node n at step time t (nonlinearSystem.cpp:1096-1107; rhs = -fint + fext).
+// External force of node n in the node kernel: the flagged node's value of this step (lf), else its
+// gravity load alone (zero without gravity).
+__device__ __forceinline__ void node_fext(const MpsInst& I, int n, uint32_t info, double* fe) {
+    if (info & MPS_NODE_LOADED) {
+        const double* f = I.lf + 3 * (size_t)I.lidx[n];
+        fe[0] = f[0];
+        fe[1] = f[1];
+        fe[2] = f[2];
+    } else {
+        fe[0] = 0.0;
+        fe[1] = 0.0;
+        fe[2] = I.fgrav ? I.fgrav[n] : 0.0;
+    }
+}
+
+// Surface forces of pressure Gauss point q at time t (BCsPressureRamp / BCsPressureInst,
+// boundaryConditions.cpp:370-496): F of the bulk element at the point (classSpatialPoint::getF,
+// classSpatialPoint.cpp:41-68: node terms first, then the identity), J = det F, n = F^-T n0, and
+// pc[(q*8 + a)*3 + i] = w|J_s| phi_a J P(t) n_i for the scatter nodes a.  FP64, reference operation order.
+__device__ __forceinline__ void surface_point(const MpsInst& I, int q, double t) {
+#pragma clang fp contract(off)
+    const size_t N = (size_t)I.N;
+    const int nen = I.nen;
+    const int e = I.sge[q];
+    const double* D = I.sgd + (size_t)q * MPS_SGP_STRIDE;
+    double F[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    for (int a = 0; a < nen; a++) {
+        const int nd = I.conn[(size_t)e * nen + a];
+        double disp[3];
+        for (int i = 0; i < 3; i++) disp[i] = I.u[i * N + nd];
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++) F[i * 3 + j] += D[a * 3 + j] * disp[i];
+    }
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) F[i * 3 + j] += (i == j) ? 1.0 : 0.0;
+    const double J = det3_strict(F);
+    const double idet = 1.0 / J;
+    double Fi[9];
+    Fi[0] = idet * (F[4] * F[8] - F[5] * F[7]);
+    Fi[3] = -idet * (F[3] * F[8] - F[5] * F[6]);
+    Fi[6] = idet * (F[3] * F[7] - F[4] * F[6]);
+    Fi[1] = -idet * (F[1] * F[8] - F[2] * F[7]);
+    Fi[4] = idet * (F[0] * F[8] - F[2] * F[6]);
+    Fi[7] = -idet * (F[0] * F[7] - F[1] * F[6]);
+    Fi[2] = idet * (F[1] * F[5] - F[2] * F[4]);
+    Fi[5] = -idet * (F[0] * F[5] - F[2] * F[3]);
+    Fi[8] = idet * (F[0] * F[4] - F[1] * F[3]);
+    const double* n0 = D + 24;
+    double nv[3];
+    for (int i = 0; i < 3; i++) {
+        double r = 0.0;
+        for (int j = 0; j < 3; j++) r += Fi[j * 3 + i] * n0[j];  // (F^-1)^T, row i
+        nv[i] = r;
+    }
+    const double* B = I.sblk + 4 * (size_t)I.sgb[q];
+    const bool ramp = B[0] == 2.0;
+    double P = 0.;
+    if (t >= B[2] && t <= B[3]) P = ramp ? B[1] / (B[3] - B[2]) * (t - B[2]) : B[1];
+    const double ws = D[27];
+    double* out = I.pc + (size_t)q * 24;
+    for (int a = 0; a < 8; a++) {
+        const double ph = D[28 + a];
+        for (int i = 0; i < 3; i++) out[a * 3 + i] = ws * ph * J * P * nv[i];
+    }
+}
+
+// Gather + lumped-mass solve of node n (nonlinearSystem.cpp:1096-1107; rhs = -fint + fext).
 // The node's internal force is the sum of its patch partials in ascending patch order.
-__device__ __forceinline__ void solve_node(const MpsInst& I, int n, double t, double dt, double* v, double* a) {
+__device__ __forceinline__ void solve_node(const MpsInst& I, int n, double dt, double* v, double* a) {
 #pragma clang fp contract(off)
     const uint32_t info = I.ninfo[n];
     const uint32_t mask = info & 7u;
@@ -773,7 +870,7 @@ __device__ __forceinline__ void solve_node(const MpsInst& I, int n, double t, do
         f2 += c[2];
     }
     double fe[3];
-    node_fext(I, n, info, t, fe);
+    node_fext(I, n, info, fe);
     double m = I.mass[n];
     double fi[3] = {f0, f1, f2};
 #pragma unroll

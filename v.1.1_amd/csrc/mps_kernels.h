@@ -15,8 +15,11 @@ hipError_t launch_setup(const MpsInst& I, double* mslot, double* l0out, unsigned
 hipError_t launch_fint(const MpsInst& I, double* fint, hipStream_t st);
 // Gravity load fgrav[N] (z) for g; tslot: scratch of Ne*ngp doubles.
 hipError_t launch_gravity(const MpsInst& I, double grav, double* tslot, double* fgrav, hipStream_t st);
-// External force fext[3N] at time t from the instance's loads.
+// External force fext[3N] at time t from the instance's loads (surface forces from the current u).
 hipError_t launch_fext(const MpsInst& I, double t, double* fext, hipStream_t st);
+// External force of the flagged nodes for the step in flight (surface kernel, then the per-node sums);
+// runs after the step's element kernel, before its node kernel.
+hipError_t launch_loads(const MpsInst& I, hipStream_t st);
 hipError_t launch_energy(const MpsInst& I, hipStream_t st);
 int energy_blocks(const MpsInst& I);
 // Element fields of the VTK output: out[21*Ne] (Cauchy 9, strain 9, VMS, volumetric, equivalent strain).

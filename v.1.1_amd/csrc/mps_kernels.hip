@@ -159,7 +159,7 @@ __device__ __forceinline__ void node_item(const MpsInst& I, int n, bool first, i
         v[i] = I.v[i * N + n];
         a[i] = I.a[i * N + n];
     }
-    solve_node(I, n, s.time, s.dt, v, a);
+    solve_node(I, n, s.dt, v, a);
     if (n == I.dof1_node) {
         double x = u[I.dof1_dir];
         if (!isfinite(x)) {
@@ -283,42 +283,6 @@ __global__ __launch_bounds__(256) void k_dt(MpsInst I, const double* u, unsigned
 // min l0/c.  No FMA contraction: the lumped mass is bitwise the reference's
 // per-element sums.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ void hex_shape_ref(double xi, double eta, double zeta, double* phi, double* nd) {
-#pragma clang fp contract(off)
-    const double x = xi, y = eta, z = zeta;
-    phi[0] = 0.125 * (1 - x) * (1 - y) * (1 - z);
-    phi[1] = 0.125 * (1 + x) * (1 - y) * (1 - z);
-    phi[2] = 0.125 * (1 + x) * (1 + y) * (1 - z);
-    phi[3] = 0.125 * (1 - x) * (1 + y) * (1 - z);
-    phi[4] = 0.125 * (1 - x) * (1 - y) * (1 + z);
-    phi[5] = 0.125 * (1 + x) * (1 - y) * (1 + z);
-    phi[6] = 0.125 * (1 + x) * (1 + y) * (1 + z);
-    phi[7] = 0.125 * (1 - x) * (1 + y) * (1 + z);
-    nd[0] = -0.125 * (1 - y) * (1 - z);
-    nd[3] = 0.125 * (1 - y) * (1 - z);
-    nd[6] = 0.125 * (1 + y) * (1 - z);
-    nd[9] = -0.125 * (1 + y) * (1 - z);
-    nd[12] = -0.125 * (1 - y) * (1 + z);
-    nd[15] = 0.125 * (1 - y) * (1 + z);
-    nd[18] = 0.125 * (1 + y) * (1 + z);
-    nd[21] = -0.125 * (1 + y) * (1 + z);
-    nd[1] = -0.125 * (1 - x) * (1 - z);
-    nd[4] = -0.125 * (1 + x) * (1 - z);
-    nd[7] = 0.125 * (1 + x) * (1 - z);
-    nd[10] = 0.125 * (1 - x) * (1 - z);
-    nd[13] = -0.125 * (1 - x) * (1 + z);
-    nd[16] = -0.125 * (1 + x) * (1 + z);
-    nd[19] = 0.125 * (1 + x) * (1 + z);
-    nd[22] = 0.125 * (1 - x) * (1 + z);
-    nd[2] = -0.125 * (1 - x) * (1 - y);
-    nd[5] = -0.125 * (1 + x) * (1 - y);
-    nd[8] = -0.125 * (1 + x) * (1 + y);
-    nd[11] = -0.125 * (1 - x) * (1 + y);
-    nd[14] = 0.125 * (1 - x) * (1 - y);
-    nd[17] = 0.125 * (1 + x) * (1 - y);
-    nd[20] = 0.125 * (1 + x) * (1 + y);
-    nd[23] = 0.125 * (1 - x) * (1 + y);
-}
 
 // Shape values, natural derivatives and weight of Gauss point g of the set-up quadrature: 2x2x2 for
 // hex (GPsDistribution.cpp:392-432), one point for tet (:339-343).
@@ -484,12 +448,39 @@ __global__ __launch_bounds__(256) void k_grav_node(MpsInst I, const double* __re
     fgrav[n] = acc;
 }
 
-// External force of every DOF at time t (NeumannBCManagement, boundaryConditions.cpp:75-140).
-__global__ __launch_bounds__(256) void k_fext(MpsInst I, double t, double* fext) {
+// Surface forces of every pressure Gauss point: mode 0 at the time of the step in flight (ctl->B, after its
+// element kernel), mode 1 at time t (standalone evaluation).
+__global__ __launch_bounds__(256) void k_press(MpsInst I, int mode, double t) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (mode == 0) {
+        const MpsStepSlot s = I.ctl->slot[I.ctl->B & 1];
+        if (!s.go) return;
+        t = s.time;
+    }
+    if (q < I.n_sgp) surface_point(I, q, t);
+}
+
+// External force of every flagged node: mode 0 at the time of the step in flight (after its surface
+// kernel), mode 1 at time t.
+__global__ __launch_bounds__(256) void k_loads(MpsInst I, int mode, double t) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (mode == 0) {
+        const MpsStepSlot s = I.ctl->slot[I.ctl->B & 1];
+        if (!s.go) return;
+        t = s.time;
+    }
+    if (j >= I.n_lnode) return;
+    double fe[3];
+    loaded_node_fext(I, j, t, fe);
+    for (int d = 0; d < 3; d++) I.lf[3 * (size_t)j + d] = fe[d];
+}
+
+// External force of every DOF (NeumannBCManagement, boundaryConditions.cpp:75-140) from the values of k_loads.
+__global__ __launch_bounds__(256) void k_fext(MpsInst I, double* fext) {
     const int n = blockIdx.x * blockDim.x + threadIdx.x;
     if (n >= I.N) return;
     double fe[3];
-    node_fext(I, n, I.ninfo[n], t, fe);
+    node_fext(I, n, I.ninfo[n], fe);
     const size_t N = (size_t)I.N;
     fext[n] = fe[0];
     fext[N + n] = fe[1];
@@ -798,7 +789,15 @@ hipError_t launch_gravity(const MpsInst& I, double grav, double* tslot, double* 
 }
 
 hipError_t launch_fext(const MpsInst& I, double t, double* fext, hipStream_t st) {
-    k_fext<<<blocks_for(I.N, 256), 256, 0, st>>>(I, t, fext);
+    if (I.n_sgp > 0) k_press<<<blocks_for(I.n_sgp, 256), 256, 0, st>>>(I, 1, t);
+    if (I.n_lnode > 0) k_loads<<<blocks_for(I.n_lnode, 256), 256, 0, st>>>(I, 1, t);
+    k_fext<<<blocks_for(I.N, 256), 256, 0, st>>>(I, fext);
+    return hipGetLastError();
+}
+
+hipError_t launch_loads(const MpsInst& I, hipStream_t st) {
+    if (I.n_sgp > 0) k_press<<<blocks_for(I.n_sgp, 256), 256, 0, st>>>(I, 0, 0.0);
+    if (I.n_lnode > 0) k_loads<<<blocks_for(I.n_lnode, 256), 256, 0, st>>>(I, 0, 0.0);
     return hipGetLastError();
 }
 

@@ -315,7 +315,8 @@ bool readSolvers(Reader &R, InpModel &m) {
                 if (!timeSeen)
                     return R.fail(i, "*Time must precede *BOUNDARY inside *SOLVER (the ramp duration is taken at parse time)");
                 if (!readDirichlet(R, i, m, s)) return false;
-            } else if (starts(l, "*BOUNDARY") && tail(l, 16) == "FORCE") {
+            } else if (starts(l, "*BOUNDARY") &&
+                       (tail(l, 16) == "FORCE" || tail(l, 16) == "PRESSURE RAMP" || tail(l, 16) == "PRESSURE INST")) {
                 // a Neumann block: read with the whole file by readLoads, for every solver
                 for (i++; i + 1 < R.L.size() && !starts(R.L[i + 1], "*"); i++) {
                 }
@@ -449,22 +450,28 @@ bool readExtractions(Reader &R, InpModel &m) {
     return true;
 }
 
-// NeumannCommonManagement (inputFile.cpp:4703-4760): the whole file, in order.  FORCE blocks
-// (forceBCManagement, :2841-2941): a "t0, tf, Ff" line, then "node, Fx, Fy, Fz" lines up to the next
-// keyword, each entry F * Ff over the window [t0, tf].  *GRAVITY: the value g on the next line.
-// The other Neumann types and contact are not on this path and are rejected.
+// NeumannCommonManagement (inputFile.cpp:4703-4760): the whole file, in order.
+//   FORCE (forceBCManagement, :2841-2941): a "t0, tf, Ff" line, then "node, Fx, Fy, Fz" lines up to the next
+//     keyword, each entry F * Ff over the window [t0, tf].
+//   PRESSURE RAMP / INST (pressureRampBCManagement :3580-3684, pressureInstBCManagement :3686-3790): a
+//     "t0, tf, P, S" line, then one element id per line (its leading integer); face S of each element.
+//   *GRAVITY: the value g on the next line.
+// Traction, contact and flux loads are not on this path and are rejected.
 bool readLoads(Reader &R, InpModel &m) {
+    bool gravity = false;
     for (size_t i = 0; i < R.L.size(); i++) {
         const std::string &l = R.L[i];
         if (starts(l, "*BOUNDARY")) {
             const std::string type = tail(l, 16);
             if (type == "FORCE") {
+                InpLoadBlock b;
+                b.type = MPS_LOAD_FORCE;
                 if (i + 1 >= R.L.size()) return R.fail(i, "FORCE needs a \"t0, tf, Ff\" line");
                 std::vector<std::string> f = fields(R.L[i + 1]);
-                double t0, tf, Ff;
-                if (f.size() != 3 || !toDouble(f[0], t0) || !toDouble(f[1], tf) || !toDouble(f[2], Ff))
+                double Ff;
+                if (f.size() != 3 || !toDouble(f[0], b.t0) || !toDouble(f[1], b.tf) || !toDouble(f[2], Ff))
                     return R.fail(i + 1, "FORCE needs a \"t0, tf, Ff\" line");
-                if (!(tf > t0)) return R.fail(i + 1, "FORCE window needs tf > t0 (the load grows as F/(tf-t0) * t)");
+                if (!(b.tf > b.t0)) return R.fail(i + 1, "FORCE window needs tf > t0 (the load grows as F/(tf-t0) * t)");
                 for (i += 2; i < R.L.size() && !starts(R.L[i], "*"); i++) {
                     if (isBlank(R.L[i])) continue;
                     std::vector<std::string> g = fields(R.L[i]);
@@ -474,23 +481,44 @@ bool readLoads(Reader &R, InpModel &m) {
                         !toDouble(g[3], F[2]))
                         return R.fail(i, "FORCE entry needs \"node, Fx, Fy, Fz\"");
                     if (id < 1 || id > m.nNodes) return R.fail(i, "node id outside 1..N");
-                    m.loads.node.push_back((int32_t)(id - 1));
-                    for (int d = 0; d < 3; d++) m.loads.force.push_back(F[d] * Ff);
-                    m.loads.t0.push_back(t0);
-                    m.loads.tf.push_back(tf);
+                    b.index.push_back((int32_t)(id - 1));
+                    for (int d = 0; d < 3; d++) b.force.push_back(F[d] * Ff);
                 }
                 i--;
-            } else if (type == "TRACTION RAMP" || type == "TRACTION INST" || type == "PRESSURE RAMP" ||
-                       type == "PRESSURE INST") {
-                return R.fail(i, "*BOUNDARY, TYPE=" + type + " is not on this path (FORCE and *GRAVITY are)");
+                m.loads.blocks.push_back(b);
+            } else if (type == "PRESSURE RAMP" || type == "PRESSURE INST") {
+                InpLoadBlock b;
+                b.type = type == "PRESSURE RAMP" ? MPS_LOAD_PRESSURE_RAMP : MPS_LOAD_PRESSURE_INST;
+                if (i + 1 >= R.L.size()) return R.fail(i, "PRESSURE needs a \"t0, tf, P, S\" line");
+                std::vector<std::string> f = fields(R.L[i + 1]);
+                double S;
+                if (f.size() != 4 || !toDouble(f[0], b.t0) || !toDouble(f[1], b.tf) || !toDouble(f[2], b.value) ||
+                    !toDouble(f[3], S))
+                    return R.fail(i + 1, "PRESSURE needs a \"t0, tf, P, S\" line");
+                const int nf = m.nen == 8 ? 6 : 4;
+                if ((int)S < 1 || (int)S > nf) return R.fail(i + 1, "PRESSURE face S outside 1.." + std::to_string(nf));
+                if (b.type == MPS_LOAD_PRESSURE_RAMP ? !(b.tf > b.t0) : !(b.tf >= b.t0))
+                    return R.fail(i + 1, "PRESSURE window needs tf > t0 (tf >= t0 for INST)");
+                for (i += 2; i < R.L.size() && !starts(R.L[i], "*"); i++) {
+                    if (isBlank(R.L[i])) continue;
+                    long long id;
+                    if (!toInt(fields(R.L[i])[0], id) || id < 1 || id > m.nElem)
+                        return R.fail(i, "PRESSURE element id outside 1..N_e");
+                    b.index.push_back((int32_t)(id - 1));
+                    b.face.push_back((int32_t)S);
+                }
+                i--;
+                m.loads.blocks.push_back(b);
+            } else if (type == "TRACTION RAMP" || type == "TRACTION INST") {
+                return R.fail(i, "*BOUNDARY, TYPE=" + type + " is not on this path (FORCE, PRESSURE and *GRAVITY are)");
             }
         } else if (starts(l, "*GRAVITY")) {
-            double g;
-            if (i + 1 >= R.L.size() || !toDouble(R.L[i + 1], g)) return R.fail(i, "*GRAVITY needs the value g");
-            if (m.loads.hasGravity) return R.fail(i, "a second *GRAVITY: one gravity load per model");
-            m.loads.gravity = g;
-            m.loads.hasGravity = true;
-            m.loads.gravityPos = (int)m.loads.node.size();
+            InpLoadBlock b;
+            b.type = MPS_LOAD_GRAVITY;
+            if (i + 1 >= R.L.size() || !toDouble(R.L[i + 1], b.value)) return R.fail(i, "*GRAVITY needs the value g");
+            if (gravity) return R.fail(i, "a second *GRAVITY: one gravity load per model");
+            gravity = true;
+            m.loads.blocks.push_back(b);
             i++;
         } else if (starts(l, "*HERTZIAN CONTACT") || starts(l, "*STOCHASTIC HERTZIAN CONTACT") ||
                    starts(l, "*FLUX EXTRADOF")) {
@@ -508,7 +536,8 @@ bool rejectUnsupported(Reader &R) {
         if (starts(l, "*SOLVER")) inSolver = true;
         if (starts(l, "*END SOLVER")) inSolver = false;
         if (inSolver) continue;
-        if (starts(l, "*BOUNDARY") && tail(l, 16) != "FORCE")
+        const std::string type = tail(l, 16);
+        if (starts(l, "*BOUNDARY") && type != "FORCE" && type != "PRESSURE RAMP" && type != "PRESSURE INST")
             return R.fail(i, "Dirichlet conditions outside *SOLVER are not on this path");
     }
     return true;

@@ -9,8 +9,9 @@
 //   *SOLVER, EXPLICIT ... *END SOLVER                  solverManagement inputFile.cpp:2253-2762
 //   *BOUNDARY, TYPE=DISPLACEMENT RAMP|INST             dispBCManagement inputFile.cpp:2960-3217
 //   *Outputs                    outManagement          inputFile.cpp:4769-4829
-//   *BOUNDARY, TYPE=FORCE       forceBCManagement      inputFile.cpp:2841-2941  (anywhere in the file:
-//   *GRAVITY                    NeumannCommonManagement inputFile.cpp:4703-4735   global for all solvers)
+//   *BOUNDARY, TYPE=FORCE       forceBCManagement      inputFile.cpp:2841-2941  (anywhere in the file,
+//   *BOUNDARY, TYPE=PRESSURE RAMP|INST  pressure*BCManagement inputFile.cpp:3580-3790  in order; global
+//   *GRAVITY                    NeumannCommonManagement inputFile.cpp:4703-4735   for all solvers)
 //   *EXTRACTION, NODE|ELEMENT   dataExtractionManagement inputFile.cpp:2229-2243, 4950-5120
 // Keywords are matched as prefixes, case-sensitively, as the reference does.
 // Where the reference would read garbage, dereference null or silently
@@ -73,16 +74,20 @@ struct InpElementExtraction {
     std::vector<int32_t> elems; // 0-based
 };
 
-// Neumann loads of the path in input order (NeumannCommonManagement reads the whole file; the loads
-// apply to every solver, solverExplicit.cpp:47).
+// One Neumann block (an entry of the reference's NeumannBCs list).  NeumannCommonManagement reads the whole
+// file; the blocks apply to every solver (solverExplicit.cpp:47).
+struct InpLoadBlock {
+    int type = 0;                // MPS_LOAD_FORCE, _GRAVITY, _PRESSURE_RAMP, _PRESSURE_INST
+    double t0 = 0, tf = 0;       // FORCE, PRESSURE window
+    double value = 0;            // GRAVITY g, PRESSURE P
+    std::vector<int32_t> index;  // FORCE: node; PRESSURE: element (0-based)
+    std::vector<int32_t> face;   // PRESSURE: face S as getSubElement numbers it
+    std::vector<double> force;   // FORCE: 3 per entry, F * Ff
+};
+
 struct InpLoads {
-    std::vector<int32_t> node;   // force entries, 0-based node
-    std::vector<double> force;   // 3 per entry: F * Ff
-    std::vector<double> t0, tf;  // window of the block the entry belongs to
-    double gravity = 0;          // *GRAVITY g
-    bool hasGravity = false;
-    int gravityPos = -1;         // force entries read before *GRAVITY
-    bool empty() const { return node.empty() && !hasGravity; }
+    std::vector<InpLoadBlock> blocks;  // input order
+    bool empty() const { return blocks.empty(); }
 };
 
 struct InpModel {

@@ -84,14 +84,22 @@ static int check(const std::string &path) {
         std::printf("%s\"%s\"", first ? "" : ", ", kv.first.c_str());
         first = false;
     }
-    const mps::InpLoads &ld = m.loads;
-    char gbuf[40] = "null";
-    if (ld.hasGravity) std::snprintf(gbuf, sizeof gbuf, "%.17g", ld.gravity);
-    std::printf("], \"loads\": {\"gravity\": %s, \"gravity_pos\": %d, \"force\": [", gbuf, ld.gravityPos);
-    for (size_t k = 0; k < ld.node.size(); k++)
-        std::printf("%s[%d, %.17g, %.17g, %.17g, %.17g, %.17g]", k ? ", " : "", ld.node[k], ld.force[3 * k],
-                    ld.force[3 * k + 1], ld.force[3 * k + 2], ld.t0[k], ld.tf[k]);
-    std::printf("]}, \"warnings\": %zu}\n", m.warnings.size());
+    std::printf("], \"loads\": [");
+    const char *names[4] = {"FORCE", "GRAVITY", "PRESSURE RAMP", "PRESSURE INST"};
+    for (size_t k = 0; k < m.loads.blocks.size(); k++) {
+        const mps::InpLoadBlock &b = m.loads.blocks[k];
+        std::printf("%s{\"type\": \"%s\", \"t0\": %.17g, \"tf\": %.17g, \"value\": %.17g, \"entries\": [", k ? ", " : "",
+                    names[b.type], b.t0, b.tf, b.value);
+        for (size_t j = 0; j < b.index.size(); j++) {
+            if (b.type == MPS_LOAD_FORCE)
+                std::printf("%s[%d, %.17g, %.17g, %.17g]", j ? ", " : "", b.index[j], b.force[3 * j], b.force[3 * j + 1],
+                            b.force[3 * j + 2]);
+            else
+                std::printf("%s[%d, %d]", j ? ", " : "", b.index[j], b.face[j]);
+        }
+        std::printf("]}");
+    }
+    std::printf("], \"warnings\": %zu}\n", m.warnings.size());
     return 0;
 }
 

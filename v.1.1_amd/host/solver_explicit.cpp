@@ -121,9 +121,11 @@ int classSolverExplicit::calculate(const InpModel &m, const std::vector<std::uni
     // Neumann loads, global for all solvers (solverExplicit.cpp:47; NeumannBCManagement at every step)
     const InpLoads &ld = m.loads;
     if (!ld.empty()) {
-        mps_loads L{(int32_t)ld.node.size(), ld.hasGravity ? ld.gravityPos : -1, ld.hasGravity ? ld.gravity : 0.0,
-                    ld.node.data(), ld.force.data(), ld.t0.data(), ld.tf.data()};
-        if (mps_set_loads(h, &L) != MPS_OK) return fail("mps_set_loads");
+        std::vector<mps_load> L;
+        for (const InpLoadBlock &b : ld.blocks)
+            L.push_back(mps_load{b.type, (int32_t)(b.type == MPS_LOAD_GRAVITY ? 0 : b.index.size()), b.t0, b.tf, b.value,
+                                 b.index.data(), b.face.data(), b.force.data()});
+        if (mps_set_loads(h, (int32_t)L.size(), L.data()) != MPS_OK) return fail("mps_set_loads");
     }
     double dt0 = 0;
     if (mps_critical_dt(h, nullptr, &dt0) != MPS_OK) return fail("mps_critical_dt");

@@ -44,9 +44,34 @@ class SolverCfg(C.Structure):
                 ("device", C.c_int32)]
 
 
-class Loads(C.Structure):
-    _fields_ = [("n_force", C.c_int32), ("gravity_pos", C.c_int32), ("gravity", C.c_double), ("node", _ip),
-                ("force", _dp), ("t0", _dp), ("tf", _dp)]
+class Load(C.Structure):
+    _fields_ = [("type", C.c_int32), ("n", C.c_int32), ("t0", C.c_double), ("tf", C.c_double),
+                ("value", C.c_double), ("index", _ip), ("face", _ip), ("force", _dp)]
+
+
+LOAD_FORCE, LOAD_GRAVITY, LOAD_PRESSURE_RAMP, LOAD_PRESSURE_INST = 0, 1, 2, 3
+_LOAD_TYPES = {"force": LOAD_FORCE, "gravity": LOAD_GRAVITY, "pressure_ramp": LOAD_PRESSURE_RAMP,
+               "pressure_inst": LOAD_PRESSURE_INST}
+
+
+def load_blocks(forces=None, gravity=0.0, gravity_pos=-1, loads=None):
+    """Neumann blocks in input order, as tuples:
+      ("force", nodes, force[k,3], t0, tf)                *BOUNDARY, TYPE=FORCE (entries F*Ff, one window)
+      ("gravity", g)                                      *GRAVITY
+      ("pressure_ramp" | "pressure_inst", P, t0, tf, elements, faces)
+    `loads` is such a list; otherwise forces = (node[k], force[k,3], t0[k], tf[k]) gives one block per entry and
+    a non-zero gravity goes before entry gravity_pos (-1: after all)."""
+    if loads is not None:
+        return list(loads)
+    out = []
+    if forces is not None:
+        fn, fv, f0, f1 = forces
+        fv = np.reshape(np.asarray(fv, dtype=np.float64), (-1, 3))
+        out = [("force", [int(fn[k])], fv[k:k + 1], float(f0[k]), float(f1[k])) for k in range(len(fn))]
+    if gravity != 0.0:
+        pos = gravity_pos if 0 <= gravity_pos <= len(out) else len(out)
+        out.insert(pos, ("gravity", float(gravity)))
+    return out
 
 
 class Report(C.Structure):
@@ -102,7 +127,7 @@ def load(path=None):
     L.mps_internal_force.argtypes = [H, _dp, _dp]
     L.mps_critical_dt.argtypes = [H, _dp, _dp]
     L.mps_element_outputs.argtypes = [H, _dp]
-    L.mps_set_loads.argtypes = [H, C.POINTER(Loads)]
+    L.mps_set_loads.argtypes = [H, C.c_int32, C.POINTER(Load)]
     L.mps_external_force.argtypes = [H, C.c_double, _dp]
     L.mps_get_dofs.argtypes = [H, C.c_int32, C.c_int64, _ip, _dp]
     L.mps_destroy.argtypes = [H]
@@ -226,21 +251,29 @@ class Solver:
         arrs = [None if x is None else np.ascontiguousarray(x, dtype=np.float64) for x in (u, v, a)]
         self._chk(self.L.mps_set_state(self.h, *[None if x is None else _d(x) for x in arrs]))
 
-    def set_loads(self, forces=None, gravity=0.0, gravity_pos=-1):
-        """forces: None or (node[k], force[k,3], t0[k], tf[k]) in input order (*BOUNDARY, TYPE=FORCE entries,
-        force = F * Ff); gravity: g of *GRAVITY; gravity_pos: the force entry gravity precedes (-1: after all)."""
-        if forces is None:
-            node, vec, t0, tf = np.zeros(0, np.int32), np.zeros((0, 3)), np.zeros(0), np.zeros(0)
-        else:
-            node, vec, t0, tf = forces
-        self._loads = [np.ascontiguousarray(node, dtype=np.int32),
-                       np.ascontiguousarray(np.reshape(vec, (-1, 3)), dtype=np.float64),
-                       np.ascontiguousarray(t0, dtype=np.float64), np.ascontiguousarray(tf, dtype=np.float64)]
-        nf = self._loads[0].size
-        ld = Loads(n_force=nf, gravity_pos=gravity_pos, gravity=gravity,
-                   node=_i(self._loads[0]) if nf else None, force=_d(self._loads[1]) if nf else None,
-                   t0=_d(self._loads[2]) if nf else None, tf=_d(self._loads[3]) if nf else None)
-        self._chk(self.L.mps_set_loads(self.h, C.byref(ld)))
+    def set_loads(self, forces=None, gravity=0.0, gravity_pos=-1, loads=None):
+        """Neumann loads in input order: see load_blocks."""
+        blocks = load_blocks(forces, gravity, gravity_pos, loads)
+        arr = (Load * max(len(blocks), 1))()
+        keep = []
+        for b, blk in enumerate(blocks):
+            kind = _LOAD_TYPES[blk[0]]
+            Lb = arr[b]
+            Lb.type = kind
+            if kind == LOAD_FORCE:
+                nodes = np.ascontiguousarray(blk[1], dtype=np.int32)
+                vec = np.ascontiguousarray(np.reshape(blk[2], (-1, 3)), dtype=np.float64)
+                keep += [nodes, vec]
+                Lb.n, Lb.t0, Lb.tf, Lb.index, Lb.force = nodes.size, blk[3], blk[4], _i(nodes), _d(vec)
+            elif kind == LOAD_GRAVITY:
+                Lb.n, Lb.value = 0, blk[1]
+            else:
+                el = np.ascontiguousarray(blk[4], dtype=np.int32)
+                fc = np.ascontiguousarray(blk[5], dtype=np.int32)
+                keep += [el, fc]
+                Lb.n, Lb.value, Lb.t0, Lb.tf, Lb.index, Lb.face = el.size, blk[1], blk[2], blk[3], _i(el), _i(fc)
+        self._loads = keep
+        self._chk(self.L.mps_set_loads(self.h, len(blocks), arr))
 
     def external_force(self, t):
         f = np.zeros(3 * self.n)

@@ -1027,7 +1027,41 @@ static void external_force(orc_state *s, const double *u, double t)
 }
 
 /* computeInternalForce: solverExplicit.cpp:363-379 (+ computeState, solvers.cpp:615-641) */
-static void internal_force(orc_state *s, const double *u)
+/* bulkViscosity (solverExplicit.cpp:513-542): Fdot = (F1 - F0) * (1/dt), trD = tr Fdot,
+ * q = rho/J1 h (c1^2 h |trD| min(0, trD) + cl c0 trD), Cauchy = -q I, P = Cauchy F1^-T * (-J1). */
+static void bulk_viscosity(const orc_problem *p, double h, double c0, const double *F1, const double *F0, double rho,
+                           double dt, double *P)
+{
+    double Fdot[9];
+    if (dt == 0) {
+        for (int i = 0; i < 9; i++) Fdot[i] = 0.;
+    } else {
+        for (int i = 0; i < 9; i++) Fdot[i] = F1[i] + -1 * F0[i];
+        for (int i = 0; i < 9; i++) Fdot[i] = Fdot[i] * (1 / dt);
+    }
+    double trD = 0;
+    for (int i = 0; i < 3; i++) trD += Fdot[i * 3 + i];
+    const double J1 = orc_det3(F1);
+    const double Rho_1 = rho / J1;
+    const double mn = (trD < 0.) ? trD : 0.; /* std::min(0., trD) */
+    const double qBV = Rho_1 * h * (p->visc_c1 * p->visc_c1 * h * fabs(trD) * mn + p->visc_cl * c0 * trD);
+    double C[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, Fi[9];
+    for (int i = 0; i < 3; i++) C[i + i * 3] = -1 * qBV;
+    orc_inverse3(F1, Fi);
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) {
+            double c = 0;
+            for (int k = 0; k < 3; k++) c += C[i * 3 + k] * Fi[j * 3 + k];
+            P[i * 3 + j] = c;
+        }
+    for (int i = 0; i < 9; i++) P[i] = P[i] * (-1 * J1);
+}
+
+/* computeInternalForce: solverExplicit.cpp:363-423 (+ computeState, solvers.cpp:615-641).  With bulk
+ * viscosity, a second pass over the GPs adds wJ P_vis dphi with F0 the GP's F at the previous step, i.e.
+ * at uprev.  The deviatoric variant (artificialViscosity, :469-503) never copies its stress into the PK1
+ * (Cauchy_vis stays zero), so it adds nothing. */
+static void internal_force_visc(orc_state *s, const double *u, const double *uprev, double dt)
 {
     size_t N = s->N, Ne = s->Ne, ng = s->ngp, nen = s->nen;
     if (s->p->sum_order == ORC_SUM_REFERENCE) {
@@ -1047,11 +1081,37 @@ static void internal_force(orc_state *s, const double *u)
                     for (size_t a = 0; a < nen; a++) s->fint[cn[a] + N * i] += f[a + i * nen];
             }
         }
+        if (uprev && s->p->visc == ORC_VISC_BULK) {
+            double uloc0[24], F0[9];
+            for (size_t e = 0; e < Ne; e++) {
+                const int32_t *cn = s->p->conn + e * nen;
+                for (size_t a = 0; a < nen; a++)
+                    for (int i = 0; i < 3; i++) {
+                        uloc[a * 3 + i] = u[cn[a] + N * i];
+                        uloc0[a * 3 + i] = uprev[cn[a] + N * i];
+                    }
+                const double h = s->p->visc_h > 0 ? s->p->visc_h : s->l0[e];
+                const double c0 = s->p->visc_c0 > 0 ? s->p->visc_c0 : s->c[e];
+                for (size_t g = 0; g < ng; g++) {
+                    size_t gi = e * ng + g;
+                    const double *dphi = s->dphi + gi * nen * 3;
+                    orc_kinematics(s->nen, dphi, uloc, F);
+                    orc_kinematics(s->nen, dphi, uloc0, F0);
+                    bulk_viscosity(s->p, h, c0, F, F0, s->rho[e], dt, P);
+                    for (size_t a = 0; a < nen; a++)
+                        for (int i = 0; i < 3; i++)
+                            for (int J = 0; J < 3; J++)
+                                s->fint[cn[a] + N * i] += s->wJ[gi] * P[i * 3 + J] * dphi[a * 3 + J];
+                }
+            }
+        }
     } else {
         run_par(s, task_elements, (void *)u);
         run_par(s, task_gather, NULL);
     }
 }
+
+static void internal_force(orc_state *s, const double *u) { internal_force_visc(s, u, NULL, 0.0); }
 
 /* calculateCriticalTimeStep(nod, elem, uK): solvers.cpp:296-320 (u == NULL gives the
  * undeformed variant solvers.cpp:276-287, since min(l0, l0) = l0). */
@@ -1143,6 +1203,8 @@ static int setup(orc_state *s, const orc_problem *p)
     if (!p || (p->nen != 8 && p->nen != 4) || p->n_nodes <= 0 || p->n_elem <= 0) return ORC_ERR_ARG;
     if (p->n_threads > 1 && p->sum_order != ORC_SUM_ELEMENT) return ORC_ERR_ARG;
     if (p->law < ORC_LAW_NH_COMPRESSIBLE || p->law > ORC_LAW_LINEAR) return ORC_ERR_ARG;
+    if (p->visc < ORC_VISC_NONE || p->visc > ORC_VISC_DEV) return ORC_ERR_ARG;
+    if (p->visc == ORC_VISC_BULK && p->sum_order != ORC_SUM_REFERENCE) return ORC_ERR_ARG;
     int rc = state_init(s, p);
     if (rc) return rc;
     s->nt = p->n_threads > 1 ? p->n_threads : 1;
@@ -1191,6 +1253,18 @@ int32_t orc_lumped_mass(const orc_problem *p, double *mass)
     if (rc == ORC_OK) {
         compute_mass(&s);
         memcpy(mass, s.mass, sizeof(double) * (size_t)s.N);
+    }
+    state_free(&s);
+    return rc;
+}
+
+int32_t orc_internal_force_visc(const orc_problem *p, const double *u, const double *uprev, double dt, double *fint)
+{
+    orc_state s;
+    int rc = setup(&s, p);
+    if (rc == ORC_OK) {
+        internal_force_visc(&s, u, uprev, dt);
+        memcpy(fint, s.fint, sizeof(double) * 3 * (size_t)s.N);
     }
     state_free(&s);
     return rc;
@@ -1313,7 +1387,7 @@ int32_t orc_run(const orc_problem *p, double *u, double *v, double *a, orc_histo
     orc_report r = {0, 0, 0, ORC_OK, 0};
     worker_arg *wa = NULL;
     pthread_t *th = NULL;
-    double *uzero = NULL;
+    double *uzero = NULL, *uprev = NULL;
     if (rc != ORC_OK) goto out;
     th = pool_start(&s, &wa);
     size_t N = s.N;
@@ -1326,6 +1400,10 @@ int32_t orc_run(const orc_problem *p, double *u, double *v, double *a, orc_histo
     double dt = critical_dt(&s, NULL) * p->scale;
     /* uK is fetched from the system only if timeRun < 1e-12 (:109, :176-178) */
     uzero = calloc(3 * N, sizeof(double));
+    /* F of the previous state (solverExplicit.cpp:149: computeKinematicVariables(u0) + nextStep) */
+    uprev = malloc(sizeof(double) * 3 * N);
+    if (!uzero || !uprev) { rc = ORC_ERR_MEM; goto out; }
+    memcpy(uprev, u, sizeof(double) * 3 * N);
     const double *uK = (timeRun < 1E-12) ? u : uzero;
     /* a_prev is a at the last nextStep (:149 -> nonlinearSystem.cpp:710-724); it is
      * the same array as a until the solve overwrites a, so one array suffices. */
@@ -1377,7 +1455,7 @@ int32_t orc_run(const orc_problem *p, double *u, double *v, double *a, orc_histo
             a[ind] = 0.;
         }
         /* computeState + computeRightHandSide: fint(u) */
-        internal_force(&s, u);
+        internal_force_visc(&s, u, uprev, dt);
         /* NeumannBCManagement at timeRun (boundaryConditions.cpp:75-140) */
         external_force(&s, u, timeRun);
         /* systemSolve, nonlinearSystem.cpp:1096-1107: a = (-fint + fext)/m, v update */
@@ -1389,6 +1467,7 @@ int32_t orc_run(const orc_problem *p, double *u, double *v, double *a, orc_histo
             v[i] += dt * ((1. - 0.5) * a[i] + 0.5 * anew);
             a[i] = anew;
         }
+        if (p->visc == ORC_VISC_BULK) memcpy(uprev, u, sizeof(double) * 3 * N); /* nextStep: previous := current */
         if (!isfinite(u[1])) { rc = ORC_ERR_NAN; break; } /* :306-308 */
         int64_t k = nStep - 1;
         if (hist && k < hist->cap) {
@@ -1423,6 +1502,7 @@ int32_t orc_run(const orc_problem *p, double *u, double *v, double *a, orc_histo
 out:
     pool_stop(&s, th, wa);
     free(uzero);
+    free(uprev);
     state_free(&s);
     if (rep) *rep = r;
     return rc;

@@ -31,6 +31,7 @@ enum { ORC_BC_RAMP = 0, ORC_BC_INST = 1, ORC_BC_VELOCITY = 2 };
 enum { ORC_SUM_REFERENCE = 0, ORC_SUM_ELEMENT = 1 };
 enum { ORC_OK = 0, ORC_ERR_DT = -1, ORC_ERR_NAN = -2, ORC_ERR_ARG = -3, ORC_ERR_MEM = -4 };
 
+enum { ORC_VISC_NONE = 0, ORC_VISC_BULK = 1, ORC_VISC_DEV = 2 };
 enum { ORC_LOAD_FORCE = 0, ORC_LOAD_GRAVITY = 1, ORC_LOAD_PRESSURE_RAMP = 2, ORC_LOAD_PRESSURE_INST = 3 };
 
 /* One Neumann block.
@@ -77,6 +78,13 @@ typedef struct orc_problem {
     int32_t n_loads;
     int32_t _pad2;
     const struct orc_load *loads;
+    /* Artificial viscosity (*Bulk viscosity / *Dev viscosity, inputFile.cpp:2462-2510; solverExplicit.cpp:381-421,
+     * 469-542), reference summation order only.  The reference's element size and c0 (solvers.h:53-54) are never
+     * set; here they are visc_h and visc_c0 when > 0, else the element's undeformed length l0 and sound speed. */
+    int32_t visc;           /* ORC_VISC_* */
+    int32_t _pad3;
+    double visc_c1, visc_cl;
+    double visc_h, visc_c0;
 } orc_problem;
 
 typedef struct orc_report {
@@ -127,6 +135,9 @@ int32_t orc_lumped_mass(const orc_problem *p, double *mass);
 
 /* Internal force fint[3*n_nodes] for a given displacement. */
 int32_t orc_internal_force(const orc_problem *p, const double *u, double *fint);
+
+/* Internal force with the step's artificial viscosity: u now, uprev at the previous step, step dt. */
+int32_t orc_internal_force_visc(const orc_problem *p, const double *u, const double *uprev, double dt, double *fint);
 
 /* Critical time step s*min_e min(l0,l(u))/c (solvers.cpp:296-320); u may be NULL. */
 double orc_critical_dt(const orc_problem *p, const double *u);

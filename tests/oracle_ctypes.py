@@ -33,7 +33,12 @@ class Problem(C.Structure):
         ("n_steps", C.c_int64), ("sum_order", C.c_int32), ("n_threads", C.c_int32),
         ("energy_every", C.c_int32), ("_pad", C.c_int32),
         ("n_loads", C.c_int32), ("_pad2", C.c_int32), ("loads", C.c_void_p),
+        ("visc", C.c_int32), ("_pad3", C.c_int32), ("visc_c1", C.c_double), ("visc_cl", C.c_double),
+        ("visc_h", C.c_double), ("visc_c0", C.c_double),
     ]
+
+
+VISC_NONE, VISC_BULK, VISC_DEV = 0, 1, 2
 
 
 class Load(C.Structure):
@@ -137,6 +142,8 @@ def lib():
         L.orc_element_outputs.restype = C.c_int32
         L.orc_external_force.restype = C.c_int32
         L.orc_external_force.argtypes = [C.c_void_p, _dp, C.c_double, _dp]
+        L.orc_internal_force_visc.restype = C.c_int32
+        L.orc_internal_force_visc.argtypes = [C.c_void_p, _dp, _dp, C.c_double, _dp]
         L.orc_pressure_face.restype = C.c_int32
         L.orc_pressure_face.argtypes = [C.c_void_p, C.c_int32, C.c_int32, _dp, _dp, _dp, _dp, _dp, _dp]
         L.orc_internal_force.argtypes = [C.POINTER(Problem), _dp, _dp]
@@ -162,7 +169,8 @@ class OracleProblem:
     def __init__(self, mesh, law=LAW_NH_COMPRESSIBLE, E=1e6, nu=0.3, rho=1000.0,
                  dir_dof=None, dir_val=None, bc_mode=BC_VELOCITY, vel_ramp_steps=100,
                  t0=0.0, tf=1.0, scale=0.5, n_steps=0, sum_order=SUM_REFERENCE,
-                 n_threads=1, energy_every=0, gravity=0.0, forces=None, gravity_pos=-1, loads=None):
+                 n_threads=1, energy_every=0, gravity=0.0, forces=None, gravity_pos=-1, loads=None,
+                 visc=0, visc_c1=0.06, visc_cl=1.2, visc_h=0.0, visc_c0=0.0):
         """Neumann loads: see load_blocks (forces / gravity / gravity_pos, or an explicit block list)."""
         self.X = np.ascontiguousarray(mesh["X"], dtype=np.float64)
         self.conn = np.ascontiguousarray(mesh["conn"], dtype=np.int32)
@@ -190,7 +198,8 @@ class OracleProblem:
             dir_val=dptr(self.dir_val) if self.dir_val.size else None,
             bc_mode=bc_mode, vel_ramp_steps=vel_ramp_steps, t0=t0, tf=tf, scale=scale,
             n_steps=n_steps, sum_order=sum_order, n_threads=n_threads,
-            energy_every=energy_every, n_loads=len(self.blocks),
+            energy_every=energy_every, n_loads=len(self.blocks), visc=visc, visc_c1=visc_c1, visc_cl=visc_cl,
+            visc_h=visc_h, visc_c0=visc_c0,
             loads=C.cast(self.load_arr, C.c_void_p) if self.blocks else None)
 
     def run(self, u0=None, v0=None, a0=None, hist_cap=0):
@@ -220,6 +229,14 @@ class OracleProblem:
         u = np.ascontiguousarray(u, dtype=np.float64)
         f = np.zeros(3 * self.n_nodes)
         rc = lib().orc_internal_force(C.byref(self.p), dptr(u), dptr(f))
+        assert rc == 0
+        return f
+
+    def internal_force_visc(self, u, uprev, dt):
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        uprev = np.ascontiguousarray(uprev, dtype=np.float64)
+        f = np.zeros(3 * self.n_nodes)
+        rc = lib().orc_internal_force_visc(C.byref(self.p), dptr(u), dptr(uprev), dt, dptr(f))
         assert rc == 0
         return f
 

@@ -787,3 +787,71 @@ def test_pressure_blocks_in_input_order_and_dynamics():
     d = (u[:, corners] * np.sign(0.5 - X[:, corners]))
     assert np.all(d > 0)
     np.testing.assert_allclose(d, d[0, 0], rtol=1e-9)
+
+
+# ---------------------------------------------------------------------------
+# Artificial viscosity (solverExplicit.cpp:381-421, 469-542)
+# ---------------------------------------------------------------------------
+
+@pytest.mark.parametrize("eps", [1e-3, -2e-3])
+def test_bulk_viscosity_closed_form_unit_cube(eps):
+    """A unit cube stretched uniformly from F0 = I to F1 = (1+eps) I in one step dt: trD = 3 eps/dt,
+    q = rho/J h (c1^2 h |trD| min(0, trD) + cl c0 trD), P_vis = J q F^-T, and the viscous nodal force is
+    P_vis int grad phi_a = J q/(1+eps) s_a/4 (s_a the corner's outward signs); expansion only uses the linear
+    term, compression both."""
+    m = box(1, 1, 1)
+    N = m["n_nodes"]
+    X = m["X"].reshape(3, N)
+    rho, dt, c1, cl = 1000.0, 1e-4, 0.3, 1.5
+    p0 = O.OracleProblem(m, rho=rho, n_steps=1)
+    pv = O.OracleProblem(m, rho=rho, n_steps=1, visc=O.VISC_BULK, visc_c1=c1, visc_cl=cl)
+    u = (eps * X).ravel()
+    fv = pv.internal_force_visc(u, np.zeros(3 * N), dt) - p0.internal_force(u)
+    c0 = O.lib().orc_sound_speed(1e6, 0.3, rho)
+    h = 1.0  # the cube's undeformed characteristic length
+    J = (1 + eps) ** 3
+    trD = 3 * eps / dt
+    q = rho / J * h * (c1 * c1 * h * abs(trD) * min(0.0, trD) + cl * c0 * trD)
+    s = np.sign(X - 0.5)
+    np.testing.assert_allclose(fv.reshape(3, N), J * q / (1 + eps) * s / 4, rtol=1e-9, atol=1e-9 * abs(q))
+    # a general homogeneous step F0 = I -> F1 = G: P_vis = det(G) q G^-T
+    G = np.array([[1.0 + eps, 0.3 * eps, -0.2 * eps], [0.1 * eps, 1.0 - 0.5 * eps, 0.4 * eps],
+                  [-0.3 * eps, 0.2 * eps, 1.0 + 2 * eps]])
+    ug = ((G - np.eye(3)) @ X).ravel()
+    fg = pv.internal_force_visc(ug, np.zeros(3 * N), dt) - p0.internal_force(ug)
+    trG = np.trace(G - np.eye(3)) / dt
+    JG = np.linalg.det(G)
+    qg = rho / JG * h * (c1 * c1 * h * abs(trG) * min(0.0, trG) + cl * c0 * trG)
+    Pg = JG * qg * np.linalg.inv(G).T
+    np.testing.assert_allclose(fg.reshape(3, N), Pg @ s / 4, rtol=1e-8, atol=1e-8 * np.abs(Pg).max())
+    # no rate, no viscous force
+    assert np.array_equal(pv.internal_force_visc(u, u, dt), p0.internal_force(u))
+    # explicit element size and sound speed replace the element's own
+    pw = O.OracleProblem(m, rho=rho, n_steps=1, visc=O.VISC_BULK, visc_c1=c1, visc_cl=cl, visc_h=0.5, visc_c0=7.0)
+    fw = pw.internal_force_visc(u, np.zeros(3 * N), dt) - p0.internal_force(u)
+    q2 = rho / J * 0.5 * (c1 * c1 * 0.5 * abs(trD) * min(0.0, trD) + cl * 7.0 * trD)
+    np.testing.assert_allclose(fw.reshape(3, N), J * q2 / (1 + eps) * s / 4, rtol=1e-9, atol=1e-9 * abs(q2))
+
+
+def test_viscosity_dissipates_and_dev_variant_adds_nothing():
+    """Free vibration from an initial velocity field: without viscosity E_def + E_kin keeps its level up to the
+    scheme's per-step wobble; with bulk viscosity it loses a good part of the initial kinetic energy; the
+    deviatoric variant leaves every bit unchanged (its stress never reaches the PK1).  (The reference's
+    defaults c1 = 0.06, cl = 1.2 are unstable at a scale factor of 0.5 once h and c0 are the element's: the
+    linear term alone needs dt <= (sqrt(1 + cl^2) - cl) h/c.)"""
+    m = perturb(box(3, 3, 3), 0.1, 31)
+    N = m["n_nodes"]
+    X = m["X"].reshape(3, N)
+    v0 = np.concatenate([0.02 * (X[0] - 0.5), -0.01 * (X[1] - 0.5), 0.03 * np.sin(np.pi * X[2])])
+    runs = {}
+    for visc in (O.VISC_NONE, O.VISC_BULK, O.VISC_DEV):
+        p = O.OracleProblem(m, n_steps=300, bc_mode=O.BC_VELOCITY, energy_every=1, visc=visc, visc_c1=0.5,
+                            visc_cl=0.06)
+        runs[visc] = p.run(v0=v0, hist_cap=300)
+        assert runs[visc]["rc"] == 0
+    for k in ("u", "v", "a"):
+        assert np.array_equal(runs[O.VISC_DEV][k], runs[O.VISC_NONE][k])
+    E = {k: r["hist"]["e_def"] + r["hist"]["e_kin"] for k, r in runs.items()}
+    ke0 = runs[O.VISC_NONE]["hist"]["e_kin"][0]
+    assert abs(E[O.VISC_NONE][-50:].mean() - E[O.VISC_NONE][0]) < 0.1 * ke0
+    assert E[O.VISC_BULK][-50:].mean() - E[O.VISC_BULK][0] < -0.25 * ke0

@@ -154,6 +154,21 @@ int mps_set_loads(mps_handle h, int32_t n_loads, const mps_load *loads);
  * Replaces: NeumannBCManagement (boundaryConditions.cpp:75-140). */
 int mps_external_force(mps_handle h, double t, double *fext);
 
+/* ---- artificial viscosity (*Bulk viscosity / *Dev viscosity of a *SOLVER block) ---- */
+#define MPS_VISC_NONE 0
+#define MPS_VISC_BULK 1 /* bulkViscosity (solverExplicit.cpp:505-542); reader defaults c1 = 0.06, cl = 1.2 */
+#define MPS_VISC_DEV 2  /* artificialViscosity (:462-503): its deviatoric stress never reaches the PK1, so it
+                           adds no force; accepted and run as MPS_VISC_NONE */
+
+/* Artificial viscosity of the solver (default none).  Valid only before the first step.  With MPS_VISC_BULK
+ * every Gauss point adds the PK1 q cof(F), q = rho/J h (c1^2 h |trD| min(0, trD) + cl c0 trD),
+ * trD = tr(F - F_prev)/dt, F_prev the point's F at the previous step (the state handed to the solver at
+ * its first step).  The reference never initialises its element size and c0 (solvers.h:53-54): here they
+ * are elem_size and c0 when > 0, else each element's undeformed characteristic length and sound speed.
+ * mps_internal_force returns the elastic part only.
+ * Replaces: the viscosity pass of computeInternalForce (solverExplicit.cpp:381-421). */
+int mps_set_viscosity(mps_handle h, int32_t type, double c1, double cl, double elem_size, double c0);
+
 /* Copy the current u, v, a to host (any pointer may be NULL).
  * Replaces: solvers::getFinalValues (solvers.h:171-187). */
 int mps_get_state(mps_handle h, double *u, double *v, double *a);

@@ -203,6 +203,29 @@ def test_reads_force_and_gravity(tmp_path):
     assert rc == 0 and j["loads"] == []
 
 
+@pytest.mark.parametrize("lines,expect", [
+    (["*Bulk viscosity", "0.3, 2.5"], (1, 0.3, 2.5)),
+    (["*Bulk viscosity", "0.7"], (1, 0.06, 0.7)),         # one number closes cl (the last character)
+    (["*Bulk viscosity"], (1, 0.06, 1.2)),                # defaults; the next keyword is still read
+    (["*Dev viscosity", "2, 0.25"], (2, 2.0, 0.25)),
+    ([], (0, 0.0, 0.0)),
+])
+def test_viscosity_keywords(tmp_path, lines, expect):
+    """solverManagement (inputFile.cpp:2462-2510): *Bulk viscosity / *Dev viscosity with an optional "c1, cl" line,
+    per solver (reset at every *SOLVER, :2316-2318)."""
+    m, bc = cube(2)
+    p = tmp_path / "v.inp"
+    inpgen.write_inp(p, m, [dict(rho=1000.0, E=1e6, nu=0.3)],
+                     [dict(scale=0.5, t0=0.0, tf=0.01, outputs=1, bc=bc, pre_lines=lines),
+                      dict(scale=0.5, t0=0.01, tf=0.02, outputs=1, bc=bc)])
+    rc, j, err = check(p)
+    assert rc == 0, err
+    s0, s1 = j["solvers"]
+    assert (s0["visc"], s0["visc_c1"], s0["visc_cl"]) == expect
+    assert s1["visc"] == 0
+    assert (s0["outputs"], s0["t0"]) == (1, 0.0)
+
+
 def test_coordinates_round_trip(tmp_path):
     """The reader gets every coordinate bit for bit (written with %.17g) and every connectivity entry."""
     m = configs.perturb(configs.box(6, 5, 4), 0.1, 7)

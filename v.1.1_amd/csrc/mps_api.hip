@@ -263,6 +263,7 @@ struct mps_solver_s {
     double* pick_out = nullptr;
     int64_t pick_cap = 0;
     std::vector<void*> load_allocs;  // mps_set_loads buffers (replaced by the next call)
+    double* uprev_buf = nullptr;     // previous-step displacement for bulk viscosity (in allocs)
     int energy_nb = 0;
     bool primed = false;       // prologue (first dt) done
     long long next_step = 1;   // host mirror of ctl->A
@@ -519,7 +520,34 @@ extern "C" int mps_set_state(mps_handle h, const double* u, const double* v, con
         if (src[i]) HCHK(h, hipMemcpyAsync(dst[i], src[i], bytes, hipMemcpyHostToDevice, h->stream));
         else HCHK(h, hipMemsetAsync(dst[i], 0, bytes, h->stream));
     }
+    // the handed-over u is also the previous state of the first step (solverExplicit.cpp:149)
+    if (h->I.uprev) HCHK(h, hipMemcpyAsync(h->I.uprev, h->I.u, bytes, hipMemcpyDeviceToDevice, h->stream));
     HCHK(h, hipStreamSynchronize(h->stream));
+    return MPS_OK;
+}
+
+extern "C" int mps_set_viscosity(mps_handle h, int32_t type, double c1, double cl, double elem_size, double c0) {
+    if (!h) return MPS_ERR_ARG;
+    if (h->primed || h->next_step != 1)
+        return h->fail(MPS_ERR_STATE, "mps_set_viscosity: only valid before the first step");
+    if (type < MPS_VISC_NONE || type > MPS_VISC_DEV) return h->fail(MPS_ERR_ARG, "mps_set_viscosity: unknown type");
+    if (!std::isfinite(c1) || !std::isfinite(cl) || !std::isfinite(elem_size) || !std::isfinite(c0))
+        return h->fail(MPS_ERR_ARG, "mps_set_viscosity: parameters must be finite");
+    MpsInst& I = h->I;
+    if (type != MPS_VISC_BULK) {  // the deviatoric variant adds no force
+        I.uprev = nullptr;
+        return MPS_OK;
+    }
+    (void)hipSetDevice(h->device);
+    const size_t bytes = 3 * (size_t)I.N * 8;
+    if (!h->uprev_buf) HCHK(h, dalloc(h, &h->uprev_buf, bytes));
+    HCHK(h, hipMemcpyAsync(h->uprev_buf, I.u, bytes, hipMemcpyDeviceToDevice, h->stream));
+    HCHK(h, hipStreamSynchronize(h->stream));
+    I.uprev = h->uprev_buf;
+    I.visc_c1 = c1;
+    I.visc_cl = cl;
+    I.visc_h = elem_size;
+    I.visc_c0 = c0;
     return MPS_OK;
 }
 

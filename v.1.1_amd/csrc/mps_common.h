@@ -31,6 +31,7 @@
 //     sgd           double[MPS_SGP_STRIDE*n_sgp]  bulk dphi (24), n0 (3), w|J_s| (1), scatter phi (8)
 //     sblk          double[4*blocks]  block type, P, t0, tf
 //     pc            double[24*n_sgp]  the step's surface forces [(g*8 + a)*3 + dir]
+//   uprev           double[3N]   displacement of the previous step (bulk viscosity only; NULL otherwise)
 //   ctl             MpsCtl       step control (time, dt, go flags)
 //   dtred           u64[2*MPS_DT_SLOTS*MPS_DT_STRIDE]  stable-dt reduction slots by step parity
 #pragma once
@@ -122,6 +123,10 @@ struct MpsInst {
     const double* sgd;
     const double* sblk;
     double* pc;
+    // bulk artificial viscosity (solverExplicit.cpp:381-421, 505-542): active when uprev != NULL
+    double visc_c1, visc_cl;   // quadratic and linear coefficients
+    double visc_h, visc_c0;    // element size and sound speed (> 0), or <= 0: the element's l0 and c
+    double* uprev;
 };
 
 // One unit of work for the persistent batched kernels: a contiguous range of

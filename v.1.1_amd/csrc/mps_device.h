@@ -508,11 +508,45 @@ __device__ __forceinline__ void elem_mat(const MpsInst& I, int e, double& pa, do
 }
 
 // ---------------------------------------------------------------------------
-// Hex8 element: nodal forces f[a][i] (T) and the element dt contribution (FP64).
+// Bulk artificial viscosity at one Gauss point (bulkViscosity, solverExplicit.cpp:513-542):
+// trD = tr(F1 - F0)/dt, q = rho/J h (c1^2 h |trD| min(0, trD) + cl c0 trD) and the PK1
+// -J (-q I) F^-T = q cof(F), added to P.  qa = rho h c1^2 h and qb = rho h cl c0 per element.
 // ---------------------------------------------------------------------------
-template <typename T, int LAW>
+template <typename T>
+__device__ __forceinline__ void add_bulk_visc(const T* H, T trD, T qa, T qb, T* P) {
+    const T J = T(1) + detIpH_m1(H);
+    const T mn = trD < T(0) ? trD : T(0);
+    const T q = (qa * fabs(trD) * mn + qb * trD) / J;
+    T F[9], cF[9];
+#pragma unroll
+    for (int i = 0; i < 9; i++) F[i] = H[i];
+    F[0] += T(1);
+    F[4] += T(1);
+    F[8] += T(1);
+    cof3(F, cF);
+#pragma unroll
+    for (int i = 0; i < 9; i++) P[i] += q * cF[i];
+}
+
+// Viscosity coefficients of element e (qa, qb of add_bulk_visc); element size and sound speed are the
+// element's undeformed length and sound speed unless given.
+__device__ __forceinline__ void visc_coeffs(const MpsInst& I, int e, double c, double* qa, double* qb) {
+    const double rho = I.rho ? I.rho[e] : I.rho0;
+    const double h = I.visc_h > 0 ? I.visc_h : I.l0[e];
+    const double c0 = I.visc_c0 > 0 ? I.visc_c0 : c;
+    *qa = rho * h * I.visc_c1 * I.visc_c1 * h;
+    *qb = rho * h * I.visc_cl * c0;
+}
+
+// ---------------------------------------------------------------------------
+// Hex8 element: nodal forces f[a][i] (T) and the element dt contribution (FP64).
+// VISC adds bulk viscosity from the displacement increment u - uprev over the step (rdt = 1/dt, 0 when
+// dt = 0); its 7 butterfly coefficient vectors wait in the element's LDS force slots vs[m*3+d] (stride
+// MPS_PATCH), which the forces overwrite only after the Gauss-point loop.
+// ---------------------------------------------------------------------------
+template <typename T, int LAW, bool VISC = false>
 __device__ __forceinline__ void hex8_element(const MpsInst& I, int e, const double* __restrict__ u, T (*f)[3],
-                                             double* dte) {
+                                             double* dte, T* vs = nullptr, double rdt = 0.0) {
     const int32_t* cn = I.conn + (size_t)e * 8;
     int4 q0 = *reinterpret_cast<const int4*>(cn);
     int4 q1 = *reinterpret_cast<const int4*>(cn + 4);
@@ -538,6 +572,26 @@ __device__ __forceinline__ void hex8_element(const MpsInst& I, int e, const doub
             for (int d = 0; d < 3; d++) x[a][d] = u[d * N + nd[a]];
         hex_butterfly(x, cx);
         hex_scale<T>(cx, Au);
+    }
+    T qa = T(0), qb = T(0), trs = T(0);
+    if (VISC) {
+        double x[8][3], cx[7][3];
+        T dA[7][3];
+#pragma unroll
+        for (int a = 0; a < 8; a++)
+#pragma unroll
+            for (int d = 0; d < 3; d++) x[a][d] = u[d * N + nd[a]] - I.uprev[d * N + nd[a]];
+        hex_butterfly(x, cx);
+        hex_scale<T>(cx, dA);
+#pragma unroll
+        for (int m = 0; m < 7; m++)
+#pragma unroll
+            for (int d = 0; d < 3; d++) vs[(m * 3 + d) * MPS_PATCH] = dA[m][d];
+        double da, db;
+        visc_coeffs(I, e, c, &da, &db);
+        qa = (T)da;
+        qb = (T)db;
+        trs = (T)rdt;
     }
 #pragma unroll
     for (int m = 0; m < 7; m++)
@@ -572,6 +626,19 @@ __device__ __forceinline__ void hex8_element(const MpsInst& I, int e, const doub
             for (int j = 0; j < 3; j++)
                 H[i * 3 + j] = (Du[i * 3] * cJ[j * 3] + Du[i * 3 + 1] * cJ[j * 3 + 1] + Du[i * 3 + 2] * cJ[j * 3 + 2]) * rJ;
         pk1<LAW>(H, tpa, tpb, P);
+        if (VISC) {  // tr of grad(u - uprev) = sum_ik Dd(i,k) cof(J)(i,k) / det J
+            T tr = T(0);
+#pragma unroll
+            for (int i = 0; i < 3; i++) {
+                const T a0 = vs[(0 * 3 + i) * MPS_PATCH], a1 = vs[(1 * 3 + i) * MPS_PATCH], a2 = vs[(2 * 3 + i) * MPS_PATCH];
+                const T a3 = vs[(3 * 3 + i) * MPS_PATCH], a4 = vs[(4 * 3 + i) * MPS_PATCH], a5 = vs[(5 * 3 + i) * MPS_PATCH];
+                const T a6 = vs[(6 * 3 + i) * MPS_PATCH];
+                tr += (a0 + sy * a3 + sz * a5 + (sy * sz) * a6) * cJ[i * 3 + 0] +
+                      (a1 + sx * a3 + sz * a4 + (sx * sz) * a6) * cJ[i * 3 + 1] +
+                      (a2 + sx * a5 + sy * a4 + (sx * sy) * a6) * cJ[i * 3 + 2];
+            }
+            add_bulk_visc<T>(H, tr * rJ * trs, qa, qb, P);
+        }
         // Q = wJ P J^-T: Q(i,k) = sign(det J) sum_J P(i,J) cof(J)(J,k); columns Q(:,k).  The sign is
         // folded into P only when some lane of the wave has det J <= 0 (a positively oriented
         // mesh never takes that branch).
@@ -632,9 +699,9 @@ __device__ __forceinline__ void hex8_element(const MpsInst& I, int e, const doub
 // Tet4 element (1 Gauss point): classLinearTetrahedra.cpp:171-292.
 // J(i,j) = x_{j+1,i} - x_{0,i}; dphi_0 = -(sum of J^-1 rows), dphi_a = row a-1 of J^-1.
 // ---------------------------------------------------------------------------
-template <typename T, int LAW>
+template <typename T, int LAW, bool VISC = false>
 __device__ __forceinline__ void tet4_element(const MpsInst& I, int e, const double* __restrict__ u, T (*f)[3],
-                                             double* dte) {
+                                             double* dte, double rdt = 0.0) {
     const int4 q = *reinterpret_cast<const int4*>(I.conn + (size_t)e * 4);
     const int nd[4] = {q.x, q.y, q.z, q.w};
     const size_t N = (size_t)I.N;
@@ -673,6 +740,20 @@ __device__ __forceinline__ void tet4_element(const MpsInst& I, int e, const doub
 #pragma unroll
         for (int j = 0; j < 3; j++) H[i * 3 + j] = Du[i * 3] * iJ[j] + Du[i * 3 + 1] * iJ[3 + j] + Du[i * 3 + 2] * iJ[6 + j];
     pk1<LAW>(H, (T)pa, (T)pb, P);
+    if (VISC) {  // tr of grad(u - uprev) = sum_ik Dd(i,k) J^-1(k,i)
+        T Dd[9];
+#pragma unroll
+        for (int i = 0; i < 3; i++)
+#pragma unroll
+            for (int j = 0; j < 3; j++)
+                Dd[i * 3 + j] = (T)((uu[j + 1][i] - I.uprev[i * N + nd[j + 1]]) - (uu[0][i] - I.uprev[i * N + nd[0]]));
+        T tr = T(0);
+#pragma unroll
+        for (int i = 0; i < 3; i++) tr += Dd[i * 3] * iJ[i] + Dd[i * 3 + 1] * iJ[3 + i] + Dd[i * 3 + 2] * iJ[6 + i];
+        double da, db;
+        visc_coeffs(I, e, c, &da, &db);
+        add_bulk_visc<T>(H, tr * (T)rdt, (T)da, (T)db, P);
+    }
     T dphi[4][3];
 #pragma unroll
     for (int j = 0; j < 3; j++) {

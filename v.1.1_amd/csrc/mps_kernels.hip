@@ -24,16 +24,17 @@ using namespace mpsd;
 // force per node to part.  No float atomics: the node kernel adds the partials of a node in
 // ascending patch order, so the sum order is fixed.  Returns the block's min element dt
 // candidate (64-bit pattern, lanes other than those computing elements hold +inf).
-template <typename T, int NEN, int LAW, int BS>
-__device__ __forceinline__ unsigned long long patch_forces(const MpsInst& I, int p, bool want_dt, T* __restrict__ sf) {
+template <typename T, int NEN, int LAW, int BS, bool VISC = false>
+__device__ __forceinline__ unsigned long long patch_forces(const MpsInst& I, int p, bool want_dt, T* __restrict__ sf,
+                                                           double rdt = 0.0) {
     const int e0 = p * MPS_PATCH;
     const int ne = min(MPS_PATCH, I.Ne - e0);
     unsigned long long dmin = INF_BITS;
     for (int i = threadIdx.x; i < ne; i += BS) {
         T f[NEN][3];
         double dte = bits2d(INF_BITS);
-        if (NEN == 8) hex8_element<T, LAW>(I, e0 + i, I.u, (T(*)[3])f, want_dt ? &dte : nullptr);
-        else tet4_element<T, LAW>(I, e0 + i, I.u, (T(*)[3])f, want_dt ? &dte : nullptr);
+        if (NEN == 8) hex8_element<T, LAW, VISC>(I, e0 + i, I.u, (T(*)[3])f, want_dt ? &dte : nullptr, sf + i, rdt);
+        else tet4_element<T, LAW, VISC>(I, e0 + i, I.u, (T(*)[3])f, want_dt ? &dte : nullptr, rdt);
 #pragma unroll
         for (int a = 0; a < NEN; a++)
 #pragma unroll
@@ -66,18 +67,22 @@ __device__ __forceinline__ unsigned long long patch_forces(const MpsInst& I, int
 // mode 0: step (reads ctl->A, writes ctl->B, reduces dt for step k+1)
 // mode 1: standalone force evaluation (no ctl access)
 // One workgroup per patch, one element per work-item.
-template <typename T, int NEN, int LAW>
+// VISC (mode 0 only): bulk artificial viscosity over the step, from u - uprev and the step's dt.
+template <typename T, int NEN, int LAW, bool VISC = false>
 __global__ __launch_bounds__(MPS_PATCH) void k_elem(MpsInst I, int mode) {
     __shared__ T sf[NEN * 3 * MPS_PATCH];
     __shared__ unsigned long long sh[MPS_PATCH / 64];
     long long k = 0;
+    double rdt = 0.0;
     if (mode == 0) {
         k = I.ctl->A;
         if (blockIdx.x == 0 && threadIdx.x == 0) I.ctl->B = k;
-        if (!I.ctl->slot[k & 1].go) return;
+        const MpsStepSlot s = I.ctl->slot[k & 1];
+        if (!s.go) return;
+        if (VISC) rdt = (s.dt == 0.0) ? 0.0 : 1 / s.dt;
     }
     const int p = xcd_tile(blockIdx.x, gridDim.x);
-    unsigned long long m = patch_forces<T, NEN, LAW, MPS_PATCH>(I, p, mode == 0, sf);
+    unsigned long long m = patch_forces<T, NEN, LAW, MPS_PATCH, VISC>(I, p, mode == 0, sf, rdt);
     if (mode == 0) {
         m = block_min_u64(m, sh);
         if (threadIdx.x == 0) atomicMin(dt_slot(I, (k + 1) & 1, blockIdx.x & (MPS_DT_SLOTS - 1)), m);
@@ -167,7 +172,13 @@ __device__ __forceinline__ void node_item(const MpsInst& I, int n, bool first, i
             atomicMin((unsigned long long*)&ctl->nan_step, (unsigned long long)k);
         }
     }
-    if (fuse && nx.go) predict_node(I, n, k + 1, nx.dt, u, v, a);
+    if (fuse && nx.go) {
+        if (I.uprev) {  // the F of step k becomes the previous state of step k+1 (nextStep)
+#pragma unroll
+            for (int i = 0; i < 3; i++) I.uprev[i * N + n] = u[i];
+        }
+        predict_node(I, n, k + 1, nx.dt, u, v, a);
+    }
 #pragma unroll
     for (int i = 0; i < 3; i++) {
         I.u[i * N + n] = u[i];
@@ -217,6 +228,10 @@ __device__ __forceinline__ void begin_item(const MpsInst& I, int n, bool first) 
         u[i] = I.u[i * N + n];
         v[i] = I.v[i * N + n];
         a[i] = I.a[i * N + n];
+    }
+    if (I.uprev) {
+#pragma unroll
+        for (int i = 0; i < 3; i++) I.uprev[i * N + n] = u[i];
     }
     predict_node(I, n, k, nx.dt, u, v, a);
 #pragma unroll
@@ -734,14 +749,21 @@ static inline unsigned blocks_for(long long n, int bs) { return (unsigned)((n + 
 hipError_t launch_elem(const MpsInst& I, int mode, hipStream_t st) {
     const unsigned nb = (unsigned)I.n_patch;
     if (nb == 0) return hipSuccess;
-#define MPS_ELEM_LAUNCH(T, NEN)                                                   \
-    do {                                                                          \
-        switch (I.law) {                                                          \
-            case 0: k_elem<T, NEN, 0><<<nb, MPS_PATCH, 0, st>>>(I, mode); break;  \
-            case 1: k_elem<T, NEN, 1><<<nb, MPS_PATCH, 0, st>>>(I, mode); break;  \
-            case 2: k_elem<T, NEN, 2><<<nb, MPS_PATCH, 0, st>>>(I, mode); break;  \
-            default: k_elem<T, NEN, 3><<<nb, MPS_PATCH, 0, st>>>(I, mode); break; \
-        }                                                                         \
+    const bool visc = mode == 0 && I.uprev != nullptr;
+#define MPS_ELEM_LAW(T, NEN, V)                                                      \
+    switch (I.law) {                                                                 \
+        case 0: k_elem<T, NEN, 0, V><<<nb, MPS_PATCH, 0, st>>>(I, mode); break;      \
+        case 1: k_elem<T, NEN, 1, V><<<nb, MPS_PATCH, 0, st>>>(I, mode); break;      \
+        case 2: k_elem<T, NEN, 2, V><<<nb, MPS_PATCH, 0, st>>>(I, mode); break;      \
+        default: k_elem<T, NEN, 3, V><<<nb, MPS_PATCH, 0, st>>>(I, mode); break;     \
+    }
+#define MPS_ELEM_LAUNCH(T, NEN)                      \
+    do {                                             \
+        if (visc) {                                  \
+            MPS_ELEM_LAW(T, NEN, true)               \
+        } else {                                     \
+            MPS_ELEM_LAW(T, NEN, false)              \
+        }                                            \
     } while (0)
     if (I.nen == 8) {
         if (I.precision == 1) MPS_ELEM_LAUNCH(float, 8);
@@ -751,6 +773,7 @@ hipError_t launch_elem(const MpsInst& I, int mode, hipStream_t st) {
         else MPS_ELEM_LAUNCH(double, 4);
     }
 #undef MPS_ELEM_LAUNCH
+#undef MPS_ELEM_LAW
     return hipGetLastError();
 }
 

@@ -368,7 +368,30 @@ bool readSolvers(Reader &R, InpModel &m) {
                 else
                     return R.fail(i, "*MPS Precision is FP64 or FP32_ELEM");
                 i++;
-            } else if (starts(l, "*Bulk viscosity") || starts(l, "*Dev viscosity") || starts(l, "*ACTIVATION") ||
+            } else if (starts(l, "*Bulk viscosity") || starts(l, "*Dev viscosity")) {
+                // solverManagement (inputFile.cpp:2462-2510): defaults, then the next line's characters:
+                // spaces skipped, '*' ends it (no parameters), ',' closes c1, the last character closes cl.
+                // The reference consumes that line even when it is a keyword; here a keyword is left to be read.
+                const bool bulk = starts(l, "*Bulk viscosity");
+                s.visc = bulk ? MPS_VISC_BULK : MPS_VISC_DEV;
+                s.viscC1 = bulk ? 0.06 : 1;
+                s.viscCl = bulk ? 1.2 : 0.5;
+                if (i + 1 < R.L.size() && !starts(R.L[i + 1], "*")) {
+                    const std::string &t = R.L[++i];
+                    std::string tok;
+                    for (size_t c = 0; c < t.size(); c++) {
+                        if (t[c] == '*') break;
+                        if (t[c] == ' ') continue;
+                        if (t[c] == ',') {
+                            s.viscC1 = std::atof(tok.c_str());
+                            tok.clear();
+                        } else {
+                            tok += t[c];
+                            if (c == t.size() - 1) s.viscCl = std::atof(tok.c_str());
+                        }
+                    }
+                }
+            } else if (starts(l, "*ACTIVATION") ||
                        starts(l, "*FSI") || starts(l, "*ADAPTIVE TIME STEP") || starts(l, "*TIME STEPPING PLAN") ||
                        starts(l, "*Forces")) {
                 return R.fail(i, "solver option not on this path");

@@ -9,6 +9,7 @@
 //   *SOLVER, EXPLICIT ... *END SOLVER                  solverManagement inputFile.cpp:2253-2762
 //   *BOUNDARY, TYPE=DISPLACEMENT RAMP|INST             dispBCManagement inputFile.cpp:2960-3217
 //   *Outputs                    outManagement          inputFile.cpp:4769-4829
+//   *Bulk viscosity / *Dev viscosity  solverManagement    inputFile.cpp:2462-2510
 //   *BOUNDARY, TYPE=FORCE       forceBCManagement      inputFile.cpp:2841-2941  (anywhere in the file,
 //   *BOUNDARY, TYPE=PRESSURE RAMP|INST  pressure*BCManagement inputFile.cpp:3580-3790  in order; global
 //   *GRAVITY                    NeumannCommonManagement inputFile.cpp:4703-4735   for all solvers)
@@ -59,6 +60,8 @@ struct InpSolver {
     int velRampSteps = 0;
     int64_t fixedSteps = 0;
     int precision = 0;
+    int visc = 0;                  // *Bulk viscosity (MPS_VISC_BULK) / *Dev viscosity (MPS_VISC_DEV)
+    double viscC1 = 0, viscCl = 0; // their "c1, cl" line (defaults 0.06, 1.2 / 1, 0.5)
     std::vector<InpDirichlet> dirichlet;
 };
 

@@ -66,9 +66,9 @@ static int check(const std::string &path) {
         const mps::InpSolver &s = m.solvers[k];
         std::printf("%s{\"scale\": %.17g, \"t0\": %.17g, \"tf\": %.17g, \"outputs\": %d, \"bc_mode\": %d, "
                     "\"dirichlet\": %zu, \"fixed_steps\": %lld, \"vel_ramp_steps\": %d, \"precision\": %d, "
-                    "\"out_vars\": [",
+                    "\"visc\": %d, \"visc_c1\": %.17g, \"visc_cl\": %.17g, \"out_vars\": [",
                     k ? ", " : "", s.scale, s.t0, s.tf, s.numFiles, s.bcMode, s.dirichlet.size(),
-                    (long long)s.fixedSteps, s.velRampSteps, s.precision);
+                    (long long)s.fixedSteps, s.velRampSteps, s.precision, s.visc, s.viscC1, s.viscCl);
         for (size_t j = 0; j < s.outVars.size(); j++) std::printf("%s%d", j ? ", " : "", s.outVars[j]);
         std::printf("]}");
     }

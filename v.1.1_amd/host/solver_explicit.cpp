@@ -118,6 +118,9 @@ int classSolverExplicit::calculate(const InpModel &m, const std::vector<std::uni
                           _a0.empty() ? nullptr : _a0.data()) != MPS_OK)
             return fail("mps_set_state");
     }
+    // artificial viscosity of this solver (solverExplicit.cpp:381-421)
+    if (_s.visc != 0 && mps_set_viscosity(h, _s.visc, _s.viscC1, _s.viscCl, 0.0, 0.0) != MPS_OK)
+        return fail("mps_set_viscosity");
     // Neumann loads, global for all solvers (solverExplicit.cpp:47; NeumannBCManagement at every step)
     const InpLoads &ld = m.loads;
     if (!ld.empty()) {

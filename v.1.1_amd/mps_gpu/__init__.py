@@ -50,6 +50,7 @@ class Load(C.Structure):
 
 
 LOAD_FORCE, LOAD_GRAVITY, LOAD_PRESSURE_RAMP, LOAD_PRESSURE_INST = 0, 1, 2, 3
+VISC_NONE, VISC_BULK, VISC_DEV = 0, 1, 2
 _LOAD_TYPES = {"force": LOAD_FORCE, "gravity": LOAD_GRAVITY, "pressure_ramp": LOAD_PRESSURE_RAMP,
                "pressure_inst": LOAD_PRESSURE_INST}
 
@@ -128,6 +129,7 @@ def load(path=None):
     L.mps_critical_dt.argtypes = [H, _dp, _dp]
     L.mps_element_outputs.argtypes = [H, _dp]
     L.mps_set_loads.argtypes = [H, C.c_int32, C.POINTER(Load)]
+    L.mps_set_viscosity.argtypes = [H, C.c_int32, C.c_double, C.c_double, C.c_double, C.c_double]
     L.mps_external_force.argtypes = [H, C.c_double, _dp]
     L.mps_get_dofs.argtypes = [H, C.c_int32, C.c_int64, _ip, _dp]
     L.mps_destroy.argtypes = [H]
@@ -274,6 +276,10 @@ class Solver:
                 Lb.n, Lb.value, Lb.t0, Lb.tf, Lb.index, Lb.face = el.size, blk[1], blk[2], blk[3], _i(el), _i(fc)
         self._loads = keep
         self._chk(self.L.mps_set_loads(self.h, len(blocks), arr))
+
+    def set_viscosity(self, kind, c1=0.06, cl=1.2, elem_size=0.0, c0=0.0):
+        """kind: VISC_NONE / VISC_BULK / VISC_DEV (*Bulk viscosity / *Dev viscosity)."""
+        self._chk(self.L.mps_set_viscosity(self.h, kind, c1, cl, elem_size, c0))
 
     def external_force(self, t):
         f = np.zeros(3 * self.n)

@@ -89,6 +89,8 @@ typedef struct mps_bcs {
 #define MPS_LOAD_GRAVITY 1       /* *GRAVITY (classGravity; inputFile.cpp:4723-4735) */
 #define MPS_LOAD_PRESSURE_RAMP 2 /* *BOUNDARY, TYPE=PRESSURE RAMP (classPressureRamp; inputFile.cpp:3580-3684) */
 #define MPS_LOAD_PRESSURE_INST 3 /* *BOUNDARY, TYPE=PRESSURE INST (classPressureInst; inputFile.cpp:3686-3790) */
+#define MPS_LOAD_TRACTION_RAMP 4 /* *BOUNDARY, TYPE=TRACTION RAMP (classTractionRamp; inputFile.cpp:3342-3460) */
+#define MPS_LOAD_TRACTION_INST 5 /* *BOUNDARY, TYPE=TRACTION INST (classTractionInst; inputFile.cpp:3462-3578) */
 
 /* The external force of every step is zero plus the blocks in order (NeumannBCManagement,
  * boundaryConditions.cpp:75-140):
@@ -98,15 +100,18 @@ typedef struct mps_bcs {
  *   PRESSURE: face face[k] (getSubElement numbering: hex 1..6, tet 1..4) of element index[k] carries
  *             P(t) w |J_s| phi_a det(F) F^-T n0 per surface Gauss point, F of the bulk element at the point
  *             (BCsPressureRamp :370-452, BCsPressureInst :454-496); P(t) = value/(tf-t0) * (t-t0) (ramp) or
- *             value (inst) for t0 <= t <= tf, else 0 (classNeumannBCs.h:144-176). */
+ *             value (inst) for t0 <= t <= tf, else 0 (classNeumannBCs.h:144-176).
+ *   TRACTION: faces as for PRESSURE carry the dead load w |J_s| phi_a T_i(t) over the face's nodes
+ *             (BCsTractionRamp / Inst :218-271); T(t) = force[0..2]/(tf-t0) * (t-t0) (ramp) or force[0..2]
+ *             (inst) for t0 <= t <= tf, else 0 (classNeumannBCs.h:48-99). */
 typedef struct mps_load {
     int32_t type;         /* MPS_LOAD_* */
-    int32_t n;            /* FORCE: node entries; PRESSURE: faces; GRAVITY: 0 */
-    double t0, tf;        /* FORCE, PRESSURE: window (tf > t0; PRESSURE INST: tf >= t0) */
+    int32_t n;            /* FORCE: node entries; PRESSURE, TRACTION: faces; GRAVITY: 0 */
+    double t0, tf;        /* FORCE, PRESSURE, TRACTION: window (tf > t0; the INST types: tf >= t0) */
     double value;         /* GRAVITY: g; PRESSURE: P */
-    const int32_t *index; /* FORCE: node [n]; PRESSURE: bulk element [n] (0-based) */
-    const int32_t *face;  /* PRESSURE: face [n] */
-    const double *force;  /* FORCE: [3n] */
+    const int32_t *index; /* FORCE: node [n]; PRESSURE, TRACTION: bulk element [n] (0-based) */
+    const int32_t *face;  /* PRESSURE, TRACTION: face [n] */
+    const double *force;  /* FORCE: [3n]; TRACTION: [3] the traction vector */
 } mps_load;
 
 typedef struct mps_solver_cfg {

@@ -709,7 +709,8 @@ static int loads_init(orc_state *s, const double *xw)
                     l->tg[e * ng + g] = -w * s->rho[e] * J * L->value;
                 }
             }
-        } else if (L->type == ORC_LOAD_PRESSURE_RAMP || L->type == ORC_LOAD_PRESSURE_INST) {
+        } else if (L->type >= ORC_LOAD_PRESSURE_RAMP && L->type <= ORC_LOAD_TRACTION_INST) {
+            if (L->type >= ORC_LOAD_TRACTION_RAMP && !L->force) return ORC_ERR_ARG;
             const int per = (s->nen == 8) ? 4 : 1;
             const size_t ns = (size_t)L->n * per + 1;
             l->se = malloc(sizeof(int32_t) * ns);
@@ -1003,6 +1004,20 @@ static void add_pressure(orc_state *s, const orc_load *L, const struct load_stat
     }
 }
 
+/* BCsTractionRamp / BCsTractionInst (boundaryConditions.cpp:218-271): per surface GP of the faces,
+ * force += w |J_s| phi_a T_i over the face's nodes, T from getTraction (classNeumannBCs.h:48-99). */
+static void add_traction(orc_state *s, const orc_load *L, const struct load_state *l, double t)
+{
+    const size_t N = s->N;
+    const int nf = (s->nen == 8) ? 4 : 3, ramp = (L->type == ORC_LOAD_TRACTION_RAMP);
+    double T[3] = {0, 0, 0};
+    if (t >= L->t0 && t <= L->tf)
+        for (int i = 0; i < 3; i++) T[i] = ramp ? L->force[i] / (L->tf - L->t0) * (t - L->t0) : L->force[i];
+    for (int g = 0; g < l->ns; g++)
+        for (int a = 0; a < nf; a++)
+            for (int i = 0; i < 3; i++) s->fext[l->sn[4 * g + a] + N * i] += l->ws[g] * l->phs[4 * g + a] * T[i];
+}
+
 /* fext at time t for displacement u (NeumannBCManagement, boundaryConditions.cpp:75-140): zero, then every
  * block in input order -- nodal forces (BCsNodalForces, :1120-1134, with classNeumann::getTracAmount,
  * classNodes.h:101-108: F/(tf-t0) * t for t0 <= t <= tf, else 0), gravity and pressure. */
@@ -1020,6 +1035,8 @@ static void external_force(orc_state *s, const double *u, double t)
                 for (int d = 0; d < 3; d++) s->fext[L->index[k] + N * d] += in ? l->fm[3 * k + d] * t : 0.0;
         } else if (L->type == ORC_LOAD_GRAVITY) {
             add_gravity(s, l->tg);
+        } else if (L->type >= ORC_LOAD_TRACTION_RAMP) {
+            add_traction(s, L, l, t);
         } else {
             add_pressure(s, L, l, u, t);
         }

@@ -32,7 +32,8 @@ enum { ORC_SUM_REFERENCE = 0, ORC_SUM_ELEMENT = 1 };
 enum { ORC_OK = 0, ORC_ERR_DT = -1, ORC_ERR_NAN = -2, ORC_ERR_ARG = -3, ORC_ERR_MEM = -4 };
 
 enum { ORC_VISC_NONE = 0, ORC_VISC_BULK = 1, ORC_VISC_DEV = 2 };
-enum { ORC_LOAD_FORCE = 0, ORC_LOAD_GRAVITY = 1, ORC_LOAD_PRESSURE_RAMP = 2, ORC_LOAD_PRESSURE_INST = 3 };
+enum { ORC_LOAD_FORCE = 0, ORC_LOAD_GRAVITY = 1, ORC_LOAD_PRESSURE_RAMP = 2, ORC_LOAD_PRESSURE_INST = 3,
+       ORC_LOAD_TRACTION_RAMP = 4, ORC_LOAD_TRACTION_INST = 5 };
 
 /* One Neumann block.
  *   FORCE (forceBCManagement, inputFile.cpp:2841-2941): n nodes index[], forces force[3n] = F * Ff over [t0, tf];
@@ -40,14 +41,17 @@ enum { ORC_LOAD_FORCE = 0, ORC_LOAD_GRAVITY = 1, ORC_LOAD_PRESSURE_RAMP = 2, ORC
  *   GRAVITY (inputFile.cpp:4723-4735): value = g; f_z += -w rho |J| g phi_a per GP (boundaryConditions.cpp:1146-1172).
  *   PRESSURE RAMP / INST (pressure*BCManagement, inputFile.cpp:3580-3790): n faces, bulk element index[] and
  *     face[] (getSubElement numbering); value = P; P(t) = P/(tf-t0) * (t-t0) (ramp) or P (inst) inside
- *     [t0, tf], else 0 (classNeumannBCs.h:144-176); Nanson force, boundaryConditions.cpp:370-496. */
+ *     [t0, tf], else 0 (classNeumannBCs.h:144-176); Nanson force, boundaryConditions.cpp:370-496.
+ *   TRACTION RAMP / INST (traction*BCManagement, inputFile.cpp:3342-3578): faces as for pressure, force[3] the
+ *     traction T; T(t) = T/(tf-t0) * (t-t0) (ramp) or T (inst) inside [t0, tf], else 0 (classNeumannBCs.h:48-99);
+ *     force += w |J_s| phi_a T_i over the face's nodes, boundaryConditions.cpp:218-271. */
 typedef struct orc_load {
     int32_t type;          /* ORC_LOAD_* */
     int32_t n;
     double t0, tf, value;
     const int32_t *index;  /* FORCE: node; PRESSURE: bulk element (0-based) */
     const int32_t *face;   /* PRESSURE: face S, 1..6 (hex) or 1..4 (tet) */
-    const double *force;   /* FORCE: [3n] */
+    const double *force;   /* FORCE: [3n]; TRACTION: [3] */
 } orc_load;
 
 typedef struct orc_problem {

@@ -46,7 +46,7 @@ class Load(C.Structure):
                 ("value", C.c_double), ("index", _ip), ("face", _ip), ("force", _dp)]
 
 
-LOAD_FORCE, LOAD_GRAVITY, LOAD_PRESSURE_RAMP, LOAD_PRESSURE_INST = 0, 1, 2, 3
+LOAD_FORCE, LOAD_GRAVITY, LOAD_PRESSURE_RAMP, LOAD_PRESSURE_INST, LOAD_TRACTION_RAMP, LOAD_TRACTION_INST = 0, 1, 2, 3, 4, 5
 
 
 def load_blocks(forces=None, gravity=0.0, gravity_pos=-1, loads=None):
@@ -54,6 +54,7 @@ def load_blocks(forces=None, gravity=0.0, gravity_pos=-1, loads=None):
       ("force", nodes, vec[k,3], t0, tf)                 *BOUNDARY, TYPE=FORCE (entries F*Ff, one window)
       ("gravity", g)                                     *GRAVITY
       ("pressure_ramp" | "pressure_inst", P, t0, tf, elems, faces)
+      ("traction_ramp" | "traction_inst", (Tx, Ty, Tz), t0, tf, elems, faces)
     `loads` is such a list; otherwise forces = (node[k], vec[k,3], t0[k], tf[k]) gives one block per entry and
     a non-zero gravity is placed before entry gravity_pos (-1: after all)."""
     if loads is not None:
@@ -84,13 +85,20 @@ def pack_loads(blocks):
             L.index, L.force = iptr(nodes), dptr(vec)
         elif kind == "gravity":
             L.type, L.n, L.value = LOAD_GRAVITY, 0, blk[1]
-        elif kind in ("pressure_ramp", "pressure_inst"):
+        elif kind in ("pressure_ramp", "pressure_inst", "traction_ramp", "traction_inst"):
             el = np.ascontiguousarray(blk[4], dtype=np.int32)
             fc = np.ascontiguousarray(blk[5], dtype=np.int32)
             keep += [el, fc]
-            L.type = LOAD_PRESSURE_RAMP if kind == "pressure_ramp" else LOAD_PRESSURE_INST
-            L.n, L.value, L.t0, L.tf = el.size, blk[1], blk[2], blk[3]
+            L.type = {"pressure_ramp": LOAD_PRESSURE_RAMP, "pressure_inst": LOAD_PRESSURE_INST,
+                      "traction_ramp": LOAD_TRACTION_RAMP, "traction_inst": LOAD_TRACTION_INST}[kind]
+            L.n, L.t0, L.tf = el.size, blk[2], blk[3]
             L.index, L.face = iptr(el), iptr(fc)
+            if kind.startswith("traction"):
+                T = np.ascontiguousarray(blk[1], dtype=np.float64).reshape(3)
+                keep.append(T)
+                L.force = dptr(T)
+            else:
+                L.value = blk[1]
         else:
             raise ValueError(kind)
     return arr, keep

@@ -191,7 +191,7 @@ def boundary_faces(mesh, axis=2, value=None):
 
 @pytest.mark.parametrize("tet", [False, True])
 def test_pressure_external_force_matches_oracle(tet):
-    """Surface forces at a deformed state, every face of the body, both pressure types."""
+    """Surface forces at a deformed state, every face of the body, both pressure and both traction types."""
     m = perturb(box(3, 2, 2, tet=tet), 0.1, 17)
     N = m["n_nodes"]
     el, fc = boundary_faces(m)
@@ -199,7 +199,9 @@ def test_pressure_external_force_matches_oracle(tet):
     u = rng.uniform(-0.02, 0.02, 3 * N)
     blocks = [("pressure_ramp", 2e3, 0.1, 0.7, el[::2], fc[::2]), ("gravity", 9.81),
               ("force", [3, 7], [[1.0, 2.0, 3.0], [-4.0, 0.5, 0.0]], 0.0, 1.0),
-              ("pressure_inst", -7e2, 0.2, 0.9, el[1::2], fc[1::2])]
+              ("pressure_inst", -7e2, 0.2, 0.9, el[1::2], fc[1::2]),
+              ("traction_ramp", (1e2, -3e2, 2e2), 0.0, 0.8, el[::3], fc[::3]),
+              ("traction_inst", (-5e1, 0.0, 4e2), 0.3, 0.9, el[1::3], fc[1::3])]
     op = O.OracleProblem(m, rho=1100.0, n_steps=1, loads=blocks)
     with G.Solver(m, rho=1100.0, n_steps=1) as s:
         s.set_state(u)
@@ -251,3 +253,17 @@ def test_pressure_time_window_run():
                                      loads=blocks)
     assert ro["rc"] == 0 and rc == 0 and rep["steps"] == ro["steps"] and rep["time"] == ro["time"]
     assert relerr(u, ro["u"]) <= TOL64 and np.abs(u).max() > 0
+
+
+@pytest.mark.parametrize("tet", [False, True])
+def test_traction_run_fp64(tet):
+    """A clamped block with a shear traction ramp on its top face and an inward traction on one side."""
+    m = perturb(box(3, 3, 4, tet=tet), 0.08, 29)
+    top = boundary_faces(m, 2, 1.0)
+    side = boundary_faces(m, 0, 1.0)
+    blocks = [("traction_ramp", (3e4, 1e4, 0.0), 0.0, 0.2, top[0], top[1]),
+              ("traction_inst", (-2e4, 0.0, 0.0), 0.05, 0.15, side[0], side[1])]
+    ro, rc, rep, u, _v = loaded_pair(m, 200, None, 0.0, -1, loads=blocks)
+    err = relerr(u, ro["u"])
+    print(f"tet={tet} traction fp64 {err:.2e}, max|u| {np.abs(ro['u']).max():.3e}")
+    assert ro["rc"] == 0 and rc == 0 and err <= TOL64 and np.abs(ro["u"]).max() > 1e-5

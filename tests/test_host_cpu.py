@@ -114,7 +114,7 @@ def test_reads_svk_and_linear_elastic(tmp_path):
     ("no_dirichlet", "at least one mechanical Dirichlet DOF"),
     ("implicit", "is not on this path"),
     ("stvk", "has no device descriptor"),
-    ("neumann", "TYPE=TRACTION INST is not on this path"),
+    ("neumann", "TRACTION needs a"),
     ("pressure_face", "face S outside 1..6"),
     ("force_window", "tf > t0"),
     ("force_entry", "node, Fx, Fy, Fz"),
@@ -142,7 +142,7 @@ def test_input_errors(tmp_path, case, needle):
     elif case == "bad_direction":
         sol["bc"] = [(0, 3, 0.0)]
     elif case == "neumann":
-        extra = "*BOUNDARY, TYPE=TRACTION INST\n0,2,1e3,2\n4"
+        extra = "*BOUNDARY, TYPE=TRACTION INST\n0,2,1e3,2\n4"  # four fields: a pressure line
     elif case == "pressure_face":
         extra = "*BOUNDARY, TYPE=PRESSURE INST\n0, 1, 1e3, 7\n1"
     elif case == "force_window":
@@ -197,6 +197,13 @@ def test_reads_force_and_gravity(tmp_path):
     assert j["loads"] == [dict(type="PRESSURE RAMP", t0=0, tf=0.5, value=3e3, entries=[[4, 2], [5, 2]]),
                           dict(type="GRAVITY", t0=0, tf=0, value=2.0, entries=[]),
                           dict(type="PRESSURE INST", t0=0.25, tf=0.25, value=-1e3, entries=[[7, 6]])]
+    # traction blocks: "t0, tf, Tx, Ty, Tz, S" then element ids
+    inpgen.write_inp(p, m, [dict(rho=1000.0, E=1e6, nu=0.3)], [dict(scale=0.5, t0=0.0, tf=0.01, outputs=1, bc=bc)],
+                     extra="\n".join(["*BOUNDARY, TYPE=TRACTION RAMP", "0.0, 0.5, 1, -2, 3e2, 2", "5", "6"]))
+    rc, j, err = check(p)
+    assert rc == 0, err
+    assert j["loads"] == [dict(type="TRACTION RAMP", t0=0, tf=0.5, value=0, traction=[1, -2, 300],
+                               entries=[[4, 2], [5, 2]])]
     # no loads
     inpgen.write_inp(p, m, [dict(rho=1000.0, E=1e6, nu=0.3)], [dict(scale=0.5, t0=0.0, tf=0.01, outputs=1, bc=bc)])
     rc, j, err = check(p)

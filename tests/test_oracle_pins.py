@@ -855,3 +855,28 @@ def test_viscosity_dissipates_and_dev_variant_adds_nothing():
     ke0 = runs[O.VISC_NONE]["hist"]["e_kin"][0]
     assert abs(E[O.VISC_NONE][-50:].mean() - E[O.VISC_NONE][0]) < 0.1 * ke0
     assert E[O.VISC_BULK][-50:].mean() - E[O.VISC_BULK][0] < -0.25 * ke0
+
+
+@pytest.mark.parametrize("tet", [False, True])
+def test_traction_dead_load_totals_and_time_function(tet):
+    """BCsTractionRamp / BCsTractionInst (boundaryConditions.cpp:218-271): a dead load -- the total on faces of
+    area A is T(t) A whatever the displacement, spread by the face shape; T(t) = T/(tf-t0) (t-t0) for RAMP,
+    T for INST inside the window, 0 outside."""
+    m = perturb(box(2, 2, 3, tet=tet), 0.1, 12)
+    N = m["n_nodes"]
+    el, fc = _boundary_faces(m)
+    top = [i for i in range(el.size) if p_face_z(m, el[i], fc[i]) == 1.0]
+    el, fc = el[top], fc[top]
+    T, t0, tf = np.array([3.0e2, -1.5e2, 7.0e2]), 0.1, 0.5
+    u = np.random.default_rng(2).uniform(-0.05, 0.05, 3 * N)
+    for kind in ("traction_ramp", "traction_inst"):
+        p = O.OracleProblem(m, n_steps=1, loads=[(kind, T, t0, tf, el, fc)])
+        for t in (0.05, 0.1, 0.3, 0.5, 0.6):
+            Tt = (T / (tf - t0) * (t - t0) if kind == "traction_ramp" else T) if t0 <= t <= tf else 0.0 * T
+            f0 = p.external_force(t).reshape(3, N)
+            fu = p.external_force(t, u).reshape(3, N)
+            assert np.array_equal(f0, fu)  # not a follower load
+            np.testing.assert_allclose(f0.sum(axis=1), Tt * 1.0, rtol=1e-12, atol=1e-12 * np.abs(T).max())
+            X = m["X"].reshape(3, -1)
+            off = np.abs(X[2] - 1.0) > 1e-12
+            assert np.all(f0[:, off] == 0.0)

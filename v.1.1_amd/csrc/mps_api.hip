@@ -765,12 +765,15 @@ extern "C" int mps_set_loads(mps_handle h, int32_t n_loads, const mps_load* load
             if (++n_grav > 1) return h->fail(MPS_ERR_ARG, at + "a second GRAVITY block: one gravity load per solve");
             grav = L.value;
             grav_block = b;
-        } else if (L.type == MPS_LOAD_PRESSURE_RAMP || L.type == MPS_LOAD_PRESSURE_INST) {
-            const bool ramp = L.type == MPS_LOAD_PRESSURE_RAMP;
-            if (L.n > 0 && (!L.index || !L.face)) return h->fail(MPS_ERR_ARG, at + "PRESSURE needs index and face");
-            if (!std::isfinite(L.value) || !std::isfinite(L.t0) || !std::isfinite(L.tf) ||
+        } else if (L.type >= MPS_LOAD_PRESSURE_RAMP && L.type <= MPS_LOAD_TRACTION_INST) {
+            const bool ramp = L.type == MPS_LOAD_PRESSURE_RAMP || L.type == MPS_LOAD_TRACTION_RAMP;
+            const bool trac = L.type >= MPS_LOAD_TRACTION_RAMP;
+            if (L.n > 0 && (!L.index || !L.face)) return h->fail(MPS_ERR_ARG, at + "surface load needs index and face");
+            if (trac && (!L.force || !std::isfinite(L.force[0]) || !std::isfinite(L.force[1]) || !std::isfinite(L.force[2])))
+                return h->fail(MPS_ERR_ARG, at + "TRACTION needs a finite traction vector in force[0..2]");
+            if ((!trac && !std::isfinite(L.value)) || !std::isfinite(L.t0) || !std::isfinite(L.tf) ||
                 !(ramp ? L.tf > L.t0 : L.tf >= L.t0))
-                return h->fail(MPS_ERR_ARG, at + "PRESSURE needs a finite P and window (tf > t0 for a ramp)");
+                return h->fail(MPS_ERR_ARG, at + "surface load needs a finite value and window (tf > t0 for a ramp)");
             for (int k = 0; k < L.n; k++) {
                 if (L.index[k] < 0 || L.index[k] >= Ne)
                     return h->fail(MPS_ERR_ARG, at + fmt("element %d outside [0,%lld)", L.index[k], (long long)Ne));
@@ -796,15 +799,18 @@ extern "C" int mps_set_loads(mps_handle h, int32_t n_loads, const mps_load* load
     HCHK(h, hipMemcpy(ninfo.data(), I.ninfo, (size_t)N * 4, hipMemcpyDeviceToHost));
     for (uint32_t& f : ninfo) f &= MPS_REC_MASK;
     std::vector<LoadItem> items;
-    std::vector<double> lm, lwin, sgd, sblk(4 * (size_t)std::max(n_loads, 1), 0.0);
+    std::vector<double> lm, lwin, sgd, sblk(MPS_SBLK_STRIDE * (size_t)std::max(n_loads, 1), 0.0);
     std::vector<int32_t> sge, sgb;
     int64_t seq = 0;
     for (int b = 0; b < n_loads; b++) {
         const mps_load& L = loads[b];
-        sblk[4 * (size_t)b] = (double)L.type;
-        sblk[4 * (size_t)b + 1] = L.value;
-        sblk[4 * (size_t)b + 2] = L.t0;
-        sblk[4 * (size_t)b + 3] = L.tf;
+        double* B = sblk.data() + MPS_SBLK_STRIDE * (size_t)b;
+        B[0] = (double)L.type;
+        B[1] = L.value;
+        B[2] = L.t0;
+        B[3] = L.tf;
+        if (L.type >= MPS_LOAD_TRACTION_RAMP)
+            for (int d = 0; d < 3; d++) B[4 + d] = L.force[d];
         if (L.type == MPS_LOAD_FORCE) {
             for (int k = 0; k < L.n; k++) {
                 const size_t j = lwin.size() / 2;
@@ -814,7 +820,7 @@ extern "C" int mps_set_loads(mps_handle h, int32_t n_loads, const mps_load* load
                 items.push_back({L.index[k], b, seq++, (MPS_ITEM_FORCE << 30) | (uint32_t)j});
             }
         } else if (L.type != MPS_LOAD_GRAVITY) {
-            const bool ramp = L.type == MPS_LOAD_PRESSURE_RAMP;
+            const bool ramp = L.type == MPS_LOAD_PRESSURE_RAMP;  // bulk-shape scatter (tractions use the face's)
             for (int k = 0; k < L.n; k++) {
                 double pts[4 * MPS_SGP_STRIDE];
                 int32_t nodes[32];

@@ -29,7 +29,7 @@
 //     fgrav         double[N]      gravity load along z (NULL: no gravity)
 //     sge, sgb      int32[n_sgp]   pressure surface Gauss point: bulk element, block
 //     sgd           double[MPS_SGP_STRIDE*n_sgp]  bulk dphi (24), n0 (3), w|J_s| (1), scatter phi (8)
-//     sblk          double[4*blocks]  block type, P, t0, tf
+//     sblk          double[8*blocks]  block type, P, t0, tf, traction (3), -
 //     pc            double[24*n_sgp]  the step's surface forces [(g*8 + a)*3 + dir]
 //   uprev           double[3N]   displacement of the previous step (bulk viscosity only; NULL otherwise)
 //   ctl             MpsCtl       step control (time, dt, go flags)
@@ -48,6 +48,7 @@
 #define MPS_ITEM_GRAVITY 2u
 #define MPS_ITEM_INDEX 0x3FFFFFFFu
 #define MPS_SGP_STRIDE 36            // doubles per pressure surface Gauss point
+#define MPS_SBLK_STRIDE 8            // doubles per surface-load block: type, P, t0, tf, Tx, Ty, Tz, -
 
 enum : int32_t { MPS_TIME_WINDOW = 0, MPS_TIME_FIXED = 1 };
 

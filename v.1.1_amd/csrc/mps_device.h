@@ -545,8 +545,8 @@ __device__ __forceinline__ void visc_coeffs(const MpsInst& I, int e, double c, d
 // MPS_PATCH), which the forces overwrite only after the Gauss-point loop.
 // ---------------------------------------------------------------------------
 template <typename T, int LAW, bool VISC = false>
-__device__ __forceinline__ void hex8_element(const MpsInst& I, int e, const double* __restrict__ u, T (*f)[3],
-                                             double* dte, T* vs = nullptr, double rdt = 0.0) {
+__device__ __forceinline__ double hex8_element(const MpsInst& I, int e, const double* __restrict__ u, T (*f)[3],
+                                               bool want_dt, T* vs = nullptr, double rdt = 0.0) {
     const int32_t* cn = I.conn + (size_t)e * 8;
     int4 q0 = *reinterpret_cast<const int4*>(cn);
     int4 q1 = *reinterpret_cast<const int4*>(cn + 4);
@@ -675,7 +675,8 @@ __device__ __forceinline__ void hex8_element(const MpsInst& I, int e, const doub
         G[6][i] *= s3;
     }
     hex_unbutterfly<T>(G, f);
-    if (dte) {
+    double dte = bits2d(0x7FF0000000000000ull);
+    if (want_dt) {
         // deformed characteristic length (solvers.cpp:300-317); X and u are re-read (cache hits).
         // The empty asm makes the node ids opaque so that the 48 gather addresses are not kept
         // live across the Gauss-point loop.
@@ -691,8 +692,9 @@ __device__ __forceinline__ void hex8_element(const MpsInst& I, int e, const doub
 #pragma unroll
             for (int d = 0; d < 3; d++) x[a][d] = I.X[d * N + ndr[a]] + u[d * N + ndr[a]];
         double l = hex_char_length(x);
-        *dte = elem_dt(l, I.l0[e], c);
+        dte = elem_dt(l, I.l0[e], c);
     }
+    return dte;
 }
 
 // ---------------------------------------------------------------------------
@@ -700,8 +702,8 @@ __device__ __forceinline__ void hex8_element(const MpsInst& I, int e, const doub
 // J(i,j) = x_{j+1,i} - x_{0,i}; dphi_0 = -(sum of J^-1 rows), dphi_a = row a-1 of J^-1.
 // ---------------------------------------------------------------------------
 template <typename T, int LAW, bool VISC = false>
-__device__ __forceinline__ void tet4_element(const MpsInst& I, int e, const double* __restrict__ u, T (*f)[3],
-                                             double* dte, double rdt = 0.0) {
+__device__ __forceinline__ double tet4_element(const MpsInst& I, int e, const double* __restrict__ u, T (*f)[3],
+                                               bool want_dt, double rdt = 0.0) {
     const int4 q = *reinterpret_cast<const int4*>(I.conn + (size_t)e * 4);
     const int nd[4] = {q.x, q.y, q.z, q.w};
     const size_t N = (size_t)I.N;
@@ -723,13 +725,14 @@ __device__ __forceinline__ void tet4_element(const MpsInst& I, int e, const doub
             Jm[i * 3 + j] = (T)(x[j + 1][i] - x[0][i]);
             Du[i * 3 + j] = (T)(uu[j + 1][i] - uu[0][i]);
         }
-    if (dte) {
+    double dte = bits2d(0x7FF0000000000000ull);
+    if (want_dt) {
 #pragma unroll
         for (int a = 0; a < 4; a++)
 #pragma unroll
             for (int d = 0; d < 3; d++) x[a][d] = x[a][d] + uu[a][d];
         double l = tet_char_length(x);
-        *dte = elem_dt(l, I.l0[e], c);
+        dte = elem_dt(l, I.l0[e], c);
     }
     T iJ[9], H[9], P[9];
     T dJ = det3(Jm);
@@ -767,6 +770,7 @@ __device__ __forceinline__ void tet4_element(const MpsInst& I, int e, const doub
 #pragma unroll
         for (int i = 0; i < 3; i++)
             f[a][i] = wJ * P[i * 3] * dphi[a][0] + wJ * P[i * 3 + 1] * dphi[a][1] + wJ * P[i * 3 + 2] * dphi[a][2];
+    return dte;
 }
 
 // ---------------------------------------------------------------------------
@@ -883,6 +887,17 @@ __device__ __forceinline__ void surface_point(const MpsInst& I, int q, double t)
     const int nen = I.nen;
     const int e = I.sge[q];
     const double* D = I.sgd + (size_t)q * MPS_SGP_STRIDE;
+    const double* B = I.sblk + MPS_SBLK_STRIDE * (size_t)I.sgb[q];
+    double* out = I.pc + (size_t)q * 24;
+    if (B[0] >= 4.0) {  // traction (BCsTractionRamp / Inst, boundaryConditions.cpp:218-271): w|J_s| phi_a T_i
+        const bool tramp = B[0] == 4.0;
+        double T[3] = {0., 0., 0.};
+        if (t >= B[2] && t <= B[3])
+            for (int i = 0; i < 3; i++) T[i] = tramp ? B[4 + i] / (B[3] - B[2]) * (t - B[2]) : B[4 + i];
+        for (int a = 0; a < 8; a++)
+            for (int i = 0; i < 3; i++) out[a * 3 + i] = D[27] * D[28 + a] * T[i];
+        return;
+    }
     double F[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
     for (int a = 0; a < nen; a++) {
         const int nd = I.conn[(size_t)e * nen + a];
@@ -912,12 +927,10 @@ __device__ __forceinline__ void surface_point(const MpsInst& I, int q, double t)
         for (int j = 0; j < 3; j++) r += Fi[j * 3 + i] * n0[j];  // (F^-1)^T, row i
         nv[i] = r;
     }
-    const double* B = I.sblk + 4 * (size_t)I.sgb[q];
     const bool ramp = B[0] == 2.0;
     double P = 0.;
     if (t >= B[2] && t <= B[3]) P = ramp ? B[1] / (B[3] - B[2]) * (t - B[2]) : B[1];
     const double ws = D[27];
-    double* out = I.pc + (size_t)q * 24;
     for (int a = 0; a < 8; a++) {
         const double ph = D[28 + a];
         for (int i = 0; i < 3; i++) out[a * 3 + i] = ws * ph * J * P * nv[i];

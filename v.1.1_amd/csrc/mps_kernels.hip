@@ -32,9 +32,8 @@ __device__ __forceinline__ unsigned long long patch_forces(const MpsInst& I, int
     unsigned long long dmin = INF_BITS;
     for (int i = threadIdx.x; i < ne; i += BS) {
         T f[NEN][3];
-        double dte = bits2d(INF_BITS);
-        if (NEN == 8) hex8_element<T, LAW, VISC>(I, e0 + i, I.u, (T(*)[3])f, want_dt ? &dte : nullptr, sf + i, rdt);
-        else tet4_element<T, LAW, VISC>(I, e0 + i, I.u, (T(*)[3])f, want_dt ? &dte : nullptr, rdt);
+        const double dte = (NEN == 8) ? hex8_element<T, LAW, VISC>(I, e0 + i, I.u, (T(*)[3])f, want_dt, sf + i, rdt)
+                                      : tet4_element<T, LAW, VISC>(I, e0 + i, I.u, (T(*)[3])f, want_dt, rdt);
 #pragma unroll
         for (int a = 0; a < NEN; a++)
 #pragma unroll

@@ -316,7 +316,8 @@ bool readSolvers(Reader &R, InpModel &m) {
                     return R.fail(i, "*Time must precede *BOUNDARY inside *SOLVER (the ramp duration is taken at parse time)");
                 if (!readDirichlet(R, i, m, s)) return false;
             } else if (starts(l, "*BOUNDARY") &&
-                       (tail(l, 16) == "FORCE" || tail(l, 16) == "PRESSURE RAMP" || tail(l, 16) == "PRESSURE INST")) {
+                       (tail(l, 16) == "FORCE" || tail(l, 16) == "PRESSURE RAMP" || tail(l, 16) == "PRESSURE INST" ||
+                        tail(l, 16) == "TRACTION RAMP" || tail(l, 16) == "TRACTION INST")) {
                 // a Neumann block: read with the whole file by readLoads, for every solver
                 for (i++; i + 1 < R.L.size() && !starts(R.L[i + 1], "*"); i++) {
                 }
@@ -509,31 +510,45 @@ bool readLoads(Reader &R, InpModel &m) {
                 }
                 i--;
                 m.loads.blocks.push_back(b);
-            } else if (type == "PRESSURE RAMP" || type == "PRESSURE INST") {
+            } else if (type == "PRESSURE RAMP" || type == "PRESSURE INST" || type == "TRACTION RAMP" ||
+                       type == "TRACTION INST") {
+                // pressure: "t0, tf, P, S"; traction (tractionRampBCManagement :3342-3460): "t0, tf, Tx, Ty, Tz, S"
                 InpLoadBlock b;
-                b.type = type == "PRESSURE RAMP" ? MPS_LOAD_PRESSURE_RAMP : MPS_LOAD_PRESSURE_INST;
-                if (i + 1 >= R.L.size()) return R.fail(i, "PRESSURE needs a \"t0, tf, P, S\" line");
+                const bool trac = type[0] == 'T';
+                b.type = type == "PRESSURE RAMP"   ? MPS_LOAD_PRESSURE_RAMP
+                         : type == "PRESSURE INST" ? MPS_LOAD_PRESSURE_INST
+                         : type == "TRACTION RAMP" ? MPS_LOAD_TRACTION_RAMP
+                                                   : MPS_LOAD_TRACTION_INST;
+                const char *form = trac ? "TRACTION needs a \"t0, tf, Tx, Ty, Tz, S\" line"
+                                        : "PRESSURE needs a \"t0, tf, P, S\" line";
+                if (i + 1 >= R.L.size()) return R.fail(i, form);
                 std::vector<std::string> f = fields(R.L[i + 1]);
                 double S;
-                if (f.size() != 4 || !toDouble(f[0], b.t0) || !toDouble(f[1], b.tf) || !toDouble(f[2], b.value) ||
-                    !toDouble(f[3], S))
-                    return R.fail(i + 1, "PRESSURE needs a \"t0, tf, P, S\" line");
+                if (f.size() != (trac ? 6u : 4u) || !toDouble(f[0], b.t0) || !toDouble(f[1], b.tf) ||
+                    !toDouble(f.back(), S))
+                    return R.fail(i + 1, form);
+                if (trac) {
+                    b.force.resize(3);
+                    for (int d = 0; d < 3; d++)
+                        if (!toDouble(f[2 + d], b.force[d])) return R.fail(i + 1, form);
+                } else if (!toDouble(f[2], b.value)) {
+                    return R.fail(i + 1, form);
+                }
                 const int nf = m.nen == 8 ? 6 : 4;
-                if ((int)S < 1 || (int)S > nf) return R.fail(i + 1, "PRESSURE face S outside 1.." + std::to_string(nf));
-                if (b.type == MPS_LOAD_PRESSURE_RAMP ? !(b.tf > b.t0) : !(b.tf >= b.t0))
-                    return R.fail(i + 1, "PRESSURE window needs tf > t0 (tf >= t0 for INST)");
+                if ((int)S < 1 || (int)S > nf) return R.fail(i + 1, "face S outside 1.." + std::to_string(nf));
+                const bool ramp = b.type == MPS_LOAD_PRESSURE_RAMP || b.type == MPS_LOAD_TRACTION_RAMP;
+                if (ramp ? !(b.tf > b.t0) : !(b.tf >= b.t0))
+                    return R.fail(i + 1, "window needs tf > t0 (tf >= t0 for INST)");
                 for (i += 2; i < R.L.size() && !starts(R.L[i], "*"); i++) {
                     if (isBlank(R.L[i])) continue;
                     long long id;
                     if (!toInt(fields(R.L[i])[0], id) || id < 1 || id > m.nElem)
-                        return R.fail(i, "PRESSURE element id outside 1..N_e");
+                        return R.fail(i, "surface load element id outside 1..N_e");
                     b.index.push_back((int32_t)(id - 1));
                     b.face.push_back((int32_t)S);
                 }
                 i--;
                 m.loads.blocks.push_back(b);
-            } else if (type == "TRACTION RAMP" || type == "TRACTION INST") {
-                return R.fail(i, "*BOUNDARY, TYPE=" + type + " is not on this path (FORCE, PRESSURE and *GRAVITY are)");
             }
         } else if (starts(l, "*GRAVITY")) {
             InpLoadBlock b;
@@ -560,7 +575,8 @@ bool rejectUnsupported(Reader &R) {
         if (starts(l, "*END SOLVER")) inSolver = false;
         if (inSolver) continue;
         const std::string type = tail(l, 16);
-        if (starts(l, "*BOUNDARY") && type != "FORCE" && type != "PRESSURE RAMP" && type != "PRESSURE INST")
+        if (starts(l, "*BOUNDARY") && type != "FORCE" && type != "PRESSURE RAMP" && type != "PRESSURE INST" &&
+            type != "TRACTION RAMP" && type != "TRACTION INST")
             return R.fail(i, "Dirichlet conditions outside *SOLVER are not on this path");
     }
     return true;

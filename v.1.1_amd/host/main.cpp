@@ -85,11 +85,14 @@ static int check(const std::string &path) {
         first = false;
     }
     std::printf("], \"loads\": [");
-    const char *names[4] = {"FORCE", "GRAVITY", "PRESSURE RAMP", "PRESSURE INST"};
+    const char *names[6] = {"FORCE", "GRAVITY", "PRESSURE RAMP", "PRESSURE INST", "TRACTION RAMP", "TRACTION INST"};
     for (size_t k = 0; k < m.loads.blocks.size(); k++) {
         const mps::InpLoadBlock &b = m.loads.blocks[k];
-        std::printf("%s{\"type\": \"%s\", \"t0\": %.17g, \"tf\": %.17g, \"value\": %.17g, \"entries\": [", k ? ", " : "",
+        std::printf("%s{\"type\": \"%s\", \"t0\": %.17g, \"tf\": %.17g, \"value\": %.17g, ", k ? ", " : "",
                     names[b.type], b.t0, b.tf, b.value);
+        if (b.type >= MPS_LOAD_TRACTION_RAMP)
+            std::printf("\"traction\": [%.17g, %.17g, %.17g], ", b.force[0], b.force[1], b.force[2]);
+        std::printf("\"entries\": [");
         for (size_t j = 0; j < b.index.size(); j++) {
             if (b.type == MPS_LOAD_FORCE)
                 std::printf("%s[%d, %.17g, %.17g, %.17g]", j ? ", " : "", b.index[j], b.force[3 * j], b.force[3 * j + 1],

@@ -127,7 +127,7 @@ int classSolverExplicit::calculate(const InpModel &m, const std::vector<std::uni
         std::vector<mps_load> L;
         for (const InpLoadBlock &b : ld.blocks)
             L.push_back(mps_load{b.type, (int32_t)(b.type == MPS_LOAD_GRAVITY ? 0 : b.index.size()), b.t0, b.tf, b.value,
-                                 b.index.data(), b.face.data(), b.force.data()});
+                                 b.index.data(), b.face.data(), b.force.data()});  // TRACTION: force = T
         if (mps_set_loads(h, (int32_t)L.size(), L.data()) != MPS_OK) return fail("mps_set_loads");
     }
     double dt0 = 0;

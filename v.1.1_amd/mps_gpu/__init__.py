@@ -49,10 +49,11 @@ class Load(C.Structure):
                 ("value", C.c_double), ("index", _ip), ("face", _ip), ("force", _dp)]
 
 
-LOAD_FORCE, LOAD_GRAVITY, LOAD_PRESSURE_RAMP, LOAD_PRESSURE_INST = 0, 1, 2, 3
+LOAD_FORCE, LOAD_GRAVITY, LOAD_PRESSURE_RAMP, LOAD_PRESSURE_INST, LOAD_TRACTION_RAMP, LOAD_TRACTION_INST = 0, 1, 2, 3, 4, 5
 VISC_NONE, VISC_BULK, VISC_DEV = 0, 1, 2
 _LOAD_TYPES = {"force": LOAD_FORCE, "gravity": LOAD_GRAVITY, "pressure_ramp": LOAD_PRESSURE_RAMP,
-               "pressure_inst": LOAD_PRESSURE_INST}
+               "pressure_inst": LOAD_PRESSURE_INST, "traction_ramp": LOAD_TRACTION_RAMP,
+               "traction_inst": LOAD_TRACTION_INST}
 
 
 def load_blocks(forces=None, gravity=0.0, gravity_pos=-1, loads=None):
@@ -60,6 +61,7 @@ def load_blocks(forces=None, gravity=0.0, gravity_pos=-1, loads=None):
       ("force", nodes, force[k,3], t0, tf)                *BOUNDARY, TYPE=FORCE (entries F*Ff, one window)
       ("gravity", g)                                      *GRAVITY
       ("pressure_ramp" | "pressure_inst", P, t0, tf, elements, faces)
+      ("traction_ramp" | "traction_inst", (Tx, Ty, Tz), t0, tf, elements, faces)
     `loads` is such a list; otherwise forces = (node[k], force[k,3], t0[k], tf[k]) gives one block per entry and
     a non-zero gravity goes before entry gravity_pos (-1: after all)."""
     if loads is not None:
@@ -273,7 +275,13 @@ class Solver:
                 el = np.ascontiguousarray(blk[4], dtype=np.int32)
                 fc = np.ascontiguousarray(blk[5], dtype=np.int32)
                 keep += [el, fc]
-                Lb.n, Lb.value, Lb.t0, Lb.tf, Lb.index, Lb.face = el.size, blk[1], blk[2], blk[3], _i(el), _i(fc)
+                Lb.n, Lb.t0, Lb.tf, Lb.index, Lb.face = el.size, blk[2], blk[3], _i(el), _i(fc)
+                if kind in (LOAD_TRACTION_RAMP, LOAD_TRACTION_INST):
+                    T = np.ascontiguousarray(blk[1], dtype=np.float64).reshape(3)
+                    keep.append(T)
+                    Lb.force = _d(T)
+                else:
+                    Lb.value = blk[1]
         self._loads = keep
         self._chk(self.L.mps_set_loads(self.h, len(blocks), arr))
 

@@ -301,7 +301,7 @@ def main():
     eb = elem_kernel_bytes(ne, nn, 8, n_part, False)
     sb = step_bytes(ne, nn, 8, n_part, False)
     achieved = eb / (elem_avg_ms * 1e-3) / 1e9
-    valu_wave = measured_valu_per_wave("k_elem<double,8,0>")
+    valu_wave = measured_valu_per_wave("k_elem<double,8,0,false>")
     n_waves = -(-ne // PATCH) * (PATCH // 64)
     valu_peak = VALU_ISSUE_PEAK  # wave64 VALU instructions per second (one per CU per cycle)
     valu = None if valu_wave is None else {
@@ -325,9 +325,9 @@ def main():
                                "dt=0.5 x stable dt, FP64",
                    "n_elem": ne, "n_nodes": nn, "gauss_points": 8, "form": "fp64"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic("k_elem<double,8,0>"),
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic("k_elem<double,8,0,false>"),
                      "traffic_source": os.path.relpath(TRAFFIC_TABLE, ROOT),
-                     "kernel": "k_elem<double,8,0> (hex8 element forces + stable dt)",
+                     "kernel": "k_elem<double,8,0,false> (hex8 element forces + stable dt)",
                      "kernel_avg_ms": elem_avg_ms, "algorithmic_bytes_per_launch": eb,
                      "measured_copy_ceiling_gbs": copy_gbs,
                      "valu_check": valu},

@@ -406,6 +406,64 @@ __device__ __forceinline__ double tet_char_length(const double (*x)[3]) {
     return 2 * r;
 }
 
+// FP32 characteristic length for the precision form: the same formulas with edge vectors differenced in
+// FP64 (no cancellation against the absolute coordinates) and the areas, volume and ratio in FP32.
+__device__ __forceinline__ float area_tri_f32(const double* p, const double* q, const double* r) {
+    const float A0 = (float)(p[0] - r[0]), A1 = (float)(p[1] - r[1]), A2 = (float)(p[2] - r[2]);
+    const float B0 = (float)(q[0] - r[0]), B1 = (float)(q[1] - r[1]), B2 = (float)(q[2] - r[2]);
+    const float C0 = A1 * B2 - A2 * B1, C1 = A2 * B0 - A0 * B2, C2 = A0 * B1 - A1 * B0;
+    return 0.5f * sqrtf(C0 * C0 + C1 * C1 + C2 * C2);
+}
+
+__device__ __forceinline__ float hex_char_length_f32(const double (*x)[3]) {
+    const int fc[6][4] = {{0, 1, 2, 3}, {4, 5, 6, 7}, {0, 1, 5, 4}, {1, 2, 6, 5}, {2, 3, 7, 6}, {0, 3, 7, 4}};
+    float A = 0.0f;
+#pragma unroll
+    for (int f = 0; f < 6; f++)
+        A += area_tri_f32(x[fc[f][0]], x[fc[f][1]], x[fc[f][2]]) + area_tri_f32(x[fc[f][0]], x[fc[f][2]], x[fc[f][3]]);
+    float M[9];
+    const int c[3] = {1, 3, 4};
+#pragma unroll
+    for (int j = 0; j < 3; j++)
+#pragma unroll
+        for (int d = 0; d < 3; d++) M[j * 3 + d] = (float)(x[c[j]][d] - x[0][d]);
+    return 6.0f * fabsf(det3(M)) / A;
+}
+
+// The same hex length from FP32 positions (any common origin).
+__device__ __forceinline__ float area_tri_pf(const float* p, const float* q, const float* r) {
+    const float A0 = p[0] - r[0], A1 = p[1] - r[1], A2 = p[2] - r[2];
+    const float B0 = q[0] - r[0], B1 = q[1] - r[1], B2 = q[2] - r[2];
+    const float C0 = A1 * B2 - A2 * B1, C1 = A2 * B0 - A0 * B2, C2 = A0 * B1 - A1 * B0;
+    return 0.5f * sqrtf(C0 * C0 + C1 * C1 + C2 * C2);
+}
+
+__device__ __forceinline__ float hex_char_length_pf(const float (*x)[3]) {
+    const int fc[6][4] = {{0, 1, 2, 3}, {4, 5, 6, 7}, {0, 1, 5, 4}, {1, 2, 6, 5}, {2, 3, 7, 6}, {0, 3, 7, 4}};
+    float A = 0.0f;
+#pragma unroll
+    for (int f = 0; f < 6; f++)
+        A += area_tri_pf(x[fc[f][0]], x[fc[f][1]], x[fc[f][2]]) + area_tri_pf(x[fc[f][0]], x[fc[f][2]], x[fc[f][3]]);
+    float M[9];
+    const int c[3] = {1, 3, 4};
+#pragma unroll
+    for (int j = 0; j < 3; j++)
+#pragma unroll
+        for (int d = 0; d < 3; d++) M[j * 3 + d] = x[c[j]][d] - x[0][d];
+    return 6.0f * fabsf(det3(M)) / A;
+}
+
+__device__ __forceinline__ float tet_char_length_f32(const double (*x)[3]) {
+    const float A = area_tri_f32(x[0], x[1], x[2]) + area_tri_f32(x[1], x[2], x[3]) + area_tri_f32(x[0], x[2], x[3]) +
+                    area_tri_f32(x[0], x[1], x[3]);
+    float M[9];
+#pragma unroll
+    for (int j = 0; j < 3; j++)
+#pragma unroll
+        for (int d = 0; d < 3; d++) M[j * 3 + d] = (float)(x[j + 1][d] - x[0][d]);
+    return fabsf(det3(M)) / A;
+}
+
 // Element contribution to the stable dt: std::min(l0, l)/c (solvers.cpp:317).
 __device__ __forceinline__ double elem_dt(double l, double l0, double c) {
 #pragma clang fp contract(off)
@@ -676,10 +734,28 @@ __device__ __forceinline__ double hex8_element(const MpsInst& I, int e, const do
     }
     hex_unbutterfly<T>(G, f);
     double dte = bits2d(0x7FF0000000000000ull);
-    if (want_dt) {
-        // deformed characteristic length (solvers.cpp:300-317); X and u are re-read (cache hits).
-        // The empty asm makes the node ids opaque so that the 48 gather addresses are not kept
-        // live across the Gauss-point loop.
+    if (want_dt && sizeof(T) == 4) {
+        // precision form: FP32 length from the deformed node positions relative to their centroid,
+        // x_a - xbar = sum_m sigma_m(a) C_m / 8 with C the butterfly coefficients of X + u (Ax + Au
+        // unscaled), so no node data is read again; dt and its reduction stay FP64
+        const T ig = (T)(1.0 / MPS_GP), ig2 = (T)(1.0 / (MPS_GP * MPS_GP));
+        T R[7][3], xr[8][3];
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            R[0][d] = Ax[0][d] + Au[0][d];
+            R[1][d] = Ax[1][d] + Au[1][d];
+            R[2][d] = Ax[2][d] + Au[2][d];
+            R[3][d] = (Ax[3][d] + Au[3][d]) * ig;
+            R[4][d] = (Ax[4][d] + Au[4][d]) * ig;
+            R[5][d] = (Ax[5][d] + Au[5][d]) * ig;
+            R[6][d] = (Ax[6][d] + Au[6][d]) * ig2;
+        }
+        hex_unbutterfly<T>(R, xr);
+        dte = elem_dt((double)hex_char_length_pf((const float(*)[3])xr), I.l0[e], c);
+    } else if (want_dt) {
+        // deformed characteristic length (solvers.cpp:300-317), strict FP64: bitwise the reference's.
+        // X and u are re-read (cache hits). The empty asm makes the node ids opaque so that the 48
+        // gather addresses are not kept live across the Gauss-point loop.
         int ndr[8];
 #pragma unroll
         for (int a = 0; a < 8; a++) {
@@ -691,7 +767,7 @@ __device__ __forceinline__ double hex8_element(const MpsInst& I, int e, const do
         for (int a = 0; a < 8; a++)
 #pragma unroll
             for (int d = 0; d < 3; d++) x[a][d] = I.X[d * N + ndr[a]] + u[d * N + ndr[a]];
-        double l = hex_char_length(x);
+        const double l = hex_char_length(x);
         dte = elem_dt(l, I.l0[e], c);
     }
     return dte;
@@ -731,7 +807,7 @@ __device__ __forceinline__ double tet4_element(const MpsInst& I, int e, const do
         for (int a = 0; a < 4; a++)
 #pragma unroll
             for (int d = 0; d < 3; d++) x[a][d] = x[a][d] + uu[a][d];
-        double l = tet_char_length(x);
+        const double l = (sizeof(T) == 8) ? tet_char_length(x) : (double)tet_char_length_f32(x);
         dte = elem_dt(l, I.l0[e], c);
     }
     T iJ[9], H[9], P[9];
@@ -937,6 +1013,21 @@ __device__ __forceinline__ void surface_point(const MpsInst& I, int q, double t)
     }
 }
 
+// Partial nodal force g (FP64 partials; FP32 in the precision form).
+__device__ __forceinline__ void load_partial(const MpsInst& I, int g, double* c) {
+    if (I.precision == 1) {
+        const float* p = reinterpret_cast<const float*>(I.part) + 3 * (size_t)g;
+        c[0] = p[0];
+        c[1] = p[1];
+        c[2] = p[2];
+    } else {
+        const double* p = I.part + 3 * (size_t)g;
+        c[0] = p[0];
+        c[1] = p[1];
+        c[2] = p[2];
+    }
+}
+
 // Gather + lumped-mass solve of node n (nonlinearSystem.cpp:1096-1107; rhs = -fint + fext).
 // The node's internal force is the sum of its patch partials in ascending patch order.
 __device__ __forceinline__ void solve_node(const MpsInst& I, int n, double dt, double* v, double* a) {
@@ -944,24 +1035,42 @@ __device__ __forceinline__ void solve_node(const MpsInst& I, int n, double dt, d
     const uint32_t info = I.ninfo[n];
     const uint32_t mask = info & 7u;
     if (mask == 7u) return;
-    const double* __restrict__ part = I.part;
     int q = I.nrow[n];
     const int q1 = I.nrow[n + 1];
     double f0 = 0.0, f1 = 0.0, f2 = 0.0;
-    // 2 entries per iteration: both loads issue before the (ordered) additions
-    for (; q + 2 <= q1; q += 2) {
-        const double* c0 = part + 3 * (size_t)I.ncol[q];
-        const double* c1 = part + 3 * (size_t)I.ncol[q + 1];
-        const double a0 = c0[0], b0 = c0[1], d0 = c0[2];
-        const double a1 = c1[0], b1 = c1[1], d1 = c1[2];
-        f0 += a0; f1 += b0; f2 += d0;
-        f0 += a1; f1 += b1; f2 += d1;
-    }
-    if (q < q1) {
-        const double* c = part + 3 * (size_t)I.ncol[q];
-        f0 += c[0];
-        f1 += c[1];
-        f2 += c[2];
+    if (I.precision == 1) {  // FP32 partials of the precision form, summed in FP64
+        const float* __restrict__ part = reinterpret_cast<const float*>(I.part);
+        for (; q + 2 <= q1; q += 2) {
+            const float* c0 = part + 3 * (size_t)I.ncol[q];
+            const float* c1 = part + 3 * (size_t)I.ncol[q + 1];
+            const double a0 = c0[0], b0 = c0[1], d0 = c0[2];
+            const double a1 = c1[0], b1 = c1[1], d1 = c1[2];
+            f0 += a0; f1 += b0; f2 += d0;
+            f0 += a1; f1 += b1; f2 += d1;
+        }
+        if (q < q1) {
+            const float* c = part + 3 * (size_t)I.ncol[q];
+            f0 += (double)c[0];
+            f1 += (double)c[1];
+            f2 += (double)c[2];
+        }
+    } else {
+        const double* __restrict__ part = I.part;
+        // 2 entries per iteration: both loads issue before the (ordered) additions
+        for (; q + 2 <= q1; q += 2) {
+            const double* c0 = part + 3 * (size_t)I.ncol[q];
+            const double* c1 = part + 3 * (size_t)I.ncol[q + 1];
+            const double a0 = c0[0], b0 = c0[1], d0 = c0[2];
+            const double a1 = c1[0], b1 = c1[1], d1 = c1[2];
+            f0 += a0; f1 += b0; f2 += d0;
+            f0 += a1; f1 += b1; f2 += d1;
+        }
+        if (q < q1) {
+            const double* c = part + 3 * (size_t)I.ncol[q];
+            f0 += c[0];
+            f1 += c[1];
+            f2 += c[2];
+        }
     }
     double fe[3];
     node_fext(I, n, info, fe);

@@ -55,10 +55,17 @@ __device__ __forceinline__ unsigned long long patch_forces(const MpsInst& I, int
             s1 += (double)sf[(a * 3 + 1) * MPS_PATCH + i];
             s2 += (double)sf[(a * 3 + 2) * MPS_PATCH + i];
         }
-        double* dst = I.part + 3 * (size_t)(base + j);
-        dst[0] = s0;
-        dst[1] = s1;
-        dst[2] = s2;
+        if (sizeof(T) == 4) {  // precision form: partials rounded once to FP32, summed in FP64 by the node kernel
+            float* dst = reinterpret_cast<float*>(I.part) + 3 * (size_t)(base + j);
+            dst[0] = (float)s0;
+            dst[1] = (float)s1;
+            dst[2] = (float)s2;
+        } else {
+            double* dst = I.part + 3 * (size_t)(base + j);
+            dst[0] = s0;
+            dst[1] = s1;
+            dst[2] = s2;
+        }
     }
     return dmin;
 }
@@ -508,7 +515,8 @@ __global__ __launch_bounds__(256) void k_gather_fint(MpsInst I, double* fint) {
     if (n >= I.N) return;
     double f0 = 0.0, f1 = 0.0, f2 = 0.0;
     for (int q = I.nrow[n]; q < I.nrow[n + 1]; q++) {
-        const double* c = I.part + 3 * (size_t)I.ncol[q];
+        double c[3];
+        load_partial(I, I.ncol[q], c);
         f0 += c[0];
         f1 += c[1];
         f2 += c[2];

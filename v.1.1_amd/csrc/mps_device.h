@@ -406,36 +406,21 @@ __device__ __forceinline__ double tet_char_length(const double (*x)[3]) {
     return 2 * r;
 }
 
-// FP32 characteristic length for the precision form: the same formulas with edge vectors differenced in
-// FP64 (no cancellation against the absolute coordinates) and the areas, volume and ratio in FP32.
+// FP32 characteristic lengths for the precision form (the reference formulas with FP32 areas, volume and
+// ratio, hardware square root and reciprocal).  Tet: edge vectors differenced in FP64, so there is no
+// cancellation against the absolute coordinates; hex: from node positions relative to their centroid.
 __device__ __forceinline__ float area_tri_f32(const double* p, const double* q, const double* r) {
     const float A0 = (float)(p[0] - r[0]), A1 = (float)(p[1] - r[1]), A2 = (float)(p[2] - r[2]);
     const float B0 = (float)(q[0] - r[0]), B1 = (float)(q[1] - r[1]), B2 = (float)(q[2] - r[2]);
     const float C0 = A1 * B2 - A2 * B1, C1 = A2 * B0 - A0 * B2, C2 = A0 * B1 - A1 * B0;
-    return 0.5f * sqrtf(C0 * C0 + C1 * C1 + C2 * C2);
+    return 0.5f * __builtin_amdgcn_sqrtf(C0 * C0 + C1 * C1 + C2 * C2);  // hardware sqrt, 1 ulp
 }
 
-__device__ __forceinline__ float hex_char_length_f32(const double (*x)[3]) {
-    const int fc[6][4] = {{0, 1, 2, 3}, {4, 5, 6, 7}, {0, 1, 5, 4}, {1, 2, 6, 5}, {2, 3, 7, 6}, {0, 3, 7, 4}};
-    float A = 0.0f;
-#pragma unroll
-    for (int f = 0; f < 6; f++)
-        A += area_tri_f32(x[fc[f][0]], x[fc[f][1]], x[fc[f][2]]) + area_tri_f32(x[fc[f][0]], x[fc[f][2]], x[fc[f][3]]);
-    float M[9];
-    const int c[3] = {1, 3, 4};
-#pragma unroll
-    for (int j = 0; j < 3; j++)
-#pragma unroll
-        for (int d = 0; d < 3; d++) M[j * 3 + d] = (float)(x[c[j]][d] - x[0][d]);
-    return 6.0f * fabsf(det3(M)) / A;
-}
-
-// The same hex length from FP32 positions (any common origin).
 __device__ __forceinline__ float area_tri_pf(const float* p, const float* q, const float* r) {
     const float A0 = p[0] - r[0], A1 = p[1] - r[1], A2 = p[2] - r[2];
     const float B0 = q[0] - r[0], B1 = q[1] - r[1], B2 = q[2] - r[2];
     const float C0 = A1 * B2 - A2 * B1, C1 = A2 * B0 - A0 * B2, C2 = A0 * B1 - A1 * B0;
-    return 0.5f * sqrtf(C0 * C0 + C1 * C1 + C2 * C2);
+    return 0.5f * __builtin_amdgcn_sqrtf(C0 * C0 + C1 * C1 + C2 * C2);  // hardware sqrt, 1 ulp
 }
 
 __device__ __forceinline__ float hex_char_length_pf(const float (*x)[3]) {
@@ -450,7 +435,7 @@ __device__ __forceinline__ float hex_char_length_pf(const float (*x)[3]) {
     for (int j = 0; j < 3; j++)
 #pragma unroll
         for (int d = 0; d < 3; d++) M[j * 3 + d] = x[c[j]][d] - x[0][d];
-    return 6.0f * fabsf(det3(M)) / A;
+    return 6.0f * fabsf(det3(M)) * __builtin_amdgcn_rcpf(A);
 }
 
 __device__ __forceinline__ float tet_char_length_f32(const double (*x)[3]) {
@@ -461,7 +446,7 @@ __device__ __forceinline__ float tet_char_length_f32(const double (*x)[3]) {
     for (int j = 0; j < 3; j++)
 #pragma unroll
         for (int d = 0; d < 3; d++) M[j * 3 + d] = (float)(x[j + 1][d] - x[0][d]);
-    return fabsf(det3(M)) / A;
+    return fabsf(det3(M)) * __builtin_amdgcn_rcpf(A);
 }
 
 // Element contribution to the stable dt: std::min(l0, l)/c (solvers.cpp:317).

@@ -306,7 +306,7 @@ def main():
     valu_peak = VALU_ISSUE_PEAK  # wave64 VALU instructions per second (one per CU per cycle)
     valu = None if valu_wave is None else {
         "achieved": valu_wave * n_waves / (elem_avg_ms * 1e-3), "peak": valu_peak, "unit": "wave-instr/s",
-        "frac": valu_wave * n_waves / (elem_avg_ms * 1e-3) / valu_peak, "instr_per_element": valu_wave}
+        "frac": valu_wave * n_waves / (elem_avg_ms * 1e-3) / valu_peak, "instr_per_wave": valu_wave}
     out = {
         "metric": "elem-GP updates/s (1M hex8 NH explicit) + batched solves/s, p99 latency",
         "value": value,

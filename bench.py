@@ -76,11 +76,13 @@ def patch_partials(mesh):
 
 
 def elem_kernel_bytes(ne, nn, nen, n_part, per_elem_mat):
-    """Algorithmic HBM bytes of one element-kernel launch: per element conn 4 nen + l0 8 (+ material
-    24 when per element) + patch contribution list 2 nen; per partial its offset 2 + FP64 force write 24;
-    per node X and u, 24 B each (read once)."""
+    """Algorithmic HBM bytes of one element-kernel launch: per element its node references + l0 8 (+
+    material 24 when per element) + patch contribution list 2 nen; per partial its offset 2 + FP64 force
+    write 24; per node X and u, 24 B each (read once).  Node references: hex8 conn 4 nen; tet4 (node
+    staging) the local indices 2 nen per element and the patch node id 4 per partial."""
     n_patch = -(-ne // PATCH)
-    return ne * (4 * nen + 8 + (24 if per_elem_mat else 0) + 2 * nen) + n_part * (2 + 24) + 4 * (n_patch + 1) \
+    refs = ne * 4 * nen if nen == 8 else ne * 2 * nen + n_part * 4
+    return refs + ne * (8 + (24 if per_elem_mat else 0) + 2 * nen) + n_part * (2 + 24) + 4 * (n_patch + 1) \
         + nn * (24 + 24)
 
 

@@ -91,6 +91,44 @@ def test_mass_and_internal_force():
         assert relerr(fg, fo) <= 1e-12
 
 
+@pytest.mark.parametrize("precision", [G.PREC_FP64, G.PREC_FP32_ELEM])
+@pytest.mark.parametrize("law", [0, 2])
+def test_tet_node_staging_bitwise(precision, law, monkeypatch):
+    """The tet element kernel that stages each patch's distinct nodes in LDS gives bit for bit the
+    results of the kernel that gathers every element slot (MPS_TET_STAGING=0): displacements, dt
+    history and the standalone internal force, over a ragged last patch (1260 tets = 4 x 256 + 236),
+    heterogeneous materials, and both match the oracle to the form's bar."""
+    m = perturb(box(7, 6, 5, tet=True), 0.12, 9 + law)
+    assert m["n_elem"] % 256 != 0
+    rng = np.random.default_rng(5)
+    E = rng.uniform(5e5, 2e6, m["n_elem"])
+    nu = rng.uniform(0.2, 0.4, m["n_elem"])
+    rho = rng.uniform(800.0, 1200.0, m["n_elem"])
+    dd, dv = clamp_and_pull(m)
+    uf = rng.uniform(-1e-3, 1e-3, 3 * m["n_nodes"])
+    out = {}
+    for staged in (True, False):
+        if staged:
+            monkeypatch.delenv("MPS_TET_STAGING", raising=False)
+        else:
+            monkeypatch.setenv("MPS_TET_STAGING", "0")
+        with G.Solver(m, law=law, E=E, nu=nu, rho=rho, dir_dof=dd, dir_val=dv, n_steps=150,
+                      precision=precision, history_cap=200) as s:
+            f = s.internal_force(uf)
+            rc, rep = s.run(150)
+            assert rc == 0 and rep["steps"] == 150
+            u, v, a = s.get_state()
+            out[staged] = (u, v, f, s.history(0, 150)["dt"])
+    for k in range(4):
+        assert np.array_equal(out[True][k], out[False][k]), k
+    op = O.OracleProblem(m, law=law, E=E, nu=nu, rho=rho, dir_dof=dd, dir_val=dv, n_steps=150,
+                         sum_order=O.SUM_REFERENCE)
+    ro = op.run(hist_cap=150)
+    tol = TOL64 if precision == G.PREC_FP64 else TOL32
+    assert relerr(out[True][0], ro["u"]) <= tol
+    assert relerr(out[True][2], op.internal_force(uf)) <= (1e-12 if precision == G.PREC_FP64 else 1e-5)
+
+
 @pytest.mark.parametrize("law", [0, 1])
 def test_perturbed_hex_fp64_and_fp32(law):
     m = perturb(box(12, 12, 12), 0.1, 11)

@@ -177,6 +177,7 @@ int prepare(const mps_mesh* mesh, const mps_material* mat, const mps_bcs* bcs, c
         hp->ncol.resize(pnode.size());
         std::vector<int32_t> fill(hp->nrow.begin(), hp->nrow.end() - 1);
         for (size_t g = 0; g < pnode.size(); g++) hp->ncol[fill[pnode[g]]++] = (int32_t)g;
+        hp->pnd = std::move(pnode);
     }
     // materials
     hp->per_elem = mat->per_element != 0;
@@ -401,6 +402,36 @@ static int upload_instance(mps_solver_s* h, const HostPrep& hp) {
     HCHK(h, hipMemcpyAsync(nrow, hp.nrow.data(), (N + 1) * 4, hipMemcpyHostToDevice, h->stream));
     HCHK(h, hipMemcpyAsync(ncol, hp.ncol.data(), NP * 4, hipMemcpyHostToDevice, h->stream));
     HCHK(h, hipMemcpyAsync(ninfo, hp.ninfo.data(), N * 4, hipMemcpyHostToDevice, h->stream));
+    // tet4 node staging tables: local index of every element slot, and the largest patch node count,
+    // when the staging buffer and the force slots fit the 64 KB of LDS a workgroup may use.  The
+    // environment variable MPS_TET_STAGING=0 keeps the per-element gathers (same results, bit for bit).
+    const char* stage_env = getenv("MPS_TET_STAGING");
+    if (nen == 4 && !(stage_env && stage_env[0] == '0')) {
+        std::vector<uint16_t> plx((size_t)(Ne * nen));
+        int32_t pu = 0;
+        for (int64_t p = 0; p < (int64_t)NPA; p++) {
+            const int32_t b = hp.poff[p], U = hp.poff[p + 1] - b;
+            const uint16_t* pr = hp.prow.data() + b + p;
+            const uint16_t* pc = hp.pcol.data() + p * MPS_PATCH * nen;
+            for (int32_t j = 0; j < U; j++)
+                for (int q = pr[j]; q < pr[j + 1]; q++) plx[(size_t)p * MPS_PATCH * nen + pc[q]] = (uint16_t)j;
+            pu = std::max(pu, U);
+        }
+        const size_t lds = 48 * (size_t)pu + 12 * MPS_PATCH * 8 + 64;  // + the kernel's static block-min slots
+        if (lds <= 65536) {
+            int32_t* pnd;
+            uint16_t* plxd;
+            HCHK(h, dalloc(h, &pnd, NP * 4));
+            HCHK(h, dalloc(h, &plxd, Ne * nen * 2));
+            HCHK(h, hipMemcpyAsync(pnd, hp.pnd.data(), NP * 4, hipMemcpyHostToDevice, h->stream));
+            HCHK(h, hipMemcpyAsync(plxd, plx.data(), Ne * nen * 2, hipMemcpyHostToDevice, h->stream));
+            // the host vector must outlive the asynchronous copy
+            HCHK(h, hipStreamSynchronize(h->stream));
+            I.pnd = pnd;
+            I.plx = plxd;
+            I.pu_max = pu;
+        }
+    }
     HCHK(h, hipMemcpyAsync(bcval, hp.bcval.data(), hp.bcval.size() * 8, hipMemcpyHostToDevice, h->stream));
     HCHK(h, hipMemsetAsync(hist, 0xFF, 5 * (size_t)cap * 8, h->stream));  // NaN
     I.X = X;

@@ -17,6 +17,7 @@ struct HostPrep {
     int32_t n_patch = 0, n_part = 0;
     std::vector<int32_t> poff, nrow, ncol;
     std::vector<uint16_t> prow, pcol;
+    std::vector<int32_t> pnd;  // node id of each partial (the patches' distinct nodes, ascending per patch)
     bool per_elem = false;
     std::vector<double> mat, cel, rho;  // per element
     double pa = 0, pb = 0, c0 = 0, rho0 = 0;  // uniform

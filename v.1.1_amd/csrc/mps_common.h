@@ -18,6 +18,8 @@
 //     pcol          uint16[nen*Ne]         per patch, contributions i*nen+a (local element i,
 //                                          slot a) grouped by local node, ascending
 //     nrow/ncol     CSR node -> partials, ascending patch (= ascending element)
+//     pnd           int32[n_part]          tet4: node id of each partial (the patch's distinct nodes)
+//     plx           uint16[nen*Ne]         tet4: local node index (within its patch) of each element slot
 //   part            double[3*n_part]       partial nodal forces [(g)*3 + dir]; also the
 //                                          set-up scratch (>= nen*Ne doubles)
 //   loads           the n_lnode nodes with load items (flag bit 31 of ninfo), each with its items in input order:
@@ -128,6 +130,10 @@ struct MpsInst {
     double visc_c1, visc_cl;   // quadratic and linear coefficients
     double visc_h, visc_c0;    // element size and sound speed (> 0), or <= 0: the element's l0 and c
     double* uprev;
+    // tet4 node staging: the element kernel loads each patch's distinct nodes once into LDS
+    const int32_t* pnd;
+    const uint16_t* plx;
+    int32_t pu_max;  // largest number of distinct nodes of a patch
 };
 
 // One unit of work for the persistent batched kernels: a contiguous range of

@@ -762,22 +762,17 @@ __device__ __forceinline__ double hex8_element(const MpsInst& I, int e, const do
 // Tet4 element (1 Gauss point): classLinearTetrahedra.cpp:171-292.
 // J(i,j) = x_{j+1,i} - x_{0,i}; dphi_0 = -(sum of J^-1 rows), dphi_a = row a-1 of J^-1.
 // ---------------------------------------------------------------------------
+// tet4_core takes the element's node coordinates x and displacements uu; nd (node ids) is read only
+// by the viscous form.  tet4_element gathers them from X and u.
 template <typename T, int LAW, bool VISC = false>
-__device__ __forceinline__ double tet4_element(const MpsInst& I, int e, const double* __restrict__ u, T (*f)[3],
-                                               bool want_dt, double rdt = 0.0) {
-    const int4 q = *reinterpret_cast<const int4*>(I.conn + (size_t)e * 4);
-    const int nd[4] = {q.x, q.y, q.z, q.w};
+__device__ __forceinline__ double tet4_core(const MpsInst& I, int e, const int* nd, double (*x)[3], const double (*uu)[3],
+                                            T (*f)[3], bool want_dt, double rdt) {
     const size_t N = (size_t)I.N;
-    double x[4][3], uu[4][3];
-#pragma unroll
-    for (int a = 0; a < 4; a++)
-#pragma unroll
-        for (int d = 0; d < 3; d++) {
-            x[a][d] = I.X[d * N + nd[a]];
-            uu[a][d] = u[d * N + nd[a]];
-        }
     double pa, pb, c;
     elem_mat(I, e, pa, pb, c);
+    // a register value, so that the compiler does not turn the uniform/per-element choice of c into a
+    // pointer select with a private copy of the kernel argument
+    asm volatile("" : "+v"(c));
     T Jm[9], Du[9];
 #pragma unroll
     for (int i = 0; i < 3; i++)
@@ -832,6 +827,23 @@ __device__ __forceinline__ double tet4_element(const MpsInst& I, int e, const do
         for (int i = 0; i < 3; i++)
             f[a][i] = wJ * P[i * 3] * dphi[a][0] + wJ * P[i * 3 + 1] * dphi[a][1] + wJ * P[i * 3 + 2] * dphi[a][2];
     return dte;
+}
+
+template <typename T, int LAW, bool VISC = false>
+__device__ __forceinline__ double tet4_element(const MpsInst& I, int e, const double* __restrict__ u, T (*f)[3],
+                                               bool want_dt, double rdt = 0.0) {
+    const int4 q = *reinterpret_cast<const int4*>(I.conn + (size_t)e * 4);
+    const int nd[4] = {q.x, q.y, q.z, q.w};
+    const size_t N = (size_t)I.N;
+    double x[4][3], uu[4][3];
+#pragma unroll
+    for (int a = 0; a < 4; a++)
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            x[a][d] = I.X[d * N + nd[a]];
+            uu[a][d] = u[d * N + nd[a]];
+        }
+    return tet4_core<T, LAW, VISC>(I, e, nd, x, uu, f, want_dt, rdt);
 }
 
 // ---------------------------------------------------------------------------

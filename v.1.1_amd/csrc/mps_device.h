@@ -762,17 +762,28 @@ __device__ __forceinline__ double hex8_element(const MpsInst& I, int e, const do
 // Tet4 element (1 Gauss point): classLinearTetrahedra.cpp:171-292.
 // J(i,j) = x_{j+1,i} - x_{0,i}; dphi_0 = -(sum of J^-1 rows), dphi_a = row a-1 of J^-1.
 // ---------------------------------------------------------------------------
-// tet4_core takes the element's node coordinates x and displacements uu; nd (node ids) is read only
-// by the viscous form.  tet4_element gathers them from X and u.
+// Per-element constants of the tet kernels: law parameters, sound speed and undeformed length (read
+// only when the element's dt is wanted).  The empty asm makes c a register value, so that the compiler
+// does not turn the uniform/per-element choice into a pointer select with a private copy of the kernel
+// argument.
+struct TetConsts {
+    double pa, pb, c, l0;
+};
+__device__ __forceinline__ TetConsts tet_consts(const MpsInst& I, int e, bool want_dt) {
+    TetConsts k;
+    elem_mat(I, e, k.pa, k.pb, k.c);
+    asm volatile("" : "+v"(k.c));
+    k.l0 = want_dt ? I.l0[e] : 0.0;
+    return k;
+}
+
+// tet4_core takes the element's node coordinates x, displacements uu and constants; nd (node ids) is
+// read only by the viscous form.  tet4_element gathers them from X, u and the material arrays.
 template <typename T, int LAW, bool VISC = false>
 __device__ __forceinline__ double tet4_core(const MpsInst& I, int e, const int* nd, double (*x)[3], const double (*uu)[3],
-                                            T (*f)[3], bool want_dt, double rdt) {
+                                            const TetConsts& k, T (*f)[3], bool want_dt, double rdt) {
     const size_t N = (size_t)I.N;
-    double pa, pb, c;
-    elem_mat(I, e, pa, pb, c);
-    // a register value, so that the compiler does not turn the uniform/per-element choice of c into a
-    // pointer select with a private copy of the kernel argument
-    asm volatile("" : "+v"(c));
+    const double pa = k.pa, pb = k.pb, c = k.c;
     T Jm[9], Du[9];
 #pragma unroll
     for (int i = 0; i < 3; i++)
@@ -788,7 +799,7 @@ __device__ __forceinline__ double tet4_core(const MpsInst& I, int e, const int* 
 #pragma unroll
             for (int d = 0; d < 3; d++) x[a][d] = x[a][d] + uu[a][d];
         const double l = (sizeof(T) == 8) ? tet_char_length(x) : (double)tet_char_length_f32(x);
-        dte = elem_dt(l, I.l0[e], c);
+        dte = elem_dt(l, k.l0, c);
     }
     T iJ[9], H[9], P[9];
     T dJ = det3(Jm);
@@ -843,7 +854,7 @@ __device__ __forceinline__ double tet4_element(const MpsInst& I, int e, const do
             x[a][d] = I.X[d * N + nd[a]];
             uu[a][d] = u[d * N + nd[a]];
         }
-    return tet4_core<T, LAW, VISC>(I, e, nd, x, uu, f, want_dt, rdt);
+    return tet4_core<T, LAW, VISC>(I, e, nd, x, uu, tet_consts(I, e, want_dt), f, want_dt, rdt);
 }
 
 // ---------------------------------------------------------------------------

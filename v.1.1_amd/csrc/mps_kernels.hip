@@ -18,37 +18,91 @@ using namespace mpsd;
 // ---------------------------------------------------------------------------
 // Element kernels
 // ---------------------------------------------------------------------------
+// The patch's contribution list (pcol, NEN*ne entries, 16-byte aligned) for the kernels with one
+// work-item per element: each work-item loads its 8-entry pieces (pc8: entries q..q+7, those past m
+// read as 0) into registers at the start of the patch, so that their latency hides behind the element
+// work, and writes them to LDS just before the barrier that ends the element phase.
+__device__ __forceinline__ uint4 pc8(const uint16_t* __restrict__ gpc, int q, int m) {
+    if (q + 8 <= m) return *reinterpret_cast<const uint4*>(gpc + q);
+    uint32_t w[4] = {0u, 0u, 0u, 0u};
+#pragma unroll
+    for (int r = 0; r < 8; r++)
+        if (q + r < m) w[r >> 1] |= (uint32_t)gpc[q + r] << (16 * (r & 1));
+    return make_uint4(w[0], w[1], w[2], w[3]);
+}
+
+template <int NEN>
+struct PatchList {
+    static constexpr int NQ = (NEN * MPS_PATCH + 8 * MPS_PATCH - 1) / (8 * MPS_PATCH);  // pieces per work-item
+    uint4 v[NQ];
+    __device__ __forceinline__ void load(const MpsInst& I, int e0, int m) {
+        const uint16_t* gpc = I.pcol + (size_t)e0 * NEN;
+#pragma unroll
+        for (int r = 0; r < NQ; r++) v[r] = pc8(gpc, (threadIdx.x + r * MPS_PATCH) * 8, m);
+    }
+    __device__ __forceinline__ void store(uint16_t* spc, int m) const {
+#pragma unroll
+        for (int r = 0; r < NQ; r++) {
+            const int q = (threadIdx.x + r * MPS_PATCH) * 8;
+            if (q < m) *reinterpret_cast<uint4*>(spc + q) = v[r];
+        }
+    }
+};
+
 // Second half of a patch: after the barrier that follows the LDS force writes, the block sums, per
 // distinct node of the patch, its contributions in ascending (element, slot) order in FP64 and writes
-// one partial force per node.
-template <typename T, int NEN, int BS>
-__device__ __forceinline__ void patch_sums(const MpsInst& I, int p, const T* __restrict__ sf) {
-    const int e0 = p * MPS_PATCH;
+// one partial force per node.  With PRE (one work-item per element, BS = MPS_PATCH) the contribution
+// list is in LDS (spc) and the offsets of all the work-item's nodes are loaded first, in one round
+// trip; otherwise (the persistent batched kernels) both are read from global memory as they are needed.
+template <typename T>
+__device__ __forceinline__ void node_sum(const MpsInst& I, int g, int q0, int q1, int nen, const T* __restrict__ sf,
+                                         const uint16_t* __restrict__ pc) {
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+    for (int q = q0; q < q1; q++) {
+        const int c = pc[q];
+        const int i = c / nen, a = c % nen;
+        s0 += (double)sf[(a * 3 + 0) * MPS_PATCH + i];
+        s1 += (double)sf[(a * 3 + 1) * MPS_PATCH + i];
+        s2 += (double)sf[(a * 3 + 2) * MPS_PATCH + i];
+    }
+    if (sizeof(T) == 4) {  // precision form: partials rounded once to FP32, summed in FP64 by the node kernel
+        float* dst = reinterpret_cast<float*>(I.part) + 3 * (size_t)g;
+        dst[0] = (float)s0;
+        dst[1] = (float)s1;
+        dst[2] = (float)s2;
+    } else {
+        double* dst = I.part + 3 * (size_t)g;
+        dst[0] = s0;
+        dst[1] = s1;
+        dst[2] = s2;
+    }
+}
+
+template <typename T, int NEN, int BS, bool PRE>
+__device__ __forceinline__ void patch_sums(const MpsInst& I, int p, const T* __restrict__ sf,
+                                           const uint16_t* __restrict__ spc) {
+    static_assert(!PRE || BS == MPS_PATCH, "PRE needs one work-item per element");
     const int base = I.poff[p];
     const int U = I.poff[p + 1] - base;
     const uint16_t* __restrict__ pr = I.prow + base + p;
-    const uint16_t* __restrict__ pc = I.pcol + (size_t)e0 * NEN;
-    for (int j = threadIdx.x; j < U; j += BS) {
-        double s0 = 0.0, s1 = 0.0, s2 = 0.0;
-        const int q1 = pr[j + 1];
-        for (int q = pr[j]; q < q1; q++) {
-            const int c = pc[q];
-            const int i = c / NEN, a = c % NEN;
-            s0 += (double)sf[(a * 3 + 0) * MPS_PATCH + i];
-            s1 += (double)sf[(a * 3 + 1) * MPS_PATCH + i];
-            s2 += (double)sf[(a * 3 + 2) * MPS_PATCH + i];
+    if constexpr (PRE) {
+        constexpr int NJ = (NEN * MPS_PATCH + BS) / BS;  // ceil((NEN*MPS_PATCH + 1) / BS) >= nodes per work-item
+        int q0[NJ], q1[NJ];
+#pragma unroll
+        for (int r = 0; r < NJ; r++) {
+            const int j = threadIdx.x + r * BS;
+            q0[r] = j < U ? pr[j] : 0;
+            q1[r] = j < U ? pr[j + 1] : 0;
         }
-        if (sizeof(T) == 4) {  // precision form: partials rounded once to FP32, summed in FP64 by the node kernel
-            float* dst = reinterpret_cast<float*>(I.part) + 3 * (size_t)(base + j);
-            dst[0] = (float)s0;
-            dst[1] = (float)s1;
-            dst[2] = (float)s2;
-        } else {
-            double* dst = I.part + 3 * (size_t)(base + j);
-            dst[0] = s0;
-            dst[1] = s1;
-            dst[2] = s2;
+#pragma unroll
+        for (int r = 0; r < NJ; r++) {
+            const int j = threadIdx.x + r * BS;
+            if (j >= U) break;
+            node_sum<T>(I, base + j, q0[r], q1[r], NEN, sf, spc);
         }
+    } else {
+        const uint16_t* __restrict__ pc = I.pcol + (size_t)p * MPS_PATCH * NEN;
+        for (int j = threadIdx.x; j < U; j += BS) node_sum<T>(I, base + j, pr[j], pr[j + 1], NEN, sf, pc);
     }
 }
 
@@ -58,11 +112,13 @@ __device__ __forceinline__ void patch_sums(const MpsInst& I, int p, const T* __r
 // force per node to part.  No float atomics: the node kernel adds the partials of a node in
 // ascending patch order, so the sum order is fixed.  Returns the block's min element dt
 // candidate (64-bit pattern, lanes other than those computing elements hold +inf).
-template <typename T, int NEN, int LAW, int BS, bool VISC = false>
+template <typename T, int NEN, int LAW, int BS, bool VISC = false, bool PRE = false>
 __device__ __forceinline__ unsigned long long patch_forces(const MpsInst& I, int p, bool want_dt, T* __restrict__ sf,
-                                                           double rdt = 0.0) {
+                                                           uint16_t* __restrict__ spc, double rdt = 0.0) {
     const int e0 = p * MPS_PATCH;
     const int ne = min(MPS_PATCH, I.Ne - e0);
+    PatchList<NEN> pl;
+    if constexpr (PRE) pl.load(I, e0, ne * NEN);
     unsigned long long dmin = INF_BITS;
     for (int i = threadIdx.x; i < ne; i += BS) {
         T f[NEN][3];
@@ -74,8 +130,9 @@ __device__ __forceinline__ unsigned long long patch_forces(const MpsInst& I, int
             for (int d = 0; d < 3; d++) sf[(a * 3 + d) * MPS_PATCH + i] = f[a][d];
         dmin = umin64(dmin, d2bits(dte));
     }
+    if constexpr (PRE) pl.store(spc, ne * NEN);
     __syncthreads();
-    patch_sums<T, NEN, BS>(I, p, sf);
+    patch_sums<T, NEN, BS, PRE>(I, p, sf, spc);
     return dmin;
 }
 
@@ -86,6 +143,7 @@ __device__ __forceinline__ unsigned long long patch_forces(const MpsInst& I, int
 template <typename T, int NEN, int LAW, bool VISC = false>
 __global__ __launch_bounds__(MPS_PATCH) void k_elem(MpsInst I, int mode) {
     __shared__ T sf[NEN * 3 * MPS_PATCH];
+    __shared__ uint16_t spc[NEN * MPS_PATCH];
     __shared__ unsigned long long sh[MPS_PATCH / 64];
     long long k = 0;
     double rdt = 0.0;
@@ -97,7 +155,7 @@ __global__ __launch_bounds__(MPS_PATCH) void k_elem(MpsInst I, int mode) {
         if (VISC) rdt = (s.dt == 0.0) ? 0.0 : 1 / s.dt;
     }
     const int p = xcd_tile(blockIdx.x, gridDim.x);
-    unsigned long long m = patch_forces<T, NEN, LAW, MPS_PATCH, VISC>(I, p, mode == 0, sf, rdt);
+    unsigned long long m = patch_forces<T, NEN, LAW, MPS_PATCH, VISC, true>(I, p, mode == 0, sf, spc, rdt);
     if (mode == 0) {
         m = block_min_u64(m, sh);
         if (threadIdx.x == 0) atomicMin(dt_slot(I, (k + 1) & 1, blockIdx.x & (MPS_DT_SLOTS - 1)), m);
@@ -108,8 +166,8 @@ __global__ __launch_bounds__(MPS_PATCH) void k_elem(MpsInst I, int mode) {
 // its patch (ascending ids, so the loads run along node rows) into LDS, X and u SoA by direction; each
 // element then reads its four nodes from LDS through its local indices plx.  Every node is fetched
 // once per patch instead of once per element slot (about 1 against 5.8 node loads per partial on a
-// structured tet mesh), which is what limits the gather-bound tet kernel.  Dynamic LDS: 48*pu_max
-// bytes of staging, then the 12*MPS_PATCH force slots of T.
+// structured tet mesh).  Dynamic LDS: 48*pu_max bytes of staging, then the 12*MPS_PATCH force slots
+// of T, then the 4*MPS_PATCH entries of the contribution list.
 template <typename T, int LAW>
 __global__ __launch_bounds__(MPS_PATCH) void k_elem_tet_staged(MpsInst I, int mode) {
     extern __shared__ double smem[];
@@ -117,6 +175,7 @@ __global__ __launch_bounds__(MPS_PATCH) void k_elem_tet_staged(MpsInst I, int mo
     const int pu = I.pu_max;
     double* __restrict__ sxu = smem;                                  // [6][pu]: X (3), then u (3)
     T* __restrict__ sf = reinterpret_cast<T*>(smem + 6 * (size_t)pu);  // [12][MPS_PATCH]
+    uint16_t* __restrict__ spc = reinterpret_cast<uint16_t*>(sf + 12 * MPS_PATCH);  // [4*MPS_PATCH]
     long long k = 0;
     if (mode == 0) {
         k = I.ctl->A;
@@ -130,6 +189,13 @@ __global__ __launch_bounds__(MPS_PATCH) void k_elem_tet_staged(MpsInst I, int mo
     const int base = I.poff[p];
     const int U = I.poff[p + 1] - base;  // <= pu_max (set-up)
     const size_t N = (size_t)I.N;
+    PatchList<4> pl;
+    pl.load(I, e0, ne * 4);
+    // the element's own inputs are loaded before the staging barrier, so their latency overlaps it
+    const int i = threadIdx.x;
+    const int e = e0 + min(i, ne - 1);
+    const uint2 l4 = *reinterpret_cast<const uint2*>(I.plx + (size_t)e * 4);
+    const TetConsts kc = tet_consts(I, e, mode == 0);
     for (int j = threadIdx.x; j < U; j += MPS_PATCH) {
         const size_t n = (size_t)I.pnd[base + j];
 #pragma unroll
@@ -140,10 +206,7 @@ __global__ __launch_bounds__(MPS_PATCH) void k_elem_tet_staged(MpsInst I, int mo
     }
     __syncthreads();
     unsigned long long dmin = INF_BITS;
-    const int i = threadIdx.x;
     if (i < ne) {
-        const int e = e0 + i;
-        const uint2 l4 = *reinterpret_cast<const uint2*>(I.plx + (size_t)e * 4);
         const int l[4] = {(int)(l4.x & 0xffffu), (int)(l4.x >> 16), (int)(l4.y & 0xffffu), (int)(l4.y >> 16)};
         double x[4][3], uu[4][3];
 #pragma unroll
@@ -154,15 +217,16 @@ __global__ __launch_bounds__(MPS_PATCH) void k_elem_tet_staged(MpsInst I, int mo
                 uu[a][d] = sxu[(3 + d) * pu + l[a]];
             }
         T f[4][3];
-        const double dte = tet4_core<T, LAW, false>(I, e, nullptr, x, uu, f, mode == 0, 0.0);
+        const double dte = tet4_core<T, LAW, false>(I, e, nullptr, x, uu, kc, f, mode == 0, 0.0);
 #pragma unroll
         for (int a = 0; a < 4; a++)
 #pragma unroll
             for (int d = 0; d < 3; d++) sf[(a * 3 + d) * MPS_PATCH + i] = f[a][d];
         dmin = d2bits(dte);
     }
+    pl.store(spc, ne * 4);
     __syncthreads();
-    patch_sums<T, 4, MPS_PATCH>(I, p, sf);
+    patch_sums<T, 4, MPS_PATCH, true>(I, p, sf, spc);
     if (mode == 0) {
         dmin = block_min_u64(dmin, sh);
         if (threadIdx.x == 0) atomicMin(dt_slot(I, (k + 1) & 1, blockIdx.x & (MPS_DT_SLOTS - 1)), dmin);
@@ -198,7 +262,7 @@ __global__ __launch_bounds__(BS) void k_elem_batch(const MpsInst* __restrict__ i
         const long long k = I.ctl->A;
         if (W.first && threadIdx.x == 0) I.ctl->B = k;
         if (I.ctl->slot[k & 1].go) {
-            unsigned long long m = patch_forces<T, NEN, LAW, BS>(I, W.begin / MPS_PATCH, true, sf);
+            unsigned long long m = patch_forces<T, NEN, LAW, BS>(I, W.begin / MPS_PATCH, true, sf, nullptr);
             m = block_min_u64(m, sh);
             if (threadIdx.x == 0) atomicMin(dt_slot(I, (k + 1) & 1, w & (MPS_DT_SLOTS - 1)), m);
         }
@@ -850,7 +914,7 @@ hipError_t launch_elem(const MpsInst& I, int mode, hipStream_t st) {
         if (I.precision == 1) MPS_ELEM_LAUNCH(float, 8);
         else MPS_ELEM_LAUNCH(double, 8);
     } else if (!visc && I.pnd != nullptr) {
-        const size_t lds = 48 * (size_t)I.pu_max + 12 * MPS_PATCH * (I.precision == 1 ? 4 : 8);
+        const size_t lds = 48 * (size_t)I.pu_max + 12 * MPS_PATCH * (I.precision == 1 ? 4 : 8) + 8 * MPS_PATCH;
 #define MPS_TET_STAGED(T)                                                                        \
     switch (I.law) {                                                                             \
         case 0: k_elem_tet_staged<T, 0><<<nb, MPS_PATCH, lds, st>>>(I, mode); break;             \

@@ -598,22 +598,30 @@ __device__ __forceinline__ double hex8_element(const MpsInst& I, int e, const do
     double pa, pb, c;
     elem_mat(I, e, pa, pb, c);
     T Ax[7][3], Au[7][3], G[7][3];
+    double dte = bits2d(0x7FF0000000000000ull);
     {
-        double x[8][3], cx[7][3];
+        double xX[8][3], xU[8][3], cx[7][3];
 #pragma unroll
         for (int a = 0; a < 8; a++)
 #pragma unroll
-            for (int d = 0; d < 3; d++) x[a][d] = I.X[d * N + nd[a]];
-        hex_butterfly(x, cx);
+            for (int d = 0; d < 3; d++) {
+                xX[a][d] = I.X[d * N + nd[a]];
+                xU[a][d] = u[d * N + nd[a]];
+            }
+        if (want_dt && sizeof(T) == 8) {
+            // deformed characteristic length (solvers.cpp:300-317), strict FP64: bitwise the reference's.
+            // Taken here, while the gathered nodes are in registers: after the Gauss-point loop the
+            // nodes would have to be read again, with their latency exposed.
+            double x[8][3];
+#pragma unroll
+            for (int a = 0; a < 8; a++)
+#pragma unroll
+                for (int d = 0; d < 3; d++) x[a][d] = xX[a][d] + xU[a][d];
+            dte = elem_dt(hex_char_length(x), I.l0[e], c);
+        }
+        hex_butterfly(xX, cx);
         hex_scale<T>(cx, Ax);
-    }
-    {
-        double x[8][3], cx[7][3];
-#pragma unroll
-        for (int a = 0; a < 8; a++)
-#pragma unroll
-            for (int d = 0; d < 3; d++) x[a][d] = u[d * N + nd[a]];
-        hex_butterfly(x, cx);
+        hex_butterfly(xU, cx);
         hex_scale<T>(cx, Au);
     }
     T qa = T(0), qb = T(0), trs = T(0);
@@ -718,7 +726,6 @@ __device__ __forceinline__ double hex8_element(const MpsInst& I, int e, const do
         G[6][i] *= s3;
     }
     hex_unbutterfly<T>(G, f);
-    double dte = bits2d(0x7FF0000000000000ull);
     if (want_dt && sizeof(T) == 4) {
         // precision form: FP32 length from the deformed node positions relative to their centroid,
         // x_a - xbar = sum_m sigma_m(a) C_m / 8 with C the butterfly coefficients of X + u (Ax + Au
@@ -737,23 +744,6 @@ __device__ __forceinline__ double hex8_element(const MpsInst& I, int e, const do
         }
         hex_unbutterfly<T>(R, xr);
         dte = elem_dt((double)hex_char_length_pf((const float(*)[3])xr), I.l0[e], c);
-    } else if (want_dt) {
-        // deformed characteristic length (solvers.cpp:300-317), strict FP64: bitwise the reference's.
-        // X and u are re-read (cache hits). The empty asm makes the node ids opaque so that the 48
-        // gather addresses are not kept live across the Gauss-point loop.
-        int ndr[8];
-#pragma unroll
-        for (int a = 0; a < 8; a++) {
-            ndr[a] = nd[a];
-            asm volatile("" : "+v"(ndr[a]));
-        }
-        double x[8][3];
-#pragma unroll
-        for (int a = 0; a < 8; a++)
-#pragma unroll
-            for (int d = 0; d < 3; d++) x[a][d] = I.X[d * N + ndr[a]] + u[d * N + ndr[a]];
-        const double l = hex_char_length(x);
-        dte = elem_dt(l, I.l0[e], c);
     }
     return dte;
 }

@@ -368,8 +368,9 @@ static int upload_instance(mps_solver_s* h, const HostPrep& hp) {
     HCHK(h, dalloc(h, &bcval, hp.bcval.size() * 8));
     HCHK(h, dalloc(h, &pforce, pf_bytes));
     HCHK(h, dalloc(h, &poff, (NPA + 1) * 4));
-    HCHK(h, dalloc(h, &prow, (NP + NPA) * 2));
-    HCHK(h, dalloc(h, &pcol, Ne * nen * 2));
+    // prow and pcol carry 8 entries of padding: the element kernels read them in 8-entry pieces
+    HCHK(h, dalloc(h, &prow, (NP + NPA + 8) * 2));
+    HCHK(h, dalloc(h, &pcol, (Ne * nen + 8) * 2));
     HCHK(h, dalloc(h, &nrow, (N + 1) * 4));
     HCHK(h, dalloc(h, &ncol, NP * 4));
     HCHK(h, dalloc(h, &ctl, sizeof(MpsCtl)));
@@ -399,6 +400,8 @@ static int upload_instance(mps_solver_s* h, const HostPrep& hp) {
     HCHK(h, hipMemcpyAsync(poff, hp.poff.data(), (NPA + 1) * 4, hipMemcpyHostToDevice, h->stream));
     HCHK(h, hipMemcpyAsync(prow, hp.prow.data(), (NP + NPA) * 2, hipMemcpyHostToDevice, h->stream));
     HCHK(h, hipMemcpyAsync(pcol, hp.pcol.data(), Ne * nen * 2, hipMemcpyHostToDevice, h->stream));
+    HCHK(h, hipMemsetAsync(prow + NP + NPA, 0, 16, h->stream));
+    HCHK(h, hipMemsetAsync(pcol + Ne * nen, 0, 16, h->stream));
     HCHK(h, hipMemcpyAsync(nrow, hp.nrow.data(), (N + 1) * 4, hipMemcpyHostToDevice, h->stream));
     HCHK(h, hipMemcpyAsync(ncol, hp.ncol.data(), NP * 4, hipMemcpyHostToDevice, h->stream));
     HCHK(h, hipMemcpyAsync(ninfo, hp.ninfo.data(), N * 4, hipMemcpyHostToDevice, h->stream));
@@ -417,8 +420,9 @@ static int upload_instance(mps_solver_s* h, const HostPrep& hp) {
                 for (int q = pr[j]; q < pr[j + 1]; q++) plx[(size_t)p * MPS_PATCH * nen + pc[q]] = (uint16_t)j;
             pu = std::max(pu, U);
         }
-        // staging, force slots, contribution list, and the kernel's static block-min slots
-        const size_t lds = 48 * (size_t)pu + 12 * MPS_PATCH * 8 + 8 * MPS_PATCH + 64;
+        // staging, force slots, contribution list, staged node offsets (at most 2.1 KB), and the kernel's
+        // static block-min slots
+        const size_t lds = 48 * (size_t)pu + 12 * MPS_PATCH * 8 + 8 * MPS_PATCH + 2200 + 64;
         if (lds <= 65536) {
             int32_t* pnd;
             uint16_t* plxd;

@@ -15,8 +15,10 @@
 //                   nodes (ascending) own partial forces part[poff[p] .. poff[p+1])
 //     poff          int32[n_patch+1]       first partial of each patch
 //     prow          uint16[n_part+n_patch] per patch, U_p+1 offsets into its pcol list
+//                                          (+8 entries of padding in the single-instance form)
 //     pcol          uint16[nen*Ne]         per patch, contributions i*nen+a (local element i,
-//                                          slot a) grouped by local node, ascending
+//                                          slot a) grouped by local node, ascending (+8 entries of
+//                                          padding in the single-instance form)
 //     nrow/ncol     CSR node -> partials, ascending patch (= ascending element)
 //     pnd           int32[n_part]          tet4: node id of each partial (the patch's distinct nodes)
 //     plx           uint16[nen*Ne]         tet4: local node index (within its patch) of each element slot

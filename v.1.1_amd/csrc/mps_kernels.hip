@@ -18,33 +18,44 @@ using namespace mpsd;
 // ---------------------------------------------------------------------------
 // Element kernels
 // ---------------------------------------------------------------------------
-// The patch's contribution list (pcol, NEN*ne entries, 16-byte aligned) for the kernels with one
-// work-item per element: each work-item loads its 8-entry pieces (pc8: entries q..q+7, those past m
-// read as 0) into registers at the start of the patch, so that their latency hides behind the element
-// work, and writes them to LDS just before the barrier that ends the element phase.
-__device__ __forceinline__ uint4 pc8(const uint16_t* __restrict__ gpc, int q, int m) {
-    if (q + 8 <= m) return *reinterpret_cast<const uint4*>(gpc + q);
-    uint32_t w[4] = {0u, 0u, 0u, 0u};
-#pragma unroll
-    for (int r = 0; r < 8; r++)
-        if (q + r < m) w[r >> 1] |= (uint32_t)gpc[q + r] << (16 * (r & 1));
-    return make_uint4(w[0], w[1], w[2], w[3]);
+// The patch's lists for the kernels with one work-item per element: its contribution list (pcol,
+// NEN*ne entries, 16-byte aligned) and its node offsets (prow, U+1 entries from entry poff[p]+p,
+// staged from the 8-entry boundary below).  Each work-item loads its 8-entry pieces (pc8: entries
+// q..q+7 when q < m; the single-instance set-up pads both arrays by 8 entries, so a piece that
+// straddles m stays inside the allocation and the entries past m are never used) into registers at
+// the start of the patch, so that their latency hides behind the element work, and writes them to LDS
+// just before the barrier that ends the element phase.
+__device__ __forceinline__ uint4 pc8(const uint16_t* __restrict__ g, int q, int m) {
+    return q < m ? *reinterpret_cast<const uint4*>(g + q) : make_uint4(0u, 0u, 0u, 0u);
 }
 
 template <int NEN>
 struct PatchList {
-    static constexpr int NQ = (NEN * MPS_PATCH + 8 * MPS_PATCH - 1) / (8 * MPS_PATCH);  // pieces per work-item
-    uint4 v[NQ];
-    __device__ __forceinline__ void load(const MpsInst& I, int e0, int m) {
+    static constexpr int NQ = (NEN * MPS_PATCH + 8 * MPS_PATCH - 1) / (8 * MPS_PATCH);  // pcol pieces per work-item
+    static constexpr int NR = ((NEN * MPS_PATCH + 1 + 7 + 7) / 8 + MPS_PATCH - 1) / MPS_PATCH;  // prow pieces
+    static constexpr int SPR = (NEN * MPS_PATCH + 15) / 8 * 8;  // LDS entries for the staged prow (>= len rounded up to 8)
+    uint4 v[NQ], w[NR];
+    int off, len;  // first prow entry of the patch relative to the staged start; staged entries
+    __device__ __forceinline__ void load(const MpsInst& I, int p, int e0, int m, int U) {
         const uint16_t* gpc = I.pcol + (size_t)e0 * NEN;
 #pragma unroll
         for (int r = 0; r < NQ; r++) v[r] = pc8(gpc, (threadIdx.x + r * MPS_PATCH) * 8, m);
+        const int s = I.poff[p] + p, a0 = s & ~7;
+        off = s - a0;
+        len = off + U + 1;
+#pragma unroll
+        for (int r = 0; r < NR; r++) w[r] = pc8(I.prow + a0, (threadIdx.x + r * MPS_PATCH) * 8, len);
     }
-    __device__ __forceinline__ void store(uint16_t* spc, int m) const {
+    __device__ __forceinline__ void store(uint16_t* spc, uint16_t* spr, int m) const {
 #pragma unroll
         for (int r = 0; r < NQ; r++) {
             const int q = (threadIdx.x + r * MPS_PATCH) * 8;
             if (q < m) *reinterpret_cast<uint4*>(spc + q) = v[r];
+        }
+#pragma unroll
+        for (int r = 0; r < NR; r++) {
+            const int q = (threadIdx.x + r * MPS_PATCH) * 8;
+            if (q < len) *reinterpret_cast<uint4*>(spr + q) = w[r];
         }
     }
 };
@@ -52,8 +63,8 @@ struct PatchList {
 // Second half of a patch: after the barrier that follows the LDS force writes, the block sums, per
 // distinct node of the patch, its contributions in ascending (element, slot) order in FP64 and writes
 // one partial force per node.  With PRE (one work-item per element, BS = MPS_PATCH) the contribution
-// list is in LDS (spc) and the offsets of all the work-item's nodes are loaded first, in one round
-// trip; otherwise (the persistent batched kernels) both are read from global memory as they are needed.
+// list (spc) and the node offsets (spr, the patch's first at spr[0]) are in LDS; otherwise (the
+// persistent batched kernels) both are read from global memory as they are needed.
 template <typename T>
 __device__ __forceinline__ void node_sum(const MpsInst& I, int g, int q0, int q1, int nen, const T* __restrict__ sf,
                                          const uint16_t* __restrict__ pc) {
@@ -80,27 +91,14 @@ __device__ __forceinline__ void node_sum(const MpsInst& I, int g, int q0, int q1
 
 template <typename T, int NEN, int BS, bool PRE>
 __device__ __forceinline__ void patch_sums(const MpsInst& I, int p, const T* __restrict__ sf,
-                                           const uint16_t* __restrict__ spc) {
+                                           const uint16_t* __restrict__ spc, const uint16_t* __restrict__ spr) {
     static_assert(!PRE || BS == MPS_PATCH, "PRE needs one work-item per element");
     const int base = I.poff[p];
     const int U = I.poff[p + 1] - base;
-    const uint16_t* __restrict__ pr = I.prow + base + p;
     if constexpr (PRE) {
-        constexpr int NJ = (NEN * MPS_PATCH + BS) / BS;  // ceil((NEN*MPS_PATCH + 1) / BS) >= nodes per work-item
-        int q0[NJ], q1[NJ];
-#pragma unroll
-        for (int r = 0; r < NJ; r++) {
-            const int j = threadIdx.x + r * BS;
-            q0[r] = j < U ? pr[j] : 0;
-            q1[r] = j < U ? pr[j + 1] : 0;
-        }
-#pragma unroll
-        for (int r = 0; r < NJ; r++) {
-            const int j = threadIdx.x + r * BS;
-            if (j >= U) break;
-            node_sum<T>(I, base + j, q0[r], q1[r], NEN, sf, spc);
-        }
+        for (int j = threadIdx.x; j < U; j += BS) node_sum<T>(I, base + j, spr[j], spr[j + 1], NEN, sf, spc);
     } else {
+        const uint16_t* __restrict__ pr = I.prow + base + p;
         const uint16_t* __restrict__ pc = I.pcol + (size_t)p * MPS_PATCH * NEN;
         for (int j = threadIdx.x; j < U; j += BS) node_sum<T>(I, base + j, pr[j], pr[j + 1], NEN, sf, pc);
     }
@@ -114,11 +112,12 @@ __device__ __forceinline__ void patch_sums(const MpsInst& I, int p, const T* __r
 // candidate (64-bit pattern, lanes other than those computing elements hold +inf).
 template <typename T, int NEN, int LAW, int BS, bool VISC = false, bool PRE = false>
 __device__ __forceinline__ unsigned long long patch_forces(const MpsInst& I, int p, bool want_dt, T* __restrict__ sf,
-                                                           uint16_t* __restrict__ spc, double rdt = 0.0) {
+                                                           uint16_t* __restrict__ spc, uint16_t* __restrict__ spr,
+                                                           double rdt = 0.0) {
     const int e0 = p * MPS_PATCH;
     const int ne = min(MPS_PATCH, I.Ne - e0);
     PatchList<NEN> pl;
-    if constexpr (PRE) pl.load(I, e0, ne * NEN);
+    if constexpr (PRE) pl.load(I, p, e0, ne * NEN, I.poff[p + 1] - I.poff[p]);
     unsigned long long dmin = INF_BITS;
     for (int i = threadIdx.x; i < ne; i += BS) {
         T f[NEN][3];
@@ -130,9 +129,9 @@ __device__ __forceinline__ unsigned long long patch_forces(const MpsInst& I, int
             for (int d = 0; d < 3; d++) sf[(a * 3 + d) * MPS_PATCH + i] = f[a][d];
         dmin = umin64(dmin, d2bits(dte));
     }
-    if constexpr (PRE) pl.store(spc, ne * NEN);
+    if constexpr (PRE) pl.store(spc, spr, ne * NEN);
     __syncthreads();
-    patch_sums<T, NEN, BS, PRE>(I, p, sf, spc);
+    patch_sums<T, NEN, BS, PRE>(I, p, sf, spc, spr + (PRE ? pl.off : 0));
     return dmin;
 }
 
@@ -143,7 +142,8 @@ __device__ __forceinline__ unsigned long long patch_forces(const MpsInst& I, int
 template <typename T, int NEN, int LAW, bool VISC = false>
 __global__ __launch_bounds__(MPS_PATCH) void k_elem(MpsInst I, int mode) {
     __shared__ T sf[NEN * 3 * MPS_PATCH];
-    __shared__ uint16_t spc[NEN * MPS_PATCH];
+    __shared__ __align__(16) uint16_t spc[NEN * MPS_PATCH];
+    __shared__ __align__(16) uint16_t spr[PatchList<NEN>::SPR];
     __shared__ unsigned long long sh[MPS_PATCH / 64];
     long long k = 0;
     double rdt = 0.0;
@@ -155,7 +155,7 @@ __global__ __launch_bounds__(MPS_PATCH) void k_elem(MpsInst I, int mode) {
         if (VISC) rdt = (s.dt == 0.0) ? 0.0 : 1 / s.dt;
     }
     const int p = xcd_tile(blockIdx.x, gridDim.x);
-    unsigned long long m = patch_forces<T, NEN, LAW, MPS_PATCH, VISC, true>(I, p, mode == 0, sf, spc, rdt);
+    unsigned long long m = patch_forces<T, NEN, LAW, MPS_PATCH, VISC, true>(I, p, mode == 0, sf, spc, spr, rdt);
     if (mode == 0) {
         m = block_min_u64(m, sh);
         if (threadIdx.x == 0) atomicMin(dt_slot(I, (k + 1) & 1, blockIdx.x & (MPS_DT_SLOTS - 1)), m);
@@ -167,7 +167,7 @@ __global__ __launch_bounds__(MPS_PATCH) void k_elem(MpsInst I, int mode) {
 // element then reads its four nodes from LDS through its local indices plx.  Every node is fetched
 // once per patch instead of once per element slot (about 1 against 5.8 node loads per partial on a
 // structured tet mesh).  Dynamic LDS: 48*pu_max bytes of staging, then the 12*MPS_PATCH force slots
-// of T, then the 4*MPS_PATCH entries of the contribution list.
+// of T, the 4*MPS_PATCH entries of the contribution list and the staged node offsets.
 template <typename T, int LAW>
 __global__ __launch_bounds__(MPS_PATCH) void k_elem_tet_staged(MpsInst I, int mode) {
     extern __shared__ double smem[];
@@ -176,6 +176,7 @@ __global__ __launch_bounds__(MPS_PATCH) void k_elem_tet_staged(MpsInst I, int mo
     double* __restrict__ sxu = smem;                                  // [6][pu]: X (3), then u (3)
     T* __restrict__ sf = reinterpret_cast<T*>(smem + 6 * (size_t)pu);  // [12][MPS_PATCH]
     uint16_t* __restrict__ spc = reinterpret_cast<uint16_t*>(sf + 12 * MPS_PATCH);  // [4*MPS_PATCH]
+    uint16_t* __restrict__ spr = spc + 4 * MPS_PATCH;                                // [PatchList<4>::SPR]
     long long k = 0;
     if (mode == 0) {
         k = I.ctl->A;
@@ -190,7 +191,7 @@ __global__ __launch_bounds__(MPS_PATCH) void k_elem_tet_staged(MpsInst I, int mo
     const int U = I.poff[p + 1] - base;  // <= pu_max (set-up)
     const size_t N = (size_t)I.N;
     PatchList<4> pl;
-    pl.load(I, e0, ne * 4);
+    pl.load(I, p, e0, ne * 4, U);
     // the element's own inputs are loaded before the staging barrier, so their latency overlaps it
     const int i = threadIdx.x;
     const int e = e0 + min(i, ne - 1);
@@ -224,9 +225,9 @@ __global__ __launch_bounds__(MPS_PATCH) void k_elem_tet_staged(MpsInst I, int mo
             for (int d = 0; d < 3; d++) sf[(a * 3 + d) * MPS_PATCH + i] = f[a][d];
         dmin = d2bits(dte);
     }
-    pl.store(spc, ne * 4);
+    pl.store(spc, spr, ne * 4);
     __syncthreads();
-    patch_sums<T, 4, MPS_PATCH, true>(I, p, sf, spc);
+    patch_sums<T, 4, MPS_PATCH, true>(I, p, sf, spc, spr + pl.off);
     if (mode == 0) {
         dmin = block_min_u64(dmin, sh);
         if (threadIdx.x == 0) atomicMin(dt_slot(I, (k + 1) & 1, blockIdx.x & (MPS_DT_SLOTS - 1)), dmin);
@@ -262,7 +263,7 @@ __global__ __launch_bounds__(BS) void k_elem_batch(const MpsInst* __restrict__ i
         const long long k = I.ctl->A;
         if (W.first && threadIdx.x == 0) I.ctl->B = k;
         if (I.ctl->slot[k & 1].go) {
-            unsigned long long m = patch_forces<T, NEN, LAW, BS>(I, W.begin / MPS_PATCH, true, sf, nullptr);
+            unsigned long long m = patch_forces<T, NEN, LAW, BS>(I, W.begin / MPS_PATCH, true, sf, nullptr, nullptr);
             m = block_min_u64(m, sh);
             if (threadIdx.x == 0) atomicMin(dt_slot(I, (k + 1) & 1, w & (MPS_DT_SLOTS - 1)), m);
         }
@@ -914,7 +915,8 @@ hipError_t launch_elem(const MpsInst& I, int mode, hipStream_t st) {
         if (I.precision == 1) MPS_ELEM_LAUNCH(float, 8);
         else MPS_ELEM_LAUNCH(double, 8);
     } else if (!visc && I.pnd != nullptr) {
-        const size_t lds = 48 * (size_t)I.pu_max + 12 * MPS_PATCH * (I.precision == 1 ? 4 : 8) + 8 * MPS_PATCH;
+        const size_t lds = 48 * (size_t)I.pu_max + 12 * MPS_PATCH * (I.precision == 1 ? 4 : 8) + 8 * MPS_PATCH +
+                           2 * PatchList<4>::SPR;
 #define MPS_TET_STAGED(T)                                                                        \
     switch (I.law) {                                                                             \
         case 0: k_elem_tet_staged<T, 0><<<nb, MPS_PATCH, lds, st>>>(I, mode); break;             \

@@ -170,7 +170,7 @@ __global__ __launch_bounds__(MPS_PATCH) void k_elem(MpsInst I, int mode) {
 // of T, the 4*MPS_PATCH entries of the contribution list and the staged node offsets.
 template <typename T, int LAW>
 __global__ __launch_bounds__(MPS_PATCH) void k_elem_tet_staged(MpsInst I, int mode) {
-    extern __shared__ double smem[];
+    extern __shared__ __align__(16) double smem[];  // 16-byte pieces are stored at offsets that are multiples of 16
     __shared__ unsigned long long sh[MPS_PATCH / 64];
     const int pu = I.pu_max;
     double* __restrict__ sxu = smem;                                  // [6][pu]: X (3), then u (3)

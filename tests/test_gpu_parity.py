@@ -160,6 +160,20 @@ def test_svk_and_linear_elastic(law, tet):
 
 @pytest.mark.parametrize("law", [0, 1, 2, 3])
 @pytest.mark.parametrize("tet", [False, True])
+def test_less_than_one_wave_of_elements(law, tet):
+    """A mesh smaller than one wavefront (8 hexes, or 24 tets): one partial patch whose elements do not
+    fill a wave, in both forms and every law, against the oracle."""
+    m = perturb(box(2, 2, 2, tet=False) if not tet else box(2, 2, 1, tet=True), 0.1, 51 + law)
+    assert m["n_elem"] < 64
+    ro, (rc, rep, u, v, a, hist) = run_pair(m, 120, law=law)
+    assert rc == 0 and relerr(u, ro["u"]) <= TOL64, relerr(u, ro["u"])
+    np.testing.assert_allclose(hist["dt"], ro["hist"]["dt"], rtol=1e-13, atol=0)
+    ro2, (rc2, rep2, u2, *_r) = run_pair(m, 120, law=law, precision=G.PREC_FP32_ELEM)
+    assert rc2 == 0 and relerr(u2, ro2["u"]) <= TOL32, relerr(u2, ro2["u"])
+
+
+@pytest.mark.parametrize("law", [0, 1, 2, 3])
+@pytest.mark.parametrize("tet", [False, True])
 def test_element_outputs(law, tet):
     """VTK element fields (GP-averaged Cauchy, strain, VMS, volumetric/equivalent strain) of the device state
     against the oracle evaluated on the same displacement."""

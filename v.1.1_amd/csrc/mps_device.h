@@ -265,11 +265,34 @@ __device__ __forceinline__ void pk1(const double* H, double pa, double pb, doubl
 }
 
 // FP32: H-based so that small strains carry no cancellation:
-//   J - 1 = tr H + i2(H) + det H,  F - F^-T = H + H^T F^-T,  B - I = H + H^T + H H^T.
+//   J - 1 = tr H + i2(H) + det H,  B - I = H + H^T + H H^T.
+// Compressible: cof(F) = (1 + tr H) I - H^T + cof(H), so with K = H^T - cof(H) and s = i2(H) + det H
+// (i2 = tr cof(H), det H = H(0,:).cof(H)(0,:)), F^-T = ((1 + tr H) I - K)/J, F - F^-T = H + (K + s I)/J
+// and P = mu (F - F^-T) + lambda lnJ F^-T = mu H + a K + b I, a = (mu - lambda lnJ)/J,
+// b = (mu s + lambda lnJ (1 + tr H))/J: every term is of the order of the strain.
 template <int LAW>
 __device__ __forceinline__ void pk1(const float* H, float pa, float pb, float* P) {
     if (LAW >= 2) {
         pk1_svk_lin<float, LAW>(H, pa, pb, P);
+        return;
+    }
+    if (LAW == 0) {  // pa = lambda, pb = mu
+        float cH[9];
+        cof3(H, cH);
+        const float tr = H[0] + H[4] + H[8];
+        const float sv = (cH[0] + cH[4] + cH[8]) + (H[0] * cH[0] + H[1] * cH[1] + H[2] * cH[2]);
+        const float Jm1 = tr + sv;
+        const float lt = pa * log1p_fast(Jm1);
+        const float rJ = rcp(1.f + Jm1);
+        const float a = (pb - lt) * rJ;
+        const float b = (pb * sv + lt * (1.f + tr)) * rJ;
+#pragma unroll
+        for (int i = 0; i < 3; i++)
+#pragma unroll
+            for (int j = 0; j < 3; j++) P[i * 3 + j] = pb * H[i * 3 + j] + a * (H[j * 3 + i] - cH[i * 3 + j]);
+        P[0] += b;
+        P[4] += b;
+        P[8] += b;
         return;
     }
     const float Jm1 = detIpH_m1(H);
@@ -282,20 +305,7 @@ __device__ __forceinline__ void pk1(const float* H, float pa, float pb, float* P
     F[8] += 1.f;
     float cF[9];
     cof3(F, cF);
-    if (LAW == 0) {  // P = mu (H + H^T F^-T) + lambda lnJ F^-T
-        const float rJ = rcp(1.f + Jm1);
-        float FiT[9];
-#pragma unroll
-        for (int i = 0; i < 9; i++) FiT[i] = cF[i] * rJ;
-        const float lt = pa * lnJ;
-#pragma unroll
-        for (int i = 0; i < 3; i++)
-#pragma unroll
-            for (int j = 0; j < 3; j++) {
-                const float t = H[i] * FiT[j] + H[3 + i] * FiT[3 + j] + H[6 + i] * FiT[6 + j];
-                P[i * 3 + j] = pb * (H[i * 3 + j] + t) + lt * FiT[i * 3 + j];
-            }
-    } else {  // sigma = mu J^{-5/3} dev(B - I) + K (J-1) I ; P = sigma cof(F)
+    {  // Belytschko (pa = mu, pb = K1): sigma = mu J^{-5/3} dev(B - I) + K (J-1) I ; P = sigma cof(F)
         float Bm[9];
 #pragma unroll
         for (int i = 0; i < 3; i++)

@@ -369,25 +369,63 @@ __device__ __forceinline__ double defo_energy(int law, const double* F, double l
 // Characteristic length (always FP64, no FMA contraction: same rounding as the
 // reference geometry.cpp, so dt is bitwise reproducible for identical u).
 // ---------------------------------------------------------------------------
-// areaTriangle 3D: geometry.cpp:112-130 ; p, q, r are vertices 1, 2, 3.
-__device__ __forceinline__ double area_tri(const double* p, const double* q, const double* r) {
+// areaTriangle 3D: geometry.cpp:112-130 ; p, q, r are vertices 1, 2, 3.  tri_norm2 is the squared norm
+// of the cross product; the area is 0.5 * its square root.
+__device__ __forceinline__ double tri_norm2(const double* p, const double* q, const double* r) {
 #pragma clang fp contract(off)
     double A0 = p[0] - r[0], A1 = p[1] - r[1], A2 = p[2] - r[2];
     double B0 = q[0] - r[0], B1 = q[1] - r[1], B2 = q[2] - r[2];
     double C0 = A1 * B2 - A2 * B1;
     double C1 = A2 * B0 - A0 * B2;
     double C2 = A0 * B1 - A1 * B0;
-    return 0.5 * sqrt(C0 * C0 + C1 * C1 + C2 * C2);
+    return C0 * C0 + C1 * C1 + C2 * C2;
+}
+
+// NS square roots in place, bitwise those of sqrt().  sqrt() on this target is correctly rounded: the
+// hardware reciprocal square root and two Newton corrections, with a scaling by 2^256 below 2^-767 and a
+// final test for zero and infinity.  When every argument of every active lane of the wave lies in
+// [2^-767, inf), the scaling is the identity and the test passes the value through, so the corrections
+// alone give the same bits; otherwise the wave calls sqrt().
+template <int NS>
+__device__ __forceinline__ void sqrt_n(double* v) {
+    bool ok = true;
+#pragma unroll
+    for (int k = 0; k < NS; k++) ok = ok && v[k] >= 0x1p-767 && v[k] < __builtin_inf();
+    if (__all(ok)) {
+#pragma unroll
+        for (int k = 0; k < NS; k++) {
+            const double x = v[k];
+            const double y = __builtin_amdgcn_rsq(x);
+            double g = x * y, h = 0.5 * y;
+            const double r = fma(-h, g, 0.5);
+            g = fma(g, r, g);
+            h = fma(h, r, h);
+            double d = fma(-g, g, x);
+            g = fma(d, h, g);
+            d = fma(-g, g, x);
+            v[k] = fma(d, h, g);
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < NS; k++) v[k] = sqrt(v[k]);
+    }
 }
 
 // inscribedSphereHex (geometry.cpp:308-355) x2: classLinearHexahedra.cpp:209-234.
 __device__ __forceinline__ double hex_char_length(const double (*x)[3]) {
 #pragma clang fp contract(off)
     const int fc[6][4] = {{0, 1, 2, 3}, {4, 5, 6, 7}, {0, 1, 5, 4}, {1, 2, 6, 5}, {2, 3, 7, 6}, {0, 3, 7, 4}};
+    double n2[12];
+#pragma unroll
+    for (int f = 0; f < 6; f++) {
+        n2[2 * f] = tri_norm2(x[fc[f][0]], x[fc[f][1]], x[fc[f][2]]);
+        n2[2 * f + 1] = tri_norm2(x[fc[f][0]], x[fc[f][2]], x[fc[f][3]]);
+    }
+    sqrt_n<12>(n2);
     double A = 0.0;
 #pragma unroll
     for (int f = 0; f < 6; f++) {
-        double s = area_tri(x[fc[f][0]], x[fc[f][1]], x[fc[f][2]]) + area_tri(x[fc[f][0]], x[fc[f][2]], x[fc[f][3]]);
+        double s = 0.5 * n2[2 * f] + 0.5 * n2[2 * f + 1];
         A = (f == 0) ? fabs(s) : A + fabs(s);
     }
     double M[9] = {x[1][0] - x[0][0], x[1][1] - x[0][1], x[1][2] - x[0][2],
@@ -402,11 +440,10 @@ __device__ __forceinline__ double hex_char_length(const double (*x)[3]) {
 // inscribedSphere (geometry.cpp:264-301) x2: classLinearTetrahedra.cpp:146-163.
 __device__ __forceinline__ double tet_char_length(const double (*x)[3]) {
 #pragma clang fp contract(off)
-    double s1 = area_tri(x[0], x[1], x[2]);
-    double s2 = area_tri(x[1], x[2], x[3]);
-    double s3 = area_tri(x[0], x[2], x[3]);
-    double s4 = area_tri(x[0], x[1], x[3]);
-    double A = fabs(s1) + fabs(s2) + fabs(s3) + fabs(s4);
+    double n2[4] = {tri_norm2(x[0], x[1], x[2]), tri_norm2(x[1], x[2], x[3]), tri_norm2(x[0], x[2], x[3]),
+                    tri_norm2(x[0], x[1], x[3])};
+    sqrt_n<4>(n2);
+    double A = fabs(0.5 * n2[0]) + fabs(0.5 * n2[1]) + fabs(0.5 * n2[2]) + fabs(0.5 * n2[3]);
     double M[9] = {x[1][0] - x[0][0], x[1][1] - x[0][1], x[1][2] - x[0][2],
                    x[2][0] - x[0][0], x[2][1] - x[0][1], x[2][2] - x[0][2],
                    x[3][0] - x[0][0], x[3][1] - x[0][1], x[3][2] - x[0][2]};

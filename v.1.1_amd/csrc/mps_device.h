@@ -246,8 +246,9 @@ __device__ __forceinline__ void pk1(const double* H, double pa, double pb, doubl
                 B[j * 3 + i] = B[i * 3 + j];
             }
         const double trB = B[0] + B[4] + B[8];
-        const double f1 = pow(J, 5.0 / 3.0);
-        const double rf1 = rcp(f1);
+        // J^(-5/3) = exp(-5/3 ln J): a few ulp from 1/pow(J, 5/3), with far fewer registers than the
+        // inlined pow (which held the FP64 Belytschko kernel at one wave per SIMD)
+        const double rf1 = exp(-(5.0 / 3.0) * log_near1(J));
         const double f2 = pa * rf1;
         const double f3 = -(pa / 3.0) * trB * rf1 + pb * (J - 1.0);
         double sg[9];

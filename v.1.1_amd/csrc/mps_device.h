@@ -726,6 +726,10 @@ __device__ __forceinline__ double hex8_element(const MpsInst& I, int e, const do
                 H[i * 3 + j] = (Du[i * 3] * cJ[j * 3] + Du[i * 3 + 1] * cJ[j * 3 + 1] + Du[i * 3 + 2] * cJ[j * 3 + 2]) * rJ;
         pk1<LAW>(H, tpa, tpb, P);
         if (VISC) {  // tr of grad(u - uprev) = sum_ik Dd(i,k) cof(J)(i,k) / det J
+            // the memory clobber keeps the 21 LDS reads at each Gauss point: hoisted out of the loop
+            // they would hold 21 more values in registers throughout it.  The FP64 Belytschko kernel
+            // runs one wave per SIMD either way and is faster with them hoisted.
+            if (!(sizeof(T) == 8 && LAW == 1)) asm volatile("" ::: "memory");
             T tr = T(0);
 #pragma unroll
             for (int i = 0; i < 3; i++) {

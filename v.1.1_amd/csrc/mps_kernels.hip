@@ -140,7 +140,7 @@ __device__ __forceinline__ unsigned long long patch_forces(const MpsInst& I, int
 // One workgroup per patch, one element per work-item.
 // VISC (mode 0 only): bulk artificial viscosity over the step, from u - uprev and the step's dt.
 template <typename T, int NEN, int LAW, bool VISC = false>
-__global__ __launch_bounds__(MPS_PATCH) __attribute__((amdgpu_waves_per_eu(VISC ? 1 : 2))) void k_elem(MpsInst I, int mode) {
+__global__ __launch_bounds__(MPS_PATCH) __attribute__((amdgpu_waves_per_eu(VISC && LAW == 1 ? 1 : 2))) void k_elem(MpsInst I, int mode) {
     __shared__ T sf[NEN * 3 * MPS_PATCH];
     __shared__ __align__(16) uint16_t spc[NEN * MPS_PATCH];
     __shared__ __align__(16) uint16_t spr[PatchList<NEN>::SPR];

@@ -250,9 +250,11 @@ __device__ __forceinline__ int batch_end(int n) {
 __device__ __forceinline__ int batch_stride() { return (gridDim.x & 7) ? gridDim.x : (gridDim.x >> 3); }
 
 // Persistent batched element kernel: a work item is one patch of one instance (begin = its
-// first element); BS work-items share the patch's elements.
+// first element); BS work-items share the patch's elements.  At least 2 waves per SIMD: the FP64 hex
+// instantiations otherwise take more than 256 registers and run one wave per SIMD (10% slower for
+// a batch of FP64 cubes, against a small spill).
 template <typename T, int NEN, int LAW, int BS>
-__global__ __launch_bounds__(BS) void k_elem_batch(const MpsInst* __restrict__ insts,
+__global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(2))) void k_elem_batch(const MpsInst* __restrict__ insts,
                                                     const MpsWork* __restrict__ work, const int* __restrict__ nwork_p) {
     __shared__ T sf[NEN * 3 * MPS_PATCH];
     __shared__ unsigned long long sh[BS / 64 + 1];

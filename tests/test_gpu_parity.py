@@ -134,6 +134,25 @@ def test_tet_node_staging_bitwise(precision, law, drop_first, monkeypatch):
     assert relerr(out[True][2], op.internal_force(uf)) <= (1e-12 if precision == G.PREC_FP64 else 1e-5)
 
 
+@pytest.mark.parametrize("precision", [G.PREC_FP64, G.PREC_FP32_ELEM])
+def test_tet_without_shared_nodes_uses_the_gathering_kernel(precision):
+    """Tets that share no nodes: a patch of 256 tets has 1024 distinct nodes, whose staging (48 KB) and
+    force slots do not fit the 64 KB of LDS a workgroup may use, so the set-up keeps the per-element
+    gathers.  The solve must run and match the oracle to the form's bar."""
+    base = perturb(box(6, 5, 4, tet=True), 0.1, 77)
+    X0 = base["X"].reshape(3, -1)
+    conn0 = base["conn"].reshape(-1, 4)
+    ne = conn0.shape[0]
+    X = X0[:, conn0.ravel()].copy()  # node 4*t + a is a private copy of the tet's node a
+    m = dict(base, X=X.ravel(), conn=np.arange(4 * ne, dtype=np.int32), n_nodes=4 * ne)
+    assert ne > 256
+    tol = TOL64 if precision == G.PREC_FP64 else TOL32
+    ro, (rc, rep, u, v, a, hist) = run_pair(m, 150, precision=precision)
+    assert rc == 0 and rep["steps"] == 150
+    assert np.abs(ro["u"]).max() > 0
+    assert relerr(u, ro["u"]) <= tol, relerr(u, ro["u"])
+
+
 @pytest.mark.parametrize("law", [0, 1])
 def test_perturbed_hex_fp64_and_fp32(law):
     m = perturb(box(12, 12, 12), 0.1, 11)

@@ -1217,6 +1217,7 @@ static void energies(orc_state *s, const double *u, const double *v, double *out
 
 static int setup(orc_state *s, const orc_problem *p)
 {
+    memset(s, 0, sizeof(*s)); /* the callers free the state on every path, also after a rejected problem */
     if (!p || (p->nen != 8 && p->nen != 4) || p->n_nodes <= 0 || p->n_elem <= 0) return ORC_ERR_ARG;
     if (p->n_threads > 1 && p->sum_order != ORC_SUM_ELEMENT) return ORC_ERR_ARG;
     if (p->law < ORC_LAW_NH_COMPRESSIBLE || p->law > ORC_LAW_LINEAR) return ORC_ERR_ARG;

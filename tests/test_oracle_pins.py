@@ -501,6 +501,9 @@ def test_summation_orders_and_threads_agree():
         runs.append(O.OracleProblem(m, dir_dof=dd, dir_val=dv, n_steps=200, sum_order=so, n_threads=nt).run()["u"])
     assert np.abs(runs[0] - runs[1]).max() <= 1e-13 * np.abs(runs[0]).max()
     assert np.array_equal(runs[1], runs[2])
+    # threads need the element-order sums: the reference order with threads is rejected, not run
+    bad = O.OracleProblem(m, dir_dof=dd, dir_val=dv, n_steps=5, sum_order=O.SUM_REFERENCE, n_threads=4).run()
+    assert bad["rc"] == -3 and bad["steps"] == 0
 
 
 @pytest.mark.parametrize("law", [0, 1, 2, 3])

@@ -1,5 +1,6 @@
-"""Full-size checks on the device through size-independent properties (the CPU oracle is too slow at
-these sizes; it pins the same kernels at small sizes in test_gpu_parity.py):
+"""Full-size checks on the device: against the threaded CPU oracle for the first 100 steps of the
+1M-hex8 and 3.07M-tet4 workloads (about 10 s each on 16 host threads), and otherwise through
+size-independent properties (the oracle pins the same kernels at small sizes in test_gpu_parity.py):
 
 - the 1M-hex8 workload: FP32-element form within 1e-6 of the FP64 form; bitwise determinism across graph
   replay, direct launches and chunking; mirror symmetry (x <-> y) of the unperturbed cube;
@@ -33,6 +34,29 @@ def solve(cfg, steps, precision=G.PREC_FP64, graphs=True, chunks=None):
             assert rc == 0, rep
         assert rep["steps"] == steps and rep["status"] == G.MPS_DONE
         return s.get_state()[0]
+
+
+def oracle_threads():
+    import os
+    return max(1, min(16, os.cpu_count() or 1))
+
+
+@pytest.mark.parametrize("k,steps", [(1, 100), (2, 100)])
+def test_full_size_against_the_oracle(k, steps):
+    """The 1M-hex8 (cfg1) and 3.07M-tet4 (cfg2, per-element materials) workloads: FP64 GPU against the
+    oracle (element-order sums, on the host's threads) after 100 steps, to the FP64 bar 1e-12."""
+    cfg = configs.config(k)
+    m = cfg["mesh"]
+    u = solve(cfg, steps)
+    print(f"oracle on {oracle_threads()} threads, cfg{k}, {steps} steps", flush=True)
+    o = O.OracleProblem(m, law=cfg.get("law", 0), E=cfg["E"], nu=cfg["nu"], rho=cfg.get("rho", 1000.0),
+                        dir_dof=cfg["dir_dof"], dir_val=cfg["dir_val"], bc_mode=O.BC_VELOCITY, vel_ramp_steps=100,
+                        scale=cfg.get("scale", 0.5), n_steps=steps, sum_order=O.SUM_ELEMENT,
+                        n_threads=oracle_threads()).run()
+    assert o["rc"] == 0 and o["steps"] == steps
+    err = rel(u, o["u"])
+    print(f"cfg{k} fp64 vs oracle after {steps} steps: {err:.3e}", flush=True)
+    assert err <= 1e-12
 
 
 def test_cfg1_fp32_form_and_determinism():

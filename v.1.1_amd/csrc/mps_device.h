@@ -362,7 +362,7 @@ __device__ __forceinline__ double defo_energy(int law, const double* F, double l
     double trC = 0;
 #pragma unroll
     for (int i = 0; i < 3; i++) trC += F[i] * F[i] + F[3 + i] * F[3 + i] + F[6 + i] * F[6 + i];
-    double l = log(J);
+    double l = log_near1(J);
     return 0.5 * lambda * l * l - mu * l + 0.5 * mu * trC;
 }
 

@@ -672,6 +672,7 @@ __global__ __launch_bounds__(256) void k_gather_fint(MpsInst I, double* fint) {
 // Energies of step B: DEFO_ENERGY, KIN_ENERGY, VOLUME (Sum), dataExtraction.cpp:569-776.
 // FP64 in every precision form.
 // ---------------------------------------------------------------------------
+template <int LAW>
 __device__ __forceinline__ void elem_energy(const MpsInst& I, int e, double* out) {
     const size_t N = (size_t)I.N;
     double pa, pb, c;
@@ -679,21 +680,40 @@ __device__ __forceinline__ void elem_energy(const MpsInst& I, int e, double* out
     const double rho = I.rho ? I.rho[e] : I.rho0;
     double ed = 0.0, ek = 0.0, vol = 0.0;
     if (I.nen == 8) {
+        // J, grad u and the GP velocity from the butterfly coefficients of X, u and v: the velocity at
+        // a GP is (sum_a v_a + sum_m s_m(gp) g^|m| C_m) / 8, the trilinear interpolation sum_a phi_a v_a
         int nd[8];
-        double x[8][3], uu[8][3], vv[8][3];
-        for (int a = 0; a < 8; a++) {
-            nd[a] = I.conn[(size_t)e * 8 + a];
+#pragma unroll
+        for (int a = 0; a < 8; a++) nd[a] = I.conn[(size_t)e * 8 + a];
+        double Ax[7][3], Au[7][3], Cv[7][3], vs[3];
+        {
+            double x[8][3], cx[7][3];
+#pragma unroll
+            for (int a = 0; a < 8; a++)
+#pragma unroll
+                for (int d = 0; d < 3; d++) x[a][d] = I.X[d * N + nd[a]];
+            hex_butterfly(x, cx);
+            hex_scale<double>(cx, Ax);
+#pragma unroll
+            for (int a = 0; a < 8; a++)
+#pragma unroll
+                for (int d = 0; d < 3; d++) x[a][d] = I.u[d * N + nd[a]];
+            hex_butterfly(x, cx);
+            hex_scale<double>(cx, Au);
+#pragma unroll
+            for (int a = 0; a < 8; a++)
+#pragma unroll
+                for (int d = 0; d < 3; d++) x[a][d] = I.v[d * N + nd[a]];
+            hex_butterfly(x, Cv);
+#pragma unroll
             for (int d = 0; d < 3; d++) {
-                x[a][d] = I.X[d * N + nd[a]];
-                uu[a][d] = I.u[d * N + nd[a]];
-                vv[a][d] = I.v[d * N + nd[a]];
+                double t = 0.0;
+#pragma unroll
+                for (int a = 0; a < 8; a++) t += x[a][d];
+                vs[d] = t;
             }
         }
-        double cx[7][3], cu[7][3], Ax[7][3], Au[7][3];
-        hex_butterfly(x, cx);
-        hex_butterfly(uu, cu);
-        hex_scale<double>(cx, Ax);
-        hex_scale<double>(cu, Au);
+        const double g = MPS_GP, g2 = MPS_GP * MPS_GP, g3 = MPS_GP * MPS_GP * MPS_GP;
 #pragma unroll 1
         for (int gp = 0; gp < 8; gp++) {
             const int sx = sxi(gp), sy = seta(gp), sz = szeta(gp);
@@ -701,19 +721,26 @@ __device__ __forceinline__ void elem_energy(const MpsInst& I, int e, double* out
             hex_grad_at<double>(Ax, sx, sy, sz, Jm);
             hex_grad_at<double>(Au, sx, sy, sz, Du);
             double dJ = det3(Jm);
-            inv3(Jm, iJ, dJ);
+            double cJ[9];
+            cof3(Jm, cJ);
+            const double rJ = rcp(dJ);
+#pragma unroll
+            for (int i = 0; i < 3; i++)
+#pragma unroll
+                for (int j = 0; j < 3; j++) iJ[j * 3 + i] = cJ[i * 3 + j] * rJ;
             double wJ = fabs(dJ);
             for (int i = 0; i < 3; i++)
                 for (int j = 0; j < 3; j++)
                     F[i * 3 + j] = (i == j ? 1.0 : 0.0) + Du[i * 3] * iJ[j] + Du[i * 3 + 1] * iJ[3 + j] + Du[i * 3 + 2] * iJ[6 + j];
-            ed += defo_energy(I.law, F, pa, pb) * wJ;
-            double phi[8], ndv[24];
-            hex_shape_ref(sx > 0 ? MPS_GP : -MPS_GP, sy > 0 ? MPS_GP : -MPS_GP, sz > 0 ? MPS_GP : -MPS_GP, phi, ndv);
-            double vel[3] = {0.0, 0.0, 0.0};
-            for (int a = 0; a < 8; a++)
-                for (int j = 0; j < 3; j++) vel[j] += phi[a] * vv[a][j];
+            ed += defo_energy(LAW, F, pa, pb) * wJ;
             double kv = 0.0;
-            for (int j = 0; j < 3; j++) kv += 0.5 * rho * vel[j] * vel[j];
+#pragma unroll
+            for (int j = 0; j < 3; j++) {
+                const double vel = 0.125 * (vs[j] + (sx * g) * Cv[0][j] + (sy * g) * Cv[1][j] + (sz * g) * Cv[2][j] +
+                                            (sx * sy * g2) * Cv[3][j] + (sy * sz * g2) * Cv[4][j] +
+                                            (sx * sz * g2) * Cv[5][j] + (sx * sy * sz * g3) * Cv[6][j]);
+                kv += 0.5 * rho * vel * vel;
+            }
             ek += kv * wJ;
             vol += wJ * det3(F);
         }
@@ -739,7 +766,7 @@ __device__ __forceinline__ void elem_energy(const MpsInst& I, int e, double* out
         for (int i = 0; i < 3; i++)
             for (int j = 0; j < 3; j++)
                 F[i * 3 + j] = (i == j ? 1.0 : 0.0) + Du[i * 3] * iJ[j] + Du[i * 3 + 1] * iJ[3 + j] + Du[i * 3 + 2] * iJ[6 + j];
-        ed = defo_energy(I.law, F, pa, pb) * wJ;
+        ed = defo_energy(LAW, F, pa, pb) * wJ;
         double vel[3] = {0.0, 0.0, 0.0};
         for (int a = 0; a < 4; a++)
             for (int j = 0; j < 3; j++) vel[j] += 0.25 * vv[a][j];
@@ -861,13 +888,14 @@ __global__ __launch_bounds__(256) void k_elem_outputs(MpsInst I, double* __restr
     for (int i = 0; i < 21; i++) out[(size_t)e * 21 + i] = acc[i] / (double)ngp;
 }
 
+template <int LAW>
 __global__ __launch_bounds__(256) void k_energy(MpsInst I) {
     __shared__ double sh[3 * 256];
     const long long k = I.ctl->B;
     if (!I.ctl->slot[k & 1].go) return;
     const int e = blockIdx.x * blockDim.x + threadIdx.x;
     double vals[3] = {0.0, 0.0, 0.0};
-    if (e < I.Ne) elem_energy(I, e, vals);
+    if (e < I.Ne) elem_energy<LAW>(I, e, vals);
     block_sum<3>(vals, sh);
     if (threadIdx.x == 0)
         for (int j = 0; j < 3; j++) I.partials[(size_t)blockIdx.x * 3 + j] = vals[j];
@@ -1009,7 +1037,12 @@ hipError_t launch_init_ctl(const MpsInst& I, const unsigned long long* dt0, hipS
 
 hipError_t launch_energy(const MpsInst& I, hipStream_t st) {
     const int nb = energy_blocks(I);
-    k_energy<<<nb, 256, 0, st>>>(I);
+    switch (I.law) {  // one instantiation per law: a runtime choice would size registers for every law
+        case 0: k_energy<0><<<nb, 256, 0, st>>>(I); break;
+        case 1: k_energy<1><<<nb, 256, 0, st>>>(I); break;
+        case 2: k_energy<2><<<nb, 256, 0, st>>>(I); break;
+        default: k_energy<3><<<nb, 256, 0, st>>>(I); break;
+    }
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     k_energy_reduce<<<1, 256, 0, st>>>(I, nb);

@@ -229,6 +229,11 @@ int mps_destroy(mps_handle h);
  * launching its kernels one by one.  No reference counterpart. */
 int mps_set_graphs(mps_handle h, int32_t enable);
 
+/* Capture and instantiate, without running anything, the graphs that mps_run(h, max_steps) would
+ * replay from the current step, so that the capture cost does not fall inside a later timed call.
+ * Does nothing when mps_run would launch the kernels one by one.  No reference counterpart. */
+int mps_prepare_run(mps_handle h, int64_t max_steps);
+
 /* Kernel timing: when enabled, mps_run records HIP events around every kernel
  * launch on the solver's stream.  mps_kernel_times returns, per kernel kind
  * [0 element, 1 node, 2 begin, 3 energy], the summed device milliseconds and the

@@ -206,6 +206,45 @@ def test_pressure_loads_from_input(tmp_path):
     assert rel(u, r["u"]) <= 1e-12 and np.abs(u).max() > 1e-6
 
 
+def test_loads_with_displacement_extraction(tmp_path):
+    """FORCE, PRESSURE and GRAVITY loads together with an Unknown (displacement) node extraction: the per-step
+    displacement gather must leave the load tables alone (regression: growing the gather buffers once released
+    them, so every later step read freed memory)."""
+    m = configs.perturb(configs.box(3, 3, 4), 0.1, 17)
+    N = m["n_nodes"]
+    X = m["X"].reshape(3, N)
+    conn = m["conn"].reshape(-1, 8)
+    bot = [n for n in range(N) if X[2, n] < 1e-12]
+    top_nodes = [n for n in range(N) if abs(X[2, n] - X[2].max()) < 1e-12]
+    bc = [(n, d, 0.0) for n in bot for d in range(3)]
+    dof = np.array([n + N * d for n, d, _ in bc], np.int32)
+    val = np.zeros(dof.size)
+    top = [e for e in range(conn.shape[0]) if np.all(X[2, conn[e, 4:]] > X[2].max() - 1e-12)]
+    dt0 = O.OracleProblem(m, dir_dof=dof, dir_val=val, bc_mode=O.BC_RAMP, tf=1.0, scale=0.5).critical_dt()
+    tf = 70.5 * dt0
+    F = np.array([[300.0, -200.0, 500.0], [-100.0, 400.0, 250.0]])
+    fn = top_nodes[:2]
+    lines = ["*BOUNDARY, TYPE=FORCE", f"0, {0.7 * tf:.17g}, 1.0"]
+    lines += [f"{n + 1}, {f[0]:.17g}, {f[1]:.17g}, {f[2]:.17g}" for n, f in zip(fn, F)]
+    lines += ["*GRAVITY", "9.81", "*BOUNDARY, TYPE=PRESSURE RAMP", f"0, {0.8 * tf:.17g}, 3e4, 2"]
+    lines += [str(e + 1) for e in top]
+    inp = tmp_path / "loads_u.inp"
+    inpgen.write_inp(inp, m, [dict(rho=1000.0, E=1e6, nu=0.3)], [dict(scale=0.5, t0=0.0, tf=tf, outputs=1, bc=bc)],
+                     node_extractions=[("U", "Unknown", "Rough", 2, top_nodes), ("V", "Unknown", "Max", 0, top_nodes[:3])],
+                     extra="\n".join(lines))
+    u, _ = run(inp, tmp_path, "-noVtk")
+    loads = [("force", np.array(fn, np.int32), F, 0.0, 0.7 * tf), ("gravity", 9.81),
+             ("pressure_ramp", 3e4, 0.0, 0.8 * tf, top, [2] * len(top))]
+    r = O.OracleProblem(m, dir_dof=dof, dir_val=val, bc_mode=O.BC_RAMP, t0=0.0, tf=tf, scale=0.5,
+                        loads=loads).run()
+    assert r["rc"] == 0 and r["steps"] > 60
+    assert rel(u, r["u"]) <= 1e-12 and np.abs(u).max() > 1e-6
+    head, rows = read_csv(tmp_path / "U_Unknown2_Rough.csv")
+    assert head == ["Time"] + [f"Node{n + 1}Comp2" for n in top_nodes] and rows.shape[0] == r["steps"]
+    np.testing.assert_allclose(rows[-1, 1:], r["u"][np.array(top_nodes) + 2 * N], rtol=0,
+                               atol=1e-12 * np.abs(r["u"]).max())
+
+
 def parse_vtk(path):
     """Sections of a legacy ASCII VTK file written by the series writer: name -> list of value rows."""
     lines = open(path).read().split("\n")

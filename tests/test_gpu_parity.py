@@ -282,3 +282,28 @@ def test_nan_error_matches_oracle():
     assert ro["rc"] == O.ORC_ERR_NAN
     assert rc == G.MPS_ERR_NAN
     assert rep["steps"] == ro["steps"]
+
+
+def test_prepared_graphs_give_the_same_run():
+    """mps_prepare_run captures the chunk graphs a later mps_run replays: the run that uses them, the run that
+    captures as it goes and the run launched kernel by kernel are bitwise the same."""
+    m = perturb(box(6, 6, 5), 0.1, 61)
+    dd, dv = clamp_and_pull(m)
+    out = []
+    for mode in ("prepared", "lazy", "plain"):
+        with G.Solver(m, dir_dof=dd, dir_val=dv, n_steps=90, history_cap=100) as s:
+            if mode == "plain":
+                s.set_graphs(False)
+            rc, rep = s.run(13)
+            assert rc == 0
+            if mode == "prepared":
+                s.prepare_run(77)
+                rc, rep = s.run(0)  # the capture ran nothing
+                assert rc == 0 and rep["steps"] == 13
+            rc, rep = s.run(77)
+            assert rc == 0 and rep["steps"] == 90 and rep["status"] == G.MPS_DONE
+            u, v, _ = s.get_state()
+            out.append((u, v, s.history(0, 90)["dt"]))
+    for other in out[1:]:
+        for k in range(3):
+            assert np.array_equal(out[0][k], other[k]), k

@@ -992,9 +992,7 @@ extern "C" int mps_get_dofs(mps_handle h, int32_t which, int64_t n, const int32_
     (void)hipSetDevice(h->device);
     if (n > h->pick_cap) {
         HCHK(h, hipStreamSynchronize(h->stream));
-        for (void* p : h->load_allocs) (void)hipFree(p);
-    h->load_allocs.clear();
-    if (h->pick_idx) (void)hipFree(h->pick_idx);
+        if (h->pick_idx) (void)hipFree(h->pick_idx);
         if (h->pick_out) (void)hipFree(h->pick_out);
         h->pick_idx = nullptr;
         h->pick_out = nullptr;
@@ -1119,6 +1117,68 @@ static void fill_report(mps_solver_s* h, const MpsCtl& c, mps_report* rep) {
 
 static int status_code(int st) { return st < 0 ? st : MPS_OK; }
 
+// Chunks of at most kChunk steps: the host reads the control block back once per chunk.
+static const long long kChunk = 256;
+
+// True when mps_run replays its chunks from captured graphs in the handle's current state.
+static bool graph_replay(const mps_solver_s* h) {
+    return h->use_graphs && !h->profiling && h->cfg.energy_every == 0 && !h->step_begun;
+}
+
+// The graph that replays a chunk of n steps (begin, (element, node+begin) x (n-1), element, node), captured
+// and instantiated on first use.  Capture does not execute anything: the host mirror of the step counter is
+// restored afterwards.  The kernels read the step index from the control block, so one graph serves every
+// chunk of n steps wherever it starts.
+static hipError_t chunk_graph(mps_solver_s* h, long long n, hipGraphExec_t* out) {
+    for (auto& g : h->graphs)
+        if (g.first == n) {
+            *out = g.second;
+            return hipSuccess;
+        }
+    std::vector<int> kinds;
+    hipGraph_t gr = nullptr;
+    hipGraphExec_t ge = nullptr;
+    const long long k0 = h->next_step;
+    hipError_t e = hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal);
+    if (e == hipSuccess) e = enqueue_steps(h, n, &kinds);
+    hipError_t e2 = hipStreamEndCapture(h->stream, &gr);
+    if (e == hipSuccess) e = e2;
+    if (e == hipSuccess) e = hipGraphInstantiate(&ge, gr, nullptr, nullptr, 0);
+    if (gr) (void)hipGraphDestroy(gr);
+    h->next_step = k0;
+    h->step_begun = false;
+    if (e != hipSuccess) return e;
+    h->graphs.emplace_back(n, ge);
+    *out = ge;
+    return hipSuccess;
+}
+
+// Chunk lengths mps_run(h, max_steps) would use from the current step.
+static std::vector<long long> chunk_plan(const mps_solver_s* h, long long max_steps) {
+    long long remaining = max_steps;
+    if (h->I.time_mode == MPS_TIME_FIXED) remaining = std::min<long long>(remaining, h->I.n_steps - (h->next_step - 1));
+    std::vector<long long> plan;
+    while (remaining > 0) {
+        const long long n = std::min(remaining, kChunk);
+        plan.push_back(n);
+        remaining -= n;
+    }
+    return plan;
+}
+
+extern "C" int mps_prepare_run(mps_handle h, int64_t max_steps) {
+    if (!h) return MPS_ERR_ARG;
+    if (max_steps < 0) return h->fail(MPS_ERR_ARG, "mps_prepare_run: max_steps < 0");
+    (void)hipSetDevice(h->device);
+    if (!graph_replay(h)) return MPS_OK;  // mps_run will launch the kernels one by one
+    for (long long n : chunk_plan(h, max_steps)) {
+        hipGraphExec_t ge = nullptr;
+        hipError_t e = chunk_graph(h, n, &ge);
+        if (e != hipSuccess) return h->fail(MPS_ERR_HIP, fmt("graph capture: %s", hipGetErrorString(e)));
+    }
+    return MPS_OK;
+}
+
 extern "C" int mps_run(mps_handle h, int64_t max_steps, mps_report* rep) {
     if (!h) return MPS_ERR_ARG;
     if (max_steps < 0) return h->fail(MPS_ERR_ARG, "mps_run: max_steps < 0");
@@ -1141,32 +1201,14 @@ extern "C" int mps_run(mps_handle h, int64_t max_steps, mps_report* rep) {
         h->primed = true;
     }
     HCHK(h, hipEventRecord(h->ev0, h->stream));
-    long long remaining = max_steps;
-    if (h->I.time_mode == MPS_TIME_FIXED) remaining = std::min<long long>(remaining, h->I.n_steps - (h->next_step - 1));
-    const long long chunk = 256;
     float total_ms = 0.f;
-    while (remaining > 0) {
-        long long n = std::min(remaining, chunk);
+    for (long long n : chunk_plan(h, max_steps)) {
         std::vector<int> kinds;
         hipError_t e = hipSuccess;
-        if (h->use_graphs && !h->profiling && h->cfg.energy_every == 0 && !h->step_begun) {
+        if (graph_replay(h)) {
             // every chunk of n steps is the same launch sequence: replay it from a captured graph
             hipGraphExec_t ge = nullptr;
-            for (auto& g : h->graphs)
-                if (g.first == n) ge = g.second;
-            if (!ge) {
-                hipGraph_t gr = nullptr;
-                const long long k0 = h->next_step;
-                e = hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal);
-                if (e == hipSuccess) e = enqueue_steps(h, n, &kinds);
-                hipError_t e2 = hipStreamEndCapture(h->stream, &gr);
-                if (e == hipSuccess) e = e2;
-                if (e == hipSuccess) e = hipGraphInstantiate(&ge, gr, nullptr, nullptr, 0);
-                if (gr) (void)hipGraphDestroy(gr);
-                h->next_step = k0;  // capture does not execute; replay below advances the host mirror
-                h->step_begun = false;
-                if (e == hipSuccess) h->graphs.emplace_back(n, ge);
-            }
+            e = chunk_graph(h, n, &ge);
             if (e == hipSuccess) e = hipGraphLaunch(ge, h->stream);
             if (e == hipSuccess) {
                 h->next_step += n;
@@ -1176,7 +1218,6 @@ extern "C" int mps_run(mps_handle h, int64_t max_steps, mps_report* rep) {
             e = enqueue_steps(h, n, &kinds);
         }
         if (e != hipSuccess) return h->fail(MPS_ERR_HIP, fmt("step launch: %s", hipGetErrorString(e)));
-        remaining -= n;
         HCHK(h, hipEventRecord(h->ev1, h->stream));
         HCHK(h, hipEventSynchronize(h->ev1));
         if (h->profiling) collect_times(h, kinds);

@@ -169,12 +169,21 @@ static void worker_loop(mps_server_s* S) {
         }
         const auto t0 = Clock::now();
         if (mps_batch_launch(S->batch, 1) != MPS_OK) {
+            // the running solves fail: their slots and arena blocks are released once nothing queued
+            // can still read them, and each counts as failed so that drain's predicate can hold
+            const std::string why = mps_batch_last_error(S->batch);
+            (void)mps_batch_sync(S->batch);
+            for (auto& r : S->running) (void)mps_batch_remove(S->batch, r->slot);
             std::lock_guard<std::mutex> lk(S->mu);
-            S->err = mps_batch_last_error(S->batch);
+            S->err = why;
+            const auto now = Clock::now();
             for (auto& r : S->running) {
                 r->state = MPS_REQ_FAILED;
                 r->status = MPS_ERR_HIP;
+                r->t_done = now;
+                S->failed++;
             }
+            if (!S->running.empty()) S->t_last = now;
             S->running.clear();
             S->cv_done.notify_all();
             continue;

@@ -141,6 +141,7 @@ def load(path=None):
     L.mps_bandwidth_probe.argtypes = [C.c_int32, C.c_int64, C.c_int32, _dp]
     L.mps_profile.argtypes = [H, C.c_int32]
     L.mps_set_graphs.argtypes = [H, C.c_int32]
+    L.mps_prepare_run.argtypes = [H, C.c_int64]
     L.mps_kernel_times.argtypes = [H, _dp, C.POINTER(C.c_int64)]
     if hasattr(L, "mps_batch_create"):
         B = C.c_void_p
@@ -339,6 +340,10 @@ class Solver:
 
     def set_graphs(self, enable=True):
         self._chk(self.L.mps_set_graphs(self.h, int(enable)))
+
+    def prepare_run(self, max_steps):
+        """Capture the step graphs a later run(max_steps) replays, without running them."""
+        self._chk(self.L.mps_prepare_run(self.h, max_steps))
 
     def profile(self, enable=True):
         self._chk(self.L.mps_profile(self.h, int(enable)))

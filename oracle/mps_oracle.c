@@ -1304,6 +1304,23 @@ int32_t orc_internal_force(const orc_problem *p, const double *u, double *fint)
     return rc;
 }
 
+/* DEFO_ENERGY, KIN_ENERGY and VOLUME with the Sum operation for a given u and v (dataExtraction.cpp:569-776):
+ * out[0] = sum_gp wJ W(F), out[1] = sum_gp wJ rho/2 |sum_a phi_a v_a|^2, out[2] = sum_gp wJ det F. */
+int32_t orc_energies(const orc_problem *p, const double *u, const double *v, double *out)
+{
+    orc_state s;
+    int rc = setup(&s, p);
+    worker_arg *wa = NULL;
+    pthread_t *th = NULL;
+    if (rc == ORC_OK) {
+        th = pool_start(&s, &wa);
+        energies(&s, u, v, out);
+        pool_stop(&s, th, wa);
+    }
+    state_free(&s);
+    return rc;
+}
+
 /* State quantities at one GP as updateConstitutive leaves them (hyperElasticNeoHookean.cpp:70-126, 221-257;
  * hyperElasticStVenantKirchhoff.cpp:203-227; linearElastic.cpp:63-95) with computeCauchy, computeVMS,
  * computeVolumetricStrain, computeEquivalentStrain (maths.cpp:1241-1299).  q[0..8] Cauchy, q[9..17] GL strain

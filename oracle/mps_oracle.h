@@ -143,6 +143,9 @@ int32_t orc_internal_force(const orc_problem *p, const double *u, double *fint);
 /* Internal force with the step's artificial viscosity: u now, uprev at the previous step, step dt. */
 int32_t orc_internal_force_visc(const orc_problem *p, const double *u, const double *uprev, double dt, double *fint);
 
+/* DEFO_ENERGY, KIN_ENERGY (consistent interpolation of v at the GPs) and VOLUME, Sum over all GPs, for u, v. */
+int32_t orc_energies(const orc_problem *p, const double *u, const double *v, double *out);
+
 /* Critical time step s*min_e min(l0,l(u))/c (solvers.cpp:296-320); u may be NULL. */
 double orc_critical_dt(const orc_problem *p, const double *u);
 /* Per element (out[21*n_elem]): GP averages of Cauchy (9), Green-Lagrange strain (9; small strain for

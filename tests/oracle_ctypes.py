@@ -155,6 +155,8 @@ def lib():
         L.orc_pressure_face.restype = C.c_int32
         L.orc_pressure_face.argtypes = [C.c_void_p, C.c_int32, C.c_int32, _dp, _dp, _dp, _dp, _dp, _dp]
         L.orc_internal_force.argtypes = [C.POINTER(Problem), _dp, _dp]
+        L.orc_energies.restype = C.c_int32
+        L.orc_energies.argtypes = [C.POINTER(Problem), _dp, _dp, _dp]
         L.orc_critical_dt.restype = C.c_double
         L.orc_critical_dt.argtypes = [C.POINTER(Problem), _dp]
         _lib = L
@@ -255,6 +257,15 @@ class OracleProblem:
         rc = lib().orc_element_outputs(C.byref(self.p), dptr(u), dptr(out))
         assert rc == 0
         return out.reshape(-1, 21)
+
+    def energies(self, u, v):
+        """(DEFO_ENERGY, KIN_ENERGY, VOLUME) summed over all GPs for the given u and v."""
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        v = np.ascontiguousarray(v, dtype=np.float64)
+        out = np.zeros(3)
+        rc = lib().orc_energies(C.byref(self.p), dptr(u), dptr(v), dptr(out))
+        assert rc == 0
+        return tuple(out)
 
     def external_force(self, t, u=None):
         f = np.zeros(3 * self.n_nodes)

@@ -883,3 +883,56 @@ def test_traction_dead_load_totals_and_time_function(tet):
             X = m["X"].reshape(3, -1)
             off = np.abs(X[2] - 1.0) > 1e-12
             assert np.all(f0[:, off] == 0.0)
+
+
+# ------------------------------------------------------------------ energies under non-uniform fields
+
+@pytest.mark.parametrize("n", [1, 2, 3])
+def test_kinetic_energy_is_the_consistent_interpolation(n):
+    """KIN_ENERGY interpolates v to each GP (sum_a phi_a v_a, dataExtraction.cpp:655-668).  For the linear field
+    v = alpha X on the unit cube that is 1/2 rho alpha^2 int |X|^2 dV = 1/2 rho alpha^2 (2x2x2 Gauss integrates the
+    quadratic exactly up to the 0.577350269 literal, ~1e-10).  Lumping the mass instead gives
+    1/2 sum_a m_a |v_a|^2, the trapezoidal rule of |X|^2, i.e. (1 + 1/(2 n^2)) times that value on n^3 elements
+    (1.5x on one element): a lumped-KE slip fails."""
+    m = box(n, n, n)
+    N = m["n_nodes"]
+    X = m["X"]
+    alpha, rho = 0.7, 1000.0
+    v = alpha * X
+    p = O.OracleProblem(m, rho=rho, n_steps=1)
+    _, ek, vol = p.energies(np.zeros(3 * N), v)
+    exact = 0.5 * rho * alpha ** 2 * 1.0
+    assert ek == pytest.approx(exact, rel=1e-9)
+    assert vol == pytest.approx(1.0, rel=1e-13)
+    mass = p.mass()
+    lumped = 0.5 * float(np.sum(mass * (v.reshape(3, N) ** 2).sum(axis=0)))
+    assert lumped == pytest.approx(exact * (1 + 0.5 / n ** 2), rel=1e-9)
+    assert abs(ek - lumped) > 0.4 / n ** 2 * exact
+    # a field with a constant part and a gradient along one axis only: the cross term 2 c.grad integrates too
+    c = np.array([0.3, -0.2, 0.1])
+    v2 = (c[:, None] + np.array([0.0, 0.0, 1.5])[:, None] * X.reshape(3, N)[0][None, :]).ravel()
+    _, ek2, _ = p.energies(np.zeros(3 * N), v2)
+    # int |c + (0,0,1.5 x)|^2 = |c|^2 + 2 c_z 1.5 / 2 + 1.5^2 / 3
+    assert ek2 == pytest.approx(0.5 * rho * (c @ c + 2 * c[2] * 1.5 * 0.5 + 1.5 ** 2 / 3), rel=1e-9)
+
+
+@pytest.mark.parametrize("tet", [False, True])
+def test_volume_is_det_G_under_homogeneous_deformation(tet):
+    """VOLUME = sum_gp wJ det F; under u = (G - I) X, F = G at every GP, so the deformed volume is det(G) V
+    (V = 1; for tets the weight literal 0.166666666667 makes it (1 + 2e-12) det G).  A volume formed from
+    the undeformed Jacobian alone, or from det(F) - 1, fails."""
+    m = perturb(box(3, 2, 2, tet=tet), 0.1, 23)
+    N = m["n_nodes"]
+    G = np.array([[1.1, 0.05, -0.02], [0.03, 0.93, 0.04], [-0.01, 0.02, 1.07]])
+    X = m["X"].reshape(3, N)
+    u = ((G - np.eye(3)) @ X).ravel()
+    p = O.OracleProblem(m, n_steps=1)
+    ed, ek, vol = p.energies(u, np.zeros(3 * N))
+    w = 6 * 0.166666666667 if tet else 1.0
+    assert vol == pytest.approx(np.linalg.det(G) * w, rel=1e-13)
+    assert ek == 0.0
+    # compressible NH energy density at F = G is uniform: E_def = W(G) V
+    lam, mu, _ = lame(1e6, 0.3)
+    J = np.linalg.det(G)
+    W = 0.5 * lam * np.log(J) ** 2 - mu * np.log(J) + 0.5 * mu * np.trace(G.T @ G)
+    assert ed == pytest.approx(W * w, rel=1e-12)

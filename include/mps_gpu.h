@@ -185,6 +185,16 @@ int mps_get_state(mps_handle h, double *u, double *v, double *a);
  * extraction does not copy the whole field. */
 int mps_get_dofs(mps_handle h, int32_t which, int64_t n, const int32_t *dof, double *out);
 
+/* mps_get_dofs without any device allocation in the call: the indices and gathered values are staged in a
+ * fixed per-device array of MPS_DOFS_STATIC_MAX entries, set up when the first solver is created on the
+ * device and kept until the process ends.  n > MPS_DOFS_STATIC_MAX returns MPS_ERR_CAPACITY.  All calls on
+ * one device share that array: calls that overlap in time on one device (from two threads) race.
+ * No counterpart in the reference: it is for callers that cannot allocate device memory (tight per-step
+ * extraction or serving loops, harnesses that forbid hipMalloc).  It synchronises the solver's stream, so it
+ * is not for use under stream capture. */
+#define MPS_DOFS_STATIC_MAX 65536
+int mps_get_dofs_static(mps_handle h, int32_t which, int64_t n, const int32_t *dof, double *out);
+
 /* Advance up to max_steps explicit steps (fewer if the time window or the
  * fixed step count ends, or on error).  rep may be NULL.
  * Replaces: the time loop of classSolverExplicit::calculate

@@ -22,10 +22,45 @@
 #include "mps_device.h"
 #include "mps_kernels.h"
 
+// Staging of mps_get_dofs_static: MPS_DOFS_STATIC_MAX DOF indices (int32), then as many gathered values
+// (double).  Every device the library runs on holds one copy until the process ends.  Host code never uses
+// this name as an address: each device's copy is reached through the pointer resolved once per device by
+// hipGetSymbolAddress in mps_create (scratch_base).
+#define MPS_STATIC_IDX_BYTES (((size_t)MPS_DOFS_STATIC_MAX * 4 + 255) / 256 * 256)
+static __device__ __align__(256) unsigned char scratch[MPS_STATIC_IDX_BYTES + (size_t)MPS_DOFS_STATIC_MAX * 8];
+
 namespace {
 
 std::mutex g_err_mu;
 std::string g_err;  // last mps_create failure
+
+// Device address of `scratch` on each device, resolved once with that device current.
+std::mutex g_scratch_mu;
+std::vector<void*> g_scratch_ptr;
+std::vector<hipError_t> g_scratch_err;
+
+void resolve_scratch(int device) {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if ((int)g_scratch_ptr.size() <= device) {
+        g_scratch_ptr.resize(device + 1, nullptr);
+        g_scratch_err.resize(device + 1, hipErrorNotInitialized);
+    }
+    if (g_scratch_ptr[device]) return;
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    g_scratch_err[device] = e;
+    if (e == hipSuccess) g_scratch_ptr[device] = p;
+}
+
+void* scratch_base(int device, hipError_t* why) {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (device < 0 || device >= (int)g_scratch_ptr.size()) {
+        *why = hipErrorNotInitialized;
+        return nullptr;
+    }
+    *why = g_scratch_err[device];
+    return g_scratch_ptr[device];
+}
 
 void set_global_error(const char* fmt, ...) {
     char buf[512];
@@ -533,6 +568,7 @@ extern "C" int mps_create(const mps_mesh* mesh, const mps_material* mat, const m
         delete h;
         return MPS_ERR_HIP;
     }
+    resolve_scratch(h->device);  // once per device, with the device current; a failure surfaces in its user
     rc = upload_instance(h, hp);
     if (rc == MPS_OK) rc = device_setup(h);
     if (rc != MPS_OK) {
@@ -1006,6 +1042,35 @@ extern "C" int mps_get_dofs(mps_handle h, int32_t which, int64_t n, const int32_
     hipLaunchKernelGGL(k_pick, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, h->stream, src, h->pick_idx, n, h->pick_out);
     HCHK(h, hipGetLastError());
     HCHK(h, hipMemcpyAsync(out, h->pick_out, (size_t)n * 8, hipMemcpyDeviceToHost, h->stream));
+    HCHK(h, hipStreamSynchronize(h->stream));
+    return MPS_OK;
+}
+
+// mps_get_dofs staged through the device's fixed `scratch` array: no allocation in the call.  Nothing in the
+// array needs to start at zero (the call writes the n indices it reads and the n values it copies out), and
+// the call returns only after its copies finished, so calls on one device race only when they overlap.
+extern "C" int mps_get_dofs_static(mps_handle h, int32_t which, int64_t n, const int32_t* dof, double* out) {
+    if (!h || which < 0 || which > 2 || n < 0 || (n > 0 && (!dof || !out))) return MPS_ERR_ARG;
+    if (n > MPS_DOFS_STATIC_MAX)
+        return h->fail(MPS_ERR_CAPACITY, fmt("mps_get_dofs_static: %lld DOFs requested, at most %d per call",
+                                             (long long)n, MPS_DOFS_STATIC_MAX));
+    if (n == 0) return MPS_OK;
+    const int64_t ndof = 3 * (int64_t)h->I.N;
+    for (int64_t i = 0; i < n; i++)
+        if (dof[i] < 0 || dof[i] >= ndof)
+            return h->fail(MPS_ERR_ARG, fmt("mps_get_dofs_static: dof %d outside [0, %lld)", dof[i], (long long)ndof));
+    (void)hipSetDevice(h->device);
+    hipError_t why = hipSuccess;
+    char* base = (char*)scratch_base(h->device, &why);
+    if (!base)
+        return h->fail(MPS_ERR_HIP, fmt("mps_get_dofs_static: device staging array not resolved: %s", hipGetErrorString(why)));
+    int32_t* idx = reinterpret_cast<int32_t*>(base);
+    double* vals = reinterpret_cast<double*>(base + MPS_STATIC_IDX_BYTES);
+    const double* src = which == 0 ? h->I.u : (which == 1 ? h->I.v : h->I.a);
+    HCHK(h, hipMemcpyAsync(idx, dof, (size_t)n * 4, hipMemcpyHostToDevice, h->stream));
+    hipLaunchKernelGGL(k_pick, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, h->stream, src, idx, n, vals);
+    HCHK(h, hipGetLastError());
+    HCHK(h, hipMemcpyAsync(out, vals, (size_t)n * 8, hipMemcpyDeviceToHost, h->stream));
     HCHK(h, hipStreamSynchronize(h->stream));
     return MPS_OK;
 }

@@ -19,6 +19,7 @@ MPS_RUNNING, MPS_DONE = 0, 1
 LAW_NH_COMPRESSIBLE, LAW_NH_BELYTSCHKO, LAW_SVK, LAW_LINEAR_ELASTIC = 0, 1, 2, 3
 BC_RAMP, BC_INST, BC_VELOCITY, BC_RAMP_RATE = 0, 1, 2, 3
 PREC_FP64, PREC_FP32_ELEM = 0, 1
+DOFS_STATIC_MAX = 65536  # MPS_DOFS_STATIC_MAX
 
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int32)
@@ -134,6 +135,7 @@ def load(path=None):
     L.mps_set_viscosity.argtypes = [H, C.c_int32, C.c_double, C.c_double, C.c_double, C.c_double]
     L.mps_external_force.argtypes = [H, C.c_double, _dp]
     L.mps_get_dofs.argtypes = [H, C.c_int32, C.c_int64, _ip, _dp]
+    L.mps_get_dofs_static.argtypes = [H, C.c_int32, C.c_int64, _ip, _dp]
     L.mps_destroy.argtypes = [H]
     L.mps_last_error.restype = C.c_char_p
     L.mps_last_error.argtypes = [H]
@@ -299,6 +301,20 @@ class Solver:
         u, v, a = (np.zeros(3 * self.n) for _ in range(3))
         self._chk(self.L.mps_get_state(self.h, _d(u), _d(v), _d(a)))
         return u, v, a
+
+    def get_dofs(self, which, dof):
+        """u (which=0), v (1) or a (2) at DOF indices node + n_nodes*dir (mps_get_dofs)."""
+        dof = np.ascontiguousarray(dof, dtype=np.int32)
+        out = np.zeros(dof.size)
+        self._chk(self.L.mps_get_dofs(self.h, which, dof.size, _i(dof), _d(out)))
+        return out
+
+    def get_dofs_static(self, which, dof):
+        """As get_dofs, staged through the device's fixed array (mps_get_dofs_static: no allocation)."""
+        dof = np.ascontiguousarray(dof, dtype=np.int32)
+        out = np.zeros(dof.size)
+        self._chk(self.L.mps_get_dofs_static(self.h, which, dof.size, _i(dof), _d(out)))
+        return out
 
     def run(self, max_steps):
         """Returns (status_code, report dict); errors of the solve are codes, not exceptions."""

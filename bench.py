@@ -226,14 +226,18 @@ def batch_leg(precision, n_inst=512, steps=1000, plain_sample=48):
             **({"max_rel_disp_err_vs_fp64": err} if precision == G.PREC_FP32_ELEM else {})}
 
 
-def serving_leg(lambdas, n_req=48, precision=G.PREC_FP32_ELEM, plain_sample=16):
+def serving_leg(lambdas, n_req=200, precision=G.PREC_FP32_ELEM, plain_sample=32, oracle_sample=4):
     """Poisson arrivals at rate lam of hex8 boxes (N_e log-U[1e3,2e5], aspect U[1,4], steps U{500..5000});
-    requests are admitted by device footprint into the running batch (continuous batching)."""
+    requests are admitted by device footprint into the running batch (continuous batching).  Each rate serves
+    n_req requests (its own draws); the `oracle_sample` cheapest requests of the last rate are checked against the
+    FP64 oracle on the host."""
     import threading
-    reqs = configs.serving_requests(n_req)
+    import oracle_ctypes as O
     points = []
+    last = None
     with G.Server(arena_bytes=16 << 30, max_instances=1024, precision=precision, steps_per_launch=32) as srv:
-        for lam in lambdas:
+        for li, lam in enumerate(lambdas):
+            reqs = configs.serving_requests(n_req, seed=configs.SEED_CFG4 + li)
             srv.reset_stats()
             rng = np.random.default_rng(int(lam))
             ids = []
@@ -248,23 +252,42 @@ def serving_leg(lambdas, n_req=48, precision=G.PREC_FP32_ELEM, plain_sample=16):
             th.join()
             assert srv.drain(600.0) == 0
             st = srv.stats()
+            us = []
             for rid in ids:
                 rc, u, _, _ = srv.result(rid)
                 assert rc == 0
-            points.append({"lambda_per_s": lam, "solves_per_s": st["solves_per_s"], "p50_ms": st["p50_ms"],
-                           "p99_ms": st["p99_ms"], "queue_p50_ms": st["queue_p50_ms"], "completed": st["completed"]})
+                us.append(u)
+            points.append({"lambda_per_s": lam, "requests": n_req, "solves_per_s": st["solves_per_s"],
+                           "p50_ms": st["p50_ms"], "p99_ms": st["p99_ms"], "max_ms": st["max_ms"],
+                           "queue_p50_ms": st["queue_p50_ms"], "completed": st["completed"],
+                           "elem_block": st["elem_block"], "grid_mult": st["grid_mult"],
+                           "tuned_mixes": st["tunings"]})
             note(f"serving lambda={lam:g}/s: {st['solves_per_s']:.1f} solves/s, p50 {st['p50_ms']:.0f} ms, "
-                 f"p99 {st['p99_ms']:.0f} ms")
+                 f"p99 {st['p99_ms']:.0f} ms, tuned (bs {st['elem_block']}, grid x{st['grid_mult']})")
+            last = (reqs, us)
+    # one-at-a-time plain launches of a sample of the same request distribution
+    reqs, us = last
     rng = np.random.default_rng(5)
     pick = sorted(rng.choice(n_req, plain_sample, replace=False))
     plain_t = sum(plain_solve(reqs[i], precision)[0] for i in pick)
     plain_rate = plain_sample / plain_t
-    best = max(p["solves_per_s"] for p in points)
-    return {"workload": f"{n_req} requests per rate: hex8 boxes N_e log-U[1e3,2e5], aspect U[1,4], steps U{{500..5000}}, "
-                        "Poisson arrivals",
+    for p_ in points:
+        p_["speedup_vs_plain"] = p_["solves_per_s"] / plain_rate
+    # served results against the FP64 oracle (the cheapest requests, so the host check stays short)
+    cost = [r["mesh"]["n_elem"] * r["n_steps"] for r in reqs]
+    worst = 0.0
+    for i in np.argsort(cost)[:oracle_sample]:
+        r = reqs[int(i)]
+        o = O.OracleProblem(r["mesh"], E=r["E"], nu=r["nu"], dir_dof=r["dir_dof"], dir_val=r["dir_val"],
+                            n_steps=r["n_steps"], sum_order=O.SUM_ELEMENT,
+                            n_threads=max(1, min(16, os.cpu_count() or 1))).run()
+        worst = max(worst, float(np.abs(us[int(i)] - o["u"]).max() / np.abs(o["u"]).max()))
+    assert worst <= (1e-6 if precision == G.PREC_FP32_ELEM else 1e-12), worst
+    return {"workload": f"{n_req} requests per rate: hex8 boxes N_e log-U[1e3,2e5], aspect U[1,4], "
+                        "steps U{500..5000}, Poisson arrivals",
             "precision": "fp32-element" if precision == G.PREC_FP32_ELEM else "fp64",
             "points": points, "plain_one_at_a_time_solves_per_s": plain_rate, "plain_sample": plain_sample,
-            "speedup_vs_plain": best / plain_rate}
+            "max_rel_disp_err_vs_fp64_oracle": worst, "oracle_checked_requests": oracle_sample}
 
 
 def main():

@@ -195,6 +195,14 @@ int mps_get_dofs(mps_handle h, int32_t which, int64_t n, const int32_t *dof, dou
 #define MPS_DOFS_STATIC_MAX 65536
 int mps_get_dofs_static(mps_handle h, int32_t which, int64_t n, const int32_t *dof, double *out);
 
+/* Internal force at n DOFs dof[i] = node + n_nodes*dir for the solver's current displacement, into out[i]:
+ * the force the last step assembled (with its viscous part), summed from the step's partial nodal forces in
+ * the order the update used; before the first step, the force at the initial displacement.  Only the listed
+ * DOFs cross to the host.
+ * Replaces: the intForce look-up of printForcesOut (output.cpp:153-279; intForce of computeRightHandSide,
+ * solverExplicit.cpp:293-304) and of the InternalForce node extraction (dataExtraction.cpp:784-811). */
+int mps_get_internal_force_dofs(mps_handle h, int64_t n, const int32_t *dof, double *out);
+
 /* Advance up to max_steps explicit steps (fewer if the time window or the
  * fixed step count ends, or on error).  rep may be NULL.
  * Replaces: the time loop of classSolverExplicit::calculate
@@ -278,8 +286,10 @@ typedef struct mps_batch_cfg {
     int32_t max_instances;/* instance slots (0: 1024) */
     int32_t precision;    /* MPS_PREC_* for every instance */
     int32_t steps_per_launch; /* steps captured per graph launch (0: 32) */
-    int32_t elem_block;   /* elements per workgroup (0: autotuned) */
-    int32_t grid_mult;    /* persistent grid = grid_mult * CUs workgroups (0: autotuned) */
+    int32_t elem_block;   /* work-items per element workgroup, 64/128/256 (0: 256 until mps_batch_autotune;
+                             the server tunes it per mix size when both this and grid_mult are 0) */
+    int32_t grid_mult;    /* persistent grid = grid_mult * CUs workgroups (0: 2 until mps_batch_autotune;
+                             tuned by the server as elem_block) */
     int32_t device;
 } mps_batch_cfg;
 
@@ -334,6 +344,9 @@ typedef struct mps_server_stats {
     double span_ms;                 /* first submission -> last completion */
     double solves_per_s;            /* completed / span */
     double busy_ms;                 /* time with a graph launch in flight */
+    int32_t elem_block, grid_mult;  /* tuning of the latest launch */
+    int32_t tunings;                /* mix sizes the online tuner has settled */
+    int32_t _pad;
 } mps_server_stats;
 
 int mps_server_create(const mps_server_cfg *cfg, mps_server *out);

@@ -1413,6 +1413,29 @@ double orc_critical_dt(const orc_problem *p, const double *u)
     return dt;
 }
 
+/* The energy entries of history row k for the state u, v of the step just completed: DEFO_ENERGY, KIN_ENERGY,
+ * VOLUME (dataExtraction.cpp:569-776), EXTERNAL_ENERGY = u . fext of the step (:867-875; s->fext holds it)
+ * and the lumped-mass kinetic energy 1/2 sum m v^2 (sum over DOFs in index order). */
+static void history_energies(orc_state *s, const double *u, const double *v, orc_history *hist, int64_t k)
+{
+    double en[3];
+    energies(s, u, v, en);
+    if (hist->e_def) hist->e_def[k] = en[0];
+    if (hist->e_kin) hist->e_kin[k] = en[1];
+    if (hist->volume) hist->volume[k] = en[2];
+    size_t N = s->N;
+    if (hist->e_ext) {
+        double e = 0.0;
+        for (size_t i = 0; i < 3 * N; i++) e += u[i] * s->fext[i];
+        hist->e_ext[k] = e;
+    }
+    if (hist->e_kin_lumped) {
+        double e = 0.0;
+        for (size_t i = 0; i < 3 * N; i++) e += 0.5 * s->mass[i % N] * v[i] * v[i];
+        hist->e_kin_lumped[k] = e;
+    }
+}
+
 /* classSolverExplicit::calculate: solverExplicit.cpp:63-361 (sequential branch). */
 int32_t orc_run(const orc_problem *p, double *u, double *v, double *a, orc_history *hist,
                 orc_report *rep)
@@ -1422,7 +1445,8 @@ int32_t orc_run(const orc_problem *p, double *u, double *v, double *a, orc_histo
     orc_report r = {0, 0, 0, ORC_OK, 0};
     worker_arg *wa = NULL;
     pthread_t *th = NULL;
-    double *uzero = NULL, *uprev = NULL;
+    double *uzero = NULL, *uprev = NULL, *incr = NULL, *fint_prev = NULL, *fext_prev = NULL;
+    double w_ext = 0.0, w_react = 0.0;
     if (rc != ORC_OK) goto out;
     th = pool_start(&s, &wa);
     size_t N = s.N;
@@ -1437,7 +1461,10 @@ int32_t orc_run(const orc_problem *p, double *u, double *v, double *a, orc_histo
     uzero = calloc(3 * N, sizeof(double));
     /* F of the previous state (solverExplicit.cpp:149: computeKinematicVariables(u0) + nextStep) */
     uprev = malloc(sizeof(double) * 3 * N);
-    if (!uzero || !uprev) { rc = ORC_ERR_MEM; goto out; }
+    incr = calloc(3 * N, sizeof(double));
+    fint_prev = calloc(3 * N, sizeof(double));
+    fext_prev = calloc(3 * N, sizeof(double));
+    if (!uzero || !uprev || !incr || !fint_prev || !fext_prev) { rc = ORC_ERR_MEM; goto out; }
     memcpy(uprev, u, sizeof(double) * 3 * N);
     const double *uK = (timeRun < 1E-12) ? u : uzero;
     /* a_prev is a at the last nextStep (:149 -> nonlinearSystem.cpp:710-724); it is
@@ -1464,8 +1491,13 @@ int32_t orc_run(const orc_problem *p, double *u, double *v, double *a, orc_histo
         if (dt < 1E-20) { rc = ORC_ERR_DT; break; } /* :270-273 */
         nStep++;
         /* dynamicPrediction, nonlinearSystem.cpp:1186-1201 (active DOFs) */
-        for (size_t i = 0; i < 3 * N; i++)
-            if (s.active[i]) u[i] += dt * v[i] + 0.5 * dt * dt * a[i];
+        for (size_t i = 0; i < 3 * N; i++) {
+            incr[i] = 0.0;
+            if (s.active[i]) {
+                incr[i] = dt * v[i] + 0.5 * dt * dt * a[i];
+                u[i] += incr[i];
+            }
+        }
         /* applyDisplacementBC, solverExplicit.cpp:545-575 (input order) */
         for (int k = 0; k < p->n_dir; k++) {
             int32_t ind = p->dir_dof[k];
@@ -1480,11 +1512,15 @@ int32_t orc_run(const orc_problem *p, double *u, double *v, double *a, orc_histo
                 if (f > 1.0) f = 1.0;
                 double V = p->dir_val[k] * f;
                 U = V * dt;
+                incr[ind] = U;
+                if (nStep >= 2) w_react += 0.5 * s.mass[(size_t)ind % N] * (V * V - v[ind] * v[ind]);
                 u[ind] += U;
                 v[ind] = V;
                 a[ind] = 0.;
                 continue;
             }
+            incr[ind] = U;
+            if (nStep >= 2) w_react += 0.5 * s.mass[(size_t)ind % N] * (U * U - v[ind] * v[ind]);
             u[ind] += U;
             v[ind] = U;
             a[ind] = 0.;
@@ -1493,6 +1529,16 @@ int32_t orc_run(const orc_problem *p, double *u, double *v, double *a, orc_histo
         internal_force_visc(&s, u, uprev, dt);
         /* NeumannBCManagement at timeRun (boundaryConditions.cpp:75-140) */
         external_force(&s, u, timeRun);
+        /* work of the loads and of the supports over the step (trapezoid, from step 2 on) */
+        if (nStep >= 2) {
+            for (size_t i = 0; i < 3 * N; i++) {
+                w_ext += 0.5 * (fext_prev[i] + s.fext[i]) * incr[i];
+                if (!s.active[i])
+                    w_react += 0.5 * ((fint_prev[i] - fext_prev[i]) + (s.fint[i] - s.fext[i])) * incr[i];
+            }
+        }
+        memcpy(fint_prev, s.fint, sizeof(double) * 3 * N);
+        memcpy(fext_prev, s.fext, sizeof(double) * 3 * N);
         /* systemSolve, nonlinearSystem.cpp:1096-1107: a = (-fint + fext)/m, v update */
         for (size_t i = 0; i < 3 * N; i++) {
             if (!s.active[i]) continue;
@@ -1508,28 +1554,16 @@ int32_t orc_run(const orc_problem *p, double *u, double *v, double *a, orc_histo
         if (hist && k < hist->cap) {
             if (hist->time) hist->time[k] = timeRun;
             if (hist->dt) hist->dt[k] = dt;
+            if (hist->w_ext) hist->w_ext[k] = w_ext;
+            if (hist->w_react) hist->w_react[k] = w_react;
         }
         if (every > 0 && (nStep % every) == 0) {
-            double en[3];
-            energies(&s, u, v, en);
             last_e = nStep;
-            if (hist && k < hist->cap) {
-                if (hist->e_def) hist->e_def[k] = en[0];
-                if (hist->e_kin) hist->e_kin[k] = en[1];
-                if (hist->volume) hist->volume[k] = en[2];
-            }
+            if (hist && k < hist->cap) history_energies(&s, u, v, hist, k);
         }
     }
-    if (every > 0 && nStep > 0 && last_e != nStep && rc == ORC_OK) {
-        double en[3];
-        energies(&s, u, v, en);
-        int64_t k = nStep - 1;
-        if (hist && k < hist->cap) {
-            if (hist->e_def) hist->e_def[k] = en[0];
-            if (hist->e_kin) hist->e_kin[k] = en[1];
-            if (hist->volume) hist->volume[k] = en[2];
-        }
-    }
+    if (every > 0 && nStep > 0 && last_e != nStep && rc == ORC_OK && hist && nStep - 1 < hist->cap)
+        history_energies(&s, u, v, hist, nStep - 1);
     r.steps = nStep;
     r.time = timeRun;
     r.dt_last = dt;
@@ -1538,6 +1572,9 @@ out:
     pool_stop(&s, th, wa);
     free(uzero);
     free(uprev);
+    free(incr);
+    free(fint_prev);
+    free(fext_prev);
     state_free(&s);
     if (rep) *rep = r;
     return rc;

@@ -107,6 +107,16 @@ typedef struct orc_history {
     double *e_def;          /* DEFO_ENERGY, Sum       (dataExtraction.cpp:569-629) */
     double *e_kin;          /* KIN_ENERGY, Sum        (dataExtraction.cpp:631-707) */
     double *volume;         /* VOLUME, Sum            (dataExtraction.cpp:709-776) */
+    /* Energy balance (no reference counterpart; any may be NULL).  On energy steps: */
+    double *e_ext;          /* EXTERNAL_ENERGY, sum_i u_i fext_i (dataExtraction.cpp:867-875) */
+    double *e_kin_lumped;   /* 1/2 sum_i m_i v_i^2, the kinetic energy of the lumped-mass update */
+    /* On every step, trapezoidal work since the end of step 1 (0 at step 1), with du the step's increment:
+     * w_ext   = sum over steps and DOFs of 1/2 (fext_prev + fext) . du
+     * w_react = sum over steps and constrained DOFs of 1/2 (r_prev + r) . du, r = fint - fext the support force,
+     *           plus the kinetic energy the prescribed velocity gives the DOF's lumped mass, 1/2 m (v^2 - v_prev^2)
+     *           (the update sets a = 0 on constrained DOFs, so r carries no inertia) */
+    double *w_ext;
+    double *w_react;
 } orc_history;
 
 /* ---- component functions (exported for the pin tests) ---- */

@@ -109,9 +109,11 @@ class Report(C.Structure):
                 ("status", C.c_int32), ("_pad", C.c_int32)]
 
 
+HIST_KEYS = ("time", "dt", "e_def", "e_kin", "volume", "e_ext", "e_kin_lumped", "w_ext", "w_react")
+
+
 class History(C.Structure):
-    _fields_ = [("cap", C.c_int64), ("time", _dp), ("dt", _dp), ("e_def", _dp),
-                ("e_kin", _dp), ("volume", _dp)]
+    _fields_ = [("cap", C.c_int64)] + [(k, _dp) for k in HIST_KEYS]
 
 
 _lib = None
@@ -220,7 +222,7 @@ class OracleProblem:
         hist = None
         harr = {}
         if hist_cap:
-            for k in ("time", "dt", "e_def", "e_kin", "volume"):
+            for k in HIST_KEYS:
                 harr[k] = np.full(hist_cap, np.nan)
             hist = History(cap=hist_cap, **{k: dptr(v_) for k, v_ in harr.items()})
         rep = Report()

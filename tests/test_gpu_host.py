@@ -326,3 +326,74 @@ def test_vtk_series_matches_state_and_oracle(tmp_path):
     assert not any(k.startswith("IntVars") for k in s)  # the second *Outputs names no internal variables
     _, s1, _ = parse_vtk(files[1])  # the first solver's *Outputs asked for variables 1 and 2
     assert s1["IntVars1"] == [["0"]] * Ne and s1["IntVars2"] == [["0"]] * Ne
+
+
+def test_forces_output_against_the_oracle(tmp_path):
+    """*Forces inside an EXPLICIT block (forceOutManagement, inputFile.cpp:4830-4887; printForcesOut,
+    output.cpp:153-279): externalForces<t0>-<tf>.csv gets one row per listed (node, direction) at every step whose
+    time reaches iterations*max(period, dt) + t0, with that step's internal force, std::to_string formatted."""
+    m = configs.perturb(configs.box(3, 3, 4), 0.1, 29)
+    N = m["n_nodes"]
+    dof, val = configs.clamp_bottom_pull_top(m, vel=0.02)
+    bc = inpgen.bc_triples(dof, val, N)
+    probe = O.OracleProblem(m, dir_dof=dof, dir_val=val, bc_mode=O.BC_RAMP, t0=0.0, tf=1.0, scale=0.5)
+    dt0 = probe.critical_dt()
+    tf = 60.5 * dt0
+    period = 7.3 * dt0
+    pairs = [(N - 1, 2), (0, 0), (int(np.argmin(np.abs(m["X"][:N] - 0.5) + np.abs(m["X"][N:2 * N] - 0.5)
+                                                  + np.abs(m["X"][2 * N:] - 0.5))), 1)]
+    lines = ["*Forces", f"{1.0 / period:.17g}"] + [f"{n + 1},{d}" for n, d in pairs]
+    inp = tmp_path / "forces.inp"
+    inpgen.write_inp(inp, m, [dict(rho=1000.0, E=1e6, nu=0.3)],
+                     [dict(scale=0.5, t0=0.0, tf=tf, outputs=1, bc=bc, extra_lines=lines)])
+    u, _ = run(inp, tmp_path, "-noVtk")
+    o = O.OracleProblem(m, dir_dof=dof, dir_val=val, bc_mode=O.BC_RAMP, t0=0.0, tf=tf, scale=0.5)
+    r = o.run(hist_cap=4096)
+    assert r["rc"] == 0 and rel(u, r["u"]) <= 1e-12
+    steps = r["steps"]
+    times, dts = r["hist"]["time"][:steps], r["hist"]["dt"][:steps]
+    due, it = [], 0
+    for k in range(steps):  # the test of printForcesOut, step by step
+        p = max(period, dts[k])
+        if times[k] >= it * p + 0.0:
+            due.append(k)
+            it += 1
+    assert len(due) >= 7
+    path = tmp_path / f"externalForces{0.0:f}-{tf:f}.csv"
+    rows = path.read_text().split("\n")
+    assert rows[0] == "Node,Direction,Force,Time,"
+    body = [x for x in rows[1:] if x]
+    assert len(body) == len(due) * len(pairs)
+    fscale = 0.0
+    for j, k in enumerate(due):
+        uk = O.OracleProblem(m, dir_dof=dof, dir_val=val, bc_mode=O.BC_RAMP, t0=0.0, tf=tf, scale=0.5,
+                             n_steps=k + 1).run()["u"]
+        f = o.internal_force(uk)
+        fscale = max(fscale, np.abs(f).max())
+        for q, (n, d) in enumerate(pairs):
+            node, direction, force, time, tail = body[j * len(pairs) + q].split(",")
+            assert (node, direction, tail) == (str(n + 1), str(d), "")
+            assert time == f"{times[k]:f}"
+            assert abs(float(force) - f[n + d * N]) <= 5e-7 + 1e-12 * abs(f[n + d * N]), (k, n, d)
+    assert fscale > 1.0  # forces of order newtons: the six printed decimals carry the comparison
+
+
+def test_internal_force_dofs_equal_the_full_gather():
+    """mps_get_internal_force_dofs: the step's partial forces summed at the listed DOFs, bitwise the
+    corresponding entries of mps_internal_force at the same displacement; before the first step, the force of
+    the initial state."""
+    m = configs.perturb(configs.box(5, 4, 4), 0.1, 33)
+    N = m["n_nodes"]
+    dof, val = configs.clamp_bottom_pull_top(m, vel=0.05)
+    idx = np.arange(0, 3 * N, 5, dtype=np.int32)
+    import mps_gpu as G
+    for prec in (G.PREC_FP64, G.PREC_FP32_ELEM):
+        with G.Solver(m, dir_dof=dof, dir_val=val, n_steps=50, precision=prec) as s:
+            assert np.array_equal(s.internal_force_dofs(idx), np.zeros(idx.size))  # u0 = 0
+            rc, rep = s.run(37)
+            assert rc == 0
+            u = s.get_state()[0]
+            full = s.internal_force(u)  # its own partial buffer: the step's stay valid
+            got = s.internal_force_dofs(idx)
+            assert np.array_equal(got, full[idx]), prec
+            assert np.abs(got).max() > 0

@@ -246,3 +246,25 @@ def test_coordinates_round_trip(tmp_path):
     xor = int(np.bitwise_xor.reduce(np.ascontiguousarray(m["X"]).view(np.uint64)))
     assert int(j["x_xor"], 16) == xor
     assert j["conn_sum"] == int(m["conn"].astype(np.int64).sum())
+
+
+def test_forces_block(tmp_path):
+    """*Forces: frequency line, then "node, direction" lines (direction 0-based, < ndim); a later *Forces of the
+    same block replaces the earlier one; the block belongs to its solver only."""
+    m, bc = cube(2)
+    p = tmp_path / "f.inp"
+    inpgen.write_inp(p, m, [dict(rho=1000.0, E=1e6, nu=0.3)],
+                     [dict(scale=0.5, t0=0.0, tf=0.01, outputs=1, bc=bc,
+                           extra_lines=["*Forces", "4", "27,0", "*Forces", "20", "1, 2", "5,1"]),
+                      dict(scale=0.5, t0=0.01, tf=0.02, outputs=1, bc=bc)])
+    rc, j, err = check(p)
+    assert rc == 0, err
+    assert j["solvers"][0]["forces"] == dict(period=0.05, entries=[[0, 2], [4, 1]])
+    assert j["solvers"][1]["forces"] is None
+    for bad, msg in ((["*Forces", "10", "1,3"], "exceeds the simulation dimension"),
+                     (["*Forces", "10", "28,0"], "outside the mesh"),
+                     (["*Forces", "0", "1,0"], "positive output frequency")):
+        inpgen.write_inp(p, m, [dict(rho=1000.0, E=1e6, nu=0.3)],
+                         [dict(scale=0.5, t0=0.0, tf=0.01, outputs=1, bc=bc, extra_lines=bad)])
+        rc, _, err = check(p)
+        assert rc != 0 and msg in err, err

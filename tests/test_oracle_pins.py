@@ -936,3 +936,72 @@ def test_volume_is_det_G_under_homogeneous_deformation(tet):
     J = np.linalg.det(G)
     W = 0.5 * lam * np.log(J) ** 2 - mu * np.log(J) + 0.5 * mu * np.trace(G.T @ G)
     assert ed == pytest.approx(W * w, rel=1e-12)
+
+
+# ------------------------------------------------------------------ energy balance (build-defined)
+
+def test_work_and_energies_in_free_fall():
+    """Free fall under gravity, nothing constrained: the body translates rigidly, so EXTERNAL_ENERGY = u . fext
+    = -M g u_z, the lumped kinetic energy is 1/2 M v_z^2, and the trapezoidal work of the constant weight from
+    the end of step 1 is exactly -M g (u_z(k) - u_z(1))."""
+    m = perturb(box(3, 2, 2), 0.1, 41)
+    N = m["n_nodes"]
+    g, rho = 9.81, 1000.0
+    p = O.OracleProblem(m, rho=rho, n_steps=60, energy_every=1, loads=[("gravity", g)], bc_mode=O.BC_RAMP)
+    r = p.run(hist_cap=60)
+    h = r["hist"]
+    M = rho * 1.0  # unit-volume box
+    uz = r["u"][2 * N:]
+    assert np.ptp(uz) <= 1e-12 * np.abs(uz).max()  # rigid: every node falls alike
+    assert h["e_ext"][-1] == pytest.approx(-M * g * uz[0], rel=1e-12)
+    assert h["e_kin_lumped"][-1] == pytest.approx(0.5 * M * r["v"][2 * N] ** 2, rel=1e-12)
+    # u_z after step 1 from the recurrence: a0 = 0, so step 1 moves nothing; then u_z(k) from the history
+    dts = h["dt"]
+    vz, az, u1 = 0.0, 0.0, None
+    uzk = 0.0
+    for k in range(60):
+        uzk += dts[k] * vz + 0.5 * dts[k] * dts[k] * az
+        vz, az = vz + dts[k] * 0.5 * (az - g), -g
+        if k == 0:
+            u1 = uzk
+    assert uzk == pytest.approx(uz[0], rel=1e-12)
+    assert h["w_ext"][0] == 0.0
+    assert h["w_ext"][-1] == pytest.approx(-M * g * (uzk - u1), rel=1e-10)
+    assert np.all(h["w_react"] == 0.0)
+
+
+def test_reaction_work_carries_the_prescribed_momentum():
+    """All DOFs prescribed in a rigid x-translation whose velocity ramps up: nothing deforms, so the support
+    force fint - fext vanishes and the reaction work is the kinetic energy the prescribed velocity gives the
+    lumped mass, 1/2 M (V_k^2 - V_1^2).  A reaction work without that inertia term stays 0 and fails."""
+    m = box(2, 2, 2)
+    N = m["n_nodes"]
+    dof = np.arange(3 * N, dtype=np.int32)
+    val = np.concatenate([np.full(N, 0.4), np.zeros(2 * N)])
+    p = O.OracleProblem(m, rho=1000.0, dir_dof=dof, dir_val=val, bc_mode=O.BC_VELOCITY, vel_ramp_steps=20,
+                        n_steps=30, energy_every=1)
+    r = p.run(hist_cap=30)
+    h = r["hist"]
+    V = 0.4 * np.minimum(np.arange(1, 31) / 20, 1.0)
+    M = 1000.0
+    np.testing.assert_allclose(h["w_react"], 0.5 * M * (V ** 2 - V[0] ** 2), rtol=1e-12, atol=1e-14)
+    np.testing.assert_allclose(h["e_kin_lumped"], 0.5 * M * V ** 2, rtol=1e-12)
+    assert np.all(h["w_ext"] == 0.0)
+
+
+def test_energy_balance_closes_on_a_pulled_block():
+    """Lumped kinetic + deformation energy against the work of the loads and supports, from the end of step 1:
+    the explicit update conserves it up to its O(dt^2) error, here to 1e-5 of the work at the end and within
+    3% at every step after the first few."""
+    m = perturb(box(5, 5, 5), 0.1, 17)
+    dof, val = clamp_and_pull(m, vel=0.1)
+    p = O.OracleProblem(m, dir_dof=dof, dir_val=val, bc_mode=O.BC_VELOCITY, vel_ramp_steps=50, n_steps=400,
+                        energy_every=1)
+    h = p.run(hist_cap=400)["hist"]
+    E = h["e_def"] + h["e_kin_lumped"]
+    W = h["w_ext"] + h["w_react"]
+    B = (E - E[0]) - W
+    scale = np.maximum.accumulate(np.maximum(np.abs(W), np.abs(h["e_def"] - h["e_def"][0]) + h["e_kin_lumped"]))
+    assert W[-1] > 10.0
+    assert abs(B[-1]) <= 1e-5 * scale[-1]
+    assert np.all(np.abs(B[5:]) <= 3e-2 * scale[5:])

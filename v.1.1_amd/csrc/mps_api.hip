@@ -300,6 +300,8 @@ struct mps_solver_s {
     int64_t pick_cap = 0;
     std::vector<void*> load_allocs;  // mps_set_loads buffers (replaced by the next call)
     double* uprev_buf = nullptr;     // previous-step displacement for bulk viscosity (in allocs)
+    double* fint_part = nullptr;     // partial forces of mps_internal_force (allocated on first use, in allocs)
+    bool part_valid = false;         // I.part holds the forces of the current u (false: set-up scratch)
     int energy_nb = 0;
     bool primed = false;       // prologue (first dt) done
     long long next_step = 1;   // host mirror of ctl->A
@@ -592,6 +594,7 @@ extern "C" int mps_set_state(mps_handle h, const double* u, const double* v, con
         if (src[i]) HCHK(h, hipMemcpyAsync(dst[i], src[i], bytes, hipMemcpyHostToDevice, h->stream));
         else HCHK(h, hipMemsetAsync(dst[i], 0, bytes, h->stream));
     }
+    h->part_valid = false;
     // the handed-over u is also the previous state of the first step (solverExplicit.cpp:149)
     if (h->I.uprev) HCHK(h, hipMemcpyAsync(h->I.uprev, h->I.u, bytes, hipMemcpyDeviceToDevice, h->stream));
     HCHK(h, hipStreamSynchronize(h->stream));
@@ -984,6 +987,7 @@ extern "C" int mps_set_loads(mps_handle h, int32_t n_loads, const mps_load* load
     if (grav_block >= 0 && grav != 0.0) {
         HCHK(h, upload(nullptr, (size_t)N * 8, &p));
         // the partial-force buffer (>= nen*Ne doubles) is free until the first step
+        h->part_valid = false;
         hipError_t e = mpsk::launch_gravity(I, grav, I.part, (double*)p, h->stream);
         if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
         if (e != hipSuccess) return h->fail(MPS_ERR_HIP, fmt("gravity load: %s", hipGetErrorString(e)));
@@ -1071,6 +1075,56 @@ extern "C" int mps_get_dofs_static(mps_handle h, int32_t which, int64_t n, const
     hipLaunchKernelGGL(k_pick, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, h->stream, src, idx, n, vals);
     HCHK(h, hipGetLastError());
     HCHK(h, hipMemcpyAsync(out, vals, (size_t)n * 8, hipMemcpyDeviceToHost, h->stream));
+    HCHK(h, hipStreamSynchronize(h->stream));
+    return MPS_OK;
+}
+
+// Internal force at DOF idx[i] = node + N*dir: the node's partials in ascending patch order, summed as the node
+// kernel sums them (k_gather_fint).
+__global__ void k_pick_fint(MpsInst I, const int32_t* __restrict__ idx, int64_t n, double* __restrict__ out) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int dof = idx[i], node = dof % I.N, dir = dof / I.N;
+    double f = 0.0;
+    for (int q = I.nrow[node]; q < I.nrow[node + 1]; q++) {
+        double c[3];
+        mpsd::load_partial(I, I.ncol[q], c);
+        f += c[dir];
+    }
+    out[i] = f;
+}
+
+extern "C" int mps_get_internal_force_dofs(mps_handle h, int64_t n, const int32_t* dof, double* out) {
+    if (!h || n < 0 || (n > 0 && (!dof || !out))) return MPS_ERR_ARG;
+    if (n == 0) return MPS_OK;
+    const int64_t ndof = 3 * (int64_t)h->I.N;
+    for (int64_t i = 0; i < n; i++)
+        if (dof[i] < 0 || dof[i] >= ndof)
+            return h->fail(MPS_ERR_ARG, fmt("mps_get_internal_force_dofs: dof %d outside [0, %lld)", dof[i], (long long)ndof));
+    (void)hipSetDevice(h->device);
+    if (!h->part_valid) {  // before the first step: the forces of the current u (no viscous part: no increment yet)
+        MpsInst I = h->I;
+        I.ctl = nullptr;
+        hipError_t e = mpsk::launch_elem(I, 1, h->stream);
+        if (e != hipSuccess) return h->fail(MPS_ERR_HIP, fmt("internal force: %s", hipGetErrorString(e)));
+        h->part_valid = true;
+    }
+    if (n > h->pick_cap) {
+        HCHK(h, hipStreamSynchronize(h->stream));
+        if (h->pick_idx) (void)hipFree(h->pick_idx);
+        if (h->pick_out) (void)hipFree(h->pick_out);
+        h->pick_idx = nullptr;
+        h->pick_out = nullptr;
+        h->pick_cap = 0;
+        HCHK(h, hipMalloc(&h->pick_idx, (size_t)n * 4));
+        HCHK(h, hipMalloc(&h->pick_out, (size_t)n * 8));
+        h->pick_cap = n;
+    }
+    HCHK(h, hipMemcpyAsync(h->pick_idx, dof, (size_t)n * 4, hipMemcpyHostToDevice, h->stream));
+    hipLaunchKernelGGL(k_pick_fint, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, h->stream, h->I, h->pick_idx, n,
+                       h->pick_out);
+    HCHK(h, hipGetLastError());
+    HCHK(h, hipMemcpyAsync(out, h->pick_out, (size_t)n * 8, hipMemcpyDeviceToHost, h->stream));
     HCHK(h, hipStreamSynchronize(h->stream));
     return MPS_OK;
 }
@@ -1293,6 +1347,7 @@ extern "C" int mps_run(mps_handle h, int64_t max_steps, mps_report* rep) {
     }
     HCHK(h, hipEventElapsedTime(&total_ms, h->ev0, h->ev1));
     r.device_ms = total_ms;
+    h->part_valid = true;  // the partial forces are those of the last completed step's u
     if (rep) *rep = r;
     return status_code(r.status);
 }
@@ -1340,9 +1395,12 @@ extern "C" int mps_internal_force(mps_handle h, const double* u, double* fint) {
     (void)hipSetDevice(h->device);
     const size_t bytes = 3 * (size_t)h->I.N * 8;
     HCHK(h, hipMemcpyAsync(h->scratch_u, u, bytes, hipMemcpyHostToDevice, h->stream));
+    // its own partial forces, so that the step's stay valid for mps_get_internal_force_dofs
+    if (!h->fint_part) HCHK(h, dalloc(h, &h->fint_part, std::max<size_t>((size_t)h->I.n_part * 3 * 8, 256)));
     MpsInst I = h->I;
     I.u = h->scratch_u;
     I.ctl = nullptr;
+    I.part = h->fint_part;
     hipError_t e = mpsk::launch_elem(I, 1, h->stream);
     if (e == hipSuccess) e = mpsk::launch_fint(I, h->scratch_f, h->stream);
     if (e != hipSuccess) return h->fail(MPS_ERR_HIP, fmt("internal force: %s", hipGetErrorString(e)));

@@ -42,3 +42,9 @@ int mps_batch_check(mps_batch b, const mpsh::HostPrep& hp, std::string* why);
 size_t mps_batch_footprint(mps_batch b, const mpsh::HostPrep& hp);
 void mps_batch_room(mps_batch b, int* free_slots, size_t* largest);
 int mps_batch_add_prepared(mps_batch b, const mpsh::HostPrep& hp, int32_t* slot_out);
+// Launch bookkeeping and tuning used by the server's online tuner (mps_batch.hip).
+double mps_batch_launch_work(mps_batch b, int64_t launches);
+int64_t mps_batch_active_elements(mps_batch b);
+double mps_batch_last_launch_ms(mps_batch b);
+void mps_batch_tuning(mps_batch b, int* elem_block, int* grid_mult);
+int mps_batch_set_tuning(mps_batch b, int elem_block, int grid_mult);

@@ -152,6 +152,8 @@ struct mps_batch_s {
     int steps_per_launch = 32;
     hipEvent_t last_launch = nullptr;
     bool any_launch = false;
+    hipEvent_t tl0 = nullptr, tl1 = nullptr;  // around the graph launches of the last enqueue
+    bool timed = false;
     std::string err;
     int fail(int code, const std::string& m) {
         err = m;
@@ -180,6 +182,8 @@ static void batch_release(mps_batch_s* b) {
     if (b->ctl) (void)hipFree(b->ctl);
     if (b->insts) (void)hipFree(b->insts);
     if (b->last_launch) (void)hipEventDestroy(b->last_launch);
+    if (b->tl0) (void)hipEventDestroy(b->tl0);
+    if (b->tl1) (void)hipEventDestroy(b->tl1);
     if (b->up) (void)hipStreamDestroy(b->up);
     if (b->stream) (void)hipStreamDestroy(b->stream);
 }
@@ -210,6 +214,7 @@ extern "C" int mps_batch_create(const mps_batch_cfg* cfg, mps_batch* out) {
         if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess ||
             hipStreamCreateWithFlags(&b->up, hipStreamNonBlocking) != hipSuccess ||
             hipEventCreateWithFlags(&b->last_launch, hipEventDisableTiming) != hipSuccess ||
+            hipEventCreate(&b->tl0) != hipSuccess || hipEventCreate(&b->tl1) != hipSuccess ||
             hipMalloc(&b->arena, (size_t)b->cfg.arena_bytes) != hipSuccess ||
             hipMalloc(&b->ctl, sizeof(MpsCtl) * M) != hipSuccess ||
             hipMalloc(&b->insts, sizeof(MpsInst) * M) != hipSuccess) {
@@ -497,7 +502,10 @@ static int enqueue(mps_batch_s* b, int64_t launches) {
         rc = capture(b, b->cur);
         if (rc) return rc;
     }
+    BCHK(b, hipEventRecord(b->tl0, b->stream));
     for (int64_t i = 0; i < launches; i++) BCHK(b, hipGraphLaunch(T.exec, b->stream));
+    BCHK(b, hipEventRecord(b->tl1, b->stream));
+    b->timed = launches > 0;
     BCHK(b, hipEventRecord(T.last_use, b->stream));
     BCHK(b, hipEventRecord(b->last_launch, b->stream));
     b->any_launch = true;
@@ -585,6 +593,65 @@ extern "C" int mps_batch_info(mps_batch b, int64_t* free_bytes, int64_t* largest
         for (auto& s : b->slots) n += s.used;
         *active = n;
     }
+    return MPS_OK;
+}
+
+// Element-steps the next `launches` graph launches will perform: every active solve advances by
+// steps_per_launch steps per launch until its own step count (progress from the last sync; a solve admitted
+// since then starts at step 1).
+double mps_batch_launch_work(mps_batch b, int64_t launches) {
+    double w = 0.0;
+    const int64_t S = (int64_t)b->steps_per_launch * launches;
+    for (size_t s = 0; s < b->slots.size(); s++) {
+        const Slot& sl = b->slots[s];
+        if (!sl.used) continue;
+        const MpsCtl& c = b->ctl_h[s];
+        if (c.status != MPS_RUNNING || c.nan_step != LLONG_MAX) continue;
+        const int64_t left = sl.n_steps - (c.A - 1);
+        if (left > 0) w += (double)sl.inst.Ne * (double)std::min(left, S);
+    }
+    return w;
+}
+
+// Active elements (the mix size the serving tuner keys on).
+int64_t mps_batch_active_elements(mps_batch b) {
+    int64_t n = 0;
+    for (auto& sl : b->slots)
+        if (sl.used) n += sl.inst.Ne;
+    return n;
+}
+
+// Device milliseconds of the graph launches of the last enqueue (valid after a sync); < 0 if none.
+double mps_batch_last_launch_ms(mps_batch b) {
+    if (!b->timed) return -1.0;
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, b->tl0, b->tl1) != hipSuccess) return -1.0;
+    return ms;
+}
+
+void mps_batch_tuning(mps_batch b, int* elem_block, int* grid_mult) {
+    *elem_block = b->bs;
+    *grid_mult = b->grid_mult;
+}
+
+// Element block size and resident-block multiple for the following launches.  Results do not depend on
+// them (every sum has a fixed order), so they may change between launches of a running batch.  Call only
+// when no launch is in flight (after a sync): the captured graphs are released and re-captured.
+int mps_batch_set_tuning(mps_batch b, int elem_block, int grid_mult) {
+    if (elem_block != 64 && elem_block != 128 && elem_block != 256) return MPS_ERR_ARG;
+    if (grid_mult < 1 || grid_mult > 64) return MPS_ERR_ARG;
+    if (elem_block == b->bs && grid_mult == b->grid_mult) return MPS_OK;
+    (void)hipSetDevice(b->device);
+    int rc = wait_launches(b);
+    if (rc) return rc;
+    b->bs = elem_block;
+    b->grid_mult = grid_mult;
+    b->dirty = true;
+    for (auto& t : b->tab)
+        if (t.exec) {
+            (void)hipGraphExecDestroy(t.exec);
+            t.exec = nullptr;
+        }
     return MPS_OK;
 }
 

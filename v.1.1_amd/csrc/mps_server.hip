@@ -8,6 +8,10 @@
 //      block of the request's footprint are free (uploads go on a separate stream; the set-up
 //      kernels queue behind the running launch, so an admitted solve joins the next launch);
 //   3. wait for the launch, retire finished solves (copy u out, free slot and block).
+// The element block size and resident-block multiple come from an online tuner (when the configuration
+// leaves them 0): per mix size (active elements, in half-decade buckets) it times one real launch with each
+// candidate, normalised by the element-steps the launch performed, and keeps the fastest for that bucket.
+// Results do not depend on the choice, so the trials are production launches and nothing is replayed.
 // State stays on the device for the whole solve; a failed solve (non-finite u) is retired with
 // its error code and does not affect the others.
 #include <hip/hip_runtime.h>
@@ -15,6 +19,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <map>
 #include <condition_variable>
 #include <deque>
 #include <memory>
@@ -57,6 +62,25 @@ double percentile(std::vector<double> v, double q) {
     return i + 1 < v.size() ? v[i] * (1 - f) + v[i + 1] * f : v[i];
 }
 
+// Online tuner of (element block size, grid multiple) per mix size.
+struct Tuner {
+    bool on = false;
+    std::vector<std::pair<int, int>> cand;
+    std::map<int, std::pair<int, int>> best;  // bucket -> choice
+    int explore = -1;                         // bucket under exploration (-1: none)
+    size_t next = 0;                          // candidate of the next timed launch
+    std::vector<double> cost;                 // ms per element-step of each candidate
+    int tunings = 0;
+    Tuner() {
+        for (int bs : {64, 128, 256})
+            for (int gm : {1, 2, 4, 8, 16})
+                if (bs * gm <= 4096) cand.emplace_back(bs, gm);
+    }
+    static int bucket(int64_t elements) {
+        return elements > 0 ? (int)std::floor(2.0 * std::log10((double)elements)) : 0;
+    }
+};
+
 }  // namespace
 
 struct mps_server_s {
@@ -76,8 +100,46 @@ struct mps_server_s {
     Clock::time_point t_first, t_last;
     bool have_first = false;
     double busy_ms = 0.0;
+    Tuner tuner;
+    int cur_bs = 0, cur_gm = 0;  // tuning of the latest launch (under mu)
     std::string err;
 };
+
+// Before a launch: the tuning for the current mix -- the bucket's choice, or the next candidate while the
+// bucket is explored.  Returns the bucket the launch belongs to.
+static int tune_before_launch(mps_server_s* S) {
+    Tuner& T = S->tuner;
+    const int b = Tuner::bucket(mps_batch_active_elements(S->batch));
+    if (!T.on) return b;
+    auto it = T.best.find(b);
+    if (it != T.best.end()) {
+        T.explore = -1;
+        (void)mps_batch_set_tuning(S->batch, it->second.first, it->second.second);
+        return b;
+    }
+    if (T.explore != b) {  // a new mix size: time every candidate on the next launches
+        T.explore = b;
+        T.next = 0;
+        T.cost.assign(T.cand.size(), 0.0);
+    }
+    (void)mps_batch_set_tuning(S->batch, T.cand[T.next].first, T.cand[T.next].second);
+    return b;
+}
+
+// After a launch of `work` element-steps that took `ms` on the device, in bucket b.
+static void tune_after_launch(mps_server_s* S, int b, double work, double ms) {
+    Tuner& T = S->tuner;
+    if (!T.on || T.explore != b || work <= 0.0 || ms <= 0.0) return;
+    T.cost[T.next] = ms / work;
+    if (++T.next < T.cand.size()) return;
+    size_t k = 0;
+    for (size_t i = 1; i < T.cost.size(); i++)
+        if (T.cost[i] < T.cost[k]) k = i;
+    T.best[b] = T.cand[k];
+    T.explore = -1;
+    std::lock_guard<std::mutex> lk(S->mu);
+    T.tunings++;
+}
 
 static void retire(mps_server_s* S) {
     std::vector<std::shared_ptr<Request>> keep;
@@ -168,6 +230,15 @@ static void worker_loop(mps_server_s* S) {
             continue;
         }
         const auto t0 = Clock::now();
+        const int bucket = tune_before_launch(S);
+        const double work = mps_batch_launch_work(S->batch, 1);
+        {
+            int bs = 0, gm = 0;
+            mps_batch_tuning(S->batch, &bs, &gm);
+            std::lock_guard<std::mutex> lk(S->mu);
+            S->cur_bs = bs;
+            S->cur_gm = gm;
+        }
         if (mps_batch_launch(S->batch, 1) != MPS_OK) {
             // the running solves fail: their slots and arena blocks are released once nothing queued
             // can still read them, and each counts as failed so that drain's predicate can hold
@@ -192,6 +263,7 @@ static void worker_loop(mps_server_s* S) {
         admit(S);  // overlaps with the launch in flight
         (void)mps_batch_sync(S->batch);
         S->busy_ms += ms_between(t0, Clock::now());
+        tune_after_launch(S, bucket, work, mps_batch_last_launch_ms(S->batch));
         retire(S);
     }
 }
@@ -201,6 +273,7 @@ extern "C" int mps_server_create(const mps_server_cfg* cfg, mps_server* out) {
     *out = nullptr;
     auto* S = new mps_server_s();
     S->cfg = *cfg;
+    S->tuner.on = cfg->batch.elem_block == 0 && cfg->batch.grid_mult == 0;
     int rc = mps_batch_create(&cfg->batch, &S->batch);
     if (rc != MPS_OK) {
         delete S;
@@ -300,6 +373,10 @@ extern "C" int mps_server_get_stats(mps_server S, mps_server_stats* out) {
     out->span_ms = span;
     out->solves_per_s = span > 0 ? 1e3 * (double)S->completed / span : 0.0;
     out->busy_ms = S->busy_ms;
+    out->elem_block = S->cur_bs;
+    out->grid_mult = S->cur_gm;
+    out->tunings = S->tuner.tunings;
+    out->_pad = 0;
     return MPS_OK;
 }
 

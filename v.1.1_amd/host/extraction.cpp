@@ -131,4 +131,27 @@ void classExtractData::finalise() {
     for (auto &kv : _element) kv.second->closeFile();
 }
 
+bool classPrintForces::due(double time, double dt, double t0) const {
+    double period = _f.period;
+    if (period < dt) period = dt;
+    return time >= _iterationsCompleted * period + t0;
+}
+
+bool classPrintForces::write(const std::string &dir, double t0, double tf, double time,
+                             const std::vector<double> &forces) {
+    const std::string name = dir + "/externalForces" + std::to_string(t0) + "-" + std::to_string(tf) + ".csv";
+    FILE *f = std::fopen(name.c_str(), _firstTime ? "w" : "a");
+    if (!f) return false;
+    if (_firstTime) std::fputs("Node,Direction,Force,Time,\n", f);
+    for (size_t i = 0; i < _f.nodes.size(); i++) {
+        const std::string row = std::to_string(_f.nodes[i] + 1) + "," + std::to_string(_f.dirs[i]) + "," +
+                                std::to_string(forces[i]) + "," + std::to_string(time) + ",\n";
+        std::fputs(row.c_str(), f);
+    }
+    std::fclose(f);
+    _firstTime = false;
+    _iterationsCompleted++;
+    return true;
+}
+
 } // namespace mps

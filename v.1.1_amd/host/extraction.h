@@ -67,6 +67,32 @@ class classElementData : public classDataBase {
     std::string _dataType;
 };
 
+// *Forces output of one solver (classPrintForces, output.h:51-101; printForcesOut / writeExternalForces,
+// output.cpp:106-135, 153-279): at every step whose time reaches iterations*max(period, dt) + t0 it appends
+// one row "<node+1>,<direction>,<force>,<time>," per listed pair (std::to_string formatting) to
+// <outputDir>/externalForces<t0>-<tf>.csv, created with the header "Node,Direction,Force,Time," at the first
+// write.  The force is the step's internal force (intForce; the explicit solver's inertial term is zero,
+// solverExplicit.cpp:606-613).
+class classPrintForces {
+  public:
+    classPrintForces(const InpForces &f) : _f(f) {}
+    const std::vector<int32_t> &getNodesToWrite() const { return _f.nodes; }
+    const std::vector<int32_t> &getDirectionsToWrite() const { return _f.dirs; }
+    double getPeriod() const { return _f.period; }
+    int getIterationsCompleted() const { return _iterationsCompleted; }
+    // Time before which no step can write: iterations*period + t0 (the period used is at least `period`).
+    double nextThreshold(double t0) const { return _iterationsCompleted * _f.period + t0; }
+    // Whether the step at `time` with step `dt` writes (the test of printForcesOut).
+    bool due(double time, double dt, double t0) const;
+    // Appends the rows of one step: forces[i] is the internal force of pair i.
+    bool write(const std::string &dir, double t0, double tf, double time, const std::vector<double> &forces);
+
+  private:
+    InpForces _f;
+    int _iterationsCompleted = 0;
+    bool _firstTime = true;
+};
+
 class classExtractData {
   public:
     // Validates the extractions against what the device path provides; false + err otherwise.

@@ -337,7 +337,8 @@ bool readSolvers(Reader &R, InpModel &m) {
                     std::vector<std::string> f = fields(t);
                     long long x;
                     if (starts(t, "All")) {
-                        if (!toInt(tail(t, 4), x)) return R.fail(i + 2, "*Outputs: \"All, n\" needs n");
+                        // atoi of the text from column 4: a word there ("All,max") asks for no variable
+                        if (!toInt(tail(t, 4), x)) x = 0;
                         for (long long k = 0; k < x; k++) outVars.push_back((int)k);
                     } else if (f.size() == 1) {
                         if (!toInt(t, x)) return R.fail(i + 2, "*Outputs: malformed internal variable");
@@ -392,9 +393,33 @@ bool readSolvers(Reader &R, InpModel &m) {
                         }
                     }
                 }
+            } else if (starts(l, "*Forces")) {
+                // forceOutManagement (inputFile.cpp:4830-4887): the output frequency (period 1/f), then
+                // "node, direction" lines up to the next keyword; a new *Forces replaces the block's earlier one
+                // (inputFile.cpp:2537-2539).  The direction is the field after the first comma; printForcesOut
+                // uses it as the 0-based direction of Forces[node + direction*N] and rejects one >= ndim
+                // (output.cpp:119-126).  (The reference takes it up to a second comma it assumes: with one comma
+                // per line, as in its own decks, it reads past its list of comma positions.)
+                if (!nextValue(R, i, v) || !(v > 0)) return R.fail(i, "*Forces needs a positive output frequency");
+                InpForces F;
+                F.on = true;
+                F.period = 1 / v;
+                for (i += 2; i < R.L.size() && !starts(R.L[i], "*"); i++) {
+                    if (isBlank(R.L[i])) continue;
+                    std::vector<std::string> f = fields(R.L[i]);
+                    long long node, dir;
+                    if (f.size() < 2 || !toInt(f[0], node) || !toInt(f[1], dir))
+                        return R.fail(i, "*Forces needs \"node, direction\" lines");
+                    if (node < 1 || node > m.nNodes) return R.fail(i, "*Forces: node outside the mesh");
+                    if (dir < 0 || dir >= m.ndim)
+                        return R.fail(i, "*Forces: direction of force requested exceeds the simulation dimension");
+                    F.nodes.push_back((int32_t)(node - 1));
+                    F.dirs.push_back((int32_t)dir);
+                }
+                i--;
+                s.forces = F;
             } else if (starts(l, "*ACTIVATION") ||
-                       starts(l, "*FSI") || starts(l, "*ADAPTIVE TIME STEP") || starts(l, "*TIME STEPPING PLAN") ||
-                       starts(l, "*Forces")) {
+                       starts(l, "*FSI") || starts(l, "*ADAPTIVE TIME STEP") || starts(l, "*TIME STEPPING PLAN")) {
                 return R.fail(i, "solver option not on this path");
             } else if (starts(l, "*")) {
                 m.warnings.push_back("solver option ignored by the EXPLICIT solver (" + R.where(i) + ")");

@@ -14,6 +14,7 @@
 //   *BOUNDARY, TYPE=PRESSURE RAMP|INST  pressure*BCManagement inputFile.cpp:3580-3790  in order; global
 //   *GRAVITY                    NeumannCommonManagement inputFile.cpp:4703-4735   for all solvers)
 //   *EXTRACTION, NODE|ELEMENT   dataExtractionManagement inputFile.cpp:2229-2243, 4950-5120
+//   *Forces                     forceOutManagement     inputFile.cpp:2537-2539, 4830-4887 (per solver)
 // Keywords are matched as prefixes, case-sensitively, as the reference does.
 // Where the reference would read garbage, dereference null or silently
 // mis-number (duplicate Dirichlet DOFs, *BOUNDARY before *Time, a missing
@@ -48,6 +49,15 @@ struct InpDirichlet {
     double value; // U (ramp/inst) or V (velocity)
 };
 
+// *Forces of a solver block: output frequency, then the (node, direction) pairs whose internal force the
+// solver writes to externalForces<t0>-<tf>.csv (printForcesOut, output.cpp:153-279).
+struct InpForces {
+    bool on = false;
+    double period = 0;            // 1 / frequency
+    std::vector<int32_t> nodes;   // 0-based
+    std::vector<int32_t> dirs;    // 0..2
+};
+
 struct InpSolver {
     std::string type = "EXPLICIT";
     double scale = 1.0;
@@ -63,6 +73,7 @@ struct InpSolver {
     int visc = 0;                  // *Bulk viscosity (MPS_VISC_BULK) / *Dev viscosity (MPS_VISC_DEV)
     double viscC1 = 0, viscCl = 0; // their "c1, cl" line (defaults 0.06, 1.2 / 1, 0.5)
     std::vector<InpDirichlet> dirichlet;
+    InpForces forces;
 };
 
 struct InpNodeExtraction {

@@ -70,7 +70,16 @@ static int check(const std::string &path) {
                     k ? ", " : "", s.scale, s.t0, s.tf, s.numFiles, s.bcMode, s.dirichlet.size(),
                     (long long)s.fixedSteps, s.velRampSteps, s.precision, s.visc, s.viscC1, s.viscCl);
         for (size_t j = 0; j < s.outVars.size(); j++) std::printf("%s%d", j ? ", " : "", s.outVars[j]);
-        std::printf("]}");
+        std::printf("], \"forces\": ");
+        if (!s.forces.on) {
+            std::printf("null");
+        } else {
+            std::printf("{\"period\": %.17g, \"entries\": [", s.forces.period);
+            for (size_t j = 0; j < s.forces.nodes.size(); j++)
+                std::printf("%s[%d, %d]", j ? ", " : "", s.forces.nodes[j], s.forces.dirs[j]);
+            std::printf("]}");
+        }
+        std::printf("}");
     }
     std::printf("], \"node_extractions\": [");
     bool first = true;

@@ -141,19 +141,33 @@ int classSolverExplicit::calculate(const InpModel &m, const std::vector<std::uni
     log.info("The initial time step is %g", dt0);
     log.info("done preprocessing in solver (Wall %gs, CPU %gs)", wallNow() - startWall, cpuNow() - startCpu);
 
-    // Nodal extraction DOFs, gathered on the device every step.
-    std::vector<int32_t> pickDof;
-    for (auto &kv : extract.nodeData())
-        if (kv.second->getDataType() == "Unknown")
-            for (int32_t n : kv.second->getNodes()) pickDof.push_back(n + N * kv.second->getComp());
+    // Nodal extraction DOFs, gathered on the device every step: displacements, and internal forces from the
+    // step's partial nodal forces.
+    std::vector<int32_t> pickDof, fintDof;
+    for (auto &kv : extract.nodeData()) {
+        std::vector<int32_t> &dst = kv.second->getDataType() == "Unknown" ? pickDof : fintDof;
+        if (kv.second->getDataType() == "Unknown" || kv.second->getDataType() == "InternalForce")
+            for (int32_t n : kv.second->getNodes()) dst.push_back(n + N * kv.second->getComp());
+    }
     const bool nodal = extract.needsNodal();
-    const bool needF = extract.needsInternalForce();
+    // *Forces of this solver (printForcesOut, solverExplicit.cpp:304): the step's internal force at the listed
+    // (node, direction) pairs whenever the output is due.  No step can be due before the time
+    // iterations*period + t0, and the time advances by at most dtMax per step (every step's dt is
+    // s*min_e min(l0, l)/c <= s*min_e l0/c), so the steps before that time run in chunks.
+    std::unique_ptr<classPrintForces> forcesOut;
+    std::vector<int32_t> forceDof;
+    std::vector<double> forceVal;
+    if (_s.forces.on) {
+        forcesOut.reset(new classPrintForces(_s.forces));
+        for (size_t j = 0; j < _s.forces.nodes.size(); j++) forceDof.push_back(_s.forces.nodes[j] + N * _s.forces.dirs[j]);
+        forceVal.resize(forceDof.size());
+    }
+    const double dtMax = dt0;
     // external force and EXTERNAL_ENERGY = sum_i u_i fext_i (dataExtraction.cpp:866-874): zero without loads
     const bool needFext = !ld.empty() && extract.needsExternalForce();
     const bool extEnergy = !ld.empty() && extract.needsExternalEnergy();
-    std::vector<double> picked(pickDof.size()), uFull, fint, fext;
-    if (needF || needFext || extEnergy) uFull.resize(3 * (size_t)N);
-    if (needF) fint.resize(3 * (size_t)N);
+    std::vector<double> picked(pickDof.size()), fintPicked(fintDof.size()), uFull, fext;
+    if (needFext || extEnergy) uFull.resize(3 * (size_t)N);
     if (needFext || extEnergy) fext.resize(3 * (size_t)N);
     std::vector<double> ht(kHistoryCap), hdef(kHistoryCap), hkin(kHistoryCap), hvol(kHistoryCap);
 
@@ -177,7 +191,13 @@ int classSolverExplicit::calculate(const InpModel &m, const std::vector<std::uni
     mps_report rep{};
     for (;;) {
         const long long before = rep.steps;
-        const int rc = mps_run(h, std::min(chunk, std::max(1LL, sampler - sampler2)), &rep);
+        long long n = std::min(chunk, std::max(1LL, sampler - sampler2));
+        if (forcesOut) {
+            const double now = rep.steps > 0 ? rep.time : _s.t0;
+            const double room = (forcesOut->nextThreshold(_s.t0) - now) / dtMax;
+            n = std::min(n, room > 3.0 ? (long long)room - 2 : 1LL);
+        }
+        const int rc = mps_run(h, n, &rep);
         if (rc == MPS_ERR_NAN || rep.status == MPS_ERR_NAN) {
             err = "There are NANs in the displacements. Try a smaller scale factor";
             return -1;
@@ -188,6 +208,14 @@ int classSolverExplicit::calculate(const InpModel &m, const std::vector<std::uni
         }
         if (rc != MPS_OK) return fail("mps_run");
         const long long done = rep.steps - before;
+        if (done > 0 && forcesOut && forcesOut->due(rep.time, rep.dt, _s.t0)) {
+            if (mps_get_internal_force_dofs(h, (int64_t)forceDof.size(), forceDof.data(), forceVal.data()) != MPS_OK)
+                return fail("mps_get_internal_force_dofs");
+            if (!forcesOut->write(opt.outputDir, _s.t0, _s.tf, rep.time, forceVal)) {
+                err = "cannot write the *Forces output in " + opt.outputDir;
+                return -1;
+            }
+        }
         double eext = 0.0;
         if (done > 0 && (needFext || extEnergy)) {
             if (mps_get_state(h, uFull.data(), nullptr, nullptr) != MPS_OK) return fail("mps_get_state");
@@ -214,11 +242,10 @@ int classSolverExplicit::calculate(const InpModel &m, const std::vector<std::uni
             if (!pickDof.empty() &&
                 mps_get_dofs(h, 0, (int64_t)pickDof.size(), pickDof.data(), picked.data()) != MPS_OK)
                 return fail("mps_get_dofs");
-            if (needF) {
-                if (mps_get_state(h, uFull.data(), nullptr, nullptr) != MPS_OK) return fail("mps_get_state");
-                if (mps_internal_force(h, uFull.data(), fint.data()) != MPS_OK) return fail("mps_internal_force");
-            }
-            size_t off = 0;
+            if (!fintDof.empty() && mps_get_internal_force_dofs(h, (int64_t)fintDof.size(), fintDof.data(),
+                                                                 fintPicked.data()) != MPS_OK)
+                return fail("mps_get_internal_force_dofs");
+            size_t off = 0, offF = 0;
             for (auto &kv : extract.nodeData()) {
                 classNodeData &d = *kv.second;
                 std::vector<double> vals(d.getNodes().size(), 0.0);
@@ -227,11 +254,12 @@ int classSolverExplicit::calculate(const InpModel &m, const std::vector<std::uni
                     if (d.getDataType() == "Unknown")
                         vals[j] = picked[off + j];
                     else if (d.getDataType() == "InternalForce")
-                        vals[j] = fint[g];
+                        vals[j] = fintPicked[offF + j];
                     else if (d.getDataType() == "ExternalForce" && needFext)
                         vals[j] = fext[g];
                 }
                 if (d.getDataType() == "Unknown") off += vals.size();
+                if (d.getDataType() == "InternalForce") offF += vals.size();
                 if (!d.writeData(opt.outputDir, rep.time, vals)) {
                     err = "cannot write " + kv.first + ".csv in " + opt.outputDir;
                     return -1;

@@ -96,7 +96,8 @@ class ServerCfg(C.Structure):
 class ServerStats(C.Structure):
     _fields_ = [("submitted", C.c_int64), ("completed", C.c_int64), ("failed", C.c_int64), ("launches", C.c_int64),
                 ("p50_ms", C.c_double), ("p99_ms", C.c_double), ("max_ms", C.c_double), ("queue_p50_ms", C.c_double),
-                ("span_ms", C.c_double), ("solves_per_s", C.c_double), ("busy_ms", C.c_double)]
+                ("span_ms", C.c_double), ("solves_per_s", C.c_double), ("busy_ms", C.c_double),
+                ("elem_block", C.c_int32), ("grid_mult", C.c_int32), ("tunings", C.c_int32), ("_pad", C.c_int32)]
 
 
 REQ_QUEUED, REQ_RUNNING, REQ_DONE, REQ_FAILED = 0, 1, 2, 3
@@ -136,6 +137,7 @@ def load(path=None):
     L.mps_external_force.argtypes = [H, C.c_double, _dp]
     L.mps_get_dofs.argtypes = [H, C.c_int32, C.c_int64, _ip, _dp]
     L.mps_get_dofs_static.argtypes = [H, C.c_int32, C.c_int64, _ip, _dp]
+    L.mps_get_internal_force_dofs.argtypes = [H, C.c_int64, _ip, _dp]
     L.mps_destroy.argtypes = [H]
     L.mps_last_error.restype = C.c_char_p
     L.mps_last_error.argtypes = [H]
@@ -307,6 +309,13 @@ class Solver:
         dof = np.ascontiguousarray(dof, dtype=np.int32)
         out = np.zeros(dof.size)
         self._chk(self.L.mps_get_dofs(self.h, which, dof.size, _i(dof), _d(out)))
+        return out
+
+    def internal_force_dofs(self, dof):
+        """Internal force of the current state at DOF indices (mps_get_internal_force_dofs)."""
+        dof = np.ascontiguousarray(dof, dtype=np.int32)
+        out = np.zeros(dof.size)
+        self._chk(self.L.mps_get_internal_force_dofs(self.h, dof.size, _i(dof), _d(out)))
         return out
 
     def get_dofs_static(self, which, dof):

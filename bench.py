@@ -97,16 +97,18 @@ def step_bytes(ne, nn, nen, n_part, per_elem_mat):
     return elem_kernel_bytes(ne, nn, nen, n_part, per_elem_mat) + node_kernel_bytes(nn, n_part)
 
 
-def run_gpu(cfg, precision, warmup, steps, prof_steps=None):
+def run_gpu(cfg, precision, warmup, steps, prof_steps=None, energy_every=0):
     """Warm-up, then `steps` timed steps (graph replay, no per-kernel events), then `prof_steps` (default
     `steps`) more steps launched one by one with HIP events around every kernel on the solver's stream:
     the per-kernel averages and the event-timed step rate come from that second region.  The
     displacement returned is the one after warm-up + timed steps."""
     prof_steps = steps if prof_steps is None else prof_steps
     m = cfg["mesh"]
+    total = warmup + steps + prof_steps
     s = G.Solver(m, law=cfg["law"], E=cfg["E"], nu=cfg["nu"], rho=cfg["rho"], dir_dof=cfg["dir_dof"],
                  dir_val=cfg["dir_val"], bc_mode=G.BC_VELOCITY, vel_ramp_steps=100, scale=cfg["scale"],
-                 n_steps=warmup + steps + prof_steps, precision=precision)
+                 n_steps=total, precision=precision, energy_every=energy_every,
+                 history_cap=total if energy_every else 0)
     rc, rep = s.run(warmup)
     assert rc == 0, rep
     s.prepare_run(steps)  # graph capture and instantiation of the timed chunk stay outside the clock
@@ -115,6 +117,10 @@ def run_gpu(cfg, precision, warmup, steps, prof_steps=None):
     t1 = time.perf_counter()
     assert rc == 0 and rep["steps"] == warmup + steps, (rc, rep)
     u, v, a = s.get_state()
+    hist = None
+    if energy_every:
+        hist = s.history(0, warmup + steps)
+        hist.update(s.energy_balance(0, warmup + steps))
     s.profile(True)
     t2 = time.perf_counter()
     rc, rep2 = s.run(prof_steps)
@@ -122,8 +128,27 @@ def run_gpu(cfg, precision, warmup, steps, prof_steps=None):
     assert rc == 0, rep2
     kt = s.kernel_times()
     s.close()
-    return dict(wall_s=t1 - t0, device_ms=rep["device_ms"], kt=kt, u=u, rep=rep,
+    return dict(wall_s=t1 - t0, device_ms=rep["device_ms"], kt=kt, u=u, rep=rep, hist=hist,
                 events_ms_per_step=(t3 - t2) * 1e3 / prof_steps)
+
+
+def energy_leg(cfg, warmup, steps, plain_ms):
+    """The FP64 leg with DEFO_ENERGY, KIN_ENERGY, VOLUME, EXTERNAL_ENERGY and the work sums every step: cost
+    against the plain step, and the energy balance (lumped kinetic + deformation energy - work of the loads and
+    supports, from the end of step 1) at the last step."""
+    r = run_gpu(cfg, G.PREC_FP64, warmup, steps, energy_every=1)
+    h = r["hist"]
+    E = h["e_def"] + h["e_kin_lumped"]
+    W = h["w_ext"] + h["w_react"]
+    B = (E - E[0]) - W
+    scale = max(abs(W[-1]), abs(h["e_def"][-1] - h["e_def"][0]) + h["e_kin_lumped"][-1])
+    ms = r["wall_s"] * 1e3 / steps
+    ke = r["kt"]["element"]
+    return {"ms_per_step": ms, "overhead_vs_plain": ms / plain_ms - 1.0,
+            "element_avg_ms": ke[0] / max(ke[1], 1),
+            "energy_pass_avg_ms": r["kt"]["energy"][0] / max(r["kt"]["energy"][1], 1),
+            "balance_residual_rel": float(abs(B[-1]) / scale), "work_J": float(W[-1]),
+            "e_def_J": float(h["e_def"][-1]), "e_kin_J": float(h["e_kin"][-1])}
 
 
 def tet_leg(warmup, steps):
@@ -143,7 +168,9 @@ def tet_leg(warmup, steps):
             "fp32_gp_per_s": ne * steps / r32["wall_s"], "fp32_ms_per_step": r32["wall_s"] * 1e3 / steps,
             "fp32_speedup": r64["wall_s"] / r32["wall_s"],
             "max_rel_disp_err_fp32_vs_fp64": float(np.abs(r32["u"] - r64["u"]).max() / np.abs(r64["u"]).max()),
-            "fp64_element_avg_ms": e64, "fp64_node_avg_ms": n64, "partials": n_part,
+            "fp64_element_avg_ms": e64, "fp64_node_avg_ms": n64,
+            "fp32_element_avg_ms": r32["kt"]["element"][0] / max(r32["kt"]["element"][1], 1),
+            "fp32_node_avg_ms": r32["kt"]["node"][0] / max(r32["kt"]["node"][1], 1), "partials": n_part,
             "step_algorithmic_bytes": step_bytes(ne, m["n_nodes"], 4, n_part, True)}
 
 
@@ -297,6 +324,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-fp32", action="store_true")
+    ap.add_argument("--no-energy", action="store_true", help="skip the energies-every-step leg")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched (512 solves) leg")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving (Poisson arrivals) leg")
     ap.add_argument("--no-tet", action="store_true", help="skip the 3.07M-tet4 leg")
@@ -383,6 +411,11 @@ def main():
                             "speedup_vs_fp64": v32 / value, "max_rel_disp_err_vs_fp64": err,
                             "element_avg_ms": e32 / max(n32, 1),
                             "node_avg_ms": r32["kt"]["node"][0] / max(r32["kt"]["node"][1], 1)}
+    if not args.no_energy:
+        note("FP64 leg with energies every step")
+        out["energies_every_step"] = energy_leg(cfg, args.warmup, args.steps, r64["wall_s"] * 1e3 / args.steps)
+        note(f"energies every step {out['energies_every_step']['ms_per_step']:.4f} ms/step "
+             f"(+{100 * out['energies_every_step']['overhead_vs_plain']:.1f}%)")
     if not args.no_tet:
         note("tet4 leg (80^3 x 6 tets, per-element materials)")
         out["tet4"] = tet_leg(args.warmup, 500)

@@ -211,11 +211,27 @@ int mps_run(mps_handle h, int64_t max_steps, mps_report *rep);
 
 /* Per-step history from the device ring: entries [first, first+count) with
  * step index = entry + 1.  Energies are NaN on steps where they were not
- * computed.  Any output pointer may be NULL.
+ * computed.  Any output pointer may be NULL.  Energies cost one FP64 pass over the elements on each energy step
+ * (in the element kernel itself for the FP64 form without viscosity) and keep graph replay and the fused
+ * update; the KIN_ENERGY of a step is computed from its velocity when the next step starts (or at the end of the
+ * run call), so it is complete once mps_run returns.
  * Replaces: classExtractData DEFO_ENERGY / KIN_ENERGY / VOLUME with the Sum
  * operation (dataExtraction.cpp:569-776). */
 int mps_get_history(mps_handle h, int64_t first, int64_t count, double *time, double *dt,
                     double *e_def, double *e_kin, double *volume);
+
+/* Energy balance of the same steps (solvers created with energy_every > 0; any pointer may be NULL), FP64:
+ *   e_ext        EXTERNAL_ENERGY = sum_i u_i fext_i of the step (dataExtraction.cpp:867-875), on energy steps;
+ *   e_kin_lumped 1/2 sum_i m_i v_i^2, the kinetic energy of the lumped-mass update, on energy steps;
+ *   w_ext        work of the external loads since the end of step 1, trapezoidal over each step's increment
+ *                du: sum 1/2 (fext_prev + fext) . du, every step (0 at step 1);
+ *   w_react      work of the supports likewise: sum over constrained DOFs of 1/2 (r_prev + r) . du with
+ *                r = fint - fext, plus the kinetic energy 1/2 m (v^2 - v_prev^2) the prescribed velocity gives the
+ *                DOF's lumped mass (the update sets a = 0 there), every step (0 at step 1).
+ * Balance: e_kin_lumped + DEFO_ENERGY - w_ext - w_react stays at its step-1 value up to the O(dt^2) error of
+ * the explicit update.  No reference counterpart (the reference writes EXTERNAL_ENERGY only). */
+int mps_get_energy_balance(mps_handle h, int64_t first, int64_t count, double *e_ext, double *e_kin_lumped,
+                           double *w_ext, double *w_react);
 
 /* Element fields of the current state for the VTK output: out[21*n_elem], per element the Gauss-point
  * averages of Cauchy stress (9, row-major), Green-Lagrange strain (9; the small strain for the linear

@@ -302,11 +302,14 @@ struct mps_solver_s {
     double* uprev_buf = nullptr;     // previous-step displacement for bulk viscosity (in allocs)
     double* fint_part = nullptr;     // partial forces of mps_internal_force (allocated on first use, in allocs)
     bool part_valid = false;         // I.part holds the forces of the current u (false: set-up scratch)
-    int energy_nb = 0;
     bool primed = false;       // prologue (first dt) done
     long long next_step = 1;   // host mirror of ctl->A
     bool step_begun = false;   // step next_step already predicted on the device
-    std::vector<std::pair<long long, hipGraphExec_t>> graphs;  // captured chunks by step count
+    struct Graph {
+        long long n, phase;  // steps, first step modulo energy_every (the energy launches of the chunk)
+        hipGraphExec_t exec;
+    };
+    std::vector<Graph> graphs;  // captured chunks
     bool use_graphs = true;
     // optional kernel timing (HIP events around each element / node launch)
     bool profiling = false;
@@ -344,7 +347,7 @@ static void release(mps_solver_s* h) {
     for (hipEvent_t e : h->evpool) (void)hipEventDestroy(e);
     h->evpool.clear();
     if (h->device >= 0) (void)hipSetDevice(h->device);
-    for (auto& g : h->graphs) (void)hipGraphExecDestroy(g.second);
+    for (auto& g : h->graphs) (void)hipGraphExecDestroy(g.exec);
     h->graphs.clear();
     if (h->stream) (void)hipStreamSynchronize(h->stream);
     for (void* p : h->allocs) (void)hipFree(p);
@@ -383,7 +386,7 @@ static int upload_instance(mps_solver_s* h, const HostPrep& hp) {
     I.rho0 = hp.rho0;
     I.dof1_node = (int32_t)(1 % N);
     I.dof1_dir = (int32_t)(1 / N);
-    double *X, *u, *v, *a, *mass, *l0, *matp = nullptr, *cel = nullptr, *rho = nullptr, *bcval, *hist, *part;
+    double *X, *u, *v, *a, *mass, *l0, *matp = nullptr, *cel = nullptr, *rho = nullptr, *bcval, *hist;
     double* pforce;
     int32_t *conn, *rowptr, *col, *poff, *nrow, *ncol;
     uint16_t *prow, *pcol;
@@ -413,9 +416,14 @@ static int upload_instance(mps_solver_s* h, const HostPrep& hp) {
     HCHK(h, dalloc(h, &ctl, sizeof(MpsCtl)));
     unsigned long long* dtred;
     HCHK(h, dalloc(h, &dtred, 2 * MPS_DT_SLOTS * MPS_DT_STRIDE * 8));
-    HCHK(h, dalloc(h, &hist, 5 * (size_t)cap * 8));
-    h->energy_nb = (int)((Ne + 255) / 256);
-    HCHK(h, dalloc(h, &part, 3 * (size_t)h->energy_nb * 8));
+    HCHK(h, dalloc(h, &hist, MPS_HIST_COLS * (size_t)cap * 8));
+    if (hp.cfg.energy_every > 0) {  // energy block sums (element phase, node kernel by parity), support forces
+        const size_t node_blocks = (N + 255) / 256;
+        HCHK(h, dalloc(h, &I.epart, (3 * NPA + 2 * MPS_NODE_PARTS * node_blocks) * 8));
+        HCHK(h, dalloc(h, &I.rprev, hp.bcval.size() * 8));
+        HCHK(h, hipMemsetAsync(I.rprev, 0, hp.bcval.size() * 8, h->stream));
+    }
+    I.energy_every = hp.cfg.energy_every > 0 ? hp.cfg.energy_every : 0;
     HCHK(h, dalloc(h, &h->scratch_u, 3 * N * 8));
     HCHK(h, dalloc(h, &h->scratch_f, 3 * N * 8));
     HCHK(h, dalloc(h, &h->scratch_dt, 64));
@@ -475,7 +483,7 @@ static int upload_instance(mps_solver_s* h, const HostPrep& hp) {
         }
     }
     HCHK(h, hipMemcpyAsync(bcval, hp.bcval.data(), hp.bcval.size() * 8, hipMemcpyHostToDevice, h->stream));
-    HCHK(h, hipMemsetAsync(hist, 0xFF, 5 * (size_t)cap * 8, h->stream));  // NaN
+    HCHK(h, hipMemsetAsync(hist, 0xFF, MPS_HIST_COLS * (size_t)cap * 8, h->stream));  // NaN
     I.X = X;
     I.conn = conn;
     I.l0 = l0;
@@ -501,7 +509,6 @@ static int upload_instance(mps_solver_s* h, const HostPrep& hp) {
     I.n_part = hp.n_part;
     I.ctl = ctl;
     I.hist = hist;
-    I.partials = part;
     I.hist_cap = cap;
     return MPS_OK;
 }
@@ -943,7 +950,7 @@ extern "C" int mps_set_loads(mps_handle h, int32_t n_loads, const mps_load* load
     I.n_sgp = 0;
     I.lidx = I.lnn = I.lrow = I.lcode = I.sge = I.sgb = nullptr;
     I.lm = I.lwin = I.fgrav = I.sgd = I.sblk = nullptr;
-    I.pc = I.lf = nullptr;
+    I.pc = I.lf = I.lfprev = nullptr;
     auto upload = [&](const void* src, size_t bytes, void** dst) -> hipError_t {
         hipError_t e = hipMalloc(dst, bytes < 256 ? 256 : bytes);
         if (e != hipSuccess) return e;
@@ -963,6 +970,11 @@ extern "C" int mps_set_loads(mps_handle h, int32_t n_loads, const mps_load* load
         HCHK(h, upload(nullptr, lnn.size() * 24, &p));
         I.lf = (double*)p;
         I.n_lnode = (int32_t)lnn.size();
+        if (I.energy_every > 0) {  // previous step's external force of the flagged nodes (work of the loads)
+            HCHK(h, upload(nullptr, lnn.size() * 24, &p));
+            HCHK(h, hipMemset(p, 0, lnn.size() * 24));
+            I.lfprev = (double*)p;
+        }
     }
     if (!lm.empty()) {
         HCHK(h, upload(lm.data(), lm.size() * 8, &p));
@@ -1165,11 +1177,16 @@ static void collect_times(mps_solver_s* h, const std::vector<int>& kinds) {
     h->ev_used = 0;
 }
 
-// Launch sequence of `count` steps starting at h->next_step; the first step is begun by
-// the standalone begin kernel unless it already is; the last node pass is unfused.
+// Launch sequence of `count` steps starting at h->next_step; the first step is begun by the standalone begin
+// kernel unless it already is; the last node pass is unfused.  With energies every `every` steps, step k
+// computes DEFO_ENERGY and VOLUME of u_k (k % every == 0) and KIN_ENERGY of step k-1 from the velocity that
+// still holds it before the node pass (when (k-1) % every == 0 and k-1 is in this chunk), in the element kernel
+// (FP64 inviscid form) or in the standalone pass before the node kernel; after the last node pass, the pass for
+// the KIN_ENERGY of the chunk's last step.  The node kernels accumulate the work sums every step.
 static hipError_t enqueue_steps(mps_solver_s* h, long long count, std::vector<int>* kinds) {
     const MpsInst& I = h->I;
     const int every = h->cfg.energy_every;
+    const bool fused = every > 0 && mpsk::elem_fuses_energy(I);
     hipError_t e = hipSuccess;
     for (long long i = 0; i < count && e == hipSuccess; i++) {
         const long long k = h->next_step;
@@ -1180,32 +1197,40 @@ static hipError_t enqueue_steps(mps_solver_s* h, long long count, std::vector<in
             if (t.a) kinds->push_back(2);
             if (e != hipSuccess) break;
         }
+        unsigned ef = 0u;
+        if (every > 0 && k % every == 0) ef |= MPS_E_DEF;
+        if (every > 0 && i >= 1 && k >= 2 && (k - 1) % every == 0) ef |= MPS_E_KIN;
         {
             TimedLaunch t(h, 0);
-            e = mpsk::launch_elem(I, 0, h->stream);
+            e = mpsk::launch_elem(I, 0, h->stream, fused ? ef : 0u);
             t.done();
             if (t.a) kinds->push_back(0);
         }
+        if (e == hipSuccess && ef && !fused) {
+            TimedLaunch t(h, 3);
+            e = mpsk::launch_energy(I, ef, h->stream);
+            t.done();
+            if (t.a) kinds->push_back(3);
+        }
         if (e == hipSuccess && (I.n_sgp > 0 || I.n_lnode > 0)) e = mpsk::launch_loads(I, h->stream);
         if (e != hipSuccess) break;
-        const bool want_e = every > 0 && (k % every) == 0;
         const bool last = (i == count - 1);
-        const bool fuse = !want_e && !last;
         {
             TimedLaunch t(h, 1);
-            e = mpsk::launch_node(I, fuse ? 1 : 0, h->stream);
+            e = mpsk::launch_node(I, last ? 0 : 1, h->stream, ef | (i >= 1 ? MPS_N_PREV : 0u));
             t.done();
             if (t.a) kinds->push_back(1);
         }
         if (e != hipSuccess) break;
-        if (want_e) {
+        if (last && every > 0) {  // the kernels check that the last step that ran is an energy step
             TimedLaunch t(h, 3);
-            e = mpsk::launch_energy(I, h->stream);
+            e = mpsk::launch_energy(I, MPS_E_KIN_NOW, h->stream);
+            if (e == hipSuccess) e = mpsk::launch_energy_tail(I, h->stream);
             t.done();
             if (t.a) kinds->push_back(3);
         }
         h->next_step = k + 1;
-        h->step_begun = fuse;
+        h->step_begun = !last;
     }
     return e;
 }
@@ -1240,18 +1265,17 @@ static int status_code(int st) { return st < 0 ? st : MPS_OK; }
 static const long long kChunk = 256;
 
 // True when mps_run replays its chunks from captured graphs in the handle's current state.
-static bool graph_replay(const mps_solver_s* h) {
-    return h->use_graphs && !h->profiling && h->cfg.energy_every == 0 && !h->step_begun;
-}
+static bool graph_replay(const mps_solver_s* h) { return h->use_graphs && !h->profiling && !h->step_begun; }
 
 // The graph that replays a chunk of n steps (begin, (element, node+begin) x (n-1), element, node), captured
 // and instantiated on first use.  Capture does not execute anything: the host mirror of the step counter is
 // restored afterwards.  The kernels read the step index from the control block, so one graph serves every
 // chunk of n steps wherever it starts.
 static hipError_t chunk_graph(mps_solver_s* h, long long n, hipGraphExec_t* out) {
+    const long long phase = h->cfg.energy_every > 0 ? h->next_step % h->cfg.energy_every : 0;
     for (auto& g : h->graphs)
-        if (g.first == n) {
-            *out = g.second;
+        if (g.n == n && g.phase == phase) {
+            *out = g.exec;
             return hipSuccess;
         }
     std::vector<int> kinds;
@@ -1267,7 +1291,7 @@ static hipError_t chunk_graph(mps_solver_s* h, long long n, hipGraphExec_t* out)
     h->next_step = k0;
     h->step_begun = false;
     if (e != hipSuccess) return e;
-    h->graphs.emplace_back(n, ge);
+    h->graphs.push_back({n, phase, ge});
     *out = ge;
     return hipSuccess;
 }
@@ -1290,11 +1314,17 @@ extern "C" int mps_prepare_run(mps_handle h, int64_t max_steps) {
     if (max_steps < 0) return h->fail(MPS_ERR_ARG, "mps_prepare_run: max_steps < 0");
     (void)hipSetDevice(h->device);
     if (!graph_replay(h)) return MPS_OK;  // mps_run will launch the kernels one by one
-    for (long long n : chunk_plan(h, max_steps)) {
+    const long long k0 = h->next_step;
+    for (long long n : chunk_plan(h, max_steps)) {  // each chunk's graph depends on its first step (energies)
         hipGraphExec_t ge = nullptr;
         hipError_t e = chunk_graph(h, n, &ge);
-        if (e != hipSuccess) return h->fail(MPS_ERR_HIP, fmt("graph capture: %s", hipGetErrorString(e)));
+        if (e != hipSuccess) {
+            h->next_step = k0;
+            return h->fail(MPS_ERR_HIP, fmt("graph capture: %s", hipGetErrorString(e)));
+        }
+        h->next_step += n;
     }
+    h->next_step = k0;
     return MPS_OK;
 }
 
@@ -1364,22 +1394,37 @@ extern "C" int mps_element_outputs(mps_handle h, double* out) {
     return MPS_OK;
 }
 
-extern "C" int mps_get_history(mps_handle h, int64_t first, int64_t count, double* time, double* dt, double* e_def,
-                               double* e_kin, double* volume) {
-    if (!h) return MPS_ERR_ARG;
+// Columns col0 .. col0+ncol-1 of the history ring, entries [first, first+count), into outs[j] (NULL: skipped).
+static int read_history(mps_solver_s* h, int64_t first, int64_t count, int col0, int ncol, double** outs) {
     const int cap = h->I.hist_cap;
-    if (first < 0 || count < 0) return h->fail(MPS_ERR_ARG, "mps_get_history: negative range");
-    (void)hipSetDevice(h->device);
-    HCHK(h, hipStreamSynchronize(h->stream));
-    std::vector<double> buf(5 * (size_t)cap);
-    HCHK(h, hipMemcpy(buf.data(), h->I.hist, buf.size() * 8, hipMemcpyDeviceToHost));
-    double* outs[5] = {time, dt, e_def, e_kin, volume};
+    std::vector<double> buf((size_t)ncol * cap);
+    HCHK(h, hipMemcpy(buf.data(), h->I.hist + (size_t)col0 * cap, buf.size() * 8, hipMemcpyDeviceToHost));
     for (int64_t i = 0; i < count; i++) {
-        size_t idx = (size_t)((first + i) % cap);
-        for (int j = 0; j < 5; j++)
+        const size_t idx = (size_t)((first + i) % cap);
+        for (int j = 0; j < ncol; j++)
             if (outs[j]) outs[j][i] = buf[(size_t)j * cap + idx];
     }
     return MPS_OK;
+}
+
+extern "C" int mps_get_history(mps_handle h, int64_t first, int64_t count, double* time, double* dt, double* e_def,
+                               double* e_kin, double* volume) {
+    if (!h) return MPS_ERR_ARG;
+    if (first < 0 || count < 0) return h->fail(MPS_ERR_ARG, "mps_get_history: negative range");
+    (void)hipSetDevice(h->device);
+    HCHK(h, hipStreamSynchronize(h->stream));
+    double* outs[5] = {time, dt, e_def, e_kin, volume};
+    return read_history(h, first, count, MPS_H_TIME, 5, outs);
+}
+
+extern "C" int mps_get_energy_balance(mps_handle h, int64_t first, int64_t count, double* e_ext, double* e_kin_lumped,
+                                      double* w_ext, double* w_react) {
+    if (!h) return MPS_ERR_ARG;
+    if (first < 0 || count < 0) return h->fail(MPS_ERR_ARG, "mps_get_energy_balance: negative range");
+    (void)hipSetDevice(h->device);
+    HCHK(h, hipStreamSynchronize(h->stream));
+    double* outs[4] = {e_ext, e_kin_lumped, w_ext, w_react};
+    return read_history(h, first, count, MPS_H_EEXT, 4, outs);
 }
 
 extern "C" int mps_get_mass(mps_handle h, double* mass) {

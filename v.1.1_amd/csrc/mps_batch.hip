@@ -360,7 +360,6 @@ int mps_batch_add_prepared(mps_batch b, const HostPrep& hp, int32_t* slot_out) {
     I.ctl = b->ctl + slot;
     I.hist = nullptr;
     I.hist_cap = 0;
-    I.partials = nullptr;
     if (cursor > bytes) {
         b->alloc.release(off, bytes);
         return b->fail(MPS_ERR_STATE, "internal: instance layout exceeds its block");

@@ -36,8 +36,14 @@
 //     sblk          double[8*blocks]  block type, P, t0, tf, traction (3), -
 //     pc            double[24*n_sgp]  the step's surface forces [(g*8 + a)*3 + dir]
 //   uprev           double[3N]   displacement of the previous step (bulk viscosity only; NULL otherwise)
-//   ctl             MpsCtl       step control (time, dt, go flags)
+//   ctl             MpsCtl       step control (time, dt, go flags), running work sums, reduction counters
 //   dtred           u64[2*MPS_DT_SLOTS*MPS_DT_STRIDE]  stable-dt reduction slots by step parity
+//   hist            double[MPS_HIST_COLS*hist_cap]     per-step history ring, column c at c*hist_cap
+//   energy bookkeeping (solves with energies only):
+//     epart         double[3*n_patch + 2*MPS_NODE_PARTS*node_blocks]  per-block energy sums: element phase, then
+//                   the node kernel's by step parity
+//     rprev         double[3*R]       previous step's support force fint - fext of each Dirichlet record
+//     lfprev        double[3*n_lnode] previous step's external force of each flagged node
 #pragma once
 #include <stdint.h>
 
@@ -53,6 +59,16 @@
 #define MPS_ITEM_INDEX 0x3FFFFFFFu
 #define MPS_SGP_STRIDE 36            // doubles per pressure surface Gauss point
 #define MPS_SBLK_STRIDE 8            // doubles per surface-load block: type, P, t0, tf, Tx, Ty, Tz, -
+// History ring columns: time, dt, DEFO_ENERGY, KIN_ENERGY, VOLUME, EXTERNAL_ENERGY, lumped kinetic energy,
+// work of the loads, work of the supports
+enum : int32_t { MPS_H_TIME = 0, MPS_H_DT, MPS_H_EDEF, MPS_H_EKIN, MPS_H_VOL, MPS_H_EEXT, MPS_H_EKL, MPS_H_WEXT,
+                 MPS_H_WREACT, MPS_HIST_COLS };
+#define MPS_NODE_PARTS 4  // node-kernel partial sums per block: work of loads, work of supports, lumped KE, u.fext
+// Energy flags of an element-kernel launch (fused energies) and of the standalone energy pass
+#define MPS_E_DEF 1u   // DEFO_ENERGY and VOLUME of the step in flight (u_k)
+#define MPS_E_KIN 2u   // KIN_ENERGY of the previous step (v_{k-1}, before the solve of step k)
+#define MPS_E_KIN_NOW 4u  // standalone pass after the last node kernel of a chunk: KIN_ENERGY of that step
+#define MPS_N_PREV 8u  // node kernel: also reduce the previous step's node sums (that step ran in the same launch)
 
 enum : int32_t { MPS_TIME_WINDOW = 0, MPS_TIME_FIXED = 1 };
 
@@ -77,6 +93,8 @@ struct MpsCtl {
     int32_t nonfinite;  // set when u[1] is non-finite
     long long nan_step; // first step with non-finite u[1] (LLONG_MAX if none)
     unsigned long long dt0min;  // min_e l0/c (setup)
+    double w_ext, w_react;      // work of the loads and of the supports since the end of step 1
+    long long wred;             // last step whose node-kernel energy sums were reduced
 };
 
 struct MpsInst {
@@ -107,9 +125,12 @@ struct MpsInst {
     double* part;
     unsigned long long* dtred;  // dt reduction slots (bit patterns of non-negative doubles, integer min)
     MpsCtl* ctl;
-    double* hist;      // [5*hist_cap]: time, dt, e_def, e_kin, volume
-    double* partials;  // energy partials [3*n_blocks]
+    double* hist;      // [MPS_HIST_COLS*hist_cap] (MPS_H_* columns)
     int32_t hist_cap;
+    int32_t energy_every;  // 0: no energies; k: energies every k-th step, work sums every step
+    double* epart;     // energy block sums: element phase [3*n_patch], node kernel [2][MPS_NODE_PARTS*node blocks]
+    double* rprev;     // [3*R] previous support force per Dirichlet record
+    double* lfprev;    // [3*n_lnode] previous external force per flagged node
     int32_t dof1_node, dof1_dir;
     int32_t n_patch;
     int32_t n_part;

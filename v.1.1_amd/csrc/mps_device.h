@@ -215,11 +215,22 @@ __device__ __forceinline__ void pk1_svk_lin(const T* H, T lam, T mu, T* P) {
             P[i * 3 + j] = S[i * 3 + j] + H[i * 3] * S[j] + H[i * 3 + 1] * S[3 + j] + H[i * 3 + 2] * S[6 + j];
 }
 
-// FP64: F-based, as the reference (neo-Hookean laws).
+__device__ __forceinline__ double defo_energy(int law, const double* F, double lambda, double mu);
+
+// FP64: F-based, as the reference (neo-Hookean laws).  With W and Jout (energy steps) also the energy density
+// W(F) (defo_energy) and J = det F, from the quantities P is computed with; P is the same either way.
 template <int LAW>
-__device__ __forceinline__ void pk1(const double* H, double pa, double pb, double* P) {
+__device__ __forceinline__ void pk1(const double* H, double pa, double pb, double* P, double* W = nullptr,
+                                    double* Jout = nullptr) {
     if (LAW >= 2) {
         pk1_svk_lin<double, LAW>(H, pa, pb, P);
+        if (W) {
+            double F[9];
+#pragma unroll
+            for (int i = 0; i < 9; i++) F[i] = H[i] + ((i % 4) == 0 ? 1.0 : 0.0);
+            *W = defo_energy(LAW, F, pa, pb);
+            *Jout = 1.0 + detIpH_m1(H);
+        }
         return;
     }
     double F[9];
@@ -231,12 +242,21 @@ __device__ __forceinline__ void pk1(const double* H, double pa, double pb, doubl
     double cF[9];
     cof3(F, cF);
     const double J = F[0] * cF[0] + F[1] * cF[1] + F[2] * cF[2];
+    if (Jout) *Jout = J;
     if (LAW == 0) {  // pa = lambda, pb = mu
-        const double alpha = pa * log_near1(J) - pb;
+        const double l = log_near1(J);
+        const double alpha = pa * l - pb;
         const double s = alpha * rcp(J);
 #pragma unroll
         for (int i = 0; i < 9; i++) P[i] = pb * F[i] + s * cF[i];
-    } else {  // pa = mu, pb = K1
+        if (W) {  // 1/2 lambda (ln J)^2 - mu ln J + 1/2 mu tr C (hyperElasticNeoHookean.cpp:291-304)
+            double trC = 0.0;
+#pragma unroll
+            for (int i = 0; i < 9; i++) trC += F[i] * F[i];
+            *W = 0.5 * pa * l * l - pb * l + 0.5 * pb * trC;
+        }
+    } else {  // pa = mu, pb = K1 (no defoEnergy: W = 0, constitutiveModels.h:91-94)
+        if (W) *W = 0.0;
         double B[9];
 #pragma unroll
         for (int i = 0; i < 3; i++)
@@ -528,33 +548,59 @@ __device__ __forceinline__ void hex_butterfly(const double (*x)[3], double (*A)[
     }
 }
 
-// Inverse transform: f_a = sum_m sigma_m(a) G[m]  (G[m] already scaled).
+__device__ __forceinline__ double fma_t(double a, double b, double c) { return __builtin_fma(a, b, c); }
+__device__ __forceinline__ float fma_t(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+
+// The last two levels of the inverse transform from the pair sums and differences t.. (shared by both forms below).
 template <typename T>
-__device__ __forceinline__ void hex_unbutterfly(const T (*G)[3], T (*f)[3]) {
+__device__ __forceinline__ void hex_unbutterfly_tail(T tssm, T tsdm, T tdsm, T tddm, T tssp, T tsdp, T tdsp, T tddp,
+                                                     T (*f)[3], int d) {
+#pragma clang fp contract(off)
+    // zeta-, eta- (nodes 0,1)
+    T us = tssm - tsdm, ud = tdsm - tddm;
+    f[0][d] = us - ud;
+    f[1][d] = us + ud;
+    // zeta-, eta+ (nodes 3,2)
+    us = tssm + tsdm;
+    ud = tdsm + tddm;
+    f[3][d] = us - ud;
+    f[2][d] = us + ud;
+    // zeta+, eta- (nodes 4,5)
+    us = tssp - tsdp;
+    ud = tdsp - tddp;
+    f[4][d] = us - ud;
+    f[5][d] = us + ud;
+    // zeta+, eta+ (nodes 7,6)
+    us = tssp + tsdp;
+    ud = tdsp + tddp;
+    f[7][d] = us - ud;
+    f[6][d] = us + ud;
+}
+
+// Inverse transform with the coefficient scales folded in: f_a = sum_m sigma_m(a) s_m G[m] with s_m = s1 for the
+// linear, s2 for the bilinear and s3 for the trilinear coefficient.  The products and sums are written out
+// (explicit FMAs, no contraction), so the rounding does not depend on where the compiler places this code
+// relative to other work of the kernel: the optional kinetic-energy block that follows it in the element kernel
+// moved the scaling into another basic block, which changed which multiplies the compiler fused.
+template <typename T>
+__device__ __forceinline__ void hex_unbutterfly_scaled(const T (*G)[3], T s1, T s2, T s3, T (*f)[3]) {
+#pragma clang fp contract(off)
 #pragma unroll
     for (int d = 0; d < 3; d++) {
-        T tssm = -G[2][d], tsdm = G[1][d] - G[4][d], tdsm = G[0][d] - G[5][d], tddm = G[3][d] - G[6][d];
-        T tssp = G[2][d], tsdp = G[1][d] + G[4][d], tdsp = G[0][d] + G[5][d], tddp = G[3][d] + G[6][d];
-        // zeta-, eta- (nodes 0,1)
-        T us = tssm - tsdm, ud = tdsm - tddm;
-        f[0][d] = us - ud;
-        f[1][d] = us + ud;
-        // zeta-, eta+ (nodes 3,2)
-        us = tssm + tsdm;
-        ud = tdsm + tddm;
-        f[3][d] = us - ud;
-        f[2][d] = us + ud;
-        // zeta+, eta- (nodes 4,5)
-        us = tssp - tsdp;
-        ud = tdsp - tddp;
-        f[4][d] = us - ud;
-        f[5][d] = us + ud;
-        // zeta+, eta+ (nodes 7,6)
-        us = tssp + tsdp;
-        ud = tdsp + tddp;
-        f[7][d] = us - ud;
-        f[6][d] = us + ud;
+        const T g2 = G[2][d] * s1, g4 = G[4][d] * s2, g5 = G[5][d] * s2, g6 = G[6][d] * s3;
+        hex_unbutterfly_tail<T>(-g2, fma_t(G[1][d], s1, -g4), fma_t(G[0][d], s1, -g5), fma_t(G[3][d], s2, -g6), g2,
+                                fma_t(G[1][d], s1, g4), fma_t(G[0][d], s1, g5), fma_t(G[3][d], s2, g6), f, d);
     }
+}
+
+// Inverse transform: f_a = sum_m sigma_m(a) G[m]  (G[m] already scaled), without contraction.
+template <typename T>
+__device__ __forceinline__ void hex_unbutterfly(const T (*G)[3], T (*f)[3]) {
+#pragma clang fp contract(off)
+#pragma unroll
+    for (int d = 0; d < 3; d++)
+        hex_unbutterfly_tail<T>(-G[2][d], G[1][d] - G[4][d], G[0][d] - G[5][d], G[3][d] - G[6][d], G[2][d],
+                                G[1][d] + G[4][d], G[0][d] + G[5][d], G[3][d] + G[6][d], f, d);
 }
 
 // Scale the coefficient vectors into Jacobian units: linear terms by 1/8, bilinear by
@@ -629,15 +675,69 @@ __device__ __forceinline__ void visc_coeffs(const MpsInst& I, int e, double c, d
     *qb = rho * h * I.visc_cl * c0;
 }
 
+// Consistent kinetic energy of hex8 element e (KIN_ENERGY, dataExtraction.cpp:631-707): sum over the GPs of
+// wJ[gp] rho/2 |v(gp)|^2, v(gp) = sum_a phi_a(gp) v_a evaluated from the butterfly coefficients of the nodal
+// velocities, (sum_a v_a + sum_m s_m(gp) g^|m| C_m) / 8.
+__device__ __forceinline__ double hex_kinetic(const MpsInst& I, const int* nd, double rho, const double* wJ) {
+    const size_t N = (size_t)I.N;
+    double x[8][3], Cv[7][3], vs[3];
+#pragma unroll
+    for (int a = 0; a < 8; a++)
+#pragma unroll
+        for (int d = 0; d < 3; d++) x[a][d] = I.v[d * N + nd[a]];
+    hex_butterfly(x, Cv);
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        double t = 0.0;
+#pragma unroll
+        for (int a = 0; a < 8; a++) t += x[a][d];
+        vs[d] = t;
+    }
+    const double g = MPS_GP, g2 = MPS_GP * MPS_GP, g3 = MPS_GP * MPS_GP * MPS_GP;
+    double ek = 0.0;
+#pragma unroll
+    for (int gp = 0; gp < 8; gp++) {
+        const int sx = sxi(gp), sy = seta(gp), sz = szeta(gp);
+        double kv = 0.0;
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+            const double vel = 0.125 * (vs[j] + (sx * g) * Cv[0][j] + (sy * g) * Cv[1][j] + (sz * g) * Cv[2][j] +
+                                        (sx * sy * g2) * Cv[3][j] + (sy * sz * g2) * Cv[4][j] +
+                                        (sx * sz * g2) * Cv[5][j] + (sx * sy * sz * g3) * Cv[6][j]);
+            kv += 0.5 * rho * vel * vel;
+        }
+        ek += kv * wJ[gp];
+    }
+    return ek;
+}
+
+// Jacobian (or displacement gradient) D(i,j) = d(.)_i/d xi_j at the Gauss point with signs (sx, sy, sz) from the
+// scaled butterfly coefficients A (hex_scale).
+template <typename T>
+__device__ __forceinline__ void hex_jac(const T (*A)[3], T sx, T sy, T sz, T* D) {
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        D[i * 3 + 0] = A[0][i] + sy * A[3][i] + sz * A[5][i] + (sy * sz) * A[6][i];
+        D[i * 3 + 1] = A[1][i] + sx * A[3][i] + sz * A[4][i] + (sx * sz) * A[6][i];
+        D[i * 3 + 2] = A[2][i] + sx * A[5][i] + sy * A[4][i] + (sx * sy) * A[6][i];
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Hex8 element: nodal forces f[a][i] (T) and the element dt contribution (FP64).
 // VISC adds bulk viscosity from the displacement increment u - uprev over the step (rdt = 1/dt, 0 when
 // dt = 0); its 7 butterfly coefficient vectors wait in the element's LDS force slots vs[m*3+d] (stride
 // MPS_PATCH), which the forces overwrite only after the Gauss-point loop.
+// EN (FP64, inviscid): DEFO_ENERGY and VOLUME of u in en[0], en[1] (sum over the GPs of wJ W(F), wJ det F); the
+// GPs' wJ go to vs[gp*MPS_PATCH] (an LDS array apart from the force slots), for the kinetic energy, which the
+// element kernel computes after the force phase (hex_kinetic).
 // ---------------------------------------------------------------------------
-template <typename T, int LAW, bool VISC = false>
+template <typename T, int LAW, bool VISC = false, bool EN = false>
 __device__ __forceinline__ double hex8_element(const MpsInst& I, int e, const double* __restrict__ u, T (*f)[3],
-                                               bool want_dt, T* vs = nullptr, double rdt = 0.0) {
+                                               bool want_dt, T* vs = nullptr, double rdt = 0.0,
+                                               unsigned eflags = 0u, double* en = nullptr) {
+    static_assert(!EN || (sizeof(T) == 8 && !VISC), "fused energies: FP64 inviscid element kernels");
+    double e_def = 0.0, e_vol = 0.0;
     const int32_t* cn = I.conn + (size_t)e * 8;
     int4 q0 = *reinterpret_cast<const int4*>(cn);
     int4 q1 = *reinterpret_cast<const int4*>(cn + 4);
@@ -704,15 +804,8 @@ __device__ __forceinline__ double hex8_element(const MpsInst& I, int e, const do
         const T sy = (gp == 2 || gp == 3 || gp == 6 || gp == 7) ? T(1) : T(-1);
         const T sz = (gp >= 4) ? T(1) : T(-1);
         T Jm[9], Du[9], H[9], P[9];
-#pragma unroll
-        for (int i = 0; i < 3; i++) {
-            Jm[i * 3 + 0] = Ax[0][i] + sy * Ax[3][i] + sz * Ax[5][i] + (sy * sz) * Ax[6][i];
-            Jm[i * 3 + 1] = Ax[1][i] + sx * Ax[3][i] + sz * Ax[4][i] + (sx * sz) * Ax[6][i];
-            Jm[i * 3 + 2] = Ax[2][i] + sx * Ax[5][i] + sy * Ax[4][i] + (sx * sy) * Ax[6][i];
-            Du[i * 3 + 0] = Au[0][i] + sy * Au[3][i] + sz * Au[5][i] + (sy * sz) * Au[6][i];
-            Du[i * 3 + 1] = Au[1][i] + sx * Au[3][i] + sz * Au[4][i] + (sx * sz) * Au[6][i];
-            Du[i * 3 + 2] = Au[2][i] + sx * Au[5][i] + sy * Au[4][i] + (sx * sy) * Au[6][i];
-        }
+        hex_jac<T>(Ax, sx, sy, sz, Jm);
+        hex_jac<T>(Au, sx, sy, sz, Du);
         // J^-1 = cof(J)^T / det J ; weight 1 (GPsDistribution.cpp:392-432) so wJ J^-T = sign(det J) cof(J)
         T cJ[9];
         cof3(Jm, cJ);
@@ -724,7 +817,16 @@ __device__ __forceinline__ double hex8_element(const MpsInst& I, int e, const do
 #pragma unroll
             for (int j = 0; j < 3; j++)
                 H[i * 3 + j] = (Du[i * 3] * cJ[j * 3] + Du[i * 3 + 1] * cJ[j * 3 + 1] + Du[i * 3 + 2] * cJ[j * 3 + 2]) * rJ;
-        pk1<LAW>(H, tpa, tpb, P);
+        if constexpr (EN) {  // weight 1: wJ = |det J| (classGPs.cpp:26-27)
+            double Wg, Jg;
+            pk1<LAW>(H, tpa, tpb, P, &Wg, &Jg);
+            const double wJ = fabs(dJ);
+            e_def += Wg * wJ;
+            e_vol += wJ * Jg;
+            vs[gp * MPS_PATCH] = wJ;
+        } else {
+            pk1<LAW>(H, tpa, tpb, P);
+        }
         if (VISC) {  // tr of grad(u - uprev) = sum_ik Dd(i,k) cof(J)(i,k) / det J
             // the memory clobber keeps the 21 LDS reads at each Gauss point: hoisted out of the loop
             // they would hold 21 more values in registers throughout it.  The FP64 Belytschko kernel
@@ -766,18 +868,11 @@ __device__ __forceinline__ double hex8_element(const MpsInst& I, int e, const do
             G[6][i] += (sy * sz) * Q[0][i] + (sx * sz) * Q[1][i] + (sx * sy) * Q[2][i];
         }
     }
-    const T s1 = (T)0.125, s2 = (T)(0.125 * MPS_GP), s3 = (T)(0.125 * MPS_GP * MPS_GP);
-#pragma unroll
-    for (int i = 0; i < 3; i++) {
-        G[0][i] *= s1;
-        G[1][i] *= s1;
-        G[2][i] *= s1;
-        G[3][i] *= s2;
-        G[4][i] *= s2;
-        G[5][i] *= s2;
-        G[6][i] *= s3;
+    hex_unbutterfly_scaled<T>(G, (T)0.125, (T)(0.125 * MPS_GP), (T)(0.125 * MPS_GP * MPS_GP), f);
+    if constexpr (EN) {
+        en[0] = e_def;
+        en[1] = e_vol;
     }
-    hex_unbutterfly<T>(G, f);
     if (want_dt && sizeof(T) == 4) {
         // precision form: FP32 length from the deformed node positions relative to their centroid,
         // x_a - xbar = sum_m sigma_m(a) C_m / 8 with C the butterfly coefficients of X + u (Ax + Au
@@ -821,9 +916,13 @@ __device__ __forceinline__ TetConsts tet_consts(const MpsInst& I, int e, bool wa
 
 // tet4_core takes the element's node coordinates x, displacements uu and constants; nd (node ids) is
 // read only by the viscous form.  tet4_element gathers them from X, u and the material arrays.
-template <typename T, int LAW, bool VISC = false>
+// EN (FP64, inviscid): en[0], en[1] = wJ W(F), wJ det F of the element's Gauss point, en[3] = its wJ (the
+// kinetic energy needs the node ids, which the callers hold).
+template <typename T, int LAW, bool VISC = false, bool EN = false>
 __device__ __forceinline__ double tet4_core(const MpsInst& I, int e, const int* nd, double (*x)[3], const double (*uu)[3],
-                                            const TetConsts& k, T (*f)[3], bool want_dt, double rdt) {
+                                            const TetConsts& k, T (*f)[3], bool want_dt, double rdt,
+                                            double* en = nullptr) {
+    static_assert(!EN || (sizeof(T) == 8 && !VISC), "fused energies: FP64 inviscid element kernels");
     const size_t N = (size_t)I.N;
     const double pa = k.pa, pb = k.pb, c = k.c;
     T Jm[9], Du[9];
@@ -851,7 +950,15 @@ __device__ __forceinline__ double tet4_core(const MpsInst& I, int e, const int* 
     for (int i = 0; i < 3; i++)
 #pragma unroll
         for (int j = 0; j < 3; j++) H[i * 3 + j] = Du[i * 3] * iJ[j] + Du[i * 3 + 1] * iJ[3 + j] + Du[i * 3 + 2] * iJ[6 + j];
-    pk1<LAW>(H, (T)pa, (T)pb, P);
+    if constexpr (EN) {
+        double Wg, Jg;
+        pk1<LAW>(H, pa, pb, P, &Wg, &Jg);
+        en[0] = Wg * wJ;
+        en[1] = wJ * Jg;
+        en[3] = wJ;
+    } else {
+        pk1<LAW>(H, (T)pa, (T)pb, P);
+    }
     if (VISC) {  // tr of grad(u - uprev) = sum_ik Dd(i,k) J^-1(k,i)
         T Dd[9];
 #pragma unroll
@@ -882,9 +989,25 @@ __device__ __forceinline__ double tet4_core(const MpsInst& I, int e, const int* 
     return dte;
 }
 
-template <typename T, int LAW, bool VISC = false>
+// Kinetic energy of tet4 element e at its Gauss point (dataExtraction.cpp:631-707): wJ rho/2 |v(gp)|^2 with
+// v(gp) = sum_a v_a / 4 (the shape values at (1/4, 1/4, 1/4)).
+__device__ __forceinline__ double tet_kinetic(const MpsInst& I, const int* nd, double rho, double wJ) {
+    const size_t N = (size_t)I.N;
+    double vel[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+    for (int a = 0; a < 4; a++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) vel[j] += 0.25 * I.v[j * N + nd[a]];
+    double kv = 0.0;
+#pragma unroll
+    for (int j = 0; j < 3; j++) kv += 0.5 * rho * vel[j] * vel[j];
+    return kv * wJ;
+}
+
+template <typename T, int LAW, bool VISC = false, bool EN = false>
 __device__ __forceinline__ double tet4_element(const MpsInst& I, int e, const double* __restrict__ u, T (*f)[3],
-                                               bool want_dt, double rdt = 0.0) {
+                                               bool want_dt, double rdt = 0.0, unsigned eflags = 0u,
+                                               double* en = nullptr) {
     const int4 q = *reinterpret_cast<const int4*>(I.conn + (size_t)e * 4);
     const int nd[4] = {q.x, q.y, q.z, q.w};
     const size_t N = (size_t)I.N;
@@ -896,7 +1019,16 @@ __device__ __forceinline__ double tet4_element(const MpsInst& I, int e, const do
             x[a][d] = I.X[d * N + nd[a]];
             uu[a][d] = u[d * N + nd[a]];
         }
-    return tet4_core<T, LAW, VISC>(I, e, nd, x, uu, tet_consts(I, e, want_dt), f, want_dt, rdt);
+    if constexpr (EN) {
+        double r[4];
+        const double dte = tet4_core<T, LAW, false, true>(I, e, nd, x, uu, tet_consts(I, e, want_dt), f, want_dt, rdt, r);
+        en[0] = r[0];
+        en[1] = r[1];
+        en[2] = (eflags & MPS_E_KIN) ? tet_kinetic(I, nd, I.rho ? I.rho[e] : I.rho0, r[3]) : 0.0;
+        return dte;
+    } else {
+        return tet4_core<T, LAW, VISC>(I, e, nd, x, uu, tet_consts(I, e, want_dt), f, want_dt, rdt);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -928,38 +1060,46 @@ __device__ __forceinline__ int begin_scalars(const MpsInst& I, long long k, cons
     return MPSD_RUNNING;
 }
 
-// Prediction + Dirichlet BC of node n for step k (nonlinearSystem.cpp:1186-1201,
-// solverExplicit.cpp:545-575).
+// Prescribed motion of a constrained DOF at step k with step dt (applyDisplacementBC, solverExplicit.cpp:545-575;
+// classNodes.cpp:58-74): increment U and velocity V.  Ramp (val = UU = U/duration): U = V = UU dt; instantaneous:
+// U = V = val at the first step, 0 after; velocity amplitude (extension): V = val min(k/ramp, 1), U = V dt.
+__device__ __forceinline__ void bc_motion(const MpsInst& I, double val, long long k, double dt, double* U, double* V) {
+#pragma clang fp contract(off)
+    if (I.bc_mode == 1) {
+        *U = (k == 1) ? val : 0.0;
+        *V = *U;
+    } else if (I.bc_mode == 0) {
+        *U = val * dt;
+        *V = *U;
+    } else {
+        double fr = (double)k / (double)(I.vel_ramp > 0 ? I.vel_ramp : 1);
+        if (fr > 1.0) fr = 1.0;
+        *V = val * fr;
+        *U = *V * dt;
+    }
+}
+
+// Increment of DOF i of node n at the begin of step k: the prediction dt v + dt^2/2 a on active DOFs
+// (nonlinearSystem.cpp:1186-1201), the prescribed U on constrained ones.
+__device__ __forceinline__ double step_increment(const MpsInst& I, uint32_t info, int i, long long k, double dt,
+                                                 double v, double a) {
+#pragma clang fp contract(off)
+    if (!((info >> i) & 1u)) return dt * v + 0.5 * dt * dt * a;
+    double U, V;
+    bc_motion(I, I.bcval[3 * (size_t)((info & MPS_REC_MASK) >> 3) + i], k, dt, &U, &V);
+    return U;
+}
+
+// Prediction + Dirichlet displacement of node n for step k (nonlinearSystem.cpp:1186-1201,
+// solverExplicit.cpp:545-575).  The prescribed velocity and the zero acceleration of the constrained DOFs are
+// set by the solve of the same step (solve_node), so that until then v and a hold the previous step's values
+// everywhere; nothing reads them in between except the energy of the previous step, and the state at the
+// end of the step is the reference's.
 __device__ __forceinline__ void predict_node(const MpsInst& I, int n, long long k, double dt, double* u, double* v,
                                              double* a) {
-#pragma clang fp contract(off)
-    uint32_t info = I.ninfo[n];
-    uint32_t mask = info & 7u;
-    uint32_t rec = (info & MPS_REC_MASK) >> 3;
+    const uint32_t info = I.ninfo[n];
 #pragma unroll
-    for (int i = 0; i < 3; i++) {
-        if (!((mask >> i) & 1u)) {
-            u[i] += dt * v[i] + 0.5 * dt * dt * a[i];
-        } else {
-            double val = I.bcval[3 * (size_t)rec + i];
-            double U, V;
-            if (I.bc_mode == 1) {  // instantaneous: applied once, at the first step
-                U = (k == 1) ? val : 0.0;
-                V = U;
-            } else if (I.bc_mode == 0) {  // ramp: UU*dt (val = U/duration)
-                U = val * dt;
-                V = U;
-            } else {  // velocity amplitude (extension)
-                double fr = (double)k / (double)(I.vel_ramp > 0 ? I.vel_ramp : 1);
-                if (fr > 1.0) fr = 1.0;
-                V = val * fr;
-                U = V * dt;
-            }
-            u[i] += U;
-            v[i] = V;
-            a[i] = 0.;
-        }
-    }
+    for (int i = 0; i < 3; i++) u[i] += step_increment(I, info, i, k, dt, v[i], a[i]);
 }
 
 // External force of flagged node j (lnn[j]) at time t (NeumannBCManagement, boundaryConditions.cpp:75-140):
@@ -1078,13 +1218,28 @@ __device__ __forceinline__ void load_partial(const MpsInst& I, int g, double* c)
     }
 }
 
-// Gather + lumped-mass solve of node n (nonlinearSystem.cpp:1096-1107; rhs = -fint + fext).
-// The node's internal force is the sum of its patch partials in ascending patch order.
-__device__ __forceinline__ void solve_node(const MpsInst& I, int n, double dt, double* v, double* a) {
+// Gather + lumped-mass solve of node n at step k (nonlinearSystem.cpp:1096-1107; rhs = -fint + fext), and the
+// prescribed velocity and zero acceleration of its constrained DOFs (applyDisplacementBC, solverExplicit.cpp:
+// 568-571).  The node's internal force is the sum of its patch partials in ascending patch order.  With
+// fi_out / fe_out the internal and external forces of the node are returned as well (also for a fully
+// constrained node).
+__device__ __forceinline__ void solve_node(const MpsInst& I, int n, long long k, double dt, double* v, double* a,
+                                           double* fi_out = nullptr, double* fe_out = nullptr) {
 #pragma clang fp contract(off)
     const uint32_t info = I.ninfo[n];
     const uint32_t mask = info & 7u;
-    if (mask == 7u) return;
+    if (mask) {
+        const uint32_t rec = (info & MPS_REC_MASK) >> 3;
+#pragma unroll
+        for (int i = 0; i < 3; i++)
+            if ((mask >> i) & 1u) {
+                double U, V;
+                bc_motion(I, I.bcval[3 * (size_t)rec + i], k, dt, &U, &V);
+                v[i] = V;
+                a[i] = 0.;
+            }
+    }
+    if (mask == 7u && !fi_out) return;
     int q = I.nrow[n];
     const int q1 = I.nrow[n + 1];
     double f0 = 0.0, f1 = 0.0, f2 = 0.0;
@@ -1126,6 +1281,13 @@ __device__ __forceinline__ void solve_node(const MpsInst& I, int n, double dt, d
     node_fext(I, n, info, fe);
     double m = I.mass[n];
     double fi[3] = {f0, f1, f2};
+    if (fi_out) {
+#pragma unroll
+        for (int i = 0; i < 3; i++) {
+            fi_out[i] = fi[i];
+            fe_out[i] = fe[i];
+        }
+    }
 #pragma unroll
     for (int i = 0; i < 3; i++) {
         if ((mask >> i) & 1u) continue;
@@ -1134,6 +1296,53 @@ __device__ __forceinline__ void solve_node(const MpsInst& I, int n, double dt, d
         v[i] += dt * ((1. - 0.5) * a[i] + 0.5 * an);
         a[i] = an;
     }
+}
+
+// Energy bookkeeping of node n at step k, after its solve (u: displacement of the step, v: its new velocity;
+// vold, incr: the velocity before the solve and the step's increment; fi, fe: internal and external force).
+// Adds to red[MPS_NODE_PARTS]: the work of the external force over the step, 1/2 (fext_prev + fext) . incr; the
+// work of the supports, 1/2 (r_prev + r) . incr + 1/2 m (v^2 - vold^2) over the constrained DOFs with
+// r = fint - fext (the update sets a = 0 there, so the momentum the prescribed velocity gives the node's mass
+// is the support's); the lumped kinetic energy 1/2 m v^2; and u . fext (EXTERNAL_ENERGY, dataExtraction.cpp:
+// 867-875).  Works are counted from step 2 on; every step stores the forces the next one takes as previous.
+__device__ __forceinline__ void node_energy(const MpsInst& I, int n, long long k, const double* u, const double* v,
+                                            const double* vold, const double* incr, const double* fi,
+                                            const double* fe, double* red) {
+    const uint32_t info = I.ninfo[n];
+    const uint32_t mask = info & 7u;
+    const double m = I.mass[n];
+    double fp[3] = {fe[0], fe[1], fe[2]};  // unflagged nodes: gravity alone (constant) or nothing
+    if (info & MPS_NODE_LOADED) {
+        double* lp = I.lfprev + 3 * (size_t)I.lidx[n];
+#pragma unroll
+        for (int i = 0; i < 3; i++) {
+            fp[i] = lp[i];
+            lp[i] = fe[i];
+        }
+    }
+    double we = 0.0, wr = 0.0, ek = 0.0, ee = 0.0;
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        we += 0.5 * (fp[i] + fe[i]) * incr[i];
+        ek += 0.5 * m * v[i] * v[i];
+        ee += u[i] * fe[i];
+    }
+    if (mask) {
+        double* rp = I.rprev + 3 * (size_t)((info & MPS_REC_MASK) >> 3);
+#pragma unroll
+        for (int i = 0; i < 3; i++)
+            if ((mask >> i) & 1u) {
+                const double r = fi[i] - fe[i];
+                wr += 0.5 * (rp[i] + r) * incr[i] + 0.5 * m * (v[i] * v[i] - vold[i] * vold[i]);
+                rp[i] = r;
+            }
+    }
+    if (k >= 2) {
+        red[0] += we;
+        red[1] += wr;
+    }
+    red[2] += ek;
+    red[3] += ee;
 }
 
 // ---------------------------------------------------------------------------
@@ -1203,6 +1412,41 @@ __device__ __forceinline__ void block_sum(double* vals, double* sh /*[NV*blockDi
 #pragma unroll
         for (int j = 0; j < NV; j++) vals[j] = sh[j * blockDim.x];
     }
+}
+
+// Grid-wide sums without fences or atomics: every block writes its block sums to part[blockIdx.x*NV + j]
+// (publish_block_sum); a later kernel on the same stream, whose start makes them visible on every XCD, adds
+// them in block order with one block (reduce_parts).  Deterministic, and no per-block device-scope release
+// (on gfx950 a release writes the whole L2 back, which cost more than the rest of the energy work).
+template <int NV>
+__device__ __forceinline__ void publish_block_sum(double* vals, double* __restrict__ part, double* sh) {
+    block_sum<NV>(vals, sh);
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int j = 0; j < NV; j++) part[(size_t)blockIdx.x * NV + j] = vals[j];
+    }
+}
+
+// Sum of the nb block sums part[b*NV + j] by the calling block (every thread calls it): each thread adds the
+// blocks b = threadIdx.x + i*blockDim.x in order, then the fixed tree of block_sum.  Totals in out of thread 0.
+template <int NV>
+__device__ __forceinline__ void reduce_parts(const double* __restrict__ part, int nb, double* out, double* sh) {
+    double acc[NV];
+#pragma unroll
+    for (int j = 0; j < NV; j++) acc[j] = 0.0;
+    for (int b = threadIdx.x; b < nb; b += blockDim.x)
+#pragma unroll
+        for (int j = 0; j < NV; j++) acc[j] += part[(size_t)b * NV + j];
+    __syncthreads();  // sh may still be read by an earlier block_sum
+    block_sum<NV>(acc, sh);
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int j = 0; j < NV; j++) out[j] = acc[j];
+    }
+}
+
+__device__ __forceinline__ double* hist_at(const MpsInst& I, int col, long long k) {
+    return I.hist + (size_t)col * I.hist_cap + (size_t)((k - 1) % I.hist_cap);
 }
 
 }  // namespace mpsd

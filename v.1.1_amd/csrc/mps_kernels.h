@@ -5,10 +5,14 @@
 #include "mps_common.h"
 
 namespace mpsk {
-// mode 0: step (dt reduction, ctl protocol); mode 1: standalone force evaluation.
-hipError_t launch_elem(const MpsInst& I, int mode, hipStream_t st);
-// fuse = 1: also begin step k+1 (prediction + BC) in the same pass.
-hipError_t launch_node(const MpsInst& I, int fuse, hipStream_t st);
+// mode 0: step (dt reduction, ctl protocol); mode 1: standalone force evaluation.  eflags (mode 0): energies
+// of the step (MPS_E_DEF, MPS_E_KIN) computed by the element kernel itself when elem_fuses_energy(I).
+hipError_t launch_elem(const MpsInst& I, int mode, hipStream_t st, unsigned eflags = 0u);
+// The element kernels of I compute energies themselves (FP64 form, no viscosity); otherwise the standalone pass.
+bool elem_fuses_energy(const MpsInst& I);
+// fuse = 1: also begin step k+1 (prediction + BC) in the same pass.  nflags (energies on): the element-phase
+// energies of the step to add up (MPS_E_DEF, MPS_E_KIN) and MPS_N_PREV when the previous step ran in this launch.
+hipError_t launch_node(const MpsInst& I, int fuse, hipStream_t st, unsigned nflags = 0u);
 hipError_t launch_begin(const MpsInst& I, hipStream_t st);
 hipError_t launch_dt(const MpsInst& I, const double* u, unsigned long long* out, hipStream_t st);
 hipError_t launch_setup(const MpsInst& I, double* mslot, double* l0out, unsigned long long* dt0, hipStream_t st);
@@ -20,8 +24,11 @@ hipError_t launch_fext(const MpsInst& I, double t, double* fext, hipStream_t st)
 // External force of the flagged nodes for the step in flight (surface kernel, then the per-node sums);
 // runs after the step's element kernel, before its node kernel.
 hipError_t launch_loads(const MpsInst& I, hipStream_t st);
-hipError_t launch_energy(const MpsInst& I, hipStream_t st);
-int energy_blocks(const MpsInst& I);
+// Standalone energy pass: MPS_E_DEF / MPS_E_KIN of step ctl->B between its element and node kernels, or
+// MPS_E_KIN_NOW after the last node kernel of a launch; block sums only (added up by the next kernel).
+hipError_t launch_energy(const MpsInst& I, unsigned flags, hipStream_t st);
+// End of a launch with energies, after the MPS_E_KIN_NOW pass: the last step's sums into the history.
+hipError_t launch_energy_tail(const MpsInst& I, hipStream_t st);
 // Element fields of the VTK output: out[21*Ne] (Cauchy 9, strain 9, VMS, volumetric, equivalent strain).
 hipError_t launch_elem_outputs(const MpsInst& I, double* out, hipStream_t st);
 hipError_t launch_init_ctl(const MpsInst& I, const unsigned long long* dt0, hipStream_t st);

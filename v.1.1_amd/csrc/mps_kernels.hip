@@ -110,10 +110,13 @@ __device__ __forceinline__ void patch_sums(const MpsInst& I, int p, const T* __r
 // force per node to part.  No float atomics: the node kernel adds the partials of a node in
 // ascending patch order, so the sum order is fixed.  Returns the block's min element dt
 // candidate (64-bit pattern, lanes other than those computing elements hold +inf).
-template <typename T, int NEN, int LAW, int BS, bool VISC = false, bool PRE = false>
+// EN: the elements' energies (hex8_element / tet4_element) are added to env[3]; hex8 keeps its GPs' wJ in
+// swj[gp*MPS_PATCH + i].
+template <typename T, int NEN, int LAW, int BS, bool VISC = false, bool PRE = false, bool EN = false>
 __device__ __forceinline__ unsigned long long patch_forces(const MpsInst& I, int p, bool want_dt, T* __restrict__ sf,
                                                            uint16_t* __restrict__ spc, uint16_t* __restrict__ spr,
-                                                           double rdt = 0.0) {
+                                                           double rdt = 0.0, unsigned eflags = 0u,
+                                                           double* env = nullptr, double* swj = nullptr) {
     const int e0 = p * MPS_PATCH;
     const int ne = min(MPS_PATCH, I.Ne - e0);
     PatchList<NEN> pl;
@@ -121,8 +124,15 @@ __device__ __forceinline__ unsigned long long patch_forces(const MpsInst& I, int
     unsigned long long dmin = INF_BITS;
     for (int i = threadIdx.x; i < ne; i += BS) {
         T f[NEN][3];
-        const double dte = (NEN == 8) ? hex8_element<T, LAW, VISC>(I, e0 + i, I.u, (T(*)[3])f, want_dt, sf + i, rdt)
-                                      : tet4_element<T, LAW, VISC>(I, e0 + i, I.u, (T(*)[3])f, want_dt, rdt);
+        double en[3] = {0.0, 0.0, 0.0};
+        const double dte =
+            (NEN == 8) ? hex8_element<T, LAW, VISC, EN>(I, e0 + i, I.u, (T(*)[3])f, want_dt,
+                                                         EN ? (T*)(swj + i) : sf + i, rdt, eflags, en)
+                       : tet4_element<T, LAW, VISC, EN>(I, e0 + i, I.u, (T(*)[3])f, want_dt, rdt, eflags, en);
+        if constexpr (EN) {
+#pragma unroll
+            for (int j = 0; j < 3; j++) env[j] += en[j];
+        }
 #pragma unroll
         for (int a = 0; a < NEN; a++)
 #pragma unroll
@@ -135,16 +145,26 @@ __device__ __forceinline__ unsigned long long patch_forces(const MpsInst& I, int
     return dmin;
 }
 
+// Energies of the step in flight from the element phase's per-work-item sums env (DEFO_ENERGY, VOLUME of step k,
+// KIN_ENERGY of step k-1): the block's sums go to the element region of epart, which the node kernel of step k
+// adds up.  sf (the force slots, no longer needed once the patch sums are done) is the block's scratch.
+__device__ __forceinline__ void elem_energy_publish(const MpsInst& I, double* env, double* sf) {
+    __syncthreads();
+    publish_block_sum<3>(env, I.epart, sf);
+}
+
 // mode 0: step (reads ctl->A, writes ctl->B, reduces dt for step k+1)
 // mode 1: standalone force evaluation (no ctl access)
 // One workgroup per patch, one element per work-item.
 // VISC (mode 0 only): bulk artificial viscosity over the step, from u - uprev and the step's dt.
-template <typename T, int NEN, int LAW, bool VISC = false>
-__global__ __launch_bounds__(MPS_PATCH) __attribute__((amdgpu_waves_per_eu(VISC && LAW == 1 ? 1 : 2))) void k_elem(MpsInst I, int mode) {
+// EN (mode 0, FP64 inviscid): the energies of eflags (MPS_E_DEF, MPS_E_KIN) into the history.
+template <typename T, int NEN, int LAW, bool VISC = false, bool EN = false>
+__global__ __launch_bounds__(MPS_PATCH) __attribute__((amdgpu_waves_per_eu(VISC && LAW == 1 ? 1 : 2))) void k_elem(MpsInst I, int mode, unsigned eflags) {
     __shared__ T sf[NEN * 3 * MPS_PATCH];
     __shared__ __align__(16) uint16_t spc[NEN * MPS_PATCH];
     __shared__ __align__(16) uint16_t spr[PatchList<NEN>::SPR];
     __shared__ unsigned long long sh[MPS_PATCH / 64];
+    __shared__ double swj[(EN && NEN == 8) ? 8 * MPS_PATCH : 1];  // hex8 GP weights wJ for the kinetic energy
     long long k = 0;
     double rdt = 0.0;
     if (mode == 0) {
@@ -155,10 +175,30 @@ __global__ __launch_bounds__(MPS_PATCH) __attribute__((amdgpu_waves_per_eu(VISC 
         if (VISC) rdt = (s.dt == 0.0) ? 0.0 : 1 / s.dt;
     }
     const int p = xcd_tile(blockIdx.x, gridDim.x);
-    unsigned long long m = patch_forces<T, NEN, LAW, MPS_PATCH, VISC, true>(I, p, mode == 0, sf, spc, spr, rdt);
+    double env[3] = {0.0, 0.0, 0.0};
+    unsigned long long m =
+        patch_forces<T, NEN, LAW, MPS_PATCH, VISC, true, EN>(I, p, mode == 0, sf, spc, spr, rdt, eflags, env, swj);
     if (mode == 0) {
         m = block_min_u64(m, sh);
         if (threadIdx.x == 0) atomicMin(dt_slot(I, (k + 1) & 1, blockIdx.x & (MPS_DT_SLOTS - 1)), m);
+        if constexpr (EN) {
+            // hex8 kinetic energy of the current v, after the force phase and its barriers: kept apart from the
+            // force arithmetic, whose compiled form (and rounding) it would otherwise share a basic block with
+            if (NEN == 8 && (eflags & MPS_E_KIN)) {
+                const int e0 = p * MPS_PATCH;
+                if ((int)threadIdx.x < min(MPS_PATCH, I.Ne - e0)) {
+                    const int e = e0 + threadIdx.x;
+                    const int4 q0 = *reinterpret_cast<const int4*>(I.conn + (size_t)e * 8);
+                    const int4 q1 = *reinterpret_cast<const int4*>(I.conn + (size_t)e * 8 + 4);
+                    const int nd[8] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, q1.z, q1.w};
+                    double wJ[8];
+#pragma unroll
+                    for (int gp = 0; gp < 8; gp++) wJ[gp] = swj[gp * MPS_PATCH + threadIdx.x];
+                    env[2] += hex_kinetic(I, nd, I.rho ? I.rho[e] : I.rho0, wJ);
+                }
+            }
+            elem_energy_publish(I, env, reinterpret_cast<double*>(sf));
+        }
     }
 }
 
@@ -168,8 +208,8 @@ __global__ __launch_bounds__(MPS_PATCH) __attribute__((amdgpu_waves_per_eu(VISC 
 // once per patch instead of once per element slot (about 1 against 5.8 node loads per partial on a
 // structured tet mesh).  Dynamic LDS: 48*pu_max bytes of staging, then the 12*MPS_PATCH force slots
 // of T, the 4*MPS_PATCH entries of the contribution list and the staged node offsets.
-template <typename T, int LAW>
-__global__ __launch_bounds__(MPS_PATCH) void k_elem_tet_staged(MpsInst I, int mode) {
+template <typename T, int LAW, bool EN = false>
+__global__ __launch_bounds__(MPS_PATCH) void k_elem_tet_staged(MpsInst I, int mode, unsigned eflags) {
     extern __shared__ __align__(16) double smem[];  // 16-byte pieces are stored at offsets that are multiples of 16
     __shared__ unsigned long long sh[MPS_PATCH / 64];
     const int pu = I.pu_max;
@@ -207,6 +247,7 @@ __global__ __launch_bounds__(MPS_PATCH) void k_elem_tet_staged(MpsInst I, int mo
     }
     __syncthreads();
     unsigned long long dmin = INF_BITS;
+    double env[3] = {0.0, 0.0, 0.0};
     if (i < ne) {
         const int l[4] = {(int)(l4.x & 0xffffu), (int)(l4.x >> 16), (int)(l4.y & 0xffffu), (int)(l4.y >> 16)};
         double x[4][3], uu[4][3];
@@ -218,7 +259,19 @@ __global__ __launch_bounds__(MPS_PATCH) void k_elem_tet_staged(MpsInst I, int mo
                 uu[a][d] = sxu[(3 + d) * pu + l[a]];
             }
         T f[4][3];
-        const double dte = tet4_core<T, LAW, false>(I, e, nullptr, x, uu, kc, f, mode == 0, 0.0);
+        double dte;
+        if constexpr (EN) {
+            double r[4];
+            dte = tet4_core<T, LAW, false, true>(I, e, nullptr, x, uu, kc, f, mode == 0, 0.0, r);
+            env[0] = r[0];
+            env[1] = r[1];
+            if (eflags & MPS_E_KIN) {
+                const int nd[4] = {I.pnd[base + l[0]], I.pnd[base + l[1]], I.pnd[base + l[2]], I.pnd[base + l[3]]};
+                env[2] = tet_kinetic(I, nd, I.rho ? I.rho[e] : I.rho0, r[3]);
+            }
+        } else {
+            dte = tet4_core<T, LAW, false>(I, e, nullptr, x, uu, kc, f, mode == 0, 0.0);
+        }
 #pragma unroll
         for (int a = 0; a < 4; a++)
 #pragma unroll
@@ -231,6 +284,7 @@ __global__ __launch_bounds__(MPS_PATCH) void k_elem_tet_staged(MpsInst I, int mo
     if (mode == 0) {
         dmin = block_min_u64(dmin, sh);
         if (threadIdx.x == 0) atomicMin(dt_slot(I, (k + 1) & 1, blockIdx.x & (MPS_DT_SLOTS - 1)), dmin);
+        if constexpr (EN) elem_energy_publish(I, env, smem);
     }
 }
 
@@ -284,11 +338,14 @@ __device__ __forceinline__ void write_hist_step(const MpsInst& I, long long k, c
     }
 }
 
-__device__ __forceinline__ void node_item(const MpsInst& I, int n, bool first, int fuse) {
+// EN: energy bookkeeping of the node into red[MPS_NODE_PARTS] (node_energy).  Returns false when the step is
+// not running (the same for every work-item of the launch).
+template <bool EN = false>
+__device__ __forceinline__ bool node_item(const MpsInst& I, int n, bool first, int fuse, double* red = nullptr) {
     MpsCtl* ctl = I.ctl;
     const long long k = ctl->B;
     const MpsStepSlot s = ctl->slot[k & 1];
-    if (!s.go) return;
+    if (!s.go) return false;
     MpsStepSlot nx;
     nx.go = 0;
     int st = MPSD_RUNNING;
@@ -302,7 +359,7 @@ __device__ __forceinline__ void node_item(const MpsInst& I, int n, bool first, i
             if (nx.go) write_hist_step(I, k + 1, nx);
         }
     }
-    if (n < 0) return;
+    if (n < 0) return true;
     const size_t N = (size_t)I.N;
     double u[3], v[3], a[3];
 #pragma unroll
@@ -311,7 +368,19 @@ __device__ __forceinline__ void node_item(const MpsInst& I, int n, bool first, i
         v[i] = I.v[i * N + n];
         a[i] = I.a[i * N + n];
     }
-    solve_node(I, n, s.dt, v, a);
+    if (EN) {
+        double vold[3], incr[3], fi[3], fe[3];
+        const uint32_t info = I.ninfo[n];
+#pragma unroll
+        for (int i = 0; i < 3; i++) {
+            vold[i] = v[i];
+            incr[i] = step_increment(I, info, i, k, s.dt, v[i], a[i]);
+        }
+        solve_node(I, n, k, s.dt, v, a, fi, fe);
+        node_energy(I, n, k, u, v, vold, incr, fi, fe, red);
+    } else {
+        solve_node(I, n, k, s.dt, v, a);
+    }
     if (n == I.dof1_node) {
         double x = u[I.dof1_dir];
         if (!isfinite(x)) {
@@ -332,12 +401,65 @@ __device__ __forceinline__ void node_item(const MpsInst& I, int n, bool first, i
         I.v[i * N + n] = v[i];
         I.a[i * N + n] = a[i];
     }
+    return true;
 }
 
-__global__ __launch_bounds__(256) void k_node(MpsInst I, int fuse) {
+// Node sums of step j (parity j&1 of npart, nb blocks) into the running works and the history, once per step
+// (ctl->wred).  Called by every thread of one block.
+__device__ __forceinline__ void node_sums_history(const MpsInst& I, long long j, const double* npart, int nb,
+                                                  double* sh) {
+    __shared__ int todo;
+    MpsCtl* ctl = I.ctl;
+    if (threadIdx.x == 0) todo = (j >= 1 && ctl->wred < j) ? 1 : 0;
+    __syncthreads();
+    if (!todo) return;
+    double t[MPS_NODE_PARTS];
+    reduce_parts<MPS_NODE_PARTS>(npart + (size_t)(j & 1) * MPS_NODE_PARTS * nb, nb, t, sh);
+    if (threadIdx.x == 0) {
+        ctl->w_ext += t[0];
+        ctl->w_react += t[1];
+        *hist_at(I, MPS_H_WEXT, j) = ctl->w_ext;
+        *hist_at(I, MPS_H_WREACT, j) = ctl->w_react;
+        if (j % I.energy_every == 0) {
+            *hist_at(I, MPS_H_EKL, j) = t[2];
+            *hist_at(I, MPS_H_EEXT, j) = t[3];
+        }
+        ctl->wred = j;
+    }
+}
+
+// fuse = 1: also begin step k+1.  EN (energies on): each block's work and energy sums go to epart (parity k&1),
+// and block 0 adds up what earlier kernels of this launch left there: the element phase of step k (nflags
+// MPS_E_DEF: DEFO_ENERGY and VOLUME of step k; MPS_E_KIN: KIN_ENERGY of step k-1) and, with MPS_N_PREV, the
+// node sums of step k-1.
+template <bool EN>
+__global__ __launch_bounds__(256) void k_node(MpsInst I, int fuse, unsigned nflags) {
     const int n = xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     const bool first = (n == 0);
-    node_item(I, n < I.N ? n : -1, first, fuse);
+    if (!EN) {
+        node_item(I, n < I.N ? n : -1, first, fuse);
+        return;
+    }
+    __shared__ double sh[MPS_NODE_PARTS * 256];
+    double red[MPS_NODE_PARTS] = {0.0, 0.0, 0.0, 0.0};
+    const long long k = I.ctl->B;  // read before the first work-item advances the step counters
+    if (!node_item<true>(I, n < I.N ? n : -1, first, fuse, red)) return;
+    const int nb = (int)gridDim.x;
+    double* npart = I.epart + 3 * (size_t)I.n_patch;
+    publish_block_sum<MPS_NODE_PARTS>(red, npart + (size_t)(k & 1) * MPS_NODE_PARTS * nb, sh);
+    if (blockIdx.x != 0 || I.hist_cap <= 0) return;
+    if (nflags & (MPS_E_DEF | MPS_E_KIN)) {
+        double t[3];
+        reduce_parts<3>(I.epart, I.n_patch, t, sh);
+        if (threadIdx.x == 0) {
+            if (nflags & MPS_E_DEF) {
+                *hist_at(I, MPS_H_EDEF, k) = t[0];
+                *hist_at(I, MPS_H_VOL, k) = t[1];
+            }
+            if ((nflags & MPS_E_KIN) && k >= 2) *hist_at(I, MPS_H_EKIN, k - 1) = t[2];
+        }
+    }
+    if (nflags & MPS_N_PREV) node_sums_history(I, k - 1, npart, nb, sh);
 }
 
 template <int BS>
@@ -559,6 +681,9 @@ __global__ void k_init_ctl(MpsInst I, const unsigned long long* dt0) {
     c->nonfinite = 0;
     c->nan_step = 0x7FFFFFFFFFFFFFFFll;
     c->dt0min = *dt0;
+    c->w_ext = 0.0;
+    c->w_react = 0.0;
+    c->wred = 0;
 }
 
 // Node-centric gather of per-(element,slot) scalars: out[n] = sum over CSR(n), ascending.
@@ -669,23 +794,24 @@ __global__ __launch_bounds__(256) void k_gather_fint(MpsInst I, double* fint) {
 }
 
 // ---------------------------------------------------------------------------
-// Energies of step B: DEFO_ENERGY, KIN_ENERGY, VOLUME (Sum), dataExtraction.cpp:569-776.
-// FP64 in every precision form.
+// Energies, FP64 in every form (DEFO_ENERGY, KIN_ENERGY, VOLUME with the Sum operation, dataExtraction.cpp:
+// 569-776), with the arithmetic of the fused element kernels: per hex8 GP J, cof(J), wJ = |det J| and
+// H = grad_xi(u) cof(J)^T / det J from the butterfly coefficients, W and det F from pk1, the kinetic energy from
+// the butterfly coefficients of v (hex_kinetic); per tet the single GP of tet4_core.  MPS_E_DEF: DEFO_ENERGY and
+// VOLUME of the current u; MPS_E_KIN / MPS_E_KIN_NOW: KIN_ENERGY of the current v.
 // ---------------------------------------------------------------------------
 template <int LAW>
-__device__ __forceinline__ void elem_energy(const MpsInst& I, int e, double* out) {
+__device__ __forceinline__ void elem_energy(const MpsInst& I, int e, unsigned flags, double* out) {
     const size_t N = (size_t)I.N;
     double pa, pb, c;
     elem_mat(I, e, pa, pb, c);
     const double rho = I.rho ? I.rho[e] : I.rho0;
     double ed = 0.0, ek = 0.0, vol = 0.0;
     if (I.nen == 8) {
-        // J, grad u and the GP velocity from the butterfly coefficients of X, u and v: the velocity at
-        // a GP is (sum_a v_a + sum_m s_m(gp) g^|m| C_m) / 8, the trilinear interpolation sum_a phi_a v_a
         int nd[8];
 #pragma unroll
         for (int a = 0; a < 8; a++) nd[a] = I.conn[(size_t)e * 8 + a];
-        double Ax[7][3], Au[7][3], Cv[7][3], vs[3];
+        double Ax[7][3], Au[7][3];
         {
             double x[8][3], cx[7][3];
 #pragma unroll
@@ -694,90 +820,62 @@ __device__ __forceinline__ void elem_energy(const MpsInst& I, int e, double* out
                 for (int d = 0; d < 3; d++) x[a][d] = I.X[d * N + nd[a]];
             hex_butterfly(x, cx);
             hex_scale<double>(cx, Ax);
+            if (flags & MPS_E_DEF) {
 #pragma unroll
-            for (int a = 0; a < 8; a++)
+                for (int a = 0; a < 8; a++)
 #pragma unroll
-                for (int d = 0; d < 3; d++) x[a][d] = I.u[d * N + nd[a]];
-            hex_butterfly(x, cx);
-            hex_scale<double>(cx, Au);
-#pragma unroll
-            for (int a = 0; a < 8; a++)
-#pragma unroll
-                for (int d = 0; d < 3; d++) x[a][d] = I.v[d * N + nd[a]];
-            hex_butterfly(x, Cv);
-#pragma unroll
-            for (int d = 0; d < 3; d++) {
-                double t = 0.0;
-#pragma unroll
-                for (int a = 0; a < 8; a++) t += x[a][d];
-                vs[d] = t;
+                    for (int d = 0; d < 3; d++) x[a][d] = I.u[d * N + nd[a]];
+                hex_butterfly(x, cx);
+                hex_scale<double>(cx, Au);
             }
         }
-        const double g = MPS_GP, g2 = MPS_GP * MPS_GP, g3 = MPS_GP * MPS_GP * MPS_GP;
+        double wJ[8];
 #pragma unroll 1
         for (int gp = 0; gp < 8; gp++) {
-            const int sx = sxi(gp), sy = seta(gp), sz = szeta(gp);
-            double Jm[9], Du[9], iJ[9], F[9];
-            hex_grad_at<double>(Ax, sx, sy, sz, Jm);
-            hex_grad_at<double>(Au, sx, sy, sz, Du);
-            double dJ = det3(Jm);
-            double cJ[9];
+            const double sx = (gp == 1 || gp == 2 || gp == 5 || gp == 6) ? 1.0 : -1.0;
+            const double sy = (gp == 2 || gp == 3 || gp == 6 || gp == 7) ? 1.0 : -1.0;
+            const double sz = (gp >= 4) ? 1.0 : -1.0;
+            double Jm[9], cJ[9];
+            hex_jac<double>(Ax, sx, sy, sz, Jm);
             cof3(Jm, cJ);
-            const double rJ = rcp(dJ);
+            const double dJ = Jm[0] * cJ[0] + Jm[1] * cJ[1] + Jm[2] * cJ[2];
+            const double w = fabs(dJ);
+            wJ[gp] = w;
+            if (flags & MPS_E_DEF) {
+                double Du[9], H[9], P[9], Wg, Jg;
+                hex_jac<double>(Au, sx, sy, sz, Du);
+                const double rJ = rcp(dJ);
 #pragma unroll
-            for (int i = 0; i < 3; i++)
+                for (int i = 0; i < 3; i++)
 #pragma unroll
-                for (int j = 0; j < 3; j++) iJ[j * 3 + i] = cJ[i * 3 + j] * rJ;
-            double wJ = fabs(dJ);
-            for (int i = 0; i < 3; i++)
-                for (int j = 0; j < 3; j++)
-                    F[i * 3 + j] = (i == j ? 1.0 : 0.0) + Du[i * 3] * iJ[j] + Du[i * 3 + 1] * iJ[3 + j] + Du[i * 3 + 2] * iJ[6 + j];
-            ed += defo_energy(LAW, F, pa, pb) * wJ;
-            double kv = 0.0;
-#pragma unroll
-            for (int j = 0; j < 3; j++) {
-                const double vel = 0.125 * (vs[j] + (sx * g) * Cv[0][j] + (sy * g) * Cv[1][j] + (sz * g) * Cv[2][j] +
-                                            (sx * sy * g2) * Cv[3][j] + (sy * sz * g2) * Cv[4][j] +
-                                            (sx * sz * g2) * Cv[5][j] + (sx * sy * sz * g3) * Cv[6][j]);
-                kv += 0.5 * rho * vel * vel;
+                    for (int j = 0; j < 3; j++)
+                        H[i * 3 + j] =
+                            (Du[i * 3] * cJ[j * 3] + Du[i * 3 + 1] * cJ[j * 3 + 1] + Du[i * 3 + 2] * cJ[j * 3 + 2]) * rJ;
+                pk1<LAW>(H, pa, pb, P, &Wg, &Jg);
+                ed += Wg * w;
+                vol += w * Jg;
             }
-            ek += kv * wJ;
-            vol += wJ * det3(F);
         }
+        if (flags & (MPS_E_KIN | MPS_E_KIN_NOW)) ek = hex_kinetic(I, nd, rho, wJ);
     } else {
-        double x[4][3], uu[4][3], vv[4][3];
-        for (int a = 0; a < 4; a++) {
-            int nd = I.conn[(size_t)e * 4 + a];
-            for (int d = 0; d < 3; d++) {
-                x[a][d] = I.X[d * N + nd];
-                uu[a][d] = I.u[d * N + nd];
-                vv[a][d] = I.v[d * N + nd];
-            }
-        }
-        double Jm[9], Du[9], iJ[9], F[9];
-        for (int i = 0; i < 3; i++)
-            for (int j = 0; j < 3; j++) {
-                Jm[i * 3 + j] = x[j + 1][i] - x[0][i];
-                Du[i * 3 + j] = uu[j + 1][i] - uu[0][i];
-            }
-        double dJ = det3(Jm);
-        inv3(Jm, iJ, dJ);
-        double wJ = MPS_TET_W * fabs(dJ);
-        for (int i = 0; i < 3; i++)
-            for (int j = 0; j < 3; j++)
-                F[i * 3 + j] = (i == j ? 1.0 : 0.0) + Du[i * 3] * iJ[j] + Du[i * 3 + 1] * iJ[3 + j] + Du[i * 3 + 2] * iJ[6 + j];
-        ed = defo_energy(LAW, F, pa, pb) * wJ;
-        double vel[3] = {0.0, 0.0, 0.0};
+        const int4 q = *reinterpret_cast<const int4*>(I.conn + (size_t)e * 4);
+        const int nd[4] = {q.x, q.y, q.z, q.w};
+        double x[4][3], uu[4][3], f[4][3], r[4];
+#pragma unroll
         for (int a = 0; a < 4; a++)
-            for (int j = 0; j < 3; j++) vel[j] += 0.25 * vv[a][j];
-        double kv = 0.0;
-        for (int j = 0; j < 3; j++) kv += 0.5 * rho * vel[j] * vel[j];
-        ek = kv * wJ;
-        vol = wJ * det3(F);
+#pragma unroll
+            for (int d = 0; d < 3; d++) {
+                x[a][d] = I.X[d * N + nd[a]];
+                uu[a][d] = I.u[d * N + nd[a]];
+            }
+        tet4_core<double, LAW, false, true>(I, e, nd, x, uu, tet_consts(I, e, false), f, false, 0.0, r);
+        ed = r[0];
+        vol = r[1];
+        if (flags & (MPS_E_KIN | MPS_E_KIN_NOW)) ek = tet_kinetic(I, nd, rho, r[3]);
     }
     out[0] = ed;
-    out[1] = ek;
-    out[2] = vol;
+    out[1] = vol;
+    out[2] = ek;
 }
 
 // ---------------------------------------------------------------------------
@@ -888,30 +986,33 @@ __global__ __launch_bounds__(256) void k_elem_outputs(MpsInst I, double* __restr
     for (int i = 0; i < 21; i++) out[(size_t)e * 21 + i] = acc[i] / (double)ngp;
 }
 
+// Standalone energy pass: between the element and node kernels of step k = ctl->B (flags MPS_E_DEF: DEFO_ENERGY
+// and VOLUME of step k; MPS_E_KIN: KIN_ENERGY of step k-1, whose velocity v still holds), or after the last node
+// kernel of a launch (MPS_E_KIN_NOW: KIN_ENERGY of the last step that ran, k = ctl->A - 1, when it is an energy
+// step -- in the time-window mode the solve may end before the launch's last step).
 template <int LAW>
-__global__ __launch_bounds__(256) void k_energy(MpsInst I) {
-    __shared__ double sh[3 * 256];
-    const long long k = I.ctl->B;
-    if (!I.ctl->slot[k & 1].go) return;
-    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ __launch_bounds__(MPS_PATCH) void k_energy(MpsInst I, unsigned flags) {
+    __shared__ double sh[3 * MPS_PATCH];
+    const long long k = (flags & MPS_E_KIN_NOW) ? I.ctl->A - 1 : I.ctl->B;
+    if (k < 1 || !I.ctl->slot[k & 1].go) return;
+    if ((flags & MPS_E_KIN_NOW) && k % I.energy_every != 0) return;
+    const int e = blockIdx.x * MPS_PATCH + threadIdx.x;  // n_patch blocks: the element phase's block sums
     double vals[3] = {0.0, 0.0, 0.0};
-    if (e < I.Ne) elem_energy<LAW>(I, e, vals);
-    block_sum<3>(vals, sh);
-    if (threadIdx.x == 0)
-        for (int j = 0; j < 3; j++) I.partials[(size_t)blockIdx.x * 3 + j] = vals[j];
+    if (e < I.Ne) elem_energy<LAW>(I, e, flags, vals);
+    publish_block_sum<3>(vals, I.epart, sh);
 }
 
-__global__ __launch_bounds__(256) void k_energy_reduce(MpsInst I, int nb) {
-    __shared__ double sh[3 * 256];
-    const long long k = I.ctl->B;
-    if (!I.ctl->slot[k & 1].go) return;
-    double vals[3] = {0.0, 0.0, 0.0};
-    for (int b = threadIdx.x; b < nb; b += blockDim.x)
-        for (int j = 0; j < 3; j++) vals[j] += I.partials[(size_t)b * 3 + j];
-    block_sum<3>(vals, sh);
-    if (threadIdx.x == 0 && I.hist_cap > 0) {
-        size_t idx = (size_t)((k - 1) % I.hist_cap);
-        for (int j = 0; j < 3; j++) I.hist[(size_t)(2 + j) * I.hist_cap + idx] = vals[j];
+// End of a launch (one block), after the MPS_E_KIN_NOW pass: the node sums of the last step that ran,
+// k = ctl->A - 1, and its KIN_ENERGY when it is an energy step.
+__global__ __launch_bounds__(256) void k_energy_tail(MpsInst I) {
+    __shared__ double sh[MPS_NODE_PARTS * 256];
+    const long long k = I.ctl->A - 1;
+    if (k < 1 || I.hist_cap <= 0) return;
+    node_sums_history(I, k, I.epart + 3 * (size_t)I.n_patch, (I.N + 255) / 256, sh);
+    if (k % I.energy_every == 0 && I.ctl->slot[k & 1].go) {  // the conditions of the pass before
+        double t[3];
+        reduce_parts<3>(I.epart, I.n_patch, t, sh);
+        if (threadIdx.x == 0) *hist_at(I, MPS_H_EKIN, k) = t[2];
     }
 }
 
@@ -922,52 +1023,66 @@ namespace mpsk {
 
 static inline unsigned blocks_for(long long n, int bs) { return (unsigned)((n + bs - 1) / bs); }
 
-hipError_t launch_elem(const MpsInst& I, int mode, hipStream_t st) {
+bool elem_fuses_energy(const MpsInst& I) { return I.precision == 0 && I.uprev == nullptr; }
+
+hipError_t launch_elem(const MpsInst& I, int mode, hipStream_t st, unsigned eflags) {
     const unsigned nb = (unsigned)I.n_patch;
     if (nb == 0) return hipSuccess;
     const bool visc = mode == 0 && I.uprev != nullptr;
-#define MPS_ELEM_LAW(T, NEN, V)                                                      \
-    switch (I.law) {                                                                 \
-        case 0: k_elem<T, NEN, 0, V><<<nb, MPS_PATCH, 0, st>>>(I, mode); break;      \
-        case 1: k_elem<T, NEN, 1, V><<<nb, MPS_PATCH, 0, st>>>(I, mode); break;      \
-        case 2: k_elem<T, NEN, 2, V><<<nb, MPS_PATCH, 0, st>>>(I, mode); break;      \
-        default: k_elem<T, NEN, 3, V><<<nb, MPS_PATCH, 0, st>>>(I, mode); break;     \
+    const bool en = mode == 0 && eflags != 0u && elem_fuses_energy(I);
+#define MPS_ELEM_LAW(T, NEN, V, E)                                                            \
+    switch (I.law) {                                                                          \
+        case 0: k_elem<T, NEN, 0, V, E><<<nb, MPS_PATCH, 0, st>>>(I, mode, eflags); break;    \
+        case 1: k_elem<T, NEN, 1, V, E><<<nb, MPS_PATCH, 0, st>>>(I, mode, eflags); break;    \
+        case 2: k_elem<T, NEN, 2, V, E><<<nb, MPS_PATCH, 0, st>>>(I, mode, eflags); break;    \
+        default: k_elem<T, NEN, 3, V, E><<<nb, MPS_PATCH, 0, st>>>(I, mode, eflags); break;   \
     }
-#define MPS_ELEM_LAUNCH(T, NEN)                      \
-    do {                                             \
-        if (visc) {                                  \
-            MPS_ELEM_LAW(T, NEN, true)               \
-        } else {                                     \
-            MPS_ELEM_LAW(T, NEN, false)              \
-        }                                            \
-    } while (0)
     if (I.nen == 8) {
-        if (I.precision == 1) MPS_ELEM_LAUNCH(float, 8);
-        else MPS_ELEM_LAUNCH(double, 8);
+        if (I.precision == 1) {
+            if (visc) { MPS_ELEM_LAW(float, 8, true, false) } else { MPS_ELEM_LAW(float, 8, false, false) }
+        } else if (visc) {
+            MPS_ELEM_LAW(double, 8, true, false)
+        } else if (en) {
+            MPS_ELEM_LAW(double, 8, false, true)
+        } else {
+            MPS_ELEM_LAW(double, 8, false, false)
+        }
     } else if (!visc && I.pnd != nullptr) {
         const size_t lds = 48 * (size_t)I.pu_max + 12 * MPS_PATCH * (I.precision == 1 ? 4 : 8) + 8 * MPS_PATCH +
                            2 * PatchList<4>::SPR;
-#define MPS_TET_STAGED(T)                                                                        \
-    switch (I.law) {                                                                             \
-        case 0: k_elem_tet_staged<T, 0><<<nb, MPS_PATCH, lds, st>>>(I, mode); break;             \
-        case 1: k_elem_tet_staged<T, 1><<<nb, MPS_PATCH, lds, st>>>(I, mode); break;             \
-        case 2: k_elem_tet_staged<T, 2><<<nb, MPS_PATCH, lds, st>>>(I, mode); break;             \
-        default: k_elem_tet_staged<T, 3><<<nb, MPS_PATCH, lds, st>>>(I, mode); break;            \
+#define MPS_TET_STAGED(T, E)                                                                             \
+    switch (I.law) {                                                                                     \
+        case 0: k_elem_tet_staged<T, 0, E><<<nb, MPS_PATCH, lds, st>>>(I, mode, eflags); break;          \
+        case 1: k_elem_tet_staged<T, 1, E><<<nb, MPS_PATCH, lds, st>>>(I, mode, eflags); break;          \
+        case 2: k_elem_tet_staged<T, 2, E><<<nb, MPS_PATCH, lds, st>>>(I, mode, eflags); break;          \
+        default: k_elem_tet_staged<T, 3, E><<<nb, MPS_PATCH, lds, st>>>(I, mode, eflags); break;         \
     }
-        if (I.precision == 1) MPS_TET_STAGED(float)
-        else MPS_TET_STAGED(double)
+        if (I.precision == 1) {
+            MPS_TET_STAGED(float, false)
+        } else if (en) {
+            MPS_TET_STAGED(double, true)
+        } else {
+            MPS_TET_STAGED(double, false)
+        }
 #undef MPS_TET_STAGED
     } else {
-        if (I.precision == 1) MPS_ELEM_LAUNCH(float, 4);
-        else MPS_ELEM_LAUNCH(double, 4);
+        if (I.precision == 1) {
+            if (visc) { MPS_ELEM_LAW(float, 4, true, false) } else { MPS_ELEM_LAW(float, 4, false, false) }
+        } else if (visc) {
+            MPS_ELEM_LAW(double, 4, true, false)
+        } else if (en) {
+            MPS_ELEM_LAW(double, 4, false, true)
+        } else {
+            MPS_ELEM_LAW(double, 4, false, false)
+        }
     }
-#undef MPS_ELEM_LAUNCH
 #undef MPS_ELEM_LAW
     return hipGetLastError();
 }
 
-hipError_t launch_node(const MpsInst& I, int fuse, hipStream_t st) {
-    k_node<<<blocks_for(I.N, 256), 256, 0, st>>>(I, fuse);
+hipError_t launch_node(const MpsInst& I, int fuse, hipStream_t st, unsigned nflags) {
+    if (I.energy_every > 0) k_node<true><<<blocks_for(I.N, 256), 256, 0, st>>>(I, fuse, nflags);
+    else k_node<false><<<blocks_for(I.N, 256), 256, 0, st>>>(I, fuse, 0u);
     return hipGetLastError();
 }
 
@@ -1018,8 +1133,6 @@ hipError_t launch_fint(const MpsInst& I, double* fint, hipStream_t st) {
     return hipGetLastError();
 }
 
-int energy_blocks(const MpsInst& I) { return (int)blocks_for(I.Ne, 256); }
-
 hipError_t launch_elem_outputs(const MpsInst& I, double* out, hipStream_t st) {
     k_elem_outputs<<<blocks_for(I.Ne, 256), 256, 0, st>>>(I, out);
     return hipGetLastError();
@@ -1035,17 +1148,20 @@ hipError_t launch_init_ctl(const MpsInst& I, const unsigned long long* dt0, hipS
     return hipGetLastError();
 }
 
-hipError_t launch_energy(const MpsInst& I, hipStream_t st) {
-    const int nb = energy_blocks(I);
+hipError_t launch_energy(const MpsInst& I, unsigned flags, hipStream_t st) {
+    const unsigned nb = (unsigned)I.n_patch;
+    if (nb == 0) return hipSuccess;
     switch (I.law) {  // one instantiation per law: a runtime choice would size registers for every law
-        case 0: k_energy<0><<<nb, 256, 0, st>>>(I); break;
-        case 1: k_energy<1><<<nb, 256, 0, st>>>(I); break;
-        case 2: k_energy<2><<<nb, 256, 0, st>>>(I); break;
-        default: k_energy<3><<<nb, 256, 0, st>>>(I); break;
+        case 0: k_energy<0><<<nb, MPS_PATCH, 0, st>>>(I, flags); break;
+        case 1: k_energy<1><<<nb, MPS_PATCH, 0, st>>>(I, flags); break;
+        case 2: k_energy<2><<<nb, MPS_PATCH, 0, st>>>(I, flags); break;
+        default: k_energy<3><<<nb, MPS_PATCH, 0, st>>>(I, flags); break;
     }
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
-    k_energy_reduce<<<1, 256, 0, st>>>(I, nb);
+    return hipGetLastError();
+}
+
+hipError_t launch_energy_tail(const MpsInst& I, hipStream_t st) {
+    k_energy_tail<<<1, 256, 0, st>>>(I);
     return hipGetLastError();
 }
 

@@ -163,15 +163,14 @@ int classSolverExplicit::calculate(const InpModel &m, const std::vector<std::uni
         forceVal.resize(forceDof.size());
     }
     const double dtMax = dt0;
-    // external force and EXTERNAL_ENERGY = sum_i u_i fext_i (dataExtraction.cpp:866-874): zero without loads
+    // external force at the extracted nodes (zero without loads); EXTERNAL_ENERGY = sum_i u_i fext_i
+    // (dataExtraction.cpp:866-874) comes with the other energies from the device history
     const bool needFext = !ld.empty() && extract.needsExternalForce();
-    const bool extEnergy = !ld.empty() && extract.needsExternalEnergy();
-    std::vector<double> picked(pickDof.size()), fintPicked(fintDof.size()), uFull, fext;
-    if (needFext || extEnergy) uFull.resize(3 * (size_t)N);
-    if (needFext || extEnergy) fext.resize(3 * (size_t)N);
-    std::vector<double> ht(kHistoryCap), hdef(kHistoryCap), hkin(kHistoryCap), hvol(kHistoryCap);
+    std::vector<double> picked(pickDof.size()), fintPicked(fintDof.size()), fext;
+    if (needFext) fext.resize(3 * (size_t)N);
+    std::vector<double> ht(kHistoryCap), hdef(kHistoryCap), hkin(kHistoryCap), hvol(kHistoryCap), hext(kHistoryCap);
 
-    const long long chunk = (nodal || extEnergy) ? 1 : (extract.needsEnergies() ? kHistoryCap : 1LL << 40);
+    const long long chunk = nodal ? 1 : (extract.needsEnergies() ? kHistoryCap : 1LL << 40);
     // VTK series (outputsManagement, solverExplicit.cpp:173-186, 310-322): at t0 when t0 < 1e-12, then
     // whenever `sampler` steps have passed since the last output, and at the last step.
     VtkSeriesWriter *vtk = opt.vtk;
@@ -216,22 +215,20 @@ int classSolverExplicit::calculate(const InpModel &m, const std::vector<std::uni
                 return -1;
             }
         }
-        double eext = 0.0;
-        if (done > 0 && (needFext || extEnergy)) {
-            if (mps_get_state(h, uFull.data(), nullptr, nullptr) != MPS_OK) return fail("mps_get_state");
-            if (mps_external_force(h, rep.time, fext.data()) != MPS_OK) return fail("mps_external_force");
-            for (size_t i = 0; i < fext.size(); i++) eext += uFull[i] * fext[i];
-        }
+        if (done > 0 && needFext && mps_external_force(h, rep.time, fext.data()) != MPS_OK)
+            return fail("mps_external_force");
         if (done > 0 && extract.needsEnergies()) {
             if (mps_get_history(h, before, done, ht.data(), nullptr, hdef.data(), hkin.data(), hvol.data()) != MPS_OK)
                 return fail("mps_get_history");
+            if (mps_get_energy_balance(h, before, done, hext.data(), nullptr, nullptr, nullptr) != MPS_OK)
+                return fail("mps_get_energy_balance");
             for (long long k = 0; k < done; k++)
                 for (auto &kv : extract.elementData()) {
                     const std::string &t = kv.second->getDataType();
                     const double v = t == "DEFO_ENERGY"  ? hdef[k]
                                      : t == "KIN_ENERGY" ? hkin[k]
                                      : t == "VOLUME"     ? hvol[k]
-                                                         : eext;  // EXTERNAL_ENERGY (chunks of one step with loads)
+                                                         : hext[k];  // EXTERNAL_ENERGY
                     if (!kv.second->writeData(opt.outputDir, ht[k], v)) {
                         err = "cannot write " + kv.first + ".csv in " + opt.outputDir;
                         return -1;

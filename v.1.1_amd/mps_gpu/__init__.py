@@ -128,6 +128,7 @@ def load(path=None):
     L.mps_get_state.argtypes = [H, _dp, _dp, _dp]
     L.mps_run.argtypes = [H, C.c_int64, C.POINTER(Report)]
     L.mps_get_history.argtypes = [H, C.c_int64, C.c_int64, _dp, _dp, _dp, _dp, _dp]
+    L.mps_get_energy_balance.argtypes = [H, C.c_int64, C.c_int64, _dp, _dp, _dp, _dp]
     L.mps_get_mass.argtypes = [H, _dp]
     L.mps_internal_force.argtypes = [H, _dp, _dp]
     L.mps_critical_dt.argtypes = [H, _dp, _dp]
@@ -338,6 +339,12 @@ class Solver:
         out = [np.zeros(count) for _ in range(5)]
         self._chk(self.L.mps_get_history(self.h, first, count, *[_d(x) for x in out]))
         return dict(zip(("time", "dt", "e_def", "e_kin", "volume"), out))
+
+    def energy_balance(self, first, count):
+        """e_ext (u . fext), e_kin_lumped, w_ext, w_react of steps first+1 .. first+count (mps_get_energy_balance)."""
+        out = [np.zeros(count) for _ in range(4)]
+        self._chk(self.L.mps_get_energy_balance(self.h, first, count, *[_d(x) for x in out]))
+        return dict(zip(("e_ext", "e_kin_lumped", "w_ext", "w_react"), out))
 
     def element_outputs(self):
         """Per element [n_elem, 21]: GP-averaged Cauchy (9), strain (9), VMS, volumetric, equivalent strain."""

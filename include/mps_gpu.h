@@ -258,9 +258,10 @@ int mps_critical_dt(mps_handle h, const double *u, double *dt);
  * solver destructor (solverExplicit.cpp:360; main.cpp:183-185). */
 int mps_destroy(mps_handle h);
 
-/* Graph replay of step chunks (default on): when enabled and no energies or kernel timing
- * are requested, mps_run replays each chunk of steps from a captured hipGraph instead of
- * launching its kernels one by one.  No reference counterpart. */
+/* Graph replay of step chunks (default on): when enabled and kernel timing is off, mps_run
+ * replays each chunk of steps from a captured hipGraph (one per chunk length and, with energies,
+ * per first step modulo energy_every) instead of launching its kernels one by one.  No reference
+ * counterpart. */
 int mps_set_graphs(mps_handle h, int32_t enable);
 
 /* Capture and instantiate, without running anything, the graphs that mps_run(h, max_steps) would

@@ -477,13 +477,6 @@ __device__ __forceinline__ double tet_char_length(const double (*x)[3]) {
 // FP32 characteristic lengths for the precision form (the reference formulas with FP32 areas, volume and
 // ratio, hardware square root and reciprocal).  Tet: edge vectors differenced in FP64, so there is no
 // cancellation against the absolute coordinates; hex: from node positions relative to their centroid.
-__device__ __forceinline__ float area_tri_f32(const double* p, const double* q, const double* r) {
-    const float A0 = (float)(p[0] - r[0]), A1 = (float)(p[1] - r[1]), A2 = (float)(p[2] - r[2]);
-    const float B0 = (float)(q[0] - r[0]), B1 = (float)(q[1] - r[1]), B2 = (float)(q[2] - r[2]);
-    const float C0 = A1 * B2 - A2 * B1, C1 = A2 * B0 - A0 * B2, C2 = A0 * B1 - A1 * B0;
-    return 0.5f * __builtin_amdgcn_sqrtf(C0 * C0 + C1 * C1 + C2 * C2);  // hardware sqrt, 1 ulp
-}
-
 __device__ __forceinline__ float area_tri_pf(const float* p, const float* q, const float* r) {
     const float A0 = p[0] - r[0], A1 = p[1] - r[1], A2 = p[2] - r[2];
     const float B0 = q[0] - r[0], B1 = q[1] - r[1], B2 = q[2] - r[2];
@@ -506,14 +499,28 @@ __device__ __forceinline__ float hex_char_length_pf(const float (*x)[3]) {
     return 6.0f * fabsf(det3(M)) * __builtin_amdgcn_rcpf(A);
 }
 
-__device__ __forceinline__ float tet_char_length_f32(const double (*x)[3]) {
-    const float A = area_tri_f32(x[0], x[1], x[2]) + area_tri_f32(x[1], x[2], x[3]) + area_tri_f32(x[0], x[2], x[3]) +
-                    area_tri_f32(x[0], x[1], x[3]);
-    float M[9];
+// Tet: from the deformed edge vectors e_j = x_{j+1} - x_0 = Jm(:,j) + Du(:,j), whose parts the element already
+// differenced in FP64: faces (0,1,2), (1,2,3), (0,2,3), (0,1,3) as |e1 x e2|, |(e2-e1) x (e3-e1)|, |e2 x e3|,
+// |e1 x e3| over 2, and 6V = |det(e1, e2, e3)|.
+__device__ __forceinline__ float cross_norm_f32(const float* a, const float* b) {
+    const float c0 = a[1] * b[2] - a[2] * b[1], c1 = a[2] * b[0] - a[0] * b[2], c2 = a[0] * b[1] - a[1] * b[0];
+    return __builtin_amdgcn_sqrtf(c0 * c0 + c1 * c1 + c2 * c2);  // hardware sqrt, 1 ulp
+}
+
+__device__ __forceinline__ float tet_char_length_f32(const float* Jm, const float* Du) {
+    float e[3][3], f[2][3];
 #pragma unroll
     for (int j = 0; j < 3; j++)
 #pragma unroll
-        for (int d = 0; d < 3; d++) M[j * 3 + d] = (float)(x[j + 1][d] - x[0][d]);
+        for (int d = 0; d < 3; d++) e[j][d] = Jm[d * 3 + j] + Du[d * 3 + j];
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        f[0][d] = e[1][d] - e[0][d];
+        f[1][d] = e[2][d] - e[0][d];
+    }
+    const float A = 0.5f * (cross_norm_f32(e[0], e[1]) + cross_norm_f32(f[0], f[1]) + cross_norm_f32(e[1], e[2]) +
+                            cross_norm_f32(e[0], e[2]));
+    const float M[9] = {e[0][0], e[0][1], e[0][2], e[1][0], e[1][1], e[1][2], e[2][0], e[2][1], e[2][2]};
     return fabsf(det3(M)) * __builtin_amdgcn_rcpf(A);
 }
 
@@ -935,11 +942,16 @@ __device__ __forceinline__ double tet4_core(const MpsInst& I, int e, const int* 
         }
     double dte = bits2d(0x7FF0000000000000ull);
     if (want_dt) {
+        double l;
+        if constexpr (sizeof(T) == 8) {
 #pragma unroll
-        for (int a = 0; a < 4; a++)
+            for (int a = 0; a < 4; a++)
 #pragma unroll
-            for (int d = 0; d < 3; d++) x[a][d] = x[a][d] + uu[a][d];
-        const double l = (sizeof(T) == 8) ? tet_char_length(x) : (double)tet_char_length_f32(x);
+                for (int d = 0; d < 3; d++) x[a][d] = x[a][d] + uu[a][d];
+            l = tet_char_length(x);
+        } else {
+            l = (double)tet_char_length_f32(Jm, Du);
+        }
         dte = elem_dt(l, k.l0, c);
     }
     T iJ[9], H[9], P[9];
@@ -1394,23 +1406,28 @@ __device__ __forceinline__ unsigned long long block_min_u64(unsigned long long v
     return v;
 }
 
-// Deterministic block sum of NV doubles (fixed tree order); result in thread 0.
+// Deterministic block sum of NV doubles, result in thread 0: a fixed butterfly within each wave, then thread 0
+// adds the waves' sums in wave order.  One barrier; sh holds NV doubles per wave.  A caller that reuses sh
+// right after must synchronise first (thread 0 may still be reading it).
 template <int NV>
-__device__ __forceinline__ void block_sum(double* vals, double* sh /*[NV*blockDim]*/) {
-    const int t = threadIdx.x;
+__device__ __forceinline__ void block_sum(double* vals, double* sh) {
 #pragma unroll
-    for (int j = 0; j < NV; j++) sh[j * blockDim.x + t] = vals[j];
-    __syncthreads();
-    for (int s = blockDim.x >> 1; s > 0; s >>= 1) {
-        if (t < s) {
+    for (int off = 32; off > 0; off >>= 1)
 #pragma unroll
-            for (int j = 0; j < NV; j++) sh[j * blockDim.x + t] += sh[j * blockDim.x + t + s];
-        }
-        __syncthreads();
+        for (int j = 0; j < NV; j++) vals[j] += __shfl_xor(vals[j], off, 64);
+    const int w = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
+    if ((threadIdx.x & 63) == 0) {
+#pragma unroll
+        for (int j = 0; j < NV; j++) sh[w * NV + j] = vals[j];
     }
-    if (t == 0) {
+    __syncthreads();
+    if (threadIdx.x == 0) {
 #pragma unroll
-        for (int j = 0; j < NV; j++) vals[j] = sh[j * blockDim.x];
+        for (int j = 0; j < NV; j++) {
+            double t = sh[j];
+            for (int q = 1; q < nw; q++) t += sh[q * NV + j];
+            vals[j] = t;
+        }
     }
 }
 
@@ -1428,15 +1445,27 @@ __device__ __forceinline__ void publish_block_sum(double* vals, double* __restri
 }
 
 // Sum of the nb block sums part[b*NV + j] by the calling block (every thread calls it): each thread adds the
-// blocks b = threadIdx.x + i*blockDim.x in order, then the fixed tree of block_sum.  Totals in out of thread 0.
+// blocks b = threadIdx.x + i*blockDim.x in order (loaded 8 at a time, so that 8 loads are in flight), then
+// block_sum.  Totals in out of thread 0.
 template <int NV>
 __device__ __forceinline__ void reduce_parts(const double* __restrict__ part, int nb, double* out, double* sh) {
+    constexpr int UN = 8;
     double acc[NV];
 #pragma unroll
     for (int j = 0; j < NV; j++) acc[j] = 0.0;
-    for (int b = threadIdx.x; b < nb; b += blockDim.x)
+    for (int b0 = threadIdx.x; b0 < nb; b0 += UN * blockDim.x) {
+        double v[UN][NV];
 #pragma unroll
-        for (int j = 0; j < NV; j++) acc[j] += part[(size_t)b * NV + j];
+        for (int q = 0; q < UN; q++) {
+            const int b = b0 + q * blockDim.x;
+#pragma unroll
+            for (int j = 0; j < NV; j++) v[q][j] = b < nb ? part[(size_t)b * NV + j] : 0.0;
+        }
+#pragma unroll
+        for (int q = 0; q < UN; q++)
+#pragma unroll
+            for (int j = 0; j < NV; j++) acc[j] += v[q][j];
+    }
     __syncthreads();  // sh may still be read by an earlier block_sum
     block_sum<NV>(acc, sh);
     if (threadIdx.x == 0) {

@@ -202,21 +202,31 @@ __global__ __launch_bounds__(MPS_PATCH) __attribute__((amdgpu_waves_per_eu(VISC 
     }
 }
 
+// Bytes of the tet4 kernel's LDS space shared by the node staging (48 per node) and the force slots
+// (12*MPS_PATCH values of tsize bytes); a multiple of 16.
+__host__ __device__ __forceinline__ size_t tet_shared_bytes(int pu, int tsize) {
+    const size_t a = 48 * (size_t)pu, b = 12 * (size_t)MPS_PATCH * tsize;
+    return a > b ? a : b;
+}
+
 // Tet4 element kernel with node staging (inviscid).  Each block first loads the U_p distinct nodes of
 // its patch (ascending ids, so the loads run along node rows) into LDS, X and u SoA by direction; each
 // element then reads its four nodes from LDS through its local indices plx.  Every node is fetched
 // once per patch instead of once per element slot (about 1 against 5.8 node loads per partial on a
-// structured tet mesh).  Dynamic LDS: 48*pu_max bytes of staging, then the 12*MPS_PATCH force slots
-// of T, the 4*MPS_PATCH entries of the contribution list and the staged node offsets.
+// structured tet mesh).  Dynamic LDS: the staging (48*pu_max bytes) and, after a barrier, the 12*MPS_PATCH
+// force slots of T in the same space, then the 4*MPS_PATCH entries of the contribution list and the staged
+// node offsets; the elements hold their forces in registers across that barrier.  Sharing the space lets
+// 5 (FP64) or 8 (FP32) workgroups reside per CU instead of 4 and 6.
 template <typename T, int LAW, bool EN = false>
-__global__ __launch_bounds__(MPS_PATCH) void k_elem_tet_staged(MpsInst I, int mode, unsigned eflags) {
+__global__ __launch_bounds__(MPS_PATCH) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 5 : 4))) void k_elem_tet_staged(MpsInst I, int mode, unsigned eflags) {
     extern __shared__ __align__(16) double smem[];  // 16-byte pieces are stored at offsets that are multiples of 16
     __shared__ unsigned long long sh[MPS_PATCH / 64];
     const int pu = I.pu_max;
-    double* __restrict__ sxu = smem;                                  // [6][pu]: X (3), then u (3)
-    T* __restrict__ sf = reinterpret_cast<T*>(smem + 6 * (size_t)pu);  // [12][MPS_PATCH]
-    uint16_t* __restrict__ spc = reinterpret_cast<uint16_t*>(sf + 12 * MPS_PATCH);  // [4*MPS_PATCH]
-    uint16_t* __restrict__ spr = spc + 4 * MPS_PATCH;                                // [PatchList<4>::SPR]
+    double* __restrict__ sxu = smem;           // [6][pu]: X (3), then u (3)
+    T* __restrict__ sf = reinterpret_cast<T*>(smem);  // [12][MPS_PATCH], over the staging once it is read
+    const size_t shared_bytes = tet_shared_bytes(pu, sizeof(T));
+    uint16_t* __restrict__ spc = reinterpret_cast<uint16_t*>(reinterpret_cast<char*>(smem) + shared_bytes);
+    uint16_t* __restrict__ spr = spc + 4 * MPS_PATCH;  // [PatchList<4>::SPR]
     long long k = 0;
     if (mode == 0) {
         k = I.ctl->A;
@@ -245,11 +255,27 @@ __global__ __launch_bounds__(MPS_PATCH) void k_elem_tet_staged(MpsInst I, int mo
             sxu[(3 + d) * pu + j] = I.u[d * N + n];
         }
     }
+    pl.store(spc, spr, ne * 4);  // the lists arrive with the nodes; stored now, they hold no registers below
     __syncthreads();
     unsigned long long dmin = INF_BITS;
     double env[3] = {0.0, 0.0, 0.0};
+    T f[4][3];
     if (i < ne) {
         const int l[4] = {(int)(l4.x & 0xffffu), (int)(l4.x >> 16), (int)(l4.y & 0xffffu), (int)(l4.y >> 16)};
+        // FP64: the stable-dt length first, from the deformed positions (the reference's operations), then the
+        // element from a second read of the staged nodes (the memory clobber keeps the reads apart), so that the
+        // 24 node values are not held in registers through the element work
+        double dt64 = bits2d(INF_BITS);
+        if (sizeof(T) == 8 && mode == 0) {
+            double xd[4][3];
+#pragma unroll
+            for (int a = 0; a < 4; a++)
+#pragma unroll
+                for (int d = 0; d < 3; d++) xd[a][d] = sxu[d * pu + l[a]] + sxu[(3 + d) * pu + l[a]];
+            dt64 = elem_dt(tet_char_length(xd), kc.l0, kc.c);
+            asm volatile("" ::: "memory");
+        }
+        const bool core_dt = sizeof(T) == 4 && mode == 0;
         double x[4][3], uu[4][3];
 #pragma unroll
         for (int a = 0; a < 4; a++)
@@ -258,11 +284,10 @@ __global__ __launch_bounds__(MPS_PATCH) void k_elem_tet_staged(MpsInst I, int mo
                 x[a][d] = sxu[d * pu + l[a]];
                 uu[a][d] = sxu[(3 + d) * pu + l[a]];
             }
-        T f[4][3];
         double dte;
         if constexpr (EN) {
             double r[4];
-            dte = tet4_core<T, LAW, false, true>(I, e, nullptr, x, uu, kc, f, mode == 0, 0.0, r);
+            dte = tet4_core<T, LAW, false, true>(I, e, nullptr, x, uu, kc, f, core_dt, 0.0, r);
             env[0] = r[0];
             env[1] = r[1];
             if (eflags & MPS_E_KIN) {
@@ -270,15 +295,18 @@ __global__ __launch_bounds__(MPS_PATCH) void k_elem_tet_staged(MpsInst I, int mo
                 env[2] = tet_kinetic(I, nd, I.rho ? I.rho[e] : I.rho0, r[3]);
             }
         } else {
-            dte = tet4_core<T, LAW, false>(I, e, nullptr, x, uu, kc, f, mode == 0, 0.0);
+            dte = tet4_core<T, LAW, false>(I, e, nullptr, x, uu, kc, f, core_dt, 0.0);
         }
+        if (sizeof(T) == 8) dte = dt64;
+        dmin = d2bits(dte);
+    }
+    __syncthreads();  // every element has read its nodes: the staging space takes the forces
+    if (i < ne) {
 #pragma unroll
         for (int a = 0; a < 4; a++)
 #pragma unroll
             for (int d = 0; d < 3; d++) sf[(a * 3 + d) * MPS_PATCH + i] = f[a][d];
-        dmin = d2bits(dte);
     }
-    pl.store(spc, spr, ne * 4);
     __syncthreads();
     patch_sums<T, 4, MPS_PATCH, true>(I, p, sf, spc, spr + pl.off);
     if (mode == 0) {
@@ -1025,6 +1053,11 @@ static inline unsigned blocks_for(long long n, int bs) { return (unsigned)((n + 
 
 bool elem_fuses_energy(const MpsInst& I) { return I.precision == 0 && I.uprev == nullptr; }
 
+// Dynamic LDS of the staged tet4 kernel: node staging / force slots, contribution list, offsets.
+static size_t tet_lds(const MpsInst& I) {
+    return tet_shared_bytes(I.pu_max, I.precision == 1 ? 4 : 8) + 8 * MPS_PATCH + 2 * PatchList<4>::SPR;
+}
+
 hipError_t launch_elem(const MpsInst& I, int mode, hipStream_t st, unsigned eflags) {
     const unsigned nb = (unsigned)I.n_patch;
     if (nb == 0) return hipSuccess;
@@ -1048,8 +1081,7 @@ hipError_t launch_elem(const MpsInst& I, int mode, hipStream_t st, unsigned efla
             MPS_ELEM_LAW(double, 8, false, false)
         }
     } else if (!visc && I.pnd != nullptr) {
-        const size_t lds = 48 * (size_t)I.pu_max + 12 * MPS_PATCH * (I.precision == 1 ? 4 : 8) + 8 * MPS_PATCH +
-                           2 * PatchList<4>::SPR;
+        const size_t lds = tet_lds(I);
 #define MPS_TET_STAGED(T, E)                                                                             \
     switch (I.law) {                                                                                     \
         case 0: k_elem_tet_staged<T, 0, E><<<nb, MPS_PATCH, lds, st>>>(I, mode, eflags); break;          \

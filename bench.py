@@ -52,12 +52,12 @@ def measured_traffic(kernel):
 
 def measured_valu_per_wave(kernel):
     """VALU instructions per wave of `kernel` from the committed rocprofv3 SQ_INSTS_VALU/SQ_WAVES table
-    (| `name` | VALU instructions / wave | ...), or None."""
+    (| `name` | VALU instructions / wave | VALU busy | resident cycles | waiting |), or None."""
     try:
         with open(TRAFFIC_TABLE) as f:
             for line in f:
                 cells = [c.strip() for c in line.strip().strip("|").split("|")]
-                if len(cells) == 4 and cells[0] == f"`{kernel}`":
+                if len(cells) == 5 and cells[0] == f"`{kernel}`":
                     return float(cells[1].replace(",", ""))
     except OSError:
         pass
@@ -358,7 +358,7 @@ def main():
     eb = elem_kernel_bytes(ne, nn, 8, n_part, False)
     sb = step_bytes(ne, nn, 8, n_part, False)
     achieved = eb / (elem_avg_ms * 1e-3) / 1e9
-    valu_wave = measured_valu_per_wave("k_elem<double,8,0,false>")
+    valu_wave = measured_valu_per_wave("k_elem<double,8,0,false,false>")
     n_waves = -(-ne // PATCH) * (PATCH // 64)
     valu_peak = VALU_ISSUE_PEAK  # wave64 VALU instructions per second (one per CU per cycle)
     valu = None if valu_wave is None else {
@@ -385,9 +385,9 @@ def main():
                                "dt=0.5 x stable dt, FP64",
                    "n_elem": ne, "n_nodes": nn, "gauss_points": 8, "form": "fp64"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic("k_elem<double,8,0,false>"),
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic("k_elem<double,8,0,false,false>"),
                      "traffic_source": os.path.relpath(TRAFFIC_TABLE, ROOT),
-                     "kernel": "k_elem<double,8,0,false> (hex8 element forces + stable dt)",
+                     "kernel": "k_elem<double,8,0,false,false> (hex8 element forces + stable dt)",
                      "kernel_avg_ms": elem_avg_ms, "algorithmic_bytes_per_launch": eb,
                      "measured_copy_ceiling_gbs": copy_gbs,
                      "binding_resource": "valu" if valu is not None and valu["frac"] > achieved / HBM_PEAK_GBS

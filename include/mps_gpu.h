@@ -241,6 +241,24 @@ int mps_get_energy_balance(mps_handle h, int64_t first, int64_t count, double *e
  * TENSORS Cauchy / E and SCALARS VMS / VolumetricStrain / EquivalentStrain sections. */
 int mps_element_outputs(mps_handle h, double *out);
 
+/* Output snapshots without stopping the solve.  mps_snapshot_begin(h, slot, with_elements) stages
+ * u, v, a of the last completed step (and, with with_elements != 0, the 21 element output fields of
+ * mps_element_outputs) on the device in stream order and starts their copy to pinned host memory on
+ * a second stream; it returns at once, and mps_run may be called again right away.
+ * mps_snapshot_wait(h, slot, &u, &v, &a, &elem) blocks until that copy is done and returns pointers
+ * to it (3*n_nodes doubles each, SoA as mps_get_state; elem 21*n_elem as mps_element_outputs, or
+ * NULL), valid until mps_snapshot_release(h, slot); a slot is begun again only after its release.
+ * Two slots (0, 1) let one snapshot be written out while the next is taken.  mps_snapshot_wait and
+ * mps_snapshot_release may run on another thread than the one calling mps_run; one slot's calls must
+ * not overlap.  The first begin on a slot allocates its buffers (device and pinned host,
+ * (9*n_nodes + 21*n_elem)*8 bytes each), kept until mps_destroy.
+ * Replaces: the synchronous copy of the solution before writeVTK / vtkPlotting2D3D at each output
+ * step (solverExplicit.cpp:310-322, output.cpp:296-918). */
+int mps_snapshot_begin(mps_handle h, int32_t slot, int32_t with_elements);
+int mps_snapshot_wait(mps_handle h, int32_t slot, const double **u, const double **v, const double **a,
+                      const double **elem);
+int mps_snapshot_release(mps_handle h, int32_t slot);
+
 /* Lumped mass per node (identical for x, y, z).
  * Replaces: classSolverExplicit::computeLumpedMassVector (solverExplicit.cpp:429-459). */
 int mps_get_mass(mps_handle h, double *mass);

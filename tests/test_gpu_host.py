@@ -8,6 +8,7 @@ import numpy as np
 import pytest
 
 import inpgen
+import mps_gpu as G
 import oracle_ctypes as O
 from mps_gpu import configs
 
@@ -326,6 +327,67 @@ def test_vtk_series_matches_state_and_oracle(tmp_path):
     assert not any(k.startswith("IntVars") for k in s)  # the second *Outputs names no internal variables
     _, s1, _ = parse_vtk(files[1])  # the first solver's *Outputs asked for variables 1 and 2
     assert s1["IntVars1"] == [["0"]] * Ne and s1["IntVars2"] == [["0"]] * Ne
+
+
+def test_vtk_on_a_mesh_formatted_in_parallel(tmp_path):
+    """A mesh above the writer's one-thread limit (20,000 nodes or elements per section): the sections are
+    formatted on several threads and the geometry once per run, from device snapshots taken while the solve
+    goes on.  Every file has the same POINTS and CELLS text, the numbers of each section in order, and the last
+    one holds the final state."""
+    m = configs.perturb(configs.box(30, 28, 26), 0.1, 17)
+    N, Ne = m["n_nodes"], m["n_elem"]
+    assert N > 20000 and Ne > 20000
+    dof, val = configs.clamp_bottom_pull_top(m, vel=0.05)
+    bc = inpgen.bc_triples(dof, val, N)
+    dt0 = O.OracleProblem(m, dir_dof=dof, dir_val=val, bc_mode=O.BC_RAMP, tf=1.0, scale=0.5).critical_dt()
+    inp = tmp_path / "big.inp"
+    inpgen.write_inp(inp, m, [dict(rho=1000.0, E=1e6, nu=0.3)],
+                     [dict(scale=0.5, t0=0.0, tf=40.5 * dt0, outputs=6, bc=bc)])
+    u, _ = run(inp, tmp_path)
+    files = sorted(tmp_path.glob("MuPhiSimOutputs_0_.*.vtk"), key=lambda p: int(p.name.split(".")[1]))
+    assert len(files) >= 4
+    X = m["X"].reshape(3, N)
+    conn = m["conn"].reshape(Ne, 8)
+    points = [[f"{X[d, n]:.6g}" for d in range(3)] for n in range(N)]
+    cells = [["8"] + [str(c) for c in conn[e]] for e in range(Ne)]
+    for f in files:
+        _, s, _ = parse_vtk(f)
+        assert s["POINTS"] == points and s["CELLS"] == cells, f.name
+        assert len(s["Displacement"]) == N and len(s["Cauchy"]) == 3 * Ne and len(s["Pressure"]) == Ne
+    _, s, _ = parse_vtk(files[-1])
+    U = u.reshape(3, N)
+    assert s["Displacement"] == [[f"{U[d, n]:.6g}" for d in range(3)] for n in range(N)]
+    vms = np.array([float(r[0]) for r in s["VMS"]])
+    assert np.all(np.isfinite(vms)) and vms.max() > 0
+
+
+def test_snapshots_taken_while_stepping():
+    """mps_snapshot_*: a snapshot holds the state of the step it was begun at although the solve went on before
+    it was read; two slots; a slot is begun again only after its release."""
+    m = configs.perturb(configs.box(6, 5, 4), 0.1, 3)
+    dof, val = configs.clamp_bottom_pull_top(m, vel=0.05)
+    with G.Solver(m, dir_dof=dof, dir_val=val, n_steps=200) as s:
+        s.run(30)
+        u1, v1, a1 = s.get_state()
+        e1 = s.element_outputs()
+        s.snapshot_begin(0)
+        s.run(40)
+        u2, v2, a2 = s.get_state()
+        s.snapshot_begin(1, with_elements=False)
+        with pytest.raises(G.MpsError):
+            s.snapshot_begin(0)
+        su, sv, sa, se = s.snapshot_wait(0)
+        assert np.array_equal(su, u1) and np.array_equal(sv, v1) and np.array_equal(sa, a1)
+        assert np.array_equal(se, e1)
+        s.snapshot_release(0)
+        tu, tv, ta, te = s.snapshot_wait(1)
+        assert np.array_equal(tu, u2) and np.array_equal(tv, v2) and np.array_equal(ta, a2) and te is None
+        assert not np.array_equal(u1, u2)
+        s.snapshot_release(1)
+        s.snapshot_begin(0)
+        s.run(10)
+        assert np.array_equal(s.snapshot_wait(0)[0], u2)
+        s.snapshot_release(0)
 
 
 def test_forces_output_against_the_oracle(tmp_path):

@@ -8,6 +8,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <climits>
 #include <cmath>
 #include <cstdarg>
@@ -317,6 +318,16 @@ struct mps_solver_s {
     size_t ev_used = 0;
     double kt_ms[4] = {0, 0, 0, 0};   // element ms, node ms, begin ms, energy ms
     long long kt_n[4] = {0, 0, 0, 0};
+    // output snapshots (mps_snapshot_*): two slots, each a device staging copy and a pinned host copy of
+    // u, v, a and the element outputs, filled on the solver stream and copied out on copy_stream
+    struct Snap {
+        double* dev = nullptr;
+        double* host = nullptr;
+        hipEvent_t staged = nullptr, copied = nullptr;
+        std::atomic<int> busy{0};  // begun and not yet released (mps_snapshot_release may run on another thread)
+        int with_elem = 0;
+    } snap[2];
+    hipStream_t copy_stream = nullptr;
     std::string err;
 
     int fail(int code, const std::string& m) {
@@ -360,6 +371,17 @@ static void release(mps_solver_s* h) {
     h->pick_idx = nullptr;
     h->pick_out = nullptr;
     h->eout = nullptr;
+    if (h->copy_stream) (void)hipStreamSynchronize(h->copy_stream);
+    for (auto& S : h->snap) {
+        if (S.dev) (void)hipFree(S.dev);
+        if (S.host) (void)hipHostFree(S.host);
+        if (S.staged) (void)hipEventDestroy(S.staged);
+        if (S.copied) (void)hipEventDestroy(S.copied);
+        S.dev = S.host = nullptr;
+        S.staged = S.copied = nullptr;
+    }
+    if (h->copy_stream) (void)hipStreamDestroy(h->copy_stream);
+    h->copy_stream = nullptr;
     if (h->ev0) (void)hipEventDestroy(h->ev0);
     if (h->ev1) (void)hipEventDestroy(h->ev1);
     if (h->stream) (void)hipStreamDestroy(h->stream);
@@ -1380,6 +1402,62 @@ extern "C" int mps_run(mps_handle h, int64_t max_steps, mps_report* rep) {
     h->part_valid = true;  // the partial forces are those of the last completed step's u
     if (rep) *rep = r;
     return status_code(r.status);
+}
+
+// Output snapshots: the state of the last completed step staged on the device in stream order (so later
+// steps may run at once), then copied to pinned host memory on a second stream.
+extern "C" int mps_snapshot_begin(mps_handle h, int32_t slot, int32_t with_elements) {
+    if (!h) return MPS_ERR_ARG;
+    if (slot < 0 || slot > 1) return h->fail(MPS_ERR_ARG, "mps_snapshot_begin: slot must be 0 or 1");
+    auto& S = h->snap[slot];
+    if (S.busy.load()) return h->fail(MPS_ERR_STATE, "mps_snapshot_begin: slot not released (mps_snapshot_release)");
+    (void)hipSetDevice(h->device);
+    const size_t n3 = 3 * (size_t)h->I.N, ne = 21 * (size_t)h->I.Ne;
+    const size_t cap = (3 * n3 + ne) * 8;
+    if (!S.dev) {
+        HCHK(h, hipMalloc(&S.dev, cap));
+        HCHK(h, hipHostMalloc((void**)&S.host, cap, hipHostMallocDefault));
+        HCHK(h, hipEventCreateWithFlags(&S.staged, hipEventDisableTiming));
+        HCHK(h, hipEventCreateWithFlags(&S.copied, hipEventDisableTiming));
+    }
+    if (!h->copy_stream) HCHK(h, hipStreamCreateWithFlags(&h->copy_stream, hipStreamNonBlocking));
+    const MpsInst& I = h->I;
+    HCHK(h, hipMemcpyAsync(S.dev, I.u, n3 * 8, hipMemcpyDeviceToDevice, h->stream));
+    HCHK(h, hipMemcpyAsync(S.dev + n3, I.v, n3 * 8, hipMemcpyDeviceToDevice, h->stream));
+    HCHK(h, hipMemcpyAsync(S.dev + 2 * n3, I.a, n3 * 8, hipMemcpyDeviceToDevice, h->stream));
+    if (with_elements) {
+        hipError_t e = mpsk::launch_elem_outputs(I, S.dev + 3 * n3, h->stream);
+        if (e != hipSuccess) return h->fail(MPS_ERR_HIP, fmt("element outputs: %s", hipGetErrorString(e)));
+    }
+    HCHK(h, hipEventRecord(S.staged, h->stream));
+    HCHK(h, hipStreamWaitEvent(h->copy_stream, S.staged, 0));
+    HCHK(h, hipMemcpyAsync(S.host, S.dev, (3 * n3 + (with_elements ? ne : 0)) * 8, hipMemcpyDeviceToHost,
+                           h->copy_stream));
+    HCHK(h, hipEventRecord(S.copied, h->copy_stream));
+    S.with_elem = with_elements ? 1 : 0;
+    S.busy.store(1);
+    return MPS_OK;
+}
+
+// May run on another thread than the one that steps the solver: it touches only the slot.
+extern "C" int mps_snapshot_wait(mps_handle h, int32_t slot, const double** u, const double** v, const double** a,
+                                 const double** elem) {
+    if (!h || slot < 0 || slot > 1) return MPS_ERR_ARG;
+    auto& S = h->snap[slot];
+    if (!S.busy.load()) return MPS_ERR_STATE;
+    if (hipEventSynchronize(S.copied) != hipSuccess) return MPS_ERR_HIP;
+    const size_t n3 = 3 * (size_t)h->I.N;
+    if (u) *u = S.host;
+    if (v) *v = S.host + n3;
+    if (a) *a = S.host + 2 * n3;
+    if (elem) *elem = S.with_elem ? S.host + 3 * n3 : nullptr;
+    return MPS_OK;
+}
+
+extern "C" int mps_snapshot_release(mps_handle h, int32_t slot) {
+    if (!h || slot < 0 || slot > 1) return MPS_ERR_ARG;
+    h->snap[slot].busy.store(0);
+    return MPS_OK;
 }
 
 extern "C" int mps_element_outputs(mps_handle h, double* out) {

@@ -2,29 +2,31 @@
 // for the keyword list and the reference functions each part follows).
 #include "inp_reader.h"
 
+#include <algorithm>
+#include <charconv>
 #include <cstdio>
 #include <cstdlib>
+#include <cstring>
 #include <fstream>
 #include <set>
 #include <sstream>
+#include <string_view>
+#include <thread>
 
 #include "mps_gpu.h"
 
 namespace mps {
 namespace {
 
-std::string rtrim(const std::string &s) {
-    size_t e = s.size();
-    while (e > 0 && (s[e - 1] == '\r' || s[e - 1] == ' ' || s[e - 1] == '\t')) e--;
-    return s.substr(0, e);
+bool starts(std::string_view line, const char *kw) {
+    const size_t n = std::strlen(kw);
+    return line.size() >= n && std::memcmp(line.data(), kw, n) == 0;
 }
 
-bool starts(const std::string &line, const char *kw) { return line.compare(0, std::char_traits<char>::length(kw), kw) == 0; }
-
 // Text after column `col` (the reference's line.compare(col, n, "...") tests).
-std::string tail(const std::string &line, size_t col) { return line.size() > col ? line.substr(col) : std::string(); }
+std::string tail(std::string_view line, size_t col) { return line.size() > col ? std::string(line.substr(col)) : std::string(); }
 
-std::vector<std::string> fields(const std::string &line) {
+std::vector<std::string> fields(std::string_view line) {
     std::vector<std::string> out;
     std::string cur;
     for (char c : line) {
@@ -39,25 +41,53 @@ std::vector<std::string> fields(const std::string &line) {
     return out;
 }
 
-bool isBlank(const std::string &s) { return s.find_first_not_of(" \t") == std::string::npos; }
+bool isBlank(std::string_view s) { return s.find_first_not_of(" \t") == std::string_view::npos; }
 
-// atoi/atof semantics (leading blanks skipped, trailing text ignored), with a
-// check that a number was present.
-bool toInt(const std::string &s, long long &v) {
-    const char *p = s.c_str();
-    char *end = nullptr;
-    v = std::strtoll(p, &end, 10);
-    return end != p;
+// Numeric fields of the node and element blocks, parsed in place (the blocks of a 1M-element deck hold some
+// 13 million numbers): atoi/atof semantics -- leading blanks and a '+' skipped, the number ends at the first
+// character that cannot continue it -- with std::from_chars, which rounds as strtod does.  *p advances past
+// the number; false when no number starts there.
+bool parseInt(const char *&p, const char *end, long long &v) {
+    while (p < end && (*p == ' ' || *p == '\t')) p++;
+    if (p < end && *p == '+') p++;
+    const auto r = std::from_chars(p, end, v);
+    if (r.ec != std::errc()) return false;
+    p = r.ptr;
+    return true;
 }
-bool toDouble(const std::string &s, double &v) {
-    const char *p = s.c_str();
-    char *end = nullptr;
-    v = std::strtod(p, &end);
-    return end != p;
+bool parseDouble(const char *&p, const char *end, double &v) {
+    while (p < end && (*p == ' ' || *p == '\t')) p++;
+    if (p < end && *p == '+') p++;
+    const auto r = std::from_chars(p, end, v);
+    if (r.ec != std::errc()) return false;
+    p = r.ptr;
+    return true;
+}
+// The field that starts at p ends at the next comma (or the line's end): skip what is left of it.
+bool nextField(const char *&p, const char *end) {
+    const void *c = std::memchr(p, ',', (size_t)(end - p));
+    if (!c) return false;
+    p = static_cast<const char *>(c) + 1;
+    return true;
+}
+// atoi/atof semantics on one field, with a check that a number was present.
+bool toInt(std::string_view s, long long &v) {
+    const char *p = s.data();
+    return parseInt(p, s.data() + s.size(), v);
+}
+bool toDouble(std::string_view s, double &v) {
+    const char *p = s.data();
+    return parseDouble(p, s.data() + s.size(), v);
+}
+int countCommas(std::string_view s) {
+    int n = 0;
+    for (char c : s) n += c == ',';
+    return n;
 }
 
 struct Reader {
-    std::vector<std::string> L;
+    std::string buf;                    // the file
+    std::vector<std::string_view> L;   // its lines, trailing blanks removed
     std::string err;
     std::string where(size_t i) const {
         std::ostringstream o;
@@ -70,6 +100,43 @@ struct Reader {
     }
 };
 
+// Data lines [first, last) of a large block parsed in chunks on up to 16 threads: parse(i, out) appends line i's
+// values to out (a blank line appends nothing) or returns an error message.  The chunks' values are joined in
+// line order; on errors the first failing line in file order is reported.
+template <typename T, typename F>
+bool parseBlock(Reader &R, size_t first, size_t last, std::vector<T> &vals, F parse) {
+    const size_t n = last - first;
+    const unsigned hw = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    const size_t nt = n < 65536 ? 1 : hw;
+    std::vector<std::vector<T>> part(nt);
+    std::vector<size_t> badLine(nt, SIZE_MAX);
+    std::vector<std::string> badMsg(nt);
+    auto work = [&](size_t t) {
+        const size_t b = first + n * t / nt, e = first + n * (t + 1) / nt;
+        part[t].reserve(e - b);
+        for (size_t i = b; i < e; i++) {
+            const char *msg = parse(i, part[t]);
+            if (msg) {
+                badLine[t] = i;
+                badMsg[t] = msg;
+                return;
+            }
+        }
+    };
+    std::vector<std::thread> th;
+    for (size_t t = 1; t < nt; t++) th.emplace_back(work, t);
+    work(0);
+    for (auto &x : th) x.join();
+    for (size_t t = 0; t < nt; t++)
+        if (badLine[t] != SIZE_MAX) return R.fail(badLine[t], badMsg[t]);
+    size_t total = 0;
+    for (auto &p : part) total += p.size();
+    vals.clear();
+    vals.reserve(total);
+    for (auto &p : part) vals.insert(vals.end(), p.begin(), p.end());
+    return true;
+}
+
 // nodesManagement (inputFile.cpp:20-111): first block after a line starting
 // "*Node", data until the next keyword; the comma count sets ndim.
 bool readNodes(Reader &R, InpModel &m) {
@@ -79,36 +146,51 @@ bool readNodes(Reader &R, InpModel &m) {
         R.err = "no *Node block";
         return false;
     }
-    std::vector<std::pair<long long, std::vector<double>>> rows;
-    for (i++; i < R.L.size() && !starts(R.L[i], "*"); i++) {
-        if (isBlank(R.L[i])) continue;
-        std::vector<std::string> f = fields(R.L[i]);
-        int nd = (int)f.size() - 1;
-        if (nd != 3) return R.fail(i, "this path is 3-D: a node line needs id, x, y, z");
-        if (m.ndim == 0) m.ndim = nd;
+    const size_t first = i + 1;
+    size_t last = first;
+    while (last < R.L.size() && !starts(R.L[last], "*")) last++;
+    struct NodeRow {
         long long id;
-        std::vector<double> x(3);
-        if (!toInt(f[0], id) || !toDouble(f[1], x[0]) || !toDouble(f[2], x[1]) || !toDouble(f[3], x[2]))
-            return R.fail(i, "malformed node line");
-        rows.emplace_back(id, x);
+        double x[3];
+    };
+    std::vector<NodeRow> rows;
+    const bool ok = parseBlock(R, first, last, rows, [&](size_t i, std::vector<NodeRow> &out) -> const char * {
+        std::string_view l = R.L[i];
+        if (isBlank(l)) return nullptr;
+        if (countCommas(l) != 3) return "this path is 3-D: a node line needs id, x, y, z";
+        const char *p = l.data(), *e = l.data() + l.size();
+        NodeRow r;
+        if (!parseInt(p, e, r.id) || !nextField(p, e) || !parseDouble(p, e, r.x[0]) || !nextField(p, e) ||
+            !parseDouble(p, e, r.x[1]) || !nextField(p, e) || !parseDouble(p, e, r.x[2]))
+            return "malformed node line";
+        out.push_back(r);
+        return nullptr;
+    });
+    if (!ok) return false;
+    std::vector<long long> ids(rows.size());
+    std::vector<double> xyz(3 * rows.size());
+    for (size_t r = 0; r < rows.size(); r++) {
+        ids[r] = rows[r].id;
+        for (int d = 0; d < 3; d++) xyz[3 * r + d] = rows[r].x[d];
     }
-    const long long N = (long long)rows.size();
+    const long long N = (long long)ids.size();
     if (N == 0) {
         R.err = "empty *Node block";
         return false;
     }
+    m.ndim = 3;
     m.nNodes = (int32_t)N;
     m.X.assign(3 * (size_t)N, 0.0);
     std::vector<char> seen((size_t)N, 0);
-    for (auto &r : rows) {
-        if (r.first < 1 || r.first > N)
-            return (R.err = "node id " + std::to_string(r.first) + " outside 1.." + std::to_string(N) +
+    for (size_t r = 0; r < ids.size(); r++) {
+        if (ids[r] < 1 || ids[r] > N)
+            return (R.err = "node id " + std::to_string(ids[r]) + " outside 1.." + std::to_string(N) +
                             " (node ids must number the nodes 1..N)",
                     false);
-        size_t k = (size_t)(r.first - 1);
-        if (seen[k]) return (R.err = "duplicate node id " + std::to_string(r.first), false);
+        size_t k = (size_t)(ids[r] - 1);
+        if (seen[k]) return (R.err = "duplicate node id " + std::to_string(ids[r]), false);
         seen[k] = 1;
-        for (int d = 0; d < 3; d++) m.X[(size_t)d * N + k] = r.second[d];
+        for (int d = 0; d < 3; d++) m.X[(size_t)d * N + k] = xyz[3 * r + d];
     }
     return true;
 }
@@ -130,39 +212,58 @@ bool readElements(Reader &R, InpModel &m) {
     else
         return R.fail(i, "element type \"" + type + "\" is not on this path (C3D8 and C3D4 are)");
     m.elemType = type;
-    std::vector<std::pair<long long, std::vector<int32_t>>> rows;
-    for (i++; i < R.L.size() && !starts(R.L[i], "*"); i++) {
-        if (isBlank(R.L[i])) continue;
-        std::vector<std::string> f = fields(R.L[i]);
-        if ((int)f.size() != m.nen + 1) return R.fail(i, "element line needs id and " + std::to_string(m.nen) + " nodes");
+    const size_t first = i + 1;
+    size_t last = first;
+    while (last < R.L.size() && !starts(R.L[last], "*")) last++;
+    struct ElemRow {
         long long id;
-        if (!toInt(f[0], id)) return R.fail(i, "malformed element id");
-        std::vector<int32_t> c(m.nen);
-        for (int a = 0; a < m.nen; a++) {
+        int32_t n[8];
+    };
+    const int nen = m.nen;
+    const long long nNodes = m.nNodes;
+    const std::string countMsg = "element line needs id and " + std::to_string(nen) + " nodes";
+    std::vector<ElemRow> rows;
+    const bool ok = parseBlock(R, first, last, rows, [&](size_t i, std::vector<ElemRow> &out) -> const char * {
+        std::string_view l = R.L[i];
+        if (isBlank(l)) return nullptr;
+        if (countCommas(l) != nen) return countMsg.c_str();
+        const char *p = l.data(), *e = l.data() + l.size();
+        ElemRow r;
+        if (!parseInt(p, e, r.id)) return "malformed element id";
+        for (int a = 0; a < nen; a++) {
             long long n;
-            if (!toInt(f[a + 1], n)) return R.fail(i, "malformed node id");
-            if (n < 1 || n > m.nNodes) return R.fail(i, "element node id outside 1..N");
-            c[a] = (int32_t)(n - 1);
+            if (!nextField(p, e) || !parseInt(p, e, n)) return "malformed node id";
+            if (n < 1 || n > nNodes) return "element node id outside 1..N";
+            r.n[a] = (int32_t)(n - 1);
         }
-        rows.emplace_back(id, c);
+        out.push_back(r);
+        return nullptr;
+    });
+    if (!ok) return false;
+    i = last;
+    std::vector<long long> ids(rows.size());
+    std::vector<int32_t> nodes((size_t)nen * rows.size());
+    for (size_t r = 0; r < rows.size(); r++) {
+        ids[r] = rows[r].id;
+        for (int a = 0; a < nen; a++) nodes[r * nen + a] = rows[r].n[a];
     }
     for (size_t j = i; j < R.L.size(); j++)
         if (starts(R.L[j], "*Element"))
             m.warnings.push_back("only the first *Element block is read, as the reference does (" + R.where(j) + ")");
-    const long long E = (long long)rows.size();
+    const long long E = (long long)ids.size();
     if (E == 0) return (R.err = "empty *Element block", false);
     m.nElem = (int32_t)E;
     m.conn.assign((size_t)E * m.nen, 0);
     std::vector<char> seen((size_t)E, 0);
-    for (auto &r : rows) {
-        if (r.first < 1 || r.first > E)
-            return (R.err = "element id " + std::to_string(r.first) + " outside 1.." + std::to_string(E) +
+    for (size_t r = 0; r < ids.size(); r++) {
+        if (ids[r] < 1 || ids[r] > E)
+            return (R.err = "element id " + std::to_string(ids[r]) + " outside 1.." + std::to_string(E) +
                             " (element ids must number the elements 1..N_e)",
                     false);
-        size_t k = (size_t)(r.first - 1);
-        if (seen[k]) return (R.err = "duplicate element id " + std::to_string(r.first), false);
+        size_t k = (size_t)(ids[r] - 1);
+        if (seen[k]) return (R.err = "duplicate element id " + std::to_string(ids[r]), false);
         seen[k] = 1;
-        for (int a = 0; a < m.nen; a++) m.conn[k * m.nen + a] = r.second[a];
+        for (int a = 0; a < m.nen; a++) m.conn[k * m.nen + a] = nodes[r * m.nen + a];
     }
     return true;
 }
@@ -190,7 +291,7 @@ bool readMaterials(Reader &R, InpModel &m) {
         if (inMat && haveLaw) m.materials.push_back(cur);
     };
     for (size_t i = 0; i < R.L.size(); i++) {
-        const std::string &line = R.L[i];
+        std::string_view line = R.L[i];
         if (starts(line, "*MATERIAL")) {
             flush();
             cur = InpMaterial();
@@ -225,7 +326,7 @@ bool readMaterials(Reader &R, InpModel &m) {
         } else if (starts(line, "*List of Elements")) {
             if (!haveLaw) return R.fail(i, "The constitutive model has not been properly defined");
             if (i + 1 >= R.L.size()) return R.fail(i, "*List of Elements needs a list");
-            const std::string &t = R.L[i + 1];
+            std::string_view t = R.L[i + 1];
             if (starts(t, "All")) {
                 cur.all = true;
             } else {
@@ -289,7 +390,7 @@ bool readSolvers(Reader &R, InpModel &m) {
         InpSolver s;
         bool timeSeen = false, velocity = false, displacement = false, closed = false;
         for (i++; i < R.L.size(); i++) {
-            const std::string &l = R.L[i];
+            std::string_view l = R.L[i];
             double v;
             if (starts(l, "*END SOLVER")) {
                 closed = true;
@@ -333,7 +434,7 @@ bool readSolvers(Reader &R, InpModel &m) {
                 if (numFiles < 1) return R.fail(i, "*Outputs needs at least one output file");
                 outVars.clear();
                 if (i + 2 < R.L.size() && !starts(R.L[i + 2], "*")) {
-                    const std::string &t = R.L[i + 2];
+                    std::string_view t = R.L[i + 2];
                     std::vector<std::string> f = fields(t);
                     long long x;
                     if (starts(t, "All")) {
@@ -362,7 +463,7 @@ bool readSolvers(Reader &R, InpModel &m) {
                 s.velRampSteps = (int)v;
                 i++;
             } else if (starts(l, "*MPS Precision")) {
-                std::string p = i + 1 < R.L.size() ? R.L[i + 1] : "";
+                std::string p = i + 1 < R.L.size() ? std::string(R.L[i + 1]) : std::string();
                 if (p == "FP64")
                     s.precision = MPS_PREC_FP64;
                 else if (p == "FP32_ELEM")
@@ -379,7 +480,7 @@ bool readSolvers(Reader &R, InpModel &m) {
                 s.viscC1 = bulk ? 0.06 : 1;
                 s.viscCl = bulk ? 1.2 : 0.5;
                 if (i + 1 < R.L.size() && !starts(R.L[i + 1], "*")) {
-                    const std::string &t = R.L[++i];
+                    std::string_view t = R.L[++i];
                     std::string tok;
                     for (size_t c = 0; c < t.size(); c++) {
                         if (t[c] == '*') break;
@@ -509,7 +610,7 @@ bool readExtractions(Reader &R, InpModel &m) {
 bool readLoads(Reader &R, InpModel &m) {
     bool gravity = false;
     for (size_t i = 0; i < R.L.size(); i++) {
-        const std::string &l = R.L[i];
+        std::string_view l = R.L[i];
         if (starts(l, "*BOUNDARY")) {
             const std::string type = tail(l, 16);
             if (type == "FORCE") {
@@ -595,12 +696,12 @@ bool readLoads(Reader &R, InpModel &m) {
 bool rejectUnsupported(Reader &R) {
     bool inSolver = false;
     for (size_t i = 0; i < R.L.size(); i++) {
-        const std::string &l = R.L[i];
+        std::string_view l = R.L[i];
         if (starts(l, "*SOLVER")) inSolver = true;
         if (starts(l, "*END SOLVER")) inSolver = false;
-        if (inSolver) continue;
+        if (inSolver || !starts(l, "*BOUNDARY")) continue;
         const std::string type = tail(l, 16);
-        if (starts(l, "*BOUNDARY") && type != "FORCE" && type != "PRESSURE RAMP" && type != "PRESSURE INST" &&
+        if (type != "FORCE" && type != "PRESSURE RAMP" && type != "PRESSURE INST" &&
             type != "TRACTION RAMP" && type != "TRACTION INST")
             return R.fail(i, "Dirichlet conditions outside *SOLVER are not on this path");
     }
@@ -610,14 +711,32 @@ bool rejectUnsupported(Reader &R) {
 } // namespace
 
 bool readInp(const std::string &path, InpModel &m, std::string &err) {
-    std::ifstream ifs(path);
-    if (!ifs) {
+    // the whole file in one read, split into lines with trailing blanks and '\r' removed
+    std::FILE *f = std::fopen(path.c_str(), "rb");
+    if (!f) {
         err = "cannot open input file " + path;
         return false;
     }
     Reader R;
-    std::string line;
-    while (std::getline(ifs, line)) R.L.push_back(rtrim(line));
+    std::string &buf = R.buf;
+    if (std::fseek(f, 0, SEEK_END) == 0) {
+        const long size = std::ftell(f);
+        if (size > 0) buf.reserve((size_t)size);
+        std::rewind(f);
+    }
+    char chunk[1 << 16];
+    size_t got;
+    while ((got = std::fread(chunk, 1, sizeof(chunk), f)) > 0) buf.append(chunk, got);
+    std::fclose(f);
+    R.L.reserve((size_t)std::count(buf.begin(), buf.end(), '\n') + 1);
+    for (size_t b = 0; b < buf.size();) {
+        const void *nl = std::memchr(buf.data() + b, '\n', buf.size() - b);
+        const size_t e = nl ? (size_t)(static_cast<const char *>(nl) - buf.data()) : buf.size();
+        size_t t = e;
+        while (t > b && (buf[t - 1] == '\r' || buf[t - 1] == ' ' || buf[t - 1] == '\t')) t--;
+        R.L.emplace_back(buf.data() + b, t - b);
+        b = e + 1;
+    }
     m = InpModel();
     bool ok = readNodes(R, m) && readElements(R, m) && readFemSets(R) && readMaterials(R, m) && readSolvers(R, m) &&
               readExtractions(R, m) && readLoads(R, m) && rejectUnsupported(R);

@@ -5,9 +5,13 @@
 #include <sys/time.h>
 
 #include <algorithm>
+#include <condition_variable>
 #include <cstdarg>
 #include <cstdio>
 #include <ctime>
+#include <deque>
+#include <mutex>
+#include <thread>
 
 #include "mps_gpu.h"
 #include "vtk_output.h"
@@ -21,6 +25,100 @@ double wallNow() {
     return tv.tv_sec + 1e-6 * tv.tv_usec;
 }
 double cpuNow() { return (double)std::clock() / CLOCKS_PER_SEC; }
+
+// VTK outputs written on a thread of their own while the solve goes on.  Each output is a device snapshot of
+// the step (mps_snapshot_begin) in one of two slots; the thread waits for its copy to pinned memory, formats
+// and writes the file, then releases the slot.  An output whose slot is still being written waits for that
+// write first, so at most two outputs are in flight and the files are written in order.
+class AsyncVtkWriter {
+  public:
+    AsyncVtkWriter(mps_handle h, VtkSeriesWriter &w, const InpModel &m, const std::vector<int> &outVars)
+        : _h(h), _w(w), _m(m), _outVars(outVars) {}
+    ~AsyncVtkWriter() {
+        std::string ignored;
+        finish(ignored);
+    }
+    bool submit(double time, std::string &err) {
+        const int slot = _next;
+        _next ^= 1;
+        {
+            std::unique_lock<std::mutex> lk(_mu);
+            _cv.wait(lk, [&] { return !_inUse[slot] || _failed; });
+            if (_failed) {
+                err = _werr;
+                return false;
+            }
+        }
+        if (mps_snapshot_begin(_h, slot, 1) != MPS_OK) {
+            err = std::string("mps_snapshot_begin: ") + mps_last_error(_h);
+            return false;
+        }
+        {
+            std::lock_guard<std::mutex> lk(_mu);
+            _inUse[slot] = true;
+            _q.push_back({slot, time});
+        }
+        if (!_th.joinable()) _th = std::thread([this] { run(); });
+        _cv.notify_all();
+        return true;
+    }
+    // Waits for every queued output; false (with err) if one failed.
+    bool finish(std::string &err) {
+        {
+            std::lock_guard<std::mutex> lk(_mu);
+            _stop = true;
+        }
+        _cv.notify_all();
+        if (_th.joinable()) _th.join();
+        if (_failed) err = _werr;
+        return !_failed;
+    }
+
+  private:
+    struct Job {
+        int slot;
+        double time;
+    };
+    void run() {
+        for (;;) {
+            Job j;
+            {
+                std::unique_lock<std::mutex> lk(_mu);
+                _cv.wait(lk, [&] { return !_q.empty() || _stop; });
+                if (_q.empty()) return;
+                j = _q.front();
+                _q.pop_front();
+            }
+            const double *u, *v, *a, *e;
+            std::string err;
+            bool ok = mps_snapshot_wait(_h, j.slot, &u, &v, &a, &e) == MPS_OK;
+            if (!ok) err = "mps_snapshot_wait failed";
+            if (ok) ok = _w.write(_m, u, v, a, e, _outVars, j.time, err);
+            mps_snapshot_release(_h, j.slot);
+            {
+                std::lock_guard<std::mutex> lk(_mu);
+                _inUse[j.slot] = false;
+                if (!ok && !_failed) {
+                    _failed = true;
+                    _werr = err;
+                }
+            }
+            _cv.notify_all();
+        }
+    }
+    mps_handle _h;
+    VtkSeriesWriter &_w;
+    const InpModel &_m;
+    const std::vector<int> &_outVars;
+    std::thread _th;
+    std::mutex _mu;
+    std::condition_variable _cv;
+    std::deque<Job> _q;
+    bool _inUse[2] = {false, false};
+    bool _stop = false, _failed = false;
+    std::string _werr;
+    int _next = 0;
+};
 
 struct Log {
     bool quiet;
@@ -173,18 +271,10 @@ int classSolverExplicit::calculate(const InpModel &m, const std::vector<std::uni
     const long long chunk = nodal ? 1 : (extract.needsEnergies() ? kHistoryCap : 1LL << 40);
     // VTK series (outputsManagement, solverExplicit.cpp:173-186, 310-322): at t0 when t0 < 1e-12, then
     // whenever `sampler` steps have passed since the last output, and at the last step.
-    VtkSeriesWriter *vtk = opt.vtk;
-    std::vector<double> ou, ov, oa, eout;
-    auto output = [&](double time) -> int {
-        if (!vtk) return 0;
-        ou.resize(3 * (size_t)N);
-        ov.resize(3 * (size_t)N);
-        oa.resize(3 * (size_t)N);
-        eout.resize(21 * (size_t)Ne);
-        if (mps_get_state(h, ou.data(), ov.data(), oa.data()) != MPS_OK) return fail("mps_get_state");
-        if (mps_element_outputs(h, eout.data()) != MPS_OK) return fail("mps_element_outputs");
-        return vtk->write(m, ou, ov, oa, eout, _s.outVars, time, err) ? 0 : -1;
-    };
+    // The files are written on a thread of their own from device snapshots, while the solve goes on.
+    std::unique_ptr<AsyncVtkWriter> vtk;
+    if (opt.vtk) vtk.reset(new AsyncVtkWriter(h, *opt.vtk, m, _s.outVars));
+    auto output = [&](double time) -> int { return !vtk || vtk->submit(time, err) ? 0 : -1; };
     if (_s.t0 < 1E-12 && output(_s.t0) != 0) return -1;
     long long sampler2 = 0;
     mps_report rep{};
@@ -275,6 +365,7 @@ int classSolverExplicit::calculate(const InpModel &m, const std::vector<std::uni
             return -1;
         }
     }
+    if (vtk && !vtk->finish(err)) return -1;
     _steps = rep.steps;
     _time = rep.time;
     _u.resize(3 * (size_t)N);

@@ -1,8 +1,10 @@
 // vtk_output.cpp -- VTK series of the explicit path (vtk_output.h).
 #include "vtk_output.h"
 
+#include <algorithm>
 #include <cstdio>
 #include <fstream>
+#include <thread>
 
 namespace mps {
 namespace {
@@ -58,43 +60,56 @@ bool VtkSeriesWriter::updateSeries(double timeRun, std::string &err) {
     return true;
 }
 
-bool VtkSeriesWriter::write(const InpModel &m, const std::vector<double> &u, const std::vector<double> &v,
-                            const std::vector<double> &a, const std::vector<double> &eout,
-                            const std::vector<int> &outVars, double timeRun, std::string &err) {
+// Items [0, n) formatted by item(o, i) on up to 16 threads, each into its own buffer, joined in order into out:
+// the text is the same as one pass over the items would give.
+template <typename F>
+void formatItems(std::string &out, size_t n, size_t bytesPerItem, F item) {
+    const unsigned hw = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    const size_t nt = n < 20000 ? 1 : hw;
+    std::vector<Out> part(nt);
+    auto work = [&](size_t t) {
+        const size_t b = n * t / nt, e = n * (t + 1) / nt;
+        part[t].s.reserve((e - b) * bytesPerItem);
+        for (size_t i = b; i < e; i++) item(part[t], i);
+    };
+    std::vector<std::thread> th;
+    for (size_t t = 1; t < nt; t++) th.emplace_back(work, t);
+    work(0);
+    for (auto &x : th) x.join();
+    for (auto &p : part) out += p.s;
+}
+
+const std::string &VtkSeriesWriter::geometry(const InpModel &m) {
+    if (_geomModel == &m && !_geom.empty()) return _geom;
     const size_t N = (size_t)m.nNodes, Ne = (size_t)m.nElem, nen = (size_t)m.nen;
-    const std::string fname =
-        _dir + "/MuPhiSimOutputs_" + std::to_string(_rank) + "_." + std::to_string(_cont) + ".vtk";
-    _cont++;
-    if (!updateSeries(timeRun, err)) return false;
     Out o;
-    o.s.reserve(N * 160 + Ne * 400 + 1024);
     o.str("# vtk DataFile Version 1.0\n3D Unstructured Grid of ");
     o.str(m.elemType);
     o.str(" elements\nASCII\n\nDATASET UNSTRUCTURED_GRID\nPOINTS ");
     o.integer((long long)N);
     o.str(" float\n");
-    for (size_t n = 0; n < N; n++) {
+    formatItems(o.s, N, 40, [&](Out &q, size_t n) {
         for (int d = 0; d < 3; d++) {
-            o.num(m.X[d * N + n]);
-            o.str(" ");
+            q.num(m.X[d * N + n]);
+            q.str(" ");
         }
-        o.nl();
-    }
+        q.nl();
+    });
     o.nl();
     o.str("CELLS ");
     o.integer((long long)Ne);
     o.str(" ");
     o.integer((long long)(Ne * (nen + 1)));
     o.nl();
-    for (size_t e = 0; e < Ne; e++) {
-        o.integer((long long)nen);
-        o.str(" ");
+    formatItems(o.s, Ne, 8 * (nen + 1), [&](Out &q, size_t e) {
+        q.integer((long long)nen);
+        q.str(" ");
         for (size_t k = 0; k < nen; k++) {
-            o.integer(m.conn[e * nen + k]);
-            o.str(" ");
+            q.integer(m.conn[e * nen + k]);
+            q.str(" ");
         }
-        o.nl();
-    }
+        q.nl();
+    });
     o.nl();
     o.str("CELL_TYPES ");
     o.integer((long long)Ne);
@@ -102,21 +117,37 @@ bool VtkSeriesWriter::write(const InpModel &m, const std::vector<double> &u, con
     const char *ctype = nen == 8 ? "12\n" : "10\n";  // VTK_HEXAHEDRON, VTK_TETRA
     for (size_t e = 0; e < Ne; e++) o.str(ctype);
     o.nl();
+    _geom.swap(o.s);
+    _geomModel = &m;
+    return _geom;
+}
+
+bool VtkSeriesWriter::write(const InpModel &m, const double *u, const double *v, const double *a,
+                            const double *eout, const std::vector<int> &outVars, double timeRun, std::string &err) {
+    const size_t N = (size_t)m.nNodes, Ne = (size_t)m.nElem;
+    const std::string fname =
+        _dir + "/MuPhiSimOutputs_" + std::to_string(_rank) + "_." + std::to_string(_cont) + ".vtk";
+    _cont++;
+    if (!updateSeries(timeRun, err)) return false;
+    const std::string &geom = geometry(m);  // the same for every output of the run
+    Out o;
+    o.s.reserve(N * 120 + Ne * 300 + 1024);
     o.str("POINT_DATA ");
     o.integer((long long)N);
     o.nl();
     const char *vecs[3] = {"VECTORS Displacement float \n", "VECTORS Velocity float \n",
                            "VECTORS Acceleration float \n"};
-    const std::vector<double> *fields[3] = {&u, &v, &a};
+    const double *fields[3] = {u, v, a};
     for (int f = 0; f < 3; f++) {
         o.str(vecs[f]);
-        for (size_t n = 0; n < N; n++) {
+        const double *x = fields[f];
+        formatItems(o.s, N, 40, [&](Out &q, size_t n) {
             for (int d = 0; d < 3; d++) {
-                o.num((*fields[f])[d * N + n]);
-                o.str(" ");
+                q.num(x[d * N + n]);
+                q.str(" ");
             }
-            o.nl();
-        }
+            q.nl();
+        });
         o.nl();
     }
     o.str("CELL_DATA ");
@@ -125,16 +156,16 @@ bool VtkSeriesWriter::write(const InpModel &m, const std::vector<double> &u, con
     const char *tens[2] = {"TENSORS Cauchy float \n", "TENSORS E float \n"};
     for (int t = 0; t < 2; t++) {
         o.str(tens[t]);
-        for (size_t e = 0; e < Ne; e++) {
+        formatItems(o.s, Ne, 130, [&](Out &q, size_t e) {
             for (int i = 0; i < 3; i++) {
                 for (int j = 0; j < 3; j++) {
-                    o.num(eout[e * 21 + t * 9 + i * 3 + j]);
-                    o.str(" ");
+                    q.num(eout[e * 21 + t * 9 + i * 3 + j]);
+                    q.str(" ");
                 }
-                o.nl();
+                q.nl();
             }
-            o.nl();
-        }
+            q.nl();
+        });
     }
     const char *scal[3] = {"SCALARS VMS float \n", "SCALARS VolumetricStrain float \n",
                            "SCALARS EquivalentStrain float \n"};
@@ -142,18 +173,18 @@ bool VtkSeriesWriter::write(const InpModel &m, const std::vector<double> &u, con
         if (k > 0) o.nl();
         o.str(scal[k]);
         o.str("LOOKUP_TABLE default\n");
-        for (size_t e = 0; e < Ne; e++) {
-            o.num(eout[e * 21 + 18 + k]);
-            o.nl();
-        }
+        formatItems(o.s, Ne, 14, [&](Out &q, size_t e) {
+            q.num(eout[e * 21 + 18 + k]);
+            q.nl();
+        });
     }
     o.nl();
     o.str("SCALARS Pressure float \nLOOKUP_TABLE default\n");
-    for (size_t e = 0; e < Ne; e++) {
+    formatItems(o.s, Ne, 14, [&](Out &q, size_t e) {
         const double *c = &eout[e * 21];
-        o.num((c[0] + c[4] + c[8]) / 3);
-        o.nl();
-    }
+        q.num((c[0] + c[4] + c[8]) / 3);
+        q.nl();
+    });
     o.nl();
     for (int k : outVars) {
         o.str("SCALARS IntVars");
@@ -167,7 +198,8 @@ bool VtkSeriesWriter::write(const InpModel &m, const std::vector<double> &u, con
         err = "cannot write " + fname;
         return false;
     }
-    const bool ok = std::fwrite(o.s.data(), 1, o.s.size(), fp) == o.s.size();
+    const bool ok = std::fwrite(geom.data(), 1, geom.size(), fp) == geom.size() &&
+                    std::fwrite(o.s.data(), 1, o.s.size(), fp) == o.s.size();
     std::fclose(fp);
     if (!ok) err = "short write " + fname;
     return ok;

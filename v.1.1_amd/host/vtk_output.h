@@ -11,8 +11,8 @@
 //   (tr of the averaged Cauchy / 3), IntVars<k> for the internal variables named on the
 //   *Outputs line (the laws of this path have none: 0, as the reference prints for a
 //   missing variable).
-// The element fields come from the device (mps_element_outputs), the nodal fields from
-// mps_get_state.
+// The nodal and element fields come from the device (mps_snapshot_*, the element fields as
+// mps_element_outputs computes them).
 #ifndef MPS_VTK_OUTPUT_H
 #define MPS_VTK_OUTPUT_H
 
@@ -26,17 +26,20 @@ namespace mps {
 class VtkSeriesWriter {
   public:
     explicit VtkSeriesWriter(std::string dir, int rank = 0) : _dir(std::move(dir)), _rank(rank) {}
-    // u, v, a: SoA [3*n_nodes]; eout: [21*n_elem] from mps_element_outputs.
-    bool write(const InpModel &m, const std::vector<double> &u, const std::vector<double> &v,
-               const std::vector<double> &a, const std::vector<double> &eout, const std::vector<int> &outVars,
-               double timeRun, std::string &err);
+    // u, v, a: SoA [3*n_nodes]; eout: [21*n_elem] as mps_element_outputs gives them.  The sections are
+    // formatted on up to 16 threads; the geometry (POINTS, CELLS, CELL_TYPES) once per model.
+    bool write(const InpModel &m, const double *u, const double *v, const double *a, const double *eout,
+               const std::vector<int> &outVars, double timeRun, std::string &err);
     int filesWritten() const { return _cont; }
 
   private:
     bool updateSeries(double timeRun, std::string &err);
+    const std::string &geometry(const InpModel &m);
     std::string _dir;
     int _rank;
     int _cont = 0;  // output counter (static `cont` of output.cpp:28)
+    const InpModel *_geomModel = nullptr;
+    std::string _geom;
 };
 
 } // namespace mps

@@ -133,6 +133,10 @@ def load(path=None):
     L.mps_internal_force.argtypes = [H, _dp, _dp]
     L.mps_critical_dt.argtypes = [H, _dp, _dp]
     L.mps_element_outputs.argtypes = [H, _dp]
+    _cdpp = C.POINTER(C.POINTER(C.c_double))
+    L.mps_snapshot_begin.argtypes = [H, C.c_int32, C.c_int32]
+    L.mps_snapshot_wait.argtypes = [H, C.c_int32, _cdpp, _cdpp, _cdpp, _cdpp]
+    L.mps_snapshot_release.argtypes = [H, C.c_int32]
     L.mps_set_loads.argtypes = [H, C.c_int32, C.POINTER(Load)]
     L.mps_set_viscosity.argtypes = [H, C.c_int32, C.c_double, C.c_double, C.c_double, C.c_double]
     L.mps_external_force.argtypes = [H, C.c_double, _dp]
@@ -351,6 +355,21 @@ class Solver:
         out = np.zeros(21 * self.ne)
         self._chk(self.L.mps_element_outputs(self.h, _d(out)))
         return out.reshape(-1, 21)
+
+    def snapshot_begin(self, slot, with_elements=True):
+        """Stage u, v, a (and the element outputs) of the last step for a copy that runs while stepping goes on."""
+        self._chk(self.L.mps_snapshot_begin(self.h, slot, int(with_elements)))
+
+    def snapshot_wait(self, slot):
+        """Wait for the slot's copy: (u, v, a, element outputs [n_elem, 21] or None), as numpy copies."""
+        p = [C.POINTER(C.c_double)() for _ in range(4)]
+        self._chk(self.L.mps_snapshot_wait(self.h, slot, *(C.byref(x) for x in p)))
+        u, v, a = (np.ctypeslib.as_array(x, shape=(3 * self.n,)).copy() for x in p[:3])
+        e = np.ctypeslib.as_array(p[3], shape=(21 * self.ne,)).copy().reshape(-1, 21) if p[3] else None
+        return u, v, a, e
+
+    def snapshot_release(self, slot):
+        self._chk(self.L.mps_snapshot_release(self.h, slot))
 
     def mass(self):
         m = np.zeros(self.n)

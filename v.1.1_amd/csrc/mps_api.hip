@@ -487,9 +487,9 @@ static int upload_instance(mps_solver_s* h, const HostPrep& hp) {
                 for (int q = pr[j]; q < pr[j + 1]; q++) plx[(size_t)p * MPS_PATCH * nen + pc[q]] = (uint16_t)j;
             pu = std::max(pu, U);
         }
-        // node staging and FP64 force slots (one shared space), contribution list, staged node offsets (at most
-        // 2.1 KB), and the kernel's static block-min slots
-        const size_t lds = std::max<size_t>(48 * (size_t)pu, 12 * MPS_PATCH * 8) + 8 * MPS_PATCH + 2200 + 64;
+        // node staging and FP64 force slots (one shared space), contribution list, and the kernel's static
+        // block-min slots (with room to spare)
+        const size_t lds = std::max<size_t>(48 * (size_t)pu, 12 * MPS_PATCH * 8) + 8 * MPS_PATCH + 64;
         if (lds <= 65536) {
             int32_t* pnd;
             uint16_t* plxd;

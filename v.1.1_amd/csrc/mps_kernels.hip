@@ -29,12 +29,13 @@ __device__ __forceinline__ uint4 pc8(const uint16_t* __restrict__ g, int q, int 
     return q < m ? *reinterpret_cast<const uint4*>(g + q) : make_uint4(0u, 0u, 0u, 0u);
 }
 
-template <int NEN>
+// ROWS = false: only the contribution list; the node offsets are then read from prow in global memory.
+template <int NEN, bool ROWS = true>
 struct PatchList {
     static constexpr int NQ = (NEN * MPS_PATCH + 8 * MPS_PATCH - 1) / (8 * MPS_PATCH);  // pcol pieces per work-item
-    static constexpr int NR = ((NEN * MPS_PATCH + 1 + 7 + 7) / 8 + MPS_PATCH - 1) / MPS_PATCH;  // prow pieces
+    static constexpr int NR = ROWS ? ((NEN * MPS_PATCH + 1 + 7 + 7) / 8 + MPS_PATCH - 1) / MPS_PATCH : 0;  // prow pieces
     static constexpr int SPR = (NEN * MPS_PATCH + 15) / 8 * 8;  // LDS entries for the staged prow (>= len rounded up to 8)
-    uint4 v[NQ], w[NR];
+    uint4 v[NQ], w[NR > 0 ? NR : 1];
     int off, len;  // first prow entry of the patch relative to the staged start; staged entries
     __device__ __forceinline__ void load(const MpsInst& I, int p, int e0, int m, int U) {
         const uint16_t* gpc = I.pcol + (size_t)e0 * NEN;
@@ -214,9 +215,9 @@ __host__ __device__ __forceinline__ size_t tet_shared_bytes(int pu, int tsize) {
 // element then reads its four nodes from LDS through its local indices plx.  Every node is fetched
 // once per patch instead of once per element slot (about 1 against 5.8 node loads per partial on a
 // structured tet mesh).  Dynamic LDS: the staging (48*pu_max bytes) and, after a barrier, the 12*MPS_PATCH
-// force slots of T in the same space, then the 4*MPS_PATCH entries of the contribution list and the staged
-// node offsets; the elements hold their forces in registers across that barrier.  Sharing the space lets
-// 5 (FP64) or 8 (FP32) workgroups reside per CU instead of 4 and 6.
+// force slots of T in the same space, then the 4*MPS_PATCH entries of the contribution list; the elements
+// hold their forces in registers across that barrier, and the patch sums read the node offsets from global
+// memory.  So 6 (FP64) or 8 (FP32) workgroups reside per CU.
 template <typename T, int LAW, bool EN = false>
 __global__ __launch_bounds__(MPS_PATCH) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 5 : 4))) void k_elem_tet_staged(MpsInst I, int mode, unsigned eflags) {
     extern __shared__ __align__(16) double smem[];  // 16-byte pieces are stored at offsets that are multiples of 16
@@ -226,7 +227,6 @@ __global__ __launch_bounds__(MPS_PATCH) __attribute__((amdgpu_waves_per_eu(sizeo
     T* __restrict__ sf = reinterpret_cast<T*>(smem);  // [12][MPS_PATCH], over the staging once it is read
     const size_t shared_bytes = tet_shared_bytes(pu, sizeof(T));
     uint16_t* __restrict__ spc = reinterpret_cast<uint16_t*>(reinterpret_cast<char*>(smem) + shared_bytes);
-    uint16_t* __restrict__ spr = spc + 4 * MPS_PATCH;  // [PatchList<4>::SPR]
     long long k = 0;
     if (mode == 0) {
         k = I.ctl->A;
@@ -240,7 +240,7 @@ __global__ __launch_bounds__(MPS_PATCH) __attribute__((amdgpu_waves_per_eu(sizeo
     const int base = I.poff[p];
     const int U = I.poff[p + 1] - base;  // <= pu_max (set-up)
     const size_t N = (size_t)I.N;
-    PatchList<4> pl;
+    PatchList<4, false> pl;  // the node offsets stay in global memory (6 workgroups per CU in FP64)
     pl.load(I, p, e0, ne * 4, U);
     // the element's own inputs are loaded before the staging barrier, so their latency overlaps it
     const int i = threadIdx.x;
@@ -255,7 +255,7 @@ __global__ __launch_bounds__(MPS_PATCH) __attribute__((amdgpu_waves_per_eu(sizeo
             sxu[(3 + d) * pu + j] = I.u[d * N + n];
         }
     }
-    pl.store(spc, spr, ne * 4);  // the lists arrive with the nodes; stored now, they hold no registers below
+    pl.store(spc, nullptr, ne * 4);  // the list arrives with the nodes; stored now, it holds no registers below
     __syncthreads();
     unsigned long long dmin = INF_BITS;
     double env[3] = {0.0, 0.0, 0.0};
@@ -308,7 +308,7 @@ __global__ __launch_bounds__(MPS_PATCH) __attribute__((amdgpu_waves_per_eu(sizeo
             for (int d = 0; d < 3; d++) sf[(a * 3 + d) * MPS_PATCH + i] = f[a][d];
     }
     __syncthreads();
-    patch_sums<T, 4, MPS_PATCH, true>(I, p, sf, spc, spr + pl.off);
+    patch_sums<T, 4, MPS_PATCH, true>(I, p, sf, spc, I.prow + base + p);
     if (mode == 0) {
         dmin = block_min_u64(dmin, sh);
         if (threadIdx.x == 0) atomicMin(dt_slot(I, (k + 1) & 1, blockIdx.x & (MPS_DT_SLOTS - 1)), dmin);
@@ -1053,10 +1053,8 @@ static inline unsigned blocks_for(long long n, int bs) { return (unsigned)((n + 
 
 bool elem_fuses_energy(const MpsInst& I) { return I.precision == 0 && I.uprev == nullptr; }
 
-// Dynamic LDS of the staged tet4 kernel: node staging / force slots, contribution list, offsets.
-static size_t tet_lds(const MpsInst& I) {
-    return tet_shared_bytes(I.pu_max, I.precision == 1 ? 4 : 8) + 8 * MPS_PATCH + 2 * PatchList<4>::SPR;
-}
+// Dynamic LDS of the staged tet4 kernel: node staging / force slots, then the contribution list.
+static size_t tet_lds(const MpsInst& I) { return tet_shared_bytes(I.pu_max, I.precision == 1 ? 4 : 8) + 8 * MPS_PATCH; }
 
 hipError_t launch_elem(const MpsInst& I, int mode, hipStream_t st, unsigned eflags) {
     const unsigned nb = (unsigned)I.n_patch;

@@ -9,6 +9,7 @@ import pytest
 
 import mps_gpu as G
 import oracle_ctypes as O
+from mps_gpu import configs
 from meshgen import box, clamp_and_pull, perturb
 
 pytestmark = pytest.mark.gpu
@@ -219,6 +220,17 @@ def test_mixed_orientation_hex(precision):
     flip = np.random.default_rng(3).random(conn.shape[0]) < 0.5
     conn[flip] = conn[flip][:, [1, 0, 3, 2, 5, 4, 7, 6]]
     m = dict(m, conn=np.ascontiguousarray(conn.ravel()))
+    ro, (rc, rep, u, *_r) = run_pair(m, 150, precision=precision)
+    tol = TOL64 if precision == G.PREC_FP64 else TOL32
+    assert rc == 0 and relerr(u, ro["u"]) <= tol, relerr(u, ro["u"])
+
+
+@pytest.mark.parametrize("precision", [G.PREC_FP64, G.PREC_FP32_ELEM])
+def test_tet4_reflected_splits(precision):
+    """The 6-tet split in alternating mirror images (conforming, half of the tets negatively oriented, node
+    valences up to 48 instead of 24: longer and more uneven CSR rows and patch lists), through the node-staging
+    kernel, against the oracle."""
+    m = configs.perturb(configs.box(9, 8, 7, tet=True, reflected=True), 0.15, 41)
     ro, (rc, rep, u, *_r) = run_pair(m, 150, precision=precision)
     tol = TOL64 if precision == G.PREC_FP64 else TOL32
     assert rc == 0 and relerr(u, ro["u"]) <= tol, relerr(u, ro["u"])

@@ -38,8 +38,13 @@ def uniform01(seed, index):
     return (splitmix64(seed, index) >> np.uint64(11)).astype(np.float64) * (1.0 / 9007199254740992.0)
 
 
-def box(nx, ny, nz, lx=1.0, ly=1.0, lz=1.0, tet=False):
-    """Structured box; node id = i + (nx+1)*(j + (ny+1)*k), elements x-fastest."""
+def box(nx, ny, nz, lx=1.0, ly=1.0, lz=1.0, tet=False, reflected=False):
+    """Structured box; node id = i + (nx+1)*(j + (ny+1)*k), elements x-fastest.
+
+    reflected: each cell's local corner numbering mirrored along x, y, z where the cell index along that axis is
+    odd.  Hex8 cells then alternate their orientation (det J < 0 on an odd number of mirrors); tet4 cells get
+    the 6-tet split in alternating mirror images, which stays conforming (two cells sharing a face differ in one
+    axis' parity, so both split it along the same diagonal) and gives the nodes valences other than 24."""
     N = (nx + 1) * (ny + 1) * (nz + 1)
     ids = np.arange(N, dtype=np.int64)
     i = ids % (nx + 1)
@@ -54,8 +59,9 @@ def box(nx, ny, nz, lx=1.0, ly=1.0, lz=1.0, tet=False):
     ej = (e // nx) % ny
     ek = e // (nx * ny)
     hexes = np.empty((e.size, 8), dtype=np.int64)
+    fi, fj, fk = (ei & 1, ej & 1, ek & 1) if reflected else (0, 0, 0)
     for a, (ci, cj, ck) in enumerate(HEX_CORNERS):
-        hexes[:, a] = (ei + ci) + (nx + 1) * ((ej + cj) + (ny + 1) * (ek + ck))
+        hexes[:, a] = (ei + (ci ^ fi)) + (nx + 1) * ((ej + (cj ^ fj)) + (ny + 1) * (ek + (ck ^ fk)))
     if tet:
         conn = hexes[:, TET_SPLIT].reshape(-1, 4)
         nen = 4

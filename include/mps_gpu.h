@@ -209,6 +209,24 @@ int mps_get_internal_force_dofs(mps_handle h, int64_t n, const int32_t *dof, dou
  * (solverExplicit.cpp:217-338). */
 int mps_run(mps_handle h, int64_t max_steps, mps_report *rep);
 
+/* Kinds of per-step DOF series (mps_set_dof_series). */
+#define MPS_SERIES_U 0    /* displacement at the end of the step */
+#define MPS_SERIES_V 1    /* velocity at the end of the step */
+#define MPS_SERIES_A 2    /* acceleration at the end of the step */
+#define MPS_SERIES_FINT 3 /* internal force of the step (the node kernel's sum of its partial forces) */
+#define MPS_SERIES_FEXT 4 /* external force of the step (its loads at the step's time) */
+
+/* Per-step series of single DOFs, recorded on the device inside every step (graph replay kept, no host round
+ * trip per step) into a ring of history_cap rows: entry i is kind which[i] at DOF dof[i] = node +
+ * n_nodes*dir.  Replaces any earlier series; n = 0 removes them.  mps_get_dof_series(h, first, count, out)
+ * copies rows [first, first+count) (row = step index - 1, as mps_get_history) into out[count*n], row-major;
+ * the step times are the history's time column.  Run in chunks of at most history_cap steps to read every
+ * row.  Replaces: the per-step nodal look-up of classExtractData (Unknown, InternalForce, ExternalForce;
+ * dataExtraction.cpp:778-912) and the internal force of printForcesOut (output.cpp:106-135), which the
+ * reference reads from host vectors every step. */
+int mps_set_dof_series(mps_handle h, int64_t n, const int32_t *which, const int32_t *dof);
+int mps_get_dof_series(mps_handle h, int64_t first, int64_t count, double *out);
+
 /* Per-step history from the device ring: entries [first, first+count) with
  * step index = entry + 1.  Energies are NaN on steps where they were not
  * computed.  Any output pointer may be NULL.  Energies cost one FP64 pass over the elements on each energy step

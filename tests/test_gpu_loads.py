@@ -267,3 +267,46 @@ def test_traction_run_fp64(tet):
     err = relerr(u, ro["u"])
     print(f"tet={tet} traction fp64 {err:.2e}, max|u| {np.abs(ro['u']).max():.3e}")
     assert ro["rc"] == 0 and rc == 0 and err <= TOL64 and np.abs(ro["u"]).max() > 1e-5
+
+
+@pytest.mark.parametrize("precision", [G.PREC_FP64, G.PREC_FP32_ELEM])
+def test_dof_series_recorded_inside_the_steps(precision):
+    """mps_set_dof_series: u, v, a, internal and external force at chosen DOFs (free and constrained), recorded
+    by the device inside every step with graph replay and any chunking, equal to what the host reads after each
+    step of a run launched one step at a time (u, v, a and the internal force bit for bit; the external force
+    against mps_external_force at the step's time)."""
+    m = perturb(box(3, 3, 5, lz=1.5), 0.08, 23)
+    N = m["n_nodes"]
+    top = boundary_faces(m, 2, 1.5)
+    blocks = [("pressure_ramp", 4e4, 0.0, 0.3, top[0], top[1]), ("gravity", 9.81),
+              ("force", [N - 1], [[0.0, 300.0, 0.0]], 0.0, 1.0)]
+    dd, dv = clamp_bottom(m)
+    steps = 90
+    nodes = [0, N // 2, N - 1, 5]
+    entries = [(kind, n + N * d) for n in nodes for kind in range(5) for d in range(3)]
+    which = np.array([e[0] for e in entries], dtype=np.int32)
+    dof = np.array([e[1] for e in entries], dtype=np.int32)
+    kw = dict(dir_dof=dd, dir_val=dv, bc_mode=G.BC_VELOCITY, vel_ramp_steps=30, scale=0.5, n_steps=steps,
+              precision=precision, history_cap=steps)
+    ref = np.zeros((steps, len(dof)))
+    with G.Solver(m, **kw) as s:
+        s.set_loads(loads=blocks)
+        for k in range(steps):
+            rc, rep = s.run(1)
+            assert rc == 0
+            vals = {0: s.get_dofs(0, dof), 1: s.get_dofs(1, dof), 2: s.get_dofs(2, dof),
+                    3: s.internal_force_dofs(dof), 4: s.external_force(rep["time"])[dof]}
+            ref[k] = [vals[w][i] for i, w in enumerate(which)]
+    for chunks in ([steps], [37, 53], [1, 2, 87]):
+        with G.Solver(m, **kw) as s:
+            s.set_loads(loads=blocks)
+            s.set_dof_series(which, dof)
+            for c in chunks:
+                rc, rep = s.run(c)
+                assert rc == 0
+            got = s.dof_series(0, steps)
+        exact = which != G.SERIES_FEXT
+        assert np.array_equal(got[:, exact], ref[:, exact]), chunks
+        fe = ~exact
+        np.testing.assert_allclose(got[:, fe], ref[:, fe], rtol=0, atol=1e-13 * np.abs(ref[:, fe]).max())
+        assert np.abs(ref[:, which == G.SERIES_FINT]).max() > 0 and np.abs(ref[:, fe]).max() > 0

@@ -328,6 +328,7 @@ struct mps_solver_s {
         int with_elem = 0;
     } snap[2];
     hipStream_t copy_stream = nullptr;
+    std::vector<void*> series_allocs;  // mps_set_dof_series buffers (replaced by the next call)
     std::string err;
 
     int fail(int code, const std::string& m) {
@@ -365,6 +366,8 @@ static void release(mps_solver_s* h) {
     h->allocs.clear();
     for (void* p : h->load_allocs) (void)hipFree(p);
     h->load_allocs.clear();
+    for (void* p : h->series_allocs) (void)hipFree(p);
+    h->series_allocs.clear();
     if (h->pick_idx) (void)hipFree(h->pick_idx);
     if (h->pick_out) (void)hipFree(h->pick_out);
     if (h->eout) (void)hipFree(h->eout);
@@ -1235,6 +1238,7 @@ static hipError_t enqueue_steps(mps_solver_s* h, long long count, std::vector<in
             if (t.a) kinds->push_back(3);
         }
         if (e == hipSuccess && (I.n_sgp > 0 || I.n_lnode > 0)) e = mpsk::launch_loads(I, h->stream);
+        if (e == hipSuccess && I.n_series > 0) e = mpsk::launch_series(I, h->stream);
         if (e != hipSuccess) break;
         const bool last = (i == count - 1);
         {
@@ -1493,6 +1497,62 @@ extern "C" int mps_get_history(mps_handle h, int64_t first, int64_t count, doubl
     HCHK(h, hipStreamSynchronize(h->stream));
     double* outs[5] = {time, dt, e_def, e_kin, volume};
     return read_history(h, first, count, MPS_H_TIME, 5, outs);
+}
+
+extern "C" int mps_set_dof_series(mps_handle h, int64_t n, const int32_t* which, const int32_t* dof) {
+    if (!h || n < 0 || (n > 0 && (!which || !dof))) return MPS_ERR_ARG;
+    const int64_t ndof = 3 * (int64_t)h->I.N;
+    for (int64_t i = 0; i < n; i++) {
+        if (dof[i] < 0 || dof[i] >= ndof)
+            return h->fail(MPS_ERR_ARG, fmt("mps_set_dof_series: dof %d outside [0, %lld)", dof[i], (long long)ndof));
+        if (which[i] < MPS_SERIES_U || which[i] > MPS_SERIES_FEXT)
+            return h->fail(MPS_ERR_ARG, fmt("mps_set_dof_series: unknown kind %d", which[i]));
+    }
+    if (n > INT32_MAX) return h->fail(MPS_ERR_ARG, "mps_set_dof_series: too many entries");
+    (void)hipSetDevice(h->device);
+    HCHK(h, hipStreamSynchronize(h->stream));
+    for (auto& g : h->graphs) (void)hipGraphExecDestroy(g.exec);  // the captured steps launch the series kernel
+    h->graphs.clear();
+    for (void* p : h->series_allocs) (void)hipFree(p);
+    h->series_allocs.clear();
+    MpsInst& I = h->I;
+    I.n_series = 0;
+    I.ser_dof = I.ser_kind = nullptr;
+    I.ser_ring = nullptr;
+    if (n == 0) return MPS_OK;
+    void *pd = nullptr, *pk = nullptr, *pr = nullptr;
+    const size_t ring = (size_t)n * (size_t)I.hist_cap * 8;
+    HCHK(h, hipMalloc(&pd, (size_t)n * 4));
+    h->series_allocs.push_back(pd);
+    HCHK(h, hipMalloc(&pk, (size_t)n * 4));
+    h->series_allocs.push_back(pk);
+    HCHK(h, hipMalloc(&pr, ring));
+    h->series_allocs.push_back(pr);
+    HCHK(h, hipMemcpy(pd, dof, (size_t)n * 4, hipMemcpyHostToDevice));
+    HCHK(h, hipMemcpy(pk, which, (size_t)n * 4, hipMemcpyHostToDevice));
+    hipError_t e = mpsk::launch_fill_u64((unsigned long long*)pr, (long long)n * I.hist_cap, 0x7FF8000000000000ull,
+                                         h->stream);  // NaN until a step writes the row
+    if (e != hipSuccess) return h->fail(MPS_ERR_HIP, fmt("series ring: %s", hipGetErrorString(e)));
+    HCHK(h, hipStreamSynchronize(h->stream));
+    I.n_series = (int32_t)n;
+    I.ser_dof = (const int32_t*)pd;
+    I.ser_kind = (const int32_t*)pk;
+    I.ser_ring = (double*)pr;
+    return MPS_OK;
+}
+
+extern "C" int mps_get_dof_series(mps_handle h, int64_t first, int64_t count, double* out) {
+    if (!h || first < 0 || count < 0 || (count > 0 && !out)) return MPS_ERR_ARG;
+    const MpsInst& I = h->I;
+    if (I.n_series == 0 || count == 0) return MPS_OK;
+    (void)hipSetDevice(h->device);
+    HCHK(h, hipStreamSynchronize(h->stream));
+    const size_t n = (size_t)I.n_series, cap = (size_t)I.hist_cap;
+    std::vector<double> buf(n * cap);
+    HCHK(h, hipMemcpy(buf.data(), I.ser_ring, buf.size() * 8, hipMemcpyDeviceToHost));
+    for (int64_t r = 0; r < count; r++)
+        std::memcpy(out + (size_t)r * n, buf.data() + (size_t)((first + r) % (int64_t)cap) * n, n * 8);
+    return MPS_OK;
 }
 
 extern "C" int mps_get_energy_balance(mps_handle h, int64_t first, int64_t count, double* e_ext, double* e_kin_lumped,

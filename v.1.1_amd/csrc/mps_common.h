@@ -157,6 +157,11 @@ struct MpsInst {
     const int32_t* pnd;
     const uint16_t* plx;
     int32_t pu_max;  // largest number of distinct nodes of a patch
+    // per-step DOF series (mps_set_dof_series): n_series entries, recorded into ser_ring[(k-1)%hist_cap][i]
+    int32_t n_series;
+    const int32_t* ser_dof;   // node + N*dir
+    const int32_t* ser_kind;  // MPS_SERIES_*
+    double* ser_ring;
 };
 
 // One unit of work for the persistent batched kernels: a contiguous range of

@@ -27,6 +27,8 @@ hipError_t launch_loads(const MpsInst& I, hipStream_t st);
 // Standalone energy pass: MPS_E_DEF / MPS_E_KIN of step ctl->B between its element and node kernels, or
 // MPS_E_KIN_NOW after the last node kernel of a launch; block sums only (added up by the next kernel).
 hipError_t launch_energy(const MpsInst& I, unsigned flags, hipStream_t st);
+// Per-step DOF series of step ctl->B into their ring (between the step's element and node kernels).
+hipError_t launch_series(const MpsInst& I, hipStream_t st);
 // End of a launch with energies, after the MPS_E_KIN_NOW pass: the last step's sums into the history.
 hipError_t launch_energy_tail(const MpsInst& I, hipStream_t st);
 // Element fields of the VTK output: out[21*Ne] (Cauchy 9, strain 9, VMS, volumetric, equivalent strain).

@@ -8,6 +8,7 @@
 //                     a = -fint/m, v += dt(a_old + a_new)/2, and -- when fused -- the
 //                     begin of step k+1 (lagged time, dt, prediction, Dirichlet BC)
 // plus a standalone begin kernel (first step of a launch) and an energy pass.
+#include "../../include/mps_gpu.h"
 #include "mps_device.h"
 #include "mps_kernels.h"
 
@@ -501,6 +502,46 @@ __global__ __launch_bounds__(BS) void k_node_batch(const MpsInst* __restrict__ i
         const bool first = W.first && threadIdx.x == 0;
         node_item(I, n < W.end ? n : -1, first, fuse);
     }
+}
+
+// Per-step DOF series (mps_set_dof_series), after the element kernel (and the loads) of step k = ctl->B and
+// before its node kernel: u holds u_k there, part the step's partial forces, lf its external forces.  The
+// velocity and acceleration are those the node kernel is about to give the DOF (solve_node on a copy, the same
+// arithmetic), the internal force the node's partials in CSR order as the node kernel sums them.
+__global__ __launch_bounds__(256) void k_series(MpsInst I) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= I.n_series) return;
+    const long long k = I.ctl->B;
+    const MpsStepSlot s = I.ctl->slot[k & 1];
+    if (!s.go) return;
+    const size_t N = (size_t)I.N;
+    const int dof = I.ser_dof[i], kind = I.ser_kind[i];
+    const int n = dof % I.N, dir = dof / I.N;
+    double val;
+    if (kind == MPS_SERIES_U) {
+        val = I.u[dof];
+    } else if (kind == MPS_SERIES_FINT) {
+        val = 0.0;
+        for (int q = I.nrow[n]; q < I.nrow[n + 1]; q++) {
+            double c[3];
+            load_partial(I, I.ncol[q], c);
+            val += c[dir];
+        }
+    } else if (kind == MPS_SERIES_FEXT) {
+        double fe[3];
+        node_fext(I, n, I.ninfo[n], fe);
+        val = fe[dir];
+    } else {
+        double v[3], a[3];
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            v[d] = I.v[d * N + n];
+            a[d] = I.a[d * N + n];
+        }
+        solve_node(I, n, k, s.dt, v, a);
+        val = kind == MPS_SERIES_V ? v[dir] : a[dir];
+    }
+    I.ser_ring[(size_t)((k - 1) % I.hist_cap) * I.n_series + i] = val;
 }
 
 // ---------------------------------------------------------------------------
@@ -1187,6 +1228,12 @@ hipError_t launch_energy(const MpsInst& I, unsigned flags, hipStream_t st) {
         case 2: k_energy<2><<<nb, MPS_PATCH, 0, st>>>(I, flags); break;
         default: k_energy<3><<<nb, MPS_PATCH, 0, st>>>(I, flags); break;
     }
+    return hipGetLastError();
+}
+
+hipError_t launch_series(const MpsInst& I, hipStream_t st) {
+    if (I.n_series <= 0) return hipSuccess;
+    k_series<<<blocks_for(I.n_series, 256), 256, 0, st>>>(I);
     return hipGetLastError();
 }
 

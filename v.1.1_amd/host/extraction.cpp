@@ -108,24 +108,6 @@ bool classExtractData::init(const InpModel &model, std::string &err) {
     return true;
 }
 
-bool classExtractData::needsInternalForce() const {
-    for (auto &kv : _node)
-        if (kv.second->getDataType() == "InternalForce") return true;
-    return false;
-}
-
-bool classExtractData::needsExternalForce() const {
-    for (auto &kv : _node)
-        if (kv.second->getDataType() == "ExternalForce") return true;
-    return false;
-}
-
-bool classExtractData::needsExternalEnergy() const {
-    for (auto &kv : _element)
-        if (kv.second->getDataType() == "EXTERNAL_ENERGY") return true;
-    return false;
-}
-
 void classExtractData::finalise() {
     for (auto &kv : _node) kv.second->closeFile();
     for (auto &kv : _element) kv.second->closeFile();

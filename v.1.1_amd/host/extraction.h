@@ -9,9 +9,10 @@
 //   element: All_<type>_<operation>.csv for "All", else <prefix>_<type>_<operation>.csv
 // The first extraction of a given name wins (addClassNodeDataToSave, :469-487).
 //
-// Values come from the device: nodal values through mps_get_dofs /
-// mps_internal_force, element sums (DEFO_ENERGY, KIN_ENERGY, VOLUME with the
-// Sum operation over All elements) from the per-step history ring.
+// Values come from the device: nodal values from the per-step DOF series
+// (mps_set_dof_series), element sums (DEFO_ENERGY, KIN_ENERGY, VOLUME,
+// EXTERNAL_ENERGY with the Sum operation over All elements) from the per-step
+// history ring, both read once per chunk of steps.
 #ifndef MPS_EXTRACTION_H
 #define MPS_EXTRACTION_H
 
@@ -80,8 +81,6 @@ class classPrintForces {
     const std::vector<int32_t> &getDirectionsToWrite() const { return _f.dirs; }
     double getPeriod() const { return _f.period; }
     int getIterationsCompleted() const { return _iterationsCompleted; }
-    // Time before which no step can write: iterations*period + t0 (the period used is at least `period`).
-    double nextThreshold(double t0) const { return _iterationsCompleted * _f.period + t0; }
     // Whether the step at `time` with step `dt` writes (the test of printForcesOut).
     bool due(double time, double dt, double t0) const;
     // Appends the rows of one step: forces[i] is the internal force of pair i.
@@ -99,9 +98,6 @@ class classExtractData {
     bool init(const InpModel &model, std::string &err);
     bool empty() const { return _node.empty() && _element.empty(); }
     bool needsNodal() const { return !_node.empty(); }
-    bool needsInternalForce() const;
-    bool needsExternalForce() const;   // a node extraction of ExternalForce
-    bool needsExternalEnergy() const;  // an element extraction of EXTERNAL_ENERGY
     bool needsEnergies() const { return !_element.empty(); }
     std::map<std::string, std::unique_ptr<classNodeData>> &nodeData() { return _node; }
     std::map<std::string, std::unique_ptr<classElementData>> &elementData() { return _element; }

@@ -239,36 +239,41 @@ int classSolverExplicit::calculate(const InpModel &m, const std::vector<std::uni
     log.info("The initial time step is %g", dt0);
     log.info("done preprocessing in solver (Wall %gs, CPU %gs)", wallNow() - startWall, cpuNow() - startCpu);
 
-    // Nodal extraction DOFs, gathered on the device every step: displacements, and internal forces from the
-    // step's partial nodal forces.
-    std::vector<int32_t> pickDof, fintDof;
+    // Nodal extractions (Unknown, InternalForce, ExternalForce at the listed nodes and component) and the
+    // *Forces pairs (printForcesOut, solverExplicit.cpp:304: the step's internal force at the listed (node,
+    // direction) pairs whenever the output is due) are recorded by the device inside every step
+    // (mps_set_dof_series) and read once per chunk of at most kHistoryCap steps, with the steps' times and dts.
+    std::vector<int32_t> serKind, serDof;
+    std::vector<size_t> nodeOff;  // first series entry of each node extraction
     for (auto &kv : extract.nodeData()) {
-        std::vector<int32_t> &dst = kv.second->getDataType() == "Unknown" ? pickDof : fintDof;
-        if (kv.second->getDataType() == "Unknown" || kv.second->getDataType() == "InternalForce")
-            for (int32_t n : kv.second->getNodes()) dst.push_back(n + N * kv.second->getComp());
+        const classNodeData &d = *kv.second;
+        const std::string &t = d.getDataType();
+        const int32_t kind = t == "Unknown" ? MPS_SERIES_U : t == "InternalForce" ? MPS_SERIES_FINT : MPS_SERIES_FEXT;
+        nodeOff.push_back(serDof.size());
+        for (int32_t n : d.getNodes()) {
+            serKind.push_back(kind);
+            serDof.push_back(n + N * d.getComp());
+        }
     }
-    const bool nodal = extract.needsNodal();
-    // *Forces of this solver (printForcesOut, solverExplicit.cpp:304): the step's internal force at the listed
-    // (node, direction) pairs whenever the output is due.  No step can be due before the time
-    // iterations*period + t0, and the time advances by at most dtMax per step (every step's dt is
-    // s*min_e min(l0, l)/c <= s*min_e l0/c), so the steps before that time run in chunks.
     std::unique_ptr<classPrintForces> forcesOut;
-    std::vector<int32_t> forceDof;
+    const size_t forceOff = serDof.size();
     std::vector<double> forceVal;
     if (_s.forces.on) {
         forcesOut.reset(new classPrintForces(_s.forces));
-        for (size_t j = 0; j < _s.forces.nodes.size(); j++) forceDof.push_back(_s.forces.nodes[j] + N * _s.forces.dirs[j]);
-        forceVal.resize(forceDof.size());
+        for (size_t j = 0; j < _s.forces.nodes.size(); j++) {
+            serKind.push_back(MPS_SERIES_FINT);
+            serDof.push_back(_s.forces.nodes[j] + N * _s.forces.dirs[j]);
+        }
+        forceVal.resize(_s.forces.nodes.size());
     }
-    const double dtMax = dt0;
-    // external force at the extracted nodes (zero without loads); EXTERNAL_ENERGY = sum_i u_i fext_i
-    // (dataExtraction.cpp:866-874) comes with the other energies from the device history
-    const bool needFext = !ld.empty() && extract.needsExternalForce();
-    std::vector<double> picked(pickDof.size()), fintPicked(fintDof.size()), fext;
-    if (needFext) fext.resize(3 * (size_t)N);
-    std::vector<double> ht(kHistoryCap), hdef(kHistoryCap), hkin(kHistoryCap), hvol(kHistoryCap), hext(kHistoryCap);
+    const size_t nSer = serDof.size();
+    if (mps_set_dof_series(h, (int64_t)nSer, serKind.data(), serDof.data()) != MPS_OK) return fail("mps_set_dof_series");
+    const bool perStep = nSer > 0 || extract.needsEnergies();
+    std::vector<double> series(nSer * (size_t)kHistoryCap);
+    std::vector<double> ht(kHistoryCap), hdt(kHistoryCap), hdef(kHistoryCap), hkin(kHistoryCap), hvol(kHistoryCap),
+        hext(kHistoryCap);
 
-    const long long chunk = nodal ? 1 : (extract.needsEnergies() ? kHistoryCap : 1LL << 40);
+    const long long chunk = perStep ? kHistoryCap : 1LL << 40;
     // VTK series (outputsManagement, solverExplicit.cpp:173-186, 310-322): at t0 when t0 < 1e-12, then
     // whenever `sampler` steps have passed since the last output, and at the last step.
     // The files are written on a thread of their own from device snapshots, while the solve goes on.
@@ -280,12 +285,7 @@ int classSolverExplicit::calculate(const InpModel &m, const std::vector<std::uni
     mps_report rep{};
     for (;;) {
         const long long before = rep.steps;
-        long long n = std::min(chunk, std::max(1LL, sampler - sampler2));
-        if (forcesOut) {
-            const double now = rep.steps > 0 ? rep.time : _s.t0;
-            const double room = (forcesOut->nextThreshold(_s.t0) - now) / dtMax;
-            n = std::min(n, room > 3.0 ? (long long)room - 2 : 1LL);
-        }
+        const long long n = std::min(chunk, std::max(1LL, sampler - sampler2));
         const int rc = mps_run(h, n, &rep);
         if (rc == MPS_ERR_NAN || rep.status == MPS_ERR_NAN) {
             err = "There are NANs in the displacements. Try a smaller scale factor";
@@ -297,57 +297,41 @@ int classSolverExplicit::calculate(const InpModel &m, const std::vector<std::uni
         }
         if (rc != MPS_OK) return fail("mps_run");
         const long long done = rep.steps - before;
-        if (done > 0 && forcesOut && forcesOut->due(rep.time, rep.dt, _s.t0)) {
-            if (mps_get_internal_force_dofs(h, (int64_t)forceDof.size(), forceDof.data(), forceVal.data()) != MPS_OK)
-                return fail("mps_get_internal_force_dofs");
-            if (!forcesOut->write(opt.outputDir, _s.t0, _s.tf, rep.time, forceVal)) {
-                err = "cannot write the *Forces output in " + opt.outputDir;
-                return -1;
-            }
-        }
-        if (done > 0 && needFext && mps_external_force(h, rep.time, fext.data()) != MPS_OK)
-            return fail("mps_external_force");
-        if (done > 0 && extract.needsEnergies()) {
-            if (mps_get_history(h, before, done, ht.data(), nullptr, hdef.data(), hkin.data(), hvol.data()) != MPS_OK)
+        if (done > 0 && perStep) {
+            if (mps_get_history(h, before, done, ht.data(), hdt.data(), hdef.data(), hkin.data(), hvol.data()) != MPS_OK)
                 return fail("mps_get_history");
-            if (mps_get_energy_balance(h, before, done, hext.data(), nullptr, nullptr, nullptr) != MPS_OK)
+            if (extract.needsEnergies() &&
+                mps_get_energy_balance(h, before, done, hext.data(), nullptr, nullptr, nullptr) != MPS_OK)
                 return fail("mps_get_energy_balance");
-            for (long long k = 0; k < done; k++)
-                for (auto &kv : extract.elementData()) {
-                    const std::string &t = kv.second->getDataType();
-                    const double v = t == "DEFO_ENERGY"  ? hdef[k]
-                                     : t == "KIN_ENERGY" ? hkin[k]
-                                     : t == "VOLUME"     ? hvol[k]
-                                                         : hext[k];  // EXTERNAL_ENERGY
-                    if (!kv.second->writeData(opt.outputDir, ht[k], v)) {
-                        err = "cannot write " + kv.first + ".csv in " + opt.outputDir;
-                        return -1;
-                    }
-                }
+            if (nSer > 0 && mps_get_dof_series(h, before, done, series.data()) != MPS_OK)
+                return fail("mps_get_dof_series");
         }
-        if (done > 0 && nodal) {
-            if (!pickDof.empty() &&
-                mps_get_dofs(h, 0, (int64_t)pickDof.size(), pickDof.data(), picked.data()) != MPS_OK)
-                return fail("mps_get_dofs");
-            if (!fintDof.empty() && mps_get_internal_force_dofs(h, (int64_t)fintDof.size(), fintDof.data(),
-                                                                 fintPicked.data()) != MPS_OK)
-                return fail("mps_get_internal_force_dofs");
-            size_t off = 0, offF = 0;
+        for (long long k = 0; perStep && k < done; k++) {
+            const double *row = series.data() + (size_t)k * nSer;
+            if (forcesOut && forcesOut->due(ht[k], hdt[k], _s.t0)) {
+                for (size_t j = 0; j < forceVal.size(); j++) forceVal[j] = row[forceOff + j];
+                if (!forcesOut->write(opt.outputDir, _s.t0, _s.tf, ht[k], forceVal)) {
+                    err = "cannot write the *Forces output in " + opt.outputDir;
+                    return -1;
+                }
+            }
+            for (auto &kv : extract.elementData()) {
+                const std::string &t = kv.second->getDataType();
+                const double v = t == "DEFO_ENERGY"  ? hdef[k]
+                                 : t == "KIN_ENERGY" ? hkin[k]
+                                 : t == "VOLUME"     ? hvol[k]
+                                                     : hext[k];  // EXTERNAL_ENERGY
+                if (!kv.second->writeData(opt.outputDir, ht[k], v)) {
+                    err = "cannot write " + kv.first + ".csv in " + opt.outputDir;
+                    return -1;
+                }
+            }
+            size_t e = 0;
             for (auto &kv : extract.nodeData()) {
                 classNodeData &d = *kv.second;
-                std::vector<double> vals(d.getNodes().size(), 0.0);
-                for (size_t j = 0; j < vals.size(); j++) {
-                    const size_t g = (size_t)d.getNodes()[j] + (size_t)N * d.getComp();
-                    if (d.getDataType() == "Unknown")
-                        vals[j] = picked[off + j];
-                    else if (d.getDataType() == "InternalForce")
-                        vals[j] = fintPicked[offF + j];
-                    else if (d.getDataType() == "ExternalForce" && needFext)
-                        vals[j] = fext[g];
-                }
-                if (d.getDataType() == "Unknown") off += vals.size();
-                if (d.getDataType() == "InternalForce") offF += vals.size();
-                if (!d.writeData(opt.outputDir, rep.time, vals)) {
+                const std::vector<double> vals(row + nodeOff[e], row + nodeOff[e] + d.getNodes().size());
+                e++;
+                if (!d.writeData(opt.outputDir, ht[k], vals)) {
                     err = "cannot write " + kv.first + ".csv in " + opt.outputDir;
                     return -1;
                 }

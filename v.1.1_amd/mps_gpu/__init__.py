@@ -19,6 +19,7 @@ MPS_RUNNING, MPS_DONE = 0, 1
 LAW_NH_COMPRESSIBLE, LAW_NH_BELYTSCHKO, LAW_SVK, LAW_LINEAR_ELASTIC = 0, 1, 2, 3
 BC_RAMP, BC_INST, BC_VELOCITY, BC_RAMP_RATE = 0, 1, 2, 3
 PREC_FP64, PREC_FP32_ELEM = 0, 1
+SERIES_U, SERIES_V, SERIES_A, SERIES_FINT, SERIES_FEXT = 0, 1, 2, 3, 4  # MPS_SERIES_*
 DOFS_STATIC_MAX = 65536  # MPS_DOFS_STATIC_MAX
 
 _dp = C.POINTER(C.c_double)
@@ -135,6 +136,8 @@ def load(path=None):
     L.mps_element_outputs.argtypes = [H, _dp]
     _cdpp = C.POINTER(C.POINTER(C.c_double))
     L.mps_snapshot_begin.argtypes = [H, C.c_int32, C.c_int32]
+    L.mps_set_dof_series.argtypes = [H, C.c_int64, _ip, _ip]
+    L.mps_get_dof_series.argtypes = [H, C.c_int64, C.c_int64, _dp]
     L.mps_snapshot_wait.argtypes = [H, C.c_int32, _cdpp, _cdpp, _cdpp, _cdpp]
     L.mps_snapshot_release.argtypes = [H, C.c_int32]
     L.mps_set_loads.argtypes = [H, C.c_int32, C.POINTER(Load)]
@@ -355,6 +358,22 @@ class Solver:
         out = np.zeros(21 * self.ne)
         self._chk(self.L.mps_element_outputs(self.h, _d(out)))
         return out.reshape(-1, 21)
+
+    def set_dof_series(self, which, dof):
+        """Per-step series recorded on the device: which[i] in SERIES_* at DOF dof[i] = node + n_nodes*dir."""
+        which = np.ascontiguousarray(which, dtype=np.int32)
+        dof = np.ascontiguousarray(dof, dtype=np.int32)
+        assert which.size == dof.size
+        self._chk(self.L.mps_set_dof_series(self.h, dof.size, _i(which), _i(dof)))
+        self._nser = dof.size
+
+    def dof_series(self, first, count):
+        """Rows [first, first+count) (row = step - 1) of the series: array [count, n]."""
+        n = getattr(self, "_nser", 0)
+        out = np.zeros((count, n))
+        if n and count:
+            self._chk(self.L.mps_get_dof_series(self.h, first, count, _d(out)))
+        return out
 
     def snapshot_begin(self, slot, with_elements=True):
         """Stage u, v, a (and the element outputs) of the last step for a copy that runs while stepping goes on."""

@@ -2,6 +2,7 @@
 #include "vtk_output.h"
 
 #include <algorithm>
+#include <charconv>
 #include <cstdio>
 #include <fstream>
 #include <thread>
@@ -9,15 +10,16 @@
 namespace mps {
 namespace {
 
-// Buffered text writer; numbers as `stream << double` with the default precision (%.6g).
+// Buffered text writer; numbers as `stream << double` with the default precision, i.e. printf's %.6g, which
+// std::to_chars in general format with precision 6 reproduces character for character (and faster).
 struct Out {
     std::string s;
     void str(const char *t) { s += t; }
     void str(const std::string &t) { s += t; }
     void num(double x) {
         char b[32];
-        std::snprintf(b, sizeof b, "%.6g", x);
-        s += b;
+        const auto r = std::to_chars(b, b + sizeof b, x, std::chars_format::general, 6);
+        s.append(b, r.ptr);
     }
     void integer(long long x) { s += std::to_string(x); }
     void nl() { s += '\n'; }

@@ -45,9 +45,13 @@ __device__ __forceinline__ T det3(const T* A) {
            A[2] * (A[3] * A[7] - A[6] * A[4]);
 }
 
+__device__ __forceinline__ double rcp(double x);
+__device__ __forceinline__ float rcp(float x);
+
+// Inverse by cofactors; 1/det by the hardware reciprocal and Newton steps (rcp below).
 template <typename T>
 __device__ __forceinline__ void inv3(const T* A, T* R, T det) {
-    T idet = T(1) / det;
+    T idet = rcp(det);
     R[0] = idet * (A[4] * A[8] - A[5] * A[7]);
     R[3] = -idet * (A[3] * A[8] - A[5] * A[6]);
     R[6] = idet * (A[3] * A[7] - A[4] * A[6]);
@@ -627,15 +631,23 @@ __device__ __forceinline__ void hex_scale(const double (*A)[3], T (*As)[3]) {
     }
 }
 
-// Gradient matrix D(i,j) = d(.)_i / d xi_j at Gauss point with signs (sx,sy,sz).
+// Sign factors of a Gauss point of the 2x2x2 rule, in the reference order (GPsDistribution.cpp:392-432:
+// xi positive at points 1, 2, 5, 6, eta at 2, 3, 6, 7, zeta at 4-7), and their pairwise products, as +-1
+// built from the point index by integer operations.  In a loop over the points the index is uniform, so
+// the factors stay scalar operands of the FMAs instead of vector products.
 template <typename T>
-__device__ __forceinline__ void hex_grad_at(const T (*As)[3], int sx, int sy, int sz, T* D) {
-#pragma unroll
-    for (int i = 0; i < 3; i++) {
-        D[i * 3 + 0] = As[0][i] + T(sy) * As[3][i] + T(sz) * As[5][i] + T(sy * sz) * As[6][i];
-        D[i * 3 + 1] = As[1][i] + T(sx) * As[3][i] + T(sz) * As[4][i] + T(sx * sz) * As[6][i];
-        D[i * 3 + 2] = As[2][i] + T(sx) * As[5][i] + T(sy) * As[4][i] + T(sx * sy) * As[6][i];
-    }
+struct GpSigns {
+    T x, y, z, yz, xz, xy;
+};
+__device__ __forceinline__ double unit_signed(double, unsigned neg) {
+    return bits2d(0x3FF0000000000000ull | ((unsigned long long)neg << 63));
+}
+__device__ __forceinline__ float unit_signed(float, unsigned neg) { return __uint_as_float(0x3F800000u | (neg << 31)); }
+template <typename T>
+__device__ __forceinline__ GpSigns<T> gp_signs(int gp) {
+    const unsigned nx = ((gp ^ (gp >> 1)) & 1) ^ 1, ny = ((gp >> 1) & 1) ^ 1, nz = ((gp >> 2) & 1) ^ 1;
+    return {unit_signed(T(), nx),      unit_signed(T(), ny),      unit_signed(T(), nz),
+            unit_signed(T(), ny ^ nz), unit_signed(T(), nx ^ nz), unit_signed(T(), nx ^ ny)};
 }
 
 // Material parameters of element e: (pa, pb) = (lambda, mu) [compressible] or (mu, K1) [Belytschko].
@@ -721,12 +733,12 @@ __device__ __forceinline__ double hex_kinetic(const MpsInst& I, const int* nd, d
 // Jacobian (or displacement gradient) D(i,j) = d(.)_i/d xi_j at the Gauss point with signs (sx, sy, sz) from the
 // scaled butterfly coefficients A (hex_scale).
 template <typename T>
-__device__ __forceinline__ void hex_jac(const T (*A)[3], T sx, T sy, T sz, T* D) {
+__device__ __forceinline__ void hex_jac(const T (*A)[3], const GpSigns<T>& s, T* D) {
 #pragma unroll
     for (int i = 0; i < 3; i++) {
-        D[i * 3 + 0] = A[0][i] + sy * A[3][i] + sz * A[5][i] + (sy * sz) * A[6][i];
-        D[i * 3 + 1] = A[1][i] + sx * A[3][i] + sz * A[4][i] + (sx * sz) * A[6][i];
-        D[i * 3 + 2] = A[2][i] + sx * A[5][i] + sy * A[4][i] + (sx * sy) * A[6][i];
+        D[i * 3 + 0] = A[0][i] + s.y * A[3][i] + s.z * A[5][i] + s.yz * A[6][i];
+        D[i * 3 + 1] = A[1][i] + s.x * A[3][i] + s.z * A[4][i] + s.xz * A[6][i];
+        D[i * 3 + 2] = A[2][i] + s.x * A[5][i] + s.y * A[4][i] + s.xy * A[6][i];
     }
 }
 
@@ -807,12 +819,10 @@ __device__ __forceinline__ double hex8_element(const MpsInst& I, int e, const do
 #pragma unroll 1
     for (int gp = 0; gp < 8; gp++) {
         // Gauss point signs in the reference order (GPsDistribution.cpp:392-432)
-        const T sx = (gp == 1 || gp == 2 || gp == 5 || gp == 6) ? T(1) : T(-1);
-        const T sy = (gp == 2 || gp == 3 || gp == 6 || gp == 7) ? T(1) : T(-1);
-        const T sz = (gp >= 4) ? T(1) : T(-1);
+        const GpSigns<T> s = gp_signs<T>(gp);
         T Jm[9], Du[9], H[9], P[9];
-        hex_jac<T>(Ax, sx, sy, sz, Jm);
-        hex_jac<T>(Au, sx, sy, sz, Du);
+        hex_jac<T>(Ax, s, Jm);
+        hex_jac<T>(Au, s, Du);
         // J^-1 = cof(J)^T / det J ; weight 1 (GPsDistribution.cpp:392-432) so wJ J^-T = sign(det J) cof(J)
         T cJ[9];
         cof3(Jm, cJ);
@@ -845,9 +855,9 @@ __device__ __forceinline__ double hex8_element(const MpsInst& I, int e, const do
                 const T a0 = vs[(0 * 3 + i) * MPS_PATCH], a1 = vs[(1 * 3 + i) * MPS_PATCH], a2 = vs[(2 * 3 + i) * MPS_PATCH];
                 const T a3 = vs[(3 * 3 + i) * MPS_PATCH], a4 = vs[(4 * 3 + i) * MPS_PATCH], a5 = vs[(5 * 3 + i) * MPS_PATCH];
                 const T a6 = vs[(6 * 3 + i) * MPS_PATCH];
-                tr += (a0 + sy * a3 + sz * a5 + (sy * sz) * a6) * cJ[i * 3 + 0] +
-                      (a1 + sx * a3 + sz * a4 + (sx * sz) * a6) * cJ[i * 3 + 1] +
-                      (a2 + sx * a5 + sy * a4 + (sx * sy) * a6) * cJ[i * 3 + 2];
+                tr += (a0 + s.y * a3 + s.z * a5 + s.yz * a6) * cJ[i * 3 + 0] +
+                      (a1 + s.x * a3 + s.z * a4 + s.xz * a6) * cJ[i * 3 + 1] +
+                      (a2 + s.x * a5 + s.y * a4 + s.xy * a6) * cJ[i * 3 + 2];
             }
             add_bulk_visc<T>(H, tr * rJ * trs, qa, qb, P);
         }
@@ -864,15 +874,16 @@ __device__ __forceinline__ double hex8_element(const MpsInst& I, int e, const do
         for (int i = 0; i < 3; i++)
 #pragma unroll
             for (int k = 0; k < 3; k++) Q[k][i] = P[i * 3] * cJ[k] + P[i * 3 + 1] * cJ[3 + k] + P[i * 3 + 2] * cJ[6 + k];
+        // moment sums, one FMA per (moment, column) pair: a +-1 factor makes each product exact
 #pragma unroll
         for (int i = 0; i < 3; i++) {
             G[0][i] += Q[0][i];
             G[1][i] += Q[1][i];
             G[2][i] += Q[2][i];
-            G[3][i] += sy * Q[0][i] + sx * Q[1][i];
-            G[4][i] += sz * Q[1][i] + sy * Q[2][i];
-            G[5][i] += sz * Q[0][i] + sx * Q[2][i];
-            G[6][i] += (sy * sz) * Q[0][i] + (sx * sz) * Q[1][i] + (sx * sy) * Q[2][i];
+            G[3][i] = fma_t(s.x, Q[1][i], fma_t(s.y, Q[0][i], G[3][i]));
+            G[4][i] = fma_t(s.y, Q[2][i], fma_t(s.z, Q[1][i], G[4][i]));
+            G[5][i] = fma_t(s.x, Q[2][i], fma_t(s.z, Q[0][i], G[5][i]));
+            G[6][i] = fma_t(s.xy, Q[2][i], fma_t(s.xz, Q[1][i], fma_t(s.yz, Q[0][i], G[6][i])));
         }
     }
     hex_unbutterfly_scaled<T>(G, (T)0.125, (T)(0.125 * MPS_GP), (T)(0.125 * MPS_GP * MPS_GP), f);
@@ -1230,28 +1241,9 @@ __device__ __forceinline__ void load_partial(const MpsInst& I, int g, double* c)
     }
 }
 
-// Gather + lumped-mass solve of node n at step k (nonlinearSystem.cpp:1096-1107; rhs = -fint + fext), and the
-// prescribed velocity and zero acceleration of its constrained DOFs (applyDisplacementBC, solverExplicit.cpp:
-// 568-571).  The node's internal force is the sum of its patch partials in ascending patch order.  With
-// fi_out / fe_out the internal and external forces of the node are returned as well (also for a fully
-// constrained node).
-__device__ __forceinline__ void solve_node(const MpsInst& I, int n, long long k, double dt, double* v, double* a,
-                                           double* fi_out = nullptr, double* fe_out = nullptr) {
-#pragma clang fp contract(off)
-    const uint32_t info = I.ninfo[n];
-    const uint32_t mask = info & 7u;
-    if (mask) {
-        const uint32_t rec = (info & MPS_REC_MASK) >> 3;
-#pragma unroll
-        for (int i = 0; i < 3; i++)
-            if ((mask >> i) & 1u) {
-                double U, V;
-                bc_motion(I, I.bcval[3 * (size_t)rec + i], k, dt, &U, &V);
-                v[i] = V;
-                a[i] = 0.;
-            }
-    }
-    if (mask == 7u && !fi_out) return;
+// Internal force of node n: the sum of its patch partials in ascending patch order, in FP64 (the precision
+// form's FP32 partials widened first).
+__device__ __forceinline__ void node_partial_sum(const MpsInst& I, int n, double* f) {
     int q = I.nrow[n];
     const int q1 = I.nrow[n + 1];
     double f0 = 0.0, f1 = 0.0, f2 = 0.0;
@@ -1289,40 +1281,74 @@ __device__ __forceinline__ void solve_node(const MpsInst& I, int n, long long k,
             f2 += c[2];
         }
     }
-    double fe[3];
-    node_fext(I, n, info, fe);
-    double m = I.mass[n];
-    double fi[3] = {f0, f1, f2};
-    if (fi_out) {
+    f[0] = f0;
+    f[1] = f1;
+    f[2] = f2;
+}
+
+// rhs = -fint + fext (nonlinearSystem.cpp:1096-1107).
+__device__ __forceinline__ void node_rhs(const double* fi, const double* fe, double* rhs) {
 #pragma unroll
-        for (int i = 0; i < 3; i++) {
-            fi_out[i] = fi[i];
-            fe_out[i] = fe[i];
-        }
+    for (int i = 0; i < 3; i++) rhs[i] = -fi[i] + fe[i];
+}
+
+// Update of node n's velocity and acceleration at step k: the prescribed velocity and zero acceleration of its
+// constrained DOFs (applyDisplacementBC, solverExplicit.cpp:568-571), the lumped-mass solve of the free ones
+// (nonlinearSystem.cpp:1096-1107) and the trapezoidal velocity update.  rhs = -fint + fext of the node (read
+// for free DOFs only).  With wkin, 1/2 m (V^2 - v^2) of each constrained DOF, whose velocity v the prescribed
+// V replaces, is added to *wkin.
+__device__ __forceinline__ void node_update(const MpsInst& I, uint32_t info, long long k, double dt, double m,
+                                            const double* rhs, double* v, double* a, double* wkin = nullptr) {
+#pragma clang fp contract(off)
+    const uint32_t mask = info & 7u;
+    if (mask) {
+        const uint32_t rec = (info & MPS_REC_MASK) >> 3;
+#pragma unroll
+        for (int i = 0; i < 3; i++)
+            if ((mask >> i) & 1u) {
+                double U, V;
+                bc_motion(I, I.bcval[3 * (size_t)rec + i], k, dt, &U, &V);
+                if (wkin) *wkin += 0.5 * m * (V * V - v[i] * v[i]);
+                v[i] = V;
+                a[i] = 0.;
+            }
     }
 #pragma unroll
     for (int i = 0; i < 3; i++) {
         if ((mask >> i) & 1u) continue;
-        double rhs = -fi[i] + fe[i];
-        double an = (m != 0.0) ? rhs / m : 0.0;
+        double an = (m != 0.0) ? rhs[i] / m : 0.0;
         v[i] += dt * ((1. - 0.5) * a[i] + 0.5 * an);
         a[i] = an;
     }
 }
 
-// Energy bookkeeping of node n at step k, after its solve (u: displacement of the step, v: its new velocity;
-// vold, incr: the velocity before the solve and the step's increment; fi, fe: internal and external force).
-// Adds to red[MPS_NODE_PARTS]: the work of the external force over the step, 1/2 (fext_prev + fext) . incr; the
-// work of the supports, 1/2 (r_prev + r) . incr + 1/2 m (v^2 - vold^2) over the constrained DOFs with
-// r = fint - fext (the update sets a = 0 there, so the momentum the prescribed velocity gives the node's mass
-// is the support's); the lumped kinetic energy 1/2 m v^2; and u . fext (EXTERNAL_ENERGY, dataExtraction.cpp:
-// 867-875).  Works are counted from step 2 on; every step stores the forces the next one takes as previous.
-__device__ __forceinline__ void node_energy(const MpsInst& I, int n, long long k, const double* u, const double* v,
-                                            const double* vold, const double* incr, const double* fi,
-                                            const double* fe, double* red) {
+// Gather + lumped-mass solve of node n at step k (node_partial_sum, node_fext, node_update).  A fully
+// constrained node reads no forces.
+__device__ __forceinline__ void solve_node(const MpsInst& I, int n, long long k, double dt, double* v, double* a) {
     const uint32_t info = I.ninfo[n];
+    double rhs[3] = {0.0, 0.0, 0.0};
+    if ((info & 7u) != 7u) {
+        double fi[3], fe[3];
+        node_partial_sum(I, n, fi);
+        node_fext(I, n, info, fe);
+        node_rhs(fi, fe, rhs);
+    }
+    node_update(I, info, k, dt, I.mass[n], rhs, v, a);
+}
+
+// Energy bookkeeping of node n at step k, in two parts around its velocity update so that few values live
+// through it.  Adds to red[MPS_NODE_PARTS]: the work of the external force over the step,
+// 1/2 (fext_prev + fext) . incr; the work of the supports, 1/2 (r_prev + r) . incr + 1/2 m (v^2 - vold^2) over the
+// constrained DOFs with r = fint - fext (the update sets a = 0 there, so the momentum the prescribed velocity
+// gives the node's mass is the support's); the lumped kinetic energy 1/2 m v^2; and u . fext (EXTERNAL_ENERGY,
+// dataExtraction.cpp:867-875).  Works are counted from step 2 on; every step stores the forces the next one
+// takes as previous.
+// Before the update: the force terms (u: displacement of the step, incr: its increment; fi, fe: internal and
+// external force).
+__device__ __forceinline__ void node_energy_forces(const MpsInst& I, int n, uint32_t info, long long k,
+                                                   const double* u, const double* incr, const double* fi,
+                                                   const double* fe, double* red) {
     const uint32_t mask = info & 7u;
-    const double m = I.mass[n];
     double fp[3] = {fe[0], fe[1], fe[2]};  // unflagged nodes: gravity alone (constant) or nothing
     if (info & MPS_NODE_LOADED) {
         double* lp = I.lfprev + 3 * (size_t)I.lidx[n];
@@ -1332,11 +1358,10 @@ __device__ __forceinline__ void node_energy(const MpsInst& I, int n, long long k
             lp[i] = fe[i];
         }
     }
-    double we = 0.0, wr = 0.0, ek = 0.0, ee = 0.0;
+    double we = -0.0, wr = -0.0, ee = -0.0;  // -0.0: the identity of the additions
 #pragma unroll
     for (int i = 0; i < 3; i++) {
         we += 0.5 * (fp[i] + fe[i]) * incr[i];
-        ek += 0.5 * m * v[i] * v[i];
         ee += u[i] * fe[i];
     }
     if (mask) {
@@ -1345,7 +1370,7 @@ __device__ __forceinline__ void node_energy(const MpsInst& I, int n, long long k
         for (int i = 0; i < 3; i++)
             if ((mask >> i) & 1u) {
                 const double r = fi[i] - fe[i];
-                wr += 0.5 * (rp[i] + r) * incr[i] + 0.5 * m * (v[i] * v[i] - vold[i] * vold[i]);
+                wr += 0.5 * (rp[i] + r) * incr[i];
                 rp[i] = r;
             }
     }
@@ -1353,8 +1378,16 @@ __device__ __forceinline__ void node_energy(const MpsInst& I, int n, long long k
         red[0] += we;
         red[1] += wr;
     }
-    red[2] += ek;
     red[3] += ee;
+}
+// During the update (node_update's wkin): the kinetic term of the supports.  After it: the lumped kinetic
+// energy (v: the new velocity).
+__device__ __forceinline__ void node_energy_motion(double m, long long k, double wkin, const double* v, double* red) {
+    double ek = -0.0;
+#pragma unroll
+    for (int i = 0; i < 3; i++) ek += 0.5 * m * v[i] * v[i];
+    if (k >= 2) red[1] += wkin;
+    red[2] += ek;
 }
 
 // ---------------------------------------------------------------------------
@@ -1449,7 +1482,9 @@ __device__ __forceinline__ void publish_block_sum(double* vals, double* __restri
 // block_sum.  Totals in out of thread 0.
 template <int NV>
 __device__ __forceinline__ void reduce_parts(const double* __restrict__ part, int nb, double* out, double* sh) {
-    constexpr int UN = 8;
+    // 4 block sums per work-item in flight: more would set the register count of the whole kernel (the node
+    // kernel of an energy step calls this from one block); the order of the additions does not depend on it
+    constexpr int UN = 4;
     double acc[NV];
 #pragma unroll
     for (int j = 0; j < NV; j++) acc[j] = 0.0;

@@ -211,8 +211,21 @@ __host__ __device__ __forceinline__ size_t tet_shared_bytes(int pu, int tsize) {
     return a > b ? a : b;
 }
 
+// A staged node of the tet4 kernel: X and u of the patch's j-th distinct node at sxu[6j..6j+5], read as three
+// 16-byte pieces (one address per node, the pieces at fixed offsets from it).
+__device__ __forceinline__ void staged_node(const double* __restrict__ sxu, int j, double* x, double* u) {
+    const double2* s = reinterpret_cast<const double2*>(sxu + 6 * j);
+    const double2 a = s[0], b = s[1], c = s[2];
+    x[0] = a.x;
+    x[1] = a.y;
+    x[2] = b.x;
+    u[0] = b.y;
+    u[1] = c.x;
+    u[2] = c.y;
+}
+
 // Tet4 element kernel with node staging (inviscid).  Each block first loads the U_p distinct nodes of
-// its patch (ascending ids, so the loads run along node rows) into LDS, X and u SoA by direction; each
+// its patch (ascending ids, so the loads run along node rows) into LDS, 48 bytes per node; each
 // element then reads its four nodes from LDS through its local indices plx.  Every node is fetched
 // once per patch instead of once per element slot (about 1 against 5.8 node loads per partial on a
 // structured tet mesh).  Dynamic LDS: the staging (48*pu_max bytes) and, after a barrier, the 12*MPS_PATCH
@@ -220,11 +233,11 @@ __host__ __device__ __forceinline__ size_t tet_shared_bytes(int pu, int tsize) {
 // hold their forces in registers across that barrier, and the patch sums read the node offsets from global
 // memory.  So 6 (FP64) or 8 (FP32) workgroups reside per CU.
 template <typename T, int LAW, bool EN = false>
-__global__ __launch_bounds__(MPS_PATCH) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 5 : 4))) void k_elem_tet_staged(MpsInst I, int mode, unsigned eflags) {
+__global__ __launch_bounds__(MPS_PATCH) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 8 : 4))) void k_elem_tet_staged(MpsInst I, int mode, unsigned eflags) {
     extern __shared__ __align__(16) double smem[];  // 16-byte pieces are stored at offsets that are multiples of 16
     __shared__ unsigned long long sh[MPS_PATCH / 64];
     const int pu = I.pu_max;
-    double* __restrict__ sxu = smem;           // [6][pu]: X (3), then u (3)
+    double* __restrict__ sxu = smem;           // [pu][6]: X (3), then u (3) of each staged node
     T* __restrict__ sf = reinterpret_cast<T*>(smem);  // [12][MPS_PATCH], over the staging once it is read
     const size_t shared_bytes = tet_shared_bytes(pu, sizeof(T));
     uint16_t* __restrict__ spc = reinterpret_cast<uint16_t*>(reinterpret_cast<char*>(smem) + shared_bytes);
@@ -250,11 +263,12 @@ __global__ __launch_bounds__(MPS_PATCH) __attribute__((amdgpu_waves_per_eu(sizeo
     const TetConsts kc = tet_consts(I, e, mode == 0);
     for (int j = threadIdx.x; j < U; j += MPS_PATCH) {
         const size_t n = (size_t)I.pnd[base + j];
-#pragma unroll
-        for (int d = 0; d < 3; d++) {
-            sxu[d * pu + j] = I.X[d * N + n];
-            sxu[(3 + d) * pu + j] = I.u[d * N + n];
-        }
+        const double x0 = I.X[n], x1 = I.X[N + n], x2 = I.X[2 * N + n];
+        const double u0 = I.u[n], u1 = I.u[N + n], u2 = I.u[2 * N + n];
+        double2* s = reinterpret_cast<double2*>(sxu + 6 * j);
+        s[0] = make_double2(x0, x1);
+        s[1] = make_double2(x2, u0);
+        s[2] = make_double2(u1, u2);
     }
     pl.store(spc, nullptr, ne * 4);  // the list arrives with the nodes; stored now, it holds no registers below
     __syncthreads();
@@ -270,21 +284,19 @@ __global__ __launch_bounds__(MPS_PATCH) __attribute__((amdgpu_waves_per_eu(sizeo
         if (sizeof(T) == 8 && mode == 0) {
             double xd[4][3];
 #pragma unroll
-            for (int a = 0; a < 4; a++)
+            for (int a = 0; a < 4; a++) {
+                double xa[3], ua[3];
+                staged_node(sxu, l[a], xa, ua);
 #pragma unroll
-                for (int d = 0; d < 3; d++) xd[a][d] = sxu[d * pu + l[a]] + sxu[(3 + d) * pu + l[a]];
+                for (int d = 0; d < 3; d++) xd[a][d] = xa[d] + ua[d];
+            }
             dt64 = elem_dt(tet_char_length(xd), kc.l0, kc.c);
             asm volatile("" ::: "memory");
         }
         const bool core_dt = sizeof(T) == 4 && mode == 0;
         double x[4][3], uu[4][3];
 #pragma unroll
-        for (int a = 0; a < 4; a++)
-#pragma unroll
-            for (int d = 0; d < 3; d++) {
-                x[a][d] = sxu[d * pu + l[a]];
-                uu[a][d] = sxu[(3 + d) * pu + l[a]];
-            }
+        for (int a = 0; a < 4; a++) staged_node(sxu, l[a], x[a], uu[a]);
         double dte;
         if constexpr (EN) {
             double r[4];
@@ -367,7 +379,7 @@ __device__ __forceinline__ void write_hist_step(const MpsInst& I, long long k, c
     }
 }
 
-// EN: energy bookkeeping of the node into red[MPS_NODE_PARTS] (node_energy).  Returns false when the step is
+// EN: energy bookkeeping of the node into red[MPS_NODE_PARTS] (node_energy_forces, node_energy_motion).  Returns false when the step is
 // not running (the same for every work-item of the launch).
 template <bool EN = false>
 __device__ __forceinline__ bool node_item(const MpsInst& I, int n, bool first, int fuse, double* red = nullptr) {
@@ -398,15 +410,23 @@ __device__ __forceinline__ bool node_item(const MpsInst& I, int n, bool first, i
         a[i] = I.a[i * N + n];
     }
     if (EN) {
-        double vold[3], incr[3], fi[3], fe[3];
+        // the forces first (also of a fully constrained node, for the work of the supports), then the step's
+        // increments from v and a before their update: nothing extra is held through the partial sums
         const uint32_t info = I.ninfo[n];
+        double rhs[3];
+        {
+            double fi[3], fe[3], incr[3];
+            node_partial_sum(I, n, fi);
+            node_fext(I, n, info, fe);
 #pragma unroll
-        for (int i = 0; i < 3; i++) {
-            vold[i] = v[i];
-            incr[i] = step_increment(I, info, i, k, s.dt, v[i], a[i]);
+            for (int i = 0; i < 3; i++) incr[i] = step_increment(I, info, i, k, s.dt, v[i], a[i]);
+            node_energy_forces(I, n, info, k, u, incr, fi, fe, red);
+            node_rhs(fi, fe, rhs);
         }
-        solve_node(I, n, k, s.dt, v, a, fi, fe);
-        node_energy(I, n, k, u, v, vold, incr, fi, fe, red);
+        const double m = I.mass[n];
+        double wkin = -0.0;
+        node_update(I, info, k, s.dt, m, rhs, v, a, &wkin);
+        node_energy_motion(m, k, wkin, v, red);
     } else {
         solve_node(I, n, k, s.dt, v, a);
     }
@@ -470,7 +490,9 @@ __global__ __launch_bounds__(256) void k_node(MpsInst I, int fuse, unsigned nfla
         return;
     }
     __shared__ double sh[MPS_NODE_PARTS * 256];
-    double red[MPS_NODE_PARTS] = {0.0, 0.0, 0.0, 0.0};
+    // sums start at -0.0, the identity of FP64 addition (0.0 + x is not x for x = -0.0), so that the compiler
+    // can drop the first additions and hold no zeros through the node work
+    double red[MPS_NODE_PARTS] = {-0.0, -0.0, -0.0, -0.0};
     const long long k = I.ctl->B;  // read before the first work-item advances the step counters
     if (!node_item<true>(I, n < I.N ? n : -1, first, fuse, red)) return;
     const int nb = (int)gridDim.x;
@@ -901,18 +923,16 @@ __device__ __forceinline__ void elem_energy(const MpsInst& I, int e, unsigned fl
         double wJ[8];
 #pragma unroll 1
         for (int gp = 0; gp < 8; gp++) {
-            const double sx = (gp == 1 || gp == 2 || gp == 5 || gp == 6) ? 1.0 : -1.0;
-            const double sy = (gp == 2 || gp == 3 || gp == 6 || gp == 7) ? 1.0 : -1.0;
-            const double sz = (gp >= 4) ? 1.0 : -1.0;
+            const GpSigns<double> s = gp_signs<double>(gp);
             double Jm[9], cJ[9];
-            hex_jac<double>(Ax, sx, sy, sz, Jm);
+            hex_jac<double>(Ax, s, Jm);
             cof3(Jm, cJ);
             const double dJ = Jm[0] * cJ[0] + Jm[1] * cJ[1] + Jm[2] * cJ[2];
             const double w = fabs(dJ);
             wJ[gp] = w;
             if (flags & MPS_E_DEF) {
                 double Du[9], H[9], P[9], Wg, Jg;
-                hex_jac<double>(Au, sx, sy, sz, Du);
+                hex_jac<double>(Au, s, Du);
                 const double rJ = rcp(dJ);
 #pragma unroll
                 for (int i = 0; i < 3; i++)
@@ -1018,10 +1038,10 @@ __global__ __launch_bounds__(256) void k_elem_outputs(MpsInst I, double* __restr
         hex_scale<double>(cx, Ax);
         hex_scale<double>(cu, Au);
         for (int gp = 0; gp < 8; gp++) {
-            const int sx = sxi(gp), sy = seta(gp), sz = szeta(gp);
+            const GpSigns<double> s = gp_signs<double>(gp);
             double Jm[9], Du[9], iJ[9], F[9];
-            hex_grad_at<double>(Ax, sx, sy, sz, Jm);
-            hex_grad_at<double>(Au, sx, sy, sz, Du);
+            hex_jac<double>(Ax, s, Jm);
+            hex_jac<double>(Au, s, Du);
             inv3(Jm, iJ, det3(Jm));
             for (int i = 0; i < 3; i++)
                 for (int j = 0; j < 3; j++)

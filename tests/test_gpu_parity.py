@@ -135,6 +135,42 @@ def test_tet_node_staging_bitwise(precision, law, drop_first, monkeypatch):
     assert relerr(out[True][2], op.internal_force(uf)) <= (1e-12 if precision == G.PREC_FP64 else 1e-5)
 
 
+@pytest.mark.parametrize("precision,law,energies", [(G.PREC_FP64, 0, False), (G.PREC_FP64, 1, False),
+                                                     (G.PREC_FP64, 0, True), (G.PREC_FP32_ELEM, 0, False)])
+def test_hex_node_gathers_by_offset_bitwise(precision, law, energies, monkeypatch):
+    """The hex8 element kernels that address the gathered nodes by 32-bit offsets from scalar bases (meshes of
+    at most 2^29 nodes) give bit for bit the results of the 64-bit-address kernels (MPS_GATHER32=0, the ones
+    larger meshes run): displacements, velocities, dt history, the standalone internal force and, with
+    energies every step, the energy history; over a ragged last patch and per-element materials."""
+    m = perturb(box(9, 8, 7), 0.12, 21 + law)
+    assert m["n_elem"] % 256 != 0
+    rng = np.random.default_rng(6)
+    E = rng.uniform(5e5, 2e6, m["n_elem"])
+    nu = rng.uniform(0.2, 0.4, m["n_elem"])
+    dd, dv = clamp_and_pull(m)
+    uf = rng.uniform(-1e-3, 1e-3, 3 * m["n_nodes"])
+    keys = ("dt", "e_def", "e_kin", "volume") if energies else ("dt",)
+    out = {}
+    for o32 in (True, False):
+        if o32:
+            monkeypatch.delenv("MPS_GATHER32", raising=False)
+        else:
+            monkeypatch.setenv("MPS_GATHER32", "0")
+        with G.Solver(m, law=law, E=E, nu=nu, dir_dof=dd, dir_val=dv, n_steps=120, precision=precision,
+                      energy_every=1 if energies else 0, history_cap=200) as s:
+            f = s.internal_force(uf)
+            rc, rep = s.run(120)
+            assert rc == 0 and rep["steps"] == 120
+            u, v, _a = s.get_state()
+            h = s.history(0, 120)
+            out[o32] = [u, v, f] + [h[k] for k in keys]
+    for k in range(len(out[True])):
+        assert np.array_equal(out[True][k], out[False][k], equal_nan=True), k
+    if precision == G.PREC_FP64 and not energies:
+        op = O.OracleProblem(m, law=law, E=E, nu=nu, dir_dof=dd, dir_val=dv, n_steps=120, sum_order=O.SUM_REFERENCE)
+        assert relerr(out[True][0], op.run(hist_cap=120)["u"]) <= TOL64
+
+
 @pytest.mark.parametrize("precision", [G.PREC_FP64, G.PREC_FP32_ELEM])
 def test_tet_without_shared_nodes_uses_the_gathering_kernel(precision):
     """Tets that share no nodes: a patch of 256 tets has 1024 distinct nodes, whose staging (48 KB) and

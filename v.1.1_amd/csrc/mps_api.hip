@@ -394,6 +394,12 @@ static int upload_instance(mps_solver_s* h, const HostPrep& hp) {
     MpsInst& I = h->I;
     const size_t N = hp.N, Ne = hp.Ne, nen = hp.nen;
     I.N = hp.N;
+    // node gathers of the hex8 element kernels by 32-bit byte offsets from scalar bases when every offset fits;
+    // the environment variable MPS_GATHER32=0 keeps 64-bit addresses (same loads, same results bit for bit)
+    {
+        const char* g32 = getenv("MPS_GATHER32");
+        I.off32 = (hp.N <= MPS_OFF32_NODES && !(g32 && g32[0] == '0')) ? 1 : 0;
+    }
     I.Ne = hp.Ne;
     I.nen = hp.nen;
     I.law = hp.law;

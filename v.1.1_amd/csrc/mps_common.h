@@ -97,6 +97,9 @@ struct MpsCtl {
     long long wred;             // last step whose node-kernel energy sums were reduced
 };
 
+// Largest node count whose byte offsets within one direction array (8 * node) fit 32 bits.
+constexpr long long MPS_OFF32_NODES = 1ll << 29;
+
 struct MpsInst {
     int32_t N, Ne, nen, law;
     int32_t precision, bc_mode, vel_ramp, time_mode;
@@ -157,6 +160,8 @@ struct MpsInst {
     const int32_t* pnd;
     const uint16_t* plx;
     int32_t pu_max;  // largest number of distinct nodes of a patch
+    // 1: the hex8 element kernels gather nodes by 32-bit byte offsets (at most MPS_OFF32_NODES nodes)
+    int32_t off32;
     // per-step DOF series (mps_set_dof_series): n_series entries, recorded into ser_ring[(k-1)%hist_cap][i]
     int32_t n_series;
     const int32_t* ser_dof;   // node + N*dir

@@ -694,16 +694,46 @@ __device__ __forceinline__ void visc_coeffs(const MpsInst& I, int e, double c, d
     *qb = rho * h * I.visc_cl * c0;
 }
 
+// Gather of nodal 3-vectors stored by direction (p[d*N + n]) at the nodes nd[0..NN-1], from one array (P) or,
+// with TWO, from two (P and Q).  O32 (meshes of at most MPS_OFF32_NODES nodes): a node's value is addressed by a
+// 32-bit byte offset from the direction's base address, a scalar, so that one shift per node serves every
+// direction and array; otherwise by 64-bit addresses (one 64-bit add per value).
+template <int NN, bool O32, bool TWO = false>
+__device__ __forceinline__ void gather_nodes(size_t N, const int* nd, const double* p, double (*P)[3],
+                                             const double* q = nullptr, double (*Q)[3] = nullptr) {
+    if constexpr (O32) {
+        uint32_t off[NN];
+#pragma unroll
+        for (int a = 0; a < NN; a++) off[a] = (uint32_t)nd[a] * 8u;
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            const char* bp = reinterpret_cast<const char*>(p + d * N);
+            const char* bq = reinterpret_cast<const char*>(q + d * N);
+#pragma unroll
+            for (int a = 0; a < NN; a++) {
+                P[a][d] = *reinterpret_cast<const double*>(bp + off[a]);
+                if constexpr (TWO) Q[a][d] = *reinterpret_cast<const double*>(bq + off[a]);
+            }
+        }
+    } else {
+#pragma unroll
+        for (int a = 0; a < NN; a++)
+#pragma unroll
+            for (int d = 0; d < 3; d++) {
+                P[a][d] = p[d * N + nd[a]];
+                if constexpr (TWO) Q[a][d] = q[d * N + nd[a]];
+            }
+    }
+}
+
 // Consistent kinetic energy of hex8 element e (KIN_ENERGY, dataExtraction.cpp:631-707): sum over the GPs of
 // wJ[gp] rho/2 |v(gp)|^2, v(gp) = sum_a phi_a(gp) v_a evaluated from the butterfly coefficients of the nodal
 // velocities, (sum_a v_a + sum_m s_m(gp) g^|m| C_m) / 8.
+template <bool O32 = false>
 __device__ __forceinline__ double hex_kinetic(const MpsInst& I, const int* nd, double rho, const double* wJ) {
     const size_t N = (size_t)I.N;
     double x[8][3], Cv[7][3], vs[3];
-#pragma unroll
-    for (int a = 0; a < 8; a++)
-#pragma unroll
-        for (int d = 0; d < 3; d++) x[a][d] = I.v[d * N + nd[a]];
+    gather_nodes<8, O32>(N, nd, I.v, x);
     hex_butterfly(x, Cv);
 #pragma unroll
     for (int d = 0; d < 3; d++) {
@@ -751,7 +781,7 @@ __device__ __forceinline__ void hex_jac(const T (*A)[3], const GpSigns<T>& s, T*
 // GPs' wJ go to vs[gp*MPS_PATCH] (an LDS array apart from the force slots), for the kinetic energy, which the
 // element kernel computes after the force phase (hex_kinetic).
 // ---------------------------------------------------------------------------
-template <typename T, int LAW, bool VISC = false, bool EN = false>
+template <typename T, int LAW, bool VISC = false, bool EN = false, bool O32 = false>
 __device__ __forceinline__ double hex8_element(const MpsInst& I, int e, const double* __restrict__ u, T (*f)[3],
                                                bool want_dt, T* vs = nullptr, double rdt = 0.0,
                                                unsigned eflags = 0u, double* en = nullptr) {
@@ -768,13 +798,7 @@ __device__ __forceinline__ double hex8_element(const MpsInst& I, int e, const do
     double dte = bits2d(0x7FF0000000000000ull);
     {
         double xX[8][3], xU[8][3], cx[7][3];
-#pragma unroll
-        for (int a = 0; a < 8; a++)
-#pragma unroll
-            for (int d = 0; d < 3; d++) {
-                xX[a][d] = I.X[d * N + nd[a]];
-                xU[a][d] = u[d * N + nd[a]];
-            }
+        gather_nodes<8, O32, true>(N, nd, I.X, xX, u, xU);
         if (want_dt && sizeof(T) == 8) {
             // deformed characteristic length (solvers.cpp:300-317), strict FP64: bitwise the reference's.
             // Taken here, while the gathered nodes are in registers: after the Gauss-point loop the
@@ -1027,7 +1051,7 @@ __device__ __forceinline__ double tet_kinetic(const MpsInst& I, const int* nd, d
     return kv * wJ;
 }
 
-template <typename T, int LAW, bool VISC = false, bool EN = false>
+template <typename T, int LAW, bool VISC = false, bool EN = false, bool O32 = false>
 __device__ __forceinline__ double tet4_element(const MpsInst& I, int e, const double* __restrict__ u, T (*f)[3],
                                                bool want_dt, double rdt = 0.0, unsigned eflags = 0u,
                                                double* en = nullptr) {
@@ -1035,13 +1059,7 @@ __device__ __forceinline__ double tet4_element(const MpsInst& I, int e, const do
     const int nd[4] = {q.x, q.y, q.z, q.w};
     const size_t N = (size_t)I.N;
     double x[4][3], uu[4][3];
-#pragma unroll
-    for (int a = 0; a < 4; a++)
-#pragma unroll
-        for (int d = 0; d < 3; d++) {
-            x[a][d] = I.X[d * N + nd[a]];
-            uu[a][d] = u[d * N + nd[a]];
-        }
+    gather_nodes<4, O32, true>(N, nd, I.X, x, u, uu);
     if constexpr (EN) {
         double r[4];
         const double dte = tet4_core<T, LAW, false, true>(I, e, nd, x, uu, tet_consts(I, e, want_dt), f, want_dt, rdt, r);

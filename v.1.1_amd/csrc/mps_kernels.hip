@@ -114,7 +114,8 @@ __device__ __forceinline__ void patch_sums(const MpsInst& I, int p, const T* __r
 // candidate (64-bit pattern, lanes other than those computing elements hold +inf).
 // EN: the elements' energies (hex8_element / tet4_element) are added to env[3]; hex8 keeps its GPs' wJ in
 // swj[gp*MPS_PATCH + i].
-template <typename T, int NEN, int LAW, int BS, bool VISC = false, bool PRE = false, bool EN = false>
+template <typename T, int NEN, int LAW, int BS, bool VISC = false, bool PRE = false, bool EN = false,
+          bool O32 = false>
 __device__ __forceinline__ unsigned long long patch_forces(const MpsInst& I, int p, bool want_dt, T* __restrict__ sf,
                                                            uint16_t* __restrict__ spc, uint16_t* __restrict__ spr,
                                                            double rdt = 0.0, unsigned eflags = 0u,
@@ -128,9 +129,9 @@ __device__ __forceinline__ unsigned long long patch_forces(const MpsInst& I, int
         T f[NEN][3];
         double en[3] = {0.0, 0.0, 0.0};
         const double dte =
-            (NEN == 8) ? hex8_element<T, LAW, VISC, EN>(I, e0 + i, I.u, (T(*)[3])f, want_dt,
-                                                         EN ? (T*)(swj + i) : sf + i, rdt, eflags, en)
-                       : tet4_element<T, LAW, VISC, EN>(I, e0 + i, I.u, (T(*)[3])f, want_dt, rdt, eflags, en);
+            (NEN == 8) ? hex8_element<T, LAW, VISC, EN, O32>(I, e0 + i, I.u, (T(*)[3])f, want_dt,
+                                                              EN ? (T*)(swj + i) : sf + i, rdt, eflags, en)
+                       : tet4_element<T, LAW, VISC, EN, O32>(I, e0 + i, I.u, (T(*)[3])f, want_dt, rdt, eflags, en);
         if constexpr (EN) {
 #pragma unroll
             for (int j = 0; j < 3; j++) env[j] += en[j];
@@ -160,7 +161,8 @@ __device__ __forceinline__ void elem_energy_publish(const MpsInst& I, double* en
 // One workgroup per patch, one element per work-item.
 // VISC (mode 0 only): bulk artificial viscosity over the step, from u - uprev and the step's dt.
 // EN (mode 0, FP64 inviscid): the energies of eflags (MPS_E_DEF, MPS_E_KIN) into the history.
-template <typename T, int NEN, int LAW, bool VISC = false, bool EN = false>
+// O32: node gathers by 32-bit offsets (gather_nodes; meshes of at most MPS_OFF32_NODES nodes).
+template <typename T, int NEN, int LAW, bool VISC = false, bool EN = false, bool O32 = false>
 __global__ __launch_bounds__(MPS_PATCH) __attribute__((amdgpu_waves_per_eu(VISC && LAW == 1 ? 1 : 2))) void k_elem(MpsInst I, int mode, unsigned eflags) {
     __shared__ T sf[NEN * 3 * MPS_PATCH];
     __shared__ __align__(16) uint16_t spc[NEN * MPS_PATCH];
@@ -178,8 +180,8 @@ __global__ __launch_bounds__(MPS_PATCH) __attribute__((amdgpu_waves_per_eu(VISC 
     }
     const int p = xcd_tile(blockIdx.x, gridDim.x);
     double env[3] = {0.0, 0.0, 0.0};
-    unsigned long long m =
-        patch_forces<T, NEN, LAW, MPS_PATCH, VISC, true, EN>(I, p, mode == 0, sf, spc, spr, rdt, eflags, env, swj);
+    unsigned long long m = patch_forces<T, NEN, LAW, MPS_PATCH, VISC, true, EN, O32>(I, p, mode == 0, sf, spc, spr,
+                                                                                    rdt, eflags, env, swj);
     if (mode == 0) {
         m = block_min_u64(m, sh);
         if (threadIdx.x == 0) atomicMin(dt_slot(I, (k + 1) & 1, blockIdx.x & (MPS_DT_SLOTS - 1)), m);
@@ -196,7 +198,7 @@ __global__ __launch_bounds__(MPS_PATCH) __attribute__((amdgpu_waves_per_eu(VISC 
                     double wJ[8];
 #pragma unroll
                     for (int gp = 0; gp < 8; gp++) wJ[gp] = swj[gp * MPS_PATCH + threadIdx.x];
-                    env[2] += hex_kinetic(I, nd, I.rho ? I.rho[e] : I.rho0, wJ);
+                    env[2] += hex_kinetic<O32>(I, nd, I.rho ? I.rho[e] : I.rho0, wJ);
                 }
             }
             elem_energy_publish(I, env, reinterpret_cast<double*>(sf));
@@ -1122,22 +1124,31 @@ hipError_t launch_elem(const MpsInst& I, int mode, hipStream_t st, unsigned efla
     if (nb == 0) return hipSuccess;
     const bool visc = mode == 0 && I.uprev != nullptr;
     const bool en = mode == 0 && eflags != 0u && elem_fuses_energy(I);
-#define MPS_ELEM_LAW(T, NEN, V, E)                                                            \
-    switch (I.law) {                                                                          \
-        case 0: k_elem<T, NEN, 0, V, E><<<nb, MPS_PATCH, 0, st>>>(I, mode, eflags); break;    \
-        case 1: k_elem<T, NEN, 1, V, E><<<nb, MPS_PATCH, 0, st>>>(I, mode, eflags); break;    \
-        case 2: k_elem<T, NEN, 2, V, E><<<nb, MPS_PATCH, 0, st>>>(I, mode, eflags); break;    \
-        default: k_elem<T, NEN, 3, V, E><<<nb, MPS_PATCH, 0, st>>>(I, mode, eflags); break;   \
+#define MPS_ELEM_LAW_O(T, NEN, V, E, O)                                                          \
+    switch (I.law) {                                                                             \
+        case 0: k_elem<T, NEN, 0, V, E, O><<<nb, MPS_PATCH, 0, st>>>(I, mode, eflags); break;    \
+        case 1: k_elem<T, NEN, 1, V, E, O><<<nb, MPS_PATCH, 0, st>>>(I, mode, eflags); break;    \
+        case 2: k_elem<T, NEN, 2, V, E, O><<<nb, MPS_PATCH, 0, st>>>(I, mode, eflags); break;    \
+        default: k_elem<T, NEN, 3, V, E, O><<<nb, MPS_PATCH, 0, st>>>(I, mode, eflags); break;   \
     }
+#define MPS_ELEM_LAW(T, NEN, V, E) MPS_ELEM_LAW_O(T, NEN, V, E, false)
+    // the inviscid hex8 kernels gather their nodes by 32-bit offsets when the mesh allows it
+    const bool o32 = I.off32 != 0;
     if (I.nen == 8) {
         if (I.precision == 1) {
-            if (visc) { MPS_ELEM_LAW(float, 8, true, false) } else { MPS_ELEM_LAW(float, 8, false, false) }
+            if (visc) {
+                MPS_ELEM_LAW(float, 8, true, false)
+            } else if (o32) {
+                MPS_ELEM_LAW_O(float, 8, false, false, true)
+            } else {
+                MPS_ELEM_LAW(float, 8, false, false)
+            }
         } else if (visc) {
             MPS_ELEM_LAW(double, 8, true, false)
         } else if (en) {
-            MPS_ELEM_LAW(double, 8, false, true)
+            if (o32) { MPS_ELEM_LAW_O(double, 8, false, true, true) } else { MPS_ELEM_LAW(double, 8, false, true) }
         } else {
-            MPS_ELEM_LAW(double, 8, false, false)
+            if (o32) { MPS_ELEM_LAW_O(double, 8, false, false, true) } else { MPS_ELEM_LAW(double, 8, false, false) }
         }
     } else if (!visc && I.pnd != nullptr) {
         const size_t lds = tet_lds(I);
@@ -1168,6 +1179,7 @@ hipError_t launch_elem(const MpsInst& I, int mode, hipStream_t st, unsigned efla
         }
     }
 #undef MPS_ELEM_LAW
+#undef MPS_ELEM_LAW_O
     return hipGetLastError();
 }
 

@@ -92,9 +92,10 @@ def elem_kernel_bytes(ne, nn, nen, n_part, per_elem_mat):
 
 
 def node_kernel_bytes(nn, n_part):
-    """Algorithmic HBM bytes of one node-kernel launch: per node CSR offset 4, BC info 4, u/v/a read 72 and
-    write 72, mass 8; per partial its index 4 and FP64 force read 24."""
-    return nn * (4 + 4 + 72 + 72 + 8) + n_part * (4 + 24)
+    """Algorithmic HBM bytes of one node-kernel launch between two steps of a launch: per node CSR offset 4, BC
+    info 4, u and the half-step velocity w read 48 and written 48, mass 8; per partial its index 4 and FP64 force
+    read 24."""
+    return nn * (4 + 4 + 48 + 48 + 8) + n_part * (4 + 24)
 
 
 def step_bytes(ne, nn, nen, n_part, per_elem_mat):

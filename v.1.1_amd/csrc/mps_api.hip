@@ -1243,13 +1243,19 @@ static hipError_t enqueue_steps(mps_solver_s* h, long long count, std::vector<in
             t.done();
             if (t.a) kinds->push_back(3);
         }
+        // between two node kernels of a launch the state holds the half-step velocity w instead of v and a: in
+        // v's array, or in a's after a step whose v the next element phase reads (kinetic energy); the first
+        // node kernel of a launch finds the v and a that k_begin leaves in place
+        const bool out_v = every > 0 && k % every == 0;
+        const unsigned vaf = (i == 0 ? MPS_N_IN_VA : (every > 0 && (k - 1) % every == 0) ? MPS_N_IN_WA : 0u) |
+                             (out_v ? MPS_N_OUT_V : 0u);
         if (e == hipSuccess && (I.n_sgp > 0 || I.n_lnode > 0)) e = mpsk::launch_loads(I, h->stream);
-        if (e == hipSuccess && I.n_series > 0) e = mpsk::launch_series(I, h->stream);
+        if (e == hipSuccess && I.n_series > 0) e = mpsk::launch_series(I, h->stream, vaf);
         if (e != hipSuccess) break;
         const bool last = (i == count - 1);
         {
             TimedLaunch t(h, 1);
-            e = mpsk::launch_node(I, last ? 0 : 1, h->stream, ef | (i >= 1 ? MPS_N_PREV : 0u));
+            e = mpsk::launch_node(I, last ? 0 : 1, h->stream, ef | vaf | (i >= 1 ? MPS_N_PREV : 0u));
             t.done();
             if (t.a) kinds->push_back(1);
         }

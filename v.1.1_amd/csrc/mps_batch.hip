@@ -430,12 +430,15 @@ static int capture(mps_batch_s* b, int t) {
     hipGraph_t g = nullptr;
     BCHK(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
     hipError_t e = mpsk::launch_batch_phase(2, b->bs, prec, b->nen, b->law, b->insts, T.node, T.count + 1, grid, 0,
-                                            b->stream);
+                                            0u, b->stream);
     for (int s = 0; s < S && e == hipSuccess; s++) {
-        e = mpsk::launch_batch_phase(0, b->bs, prec, b->nen, b->law, b->insts, T.elem, T.count, grid, 0, b->stream);
+        e = mpsk::launch_batch_phase(0, b->bs, prec, b->nen, b->law, b->insts, T.elem, T.count, grid, 0, 0u,
+                                     b->stream);
+        // the first node kernel finds v and a (the begin phase leaves them in place), the others the half-step
+        // velocity; the last keeps v and a (no prediction)
         if (e == hipSuccess)
             e = mpsk::launch_batch_phase(1, b->bs, prec, b->nen, b->law, b->insts, T.node, T.count + 1, grid,
-                                         s < S - 1 ? 1 : 0, b->stream);
+                                         s < S - 1 ? 1 : 0, s == 0 ? MPS_N_IN_VA : 0u, b->stream);
     }
     hipError_t e2 = hipStreamEndCapture(b->stream, &g);
     if (e != hipSuccess || e2 != hipSuccess) {

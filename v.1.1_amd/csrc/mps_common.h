@@ -69,6 +69,10 @@ enum : int32_t { MPS_H_TIME = 0, MPS_H_DT, MPS_H_EDEF, MPS_H_EKIN, MPS_H_VOL, MP
 #define MPS_E_KIN 2u   // KIN_ENERGY of the previous step (v_{k-1}, before the solve of step k)
 #define MPS_E_KIN_NOW 4u  // standalone pass after the last node kernel of a chunk: KIN_ENERGY of that step
 #define MPS_N_PREV 8u  // node kernel: also reduce the previous step's node sums (that step ran in the same launch)
+// node kernel: where the previous step left its velocity state (default: the half-step velocity w in v's array)
+#define MPS_N_IN_VA 16u   // v and a of the previous step in their arrays
+#define MPS_N_IN_WA 32u   // w in a's array (the previous step kept v readable: MPS_N_OUT_V)
+#define MPS_N_OUT_V 64u   // node kernel: keep v of the step readable (kinetic energy) and put w in a's array
 
 enum : int32_t { MPS_TIME_WINDOW = 0, MPS_TIME_FIXED = 1 };
 

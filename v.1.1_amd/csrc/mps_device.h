@@ -1120,27 +1120,33 @@ __device__ __forceinline__ void bc_motion(const MpsInst& I, double val, long lon
     }
 }
 
-// Increment of DOF i of node n at the begin of step k: the prediction dt v + dt^2/2 a on active DOFs
-// (nonlinearSystem.cpp:1186-1201), the prescribed U on constrained ones.
+// Half-step velocity w = v + dt/2 a of a DOF.  The central-difference step of the reference
+// (nonlinearSystem.cpp:1186-1201, 1096-1107) is, with the dt of the step,
+//   u_{k+1} = u_k + dt v_k + dt^2/2 a_k,   v_{k+1} = v_k + dt (a_k + a_{k+1}) / 2,
+// which is u_{k+1} = u_k + dt w_k and v_{k+1} = w_k + dt/2 a_{k+1} with w_k = v_k + dt/2 a_k: between two steps
+// of one launch only w has to be kept (the node kernel reads and writes 24 bytes per node less than with v
+// and a).  Constrained DOFs hold a = 0, so there w = v.
+__device__ __forceinline__ double half_velocity(double dt, double v, double a) { return __builtin_fma(0.5 * dt, a, v); }
+
+// Increment of DOF i of node n at the begin of step k: the prediction dt w on active DOFs (w the half-step
+// velocity above), the prescribed U on constrained ones.
 __device__ __forceinline__ double step_increment(const MpsInst& I, uint32_t info, int i, long long k, double dt,
-                                                 double v, double a) {
+                                                 double w) {
 #pragma clang fp contract(off)
-    if (!((info >> i) & 1u)) return dt * v + 0.5 * dt * dt * a;
+    if (!((info >> i) & 1u)) return dt * w;
     double U, V;
     bc_motion(I, I.bcval[3 * (size_t)((info & MPS_REC_MASK) >> 3) + i], k, dt, &U, &V);
     return U;
 }
 
 // Prediction + Dirichlet displacement of node n for step k (nonlinearSystem.cpp:1186-1201,
-// solverExplicit.cpp:545-575).  The prescribed velocity and the zero acceleration of the constrained DOFs are
-// set by the solve of the same step (solve_node), so that until then v and a hold the previous step's values
-// everywhere; nothing reads them in between except the energy of the previous step, and the state at the
-// end of the step is the reference's.
-__device__ __forceinline__ void predict_node(const MpsInst& I, int n, long long k, double dt, double* u, double* v,
-                                             double* a) {
+// solverExplicit.cpp:545-575), from the half-step velocity w.  The prescribed velocity and the zero
+// acceleration of the constrained DOFs are set by the solve of the same step (node_update).
+__device__ __forceinline__ void predict_node(const MpsInst& I, int n, long long k, double dt, double* u,
+                                             const double* w) {
     const uint32_t info = I.ninfo[n];
 #pragma unroll
-    for (int i = 0; i < 3; i++) u[i] += step_increment(I, info, i, k, dt, v[i], a[i]);
+    for (int i = 0; i < 3; i++) u[i] += step_increment(I, info, i, k, dt, w[i]);
 }
 
 // External force of flagged node j (lnn[j]) at time t (NeumannBCManagement, boundaryConditions.cpp:75-140):
@@ -1310,39 +1316,37 @@ __device__ __forceinline__ void node_rhs(const double* fi, const double* fe, dou
     for (int i = 0; i < 3; i++) rhs[i] = -fi[i] + fe[i];
 }
 
-// Update of node n's velocity and acceleration at step k: the prescribed velocity and zero acceleration of its
-// constrained DOFs (applyDisplacementBC, solverExplicit.cpp:568-571), the lumped-mass solve of the free ones
-// (nonlinearSystem.cpp:1096-1107) and the trapezoidal velocity update.  rhs = -fint + fext of the node (read
-// for free DOFs only).  With wkin, 1/2 m (V^2 - v^2) of each constrained DOF, whose velocity v the prescribed
-// V replaces, is added to *wkin.
+// Velocity and acceleration of node n at the end of step k: the prescribed velocity and zero acceleration of
+// its constrained DOFs (applyDisplacementBC, solverExplicit.cpp:568-571), the lumped-mass solve of the free
+// ones (nonlinearSystem.cpp:1096-1107) and v = w + dt/2 a (half_velocity).  w: the node's half-step velocity of
+// the previous step (its velocity on constrained DOFs); rhs = -fint + fext of the node (read for free DOFs
+// only).  With wkin, 1/2 m (V^2 - w^2) of each constrained DOF, whose velocity the prescribed V replaces, is
+// added to *wkin.
 __device__ __forceinline__ void node_update(const MpsInst& I, uint32_t info, long long k, double dt, double m,
-                                            const double* rhs, double* v, double* a, double* wkin = nullptr) {
+                                            const double* rhs, const double* w, double* v, double* a,
+                                            double* wkin = nullptr) {
 #pragma clang fp contract(off)
     const uint32_t mask = info & 7u;
-    if (mask) {
-        const uint32_t rec = (info & MPS_REC_MASK) >> 3;
-#pragma unroll
-        for (int i = 0; i < 3; i++)
-            if ((mask >> i) & 1u) {
-                double U, V;
-                bc_motion(I, I.bcval[3 * (size_t)rec + i], k, dt, &U, &V);
-                if (wkin) *wkin += 0.5 * m * (V * V - v[i] * v[i]);
-                v[i] = V;
-                a[i] = 0.;
-            }
-    }
 #pragma unroll
     for (int i = 0; i < 3; i++) {
-        if ((mask >> i) & 1u) continue;
-        double an = (m != 0.0) ? rhs[i] / m : 0.0;
-        v[i] += dt * ((1. - 0.5) * a[i] + 0.5 * an);
-        a[i] = an;
+        if ((mask >> i) & 1u) {
+            double U, V;
+            bc_motion(I, I.bcval[3 * (size_t)((info & MPS_REC_MASK) >> 3) + i], k, dt, &U, &V);
+            if (wkin) *wkin += 0.5 * m * (V * V - w[i] * w[i]);
+            v[i] = V;
+            a[i] = 0.;
+        } else {
+            const double an = (m != 0.0) ? rhs[i] / m : 0.0;
+            v[i] = half_velocity(dt, w[i], an);
+            a[i] = an;
+        }
     }
 }
 
-// Gather + lumped-mass solve of node n at step k (node_partial_sum, node_fext, node_update).  A fully
-// constrained node reads no forces.
-__device__ __forceinline__ void solve_node(const MpsInst& I, int n, long long k, double dt, double* v, double* a) {
+// Gather + lumped-mass solve of node n at step k (node_partial_sum, node_fext, node_update): v, a of the step
+// from the half-step velocity w.  A fully constrained node reads no forces.
+__device__ __forceinline__ void solve_node(const MpsInst& I, int n, long long k, double dt, const double* w,
+                                           double* v, double* a) {
     const uint32_t info = I.ninfo[n];
     double rhs[3] = {0.0, 0.0, 0.0};
     if ((info & 7u) != 7u) {
@@ -1351,7 +1355,7 @@ __device__ __forceinline__ void solve_node(const MpsInst& I, int n, long long k,
         node_fext(I, n, info, fe);
         node_rhs(fi, fe, rhs);
     }
-    node_update(I, info, k, dt, I.mass[n], rhs, v, a);
+    node_update(I, info, k, dt, I.mass[n], rhs, w, v, a);
 }
 
 // Energy bookkeeping of node n at step k, in two parts around its velocity update so that few values live

@@ -381,10 +381,29 @@ __device__ __forceinline__ void write_hist_step(const MpsInst& I, long long k, c
     }
 }
 
-// EN: energy bookkeeping of the node into red[MPS_NODE_PARTS] (node_energy_forces, node_energy_motion).  Returns false when the step is
-// not running (the same for every work-item of the launch).
+// Half-step velocities w of node n as the step's node kernel receives them (nflags): from v and a with the dt of
+// the step (MPS_N_IN_VA: the first step of a launch), from a's array (MPS_N_IN_WA: after a step that kept v
+// readable) or from v's array.
+__device__ __forceinline__ void load_half_velocity(const MpsInst& I, int n, unsigned nflags, double dt, double* w) {
+    const size_t N = (size_t)I.N;
+    if (nflags & MPS_N_IN_VA) {
+#pragma unroll
+        for (int i = 0; i < 3; i++) w[i] = half_velocity(dt, I.v[i * N + n], I.a[i * N + n]);
+    } else {
+        const double* src = (nflags & MPS_N_IN_WA) ? I.a : I.v;
+#pragma unroll
+        for (int i = 0; i < 3; i++) w[i] = src[i * N + n];
+    }
+}
+
+// Node update of step k (node kernel).  nflags: where step k-1 left w (load_half_velocity); MPS_N_OUT_V, keep v
+// of step k readable for the next element phase (kinetic energy), w then going to a's array.  Without a
+// prediction (fuse = 0, or no step k+1) v and a are kept, so that the state between launches is u, v, a.  EN: energy bookkeeping of the node into red[MPS_NODE_PARTS]
+// (node_energy_forces, node_energy_motion).  Returns false when the step is not running (the same for every
+// work-item of the launch).
 template <bool EN = false>
-__device__ __forceinline__ bool node_item(const MpsInst& I, int n, bool first, int fuse, double* red = nullptr) {
+__device__ __forceinline__ bool node_item(const MpsInst& I, int n, bool first, int fuse, unsigned nflags,
+                                          double* red = nullptr) {
     MpsCtl* ctl = I.ctl;
     const long long k = ctl->B;
     const MpsStepSlot s = ctl->slot[k & 1];
@@ -404,16 +423,13 @@ __device__ __forceinline__ bool node_item(const MpsInst& I, int n, bool first, i
     }
     if (n < 0) return true;
     const size_t N = (size_t)I.N;
-    double u[3], v[3], a[3];
+    double u[3], w[3], v[3], a[3];
 #pragma unroll
-    for (int i = 0; i < 3; i++) {
-        u[i] = I.u[i * N + n];
-        v[i] = I.v[i * N + n];
-        a[i] = I.a[i * N + n];
-    }
+    for (int i = 0; i < 3; i++) u[i] = I.u[i * N + n];
+    load_half_velocity(I, n, nflags, s.dt, w);
     if (EN) {
         // the forces first (also of a fully constrained node, for the work of the supports), then the step's
-        // increments from v and a before their update: nothing extra is held through the partial sums
+        // increments from w: nothing extra is held through the partial sums
         const uint32_t info = I.ninfo[n];
         double rhs[3];
         {
@@ -421,16 +437,16 @@ __device__ __forceinline__ bool node_item(const MpsInst& I, int n, bool first, i
             node_partial_sum(I, n, fi);
             node_fext(I, n, info, fe);
 #pragma unroll
-            for (int i = 0; i < 3; i++) incr[i] = step_increment(I, info, i, k, s.dt, v[i], a[i]);
+            for (int i = 0; i < 3; i++) incr[i] = step_increment(I, info, i, k, s.dt, w[i]);
             node_energy_forces(I, n, info, k, u, incr, fi, fe, red);
             node_rhs(fi, fe, rhs);
         }
         const double m = I.mass[n];
         double wkin = -0.0;
-        node_update(I, info, k, s.dt, m, rhs, v, a, &wkin);
+        node_update(I, info, k, s.dt, m, rhs, w, v, a, &wkin);
         node_energy_motion(m, k, wkin, v, red);
     } else {
-        solve_node(I, n, k, s.dt, v, a);
+        solve_node(I, n, k, s.dt, w, v, a);
     }
     if (n == I.dof1_node) {
         double x = u[I.dof1_dir];
@@ -439,18 +455,33 @@ __device__ __forceinline__ bool node_item(const MpsInst& I, int n, bool first, i
             atomicMin((unsigned long long*)&ctl->nan_step, (unsigned long long)k);
         }
     }
-    if (fuse && nx.go) {
+    const bool predict = fuse && nx.go;
+    if (predict) {
         if (I.uprev) {  // the F of step k becomes the previous state of step k+1 (nextStep)
 #pragma unroll
             for (int i = 0; i < 3; i++) I.uprev[i * N + n] = u[i];
         }
-        predict_node(I, n, k + 1, nx.dt, u, v, a);
+#pragma unroll
+        for (int i = 0; i < 3; i++) w[i] = half_velocity(nx.dt, v[i], a[i]);
+        predict_node(I, n, k + 1, nx.dt, u, w);
     }
 #pragma unroll
-    for (int i = 0; i < 3; i++) {
-        I.u[i * N + n] = u[i];
-        I.v[i * N + n] = v[i];
-        I.a[i * N + n] = a[i];
+    for (int i = 0; i < 3; i++) I.u[i * N + n] = u[i];
+    if (!predict) {
+#pragma unroll
+        for (int i = 0; i < 3; i++) {
+            I.v[i * N + n] = v[i];
+            I.a[i * N + n] = a[i];
+        }
+    } else if (nflags & MPS_N_OUT_V) {
+#pragma unroll
+        for (int i = 0; i < 3; i++) {
+            I.v[i * N + n] = v[i];
+            I.a[i * N + n] = w[i];
+        }
+    } else {
+#pragma unroll
+        for (int i = 0; i < 3; i++) I.v[i * N + n] = w[i];
     }
     return true;
 }
@@ -488,7 +519,7 @@ __global__ __launch_bounds__(256) void k_node(MpsInst I, int fuse, unsigned nfla
     const int n = xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     const bool first = (n == 0);
     if (!EN) {
-        node_item(I, n < I.N ? n : -1, first, fuse);
+        node_item(I, n < I.N ? n : -1, first, fuse, nflags);
         return;
     }
     __shared__ double sh[MPS_NODE_PARTS * 256];
@@ -496,7 +527,7 @@ __global__ __launch_bounds__(256) void k_node(MpsInst I, int fuse, unsigned nfla
     // can drop the first additions and hold no zeros through the node work
     double red[MPS_NODE_PARTS] = {-0.0, -0.0, -0.0, -0.0};
     const long long k = I.ctl->B;  // read before the first work-item advances the step counters
-    if (!node_item<true>(I, n < I.N ? n : -1, first, fuse, red)) return;
+    if (!node_item<true>(I, n < I.N ? n : -1, first, fuse, nflags, red)) return;
     const int nb = (int)gridDim.x;
     double* npart = I.epart + 3 * (size_t)I.n_patch;
     publish_block_sum<MPS_NODE_PARTS>(red, npart + (size_t)(k & 1) * MPS_NODE_PARTS * nb, sh);
@@ -517,22 +548,23 @@ __global__ __launch_bounds__(256) void k_node(MpsInst I, int fuse, unsigned nfla
 
 template <int BS>
 __global__ __launch_bounds__(BS) void k_node_batch(const MpsInst* __restrict__ insts, const MpsWork* __restrict__ work,
-                                                    const int* __restrict__ nwork_p, int fuse) {
+                                                    const int* __restrict__ nwork_p, int fuse, unsigned nflags) {
     const int nwork = *nwork_p;
     for (int w = batch_first(nwork); w < batch_end(nwork); w += batch_stride()) {
         const MpsWork W = work[w];
         const MpsInst& I = insts[W.inst];
         const int n = W.begin + threadIdx.x;
         const bool first = W.first && threadIdx.x == 0;
-        node_item(I, n < W.end ? n : -1, first, fuse);
+        node_item(I, n < W.end ? n : -1, first, fuse, nflags);
     }
 }
 
 // Per-step DOF series (mps_set_dof_series), after the element kernel (and the loads) of step k = ctl->B and
 // before its node kernel: u holds u_k there, part the step's partial forces, lf its external forces.  The
 // velocity and acceleration are those the node kernel is about to give the DOF (solve_node on a copy, the same
-// arithmetic), the internal force the node's partials in CSR order as the node kernel sums them.
-__global__ __launch_bounds__(256) void k_series(MpsInst I) {
+// arithmetic, from the state as the node kernel receives it: nflags as for it), the internal force
+// the node's partials in CSR order as the node kernel sums them.
+__global__ __launch_bounds__(256) void k_series(MpsInst I, unsigned nflags) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= I.n_series) return;
     const long long k = I.ctl->B;
@@ -556,20 +588,17 @@ __global__ __launch_bounds__(256) void k_series(MpsInst I) {
         node_fext(I, n, I.ninfo[n], fe);
         val = fe[dir];
     } else {
-        double v[3], a[3];
-#pragma unroll
-        for (int d = 0; d < 3; d++) {
-            v[d] = I.v[d * N + n];
-            a[d] = I.a[d * N + n];
-        }
-        solve_node(I, n, k, s.dt, v, a);
+        double w[3], v[3], a[3];
+        load_half_velocity(I, n, nflags, s.dt, w);
+        solve_node(I, n, k, s.dt, w, v, a);
         val = kind == MPS_SERIES_V ? v[dir] : a[dir];
     }
     I.ser_ring[(size_t)((k - 1) % I.hist_cap) * I.n_series + i] = val;
 }
 
 // ---------------------------------------------------------------------------
-// Begin of step k = A (first step of a launch, or after an energy pass)
+// Begin of step k = A (first step of a launch): the prediction from v and a, which stay in place for the
+// step's node kernel (MPS_N_IN_VA)
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ void begin_item(const MpsInst& I, int n, bool first) {
     MpsCtl* ctl = I.ctl;
@@ -584,24 +613,17 @@ __device__ __forceinline__ void begin_item(const MpsInst& I, int n, bool first) 
     }
     if (n < 0 || !nx.go) return;
     const size_t N = (size_t)I.N;
-    double u[3], v[3], a[3];
+    double u[3], w[3];
 #pragma unroll
-    for (int i = 0; i < 3; i++) {
-        u[i] = I.u[i * N + n];
-        v[i] = I.v[i * N + n];
-        a[i] = I.a[i * N + n];
-    }
+    for (int i = 0; i < 3; i++) u[i] = I.u[i * N + n];
+    load_half_velocity(I, n, MPS_N_IN_VA, nx.dt, w);
     if (I.uprev) {
 #pragma unroll
         for (int i = 0; i < 3; i++) I.uprev[i * N + n] = u[i];
     }
-    predict_node(I, n, k, nx.dt, u, v, a);
+    predict_node(I, n, k, nx.dt, u, w);
 #pragma unroll
-    for (int i = 0; i < 3; i++) {
-        I.u[i * N + n] = u[i];
-        I.v[i * N + n] = v[i];
-        I.a[i * N + n] = a[i];
-    }
+    for (int i = 0; i < 3; i++) I.u[i * N + n] = u[i];
 }
 
 __global__ __launch_bounds__(256) void k_begin(MpsInst I) {
@@ -1185,7 +1207,7 @@ hipError_t launch_elem(const MpsInst& I, int mode, hipStream_t st, unsigned efla
 
 hipError_t launch_node(const MpsInst& I, int fuse, hipStream_t st, unsigned nflags) {
     if (I.energy_every > 0) k_node<true><<<blocks_for(I.N, 256), 256, 0, st>>>(I, fuse, nflags);
-    else k_node<false><<<blocks_for(I.N, 256), 256, 0, st>>>(I, fuse, 0u);
+    else k_node<false><<<blocks_for(I.N, 256), 256, 0, st>>>(I, fuse, nflags & (MPS_N_IN_VA | MPS_N_IN_WA | MPS_N_OUT_V));
     return hipGetLastError();
 }
 
@@ -1263,9 +1285,9 @@ hipError_t launch_energy(const MpsInst& I, unsigned flags, hipStream_t st) {
     return hipGetLastError();
 }
 
-hipError_t launch_series(const MpsInst& I, hipStream_t st) {
+hipError_t launch_series(const MpsInst& I, hipStream_t st, unsigned nflags) {
     if (I.n_series <= 0) return hipSuccess;
-    k_series<<<blocks_for(I.n_series, 256), 256, 0, st>>>(I);
+    k_series<<<blocks_for(I.n_series, 256), 256, 0, st>>>(I, nflags);
     return hipGetLastError();
 }
 
@@ -1276,7 +1298,7 @@ hipError_t launch_energy_tail(const MpsInst& I, hipStream_t st) {
 
 template <int BS>
 static hipError_t batch_phase_bs(int phase, int prec, int nen, int law, const MpsInst* insts, const MpsWork* work,
-                                 const int* nwork, int grid, int fuse, hipStream_t st) {
+                                 const int* nwork, int grid, int fuse, unsigned nflags, hipStream_t st) {
     if (phase == 0) {
 #define MPS_EB(T, NEN, LAW) k_elem_batch<T, NEN, LAW, BS><<<grid, BS, 0, st>>>(insts, work, nwork)
 #define MPS_EB_LAW(T, NEN)                          \
@@ -1294,7 +1316,7 @@ static hipError_t batch_phase_bs(int phase, int prec, int nen, int law, const Mp
 #undef MPS_EB_LAW
 #undef MPS_EB
     } else if (phase == 1) {
-        k_node_batch<BS><<<grid, BS, 0, st>>>(insts, work, nwork, fuse);
+        k_node_batch<BS><<<grid, BS, 0, st>>>(insts, work, nwork, fuse, nflags);
     } else {
         k_begin_batch<BS><<<grid, BS, 0, st>>>(insts, work, nwork);
     }
@@ -1302,12 +1324,12 @@ static hipError_t batch_phase_bs(int phase, int prec, int nen, int law, const Mp
 }
 
 hipError_t launch_batch_phase(int phase, int bs, int prec, int nen, int law, const MpsInst* insts, const MpsWork* work,
-                              const int* nwork, int grid, int fuse, hipStream_t st) {
+                              const int* nwork, int grid, int fuse, unsigned nflags, hipStream_t st) {
     if (grid <= 0) return hipSuccess;
     switch (bs) {
-        case 64: return batch_phase_bs<64>(phase, prec, nen, law, insts, work, nwork, grid, fuse, st);
-        case 128: return batch_phase_bs<128>(phase, prec, nen, law, insts, work, nwork, grid, fuse, st);
-        case 256: return batch_phase_bs<256>(phase, prec, nen, law, insts, work, nwork, grid, fuse, st);
+        case 64: return batch_phase_bs<64>(phase, prec, nen, law, insts, work, nwork, grid, fuse, nflags, st);
+        case 128: return batch_phase_bs<128>(phase, prec, nen, law, insts, work, nwork, grid, fuse, nflags, st);
+        case 256: return batch_phase_bs<256>(phase, prec, nen, law, insts, work, nwork, grid, fuse, nflags, st);
         default: return hipErrorInvalidValue;
     }
 }

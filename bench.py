@@ -80,12 +80,6 @@ PATCH = 256  # elements per patch of the element kernel (MPS_PATCH)
 PREWARM_STEPS = 400  # steps of a separate solver of the same workload before each leg's warm-up (about 60 ms)
 
 
-def patch_partials(mesh):
-    """Number of partial nodal forces: distinct nodes per patch of PATCH consecutive elements, summed."""
-    conn = mesh["conn"].reshape(mesh["n_elem"], mesh["nen"])
-    return int(sum(np.unique(conn[s:s + PATCH]).size for s in range(0, mesh["n_elem"], PATCH)))
-
-
 def elem_kernel_bytes(ne, nn, nen, n_part, per_elem_mat):
     """Algorithmic HBM bytes of one element-kernel launch: per element its node references + l0 8 (+
     material 24 when per element) + patch contribution list 2 nen; per partial its offset 2 + FP64 force
@@ -151,9 +145,10 @@ def run_gpu(cfg, precision, warmup, steps, prof_steps=None, energy_every=0):
     t3 = time.perf_counter()
     assert rc == 0, rep2
     kt = s.kernel_times()
+    n_part = s.layout()["n_part"]  # partial forces: distinct nodes per patch, in the library's element order
     s.close()
     return dict(wall_s=t1 - t0, device_ms=rep["device_ms"], kt=kt, u=u, rep=rep, hist=hist,
-                events_ms_per_step=(t3 - t2) * 1e3 / prof_steps)
+                events_ms_per_step=(t3 - t2) * 1e3 / prof_steps, n_part=n_part)
 
 
 def energy_leg(cfg, warmup, steps, plain_ms):
@@ -183,7 +178,7 @@ def tet_leg(warmup, steps):
     r64 = run_gpu(cfg, G.PREC_FP64, warmup, steps)
     r32 = run_gpu(cfg, G.PREC_FP32_ELEM, warmup, steps)
     ne = m["n_elem"]
-    n_part = patch_partials(m)
+    n_part = r64["n_part"]
     e64 = r64["kt"]["element"][0] / max(r64["kt"]["element"][1], 1)
     n64 = r64["kt"]["node"][0] / max(r64["kt"]["node"][1], 1)
     return {"workload": f"{ne} tet4 (80^3 x 6), 15% perturbation, per-element E log-U[1e5,1e7], nu U[0.2,0.45], "
@@ -375,7 +370,7 @@ def main():
     nms, nn_l = r64["kt"]["node"]
     elem_avg_ms = ems / max(en, 1)
     node_avg_ms = nms / max(nn_l, 1)
-    n_part = patch_partials(m)
+    n_part = r64["n_part"]
     eb = elem_kernel_bytes(ne, nn, 8, n_part, False)
     sb = step_bytes(ne, nn, 8, n_part, False)
     achieved = eb / (elem_avg_ms * 1e-3) / 1e9

@@ -314,6 +314,15 @@ int mps_prepare_run(mps_handle h, int64_t max_steps);
 int mps_profile(mps_handle h, int32_t enable);
 int mps_kernel_times(mps_handle h, double *ms_total /*[4]*/, int64_t *launches /*[4]*/);
 
+/* Storage layout of h (no reference counterpart; the element and node kernels' bytes follow from it):
+ * the number of patches (MPS_PATCH-element work units of the element kernel), of partial forces (the
+ * distinct nodes of each patch, summed over the patches), and, when elem_order is not NULL, the input id
+ * of each stored element (elem_order[n_elem]).  Hex8 elements are stored sorted along a space-filling
+ * curve of their centroids, so that each patch is a compact block (MPS_ELEM_ORDER=input in the environment
+ * at mps_create keeps the input order); tet4 elements in input order.  Every other entry point takes and
+ * returns elements in input order. */
+int mps_layout(mps_handle h, int64_t *n_patch, int64_t *n_part, int32_t *elem_order);
+
 /* Message of the last failure on h (h may be NULL for mps_create failures). */
 const char *mps_last_error(mps_handle h);
 

@@ -12,7 +12,8 @@ namespace mpsh {
 struct HostPrep {
     int32_t N = 0, Ne = 0, nen = 0, law = 0;
     std::vector<double> X;
-    std::vector<int32_t> conn, rowptr, col;
+    std::vector<int32_t> conn, rowptr, col;  // conn and the per-element arrays in stored order
+    std::vector<int32_t> eorig;              // input id of each stored element (empty: input order)
     // patch tables (see mps_common.h)
     int32_t n_patch = 0, n_part = 0;
     std::vector<int32_t> poff, nrow, ncol;

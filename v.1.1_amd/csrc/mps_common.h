@@ -6,11 +6,14 @@
 //   ninfo           uint32[N]    bits 0..2 Dirichlet mask (x,y,z), bits 3..30 Dirichlet record index,
 //                                bit 31 node carries nodal force entries (MPS_NODE_LOADED)
 //   bcval           double[3*R]  per Dirichlet record: per-direction BC value
-//   conn            int32[nen*Ne] element-major (reference node order)
+//   conn            int32[nen*Ne] element-major (reference node order), elements in stored order (hex8:
+//                                sorted along a space-filling curve of their centroids at set-up, for compact
+//                                patches; tet4: input order); l0, mat, cel, rho and the element loops follow it
+//   eorig           int32[Ne]    input id of each stored element (NULL: stored = input order)
 //   l0              double[Ne]   undeformed characteristic length
 //   mat             double[2*Ne] per-element (lambda,mu) or (mu,K1); NULL = uniform
 //   cel             double[Ne]   per-element sound speed; NULL = uniform c0
-//   rowptr/col      CSR node -> (e*nen + a), ascending element then slot (set-up gathers)
+//   rowptr/col      CSR node -> stored slot (e*nen + a), ascending input element then slot (set-up gathers)
 //   patches         elements [p*MPS_PATCH, (p+1)*MPS_PATCH) form patch p; its U_p distinct
 //                   nodes (ascending) own partial forces part[poff[p] .. poff[p+1])
 //     poff          int32[n_patch+1]       first partial of each patch
@@ -112,6 +115,7 @@ struct MpsInst {
     double mat_a, mat_b, c0, rho0;  // uniform material: (lambda,mu) or (mu,K1), sound speed, density
     const double* X;
     const int32_t* conn;
+    const int32_t* eorig;  // input id of each stored element (NULL: input order)
     const double* l0;
     const double* mat;
     const double* cel;

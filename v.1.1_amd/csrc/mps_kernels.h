@@ -35,7 +35,8 @@ hipError_t launch_energy(const MpsInst& I, unsigned flags, hipStream_t st);
 hipError_t launch_series(const MpsInst& I, hipStream_t st, unsigned nflags);
 // End of a launch with energies, after the MPS_E_KIN_NOW pass: the last step's sums into the history.
 hipError_t launch_energy_tail(const MpsInst& I, hipStream_t st);
-// Element fields of the VTK output: out[21*Ne] (Cauchy 9, strain 9, VMS, volumetric, equivalent strain).
+// Element fields of the VTK output: out[21*Ne] (Cauchy 9, strain 9, VMS, volumetric, equivalent strain), rows in
+// input element order.
 hipError_t launch_elem_outputs(const MpsInst& I, double* out, hipStream_t st);
 hipError_t launch_init_ctl(const MpsInst& I, const unsigned long long* dt0, hipStream_t st);
 // Persistent batched phases: 0 element, 1 node (fuse flag, nflags as for launch_node), 2 begin.  bs in {64,128,256}.

@@ -1096,7 +1096,8 @@ __global__ __launch_bounds__(256) void k_elem_outputs(MpsInst I, double* __restr
         gp_outputs(I.law, F, pa, pb, acc);
         ngp = 1;
     }
-    for (int i = 0; i < 21; i++) out[(size_t)e * 21 + i] = acc[i] / (double)ngp;
+    const size_t eo = I.eorig ? (size_t)I.eorig[e] : (size_t)e;  // rows in input element order
+    for (int i = 0; i < 21; i++) out[eo * 21 + i] = acc[i] / (double)ngp;
 }
 
 // Standalone energy pass: between the element and node kernels of step k = ctl->B (flags MPS_E_DEF: DEFO_ENERGY

@@ -155,6 +155,7 @@ def load(path=None):
     L.mps_set_graphs.argtypes = [H, C.c_int32]
     L.mps_prepare_run.argtypes = [H, C.c_int64]
     L.mps_kernel_times.argtypes = [H, _dp, C.POINTER(C.c_int64)]
+    L.mps_layout.argtypes = [H, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_int32)]
     if hasattr(L, "mps_batch_create"):
         B = C.c_void_p
         L.mps_batch_create.argtypes = [C.POINTER(BatchCfg), C.POINTER(B)]
@@ -424,6 +425,13 @@ class Solver:
         self._chk(self.L.mps_kernel_times(self.h, _d(ms), n))
         kinds = ("element", "node", "begin", "energy")
         return {k: (float(ms[i]), int(n[i])) for i, k in enumerate(kinds)}
+
+    def layout(self):
+        """Patches, partial forces and the stored element order (input id of each stored element)."""
+        npa, npt = C.c_int64(), C.c_int64()
+        order = np.zeros(self.ne, dtype=np.int32)
+        self._chk(self.L.mps_layout(self.h, C.byref(npa), C.byref(npt), order.ctypes.data_as(C.POINTER(C.c_int32))))
+        return {"n_patch": npa.value, "n_part": npt.value, "order": order}
 
     def close(self):
         if self.h:

@@ -93,3 +93,39 @@ def test_curve_order_matches_input_order(precision, monkeypatch):
     for k in ("e_def", "e_kin", "volume", "e_ext"):
         np.testing.assert_allclose(a["hist"][k], b["hist"][k], rtol=1e-10 if precision == G.PREC_FP64 else 1e-5,
                                    atol=1e-14 * np.abs(b["hist"][k]).max(), err_msg=k)
+
+
+@pytest.mark.parametrize("case", ["flat", "one", "two", "coincident"])
+def test_element_order_edge_meshes(case, monkeypatch):
+    """Meshes whose centroids do not fill a box: one element thick (the key's thin axis), one and two
+    elements, and every element duplicated in place (pairs of equal keys, kept in input order).  The order
+    is a permutation, ties keep input order, and a short solve matches the input-order solve."""
+    if case == "flat":
+        m = perturb(box(40, 30, 1, lz=1.0 / 30), 0.05, 5)
+    elif case == "one":
+        m = box(1, 1, 1)
+    elif case == "two":
+        m = box(2, 1, 1)
+    else:
+        b = box(8, 6, 4)
+        conn = b["conn"].reshape(-1, 8)
+        m = dict(b, conn=np.ascontiguousarray(np.repeat(conn, 2, axis=0).ravel()), n_elem=2 * b["n_elem"])
+    ne = m["n_elem"]
+    dd, dv = clamp_and_pull(m)
+    out = {}
+    for mode in ("curve", "input"):
+        if mode == "input":
+            monkeypatch.setenv("MPS_ELEM_ORDER", "input")
+        else:
+            monkeypatch.delenv("MPS_ELEM_ORDER", raising=False)
+        with G.Solver(m, dir_dof=dd, dir_val=dv, n_steps=40) as s:
+            order = s.layout()["order"]
+            rc, rep = s.run(40)
+            assert rc == 0 and rep["steps"] == 40
+            out[mode] = (order, s.get_state()[0], s.element_outputs())
+    order = out["curve"][0]
+    assert np.array_equal(np.sort(order), np.arange(ne))
+    if case == "coincident":  # each duplicate right after its original
+        assert np.all(order[1::2] == order[0::2] + 1) and np.all(order[0::2] % 2 == 0)
+    assert relerr(out["curve"][1], out["input"][1]) <= 1e-12
+    np.testing.assert_allclose(out["curve"][2], out["input"][2], rtol=0, atol=1e-9 * np.abs(out["input"][2]).max())

@@ -545,6 +545,15 @@ static int upload_instance(mps_solver_s* h, const HostPrep& hp) {
         const char* g32 = getenv("MPS_GATHER32");
         I.off32 = (hp.N <= MPS_OFF32_NODES && !(g32 && g32[0] == '0')) ? 1 : 0;
     }
+    // stable-dt skip of the FP64 hex8 element kernels: an element whose dt is certainly more than 1/64 above the
+    // previous step's min skips the strict length (the certificate keeps the min exact).  The environment
+    // variable MPS_DT_SKIP=0 turns it off; another number replaces the margin 1/64 (a negative one makes the
+    // certificate fail, which exercises the fallback).  Same results, bit for bit, either way.
+    {
+        const char* ds = getenv("MPS_DT_SKIP");
+        I.dt_skip_on = (ds && strcmp(ds, "0") == 0) ? 0 : 1;
+        I.dt_skip_margin = (ds && I.dt_skip_on) ? atof(ds) : 1.0 / 64;
+    }
     I.Ne = hp.Ne;
     I.nen = hp.nen;
     I.law = hp.law;
@@ -721,6 +730,9 @@ static int device_setup(mps_solver_s* h) {
     c.nonfinite = 0;
     c.nan_step = LLONG_MAX;
     c.dt0min = bits;
+    c.dt_skip = HUGE_VAL;  // the first step computes every element's dt
+    c.skip_hold = 0;
+    c.eblocks = 0;
     HCHK(h, hipMemcpy(I.ctl, &c, sizeof(c), hipMemcpyHostToDevice));
     hipError_t ef = mpsk::launch_fill_u64(I.dtred, 2 * MPS_DT_SLOTS * MPS_DT_STRIDE, inf, h->stream);
     if (ef == hipSuccess) ef = hipStreamSynchronize(h->stream);

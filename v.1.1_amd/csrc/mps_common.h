@@ -54,6 +54,7 @@
 #define MPS_PATCH 256  // elements per patch (element-kernel work unit and partial-sum scope)
 #define MPS_DT_SLOTS 16   // stable-dt reduction slots per step parity: workgroup b min-reduces into slot b % 16
 #define MPS_DT_STRIDE 32  // u64 between two slots (256 B), so the slots' atomics do not share a line
+#define MPS_DT_SKIP_HOLD 64  // steps without the stable-dt skip after a failed certificate
 #define MPS_NODE_LOADED 0x80000000u  // ninfo bit: the node has nodal force entries
 #define MPS_REC_MASK 0x7FFFFFFFu     // ninfo bits below the loaded flag
 #define MPS_ITEM_FORCE 0u            // load item kinds (lcode >> 30)
@@ -102,6 +103,11 @@ struct MpsCtl {
     unsigned long long dt0min;  // min_e l0/c (setup)
     double w_ext, w_react;      // work of the loads and of the supports since the end of step 1
     long long wred;             // last step whose node-kernel energy sums were reduced
+    // stable-dt skip (FP64 inviscid hex8 element kernel): elements whose lower bound of min(l, l0)/c exceeds
+    // dt_skip skip the strict length; the kernel's last workgroup certifies the step's min against it
+    double dt_skip;             // threshold for the next element kernel (+inf: every element is exact)
+    long long skip_hold;        // no skipping before this step (set when a certificate failed)
+    unsigned int eblocks;       // element-kernel workgroups done in the launch in flight
 };
 
 // Largest node count whose byte offsets within one direction array (8 * node) fit 32 bits.
@@ -170,6 +176,9 @@ struct MpsInst {
     int32_t pu_max;  // largest number of distinct nodes of a patch
     // 1: the hex8 element kernels gather nodes by 32-bit byte offsets (at most MPS_OFF32_NODES nodes)
     int32_t off32;
+    // stable-dt skip: on/off, and the threshold's margin over the previous step's min (MpsCtl::dt_skip)
+    int32_t dt_skip_on;
+    double dt_skip_margin;
     // per-step DOF series (mps_set_dof_series): n_series entries, recorded into ser_ring[(k-1)%hist_cap][i]
     int32_t n_series;
     const int32_t* ser_dof;   // node + N*dir

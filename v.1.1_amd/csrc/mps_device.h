@@ -436,6 +436,16 @@ __device__ __forceinline__ void sqrt_n(double* v) {
     }
 }
 
+// Volume term of inscribedSphereHex: |det| of the edges from node 0 to nodes 1, 3, 4 (geometry.cpp:308-355).
+__device__ __forceinline__ double hex_corner_volume(const double (*x)[3]) {
+#pragma clang fp contract(off)
+    double M[9] = {x[1][0] - x[0][0], x[1][1] - x[0][1], x[1][2] - x[0][2],
+                   x[3][0] - x[0][0], x[3][1] - x[0][1], x[3][2] - x[0][2],
+                   x[4][0] - x[0][0], x[4][1] - x[0][1], x[4][2] - x[0][2]};
+    return fabs(M[0] * (M[4] * M[8] - M[7] * M[5]) - M[1] * (M[3] * M[8] - M[6] * M[5]) +
+                M[2] * (M[3] * M[7] - M[6] * M[4]));
+}
+
 // inscribedSphereHex (geometry.cpp:308-355) x2: classLinearHexahedra.cpp:209-234.
 __device__ __forceinline__ double hex_char_length(const double (*x)[3]) {
 #pragma clang fp contract(off)
@@ -453,13 +463,29 @@ __device__ __forceinline__ double hex_char_length(const double (*x)[3]) {
         double s = 0.5 * n2[2 * f] + 0.5 * n2[2 * f + 1];
         A = (f == 0) ? fabs(s) : A + fabs(s);
     }
-    double M[9] = {x[1][0] - x[0][0], x[1][1] - x[0][1], x[1][2] - x[0][2],
-                   x[3][0] - x[0][0], x[3][1] - x[0][1], x[3][2] - x[0][2],
-                   x[4][0] - x[0][0], x[4][1] - x[0][1], x[4][2] - x[0][2]};
-    double V = fabs(M[0] * (M[4] * M[8] - M[7] * M[5]) - M[1] * (M[3] * M[8] - M[6] * M[5]) +
-                    M[2] * (M[3] * M[7] - M[6] * M[4]));
+    double V = hex_corner_volume(x);
     double r = 3.0 * V / A;
     return 2 * r;
+}
+
+// True when the element's stable dt min(l, l0)/c (l = hex_char_length(x)) is certainly above t, without
+// the strict length.  Each face triangle (a,b,c), (a,c,d) of hex_char_length has area 1/2 |p x q| with p, q
+// the two face edges at its quad corner (b, resp. d), and 1/2 |p x q| <= (|p|^2 + |q|^2)/4; every edge lies
+// on two faces, so the area sum A <= S/2 with S the sum of the 12 squared edge lengths, and l = 6V/A >=
+// 12V/S, V the same corner volume (bit for bit: hex_corner_volume).  The factor 1 + 2^-40 covers the
+// rounding of A, S and the final division, all relative errors of a few ulp.
+__device__ __forceinline__ bool hex_dt_above(const double (*x)[3], double l0, double c, double t) {
+#pragma clang fp contract(off)
+    const int ed[12][2] = {{0, 1}, {1, 2}, {2, 3}, {3, 0}, {4, 5}, {5, 6}, {6, 7}, {7, 4}, {0, 4}, {1, 5}, {2, 6}, {3, 7}};
+    double S = 0.0;
+#pragma unroll
+    for (int k = 0; k < 12; k++) {
+        const double d0 = x[ed[k][1]][0] - x[ed[k][0]][0], d1 = x[ed[k][1]][1] - x[ed[k][0]][1];
+        const double d2 = x[ed[k][1]][2] - x[ed[k][0]][2];
+        S += d0 * d0 + d1 * d1 + d2 * d2;
+    }
+    const double tc = t * c * (1.0 + 0x1p-40);
+    return 12.0 * hex_corner_volume(x) > tc * S && l0 > tc;
 }
 
 // inscribedSphere (geometry.cpp:264-301) x2: classLinearTetrahedra.cpp:146-163.
@@ -781,10 +807,13 @@ __device__ __forceinline__ void hex_jac(const T (*A)[3], const GpSigns<T>& s, T*
 // GPs' wJ go to vs[gp*MPS_PATCH] (an LDS array apart from the force slots), for the kinetic energy, which the
 // element kernel computes after the force phase (hex_kinetic).
 // ---------------------------------------------------------------------------
+// dtsk (FP64): the element kernel's dt skip threshold (MpsCtl::dt_skip; +inf: none).  A wave whose elements all
+// have hex_dt_above(dtsk) returns +inf as their dt and skips the strict length.
 template <typename T, int LAW, bool VISC = false, bool EN = false, bool O32 = false>
 __device__ __forceinline__ double hex8_element(const MpsInst& I, int e, const double* __restrict__ u, T (*f)[3],
                                                bool want_dt, T* vs = nullptr, double rdt = 0.0,
-                                               unsigned eflags = 0u, double* en = nullptr) {
+                                               unsigned eflags = 0u, double* en = nullptr,
+                                               double dtsk = __builtin_inf()) {
     static_assert(!EN || (sizeof(T) == 8 && !VISC), "fused energies: FP64 inviscid element kernels");
     double e_def = 0.0, e_vol = 0.0;
     const int32_t* cn = I.conn + (size_t)e * 8;
@@ -808,7 +837,9 @@ __device__ __forceinline__ double hex8_element(const MpsInst& I, int e, const do
             for (int a = 0; a < 8; a++)
 #pragma unroll
                 for (int d = 0; d < 3; d++) x[a][d] = xX[a][d] + xU[a][d];
-            dte = elem_dt(hex_char_length(x), I.l0[e], c);
+            const double l0 = I.l0[e];
+            // dtsk is the same in every lane: a uniform branch
+            if (!(dtsk < __builtin_inf()) || __any(!hex_dt_above(x, l0, c, dtsk))) dte = elem_dt(hex_char_length(x), l0, c);
         }
         hex_butterfly(xX, cx);
         hex_scale<T>(cx, Ax);
@@ -1099,6 +1130,14 @@ __device__ __forceinline__ int begin_scalars(const MpsInst& I, long long k, cons
     out->dt = dt;
     out->go = 1;
     return MPSD_RUNNING;
+}
+
+// Stable-dt skip threshold for the element kernel of step k, from the min dt of step k-1 (bit pattern mb, before
+// the scale factor): elements certainly above it skip the strict length (hex_dt_above), and the kernel's last
+// workgroup checks that the step's min is at most it (dt_certify).  +inf: every element exact.
+__device__ __forceinline__ double dt_skip_next(const MpsInst& I, long long k, unsigned long long mb) {
+    if (!I.dt_skip_on || k < I.ctl->skip_hold) return __builtin_inf();
+    return bits2d(mb) * (1.0 + I.dt_skip_margin);
 }
 
 // Prescribed motion of a constrained DOF at step k with step dt (applyDisplacementBC, solverExplicit.cpp:545-575;
@@ -1437,6 +1476,15 @@ __device__ __forceinline__ unsigned long long* dt_slot(const MpsInst& I, int par
 // Min over the slots of one parity, returned in every lane; all 64 lanes of the wave must call it.
 __device__ __forceinline__ unsigned long long dt_read_min(const MpsInst& I, int parity) {
     unsigned long long v = *dt_slot(I, parity, threadIdx.x & (MPS_DT_SLOTS - 1));
+#pragma unroll
+    for (int off = MPS_DT_SLOTS / 2; off > 0; off >>= 1)
+        v = umin64(v, (unsigned long long)__shfl_xor((long long)v, off, 64));
+    return v;
+}
+// dt_read_min for a reader in the kernel whose workgroups are still adding to the slots (device-scope loads).
+__device__ __forceinline__ unsigned long long dt_read_min_now(const MpsInst& I, int parity) {
+    unsigned long long v = __hip_atomic_load(dt_slot(I, parity, threadIdx.x & (MPS_DT_SLOTS - 1)), __ATOMIC_RELAXED,
+                                             __HIP_MEMORY_SCOPE_AGENT);
 #pragma unroll
     for (int off = MPS_DT_SLOTS / 2; off > 0; off >>= 1)
         v = umin64(v, (unsigned long long)__shfl_xor((long long)v, off, 64));

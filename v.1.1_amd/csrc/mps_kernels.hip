@@ -119,7 +119,8 @@ template <typename T, int NEN, int LAW, int BS, bool VISC = false, bool PRE = fa
 __device__ __forceinline__ unsigned long long patch_forces(const MpsInst& I, int p, bool want_dt, T* __restrict__ sf,
                                                            uint16_t* __restrict__ spc, uint16_t* __restrict__ spr,
                                                            double rdt = 0.0, unsigned eflags = 0u,
-                                                           double* env = nullptr, double* swj = nullptr) {
+                                                           double* env = nullptr, double* swj = nullptr,
+                                                           double dtsk = __builtin_inf()) {
     const int e0 = p * MPS_PATCH;
     const int ne = min(MPS_PATCH, I.Ne - e0);
     PatchList<NEN> pl;
@@ -130,7 +131,7 @@ __device__ __forceinline__ unsigned long long patch_forces(const MpsInst& I, int
         double en[3] = {0.0, 0.0, 0.0};
         const double dte =
             (NEN == 8) ? hex8_element<T, LAW, VISC, EN, O32>(I, e0 + i, I.u, (T(*)[3])f, want_dt,
-                                                              EN ? (T*)(swj + i) : sf + i, rdt, eflags, en)
+                                                              EN ? (T*)(swj + i) : sf + i, rdt, eflags, en, dtsk)
                        : tet4_element<T, LAW, VISC, EN, O32>(I, e0 + i, I.u, (T(*)[3])f, want_dt, rdt, eflags, en);
         if constexpr (EN) {
 #pragma unroll
@@ -156,12 +157,69 @@ __device__ __forceinline__ void elem_energy_publish(const MpsInst& I, double* en
     publish_block_sum<3>(env, I.epart, sf);
 }
 
+// Exact stable dt of hex8 element e at u (the element kernel's own arithmetic, bit for bit).
+template <bool O32>
+__device__ __forceinline__ double hex_dt_exact(const MpsInst& I, int e) {
+    const int32_t* cn = I.conn + (size_t)e * 8;
+    const int4 q0 = *reinterpret_cast<const int4*>(cn);
+    const int4 q1 = *reinterpret_cast<const int4*>(cn + 4);
+    const int nd[8] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, q1.z, q1.w};
+    double pa, pb, c;
+    elem_mat(I, e, pa, pb, c);
+    double xX[8][3], xU[8][3], x[8][3];
+    gather_nodes<8, O32, true>((size_t)I.N, nd, I.X, xX, I.u, xU);
+#pragma unroll
+    for (int a = 0; a < 8; a++)
+#pragma unroll
+        for (int d = 0; d < 3; d++) x[a][d] = xX[a][d] + xU[a][d];
+    return elem_dt(hex_char_length(x), I.l0[e], c);
+}
+
+// End of an FP64 hex8 element kernel of step k that may have skipped elements (dt threshold dtsk).  The last
+// workgroup to finish checks the certificate: the step's min over the exactly computed elements is at most
+// dtsk, so no skipped element (each certainly above dtsk) can be below it.  Otherwise this workgroup adds every
+// element's exact dt to the min and skipping stops for MPS_DT_SKIP_HOLD steps.  Either way the min the node
+// kernel reads is the min over all elements, bit for bit.  No fence orders a workgroup's atomicMin before its
+// count (a fence per workgroup made the kernel 80% slower), so the certifying workgroup may read a min that
+// misses some workgroups' values: one not yet lowered by them.  That is safe both ways: a min at most dtsk
+// only falls further, and one above it only sends the step through the exact fallback, which the certifying
+// workgroup therefore takes only after reading the slots again over up to 8 pauses of about 8K cycles.
+template <bool O32>
+__device__ void dt_certify(const MpsInst& I, long long k, double dtsk, unsigned long long* sh) {
+    __shared__ int last, fail;
+    if (threadIdx.x == 0) last = atomicAdd(&I.ctl->eblocks, 1u) == gridDim.x - 1;
+    __syncthreads();
+    if (!last) return;
+    __threadfence();
+    if (threadIdx.x < 64) {  // wave 0 decides for the workgroup
+        unsigned long long m = dt_read_min_now(I, (k + 1) & 1);
+        for (int t = 0; t < 8 && !(bits2d(m) <= dtsk); t++) {
+            __builtin_amdgcn_s_sleep(127);
+            m = dt_read_min_now(I, (k + 1) & 1);
+        }
+        if (threadIdx.x == 0) fail = !(bits2d(m) <= dtsk);
+    }
+    __syncthreads();
+    if (fail) {
+        unsigned long long mm = INF_BITS;
+        for (int e = threadIdx.x; e < I.Ne; e += blockDim.x) mm = umin64(mm, d2bits(hex_dt_exact<O32>(I, e)));
+        mm = block_min_u64(mm, sh);
+        if (threadIdx.x == 0) {
+            atomicMin(dt_slot(I, (k + 1) & 1, 0), mm);
+            I.ctl->skip_hold = k + 1 + MPS_DT_SKIP_HOLD;
+        }
+    }
+    if (threadIdx.x == 0) I.ctl->eblocks = 0;
+}
+
 // mode 0: step (reads ctl->A, writes ctl->B, reduces dt for step k+1)
 // mode 1: standalone force evaluation (no ctl access)
 // One workgroup per patch, one element per work-item.
 // VISC (mode 0 only): bulk artificial viscosity over the step, from u - uprev and the step's dt.
 // EN (mode 0, FP64 inviscid): the energies of eflags (MPS_E_DEF, MPS_E_KIN) into the history.
 // O32: node gathers by 32-bit offsets (gather_nodes; meshes of at most MPS_OFF32_NODES nodes).
+// FP64 inviscid hex8 (SKIP): elements certainly above the step's dt threshold skip the strict length
+// (hex8_element, dt_certify).
 template <typename T, int NEN, int LAW, bool VISC = false, bool EN = false, bool O32 = false>
 __global__ __launch_bounds__(MPS_PATCH) __attribute__((amdgpu_waves_per_eu(VISC && LAW == 1 ? 1 : 2))) void k_elem(MpsInst I, int mode, unsigned eflags) {
     __shared__ T sf[NEN * 3 * MPS_PATCH];
@@ -169,19 +227,21 @@ __global__ __launch_bounds__(MPS_PATCH) __attribute__((amdgpu_waves_per_eu(VISC 
     __shared__ __align__(16) uint16_t spr[PatchList<NEN>::SPR];
     __shared__ unsigned long long sh[MPS_PATCH / 64];
     __shared__ double swj[(EN && NEN == 8) ? 8 * MPS_PATCH : 1];  // hex8 GP weights wJ for the kinetic energy
+    constexpr bool SKIP = sizeof(T) == 8 && NEN == 8 && !VISC;
     long long k = 0;
-    double rdt = 0.0;
+    double rdt = 0.0, dtsk = __builtin_inf();
     if (mode == 0) {
         k = I.ctl->A;
         if (blockIdx.x == 0 && threadIdx.x == 0) I.ctl->B = k;
         const MpsStepSlot s = I.ctl->slot[k & 1];
         if (!s.go) return;
         if (VISC) rdt = (s.dt == 0.0) ? 0.0 : 1 / s.dt;
+        if (SKIP) dtsk = I.ctl->dt_skip;
     }
     const int p = xcd_tile(blockIdx.x, gridDim.x);
     double env[3] = {0.0, 0.0, 0.0};
     unsigned long long m = patch_forces<T, NEN, LAW, MPS_PATCH, VISC, true, EN, O32>(I, p, mode == 0, sf, spc, spr,
-                                                                                    rdt, eflags, env, swj);
+                                                                                    rdt, eflags, env, swj, dtsk);
     if (mode == 0) {
         m = block_min_u64(m, sh);
         if (threadIdx.x == 0) atomicMin(dt_slot(I, (k + 1) & 1, blockIdx.x & (MPS_DT_SLOTS - 1)), m);
@@ -203,6 +263,7 @@ __global__ __launch_bounds__(MPS_PATCH) __attribute__((amdgpu_waves_per_eu(VISC 
             }
             elem_energy_publish(I, env, reinterpret_cast<double*>(sf));
         }
+        if constexpr (SKIP) dt_certify<O32>(I, k, dtsk, sh);
     }
 }
 
@@ -411,11 +472,13 @@ __device__ __forceinline__ bool node_item(const MpsInst& I, int n, bool first, i
     MpsStepSlot nx;
     nx.go = 0;
     int st = MPSD_RUNNING;
-    if (fuse) st = begin_scalars(I, k + 1, s, dt_read_min(I, (k + 1) & 1), &nx);
+    const unsigned long long mb = fuse ? dt_read_min(I, (k + 1) & 1) : 0ull;
+    if (fuse) st = begin_scalars(I, k + 1, s, mb, &nx);
     if (first) {
         ctl->A = k + 1;
         dt_reset(I, k & 1);
         if (fuse) {
+            ctl->dt_skip = dt_skip_next(I, k + 1, mb);
             ctl->slot[(k + 1) & 1] = nx;
             if (st != MPSD_RUNNING) ctl->status = st;
             if (nx.go) write_hist_step(I, k + 1, nx);
@@ -605,8 +668,10 @@ __device__ __forceinline__ void begin_item(const MpsInst& I, int n, bool first) 
     const long long k = ctl->A;
     const MpsStepSlot prev = ctl->slot[(k - 1) & 1];
     MpsStepSlot nx;
-    const int st = begin_scalars(I, k, prev, dt_read_min(I, k & 1), &nx);
+    const unsigned long long mb = dt_read_min(I, k & 1);
+    const int st = begin_scalars(I, k, prev, mb, &nx);
     if (first) {
+        ctl->dt_skip = dt_skip_next(I, k, mb);
         ctl->slot[k & 1] = nx;
         if (st != MPSD_RUNNING) ctl->status = st;
         if (nx.go) write_hist_step(I, k, nx);
@@ -799,6 +864,9 @@ __global__ void k_init_ctl(MpsInst I, const unsigned long long* dt0) {
     c->w_ext = 0.0;
     c->w_react = 0.0;
     c->wred = 0;
+    c->dt_skip = __builtin_inf();
+    c->skip_hold = 0;
+    c->eblocks = 0;
 }
 
 // Node-centric gather of per-(element,slot) scalars: out[n] = sum over CSR(n), ascending.

@@ -1,4 +1,4 @@
-"""Stable-dt skip of the FP64 hex8 element kernel.  An element whose dt is certainly above the previous step's
+"""Stable-dt skip of the FP64 hex8 and tet4 element kernels.  An element whose dt is certainly above the previous step's
 min by more than a margin skips the strict characteristic length; the kernel's last workgroup certifies that
 the step's min over the exactly computed elements is at most that threshold, and otherwise adds every element's
 exact dt itself.  The min, and so every step, is the same bit for bit with the skip off (MPS_DT_SKIP=0), on
@@ -28,8 +28,10 @@ def run(m, steps, E, nu, dd, dv, energy_every, chunks):
 
 
 @pytest.mark.parametrize("energy_every", [0, 1])
-def test_dt_skip_bitwise(energy_every, monkeypatch):
-    m = perturb(box(20, 14, 10, lx=1.0, ly=0.7, lz=0.5), 0.12, 31)
+@pytest.mark.parametrize("tet", [False, True])
+def test_dt_skip_bitwise(tet, energy_every, monkeypatch):
+    """Hex8 (FP64 element kernel) and tet4 (node-staging FP64 kernel), per-element materials."""
+    m = perturb(box(20, 14, 10, lx=1.0, ly=0.7, lz=0.5, tet=tet), 0.12, 31)
     ne = m["n_elem"]
     rng = np.random.default_rng(9)
     E = 10 ** rng.uniform(5.5, 6.5, ne)

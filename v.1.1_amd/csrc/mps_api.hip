@@ -600,7 +600,8 @@ static int upload_instance(mps_solver_s* h, const HostPrep& hp) {
     HCHK(h, dalloc(h, &ncol, NP * 4));
     HCHK(h, dalloc(h, &ctl, sizeof(MpsCtl)));
     unsigned long long* dtred;
-    HCHK(h, dalloc(h, &dtred, 2 * MPS_DT_SLOTS * MPS_DT_STRIDE * 8));
+    HCHK(h, dalloc(h, &dtred, (2 * MPS_DT_SLOTS * MPS_DT_STRIDE + MPS_DT_COUNT_WORDS) * 8));
+    HCHK(h, hipMemsetAsync(dtred + 2 * MPS_DT_SLOTS * MPS_DT_STRIDE, 0, MPS_DT_COUNT_WORDS * 8, h->stream));
     HCHK(h, dalloc(h, &hist, MPS_HIST_COLS * (size_t)cap * 8));
     if (hp.cfg.energy_every > 0) {  // energy block sums (element phase, node kernel by parity), support forces
         const size_t node_blocks = (N + 255) / 256;
@@ -732,7 +733,6 @@ static int device_setup(mps_solver_s* h) {
     c.dt0min = bits;
     c.dt_skip = HUGE_VAL;  // the first step computes every element's dt
     c.skip_hold = 0;
-    c.eblocks = 0;
     HCHK(h, hipMemcpy(I.ctl, &c, sizeof(c), hipMemcpyHostToDevice));
     hipError_t ef = mpsk::launch_fill_u64(I.dtred, 2 * MPS_DT_SLOTS * MPS_DT_STRIDE, inf, h->stream);
     if (ef == hipSuccess) ef = hipStreamSynchronize(h->stream);

@@ -40,7 +40,9 @@
 //     pc            double[24*n_sgp]  the step's surface forces [(g*8 + a)*3 + dir]
 //   uprev           double[3N]   displacement of the previous step (bulk viscosity only; NULL otherwise)
 //   ctl             MpsCtl       step control (time, dt, go flags), running work sums, reduction counters
-//   dtred           u64[2*MPS_DT_SLOTS*MPS_DT_STRIDE]  stable-dt reduction slots by step parity
+//   dtred           u64[2*MPS_DT_SLOTS*MPS_DT_STRIDE]  stable-dt reduction slots by step parity; in the
+//                   single-instance form followed by MPS_DT_SLOTS + 1 counters (same stride) of the
+//                   stable-dt certificate (MPS_DT_COUNT_WORDS)
 //   hist            double[MPS_HIST_COLS*hist_cap]     per-step history ring, column c at c*hist_cap
 //   energy bookkeeping (solves with energies only):
 //     epart         double[3*n_patch + 2*MPS_NODE_PARTS*node_blocks]  per-block energy sums: element phase, then
@@ -55,6 +57,7 @@
 #define MPS_DT_SLOTS 16   // stable-dt reduction slots per step parity: workgroup b min-reduces into slot b % 16
 #define MPS_DT_STRIDE 32  // u64 between two slots (256 B), so the slots' atomics do not share a line
 #define MPS_DT_SKIP_HOLD 64  // steps without the stable-dt skip after a failed certificate
+#define MPS_DT_COUNT_WORDS ((MPS_DT_SLOTS + 1) * MPS_DT_STRIDE)  // certificate counters after the dt slots
 #define MPS_NODE_LOADED 0x80000000u  // ninfo bit: the node has nodal force entries
 #define MPS_REC_MASK 0x7FFFFFFFu     // ninfo bits below the loaded flag
 #define MPS_ITEM_FORCE 0u            // load item kinds (lcode >> 30)
@@ -107,7 +110,6 @@ struct MpsCtl {
     // dt_skip skip the strict length; the kernel's last workgroup certifies the step's min against it
     double dt_skip;             // threshold for the next element kernel (+inf: every element is exact)
     long long skip_hold;        // no skipping before this step (set when a certificate failed)
-    unsigned int eblocks;       // element-kernel workgroups done in the launch in flight
 };
 
 // Largest node count whose byte offsets within one direction array (8 * node) fit 32 bits.

@@ -488,6 +488,16 @@ __device__ __forceinline__ bool hex_dt_above(const double (*x)[3], double l0, do
     return 12.0 * hex_corner_volume(x) > tc * S && l0 > tc;
 }
 
+// Volume term of inscribedSphere (geometry.cpp:264-301).
+__device__ __forceinline__ double tet_volume(const double (*x)[3]) {
+#pragma clang fp contract(off)
+    double M[9] = {x[1][0] - x[0][0], x[1][1] - x[0][1], x[1][2] - x[0][2],
+                   x[2][0] - x[0][0], x[2][1] - x[0][1], x[2][2] - x[0][2],
+                   x[3][0] - x[0][0], x[3][1] - x[0][1], x[3][2] - x[0][2]};
+    return 1.0 / 6.0 * fabs(M[0] * (M[4] * M[8] - M[7] * M[5]) - M[1] * (M[3] * M[8] - M[6] * M[5]) +
+                            M[2] * (M[3] * M[7] - M[6] * M[4]));
+}
+
 // inscribedSphere (geometry.cpp:264-301) x2: classLinearTetrahedra.cpp:146-163.
 __device__ __forceinline__ double tet_char_length(const double (*x)[3]) {
 #pragma clang fp contract(off)
@@ -495,13 +505,33 @@ __device__ __forceinline__ double tet_char_length(const double (*x)[3]) {
                     tri_norm2(x[0], x[1], x[3])};
     sqrt_n<4>(n2);
     double A = fabs(0.5 * n2[0]) + fabs(0.5 * n2[1]) + fabs(0.5 * n2[2]) + fabs(0.5 * n2[3]);
-    double M[9] = {x[1][0] - x[0][0], x[1][1] - x[0][1], x[1][2] - x[0][2],
-                   x[2][0] - x[0][0], x[2][1] - x[0][1], x[2][2] - x[0][2],
-                   x[3][0] - x[0][0], x[3][1] - x[0][1], x[3][2] - x[0][2]};
-    double V = 1.0 / 6.0 * fabs(M[0] * (M[4] * M[8] - M[7] * M[5]) - M[1] * (M[3] * M[8] - M[6] * M[5]) +
-                               M[2] * (M[3] * M[7] - M[6] * M[4]));
+    double V = tet_volume(x);
     double r = 3.0 * V / A;
     return 2 * r;
+}
+
+// hex_dt_above for a tet4: each face's area is at most (|p|^2 + |q|^2)/4 for any two of its edges p, q, so at
+// most a quarter of its two shorter squared edges; l = 6V/A >= 6V/A_ub with the same V (tet_volume).
+__device__ __forceinline__ bool tet_dt_above(const double (*x)[3], double l0, double c, double t) {
+#pragma clang fp contract(off)
+    double q[6];  // squared edges 01 02 03 12 13 23
+    const int ed[6][2] = {{0, 1}, {0, 2}, {0, 3}, {1, 2}, {1, 3}, {2, 3}};
+#pragma unroll
+    for (int k = 0; k < 6; k++) {
+        const double d0 = x[ed[k][1]][0] - x[ed[k][0]][0], d1 = x[ed[k][1]][1] - x[ed[k][0]][1];
+        const double d2 = x[ed[k][1]][2] - x[ed[k][0]][2];
+        q[k] = d0 * d0 + d1 * d1 + d2 * d2;
+    }
+    // faces (0,1,2), (1,2,3), (0,2,3), (0,1,3): their edges' indices in q
+    const int fe[4][3] = {{0, 1, 3}, {3, 4, 5}, {1, 2, 5}, {0, 2, 4}};
+    double A4 = 0.0;  // 4 * the bound of the area sum
+#pragma unroll
+    for (int f = 0; f < 4; f++) {
+        const double a = q[fe[f][0]], b = q[fe[f][1]], c2 = q[fe[f][2]];
+        A4 += a + b + c2 - fmax(a, fmax(b, c2));
+    }
+    const double tc = t * c * (1.0 + 0x1p-40);
+    return 24.0 * tet_volume(x) > tc * A4 && l0 > tc;
 }
 
 // FP32 characteristic lengths for the precision form (the reference formulas with FP32 areas, volume and
@@ -1472,6 +1502,11 @@ __device__ __forceinline__ unsigned long long umin64(unsigned long long a, unsig
 // step parity, 256 B apart, so that thousands of workgroups do not queue on one address.
 __device__ __forceinline__ unsigned long long* dt_slot(const MpsInst& I, int parity, int s) {
     return I.dtred + (size_t)(parity * MPS_DT_SLOTS + s) * MPS_DT_STRIDE;
+}
+// Certificate counters (single-instance form): s < MPS_DT_SLOTS counts the element-kernel workgroups b with
+// b % MPS_DT_SLOTS == s that are done, s = MPS_DT_SLOTS the residues whose workgroups are all done.
+__device__ __forceinline__ unsigned int* dt_count(const MpsInst& I, int s) {
+    return reinterpret_cast<unsigned int*>(I.dtred + (size_t)(2 * MPS_DT_SLOTS + s) * MPS_DT_STRIDE);
 }
 // Min over the slots of one parity, returned in every lane; all 64 lanes of the wave must call it.
 __device__ __forceinline__ unsigned long long dt_read_min(const MpsInst& I, int parity) {

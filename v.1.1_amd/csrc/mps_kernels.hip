@@ -157,37 +157,49 @@ __device__ __forceinline__ void elem_energy_publish(const MpsInst& I, double* en
     publish_block_sum<3>(env, I.epart, sf);
 }
 
-// Exact stable dt of hex8 element e at u (the element kernel's own arithmetic, bit for bit).
-template <bool O32>
-__device__ __forceinline__ double hex_dt_exact(const MpsInst& I, int e) {
-    const int32_t* cn = I.conn + (size_t)e * 8;
-    const int4 q0 = *reinterpret_cast<const int4*>(cn);
-    const int4 q1 = *reinterpret_cast<const int4*>(cn + 4);
-    const int nd[8] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, q1.z, q1.w};
+// Exact stable dt of element e at u (the element kernels' own arithmetic, bit for bit).
+template <int NEN, bool O32>
+__device__ __forceinline__ double elem_dt_exact(const MpsInst& I, int e) {
+    int nd[NEN];
+#pragma unroll
+    for (int a = 0; a < NEN; a++) nd[a] = I.conn[(size_t)e * NEN + a];
     double pa, pb, c;
     elem_mat(I, e, pa, pb, c);
-    double xX[8][3], xU[8][3], x[8][3];
-    gather_nodes<8, O32, true>((size_t)I.N, nd, I.X, xX, I.u, xU);
+    double xX[NEN][3], xU[NEN][3], x[NEN][3];
+    gather_nodes<NEN, O32, true>((size_t)I.N, nd, I.X, xX, I.u, xU);
 #pragma unroll
-    for (int a = 0; a < 8; a++)
+    for (int a = 0; a < NEN; a++)
 #pragma unroll
         for (int d = 0; d < 3; d++) x[a][d] = xX[a][d] + xU[a][d];
-    return elem_dt(hex_char_length(x), I.l0[e], c);
+    return elem_dt(NEN == 8 ? hex_char_length(x) : tet_char_length(x), I.l0[e], c);
 }
 
-// End of an FP64 hex8 element kernel of step k that may have skipped elements (dt threshold dtsk).  The last
+// End of an FP64 element kernel of step k that may have skipped elements (dt threshold dtsk).  The last
 // workgroup to finish checks the certificate: the step's min over the exactly computed elements is at most
 // dtsk, so no skipped element (each certainly above dtsk) can be below it.  Otherwise this workgroup adds every
 // element's exact dt to the min and skipping stops for MPS_DT_SKIP_HOLD steps.  Either way the min the node
-// kernel reads is the min over all elements, bit for bit.  No fence orders a workgroup's atomicMin before its
+// kernel reads is the min over all elements, bit for bit.  The count is two-level, so that thousands of
+// workgroups do not queue on one address (one counter for all made the tet4 kernel twice as slow): the last
+// workgroup of each residue modulo MPS_DT_SLOTS counts its residue on the top counter, and each last one
+// resets the counter it completed.  No fence orders a workgroup's atomicMin before its
 // count (a fence per workgroup made the kernel 80% slower), so the certifying workgroup may read a min that
 // misses some workgroups' values: one not yet lowered by them.  That is safe both ways: a min at most dtsk
 // only falls further, and one above it only sends the step through the exact fallback, which the certifying
 // workgroup therefore takes only after reading the slots again over up to 8 pauses of about 8K cycles.
-template <bool O32>
+template <int NEN, bool O32>
 __device__ void dt_certify(const MpsInst& I, long long k, double dtsk, unsigned long long* sh) {
     __shared__ int last, fail;
-    if (threadIdx.x == 0) last = atomicAdd(&I.ctl->eblocks, 1u) == gridDim.x - 1;
+    if (threadIdx.x == 0) {
+        const unsigned r = blockIdx.x & (MPS_DT_SLOTS - 1);
+        const unsigned share = (gridDim.x - r + MPS_DT_SLOTS - 1) / MPS_DT_SLOTS;  // workgroups of residue r
+        const unsigned groups = gridDim.x < MPS_DT_SLOTS ? gridDim.x : MPS_DT_SLOTS;
+        last = 0;
+        if (atomicAdd(dt_count(I, r), 1u) == share - 1) {
+            *dt_count(I, r) = 0u;
+            last = atomicAdd(dt_count(I, MPS_DT_SLOTS), 1u) == groups - 1;
+            if (last) *dt_count(I, MPS_DT_SLOTS) = 0u;
+        }
+    }
     __syncthreads();
     if (!last) return;
     __threadfence();
@@ -202,14 +214,13 @@ __device__ void dt_certify(const MpsInst& I, long long k, double dtsk, unsigned 
     __syncthreads();
     if (fail) {
         unsigned long long mm = INF_BITS;
-        for (int e = threadIdx.x; e < I.Ne; e += blockDim.x) mm = umin64(mm, d2bits(hex_dt_exact<O32>(I, e)));
+        for (int e = threadIdx.x; e < I.Ne; e += blockDim.x) mm = umin64(mm, d2bits(elem_dt_exact<NEN, O32>(I, e)));
         mm = block_min_u64(mm, sh);
         if (threadIdx.x == 0) {
             atomicMin(dt_slot(I, (k + 1) & 1, 0), mm);
             I.ctl->skip_hold = k + 1 + MPS_DT_SKIP_HOLD;
         }
     }
-    if (threadIdx.x == 0) I.ctl->eblocks = 0;
 }
 
 // mode 0: step (reads ctl->A, writes ctl->B, reduces dt for step k+1)
@@ -263,7 +274,7 @@ __global__ __launch_bounds__(MPS_PATCH) __attribute__((amdgpu_waves_per_eu(VISC 
             }
             elem_energy_publish(I, env, reinterpret_cast<double*>(sf));
         }
-        if constexpr (SKIP) dt_certify<O32>(I, k, dtsk, sh);
+        if constexpr (SKIP) dt_certify<8, O32>(I, k, dtsk, sh);
     }
 }
 
@@ -304,12 +315,15 @@ __global__ __launch_bounds__(MPS_PATCH) __attribute__((amdgpu_waves_per_eu(sizeo
     T* __restrict__ sf = reinterpret_cast<T*>(smem);  // [12][MPS_PATCH], over the staging once it is read
     const size_t shared_bytes = tet_shared_bytes(pu, sizeof(T));
     uint16_t* __restrict__ spc = reinterpret_cast<uint16_t*>(reinterpret_cast<char*>(smem) + shared_bytes);
+    constexpr bool SKIP = sizeof(T) == 8;  // stable-dt skip (dt_certify)
     long long k = 0;
+    double dtsk = __builtin_inf();
     if (mode == 0) {
         k = I.ctl->A;
         if (blockIdx.x == 0 && threadIdx.x == 0) I.ctl->B = k;
         const MpsStepSlot s = I.ctl->slot[k & 1];
         if (!s.go) return;
+        if (SKIP) dtsk = I.ctl->dt_skip;
     }
     const int p = xcd_tile(blockIdx.x, gridDim.x);
     const int e0 = p * MPS_PATCH;
@@ -353,7 +367,10 @@ __global__ __launch_bounds__(MPS_PATCH) __attribute__((amdgpu_waves_per_eu(sizeo
 #pragma unroll
                 for (int d = 0; d < 3; d++) xd[a][d] = xa[d] + ua[d];
             }
-            dt64 = elem_dt(tet_char_length(xd), kc.l0, kc.c);
+            // dtsk is the same in every lane; the elements of a wave whose dt is certainly above it skip the
+            // strict length (tet_dt_above, dt_certify)
+            if (!(dtsk < __builtin_inf()) || __any(!tet_dt_above(xd, kc.l0, kc.c, dtsk)))
+                dt64 = elem_dt(tet_char_length(xd), kc.l0, kc.c);
             asm volatile("" ::: "memory");
         }
         const bool core_dt = sizeof(T) == 4 && mode == 0;
@@ -389,6 +406,7 @@ __global__ __launch_bounds__(MPS_PATCH) __attribute__((amdgpu_waves_per_eu(sizeo
         dmin = block_min_u64(dmin, sh);
         if (threadIdx.x == 0) atomicMin(dt_slot(I, (k + 1) & 1, blockIdx.x & (MPS_DT_SLOTS - 1)), dmin);
         if constexpr (EN) elem_energy_publish(I, env, smem);
+        if constexpr (SKIP) dt_certify<4, false>(I, k, dtsk, sh);
     }
 }
 
@@ -866,7 +884,6 @@ __global__ void k_init_ctl(MpsInst I, const unsigned long long* dt0) {
     c->wred = 0;
     c->dt_skip = __builtin_inf();
     c->skip_hold = 0;
-    c->eblocks = 0;
 }
 
 // Node-centric gather of per-(element,slot) scalars: out[n] = sum over CSR(n), ascending.
